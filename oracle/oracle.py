@@ -1,0 +1,192 @@
+"""ctypes wrapper over the CPU oracle (oracle/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg, as the checker or the timed CPU baseline. The
+product package never imports it.
+
+Gf128 arrays are numpy uint64 arrays of shape (n, 2) holding (lo, hi); their
+bytes are the reference's 16-byte wire form lo_le || hi_le.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+
+
+class GF(C.Structure):
+    _fields_ = [("lo", C.c_uint64), ("hi", C.c_uint64)]
+
+
+def _load():
+    if not os.path.exists(_LIB_PATH):
+        raise RuntimeError(f"oracle library not built: run `make -C {_HERE}`")
+    lib = C.CDLL(_LIB_PATH)
+    P = C.c_void_p
+    sig = {
+        "or_gf_mul": (GF, [GF, GF]),
+        "or_gf_square": (GF, [GF]),
+        "or_gf_inv": (GF, [GF]),
+        "or_gf_pow": (GF, [GF, C.c_uint64, C.c_uint64]),
+        "or_gf_mul_arrays": (None, [P, P, P, C.c_size_t]),
+        "or_gf_uses_pclmul": (C.c_int, []),
+        "or_gf_force_portable": (None, [C.c_int]),
+        "or_blake3_hash": (None, [P, C.c_size_t, P, C.c_size_t]),
+        "or_blake3_leaf_cv": (None, [P, C.c_size_t, P]),
+        "or_blake3_parent_cv": (None, [P, P, P]),
+        "or_blake3_compress": (None, [P, P, C.c_uint64, C.c_uint32, C.c_uint32, P]),
+        "or_ntt_evals": (None, [C.c_int, P]),
+        "or_ntt_twiddle": (GF, [C.c_int, P, C.c_int, C.c_uint64]),
+        "or_ntt_forward_interleaved": (None, [P, C.c_int, C.c_int, C.c_int, C.c_int]),
+        "or_ntt_inverse_interleaved": (None, [P, C.c_int, C.c_int, C.c_int]),
+        "or_merkle_tree": (None, [P, C.c_uint64, C.c_uint64, P]),
+        "or_merkle_multi_proof": (C.c_size_t, [P, C.c_uint64, P, C.c_size_t, P]),
+        "or_merkle_verify_multi": (C.c_int, [P, C.c_uint64, P, P, C.c_size_t, P, C.c_size_t]),
+        "or_commit": (C.c_int, [P, C.c_int, C.c_int, C.c_int, P, P, P]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    return lib
+
+
+lib = _load()
+
+
+def _ptr(a: np.ndarray):
+    assert a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def gf(x) -> GF:
+    if isinstance(x, GF):
+        return x
+    if isinstance(x, int):
+        return GF(x & (2**64 - 1), x >> 64)
+    lo, hi = x
+    return GF(int(lo), int(hi))
+
+
+def gf_int(g: GF) -> int:
+    return g.lo | (g.hi << 64)
+
+
+def mul(a: int, b: int) -> int:
+    return gf_int(lib.or_gf_mul(gf(a), gf(b)))
+
+
+def square(a: int) -> int:
+    return gf_int(lib.or_gf_square(gf(a)))
+
+
+def inv(a: int) -> int:
+    return gf_int(lib.or_gf_inv(gf(a)))
+
+
+def power(a: int, e: int) -> int:
+    return gf_int(lib.or_gf_pow(gf(a), e & (2**64 - 1), e >> 64))
+
+
+def mul_arrays(a: np.ndarray, b: np.ndarray) -> np.ndarray:
+    a = np.ascontiguousarray(a, dtype=np.uint64)
+    b = np.ascontiguousarray(b, dtype=np.uint64)
+    out = np.empty_like(a)
+    lib.or_gf_mul_arrays(_ptr(a), _ptr(b), _ptr(out), a.shape[0])
+    return out
+
+
+def blake3(data: bytes, out_len: int = 32) -> bytes:
+    buf = np.frombuffer(bytes(data), dtype=np.uint8).copy() if data else np.zeros(1, np.uint8)
+    out = np.zeros(out_len, np.uint8)
+    lib.or_blake3_hash(_ptr(buf), len(data), _ptr(out), out_len)
+    return out.tobytes()
+
+
+def leaf_cv(data: bytes) -> bytes:
+    buf = np.frombuffer(bytes(data), dtype=np.uint8).copy()
+    out = np.zeros(32, np.uint8)
+    lib.or_blake3_leaf_cv(_ptr(buf), len(data), _ptr(out))
+    return out.tobytes()
+
+
+def parent_cv(left: bytes, right: bytes) -> bytes:
+    lb = np.frombuffer(left, dtype=np.uint8).copy()
+    rb = np.frombuffer(right, dtype=np.uint8).copy()
+    out = np.zeros(32, np.uint8)
+    lib.or_blake3_parent_cv(_ptr(lb), _ptr(rb), _ptr(out))
+    return out.tobytes()
+
+
+def ntt_evals(ell: int) -> list[list[int]]:
+    n = ell * (ell + 1) // 2
+    buf = np.zeros((max(n, 1), 2), np.uint64)
+    lib.or_ntt_evals(ell, _ptr(buf))
+    rows, off = [], 0
+    for i in range(ell):
+        rows.append([int(buf[off + k, 0]) | (int(buf[off + k, 1]) << 64) for k in range(ell - i)])
+        off += ell - i
+    return rows
+
+
+def ntt_forward(data: np.ndarray, log_d: int, log_lanes: int, start_layer: int = 0,
+                ell: int | None = None) -> np.ndarray:
+    out = np.ascontiguousarray(data, dtype=np.uint64).copy()
+    assert out.shape == ((1 << (log_d + log_lanes)), 2)
+    lib.or_ntt_forward_interleaved(_ptr(out), log_d, log_lanes, start_layer,
+                                   log_d if ell is None else ell)
+    return out
+
+
+def ntt_inverse(data: np.ndarray, log_d: int, log_lanes: int, ell: int | None = None) -> np.ndarray:
+    out = np.ascontiguousarray(data, dtype=np.uint64).copy()
+    lib.or_ntt_inverse_interleaved(_ptr(out), log_d, log_lanes, log_d if ell is None else ell)
+    return out
+
+
+def merkle_tree(data: bytes | np.ndarray, num_leaves: int) -> np.ndarray:
+    buf = np.frombuffer(bytes(data), dtype=np.uint8) if isinstance(data, (bytes, bytearray)) \
+        else np.ascontiguousarray(data).view(np.uint8).reshape(-1)
+    buf = np.ascontiguousarray(buf)
+    assert buf.size % num_leaves == 0
+    tree = np.zeros((2 * num_leaves - 1, 32), np.uint8)
+    lib.or_merkle_tree(_ptr(buf), num_leaves, buf.size // num_leaves, _ptr(tree))
+    return tree
+
+
+def merkle_multi_proof(tree: np.ndarray, num_leaves: int, positions) -> np.ndarray:
+    pos = np.ascontiguousarray(np.asarray(positions, dtype=np.uint64))
+    depth = max(num_leaves.bit_length() - 1, 1)
+    out = np.zeros((max(len(pos) * depth, 1), 32), np.uint8)
+    n = lib.or_merkle_multi_proof(_ptr(np.ascontiguousarray(tree)), num_leaves, _ptr(pos),
+                                  len(pos), _ptr(out))
+    return out[:n].copy()
+
+
+def merkle_verify_multi(root: bytes, num_leaves: int, sorted_positions, leaf_hashes: np.ndarray,
+                        proof: np.ndarray) -> bool:
+    pos = np.ascontiguousarray(np.asarray(sorted_positions, dtype=np.uint64))
+    rb = np.frombuffer(root, dtype=np.uint8).copy()
+    lh = np.ascontiguousarray(leaf_hashes, dtype=np.uint8)
+    pf = np.ascontiguousarray(proof, dtype=np.uint8) if len(proof) else np.zeros((1, 32), np.uint8)
+    return bool(lib.or_merkle_verify_multi(_ptr(rb), num_leaves, _ptr(pos), _ptr(lh), len(pos),
+                                           _ptr(pf), len(proof)))
+
+
+def commit(packed: np.ndarray, log_msg_len: int, log_batch: int, log_inv_rate: int):
+    """Returns (root bytes, codeword (n,2) u64, tree (2N-1,32) u8)."""
+    packed = np.ascontiguousarray(packed, dtype=np.uint64)
+    assert packed.shape == (1 << log_msg_len, 2)
+    cw = np.zeros((1 << (log_msg_len + log_inv_rate), 2), np.uint64)
+    n_leaves = 1 << (log_msg_len - log_batch + log_inv_rate)
+    tree = np.zeros((2 * n_leaves - 1, 32), np.uint8)
+    root = np.zeros(32, np.uint8)
+    rc = lib.or_commit(_ptr(packed), log_msg_len, log_batch, log_inv_rate, _ptr(cw), _ptr(tree),
+                       _ptr(root))
+    if rc != 0:
+        raise ValueError(f"or_commit rejected the geometry (rc={rc})")
+    return root.tobytes(), cw, tree

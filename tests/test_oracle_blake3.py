@@ -1,0 +1,53 @@
+"""Pins the oracle's BLAKE3 against published digests (BLAKE3 reference
+test_vectors.json, input byte i = i mod 251, and the widely published empty /
+"abc" / "hello world" digests) and the spec's own self-consistency rules. The
+reference uses the blake3 crate 1.8.7 (Cargo.lock) for its Merkle leaves and
+parents (merkle.rs:192-202) and its outer transcript."""
+import oracle as O
+
+PUBLISHED = {
+    0: "af1349b9f5f9a1a6a0404dea36dcc9499bcb25c9adc112b7cc9a93cae41f3262",
+    1: "2d3adedff11b61f14c886e35afa036736dcd87a74d27b5c1510225d0f592e213",
+    1023: "10108970eeda3eb932baac1428c7a2163b0e924c9a9e25b35bba72b28f70bd11",
+    1024: "42214739f095a406f3fc83deb889744ac00df831c10daa55189b5d121c855af7",
+    1025: "d00278ae47eb27b34faecf67b4fe263f82d5412916c1ffd97c8cb7fb814b8444",
+    2048: "e776b6028c7cd22a4d0ba182a8bf62205d2ef576467e838ed6f2529b85fba24a",
+}
+
+
+def _input(n):
+    return bytes(i % 251 for i in range(n))
+
+
+def test_published_digests():
+    for n, hexd in PUBLISHED.items():
+        assert O.blake3(_input(n)).hex() == hexd, n
+    assert O.blake3(b"abc").hex() == \
+        "6437b3ac38465133ffb63b75273a8db548c558465d79db03fd359c6cd5bd9d85"
+    assert O.blake3(b"hello world").hex() == \
+        "d74981efa70a0c880b8d8c1985d075dbcbf679b99a5f9914e5aaf96b831a9e24"
+
+
+def test_xof_prefix_consistency():
+    d = _input(3000)
+    long = O.blake3(d, 200)
+    for k in (1, 31, 32, 64, 65, 131):
+        assert O.blake3(d, k) == long[:k]
+
+
+def test_leaf_and_parent_are_the_tree_rule():
+    # A 2048-byte input is two chunks; its root is the ROOT-flagged parent of
+    # the two non-root chunk CVs. The non-root parent CV of the same children
+    # must differ from the root digest (only the ROOT flag differs), and the
+    # 2-chunk non-root CV equals parent_cv(leaf_cv(c0), leaf_cv(c1)) only for
+    # chunk counters 0 and 1 -- the second chunk's counter is 1, so a Merkle
+    # leaf (counter 0) of the same bytes must NOT equal it.
+    d = _input(2048)
+    c0 = O.leaf_cv(d[:1024])
+    c1_counter0 = O.leaf_cv(d[1024:])
+    whole_nonroot = O.leaf_cv(d)
+    assert O.parent_cv(c0, c1_counter0) != whole_nonroot
+    assert whole_nonroot != O.blake3(d)
+    # Single chunk: the leaf CV is the root digest minus the ROOT flag, so it
+    # differs from the hash; for inputs <= 64 bytes both come from one block.
+    assert O.leaf_cv(d[:100]) != O.blake3(d[:100])

@@ -1,0 +1,105 @@
+"""ctypes binding of libbitz_hip.so (the C ABI declared in include/bitz_hip.h).
+
+The library is the product: there is no CPU fallback. If it is missing or
+fails to load, every entry point raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+_PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(_PKG_DIR, "libbitz_hip.so")
+HEADER_PATH = os.path.join(os.path.dirname(_PKG_DIR), "include", "bitz_hip.h")
+
+BITZ_OK = 0
+ERR_NAMES = {-1: "SHAPE", -2: "ALLOC", -3: "DEVICE", -4: "ARG"}
+
+
+class BitzError(RuntimeError):
+    def __init__(self, code: int, fn: str, msg: str):
+        super().__init__(f"{fn} failed ({ERR_NAMES.get(code, code)}): {msg}")
+        self.code = code
+
+
+_P = C.c_void_p
+_I = C.c_int
+_U32 = C.c_uint32
+_U64 = C.c_uint64
+
+# name -> (restype, argtypes); every symbol of include/bitz_hip.h
+SIGNATURES = {
+    "bitz_ctx_create": (_I, [_I, C.POINTER(_P)]),
+    "bitz_ctx_destroy": (None, [_P]),
+    "bitz_ctx_set_stream": (_I, [_P, _P]),
+    "bitz_ctx_stream": (_P, [_P]),
+    "bitz_ctx_synchronize": (_I, [_P]),
+    "bitz_last_error": (C.c_char_p, []),
+    "bitz_version": (C.c_char_p, []),
+    "bitz_gf_mul_dev": (_I, [_P, _P, _P, _P, _U64]),
+    "bitz_gf_mul_const_dev": (_I, [_P, _P, _P, _P, _U64]),
+    "bitz_blake3_leaves_dev": (_I, [_P, _P, _U64, _U64, _P]),
+    "bitz_ntt_forward_dev": (_I, [_P, _P, _I, _I, _I, _I]),
+    "bitz_merkle_tree_dev": (_I, [_P, _P, _U64, _U64, _P]),
+    "bitz_commit_dev": (_I, [_P, _P, _I, _I, _I, _P, _P, _P]),
+    "bitz_eq_table_dev": (_I, [_P, _P, _I, _P]),
+    "bitz_fold_columns_dev": (_I, [_P, _P, _I, _I, _P, _P]),
+    "bitz_pow_images_dev": (_I, [_P, _P, _P, _U64, _P]),
+    "bitz_pack_columns_dev": (_I, [_P, _P, _I, _I, _P]),
+    "bitz_gkr_round_sums_dev": (_I, [_P, _P, _P, _U64, _P, _I, _P, _I, _P]),
+    "bitz_gkr_leaf_round_sums_dev": (_I, [_P, _P, _P, _I, _I, _P, _I, _P, _I, _P]),
+    "bitz_fold_halves_dev": (_I, [_P, _P, _U64, _P]),
+    "bitz_gkr_leaf_fold_dev": (_I, [_P, _P, _P, _I, _I, _P, _P, _P]),
+    "bitz_gkr_level1_dev": (_I, [_P, _P, _P, _I, _I, _P]),
+    "bitz_product_level_dev": (_I, [_P, _P, _U64, _P]),
+    "bitz_xi_combined_dev": (_I, [_P, _P, _I, _I, _P, _P]),
+    "bitz_fold_rows_eq_dev": (_I, [_P, _P, _I, _I, _P, _P]),
+    "bitz_round_poly_dev": (_I, [_P, _P, _P, _U64, _P]),
+    "bitz_fold_lsb_dev": (_I, [_P, _P, _U64, _P, _P]),
+    "bitz_fold_1b_rows_dev": (_I, [_P, _P, _P, _U64, _P]),
+    "bitz_fold_b128_dev": (_I, [_P, _P, _U64, _P, _P]),
+    "bitz_axpy_dev": (_I, [_P, _P, _P, _U64, _P]),
+    "bitz_lig_msg_dev": (_I, [_P, _P, _P, _U64, _I, _P]),
+    "bitz_lig_fold_dev": (_I, [_P, _P, _P, _U64, _P, _P, _P]),
+    "bitz_induce_basis_dev": (_I, [_P, _P, _U32, _I, _P, _P]),
+    "bitz_pow_find_dev": (_I, [_P, _P, _P, _U32, C.POINTER(_U64)]),
+    "bitz_commit": (_I, [_P, _P, _U32, _U32, _U32, _U32, _P, C.POINTER(_P)]),
+    "bitz_commit_free": (None, [_P]),
+    "bitz_commit_packed": (_P, [_P]),
+    "bitz_commit_codeword": (_P, [_P]),
+    "bitz_commit_tree": (_P, [_P]),
+    "bitz_commit_read_tree": (_I, [_P, _P, _U64]),
+}
+
+_lib = None
+
+
+def load() -> C.CDLL:
+    """Load libbitz_hip.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(
+            f"libbitz_hip.so not found at {LIB_PATH}; build it with "
+            f"`make -C {_PKG_DIR}` (or __graft_entry__.build())")
+    lib = C.CDLL(LIB_PATH)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def check(rc: int, fn: str) -> None:
+    if rc != BITZ_OK:
+        msg = load().bitz_last_error()
+        raise BitzError(rc, fn, msg.decode() if msg else "")
+
+
+def call(name: str, *args) -> int:
+    lib = load()
+    rc = getattr(lib, name)(*args)
+    check(rc, name)
+    return rc

@@ -1,0 +1,139 @@
+// Shared host-side plumbing of libbitz_hip: context, error reporting,
+// checked HIP calls, and host GF(2^128) arithmetic for small setup tables.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string.h>
+
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/bitz_hip.h"
+
+namespace bitz {
+
+void set_error(const std::string& msg);
+
+struct DevBuf {
+    void* ptr = nullptr;
+    size_t bytes = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    ~DevBuf() {
+        if (ptr) (void)hipFree(ptr);
+    }
+    bool alloc(size_t n) {
+        if (ptr && bytes >= n) return true;
+        if (ptr) (void)hipFree(ptr);
+        ptr = nullptr;
+        bytes = 0;
+        if (n == 0) return true;
+        if (hipMalloc(&ptr, n) != hipSuccess) {
+            ptr = nullptr;
+            return false;
+        }
+        bytes = n;
+        return true;
+    }
+};
+
+}  // namespace bitz
+
+namespace bitz {
+// Per-context device scratch reused across calls (partial sums, small
+// tables) plus a pinned host staging buffer for round results.
+struct Scratch {
+    DevBuf partials, result, eq_lo, fold_partial, comb, what, misc;
+    void* host = nullptr;
+    uint8_t comb_base[16] = {0};
+    bool comb_ready = false;
+    int ensure(size_t partial_bytes);
+    ~Scratch();
+};
+}  // namespace bitz
+
+struct bitz_ctx {
+    int device = 0;
+    bitz::Scratch scratch;
+    hipStream_t own_stream = nullptr;
+    hipStream_t stream = nullptr;
+    // device twiddle tables per NTT domain dimension ell:
+    // tw[(1 << layer) - 1 + block] = twiddle(layer, block)
+    std::map<int, std::unique_ptr<bitz::DevBuf>> twiddles;
+};
+
+#define BITZ_HIP(expr)                                                                   \
+    do {                                                                                 \
+        hipError_t _e = (expr);                                                          \
+        if (_e != hipSuccess) {                                                          \
+            bitz::set_error(std::string(#expr) + ": " + hipGetErrorString(_e));          \
+            return BITZ_ERR_DEVICE;                                                      \
+        }                                                                                \
+    } while (0)
+
+#define BITZ_REQUIRE(cond, code, msg)  \
+    do {                               \
+        if (!(cond)) {                 \
+            bitz::set_error(msg);      \
+            return (code);             \
+        }                              \
+    } while (0)
+
+#define BITZ_LAUNCH_CHECK()                                                              \
+    do {                                                                                 \
+        hipError_t _e = hipGetLastError();                                               \
+        if (_e != hipSuccess) {                                                          \
+            bitz::set_error(std::string("kernel launch: ") + hipGetErrorString(_e));     \
+            return BITZ_ERR_DEVICE;                                                      \
+        }                                                                                \
+    } while (0)
+
+namespace bitz {
+
+// ---- host GF(2^128) for setup values: residue-class integer clmul ----
+struct hgf {
+    uint64_t lo, hi;
+};
+inline hgf hgf_add(hgf a, hgf b) { return hgf{a.lo ^ b.lo, a.hi ^ b.hi}; }
+inline void hclmul64(uint64_t a, uint64_t b, uint64_t* lo, uint64_t* hi) {
+    const uint64_t M = 0x1111111111111111ull;
+    const unsigned __int128 W = ((unsigned __int128)M << 64) | M;
+    uint64_t bl = b & 0x0fffffffffffffffull;
+    unsigned __int128 a0 = a & M, a1 = a & (M << 1), a2 = a & (M << 2), a3 = a & (M << 3);
+    unsigned __int128 b0 = bl & M, b1 = bl & (M << 1), b2 = bl & (M << 2), b3 = bl & (M << 3);
+    unsigned __int128 z0 = (a0 * b0) ^ (a1 * b3) ^ (a2 * b2) ^ (a3 * b1);
+    unsigned __int128 z1 = (a0 * b1) ^ (a1 * b0) ^ (a2 * b3) ^ (a3 * b2);
+    unsigned __int128 z2 = (a0 * b2) ^ (a1 * b1) ^ (a2 * b0) ^ (a3 * b3);
+    unsigned __int128 z3 = (a0 * b3) ^ (a1 * b2) ^ (a2 * b1) ^ (a3 * b0);
+    unsigned __int128 p = (z0 & W) | (z1 & (W << 1)) | (z2 & (W << 2)) | (z3 & (W << 3));
+    for (int i = 60; i < 64; i++)
+        if ((b >> i) & 1) p ^= ((unsigned __int128)a) << i;
+    *lo = (uint64_t)p;
+    *hi = (uint64_t)(p >> 64);
+}
+inline hgf hgf_mul(hgf a, hgf b) {
+    uint64_t l0, l1, h0, h1, m0, m1;
+    hclmul64(a.lo, b.lo, &l0, &l1);
+    hclmul64(a.hi, b.hi, &h0, &h1);
+    hclmul64(a.lo ^ a.hi, b.lo ^ b.hi, &m0, &m1);
+    uint64_t r0 = l0, r1 = l1 ^ m0 ^ l0 ^ h0, r2 = h0 ^ m1 ^ l1 ^ h1, r3 = h1;
+    uint64_t a0 = r2 ^ (r2 << 1) ^ (r2 << 2) ^ (r2 << 7);
+    uint64_t a1 = (r2 >> 63) ^ (r2 >> 62) ^ (r2 >> 57);
+    uint64_t b0 = r3 ^ (r3 << 1) ^ (r3 << 2) ^ (r3 << 7);
+    uint64_t b1 = (r3 >> 63) ^ (r3 >> 62) ^ (r3 >> 57);
+    return hgf{r0 ^ a0 ^ b1 ^ (b1 << 1) ^ (b1 << 2) ^ (b1 << 7), r1 ^ a1 ^ b0};
+}
+inline hgf hgf_inv(hgf a) {
+    hgf acc{1, 0}, sq = a;
+    for (int i = 1; i < 128; i++) {
+        sq = hgf_mul(sq, sq);
+        acc = hgf_mul(acc, sq);
+    }
+    if (a.lo == 0 && a.hi == 0) acc = hgf{0, 0};
+    return acc;
+}
+
+}  // namespace bitz

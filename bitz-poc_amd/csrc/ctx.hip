@@ -1,0 +1,133 @@
+// Context, error reporting and the small arithmetic parity entry points.
+#include "common.h"
+#include "gf128_dev.h"
+#include "blake3_dev.h"
+
+namespace bitz {
+static thread_local std::string g_err;
+void set_error(const std::string& msg) { g_err = msg; }
+}  // namespace bitz
+
+extern "C" const char* bitz_last_error(void) { return bitz::g_err.c_str(); }
+
+extern "C" const char* bitz_version(void) { return "bitz_hip gfx950 0.1"; }
+
+extern "C" int bitz_ctx_create(int device, bitz_ctx** out) {
+    BITZ_REQUIRE(out != nullptr, BITZ_ERR_ARG, "bitz_ctx_create: null out");
+    int count = 0;
+    BITZ_HIP(hipGetDeviceCount(&count));
+    BITZ_REQUIRE(device >= 0 && device < count, BITZ_ERR_ARG, "bitz_ctx_create: no such device");
+    BITZ_HIP(hipSetDevice(device));
+    auto* ctx = new bitz_ctx();
+    ctx->device = device;
+    if (hipStreamCreateWithFlags(&ctx->own_stream, hipStreamNonBlocking) != hipSuccess) {
+        delete ctx;
+        bitz::set_error("bitz_ctx_create: hipStreamCreate failed");
+        return BITZ_ERR_DEVICE;
+    }
+    ctx->stream = ctx->own_stream;
+    *out = ctx;
+    return BITZ_OK;
+}
+
+extern "C" void bitz_ctx_destroy(bitz_ctx* ctx) {
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->own_stream) {
+        (void)hipStreamSynchronize(ctx->own_stream);
+        (void)hipStreamDestroy(ctx->own_stream);
+    }
+    delete ctx;
+}
+
+extern "C" int bitz_ctx_set_stream(bitz_ctx* ctx, void* stream) {
+    BITZ_REQUIRE(ctx != nullptr, BITZ_ERR_ARG, "bitz_ctx_set_stream: null ctx");
+    ctx->stream = stream ? (hipStream_t)stream : ctx->own_stream;
+    return BITZ_OK;
+}
+
+extern "C" void* bitz_ctx_stream(bitz_ctx* ctx) { return ctx ? (void*)ctx->stream : nullptr; }
+
+extern "C" int bitz_ctx_synchronize(bitz_ctx* ctx) {
+    BITZ_REQUIRE(ctx != nullptr, BITZ_ERR_ARG, "bitz_ctx_synchronize: null ctx");
+    BITZ_HIP(hipStreamSynchronize(ctx->stream));
+    return BITZ_OK;
+}
+
+// ---- parity helpers ----
+
+__global__ void k_gf_mul(const uint4* __restrict__ a, const uint4* __restrict__ b,
+                         uint4* __restrict__ out, uint64_t n) {
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    gf_store(&out[i], gf_mul(gf_load(&a[i]), gf_load(&b[i])));
+}
+
+__global__ void k_gf_mul_const(const uint4* __restrict__ a, uint4 c, uint4* __restrict__ out,
+                               uint64_t n) {
+    __shared__ uint4 tab[512];
+    gf_table_build(tab, gf128{{c.x, c.y, c.z, c.w}}, threadIdx.x, blockDim.x);
+    __syncthreads();
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    gf_store(&out[i], gf_mul_tab(tab, gf_load(&a[i])));
+}
+
+// One thread per leaf: non-root chunk CV of `leaf_size` bytes (<= 1024).
+__global__ void k_blake3_leaves(const uint32_t* __restrict__ data, uint64_t n, uint32_t leaf_words,
+                                uint32_t* __restrict__ out) {
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t* leaf = data + i * leaf_words;
+    uint32_t cv[8];
+    b3::iv_init(cv);
+    uint32_t nblocks = leaf_words == 0 ? 1 : (leaf_words + 15) / 16;
+    for (uint32_t b = 0; b < nblocks; b++) {
+        uint32_t m[16];
+        uint32_t avail = leaf_words - b * 16;
+        for (int k = 0; k < 16; k++) m[k] = (uint32_t)k < avail ? leaf[b * 16 + k] : 0u;
+        uint32_t flags = (b == 0 ? b3::CHUNK_START : 0u) | (b + 1 == nblocks ? b3::CHUNK_END : 0u);
+        uint32_t blen = b + 1 == nblocks ? (avail >= 16 ? 64u : avail * 4u) : 64u;
+        b3::compress(cv, m, 0, blen, flags);
+    }
+    for (int k = 0; k < 8; k++) out[i * 8 + k] = cv[k];
+}
+
+extern "C" int bitz_gf_mul_dev(bitz_ctx* ctx, const void* a, const void* b, void* out, uint64_t n) {
+    BITZ_REQUIRE(ctx && a && b && out, BITZ_ERR_ARG, "bitz_gf_mul_dev: null argument");
+    if (n == 0) return BITZ_OK;
+    BITZ_HIP(hipSetDevice(ctx->device));
+    uint64_t blocks = (n + 255) / 256;
+    hipLaunchKernelGGL(k_gf_mul, dim3((unsigned)blocks), dim3(256), 0, ctx->stream,
+                       (const uint4*)a, (const uint4*)b, (uint4*)out, n);
+    BITZ_LAUNCH_CHECK();
+    return BITZ_OK;
+}
+
+extern "C" int bitz_gf_mul_const_dev(bitz_ctx* ctx, const void* a, const uint8_t c[16], void* out,
+                                     uint64_t n) {
+    BITZ_REQUIRE(ctx && a && c && out, BITZ_ERR_ARG, "bitz_gf_mul_const_dev: null argument");
+    if (n == 0) return BITZ_OK;
+    BITZ_HIP(hipSetDevice(ctx->device));
+    uint4 cc;
+    memcpy(&cc, c, 16);
+    uint64_t blocks = (n + 255) / 256;
+    hipLaunchKernelGGL(k_gf_mul_const, dim3((unsigned)blocks), dim3(256), 0, ctx->stream,
+                       (const uint4*)a, cc, (uint4*)out, n);
+    BITZ_LAUNCH_CHECK();
+    return BITZ_OK;
+}
+
+extern "C" int bitz_blake3_leaves_dev(bitz_ctx* ctx, const void* data, uint64_t n,
+                                      uint64_t leaf_size, void* out) {
+    BITZ_REQUIRE(ctx && data && out, BITZ_ERR_ARG, "bitz_blake3_leaves_dev: null argument");
+    BITZ_REQUIRE(leaf_size <= 1024 && leaf_size % 4 == 0, BITZ_ERR_SHAPE,
+                 "bitz_blake3_leaves_dev: leaf_size must be a multiple of 4 and <= 1024");
+    if (n == 0) return BITZ_OK;
+    BITZ_HIP(hipSetDevice(ctx->device));
+    uint64_t blocks = (n + 255) / 256;
+    hipLaunchKernelGGL(k_blake3_leaves, dim3((unsigned)blocks), dim3(256), 0, ctx->stream,
+                       (const uint32_t*)data, n, (uint32_t)(leaf_size / 4), (uint32_t*)out);
+    BITZ_LAUNCH_CHECK();
+    return BITZ_OK;
+}

@@ -1,0 +1,46 @@
+// Internal (C++) entry points of the opening kernels, shared by the C ABI
+// wrappers and the prover orchestration. All pointers are device pointers
+// unless named *_host; work is enqueued on ctx->stream.
+#pragma once
+#include "common.h"
+
+namespace bitz {
+
+// Sums produced by the reduction kernels land in ctx->scratch.result (or in
+// out_dev when non-null); read_result copies K field elements to the host.
+int read_result(bitz_ctx* ctx, int K, uint8_t* host_out);
+
+int eq_table(bitz_ctx* ctx, const void* r_dev, int n, void* out);
+int fold_columns(bitz_ctx* ctx, const uint64_t* rows, int t, int s, const void* w, void* out);
+int pow_images(bitz_ctx* ctx, const uint8_t g[16], const void* exps, uint64_t n, void* out);
+int pack_columns(bitz_ctx* ctx, const uint64_t* rows, int t, int s, uint64_t* packed);
+int gkr_round_sums(bitz_ctx* ctx, const void* l, const void* r, uint64_t h, const void* eq_c, int s_c,
+                   const void* eq_y, int send_one, void* out_dev);
+int gkr_leaf_round_sums(bitz_ctx* ctx, const uint64_t* packed, const void* A, int t, int s, const void* eq_c,
+                        int s_c, const void* eq_y, int send_one, void* out_dev);
+int fold_halves(bitz_ctx* ctx, void* v, uint64_t h, const uint8_t rho[16]);
+int gkr_leaf_fold(bitz_ctx* ctx, const uint64_t* packed, const void* A, int t, int s, const uint8_t rho[16],
+                  void* l_out, void* r_out);
+int gkr_level1(bitz_ctx* ctx, const uint64_t* packed, const void* A, int t, int s, void* out);
+int product_level(bitz_ctx* ctx, const void* in, uint64_t n, void* out);
+int xi_combined(bitz_ctx* ctx, const uint64_t* packed, int t, int s, const void* w2, void* out);
+int fold_rows_eq(bitz_ctx* ctx, const uint64_t* rows, int t, int s, const void* eq, void* out);
+int round_poly(bitz_ctx* ctx, const void* m, const void* w, uint64_t n, void* out_dev);
+int fold_lsb(bitz_ctx* ctx, const void* v, uint64_t n, const uint8_t r[16], void* out);
+int fold_1b_rows(bitz_ctx* ctx, const void* p, const void* suffix, uint64_t n, void* out_dev);
+int reduce_columns(bitz_ctx* ctx, const uint4* partials, uint32_t nblocks, uint32_t width, uint4* out);
+int fold_b128(bitz_ctx* ctx, const void* suffix, uint64_t n, const void* w, void* out);
+int axpy(bitz_ctx* ctx, void* acc, const void* e, uint64_t n, const uint8_t eta[16]);
+int lig_msg(bitz_ctx* ctx, const void* f, const void* b, uint64_t n, int with_eval, void* out_dev);
+int lig_fold(bitz_ctx* ctx, const void* f, const void* b, uint64_t n, const uint8_t r[16], void* fo, void* bo);
+int induce_basis(bitz_ctx* ctx, const uint64_t* queries_dev, uint32_t nq, int log_n, const void* sks,
+                 const void* inv_sks, const void* aw, void* out);
+int pow_find(bitz_ctx* ctx, const uint8_t prefix[16], const uint8_t seed[16], uint32_t bits, uint64_t* nonce);
+int gather_rows(bitz_ctx* ctx, const void* cw, const uint64_t* q_dev, uint32_t nq, uint32_t lanes, void* out);
+int gather_nodes(bitz_ctx* ctx, const void* tree, const uint64_t* idx_dev, uint32_t n, void* out);
+
+// commit pipeline (commit.hip)
+int commit_device(bitz_ctx* ctx, const void* packed, int log_msg_len, int log_batch, int log_inv_rate,
+                  void* codeword, void* tree);
+
+}  // namespace bitz

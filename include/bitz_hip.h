@@ -1,0 +1,166 @@
+/*
+ * bitz_hip.h -- C ABI of the MI355X (gfx950) BitZ prover hot path.
+ *
+ * Plain pointers and sizes only. Every entry point returns an int status:
+ * BITZ_OK (0) or a negative BITZ_ERR_* code; bitz_last_error() returns a
+ * thread-local message for the last failure. No exception or panic crosses
+ * this boundary. A bitz_ctx binds one device and one HIP stream; contexts and
+ * handles are not thread-safe (one thread per context).
+ *
+ * Device-pointer entry points ("_dev") take device memory the caller owns
+ * (hipMalloc, or a torch tensor's data_ptr()) and enqueue work on the
+ * context's stream; they return after enqueueing unless documented as
+ * synchronizing. Gf128 buffers are arrays of 16-byte elements in the
+ * reference's wire form (lo_le || hi_le, src/bitz/codec.rs:4-21).
+ *
+ * Reference interfaces each entry point replaces (albert-garreta/BitZ-PoC):
+ *   bitz_ntt_forward_dev  flock_core AdditiveNttF128::forward_transform_interleaved_from_layer
+ *                         (vendor/flock-mod/crates/flock-core/src/ntt/additive_ntt_f128.rs:205)
+ *   bitz_merkle_tree_dev  flock_core merkle::merkle_tree (merkle.rs:550) under HashKind::Blake3
+ *   bitz_commit_dev       flock_core pcs::commit::commit (pcs/commit.rs:180), Blake3 Merkle
+ *   bitz_commit           bitz::Pcs::commit (src/bitz/pcs.rs:270-284) ->
+ *                         ligerito_flock::commit_rs_ligerito_rows (src/ligerito_flock.rs:270)
+ */
+#ifndef BITZ_HIP_H
+#define BITZ_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define BITZ_OK 0
+#define BITZ_ERR_SHAPE (-1)  /* geometry rejected (reference: assert/ShapeError) */
+#define BITZ_ERR_ALLOC (-2)  /* device allocation failed */
+#define BITZ_ERR_DEVICE (-3) /* HIP runtime error */
+#define BITZ_ERR_ARG (-4)    /* null or inconsistent argument */
+
+typedef struct bitz_ctx bitz_ctx;
+typedef struct bitz_commit_handle bitz_commit_handle;
+
+/* ---- context ---- */
+int bitz_ctx_create(int device, bitz_ctx** out);
+void bitz_ctx_destroy(bitz_ctx* ctx);
+/* Use an existing HIP stream (e.g. torch.cuda.current_stream().cuda_stream);
+ * NULL restores the context's own stream. */
+int bitz_ctx_set_stream(bitz_ctx* ctx, void* hip_stream);
+void* bitz_ctx_stream(bitz_ctx* ctx);
+int bitz_ctx_synchronize(bitz_ctx* ctx);
+const char* bitz_last_error(void);
+/* Library build tag (architecture and version). */
+const char* bitz_version(void);
+
+/* ---- arithmetic parity helpers (device pointers) ---- */
+/* out[i] = a[i] * b[i] in GF(2^128) (variable x variable path) */
+int bitz_gf_mul_dev(bitz_ctx* ctx, const void* a, const void* b, void* out, uint64_t n);
+/* out[i] = a[i] * c in GF(2^128) through the LDS window table of c */
+int bitz_gf_mul_const_dev(bitz_ctx* ctx, const void* a, const uint8_t c[16], void* out,
+                          uint64_t n);
+/* BLAKE3 non-root leaf chaining values of n equal leaves of leaf_size bytes
+ * (leaf_size <= 1024): out = n * 32 bytes */
+int bitz_blake3_leaves_dev(bitz_ctx* ctx, const void* data, uint64_t n, uint64_t leaf_size,
+                           void* out);
+
+/* ---- Reed-Solomon encode / Merkle / commit (device pointers) ---- */
+/* In-place interleaved forward additive NTT over GF(2^128), LCH novel basis
+ * with the standard subspace basis {1, X, ..., X^(ell-1)}:
+ * data[pos * 2^log_lanes + lane], 2^log_d positions, layers start_layer..log_d-1.
+ * Requires log_d <= ell <= 64, start_layer <= log_d, log_lanes <= 6. */
+int bitz_ntt_forward_dev(bitz_ctx* ctx, void* data, int log_d, int log_lanes, int start_layer,
+                         int ell);
+/* Flat BLAKE3 Merkle tree (merkle.rs layout: leaves first, root last,
+ * 2*num_leaves-1 hashes of 32 B). num_leaves a power of two, leaf_size <= 1024. */
+int bitz_merkle_tree_dev(bitz_ctx* ctx, const void* data, uint64_t num_leaves,
+                         uint64_t leaf_size, void* tree);
+/* PCS commit of a packed message (2^log_msg_len Gf128): replicate into
+ * 2^log_inv_rate copies, NTT from layer log_inv_rate, BLAKE3 tree over
+ * positions (leaf = 2^log_batch Gf128). codeword: 2^(log_msg_len+log_inv_rate)
+ * Gf128; tree: (2*2^k_code-1)*32 B with k_code = log_msg_len-log_batch+log_inv_rate.
+ * root (host, 32 B) may be NULL; when given the call synchronizes the stream. */
+int bitz_commit_dev(bitz_ctx* ctx, const void* packed, int log_msg_len, int log_batch,
+                    int log_inv_rate, void* codeword, void* tree, uint8_t* root);
+
+/* ---- opening-side kernels (device pointers) ----
+ * Each replaces the reference function named beside it; field-element
+ * buffers are Gf128 arrays, "rows" is the column-major bit matrix (column c
+ * = 2^(t-6) u64 words), "packed" the row-major packing (packed[g*2^t+b] =
+ * bit b of columns 64g..64g+63). Results written to out_host are copied to
+ * host memory and the call synchronizes. */
+/* build_eq_x_r_vec (src/poly/utils.rs:46): out[x] = prod_i (x_i ? r_i : 1+r_i), n <= 28 */
+int bitz_eq_table_dev(bitz_ctx* ctx, const uint8_t* r_host, int n, void* out);
+/* fold_columns (src/bitz/fold.rs:80-170): out[c] = sum_b w[b] bit(c,b), wrapping u128 */
+int bitz_fold_columns_dev(bitz_ctx* ctx, const void* rows, int t, int s, const void* w, void* out);
+/* column_images/row_images via FixedBasePow (fold.rs:173-182, preparation.rs:177-263):
+ * out[i] = g^exps[i], exps are u128 */
+int bitz_pow_images_dev(bitz_ctx* ctx, const uint8_t g[16], const void* exps, uint64_t n, void* out);
+/* pack_columns_from_rows (src/ligerito.rs:655-690) */
+int bitz_pack_columns_dev(bitz_ctx* ctx, const void* rows, int t, int s, void* packed);
+/* kernels::round_sums (src/bitz/kernels.rs:80-110) over materialised halves l, r
+ * (2h entries each), weights eq_y[x >> s_c] * eq_c[x & (2^s_c-1)];
+ * out_host = [sum eq E_end O_end, sum eq (E_hi-E_lo)(O_hi-O_lo)] */
+int bitz_gkr_round_sums_dev(bitz_ctx* ctx, const void* l, const void* r, uint64_t h, const void* eq_c,
+                            int s_c, const void* eq_y, int send_one, uint8_t out_host[32]);
+/* the same sums on the leaf layer, leaves read from the bits and row images
+ * (Forest::bit_round, src/bitz/forest.rs:381-520) */
+int bitz_gkr_leaf_round_sums_dev(bitz_ctx* ctx, const void* packed, const void* A, int t, int s,
+                                 const void* eq_c, int s_c, const void* eq_y, int send_one,
+                                 uint8_t out_host[32]);
+/* fold_halves (src/bitz/gkr.rs:170-177): v[i] += rho (v[i+h] - v[i]), i < h */
+int bitz_fold_halves_dev(bitz_ctx* ctx, void* v, uint64_t h, const uint8_t rho[16]);
+/* the leaf layer folded once by rho into materialised halves (2^(t+s-2) each) */
+int bitz_gkr_leaf_fold_dev(bitz_ctx* ctx, const void* packed, const void* A, int t, int s,
+                           const uint8_t rho[16], void* l_out, void* r_out);
+/* level 1 of the batched product tree from the bits (GrandProductCircuit::batched_eval, gkr.rs:93) */
+int bitz_gkr_level1_dev(bitz_ctx* ctx, const void* packed, const void* A, int t, int s, void* out);
+/* one product level: out[i] = in[i] * in[i + n/2] (gkr.rs:93-108) */
+int bitz_product_level_dev(bitz_ctx* ctx, const void* in, uint64_t n, void* out);
+/* xi_combined_rows_packed (src/ligerito.rs:563-630): out[b] = sum_c w2[c] bit(c,b) */
+int bitz_xi_combined_dev(bitz_ctx* ctx, const void* packed, int t, int s, const void* w2, void* out);
+/* fold_rows_eq (src/bitz/sumcheck.rs:84-131): out[c] = sum_b eq[b] bit(c,b) */
+int bitz_fold_rows_eq_dev(bitz_ctx* ctx, const void* rows, int t, int s, const void* eq, void* out);
+/* round_polynomial (src/bitz/sumcheck.rs:404-447): out_host = [a0, a0+a1+q, q] partial sums
+ * reported as [sum m0 w0, sum m1 w1, sum (m0+m1)(w0+w1)] */
+int bitz_round_poly_dev(bitz_ctx* ctx, const void* m, const void* w, uint64_t n, uint8_t out_host[48]);
+/* sumcheck fold (sumcheck.rs:454-465): out[i] = v[2i] + r (v[2i] + v[2i+1]); out != v */
+int bitz_fold_lsb_dev(bitz_ctx* ctx, const void* v, uint64_t n, const uint8_t r[16], void* out);
+/* fold_1b_rows_naive (pcs/ring_switch.rs:1332-1381): out[i] = sum_x bit_i(p[x]) suffix[x], 128 outputs */
+int bitz_fold_1b_rows_dev(bitz_ctx* ctx, const void* p, const void* suffix, uint64_t n, void* out);
+/* fold_b128_elems (ring_switch.rs:1486-1560): out[x] = sum over set bits i of suffix[x] of w[i] */
+int bitz_fold_b128_dev(bitz_ctx* ctx, const void* suffix, uint64_t n, const void* w, void* out);
+/* acc[x] += eta * e[x] (SumcheckProver::glue, ligerito.rs:3125; add_ood_basis, src/ligerito_flock.rs:1590) */
+int bitz_axpy_dev(bitz_ctx* ctx, void* acc, const void* e, uint64_t n, const uint8_t eta[16]);
+/* round_msg_lsb / round_msg_and_eval_lsb (ligerito.rs:2499-2600): out_host = [u0, u2(, y)] */
+int bitz_lig_msg_dev(bitz_ctx* ctx, const void* f, const void* b, uint64_t n, int with_eval,
+                     uint8_t out_host[48]);
+/* fold_and_msg_lsb's fold (ligerito.rs:2634-2680): f'[j] = f0 + r (f0+f1), b' likewise */
+int bitz_lig_fold_dev(bitz_ctx* ctx, const void* f, const void* b, uint64_t n, const uint8_t r[16],
+                      void* fo, void* bo);
+/* induce_sumcheck_poly basis (ligerito.rs:1968-2070): out[j] = sum_i aw[i] X_j(q_i), 2^log_n outputs */
+int bitz_induce_basis_dev(bitz_ctx* ctx, const uint64_t* queries_host, uint32_t nq, int log_n,
+                          const void* alpha_w, void* out);
+/* find_pow (src/bitz/pcs.rs:1252-1267): smallest nonce with >= bits leading zero bits of
+ * BLAKE3(prefix16 || seed16 || nonce_le) */
+int bitz_pow_find_dev(bitz_ctx* ctx, const uint8_t prefix[16], const uint8_t seed[16], uint32_t bits,
+                      uint64_t* nonce);
+
+/* ---- host-buffer commit with a device-resident handle ---- */
+/* rows: 2^s columns, each 2^t bits as 2^(t-6) u64 words, column-major
+ * (rows + c*2^(t-6)); the packed message is exactly these words read as
+ * Gf128 (src/ligerito_flock.rs:198-217). Keeps rows/packed message, codeword
+ * and tree on the device inside the handle. Synchronizing. */
+int bitz_commit(bitz_ctx* ctx, const uint64_t* rows, uint32_t t, uint32_t s, uint32_t log_batch,
+                uint32_t log_inv_rate, uint8_t root[32], bitz_commit_handle** out);
+void bitz_commit_free(bitz_commit_handle* h);
+/* device pointers held by the handle (valid until bitz_commit_free) */
+const void* bitz_commit_packed(const bitz_commit_handle* h);
+const void* bitz_commit_codeword(const bitz_commit_handle* h);
+const void* bitz_commit_tree(const bitz_commit_handle* h);
+/* copy one codeword position (2^log_batch Gf128) and the tree to the host */
+int bitz_commit_read_tree(bitz_commit_handle* h, uint8_t* tree_out, uint64_t bytes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

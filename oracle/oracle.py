@@ -190,3 +190,197 @@ def commit(packed: np.ndarray, log_msg_len: int, log_batch: int, log_inv_rate: i
     if rc != 0:
         raise ValueError(f"or_commit rejected the geometry (rc={rc})")
     return root.tobytes(), cw, tree
+
+
+# ---------------------------------------------------------------- opening side
+def _sig_open():
+    P = C.c_void_p
+    sig = {
+        "or_eq_table": (None, [P, C.c_int, P]),
+        "or_mle_eval": (GF, [P, P, C.c_int]),
+        "or_fold_columns": (None, [P, C.c_int, C.c_int, P, P]),
+        "or_pow_many": (None, [GF, P, C.c_size_t, P]),
+        "or_forest_leaves": (None, [P, C.c_int, C.c_int, P, P]),
+        "or_product_level": (None, [P, C.c_uint64, P]),
+        "or_gkr_round_sums": (None, [P, P, C.c_uint64, P, C.c_int, P]),
+        "or_fold_halves": (None, [P, C.c_uint64, GF]),
+        "or_xi_combined_rows": (None, [P, C.c_int, C.c_int, P, P]),
+        "or_fold_rows_eq": (None, [P, C.c_int, C.c_int, P, P]),
+        "or_round_polynomial": (None, [P, P, C.c_uint64, P]),
+        "or_fold_lsb": (None, [P, C.c_uint64, GF]),
+        "or_fold_1b_rows": (None, [P, P, C.c_uint64, P]),
+        "or_fold_b128_elems": (None, [P, C.c_uint64, P, P]),
+        "or_batch_claims": (GF, [P, P]),
+        "or_lig_round_msg": (None, [P, P, C.c_uint64, P]),
+        "or_lig_glue": (None, [P, P, C.c_uint64, GF]),
+        "or_eval_sk_at_vks": (None, [C.c_int, P]),
+        "or_induce_basis": (None, [C.c_int, P, P, P, C.c_size_t, P]),
+        "or_pow_valid": (C.c_int, [P, P, C.c_uint64, C.c_uint32]),
+        "or_pow_find": (C.c_uint64, [P, P, C.c_uint32]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+
+
+_sig_open()
+
+
+def _gfa(a) -> np.ndarray:
+    return np.ascontiguousarray(np.asarray(a, dtype=np.uint64).reshape(-1, 2))
+
+
+def ints_to_gf(xs) -> np.ndarray:
+    return np.array([[x & (2**64 - 1), x >> 64] for x in xs], np.uint64).reshape(-1, 2)
+
+
+def gf_to_ints(a) -> list[int]:
+    a = np.asarray(a, dtype=np.uint64).reshape(-1, 2)
+    return [int(x[0]) | (int(x[1]) << 64) for x in a]
+
+
+def eq_table(r) -> np.ndarray:
+    r = _gfa(r)
+    out = np.zeros((1 << len(r), 2), np.uint64)
+    lib.or_eq_table(_ptr(r), len(r), _ptr(out))
+    return out
+
+
+def mle_eval(v, point) -> int:
+    v, point = _gfa(v), _gfa(point)
+    assert v.shape[0] == 1 << point.shape[0]
+    return gf_int(lib.or_mle_eval(_ptr(v), _ptr(point), point.shape[0]))
+
+
+def fold_columns(rows: np.ndarray, t: int, s: int, weights) -> np.ndarray:
+    rows = np.ascontiguousarray(rows, dtype=np.uint64).reshape(-1)
+    w = _gfa(weights)
+    out = np.zeros((1 << s, 2), np.uint64)
+    lib.or_fold_columns(_ptr(rows), t, s, _ptr(w), _ptr(out))
+    return out
+
+
+def pow_many(g: int, exps) -> np.ndarray:
+    e = _gfa(exps)
+    out = np.zeros_like(e)
+    lib.or_pow_many(gf(g), _ptr(e), e.shape[0], _ptr(out))
+    return out
+
+
+def forest_leaves(rows, t, s, images) -> np.ndarray:
+    rows = np.ascontiguousarray(rows, dtype=np.uint64).reshape(-1)
+    im = _gfa(images)
+    out = np.zeros((1 << (t + s), 2), np.uint64)
+    lib.or_forest_leaves(_ptr(rows), t, s, _ptr(im), _ptr(out))
+    return out
+
+
+def product_level(v) -> np.ndarray:
+    v = _gfa(v)
+    out = np.zeros((v.shape[0] // 2, 2), np.uint64)
+    lib.or_product_level(_ptr(v), v.shape[0], _ptr(out))
+    return out
+
+
+def gkr_round_sums(l, r, eq, send_one: bool) -> tuple[int, int]:
+    l, r, eq = _gfa(l), _gfa(r), _gfa(eq)
+    h = eq.shape[0]
+    out = np.zeros((2, 2), np.uint64)
+    lib.or_gkr_round_sums(_ptr(l), _ptr(r), h, _ptr(eq), 1 if send_one else 0, _ptr(out))
+    return tuple(gf_to_ints(out))
+
+
+def fold_halves(v, rho: int) -> np.ndarray:
+    v = _gfa(v).copy()
+    h = v.shape[0] // 2
+    lib.or_fold_halves(_ptr(v), h, gf(rho))
+    return v[:h].copy()
+
+
+def xi_combined_rows(rows, t, s, w2) -> np.ndarray:
+    rows = np.ascontiguousarray(rows, dtype=np.uint64).reshape(-1)
+    w2 = _gfa(w2)
+    out = np.zeros((1 << t, 2), np.uint64)
+    lib.or_xi_combined_rows(_ptr(rows), t, s, _ptr(w2), _ptr(out))
+    return out
+
+
+def fold_rows_eq(rows, t, s, eq) -> np.ndarray:
+    rows = np.ascontiguousarray(rows, dtype=np.uint64).reshape(-1)
+    eq = _gfa(eq)
+    out = np.zeros((1 << s, 2), np.uint64)
+    lib.or_fold_rows_eq(_ptr(rows), t, s, _ptr(eq), _ptr(out))
+    return out
+
+
+def round_polynomial(m, w) -> list[int]:
+    m, w = _gfa(m), _gfa(w)
+    out = np.zeros((3, 2), np.uint64)
+    lib.or_round_polynomial(_ptr(m), _ptr(w), m.shape[0], _ptr(out))
+    return gf_to_ints(out)
+
+
+def fold_lsb(v, r: int) -> np.ndarray:
+    v = _gfa(v).copy()
+    lib.or_fold_lsb(_ptr(v), v.shape[0], gf(r))
+    return v[: v.shape[0] // 2].copy()
+
+
+def fold_1b_rows(p, suffix) -> np.ndarray:
+    p, suffix = _gfa(p), _gfa(suffix)
+    out = np.zeros((128, 2), np.uint64)
+    lib.or_fold_1b_rows(_ptr(p), _ptr(suffix), p.shape[0], _ptr(out))
+    return out
+
+
+def fold_b128_elems(suffix, w) -> np.ndarray:
+    suffix, w = _gfa(suffix), _gfa(w)
+    out = np.zeros_like(suffix)
+    lib.or_fold_b128_elems(_ptr(suffix), suffix.shape[0], _ptr(w), _ptr(out))
+    return out
+
+
+def batch_claims(claims, w) -> int:
+    claims, w = _gfa(claims), _gfa(w)
+    return gf_int(lib.or_batch_claims(_ptr(claims), _ptr(w)))
+
+
+def lig_round_msg(f, b) -> tuple[int, int]:
+    f, b = _gfa(f), _gfa(b)
+    out = np.zeros((2, 2), np.uint64)
+    lib.or_lig_round_msg(_ptr(f), _ptr(b), f.shape[0], _ptr(out))
+    return tuple(gf_to_ints(out))
+
+
+def lig_glue(acc, bnew, alpha: int) -> np.ndarray:
+    acc, bnew = _gfa(acc).copy(), _gfa(bnew)
+    lib.or_lig_glue(_ptr(acc), _ptr(bnew), acc.shape[0], gf(alpha))
+    return acc
+
+
+def eval_sk_at_vks(log_n: int) -> np.ndarray:
+    out = np.zeros((log_n + 1, 2), np.uint64)
+    lib.or_eval_sk_at_vks(log_n, _ptr(out))
+    return out
+
+
+def induce_basis(log_n: int, queries, alpha_w) -> np.ndarray:
+    sks = eval_sk_at_vks(log_n)
+    q = np.ascontiguousarray(np.asarray(queries, dtype=np.uint64))
+    aw = _gfa(alpha_w)
+    out = np.zeros((1 << log_n, 2), np.uint64)
+    lib.or_induce_basis(log_n, _ptr(sks), _ptr(q), _ptr(aw), len(q), _ptr(out))
+    return out
+
+
+def pow_find(prefix: bytes, seed: bytes, bits: int) -> int:
+    pb = np.frombuffer(prefix, np.uint8).copy()
+    sb = np.frombuffer(seed, np.uint8).copy()
+    return int(lib.or_pow_find(_ptr(pb), _ptr(sb), bits))
+
+
+def pow_valid(prefix: bytes, seed: bytes, nonce: int, bits: int) -> bool:
+    pb = np.frombuffer(prefix, np.uint8).copy()
+    sb = np.frombuffer(seed, np.uint8).copy()
+    return bool(lib.or_pow_valid(_ptr(pb), _ptr(sb), nonce, bits))

@@ -1,0 +1,137 @@
+"""GPU parity of the commit slice (GF(2^128) arithmetic, BLAKE3 leaves,
+interleaved additive NTT, Merkle tree, PCS commit) against the CPU oracle,
+bit-exact, through the C ABI. Full-size cases (2^20 and 2^24 u64
+coefficients at rate 1/4, 16 lanes) are checked against the oracle directly
+at 2^20 and through size-independent properties at 2^24 (RS linearity,
+tree self-consistency, spot leaves re-hashed by the oracle)."""
+import numpy as np
+import pytest
+
+import oracle as O
+from gpu_util import ctx, log, rand_gf, to_dev, to_host
+
+pytestmark = pytest.mark.gpu
+
+
+def test_gf_mul_matches_oracle():
+    n = 1 << 16
+    a, b = rand_gf(n, 1), rand_gf(n, 2)
+    a[:8] = 0
+    b[8:16] = [[1, 0]] * 8
+    a[16] = [0, 1 << 63]
+    b[16] = [0, 1 << 63]
+    log(f"gf_mul n={n}")
+    got = to_host(ctx().gf_mul(to_dev(a), to_dev(b)))
+    assert np.array_equal(got, O.mul_arrays(a, b))
+
+
+@pytest.mark.parametrize("c", [0, 1, 2, 0x87, (1 << 127) | 5, 0x0123456789ABCDEF_FEDCBA9876543210])
+def test_gf_mul_const_matches_oracle(c):
+    n = 4099
+    a = rand_gf(n, 3)
+    log(f"gf_mul_const c={c:#x}")
+    got = to_host(ctx().gf_mul_const(to_dev(a), c))
+    cc = np.tile(np.array([[c & (2**64 - 1), c >> 64]], np.uint64), (n, 1))
+    assert np.array_equal(got, O.mul_arrays(a, cc))
+
+
+@pytest.mark.parametrize("leaf_size", [4, 32, 64, 128, 256, 260, 512, 1024])
+def test_blake3_leaves(leaf_size):
+    import torch
+    n = 777
+    rng = np.random.default_rng(leaf_size)
+    data = rng.integers(0, 256, size=n * leaf_size, dtype=np.uint8)
+    log(f"blake3_leaves leaf_size={leaf_size}")
+    got = ctx().blake3_leaves(torch.from_numpy(data).cuda(), n, leaf_size).cpu().numpy()
+    for i in range(0, n, 37):
+        assert bytes(got[i]) == O.leaf_cv(data[i * leaf_size:(i + 1) * leaf_size].tobytes())
+
+
+@pytest.mark.parametrize("log_d,log_lanes,start,ell", [
+    (1, 0, 0, 1), (3, 0, 0, 3), (5, 2, 1, 5), (8, 4, 0, 8), (9, 0, 0, 12), (10, 4, 2, 10),
+    (12, 3, 0, 12), (13, 4, 2, 13), (14, 1, 3, 16), (16, 4, 2, 16), (17, 4, 2, 17)])
+def test_ntt_matches_oracle(log_d, log_lanes, start, ell):
+    data = rand_gf(1 << (log_d + log_lanes), log_d * 100 + log_lanes)
+    log(f"ntt log_d={log_d} lanes=2^{log_lanes} start={start} ell={ell}")
+    d = to_dev(data)
+    ctx().ntt_forward_(d, log_d, log_lanes, start, ell)
+    assert np.array_equal(to_host(d), O.ntt_forward(data, log_d, log_lanes, start, ell))
+
+
+@pytest.mark.parametrize("n_leaves,leaf", [(1, 64), (2, 32), (64, 256), (1024, 128), (4096, 64),
+                                           (1 << 15, 16)])
+def test_merkle_tree_matches_oracle(n_leaves, leaf):
+    import torch
+    rng = np.random.default_rng(n_leaves)
+    data = rng.integers(0, 256, size=n_leaves * leaf, dtype=np.uint8)
+    log(f"merkle n_leaves={n_leaves} leaf={leaf}")
+    got = ctx().merkle_tree(torch.from_numpy(data).cuda(), n_leaves).cpu().numpy()
+    assert np.array_equal(got, O.merkle_tree(data, n_leaves))
+
+
+@pytest.mark.parametrize("log_msg,log_batch,r", [
+    (5, 2, 1), (8, 4, 2), (10, 3, 3), (12, 4, 1), (13, 5, 2), (12, 6, 2), (16, 4, 3),
+    (19, 4, 2),  # 2^20 u64 coefficients (n=26), rate 1/4, 16 lanes
+])
+def test_commit_matches_oracle(log_msg, log_batch, r):
+    msg = rand_gf(1 << log_msg, 1000 + log_msg * 10 + r)
+    log(f"commit log_msg={log_msg} log_batch={log_batch} r={r}")
+    root, cw, tree = ctx().commit(to_dev(msg), log_msg, log_batch, r)
+    o_root, o_cw, o_tree = O.commit(msg, log_msg, log_batch, r)
+    assert root == o_root
+    assert np.array_equal(to_host(cw), o_cw)
+    assert np.array_equal(tree.cpu().numpy(), o_tree)
+
+
+def test_commit_metric_size_properties():
+    """2^24 u64 coefficients (n=30, m_p=23), rate 1/4, 16 lanes: the codeword
+    is F2-linear in the message; leaves and the path to the root re-hash
+    with the oracle."""
+    import torch
+    log_msg, log_batch, r = 23, 4, 2
+    n_leaves = 1 << (log_msg - log_batch + r)
+    c = ctx()
+    log("commit 2^24 coefficients: a, b, a^b")
+    a, b = rand_gf(1 << log_msg, 7), rand_gf(1 << log_msg, 8)
+    da, db = to_dev(a), to_dev(b)
+    ra, cwa, ta = c.commit(da, log_msg, log_batch, r)
+    _, cwb, _ = c.commit(db, log_msg, log_batch, r)
+    _, cwab, _ = c.commit(da ^ db, log_msg, log_batch, r)
+    assert torch.equal(cwa ^ cwb, cwab)
+    # the first 2^log_dim positions after r replicate layers are NOT the
+    # message (the NTT is non-systematic) -- but the codeword is nonzero
+    assert bool(cwa.any())
+    tree = ta.cpu().numpy()
+    assert bytes(tree[-1]) == ra
+    cw_host = to_host(cwa)
+    leaf_elems = 1 << log_batch
+    rng = np.random.default_rng(0)
+    for pos in list(rng.integers(0, n_leaves, 16)) + [0, n_leaves - 1]:
+        leaf = cw_host[pos * leaf_elems:(pos + 1) * leaf_elems].tobytes()
+        assert bytes(tree[pos]) == O.leaf_cv(leaf)
+    # parents re-hash at every level along one path
+    off, length, idx = 0, n_leaves, int(rng.integers(0, n_leaves))
+    while length > 1:
+        left = idx & ~1
+        parent = bytes(tree[off + length + (idx >> 1)])
+        assert parent == O.parent_cv(bytes(tree[off + left]), bytes(tree[off + left + 1]))
+        off += length
+        length >>= 1
+        idx >>= 1
+    # one lane of the codeword equals the oracle NTT of that lane's message
+    lane = 5
+    msg_lane = np.ascontiguousarray(a[lane::leaf_elems])
+    log_dim = log_msg - log_batch
+    padded = np.zeros(((1 << (log_dim + r)), 2), np.uint64)
+    padded[: 1 << log_dim] = msg_lane
+    expect = O.ntt_forward(padded, log_dim + r, 0)
+    assert np.array_equal(cw_host[lane::leaf_elems], expect)
+
+
+def test_commit_rejects_bad_geometry():
+    from bitz_amd import BitzError
+    msg = to_dev(rand_gf(16, 1))
+    with pytest.raises(BitzError):
+        ctx().commit(msg, 4, 4, 1)
+    with pytest.raises(BitzError):
+        ctx().commit(msg, 4, 2, 0)

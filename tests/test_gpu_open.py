@@ -1,0 +1,211 @@
+"""GPU parity of the opening-side kernels against the CPU oracle, bit-exact,
+through the C ABI: eq tables, integer column folds and images, the GKR round
+sums / folds (materialised and leaf layer), inner-sumcheck tables and rounds,
+ring-switch folds, Ligerito sumcheck messages/folds, the induced basis and the
+proof-of-work scan. Sizes include the 2^20-coefficient workload's shapes
+(t=15, s=11, m_p=19)."""
+import functools
+import random
+
+import numpy as np
+import pytest
+
+import oracle as O
+from gpu_util import ctx, log, rand_gf, to_dev, to_host
+
+pytestmark = pytest.mark.gpu
+
+
+@functools.lru_cache(maxsize=1)
+def ops():
+    from bitz_amd.device import OpenOps
+    return OpenOps(ctx())
+
+
+def rand_ints(k, seed):
+    rng = random.Random(seed)
+    return [rng.getrandbits(128) for _ in range(k)]
+
+
+def rand_rows(t, s, seed):
+    rng = np.random.default_rng(seed)
+    return rng.integers(0, 2**64, size=(1 << s) << (t - 6), dtype=np.uint64)
+
+
+def dev_u64(a):
+    import torch
+    return torch.from_numpy(np.ascontiguousarray(a).view(np.int64)).to("cuda")
+
+
+@pytest.mark.parametrize("n", [0, 1, 5, 12, 13, 19, 22])
+def test_eq_table(n):
+    r = rand_ints(n, n)
+    r[:1] = [0] if n > 3 else r[:1]
+    log(f"eq_table n={n}")
+    got = to_host(ops().eq_table(r))
+    assert np.array_equal(got, O.eq_table(O.ints_to_gf(r)))
+
+
+@pytest.mark.parametrize("t,s", [(7, 0), (9, 3), (15, 11)])
+def test_fold_columns_and_images(t, s):
+    rows = rand_rows(t, s, t + s)
+    w = rand_gf(1 << t, 5)
+    w[::7] = [2**64 - 1, 2**64 - 1]  # force carries and wrap-around
+    log(f"fold_columns t={t} s={s}")
+    got = to_host(ops().fold_columns(dev_u64(rows), t, s, to_dev(w)))
+    want = O.fold_columns(rows, t, s, w)
+    assert np.array_equal(got, want)
+    g = 0x1F2E3D4C5B6A7988_0123456789ABCDEF
+    k = min(1 << s, 512)
+    imgs = to_host(ops().pow_images(g, to_dev(want[:k])))
+    assert np.array_equal(imgs, O.pow_many(g, want[:k]))
+
+
+@pytest.mark.parametrize("t,s", [(7, 0), (8, 6), (9, 7), (12, 9)])
+def test_pack_and_sumcheck_tables(t, s):
+    rows = rand_rows(t, s, 3)
+    packed = ops().pack_columns(dev_u64(rows), t, s)
+    w2 = rand_gf(1 << s, 1)
+    eq = rand_gf(1 << t, 2)
+    log(f"xi/fold_rows t={t} s={s}")
+    assert np.array_equal(to_host(ops().xi_combined(packed, t, s, to_dev(w2))),
+                          O.xi_combined_rows(rows, t, s, w2))
+    assert np.array_equal(to_host(ops().fold_rows_eq(dev_u64(rows), t, s, to_dev(eq))),
+                          O.fold_rows_eq(rows, t, s, eq))
+
+
+@pytest.mark.parametrize("n", [2, 64, 1 << 15, 1 << 19])
+def test_round_poly_and_fold(n):
+    m, w = rand_gf(n, 1), rand_gf(n, 2)
+    log(f"round_poly n={n}")
+    a0, a1, q = ops().round_poly(to_dev(m), to_dev(w))
+    want = O.round_polynomial(m, w)  # [a0, a0+a1+q, q]
+    assert [a0, a0 ^ a1 ^ q, q] == want
+    r = rand_ints(1, n)[0]
+    assert np.array_equal(to_host(ops().fold_lsb(to_dev(m), r)), O.fold_lsb(m, r))
+
+
+def _eq_parts(point, s_c):
+    """tensor weights over `len(point)` coordinates: eq_c over the low s_c,
+    eq_y over the rest (the layout of the GKR dense rounds)."""
+    pt = O.ints_to_gf(point)
+    return O.eq_table(pt[:s_c]), O.eq_table(pt[s_c:])
+
+
+@pytest.mark.parametrize("log_h,s_c", [(0, 0), (3, 3), (5, 2), (10, 4), (18, 11)])
+@pytest.mark.parametrize("send_one", [False, True])
+def test_gkr_round_sums(log_h, s_c, send_one):
+    h = 1 << log_h
+    l, r = rand_gf(2 * h, log_h), rand_gf(2 * h, log_h + 100)
+    point = rand_ints(log_h, 7)
+    eq_c, eq_y = _eq_parts(point, s_c)
+    log(f"gkr_round_sums h=2^{log_h} s_c={s_c} send_one={send_one}")
+    got = ops().gkr_round_sums(to_dev(l), to_dev(r), h, to_dev(eq_c), s_c, to_dev(eq_y), send_one)
+    full_eq = O.eq_table(O.ints_to_gf(point))
+    assert tuple(got) == O.gkr_round_sums(l, r, full_eq, send_one)
+    rho = rand_ints(1, 9)[0]
+    v = to_dev(l)
+    ops().fold_halves_(v, h, rho)
+    assert np.array_equal(to_host(v)[:h], O.fold_halves(l, rho))
+
+
+@pytest.mark.parametrize("t,s", [(7, 0), (7, 3), (9, 6), (10, 8), (15, 11)])
+def test_gkr_leaf_layer(t, s):
+    rows = rand_rows(t, s, 11)
+    A = rand_gf(1 << t, 12)
+    A[::7] = [1, 0]
+    packed = ops().pack_columns(dev_u64(rows), t, s)
+    dA = to_dev(A)
+    log(f"gkr leaf layer t={t} s={s}")
+    if t + s <= 20:
+        leaves = O.forest_leaves(rows, t, s, A)
+        half = 1 << (t + s - 1)
+        l, r = leaves[:half], leaves[half:]
+        # level 1 = products of the two halves
+        assert np.array_equal(to_host(ops().gkr_level1(packed, dA, t, s)), O.product_level(leaves))
+    point = rand_ints(t + s - 2, 3)
+    s_c = min(s, t + s - 2)
+    eq_c, eq_y = _eq_parts(point, s_c)
+    for send_one in (False, True):
+        got = ops().gkr_leaf_round_sums(packed, dA, t, s, to_dev(eq_c), s_c, to_dev(eq_y), send_one)
+        if t + s <= 20:
+            full_eq = O.eq_table(O.ints_to_gf(point))
+            assert tuple(got) == O.gkr_round_sums(l, r, full_eq, send_one)
+    rho = rand_ints(1, 4)[0]
+    gl, gr = ops().gkr_leaf_fold(packed, dA, t, s, rho)
+    if t + s <= 20:
+        assert np.array_equal(to_host(gl), O.fold_halves(l, rho))
+        assert np.array_equal(to_host(gr), O.fold_halves(r, rho))
+    else:
+        # full-size: spot-check rows of the fold against leaves rebuilt on the host
+        q = 1 << (t - 2)
+        glh, grh = to_host(gl), to_host(gr)
+        for y in (0, 1, q - 1):
+            for c in (0, 5, (1 << s) - 1):
+                def leaf(row):
+                    bit = (int(rows[c * (1 << (t - 6)) + row // 64]) >> (row % 64)) & 1
+                    return O.gf_to_ints(A[row:row + 1])[0] if bit else 1
+                e = leaf(y) ^ O.mul(rho, leaf(y) ^ leaf(y + q))
+                o = leaf(y + 2 * q) ^ O.mul(rho, leaf(y + 2 * q) ^ leaf(y + 3 * q))
+                x = (y << s) | c
+                assert O.gf_to_ints(glh[x:x + 1])[0] == e
+                assert O.gf_to_ints(grh[x:x + 1])[0] == o
+
+
+def test_product_level():
+    v = rand_gf(1 << 16, 3)
+    assert np.array_equal(to_host(ops().product_level(to_dev(v))), O.product_level(v))
+
+
+@pytest.mark.parametrize("n", [1, 16, 1 << 12, 1 << 19])
+def test_ring_switch_folds(n):
+    p, suffix = rand_gf(n, 1), rand_gf(n, 2)
+    w = rand_gf(128, 3)
+    log(f"ring switch n={n}")
+    assert np.array_equal(to_host(ops().fold_1b_rows(to_dev(p), to_dev(suffix))), O.fold_1b_rows(p, suffix))
+    assert np.array_equal(to_host(ops().fold_b128(to_dev(suffix), to_dev(w))), O.fold_b128_elems(suffix, w))
+    eta = rand_ints(1, n)[0]
+    acc = rand_gf(n, 4)
+    got = to_host(ops().axpy_(to_dev(acc), to_dev(suffix), eta))
+    assert np.array_equal(got, O.lig_glue(acc, suffix, eta))
+
+
+@pytest.mark.parametrize("n", [2, 256, 1 << 19])
+def test_ligerito_sumcheck(n):
+    f, b = rand_gf(n, 5), rand_gf(n, 6)
+    log(f"ligerito sumcheck n={n}")
+    u0, u2 = ops().lig_msg(to_dev(f), to_dev(b))
+    assert (u0, u2) == O.lig_round_msg(f, b)
+    u0e, u2e, y = ops().lig_msg(to_dev(f), to_dev(b), with_eval=True)
+    assert (u0e, u2e) == (u0, u2)
+    ints_f, ints_b = O.gf_to_ints(f[:64]), O.gf_to_ints(b[:64])
+    if n <= 256:
+        yy = 0
+        for a, c in zip(O.gf_to_ints(f), O.gf_to_ints(b)):
+            yy ^= O.mul(a, c)
+        assert y == yy
+    r = rand_ints(1, n + 1)[0]
+    fo, bo = ops().lig_fold(to_dev(f), to_dev(b), r)
+    assert np.array_equal(to_host(fo), O.fold_lsb(f, r))
+    assert np.array_equal(to_host(bo), O.fold_lsb(b, r))
+    del ints_f, ints_b
+
+
+@pytest.mark.parametrize("log_n,nq", [(0, 3), (4, 5), (9, 37), (13, 148)])
+def test_induce_basis(log_n, nq):
+    rng = random.Random(log_n)
+    queries = sorted(rng.sample(range(1 << (log_n + 2)), nq))
+    aw = rand_gf(nq, 8)
+    log(f"induce basis log_n={log_n} nq={nq}")
+    got = to_host(ops().induce_basis(log_n, queries, to_dev(aw)))
+    assert np.array_equal(got, O.induce_basis(log_n, queries, aw))
+
+
+@pytest.mark.parametrize("bits", [0, 1, 8, 14])
+def test_pow_find(bits):
+    prefix = b"bitz-pcs-pow-v1\0"
+    seed = bytes(range(16, 32))
+    log(f"pow bits={bits}")
+    got = ops().pow_find(prefix, seed, bits)
+    assert got == O.pow_find(prefix, seed, bits)
+    assert O.pow_valid(prefix, seed, got, bits)

@@ -69,6 +69,10 @@ SIGNATURES = {
     "bitz_commit_codeword": (_P, [_P]),
     "bitz_commit_tree": (_P, [_P]),
     "bitz_commit_read_tree": (_I, [_P, _P, _U64]),
+    "bitz_standalone_evaluation": (_I, [_P, _P, _P]),
+    "bitz_prove_standalone": (_I, [_P, _P, _P, _P, _U64, C.POINTER(_U64)]),
+    "bitz_ligerito_config": (_I, [_U32, _U32, _U32, _U32, _P, _U32]),
+    "bitz_host_blake3": (None, [_P, _U64, _P]),
 }
 
 _lib = None

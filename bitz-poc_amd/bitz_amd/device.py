@@ -270,3 +270,46 @@ class OpenOps:
         n = C.c_uint64()
         _lib.call("bitz_pow_find_dev", self.ctx.ptr, _vp(p), _vp(s), bits, C.byref(n))
         return n.value
+
+
+# ------------------------------------------------------- standalone opening
+class Commitment:
+    """A commitment of host bit rows kept on the device (bitz_commit handle):
+    Pcs::commit (src/bitz/pcs.rs:270-284)."""
+
+    def __init__(self, ctx: Context, rows, t: int, s: int, log_batch: int, log_inv_rate: int):
+        import numpy as np
+        rows = np.ascontiguousarray(rows, dtype=np.uint64).reshape(-1)
+        if rows.size != (1 << s) << (t - 6):
+            raise ValueError("rows must hold 2^s columns of 2^(t-6) words")
+        self.ctx, self.t, self.s = ctx, t, s
+        self.log_batch, self.log_inv_rate = log_batch, log_inv_rate
+        root = (C.c_uint8 * 32)()
+        h = C.c_void_p()
+        _lib.call("bitz_commit", ctx.ptr, rows.ctypes.data_as(C.c_void_p), t, s, log_batch, log_inv_rate,
+                  _vp(root), C.byref(h))
+        self._h = h
+        self.root = bytes(root)
+
+    def close(self):
+        if self._h:
+            _lib.load().bitz_commit_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def evaluation(self) -> int:
+        out = (C.c_uint8 * 16)()
+        _lib.call("bitz_standalone_evaluation", self.ctx.ptr, self._h, _vp(out))
+        return int.from_bytes(bytes(out), "little")
+
+    def prove(self, claimed: int, capacity: int = 1 << 24) -> bytes:
+        buf = (C.c_uint8 * capacity)()
+        n = C.c_uint64()
+        _lib.call("bitz_prove_standalone", self.ctx.ptr, self._h, _vp(_b16(claimed)), _vp(buf), capacity,
+                  C.byref(n))
+        return bytes(buf[: n.value])

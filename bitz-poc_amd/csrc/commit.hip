@@ -507,12 +507,6 @@ extern "C" int bitz_commit_dev(bitz_ctx* ctx, const void* packed, int log_msg_le
 }
 
 // ---- host-buffer commit with a device-resident handle ----
-struct bitz_commit_handle {
-    bitz_ctx* ctx;
-    bitz::DevBuf packed, codeword, tree;
-    uint32_t t, s, log_batch, log_inv_rate;
-    uint64_t n_leaves;
-};
 
 extern "C" int bitz_commit(bitz_ctx* ctx, const uint64_t* rows, uint32_t t, uint32_t s,
                            uint32_t log_batch, uint32_t log_inv_rate, uint8_t root[32],
@@ -529,6 +523,8 @@ extern "C" int bitz_commit(bitz_ctx* ctx, const uint64_t* rows, uint32_t t, uint
     h->log_batch = log_batch;
     h->log_inv_rate = log_inv_rate;
     h->n_leaves = 1ull << (log_msg - (int)log_batch + (int)log_inv_rate);
+    BITZ_REQUIRE(log_batch <= 6 && (int)log_batch < log_msg && log_inv_rate >= 1 && log_inv_rate <= 8,
+                 BITZ_ERR_SHAPE, "bitz_commit: need log_batch <= 6, log_batch < t+s-7, 1 <= log_inv_rate <= 8");
     size_t msg_bytes = (size_t)16 << log_msg;
     BITZ_REQUIRE(h->packed.alloc(msg_bytes) && h->codeword.alloc(msg_bytes << log_inv_rate) &&
                      h->tree.alloc((2 * h->n_leaves - 1) * 32),
@@ -538,6 +534,7 @@ extern "C" int bitz_commit(bitz_ctx* ctx, const uint64_t* rows, uint32_t t, uint
     int rc = bitz_commit_dev(ctx, h->packed.ptr, log_msg, (int)log_batch, (int)log_inv_rate,
                              h->codeword.ptr, h->tree.ptr, root);
     if (rc) return rc;
+    memcpy(h->root, root, 32);
     *out = h.release();
     return BITZ_OK;
 }

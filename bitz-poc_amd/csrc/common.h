@@ -22,6 +22,20 @@ struct DevBuf {
     DevBuf() = default;
     DevBuf(const DevBuf&) = delete;
     DevBuf& operator=(const DevBuf&) = delete;
+    DevBuf(DevBuf&& o) noexcept : ptr(o.ptr), bytes(o.bytes) {
+        o.ptr = nullptr;
+        o.bytes = 0;
+    }
+    DevBuf& operator=(DevBuf&& o) noexcept {
+        if (this != &o) {
+            if (ptr) (void)hipFree(ptr);
+            ptr = o.ptr;
+            bytes = o.bytes;
+            o.ptr = nullptr;
+            o.bytes = 0;
+        }
+        return *this;
+    }
     ~DevBuf() {
         if (ptr) (void)hipFree(ptr);
     }
@@ -63,6 +77,16 @@ struct bitz_ctx {
     // device twiddle tables per NTT domain dimension ell:
     // tw[(1 << layer) - 1 + block] = twiddle(layer, block)
     std::map<int, std::unique_ptr<bitz::DevBuf>> twiddles;
+};
+
+// A committed matrix kept on the device: the bit rows (= packed message
+// words), the L0 codeword and its Merkle tree.
+struct bitz_commit_handle {
+    bitz_ctx* ctx;
+    bitz::DevBuf packed, codeword, tree;
+    uint32_t t, s, log_batch, log_inv_rate;
+    uint64_t n_leaves;
+    uint8_t root[32];
 };
 
 #define BITZ_HIP(expr)                                                                   \

@@ -63,10 +63,10 @@ __global__ void k_gf_mul(const uint4* __restrict__ a, const uint4* __restrict__ 
     gf_store(&out[i], gf_mul(gf_load(&a[i]), gf_load(&b[i])));
 }
 
-__global__ void k_gf_mul_const(const uint4* __restrict__ a, uint4 c, uint4* __restrict__ out,
-                               uint64_t n) {
+__global__ void k_gf_mul_const(const uint4* __restrict__ a, uint64_t c_lo, uint64_t c_hi,
+                               uint4* __restrict__ out, uint64_t n) {
     __shared__ uint4 tab[512];
-    gf_table_build(tab, gf128{{c.x, c.y, c.z, c.w}}, threadIdx.x, blockDim.x);
+    gf_table_build(tab, gf_from_u64(c_lo, c_hi), threadIdx.x, blockDim.x);
     __syncthreads();
     uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -109,11 +109,12 @@ extern "C" int bitz_gf_mul_const_dev(bitz_ctx* ctx, const void* a, const uint8_t
     BITZ_REQUIRE(ctx && a && c && out, BITZ_ERR_ARG, "bitz_gf_mul_const_dev: null argument");
     if (n == 0) return BITZ_OK;
     BITZ_HIP(hipSetDevice(ctx->device));
-    uint4 cc;
-    memcpy(&cc, c, 16);
+    uint64_t c_lo, c_hi;
+    memcpy(&c_lo, c, 8);
+    memcpy(&c_hi, c + 8, 8);
     uint64_t blocks = (n + 255) / 256;
     hipLaunchKernelGGL(k_gf_mul_const, dim3((unsigned)blocks), dim3(256), 0, ctx->stream,
-                       (const uint4*)a, cc, (uint4*)out, n);
+                       (const uint4*)a, c_lo, c_hi, (uint4*)out, n);
     BITZ_LAUNCH_CHECK();
     return BITZ_OK;
 }

@@ -32,6 +32,12 @@ __device__ __forceinline__ bool gf_is_zero(gf128 a) {
     return (a.w[0] | a.w[1] | a.w[2] | a.w[3]) == 0u;
 }
 
+// Field elements cross the kernel-argument boundary as two u64 scalars
+// (vector types by value do not keep one layout on host and device).
+__device__ __forceinline__ gf128 gf_from_u64(uint64_t lo, uint64_t hi) {
+    return gf128{{(uint32_t)lo, (uint32_t)(lo >> 32), (uint32_t)hi, (uint32_t)(hi >> 32)}};
+}
+
 __device__ __forceinline__ gf128 gf_load(const uint4* p) {
     uint4 v = *p;
     return gf128{{v.x, v.y, v.z, v.w}};

@@ -105,11 +105,11 @@ __global__ void k_fold_columns_sum(const uint4* __restrict__ partial, uint64_t c
 // ------------------------------------------------------- fixed-base powers
 // comb[i*256 + v] = g^(v * 2^(8i)); g^e = prod_i comb[i][byte_i(e)]
 // (vendor/field/src/preparation.rs:177-263: a 16 x 256 window table).
-__global__ void k_comb_rows(uint4* __restrict__ comb, uint4 g) {
+__global__ void k_comb_rows(uint4* __restrict__ comb, uint64_t g_lo, uint64_t g_hi) {
     // row i base: g^(2^(8i)) computed by thread i by repeated squaring
     int i = threadIdx.x;
     if (i >= 16) return;
-    gf128 b = gf128{{g.x, g.y, g.z, g.w}};
+    gf128 b = gf_from_u64(g_lo, g_hi);
     for (int k = 0; k < 8 * i; k++) b = gf_square(b);
     // powers b^v, v in 0..255, by a running product
     gf128 acc = gf_one();
@@ -223,9 +223,10 @@ struct LeafSrc {
 };
 
 // v[i] = v[i] + rho (v[i+h] - v[i]) for i < h, with rho's window table in LDS
-__global__ __launch_bounds__(kT) void k_fold_halves(uint4* __restrict__ v, uint64_t h, uint4 rho) {
+__global__ __launch_bounds__(kT) void k_fold_halves(uint4* __restrict__ v, uint64_t h, uint64_t rho_lo,
+                                                    uint64_t rho_hi) {
     __shared__ uint4 tab[512];
-    gf_table_build(tab, gf128{{rho.x, rho.y, rho.z, rho.w}}, threadIdx.x, blockDim.x);
+    gf_table_build(tab, gf_from_u64(rho_lo, rho_hi), threadIdx.x, blockDim.x);
     __syncthreads();
     for (uint64_t i = grid_tid(); i < h; i += grid_threads()) {
         gf128 a = gf_load(&v[i]), b = gf_load(&v[i + h]);
@@ -234,10 +235,10 @@ __global__ __launch_bounds__(kT) void k_fold_halves(uint4* __restrict__ v, uint6
 }
 
 // Leaf layer folded once by rho: writes l', r' (2^(t+s-2) each).
-__global__ __launch_bounds__(kT) void k_gkr_leaf_fold(LeafSrc src, uint4 rho, uint4* __restrict__ l_out,
-                                                      uint4* __restrict__ r_out) {
+__global__ __launch_bounds__(kT) void k_gkr_leaf_fold(LeafSrc src, uint64_t rho_lo, uint64_t rho_hi,
+                                                      uint4* __restrict__ l_out, uint4* __restrict__ r_out) {
     __shared__ uint4 tab[512];
-    gf_table_build(tab, gf128{{rho.x, rho.y, rho.z, rho.w}}, threadIdx.x, blockDim.x);
+    gf_table_build(tab, gf_from_u64(rho_lo, rho_hi), threadIdx.x, blockDim.x);
     __syncthreads();
     const uint64_t h = 1ull << (src.t + src.s - 2);
     for (uint64_t x = grid_tid(); x < h; x += grid_threads()) {
@@ -322,9 +323,10 @@ __global__ __launch_bounds__(kT) void k_round_poly(const uint4* __restrict__ m, 
 
 // out[i] = v[2i] + r (v[2i] + v[2i+1]); out may alias v (reads precede the
 // write of the same index and out index i <= 2i).
-__global__ __launch_bounds__(kT) void k_fold_lsb(const uint4* v, uint64_t half, uint4 r, uint4* out) {
+__global__ __launch_bounds__(kT) void k_fold_lsb(const uint4* v, uint64_t half, uint64_t r_lo, uint64_t r_hi,
+                                                 uint4* out) {
     __shared__ uint4 tab[512];
-    gf_table_build(tab, gf128{{r.x, r.y, r.z, r.w}}, threadIdx.x, blockDim.x);
+    gf_table_build(tab, gf_from_u64(r_lo, r_hi), threadIdx.x, blockDim.x);
     __syncthreads();
     for (uint64_t i = grid_tid(); i < half; i += grid_threads()) {
         gf128 v0 = gf_load(&v[2 * i]), v1 = gf_load(&v[2 * i + 1]);
@@ -397,9 +399,9 @@ __global__ __launch_bounds__(kT) void k_fold_b128(const uint4* __restrict__ suff
 
 // acc[x] += eta * e[x]
 __global__ __launch_bounds__(kT) void k_axpy(uint4* __restrict__ acc, const uint4* __restrict__ e, uint64_t n,
-                                             uint4 eta) {
+                                             uint64_t eta_lo, uint64_t eta_hi) {
     __shared__ uint4 tab[512];
-    gf_table_build(tab, gf128{{eta.x, eta.y, eta.z, eta.w}}, threadIdx.x, blockDim.x);
+    gf_table_build(tab, gf_from_u64(eta_lo, eta_hi), threadIdx.x, blockDim.x);
     __syncthreads();
     for (uint64_t x = grid_tid(); x < n; x += grid_threads())
         gf_store(&acc[x], gf_add(gf_load(&acc[x]), gf_mul_tab(tab, gf_load(&e[x]))));
@@ -427,10 +429,10 @@ __global__ __launch_bounds__(kT) void k_lig_msg(const uint4* __restrict__ f, con
 }
 
 // f'[j] = f0 + r (f0 + f1), b' likewise; outputs may alias inputs.
-__global__ __launch_bounds__(kT) void k_lig_fold(const uint4* f, const uint4* b, uint64_t half, uint4 r,
-                                                 uint4* fo, uint4* bo) {
+__global__ __launch_bounds__(kT) void k_lig_fold(const uint4* f, const uint4* b, uint64_t half, uint64_t r_lo,
+                                                 uint64_t r_hi, uint4* fo, uint4* bo) {
     __shared__ uint4 tab[512];
-    gf_table_build(tab, gf128{{r.x, r.y, r.z, r.w}}, threadIdx.x, blockDim.x);
+    gf_table_build(tab, gf_from_u64(r_lo, r_hi), threadIdx.x, blockDim.x);
     __syncthreads();
     for (uint64_t j = grid_tid(); j < half; j += grid_threads()) {
         gf128 f0 = gf_load(&f[2 * j]), f1 = gf_load(&f[2 * j + 1]);
@@ -474,18 +476,49 @@ __global__ void k_induce_what(const uint64_t* __restrict__ queries, uint32_t nq,
     }
 }
 
+// sum_x a[x] b[x]
+__global__ __launch_bounds__(kT) void k_dot(const uint4* __restrict__ a, const uint4* __restrict__ b, uint64_t n,
+                                            uint4* __restrict__ partials) {
+    gf256 acc = gf256_zero();
+    for (uint64_t x = grid_tid(); x < n; x += grid_threads()) gf256_acc(&acc, gf_mul_wide(gf_load(&a[x]), gf_load(&b[x])));
+    gf128 v[1] = {gf_reduce(acc)};
+    block_xor_store<1>(v, partials + blockIdx.x);
+}
+
+// out[x] = a[x] * b[x]
+__global__ void k_mul_elem(const uint4* __restrict__ a, const uint4* __restrict__ b, uint64_t n,
+                           uint4* __restrict__ out) {
+    for (uint64_t x = grid_tid(); x < n; x += grid_threads()) gf_store(&out[x], gf_mul(gf_load(&a[x]), gf_load(&b[x])));
+}
+
+// out[x] = (a[x] + 1) * b[x]
+__global__ void k_mul_elem_plus1(const uint4* __restrict__ a, const uint4* __restrict__ b, uint64_t n,
+                                 uint4* __restrict__ out) {
+    for (uint64_t x = grid_tid(); x < n; x += grid_threads())
+        gf_store(&out[x], gf_mul(gf_add(gf_load(&a[x]), gf_one()), gf_load(&b[x])));
+}
+
+// out[x] = a[x] * c
+__global__ __launch_bounds__(kT) void k_scale(const uint4* __restrict__ a, uint64_t n, uint64_t c_lo,
+                                              uint64_t c_hi, uint4* __restrict__ out) {
+    __shared__ uint4 tab[512];
+    gf_table_build(tab, gf_from_u64(c_lo, c_hi), threadIdx.x, blockDim.x);
+    __syncthreads();
+    for (uint64_t x = grid_tid(); x < n; x += grid_threads()) gf_store(&out[x], gf_mul_tab(tab, gf_load(&a[x])));
+}
+
 // ----------------------------------------------------------- proof of work
 // Smallest nonce in [base, base + span) with >= bits leading zero bits of
 // BLAKE3(prefix16 || seed16 || nonce_le) (src/bitz/pcs.rs:1252-1298);
 // result via atomicMin on a u64 (stable: the minimum is unique).
-__global__ __launch_bounds__(kT) void k_pow_scan(uint4 pre0, uint4 pre1, uint64_t base,
+__global__ __launch_bounds__(kT) void k_pow_scan(const uint32_t* __restrict__ pre /* 8 words */, uint64_t base,
                                                  uint64_t span, uint32_t bits,
                                                  unsigned long long* __restrict__ best) {
     uint64_t i = grid_tid();
     if (i >= span) return;
     uint64_t nonce = base + i;
     if ((unsigned long long)nonce >= *((volatile unsigned long long*)best)) return;
-    uint32_t m[16] = {pre0.x, pre0.y, pre0.z, pre0.w, pre1.x, pre1.y, pre1.z, pre1.w,
+    uint32_t m[16] = {pre[0], pre[1], pre[2], pre[3], pre[4], pre[5], pre[6], pre[7],
                       (uint32_t)nonce, (uint32_t)(nonce >> 32), 0, 0, 0, 0, 0, 0};
     uint32_t cv[8];
     b3::iv_init(cv);
@@ -625,10 +658,11 @@ int fold_columns(bitz_ctx* ctx, const uint64_t* rows, int t, int s, const void* 
 
 int pow_images(bitz_ctx* ctx, const uint8_t g[16], const void* exps, uint64_t n, void* out) {
     if (!ctx->scratch.comb.alloc(16 * 256 * 16)) return BITZ_ERR_ALLOC;
-    uint4 gg;
-    memcpy(&gg, g, 16);
+    uint64_t g_lo, g_hi;
+    memcpy(&g_lo, g, 8);
+    memcpy(&g_hi, g + 8, 8);
     if (memcmp(ctx->scratch.comb_base, g, 16) != 0 || !ctx->scratch.comb_ready) {
-        hipLaunchKernelGGL(k_comb_rows, dim3(1), dim3(64), 0, ctx->stream, (uint4*)ctx->scratch.comb.ptr, gg);
+        hipLaunchKernelGGL(k_comb_rows, dim3(1), dim3(64), 0, ctx->stream, (uint4*)ctx->scratch.comb.ptr, g_lo, g_hi);
         BITZ_LAUNCH_CHECK();
         memcpy(ctx->scratch.comb_base, g, 16);
         ctx->scratch.comb_ready = true;
@@ -673,21 +707,26 @@ int gkr_leaf_round_sums(bitz_ctx* ctx, const uint64_t* packed, const void* A, in
     return sums_finish(ctx, nb, 2, out_dev);
 }
 
+static void split16(const uint8_t* p, uint64_t* lo, uint64_t* hi) {
+    memcpy(lo, p, 8);
+    memcpy(hi, p + 8, 8);
+}
+
 int fold_halves(bitz_ctx* ctx, void* v, uint64_t h, const uint8_t rho[16]) {
-    uint4 r;
-    memcpy(&r, rho, 16);
-    hipLaunchKernelGGL(k_fold_halves, dim3(blocks_for(h, 4)), dim3(kT), 0, ctx->stream, (uint4*)v, h, r);
+    uint64_t lo, hi;
+    split16(rho, &lo, &hi);
+    hipLaunchKernelGGL(k_fold_halves, dim3(blocks_for(h, 4)), dim3(kT), 0, ctx->stream, (uint4*)v, h, lo, hi);
     BITZ_LAUNCH_CHECK();
     return BITZ_OK;
 }
 
 int gkr_leaf_fold(bitz_ctx* ctx, const uint64_t* packed, const void* A, int t, int s, const uint8_t rho[16],
                   void* l_out, void* r_out) {
-    uint4 r;
-    memcpy(&r, rho, 16);
+    uint64_t lo, hi;
+    split16(rho, &lo, &hi);
     LeafSrc src{packed, (const uint4*)A, t, s};
-    hipLaunchKernelGGL(k_gkr_leaf_fold, dim3(blocks_for(1ull << (t + s - 2), 4)), dim3(kT), 0, ctx->stream, src, r,
-                       (uint4*)l_out, (uint4*)r_out);
+    hipLaunchKernelGGL(k_gkr_leaf_fold, dim3(blocks_for(1ull << (t + s - 2), 4)), dim3(kT), 0, ctx->stream, src, lo,
+                       hi, (uint4*)l_out, (uint4*)r_out);
     BITZ_LAUNCH_CHECK();
     return BITZ_OK;
 }
@@ -734,14 +773,14 @@ int round_poly(bitz_ctx* ctx, const void* m, const void* w, uint64_t n, void* ou
 }
 
 int fold_lsb(bitz_ctx* ctx, const void* v, uint64_t n, const uint8_t r[16], void* out) {
-    uint4 rr;
-    memcpy(&rr, r, 16);
+    uint64_t lo, hi;
+    split16(r, &lo, &hi);
     // in place needs a single pass order guarantee: out index i reads 2i, 2i+1;
     // a block may overwrite index i while another block still reads it, so
     // in-place folds go through the ping-pong buffer of the caller.
     BITZ_REQUIRE(v != out, BITZ_ERR_ARG, "fold_lsb: out must not alias v");
-    hipLaunchKernelGGL(k_fold_lsb, dim3(blocks_for(n / 2, 4)), dim3(kT), 0, ctx->stream, (const uint4*)v, n / 2, rr,
-                       (uint4*)out);
+    hipLaunchKernelGGL(k_fold_lsb, dim3(blocks_for(n / 2, 4)), dim3(kT), 0, ctx->stream, (const uint4*)v, n / 2, lo,
+                       hi, (uint4*)out);
     BITZ_LAUNCH_CHECK();
     return BITZ_OK;
 }
@@ -773,9 +812,10 @@ int fold_b128(bitz_ctx* ctx, const void* suffix, uint64_t n, const void* w, void
 }
 
 int axpy(bitz_ctx* ctx, void* acc, const void* e, uint64_t n, const uint8_t eta[16]) {
-    uint4 x;
-    memcpy(&x, eta, 16);
-    hipLaunchKernelGGL(k_axpy, dim3(blocks_for(n, 4)), dim3(kT), 0, ctx->stream, (uint4*)acc, (const uint4*)e, n, x);
+    uint64_t lo, hi;
+    split16(eta, &lo, &hi);
+    hipLaunchKernelGGL(k_axpy, dim3(blocks_for(n, 4)), dim3(kT), 0, ctx->stream, (uint4*)acc, (const uint4*)e, n, lo,
+                       hi);
     BITZ_LAUNCH_CHECK();
     return BITZ_OK;
 }
@@ -798,10 +838,10 @@ int lig_msg(bitz_ctx* ctx, const void* f, const void* b, uint64_t n, int with_ev
 
 int lig_fold(bitz_ctx* ctx, const void* f, const void* b, uint64_t n, const uint8_t r[16], void* fo, void* bo) {
     BITZ_REQUIRE(f != fo && b != bo, BITZ_ERR_ARG, "lig_fold: outputs must not alias inputs");
-    uint4 rr;
-    memcpy(&rr, r, 16);
+    uint64_t lo, hi;
+    split16(r, &lo, &hi);
     hipLaunchKernelGGL(k_lig_fold, dim3(blocks_for(n / 2, 4)), dim3(kT), 0, ctx->stream, (const uint4*)f,
-                       (const uint4*)b, n / 2, rr, (uint4*)fo, (uint4*)bo);
+                       (const uint4*)b, n / 2, lo, hi, (uint4*)fo, (uint4*)bo);
     BITZ_LAUNCH_CHECK();
     return BITZ_OK;
 }
@@ -825,15 +865,19 @@ int pow_find(bitz_ctx* ctx, const uint8_t prefix[16], const uint8_t seed[16], ui
     BITZ_REQUIRE(bits <= 40, BITZ_ERR_ARG, "pow_find: bits too large");
     int rc = ctx->scratch.ensure(4096);
     if (rc) return rc;
-    uint4 p0, p1;
-    memcpy(&p0, prefix, 16);
-    memcpy(&p1, seed, 16);
     unsigned long long* best = (unsigned long long*)ctx->scratch.result.ptr;
+    uint32_t* pre = (uint32_t*)((uint8_t*)ctx->scratch.result.ptr + 64);
+    uint8_t staged[64];
     unsigned long long init = ~0ull;
-    BITZ_HIP(hipMemcpyAsync(best, &init, 8, hipMemcpyHostToDevice, ctx->stream));
+    memcpy(staged, &init, 8);
+    memset(staged + 8, 0, 56);
+    memcpy(ctx->scratch.host, staged, 64);
+    memcpy((uint8_t*)ctx->scratch.host + 64, prefix, 16);
+    memcpy((uint8_t*)ctx->scratch.host + 80, seed, 16);
+    BITZ_HIP(hipMemcpyAsync(ctx->scratch.result.ptr, ctx->scratch.host, 96, hipMemcpyHostToDevice, ctx->stream));
     uint64_t span = 1ull << 20;
     for (uint64_t base = 0;; base += span) {
-        hipLaunchKernelGGL(k_pow_scan, dim3((unsigned)(span / kT)), dim3(kT), 0, ctx->stream, p0, p1, base, span,
+        hipLaunchKernelGGL(k_pow_scan, dim3((unsigned)(span / kT)), dim3(kT), 0, ctx->stream, pre, base, span,
                            bits, best);
         BITZ_LAUNCH_CHECK();
         unsigned long long h;
@@ -846,6 +890,36 @@ int pow_find(bitz_ctx* ctx, const uint8_t prefix[16], const uint8_t seed[16], ui
         }
         BITZ_REQUIRE(base < (1ull << 44), BITZ_ERR_DEVICE, "pow_find: search exhausted");
     }
+}
+
+int dot(bitz_ctx* ctx, const void* a, const void* b, uint64_t n, void* out_dev) {
+    unsigned nb = blocks_for(n, 4, 2048);
+    int rc = ctx->scratch.ensure((size_t)nb * 16);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_dot, dim3(nb), dim3(kT), 0, ctx->stream, (const uint4*)a, (const uint4*)b, n,
+                       (uint4*)ctx->scratch.partials.ptr);
+    BITZ_LAUNCH_CHECK();
+    return sums_finish(ctx, nb, 1, out_dev);
+}
+
+int mul_elem(bitz_ctx* ctx, const void* a, const void* b, uint64_t n, void* out, int plus_one) {
+    if (plus_one)
+        hipLaunchKernelGGL(k_mul_elem_plus1, dim3(blocks_for(n, 4)), dim3(kT), 0, ctx->stream, (const uint4*)a,
+                           (const uint4*)b, n, (uint4*)out);
+    else
+        hipLaunchKernelGGL(k_mul_elem, dim3(blocks_for(n, 4)), dim3(kT), 0, ctx->stream, (const uint4*)a,
+                           (const uint4*)b, n, (uint4*)out);
+    BITZ_LAUNCH_CHECK();
+    return BITZ_OK;
+}
+
+int scale(bitz_ctx* ctx, const void* a, uint64_t n, const uint8_t c[16], void* out) {
+    uint64_t lo, hi;
+    split16(c, &lo, &hi);
+    hipLaunchKernelGGL(k_scale, dim3(blocks_for(n, 4)), dim3(kT), 0, ctx->stream, (const uint4*)a, n, lo, hi,
+                       (uint4*)out);
+    BITZ_LAUNCH_CHECK();
+    return BITZ_OK;
 }
 
 int gather_rows(bitz_ctx* ctx, const void* cw, const uint64_t* q_dev, uint32_t nq, uint32_t lanes, void* out) {

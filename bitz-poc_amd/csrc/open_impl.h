@@ -39,6 +39,10 @@ int pow_find(bitz_ctx* ctx, const uint8_t prefix[16], const uint8_t seed[16], ui
 int gather_rows(bitz_ctx* ctx, const void* cw, const uint64_t* q_dev, uint32_t nq, uint32_t lanes, void* out);
 int gather_nodes(bitz_ctx* ctx, const void* tree, const uint64_t* idx_dev, uint32_t n, void* out);
 
+int dot(bitz_ctx* ctx, const void* a, const void* b, uint64_t n, void* out_dev);
+int mul_elem(bitz_ctx* ctx, const void* a, const void* b, uint64_t n, void* out, int plus_one);
+int scale(bitz_ctx* ctx, const void* a, uint64_t n, const uint8_t c[16], void* out);
+
 // commit pipeline (commit.hip)
 int commit_device(bitz_ctx* ctx, const void* packed, int log_msg_len, int log_batch, int log_inv_rate,
                   void* codeword, void* tree);

@@ -1,0 +1,1108 @@
+// The BitZ standalone opening, orchestrated on the host, every bulk pass on
+// the GPU. Follows src/piop/spartan/protocol/bitz_opener.rs:672-707
+// (prove_standalone) -> src/bitz/mod.rs:142-199 (BitZProver::prove) ->
+// src/bitz/pcs.rs:311-507 (prove_lin, prove_mle) ->
+// vendor/flock-mod/crates/flock-core/src/pcs/ligerito.rs:3398-3964
+// (recursive_prover_with_basis).
+//
+// Transcripts: the outer statement/claim transcript is the reference's
+// Blake3Transcript (src/transcript/mod.rs). The inner BitZ transcript keeps
+// the reference's message order, labels and encodings but runs them through
+// the same BLAKE3 framing (the reference's spongefish 0.7.4 sponge is not
+// available to restate).
+#include <algorithm>
+#include <set>
+
+#include "host_proto.h"
+#include "open_impl.h"
+
+namespace bitz {
+namespace {
+
+// ------------------------------------------------------------- host values
+using G = hgf;
+inline G gzero() { return G{0, 0}; }
+inline G gone() { return G{1, 0}; }
+inline bool geq(G a, G b) { return a.lo == b.lo && a.hi == b.hi; }
+inline G gfrom(const uint8_t* p) {
+    G g;
+    memcpy(&g.lo, p, 8);
+    memcpy(&g.hi, p + 8, 8);
+    return g;
+}
+inline void gput(G g, uint8_t* p) {
+    memcpy(p, &g.lo, 8);
+    memcpy(p + 8, &g.hi, 8);
+}
+// eq_factor (src/bitz/mod.rs:328-331): r z + (1 - r)(1 - z)
+inline G eq_factor(G r, G z) { return hgf_add(hgf_mul(r, z), hgf_mul(hgf_add(r, gone()), hgf_add(z, gone()))); }
+
+std::vector<G> host_eq(const std::vector<G>& r) {
+    std::vector<G> t(1ull << r.size());
+    t[0] = gone();
+    for (size_t i = 0; i < r.size(); i++) {
+        size_t len = 1ull << i;
+        for (size_t x = 0; x < len; x++) {
+            G hi = hgf_mul(t[x], r[i]);
+            t[x + len] = hi;
+            t[x] = hgf_add(t[x], hi);
+        }
+    }
+    return t;
+}
+
+// --------------------------------------------------------- mod-q integers
+typedef unsigned __int128 u128;
+
+u128 addmod(u128 a, u128 b, u128 q) {
+    u128 s = a + b;  // q < 2^126: no overflow
+    return s >= q ? s - q : s;
+}
+u128 mulmod(u128 a, u128 b, u128 q) {
+    u128 r = 0;
+    a %= q;
+    for (int i = 127; i >= 0; i--) {
+        r = addmod(r, r, q);
+        if ((b >> i) & 1) r = addmod(r, a, q);
+    }
+    return r;
+}
+u128 powmod(u128 a, u128 e, u128 q) {
+    u128 r = 1 % q;
+    a %= q;
+    while (e) {
+        if (e & 1) r = mulmod(r, a, q);
+        a = mulmod(a, a, q);
+        e >>= 1;
+    }
+    return r;
+}
+bool miller_rabin(u128 n) {
+    static const unsigned kBases[] = {2,  3,  5,  7,  11, 13, 17, 19, 23, 29, 31, 37,
+                                      41, 43, 47, 53, 59, 61, 67, 71, 73, 79, 83, 89};
+    if (n < 2) return false;
+    for (unsigned p : kBases) {
+        if (n == p) return true;
+        if (n % p == 0) return false;
+    }
+    u128 d = n - 1;
+    int r = 0;
+    while ((d & 1) == 0) {
+        d >>= 1;
+        r++;
+    }
+    for (unsigned a : kBases) {
+        u128 x = powmod(a, d, n);
+        if (x == 1 || x == n - 1) continue;
+        bool comp = true;
+        for (int i = 1; i < r; i++) {
+            x = mulmod(x, x, n);
+            if (x == n - 1) {
+                comp = false;
+                break;
+            }
+        }
+        if (comp) return false;
+    }
+    return true;
+}
+inline u128 u128_from(const uint8_t* p) {
+    uint64_t lo, hi;
+    memcpy(&lo, p, 8);
+    memcpy(&hi, p + 8, 8);
+    return ((u128)hi << 64) | lo;
+}
+inline void u128_put(u128 v, uint8_t* p) {
+    uint64_t lo = (uint64_t)v, hi = (uint64_t)(v >> 64);
+    memcpy(p, &lo, 8);
+    memcpy(p + 8, &hi, 8);
+}
+
+// eq_table_mod_q (src/ligerito_flock.rs:1330-1347): each coordinate is
+// appended as the new lowest index bit.
+std::vector<u128> eq_table_mod_q(const std::vector<u128>& r, u128 q) {
+    std::vector<u128> t = {1 % q};
+    for (u128 c : r) {
+        std::vector<u128> nx;
+        nx.reserve(t.size() * 2);
+        for (u128 v : t) {
+            u128 v1 = mulmod(v, c, q);
+            u128 v0 = v >= v1 ? v - v1 : v + q - v1;
+            nx.push_back(v0);
+            nx.push_back(v1);
+        }
+        t.swap(nx);
+    }
+    return t;
+}
+
+// 256-bit transcript draw reduced mod q (transcript_uniform_mod, src/prime_sampling.rs:22-33)
+u128 sample_residue(Transcript& tr, u128 q) {
+    uint8_t b[16];
+    tr.challenge(b, 16);
+    u128 lo = u128_from(b);
+    tr.challenge(b, 16);
+    u128 hi = u128_from(b);
+    // (hi * 2^128 + lo) mod q, q < 2^126
+    u128 two128 = mulmod((u128)1 << 64, (u128)1 << 64, q);
+    return addmod(mulmod(hi % q, two128, q), lo % q, q);
+}
+
+// Prime of exactly `bits` bits from the transcript's sampling stream.
+int sample_prime(Transcript& tr, int bits, u128* out) {
+    u128 top = (u128)1 << (bits - 1);
+    tr.absorb_slice(std::string("bitz/shared-prime-sampling/v1"));
+    uint8_t b[16];
+    u128_put(top, b);
+    tr.absorb_slice(b, 16);
+    u128_put(top | (top - 1), b);
+    tr.absorb_slice(b, 16);
+    int attempts = 64 * bits;
+    for (int a = 0; a < attempts; a++) {
+        tr.sampling_bytes(b, 16);
+        u128 cand = (u128_from(b) & (top - 1)) | top | 1;
+        if (miller_rabin(cand)) {
+            *out = cand;
+            return BITZ_OK;
+        }
+    }
+    set_error("prime search exhausted");
+    return BITZ_ERR_DEVICE;
+}
+
+// smallest_generator (src/pcs.rs:85-102): smallest n >= 2 generating GF(2^128)^x
+G gf_pow_u128(G b, u128 e) {
+    G acc = gone();
+    while (e) {
+        if (e & 1) acc = hgf_mul(acc, b);
+        b = hgf_mul(b, b);
+        e >>= 1;
+    }
+    return acc;
+}
+G smallest_generator() {
+    static const u128 kFactors[9] = {3, 5, 17, 257, 641, 65537, 274177, 6700417, (u128)67280421310721ull};
+    u128 order = ~(u128)0;
+    for (uint64_t n = 2;; n++) {
+        G a{n, 0};
+        bool ok = true;
+        for (u128 p : kFactors)
+            if (geq(gf_pow_u128(a, order / p), gone())) {
+                ok = false;
+                break;
+            }
+        if (ok) return a;
+    }
+}
+
+// -------------------------------------------------------------- proof writer
+struct Writer {
+    std::vector<uint8_t> b;
+    void u64(uint64_t v) { b.insert(b.end(), (uint8_t*)&v, (uint8_t*)&v + 8); }
+    void bytes(const void* p, size_t n) { b.insert(b.end(), (const uint8_t*)p, (const uint8_t*)p + n); }
+    void gf(G g) {
+        u64(g.lo);
+        u64(g.hi);
+    }
+};
+
+struct LevelOpen {
+    std::vector<G> rows;  // nq * lanes
+    uint32_t lanes = 0;
+    std::vector<uint8_t> proof;  // k * 32
+};
+
+void write_open(Writer& w, const LevelOpen& o) {
+    size_t nq = o.lanes ? o.rows.size() / o.lanes : 0;
+    w.u64(nq);
+    for (size_t i = 0; i < nq; i++) {
+        w.u64(o.lanes);
+        for (uint32_t l = 0; l < o.lanes; l++) w.gf(o.rows[i * o.lanes + l]);
+    }
+    w.u64(o.proof.size() / 32);
+    w.bytes(o.proof.data(), o.proof.size());
+}
+
+// ------------------------------------------------------------- the prover
+struct Prover {
+    bitz_ctx* ctx;
+    const bitz_commit_handle* h;
+    int t, s, m, mp;
+    LigConfig cfg;
+    G g;
+    // device buffers
+    DevBuf packed, rowimg, colimg, folds, roww, levels, eqc, eqy, tmp0, tmp1, tmp2, tmp3, qdev, pt;
+    std::vector<DevBuf*> lig_bufs;
+
+    int upload_gf(const std::vector<G>& v, DevBuf& buf) {
+        if (!buf.alloc(std::max<size_t>(v.size(), 1) * 16)) return BITZ_ERR_ALLOC;
+        if (!v.empty())
+            BITZ_HIP(hipMemcpyAsync(buf.ptr, v.data(), v.size() * 16, hipMemcpyHostToDevice, ctx->stream));
+        // the host vector may be freed by the caller: wait for the copy
+        BITZ_HIP(hipStreamSynchronize(ctx->stream));
+        return BITZ_OK;
+    }
+    int download(const void* dev, void* host, size_t bytes) {
+        BITZ_HIP(hipMemcpyAsync(host, dev, bytes, hipMemcpyDeviceToHost, ctx->stream));
+        BITZ_HIP(hipStreamSynchronize(ctx->stream));
+        return BITZ_OK;
+    }
+    int result(int K, G* out) {
+        uint8_t buf[64];
+        int rc = read_result(ctx, K, buf);
+        if (rc) return rc;
+        for (int k = 0; k < K; k++) out[k] = gfrom(buf + 16 * k);
+        return BITZ_OK;
+    }
+    // eq table over host coordinates into `buf`
+    int eq_dev(const std::vector<G>& r, DevBuf& buf) {
+        if (!pt.alloc(std::max<size_t>(r.size(), 1) * 16)) return BITZ_ERR_ALLOC;
+        if (!r.empty()) {
+            BITZ_HIP(hipMemcpyAsync(pt.ptr, r.data(), r.size() * 16, hipMemcpyHostToDevice, ctx->stream));
+            BITZ_HIP(hipStreamSynchronize(ctx->stream));
+        }
+        if (!buf.alloc((16ull << r.size()))) return BITZ_ERR_ALLOC;
+        return eq_table(ctx, pt.ptr, (int)r.size(), buf.ptr);
+    }
+
+    // Round 0 value y = MLE[p_msg](point) (ood_eval, src/ligerito_flock.rs:1459-1486)
+    int mle_packed(const std::vector<G>& point, G* y) {
+        int rc = eq_dev(point, tmp0);
+        if (rc) return rc;
+        rc = dot(ctx, h->packed.ptr, tmp0.ptr, 1ull << mp, nullptr);
+        if (rc) return rc;
+        return result(1, y);
+    }
+
+    // ---- batched grand-product GKR (Forest::prove == gpgkr_prove) ----
+    int gkr_layer(InnerTranscript& it, uint4* l, uint4* r, bool leaf, std::vector<G>& internal, G* claim) {
+        const int L = (int)internal.size();
+        std::vector<G> ext(internal.rbegin(), internal.rend());
+        std::vector<G> next;
+        G factor = gone();
+        int rc;
+        int eqc_built = -1;
+        for (int round = 0; round < L; round++) {
+            G z = internal[round];
+            int rem = L - 1 - round;
+            uint64_t hh = 1ull << rem;
+            int s_c = std::min(s, rem);
+            if (eqc_built != s_c || s_c < s) {
+                std::vector<G> c(ext.begin(), ext.begin() + s_c);
+                if ((rc = eq_dev(c, eqc))) return rc;
+                eqc_built = s_c;
+            }
+            std::vector<G> yv(ext.begin() + s_c, ext.begin() + rem);
+            if ((rc = eq_dev(yv, eqy))) return rc;
+            int send_one = geq(z, gzero()) ? 1 : 0;
+            if (leaf && round == 0)
+                rc = gkr_leaf_round_sums(ctx, (const uint64_t*)packed.ptr, rowimg.ptr, t, s, eqc.ptr, s_c, eqy.ptr,
+                                         send_one, nullptr);
+            else
+                rc = gkr_round_sums(ctx, l, r, hh, eqc.ptr, s_c, eqy.ptr, send_one, nullptr);
+            if (rc) return rc;
+            G sums[2];
+            if ((rc = result(2, sums))) return rc;
+            uint8_t msg[32];
+            gput(hgf_mul(factor, sums[0]), msg);
+            gput(hgf_mul(factor, sums[1]), msg + 16);
+            it.prover_message(msg, 32);
+            uint8_t rb[16];
+            it.verifier_gf(rb);
+            G rr = gfrom(rb);
+            next.push_back(rr);
+            if (leaf && round == 0) {
+                if ((rc = gkr_leaf_fold(ctx, (const uint64_t*)packed.ptr, rowimg.ptr, t, s, rb, l, r))) return rc;
+            } else {
+                if ((rc = fold_halves(ctx, l, hh, rb))) return rc;
+                if ((rc = fold_halves(ctx, r, hh, rb))) return rc;
+            }
+            factor = hgf_mul(factor, eq_factor(rr, z));
+        }
+        uint8_t last[32];
+        if ((rc = download(l, last, 16))) return rc;
+        if ((rc = download(r, last + 16, 16))) return rc;
+        it.prover_message(last, 32);
+        uint8_t rb[16];
+        it.verifier_gf(rb);
+        G rc_ = gfrom(rb);
+        G el = gfrom(last), er = gfrom(last + 16);
+        *claim = hgf_add(el, hgf_mul(rc_, hgf_add(er, el)));
+        internal.clear();
+        internal.push_back(rc_);
+        internal.insert(internal.end(), next.begin(), next.end());
+        return BITZ_OK;
+    }
+
+    // returns the leaf point (LE, t+s coordinates) and claim
+    int gkr(InnerTranscript& it, const std::vector<G>& zeta, std::vector<G>* point, G* claim) {
+        // levels 1..t-1 materialised: level j has 2^(t+s-j) entries, stored
+        // back to back (level 1 first)
+        uint64_t total = 0;
+        for (int j = 1; j < t; j++) total += 1ull << (t + s - j);
+        if (!levels.alloc(total * 16)) return BITZ_ERR_ALLOC;
+        std::vector<uint4*> lv(t + 1, nullptr);
+        uint4* base = (uint4*)levels.ptr;
+        uint64_t off = 0;
+        for (int j = 1; j < t; j++) {
+            lv[j] = base + off;
+            off += 1ull << (t + s - j);
+        }
+        int rc = gkr_level1(ctx, (const uint64_t*)packed.ptr, rowimg.ptr, t, s, lv[1]);
+        if (rc) return rc;
+        for (int j = 1; j + 1 < t; j++)
+            if ((rc = product_level(ctx, lv[j], 1ull << (t + s - j), lv[j + 1]))) return rc;
+        std::vector<G> internal(zeta.rbegin(), zeta.rend());
+        G cl = gzero();
+        for (int j = t - 1; j >= 1; j--) {
+            uint64_t half = 1ull << (t + s - j - 1);
+            if ((rc = gkr_layer(it, lv[j], lv[j] + half, false, internal, &cl))) return rc;
+        }
+        // leaf layer: the first fold lands in level 1's storage
+        uint64_t q = 1ull << (t + s - 2);
+        if ((rc = gkr_layer(it, lv[1], lv[1] + q, true, internal, &cl))) return rc;
+        point->assign(internal.rbegin(), internal.rend());
+        *claim = cl;
+        return BITZ_OK;
+    }
+
+    // ---- inner-product sumcheck (src/bitz/sumcheck.rs:146-198, 372-400) ----
+    int sumcheck_rounds(InnerTranscript& it, DevBuf& a, DevBuf& w, DevBuf& a2, DevBuf& w2, uint64_t n, G* target,
+                        std::vector<G>& point) {
+        int rc;
+        void* A = a.ptr;
+        void* W = w.ptr;
+        void* A2 = a2.ptr;
+        void* W2 = w2.ptr;
+        while (n > 1) {
+            if ((rc = round_poly(ctx, A, W, n, nullptr))) return rc;
+            G v[3];
+            if ((rc = result(3, v))) return rc;
+            G c0 = v[0], c2 = v[2], c1 = hgf_add(hgf_add(v[0], v[1]), v[2]);
+            if (!geq(hgf_add(c1, c2), *target)) {
+                set_error("inner sumcheck: claim does not match the committed bits");
+                return BITZ_ERR_ARG;
+            }
+            uint8_t msg[48];
+            gput(c0, msg);
+            gput(c1, msg + 16);
+            gput(c2, msg + 32);
+            it.prover_message(msg, 48);
+            uint8_t rb[16];
+            it.verifier_gf(rb);
+            G r = gfrom(rb);
+            *target = hgf_add(c0, hgf_mul(r, hgf_add(c1, hgf_mul(r, c2))));
+            point.push_back(r);
+            if ((rc = fold_lsb(ctx, A, n, rb, A2))) return rc;
+            if ((rc = fold_lsb(ctx, W, n, rb, W2))) return rc;
+            std::swap(A, A2);
+            std::swap(W, W2);
+            n >>= 1;
+        }
+        // final values live in A[0], W[0]
+        last_a = A;
+        last_w = W;
+        return BITZ_OK;
+    }
+    void* last_a = nullptr;
+    void* last_w = nullptr;
+
+    int commit_level(const void* f, int log_msg, int log_lanes, int rate, DevBuf& cw, DevBuf& tree, uint8_t root[32]) {
+        int k_code = log_msg - log_lanes + rate;
+        if (!cw.alloc((16ull << (log_msg + rate))) || !tree.alloc((2ull << k_code) * 32)) return BITZ_ERR_ALLOC;
+        int rc = commit_device(ctx, f, log_msg, log_lanes, rate, cw.ptr, tree.ptr);
+        if (rc) return rc;
+        return download((const uint8_t*)tree.ptr + ((2ull << k_code) - 2) * 32, root, 32);
+    }
+
+    // rows at sorted queries (lanes each) and the octopus multi-proof
+    // (merkle.rs:698-745) from a device codeword/tree
+    int open_level(const void* cw, const void* tree, uint64_t n_leaves, uint32_t lanes,
+                   const std::vector<uint64_t>& queries, LevelOpen* out) {
+        out->lanes = lanes;
+        out->rows.resize(queries.size() * lanes);
+        int rc;
+        if (!qdev.alloc(std::max<size_t>(queries.size(), 1) * 8 + 16)) return BITZ_ERR_ALLOC;
+        BITZ_HIP(hipMemcpyAsync(qdev.ptr, queries.data(), queries.size() * 8, hipMemcpyHostToDevice, ctx->stream));
+        if (!tmp3.alloc(std::max<size_t>(queries.size() * lanes, 1) * 16)) return BITZ_ERR_ALLOC;
+        if ((rc = gather_rows(ctx, cw, (const uint64_t*)qdev.ptr, (uint32_t)queries.size(), lanes, tmp3.ptr)))
+            return rc;
+        if ((rc = download(tmp3.ptr, out->rows.data(), out->rows.size() * 16))) return rc;
+        // sibling node indices
+        std::vector<uint64_t> active(queries), nodes;
+        uint64_t level_start = 0, level_len = n_leaves;
+        while (level_len > 1) {
+            std::vector<uint64_t> nx;
+            size_t i = 0;
+            while (i < active.size()) {
+                uint64_t p = active[i];
+                bool sib = i + 1 < active.size() && active[i + 1] == (p ^ 1);
+                if (sib)
+                    i += 2;
+                else {
+                    nodes.push_back(level_start + (p ^ 1));
+                    i += 1;
+                }
+                nx.push_back(p >> 1);
+            }
+            active.swap(nx);
+            level_start += level_len;
+            level_len >>= 1;
+        }
+        out->proof.resize(nodes.size() * 32);
+        if (!nodes.empty()) {
+            if (!qdev.alloc(nodes.size() * 8 + 16)) return BITZ_ERR_ALLOC;
+            BITZ_HIP(hipMemcpyAsync(qdev.ptr, nodes.data(), nodes.size() * 8, hipMemcpyHostToDevice, ctx->stream));
+            if (!tmp3.alloc(nodes.size() * 32)) return BITZ_ERR_ALLOC;
+            if ((rc = gather_nodes(ctx, tree, (const uint64_t*)qdev.ptr, (uint32_t)nodes.size(), tmp3.ptr)))
+                return rc;
+            if ((rc = download(tmp3.ptr, out->proof.data(), out->proof.size()))) return rc;
+        }
+        return BITZ_OK;
+    }
+
+    int run(const uint8_t* claimed_in, uint8_t* claimed_out, std::vector<uint8_t>* proof_out);
+};
+
+// Ligerito challenger over the inner transcript (src/bitz/pcs.rs:1062-1175)
+struct Challenger {
+    InnerTranscript& it;
+    bitz_ctx* ctx;
+    bool first_f128 = true;
+    void observe_label(const std::string& l) { it.public_message(l); }
+    void observe_f128(G v) {
+        uint8_t b[16];
+        gput(v, b);
+        if (first_f128) {
+            first_f128 = false;
+            it.public_message(b, 16);
+            return;
+        }
+        it.prover_message(b, 16);
+    }
+    void observe_bytes(const uint8_t* p, size_t n) { it.prover_message_bytes(p, n); }
+    G sample() {
+        uint8_t b[16];
+        it.verifier_gf(b);
+        return gfrom(b);
+    }
+    std::vector<G> sample_vec(size_t n) {
+        it.public_message(std::string("pcs/flock/sample-vector/v1"));
+        uint64_t nn = n;
+        it.public_message(&nn, 8);
+        std::vector<G> v(n);
+        for (auto& x : v) x = sample();
+        return v;
+    }
+    int grind(uint32_t bits, uint64_t* nonce) {
+        it.public_message(std::string("pcs/flock/pow/v1"));
+        it.public_message(&bits, 4);
+        uint8_t seed[16];
+        it.verifier_gf(seed);
+        static const uint8_t kPrefix[16] = {'b', 'i', 't', 'z', '-', 'p', 'c', 's', '-', 'p', 'o', 'w', '-', 'v', '1', 0};
+        int rc = pow_find(ctx, kPrefix, seed, bits, nonce);
+        if (rc) return rc;
+        it.prover_message(nonce, 8);
+        return BITZ_OK;
+    }
+    // sample_distinct_queries (ligerito.rs:3191-3222)
+    std::vector<uint64_t> queries(uint64_t block_len, int count) {
+        std::set<uint64_t> seen;
+        std::vector<uint64_t> out;
+        while ((int)out.size() < count) {
+            G v = sample();
+            uint64_t q = v.lo % block_len;
+            if (seen.insert(q).second) out.push_back(q);
+        }
+        std::sort(out.begin(), out.end());
+        return out;
+    }
+};
+
+int ceil_log2(uint64_t n) {
+    int k = 0;
+    while ((1ull << k) < n) k++;
+    return k;
+}
+
+int Prover::run(const uint8_t* claimed_in, uint8_t* claimed_out, std::vector<uint8_t>* proof_out) {
+    int rc;
+    const uint64_t nrows = 1ull << t, ncols = 1ull << s;
+    // ---------------- outer transcript: statement and Round 0 ----------------
+    Transcript outer;
+    outer.absorb_slice(std::string("bitz/ligerito-flock/standalone-mod-q-statement/v1"));
+    outer.absorb_slice(h->root, 32);
+    {
+        std::vector<uint64_t> frame = {(uint64_t)cfg.m, (uint64_t)cfg.initial_k, (uint64_t)cfg.yr_log_n,
+                                       (uint64_t)cfg.target_bits};
+        for (auto& lv : cfg.levels)
+            for (int x : {lv.log_inv_rate, lv.log_msg_cols, lv.log_num_interleaved, lv.k_recursive, lv.queries,
+                          lv.grinding_bits, lv.fold_grinding_bits, lv.ood_samples})
+                frame.push_back((uint64_t)x);
+        outer.absorb_slice(frame.data(), frame.size() * 8);
+        uint64_t ts[2] = {(uint64_t)t, (uint64_t)s};
+        outer.absorb_slice(ts, 16);
+        uint8_t gb[16];
+        gput(g, gb);
+        outer.absorb_slice(gb, 16);
+    }
+    const int q_bits = std::min(113, 126 - t);  // standalone_q_bits (src/ligerito_flock.rs:1286-1293)
+    outer.absorb_u64((uint64_t)q_bits);
+    outer.absorb_u64((uint64_t)cfg.round0_grinding_bits + 1);
+    outer.absorb_slice(std::string("bitz/bitz-opener/standalone/v1"));
+    // Round 0 (prove_ood_round_packed, src/ligerito_flock.rs:1520-1557)
+    outer.absorb_slice(std::string("bitz/core/ood-round/v1"));
+    outer.absorb_u64((uint64_t)mp);
+    uint32_t r0bits = (uint32_t)cfg.round0_grinding_bits;
+    outer.absorb_slice(&r0bits, 4);
+    uint64_t r0_nonce = 0;
+    if (r0bits) {
+        uint8_t seed[16];
+        outer.challenge(seed, 16);
+        static const uint8_t kNative[16] = {'b', 'i', 't', 'z', '-', 'n', 'a', 't', 'i', 'v', 'e', '-', 'p', 'o', 'w', '1'};
+        if ((rc = pow_find(ctx, kNative, seed, r0bits, &r0_nonce))) return rc;
+        outer.absorb_u64(r0_nonce);
+    }
+    uint8_t zb[16];
+    outer.challenge(zb, 16);
+    std::vector<G> ood_point(mp);
+    {
+        G cur = gfrom(zb);
+        for (int i = 0; i < mp; i++) {
+            ood_point[i] = cur;
+            cur = hgf_mul(cur, cur);
+        }
+    }
+    G ood_y;
+    if ((rc = mle_packed(ood_point, &ood_y))) return rc;
+    uint8_t yb[16];
+    gput(ood_y, yb);
+    outer.absorb_slice(yb, 16);
+    // ---------------- the standalone instance ----------------
+    u128 q;
+    if ((rc = sample_prime(outer, q_bits, &q))) return rc;
+    std::vector<u128> r1(t), r2(s);
+    for (auto& x : r1) x = sample_residue(outer, q);
+    for (auto& x : r2) x = sample_residue(outer, q);
+    std::vector<u128> row_w = eq_table_mod_q(r1, q), col_w = eq_table_mod_q(r2, q);
+    // folds eta_c = sum_b w_b bit(c, b) (exact u128) on the device
+    {
+        std::vector<uint8_t> wb(nrows * 16);
+        for (uint64_t b = 0; b < nrows; b++) u128_put(row_w[b], wb.data() + 16 * b);
+        if (!roww.alloc(nrows * 16) || !folds.alloc(ncols * 16)) return BITZ_ERR_ALLOC;
+        BITZ_HIP(hipMemcpyAsync(roww.ptr, wb.data(), wb.size(), hipMemcpyHostToDevice, ctx->stream));
+        if ((rc = fold_columns(ctx, (const uint64_t*)h->packed.ptr, t, s, roww.ptr, folds.ptr))) return rc;
+        BITZ_HIP(hipStreamSynchronize(ctx->stream));
+    }
+    std::vector<uint8_t> fold_bytes(ncols * 16);
+    if ((rc = download(folds.ptr, fold_bytes.data(), fold_bytes.size()))) return rc;
+    u128 honest = 0;
+    for (uint64_t c = 0; c < ncols; c++) honest = addmod(honest, mulmod(col_w[c], u128_from(&fold_bytes[16 * c]) % q, q), q);
+    u128 claimed = claimed_in ? u128_from(claimed_in) : honest;
+    if (claimed_out) u128_put(honest, claimed_out);
+    if (!proof_out) return BITZ_OK;  // evaluation only
+    if (claimed >= q) {
+        set_error("BitZ claim: value not reduced");
+        return BITZ_ERR_ARG;
+    }
+    outer.absorb_slice(std::string("bitz/ligerito-flock/standalone-mod-q-claim/v1"));
+    {
+        uint8_t qc[32];
+        u128_put(q, qc);
+        u128_put(claimed, qc + 16);
+        outer.absorb_slice(qc, 32);
+    }
+    uint8_t tag[32];
+    outer.challenge(tag, 16);
+    outer.challenge(tag + 16, 16);
+
+    // ---------------- BitZ opening on the inner transcript ----------------
+    InnerTranscript it("bitz/bitz-opener/standalone/v1", tag, 32);
+    it.public_message(h->root, 32);
+    {
+        uint8_t frame[48];
+        uint64_t tt = t, ss = s;
+        memcpy(frame, &tt, 8);
+        memcpy(frame + 8, &ss, 8);
+        u128_put(q, frame + 16);
+        gput(g, frame + 32);
+        it.public_message(frame, 48);
+        // LinearClaim encoding (src/bitz/params.rs:273-285)
+        std::vector<uint8_t> enc;
+        enc.reserve(16 * (nrows + ncols) + 32);
+        uint64_t len = nrows;
+        enc.insert(enc.end(), (uint8_t*)&len, (uint8_t*)&len + 8);
+        for (auto v : row_w) {
+            uint8_t b[16];
+            u128_put(v, b);
+            enc.insert(enc.end(), b, b + 16);
+        }
+        len = ncols;
+        enc.insert(enc.end(), (uint8_t*)&len, (uint8_t*)&len + 8);
+        for (auto v : col_w) {
+            uint8_t b[16];
+            u128_put(v, b);
+            enc.insert(enc.end(), b, b + 16);
+        }
+        uint8_t b[16];
+        u128_put(claimed, b);
+        enc.insert(enc.end(), b, b + 16);
+        it.public_message(enc.data(), enc.size());
+    }
+    // fold round (src/bitz/fold.rs:237-263)
+    for (uint64_t c = 0; c < ncols; c++) it.prover_message(&fold_bytes[16 * c], 16);
+    std::vector<G> zeta(s);
+    for (auto& z : zeta) {
+        uint8_t b[16];
+        it.verifier_gf(b);
+        z = gfrom(b);
+    }
+    uint8_t gb[16];
+    gput(g, gb);
+    if (!rowimg.alloc(nrows * 16)) return BITZ_ERR_ALLOC;
+    if ((rc = pow_images(ctx, gb, roww.ptr, nrows, rowimg.ptr))) return rc;
+    // packed columns for the leaf layer and the combined rows
+    uint64_t groups = (ncols + 63) / 64;
+    if (!packed.alloc((groups << t) * 8)) return BITZ_ERR_ALLOC;
+    if ((rc = pack_columns(ctx, (const uint64_t*)h->packed.ptr, t, s, (uint64_t*)packed.ptr))) return rc;
+    // GKR (reduce.rs:100-114)
+    std::vector<G> leaf_point;
+    G leaf_claim;
+    if ((rc = gkr(it, zeta, &leaf_point, &leaf_claim))) return rc;
+    levels = DevBuf();  // release the product tree
+    // exit_from_terminal (reduce.rs:28-58)
+    std::vector<G> alpha_c(leaf_point.begin(), leaf_point.begin() + s);
+    std::vector<G> alpha_b(leaf_point.begin() + s, leaf_point.end());
+    G ip_target = hgf_add(leaf_claim, gone());
+    DevBuf u1, u2, a2, w2;
+    if ((rc = eq_dev(alpha_b, tmp0))) return rc;
+    if (!u1.alloc(nrows * 16)) return BITZ_ERR_ALLOC;
+    if ((rc = mul_elem(ctx, rowimg.ptr, tmp0.ptr, nrows, u1.ptr, 1))) return rc;
+    if ((rc = eq_dev(alpha_c, u2))) return rc;
+    // prove_lin / InnerProduct statement (src/bitz/pcs.rs:362-372, 878-888)
+    {
+        it.public_message(std::string("bitz/pcs/bit-inner-product/v2"));
+        it.public_message(h->root, 32);
+        uint64_t pcs40[5] = {(uint64_t)m, (uint64_t)h->log_inv_rate, (uint64_t)h->log_batch, 0, 1};
+        it.public_message(pcs40, 40);
+        std::vector<uint8_t> enc(16 * (nrows + ncols) + 32);
+        uint64_t len = nrows;
+        memcpy(enc.data(), &len, 8);
+        if ((rc = download(u1.ptr, enc.data() + 8, nrows * 16))) return rc;
+        len = ncols;
+        memcpy(enc.data() + 8 + nrows * 16, &len, 8);
+        if ((rc = download(u2.ptr, enc.data() + 16 + nrows * 16, ncols * 16))) return rc;
+        gput(ip_target, enc.data() + 16 + (nrows + ncols) * 16);
+        it.public_message(enc.data(), enc.size());
+    }
+    it.public_message(std::string("bitz/pcs/inner-product-sumcheck/v1"));
+    // row rounds on the column-combined table
+    DevBuf comb;
+    if (!comb.alloc(nrows * 16) || !a2.alloc(nrows * 16) || !w2.alloc(nrows * 16)) return BITZ_ERR_ALLOC;
+    if ((rc = xi_combined(ctx, (const uint64_t*)packed.ptr, t, s, u2.ptr, comb.ptr))) return rc;
+    std::vector<G> sc_point;
+    G target = ip_target;
+    if ((rc = sumcheck_rounds(it, comb, u1, a2, w2, nrows, &target, sc_point))) return rc;
+    uint8_t rs_bytes[16];
+    if ((rc = download(last_w, rs_bytes, 16))) return rc;
+    // column rounds on the row-folded table
+    if ((rc = eq_dev(sc_point, tmp0))) return rc;
+    DevBuf folded, cols, f2, c2;
+    if (!folded.alloc(ncols * 16) || !cols.alloc(ncols * 16) || !f2.alloc(ncols * 16) || !c2.alloc(ncols * 16))
+        return BITZ_ERR_ALLOC;
+    if ((rc = fold_rows_eq(ctx, (const uint64_t*)h->packed.ptr, t, s, tmp0.ptr, folded.ptr))) return rc;
+    if ((rc = scale(ctx, u2.ptr, ncols, rs_bytes, cols.ptr))) return rc;
+    if ((rc = sumcheck_rounds(it, folded, cols, f2, c2, ncols, &target, sc_point))) return rc;
+    uint8_t evb[32];
+    if ((rc = download(last_a, evb, 16))) return rc;
+    if ((rc = download(last_w, evb + 16, 16))) return rc;
+    G evaluation = gfrom(evb);
+    if (!geq(target, hgf_mul(evaluation, gfrom(evb + 16)))) {
+        set_error("inner sumcheck: final check failed");
+        return BITZ_ERR_ARG;
+    }
+    it.prover_message(evb, 16);
+    // bind_mle_statement (src/bitz/pcs.rs:862-876)
+    {
+        it.public_message(std::string("bitz/pcs/mle-opening/v1"));
+        it.public_message(h->root, 32);
+        uint64_t pcs40[5] = {(uint64_t)m, (uint64_t)h->log_inv_rate, (uint64_t)h->log_batch, 0, 1};
+        it.public_message(pcs40, 40);
+        uint64_t n = sc_point.size();
+        it.public_message(&n, 8);
+        for (auto& c : sc_point) {
+            uint8_t b[16];
+            gput(c, b);
+            it.public_message(b, 16);
+        }
+        it.public_message(evb, 16);
+    }
+    // ---------------- ring switch (prove_mle, src/bitz/pcs.rs:439-507) ----------------
+    std::vector<G> prefix_pt(sc_point.begin(), sc_point.begin() + 7);
+    std::vector<G> suffix_pt(sc_point.begin() + 7, sc_point.end());
+    std::vector<G> prefix = host_eq(prefix_pt);
+    DevBuf suffix, claims_dev, basis;
+    if ((rc = eq_dev(suffix_pt, suffix))) return rc;
+    const uint64_t np = 1ull << mp;
+    if (!claims_dev.alloc(128 * 16)) return BITZ_ERR_ALLOC;
+    if ((rc = fold_1b_rows(ctx, h->packed.ptr, suffix.ptr, np, claims_dev.ptr))) return rc;
+    std::vector<G> claims(128);
+    if ((rc = download(claims_dev.ptr, claims.data(), 128 * 16))) return rc;
+    {
+        G cc = gzero();
+        for (int i = 0; i < 128; i++) cc = hgf_add(cc, hgf_mul(prefix[i], claims[i]));
+        if (!geq(cc, evaluation)) {
+            set_error("ring switch: claim check failed");
+            return BITZ_ERR_ARG;
+        }
+    }
+    it.public_message(std::string("bitz/pcs/mle-claims/v1"));
+    it.prover_message(claims.data(), 128 * 16);
+    it.public_message(std::string("bitz/pcs/ring-switch-challenges/v1"));
+    std::vector<G> batch_pt(7);
+    for (auto& x : batch_pt) {
+        uint8_t b[16];
+        it.verifier_gf(b);
+        x = gfrom(b);
+    }
+    std::vector<G> bw = host_eq(batch_pt);
+    // batch_claims: <transpose(claims), bw>
+    G packed_target = gzero();
+    for (int b = 0; b < 128; b++) {
+        G u = gzero();
+        for (int i = 0; i < 128; i++) {
+            uint64_t bit = b < 64 ? (claims[i].lo >> b) & 1 : (claims[i].hi >> (b - 64)) & 1;
+            if (!bit) continue;
+            if (i < 64)
+                u.lo |= 1ull << i;
+            else
+                u.hi |= 1ull << (i - 64);
+        }
+        packed_target = hgf_add(packed_target, hgf_mul(u, bw[b]));
+    }
+    if ((rc = upload_gf(bw, tmp1))) return rc;
+    if (!basis.alloc(np * 16)) return BITZ_ERR_ALLOC;
+    if ((rc = fold_b128(ctx, suffix.ptr, np, tmp1.ptr, basis.ptr))) return rc;
+    // the Round-0 claim joins the batch (bind_ood_claim, pcs.rs:904-912)
+    {
+        it.public_message(std::string("bitz/bitz/round0-claim/v1"));
+        uint64_t n = ood_point.size();
+        it.public_message(&n, 8);
+        for (auto& c : ood_point) {
+            uint8_t b[16];
+            gput(c, b);
+            it.public_message(b, 16);
+        }
+        it.public_message(yb, 16);
+        uint8_t eb[16];
+        it.verifier_gf(eb);
+        if ((rc = eq_dev(ood_point, tmp0))) return rc;
+        if ((rc = axpy(ctx, basis.ptr, tmp0.ptr, np, eb))) return rc;
+        packed_target = hgf_add(packed_target, hgf_mul(gfrom(eb), ood_y));
+    }
+    suffix = DevBuf();
+
+    // ---------------- Ligerito (recursive_prover_with_basis) ----------------
+    Challenger ch{it, ctx};
+    ch.observe_label("flock-ligerito-basis-v0");
+    ch.observe_f128(packed_target);
+    ch.observe_bytes(h->root, 32);
+    std::vector<G> sc_msgs;  // (u0, u2) pairs flattened
+    std::vector<uint64_t> grinding_nonces, fold_nonces;
+    std::vector<G> ood_values;
+    DevBuf F[2], B[2];
+    int cur = 0;
+    if (!F[0].alloc(np * 16) || !B[0].alloc(np * 16) || !F[1].alloc(np * 8) || !B[1].alloc(np * 8))
+        return BITZ_ERR_ALLOC;
+    BITZ_HIP(hipMemcpyAsync(F[0].ptr, h->packed.ptr, np * 16, hipMemcpyDeviceToDevice, ctx->stream));
+    BITZ_HIP(hipMemcpyAsync(B[0].ptr, basis.ptr, np * 16, hipMemcpyDeviceToDevice, ctx->stream));
+    basis = DevBuf();
+    uint64_t fn = np;
+    auto msg = [&](const void* f, const void* b, uint64_t n, bool with_eval, G* out) -> int {
+        int r_ = lig_msg(ctx, f, b, n, with_eval ? 1 : 0, nullptr);
+        if (r_) return r_;
+        return result(with_eval ? 3 : 2, out);
+    };
+    auto observe_msg = [&](G u0, G u2) {
+        ch.observe_f128(u0);
+        ch.observe_f128(u2);
+        sc_msgs.push_back(u0);
+        sc_msgs.push_back(u2);
+    };
+    auto fold_round = [&](int grind_bits, G* r_out) -> int {
+        if (grind_bits > 0) {
+            uint64_t nonce;
+            int r_ = ch.grind((uint32_t)grind_bits, &nonce);
+            if (r_) return r_;
+            fold_nonces.push_back(nonce);
+        }
+        G r = ch.sample();
+        uint8_t rb[16];
+        gput(r, rb);
+        int r_ = lig_fold(ctx, F[cur].ptr, B[cur].ptr, fn, rb, F[cur ^ 1].ptr, B[cur ^ 1].ptr);
+        if (r_) return r_;
+        cur ^= 1;
+        fn >>= 1;
+        G u[2];
+        if ((r_ = msg(F[cur].ptr, B[cur].ptr, fn, false, u))) return r_;
+        observe_msg(u[0], u[1]);
+        *r_out = r;
+        return BITZ_OK;
+    };
+    auto glue = [&](const void* bnew, G beta) -> int {
+        uint8_t bb[16];
+        gput(beta, bb);
+        return axpy(ctx, B[cur].ptr, bnew, fn, bb);
+    };
+    G T = packed_target;
+    {
+        G u[2];
+        if ((rc = msg(F[0].ptr, B[0].ptr, fn, false, u))) return rc;
+        observe_msg(u[0], u[1]);
+    }
+    const int r_steps = (int)cfg.levels.size() - 1;
+    std::vector<G> r_lane;
+    for (int j = 0; j < cfg.initial_k; j++) {
+        G r;
+        if ((rc = fold_round(std::max(0, cfg.levels[0].fold_grinding_bits - j), &r))) return rc;
+        r_lane.push_back(r);
+    }
+    // commit f1
+    DevBuf cw_prev, tree_prev, cw_next, tree_next;
+    int n1 = mp - cfg.initial_k;
+    uint8_t root_prev[32];
+    if ((rc = commit_level(F[cur].ptr, n1, cfg.levels[1].log_num_interleaved, cfg.levels[1].log_inv_rate, cw_prev,
+                           tree_prev, root_prev)))
+        return rc;
+    ch.observe_bytes(root_prev, 32);
+    std::vector<std::vector<uint8_t>> recursive_roots = {std::vector<uint8_t>(root_prev, root_prev + 32)};
+    auto ood_round = [&](int level, int n_vars) -> int {
+        for (int k = 0; k < cfg.levels[level].ood_samples; k++) {
+            std::vector<G> z = ch.sample_vec(n_vars);
+            int r_ = eq_dev(z, tmp0);
+            if (r_) return r_;
+            G u[3];
+            if ((r_ = msg(F[cur].ptr, tmp0.ptr, fn, true, u))) return r_;
+            ch.observe_f128(u[2]);
+            ood_values.push_back(u[2]);
+            observe_msg(u[0], u[1]);
+            G beta = ch.sample();
+            if ((r_ = glue(tmp0.ptr, beta))) return r_;
+            T = hgf_add(T, hgf_mul(beta, u[2]));
+        }
+        return BITZ_OK;
+    };
+    if ((rc = ood_round(1, n1))) return rc;
+    // induce a level's basis from its opened rows (induce_sumcheck_poly)
+    auto induce_intro = [&](int log_n, const LevelOpen& op, const std::vector<G>& v_ch,
+                            const std::vector<uint64_t>& queries, const std::vector<G>& alpha) -> int {
+        std::vector<G> aw = host_eq(alpha);
+        aw.resize(queries.size());
+        std::vector<G> eqv = host_eq(v_ch);
+        G enforced = gzero();
+        for (size_t i = 0; i < queries.size(); i++) {
+            G d = gzero();
+            for (uint32_t l = 0; l < op.lanes; l++) d = hgf_add(d, hgf_mul(op.rows[i * op.lanes + l], eqv[l]));
+            enforced = hgf_add(enforced, hgf_mul(d, aw[i]));
+        }
+        int r_ = upload_gf(aw, tmp1);
+        if (r_) return r_;
+        if (!qdev.alloc(queries.size() * 8 + 16)) return BITZ_ERR_ALLOC;
+        BITZ_HIP(hipMemcpyAsync(qdev.ptr, queries.data(), queries.size() * 8, hipMemcpyHostToDevice, ctx->stream));
+        // s_k(v_k) and inverses (eval_sk_at_vks)
+        std::vector<G> sks(log_n + 1, gzero()), inv(log_n + 1);
+        sks[0] = gone();
+        {
+            std::vector<G> layer(log_n);
+            for (int i = 1; i <= log_n; i++) layer[i - 1] = G{1ull << i, 0};
+            int c_ = log_n;
+            for (int i = 0; i < log_n; i++) {
+                for (int j = 0; j < c_; j++) {
+                    G sv = layer[j];
+                    G v = hgf_add(hgf_mul(sv, sv), hgf_mul(sks[i], sv));
+                    if (j == 0)
+                        sks[i + 1] = v;
+                    else
+                        layer[j - 1] = v;
+                }
+                c_--;
+            }
+        }
+        for (int k = 0; k <= log_n; k++) inv[k] = hgf_inv(sks[k]);
+        std::vector<G> both(sks);
+        both.insert(both.end(), inv.begin(), inv.end());
+        if ((r_ = upload_gf(both, tmp2))) return r_;
+        if (!tmp0.alloc(16ull << log_n)) return BITZ_ERR_ALLOC;
+        if ((r_ = induce_basis(ctx, (const uint64_t*)qdev.ptr, (uint32_t)queries.size(), log_n, tmp2.ptr,
+                               (const uint4*)tmp2.ptr + (log_n + 1), tmp1.ptr, tmp0.ptr)))
+            return r_;
+        G u[2];
+        if ((r_ = msg(F[cur].ptr, tmp0.ptr, fn, false, u))) return r_;
+        observe_msg(u[0], u[1]);
+        G beta = ch.sample();
+        if ((r_ = glue(tmp0.ptr, beta))) return r_;
+        T = hgf_add(T, hgf_mul(beta, enforced));
+        return BITZ_OK;
+    };
+    // L0 queries
+    uint64_t nonce;
+    if ((rc = ch.grind((uint32_t)cfg.levels[0].grinding_bits, &nonce))) return rc;
+    grinding_nonces.push_back(nonce);
+    const LigLevel& L0 = cfg.levels[0];
+    uint64_t block0 = 1ull << (L0.log_msg_cols + L0.log_inv_rate);
+    std::vector<uint64_t> q0 = ch.queries(block0, L0.queries);
+    std::vector<G> alpha0 = ch.sample_vec(ceil_log2(L0.queries));
+    LevelOpen open0;
+    if ((rc = open_level(h->codeword.ptr, h->tree.ptr, block0, 1u << cfg.initial_k, q0, &open0))) return rc;
+    if ((rc = induce_intro(n1, open0, r_lane, q0, alpha0))) return rc;
+
+    std::vector<LevelOpen> rec_opens;
+    LevelOpen final_open;
+    std::vector<G> yr;
+    int prev_level = 1;  // level index of the commitment held in cw_prev
+    for (int i = 0; i < r_steps; i++) {
+        int k_i = cfg.levels[i + 1].k_recursive;
+        std::vector<G> level_rs;
+        for (int j = 0; j < k_i; j++) {
+            G r;
+            if ((rc = fold_round(std::max(0, cfg.levels[i + 1].fold_grinding_bits - j), &r))) return rc;
+            level_rs.push_back(r);
+        }
+        const LigLevel& Lp = cfg.levels[prev_level];
+        uint64_t block_prev = 1ull << (Lp.log_msg_cols + Lp.log_inv_rate);
+        if (i == r_steps - 1) {
+            yr.resize(fn);
+            if ((rc = download(F[cur].ptr, yr.data(), fn * 16))) return rc;
+            for (auto& v : yr) ch.observe_f128(v);
+            if ((rc = ch.grind((uint32_t)cfg.levels[i + 1].grinding_bits, &nonce))) return rc;
+            grinding_nonces.push_back(nonce);
+            std::vector<uint64_t> ql = ch.queries(block_prev, cfg.levels[i + 1].queries);
+            if ((rc = open_level(cw_prev.ptr, tree_prev.ptr, block_prev, 1u << Lp.log_num_interleaved, ql,
+                                 &final_open)))
+                return rc;
+            break;
+        }
+        int n_next = ceil_log2(fn);
+        const LigLevel& Ln = cfg.levels[i + 2];
+        uint8_t root_next[32];
+        if ((rc = commit_level(F[cur].ptr, n_next, Ln.log_num_interleaved, Ln.log_inv_rate, cw_next, tree_next,
+                               root_next)))
+            return rc;
+        ch.observe_bytes(root_next, 32);
+        recursive_roots.emplace_back(root_next, root_next + 32);
+        if ((rc = ood_round(i + 2, n_next))) return rc;
+        if ((rc = ch.grind((uint32_t)cfg.levels[i + 1].grinding_bits, &nonce))) return rc;
+        grinding_nonces.push_back(nonce);
+        std::vector<uint64_t> qi = ch.queries(block_prev, cfg.levels[i + 1].queries);
+        std::vector<G> alphai = ch.sample_vec(ceil_log2(cfg.levels[i + 1].queries));
+        LevelOpen op;
+        if ((rc = open_level(cw_prev.ptr, tree_prev.ptr, block_prev, 1u << Lp.log_num_interleaved, qi, &op)))
+            return rc;
+        if ((rc = induce_intro(n_next, op, level_rs, qi, alphai))) return rc;
+        rec_opens.push_back(std::move(op));
+        std::swap(cw_prev, cw_next);
+        std::swap(tree_prev, tree_next);
+        prev_level = i + 2;
+    }
+    // ---------------- LigeritoProof as a hint (write_opening_proof) ----------------
+    Writer w;
+    w.bytes(h->root, 32);
+    write_open(w, open0);
+    w.u64(recursive_roots.size());
+    for (auto& r : recursive_roots) w.bytes(r.data(), 32);
+    w.u64(rec_opens.size());
+    for (auto& o : rec_opens) write_open(w, o);
+    w.u64(yr.size());
+    for (auto& v : yr) w.gf(v);
+    write_open(w, final_open);
+    w.u64(sc_msgs.size() / 2);
+    for (auto& v : sc_msgs) w.gf(v);
+    w.u64(grinding_nonces.size());
+    for (auto v : grinding_nonces) w.u64(v);
+    w.u64(ood_values.size());
+    for (auto& v : ood_values) w.gf(v);
+    w.u64(fold_nonces.size());
+    for (auto v : fold_nonces) w.u64(v);
+    it.hint_bytes(w.b.data(), w.b.size());
+    // BitZOpeningProof::to_bytes (bitz_opener.rs:300-323)
+    Writer out;
+    out.u64(it.narg().size());
+    out.bytes(it.narg().data(), it.narg().size());
+    out.u64(it.hints().size());
+    out.bytes(it.hints().data(), it.hints().size());
+    out.b.push_back(1);
+    out.bytes(yb, 16);
+    if (r0bits) {
+        out.b.push_back(1);
+        out.u64(r0_nonce);
+    } else {
+        out.b.push_back(0);
+    }
+    proof_out->swap(out.b);
+    return BITZ_OK;
+}
+
+int make_prover(bitz_ctx* ctx, const bitz_commit_handle* h, int target_bits, Prover* p) {
+    p->ctx = ctx;
+    p->h = h;
+    p->t = (int)h->t;
+    p->s = (int)h->s;
+    p->m = p->t + p->s;
+    p->mp = p->m - 7;
+    std::string err = custom_johnson_config(p->m, (int)h->log_inv_rate, (int)h->log_batch, target_bits, &p->cfg);
+    BITZ_REQUIRE(err.empty(), BITZ_ERR_SHAPE, "Ligerito configuration: " + err);
+    BITZ_REQUIRE(p->t >= 7 && p->s >= 1, BITZ_ERR_SHAPE, "standalone opening needs t >= 7 and s >= 1");
+    BITZ_REQUIRE(p->cfg.levels.size() >= 2, BITZ_ERR_SHAPE, "Ligerito ladder needs at least two levels");
+    static G gen = smallest_generator();
+    p->g = gen;
+    return BITZ_OK;
+}
+
+}  // namespace
+}  // namespace bitz
+
+extern "C" int bitz_standalone_evaluation(bitz_ctx* ctx, bitz_commit_handle* h, uint8_t claimed_out[16]) {
+    BITZ_REQUIRE(ctx && h && claimed_out, BITZ_ERR_ARG, "bitz_standalone_evaluation: null argument");
+    BITZ_HIP(hipSetDevice(ctx->device));
+    bitz::Prover p;
+    int rc = bitz::make_prover(ctx, h, 100, &p);
+    if (rc) return rc;
+    return p.run(nullptr, claimed_out, nullptr);
+}
+
+extern "C" int bitz_prove_standalone(bitz_ctx* ctx, bitz_commit_handle* h, const uint8_t claimed[16],
+                                     uint8_t* proof, uint64_t capacity, uint64_t* proof_len) {
+    BITZ_REQUIRE(ctx && h && claimed && proof_len, BITZ_ERR_ARG, "bitz_prove_standalone: null argument");
+    BITZ_HIP(hipSetDevice(ctx->device));
+    bitz::Prover p;
+    int rc = bitz::make_prover(ctx, h, 100, &p);
+    if (rc) return rc;
+    std::vector<uint8_t> bytes;
+    rc = p.run(claimed, nullptr, &bytes);
+    if (rc) return rc;
+    *proof_len = bytes.size();
+    BITZ_REQUIRE(proof && capacity >= bytes.size(), BITZ_ERR_ARG, "bitz_prove_standalone: proof buffer too small");
+    memcpy(proof, bytes.data(), bytes.size());
+    return BITZ_OK;
+}
+
+extern "C" int bitz_ligerito_config(uint32_t m, uint32_t log_inv_rate, uint32_t initial_k, uint32_t target_bits,
+                                    int32_t* out, uint32_t capacity) {
+    BITZ_REQUIRE(out, BITZ_ERR_ARG, "bitz_ligerito_config: null out");
+    bitz::LigConfig cfg;
+    std::string err = bitz::custom_johnson_config((int)m, (int)log_inv_rate, (int)initial_k, (int)target_bits, &cfg);
+    BITZ_REQUIRE(err.empty(), BITZ_ERR_SHAPE, "Ligerito configuration: " + err);
+    std::vector<int32_t> v = {cfg.m, cfg.log_n, cfg.initial_k, cfg.yr_log_n, cfg.round0_grinding_bits,
+                              (int32_t)cfg.levels.size()};
+    for (auto& lv : cfg.levels)
+        for (int x : {lv.log_inv_rate, lv.log_msg_cols, lv.log_num_interleaved, lv.k_recursive, lv.queries,
+                      lv.grinding_bits, lv.fold_grinding_bits, lv.ood_samples})
+            v.push_back(x);
+    BITZ_REQUIRE(capacity >= v.size(), BITZ_ERR_ARG, "bitz_ligerito_config: capacity too small");
+    memcpy(out, v.data(), v.size() * 4);
+    return (int)v.size();
+}
+
+extern "C" void bitz_host_blake3(const uint8_t* data, uint64_t len, uint8_t out[32]) {
+    bitz::blake3_hash_host(data, len, out);
+}

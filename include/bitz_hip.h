@@ -160,6 +160,29 @@ const void* bitz_commit_tree(const bitz_commit_handle* h);
 /* copy one codeword position (2^log_batch Gf128) and the tree to the host */
 int bitz_commit_read_tree(bitz_commit_handle* h, uint8_t* tree_out, uint64_t bytes);
 
+/* ---- the BitZ standalone opening ----
+ * prove_standalone (src/piop/spartan/protocol/bitz_opener.rs:672-707): the
+ * outer Blake3Transcript binds the statement (root, Ligerito ladder, shape,
+ * generator, prime width), runs Round 0, samples the prime q and the point,
+ * then BitZProver::prove (src/bitz/mod.rs:142-199) opens
+ * <eq(r1) (x) eq(r2), bits> = claimed mod q through the fold round, the
+ * batched grand-product GKR, the inner-product sumcheck, the ring switch and
+ * the recursive Ligerito opening (custom Johnson ladder at the commitment's
+ * rate and initial_k, 100-bit target). Proof bytes follow
+ * BitZOpeningProof::to_bytes (bitz_opener.rs:300-323). Synchronizing. */
+/* the honest claimed value (standalone_evaluation, bitz_opener.rs:643-670) */
+int bitz_standalone_evaluation(bitz_ctx* ctx, bitz_commit_handle* h, uint8_t claimed_out[16]);
+/* the proof of `claimed`; *proof_len receives the size (also when capacity is too small) */
+int bitz_prove_standalone(bitz_ctx* ctx, bitz_commit_handle* h, const uint8_t claimed[16], uint8_t* proof,
+                          uint64_t capacity, uint64_t* proof_len);
+/* custom Johnson ladder (src/ligerito_flock.rs:535-665): writes [m, log_n, initial_k, yr_log_n,
+ * round0_grinding, levels, then per level: log_inv_rate, log_msg_cols, log_num_interleaved,
+ * k_recursive, queries, grinding_bits, fold_grinding_bits, ood_samples]; returns the count */
+int bitz_ligerito_config(uint32_t m, uint32_t log_inv_rate, uint32_t initial_k, uint32_t target_bits,
+                         int32_t* out, uint32_t capacity);
+/* BLAKE3 of a host buffer on the host (the transcript hash) */
+void bitz_host_blake3(const uint8_t* data, uint64_t len, uint8_t out[32]);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,171 @@
+"""BitZ PCS commit+open throughput on one MI355X.
+
+A step is one pass of the hot path over one batch of synthetic input: the
+BitZ PCS commit of 2^24 uniform u64 coefficients (n = 30 committed bits,
+t = 17 row variables, s = 13 column variables, rate 1/4 Reed-Solomon code,
+16 interleaved lanes, BLAKE3 Merkle tree) followed by the standalone opening
+(statement, Round 0, fold round, batched grand-product GKR, inner-product
+sumcheck, ring switch, recursive Ligerito opening) at a transcript-derived
+point. The coefficients are already resident in HBM when the timed region
+starts; the claimed value (the reference's "claim setup",
+standalone_evaluation) is computed before it.
+
+Prints one JSON line. The roofline object describes the dominant kernel
+(most time in the timed region), from HIP-event timings of its launches on
+the library's stream; cpu_baseline times the CPU restatement of the same
+protocol on a bounded sample (n = 24) on this host.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "bitz-poc_amd"))
+
+HBM_PEAK_GBS = 8000.0
+# VALU 32-bit integer peak: 256 CUs x 4 SIMDs x 16 lanes x 2.4 GHz
+VALU_PEAK_TOPS = 256 * 64 * 2.4e9 / 1e12
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--log-coeffs", type=int, default=24, help="log2 of the u64 coefficient count")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-log-coeffs", type=int, default=18)
+    return ap.parse_args()
+
+
+def shape_for(log_coeffs: int):
+    n = log_coeffs + 6  # 64 bits per coefficient
+    t = max(7, -(-3 * n // 5) - 1)  # reference_log_rows (src/bitz/params.rs:30-35)
+    t = min(t, n - 1)
+    return n, t, n - t
+
+
+def cpu_baseline(log_coeffs: int):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import numpy as np
+
+    import bitz_protocol as P
+    n, t, s = shape_for(log_coeffs)
+    rng = np.random.default_rng(7)
+    rows = rng.integers(0, 2**64, size=(1 << s) << (t - 6), dtype=np.uint64)
+    t0 = time.perf_counter()
+    pr = P.OracleProver(rows, t, s, 2, 4)
+    t1 = time.perf_counter()
+    claimed = pr.evaluation()
+    t2 = time.perf_counter()
+    pr.prove(claimed)
+    t3 = time.perf_counter()
+    secs = (t1 - t0) + (t3 - t2)
+    return {
+        "value": round((1 << log_coeffs) / secs / 1e6, 4),
+        "unit": "Mcoeff/s",
+        "cores": 1,
+        "kind": "port",
+        "sample": f"n={n} (2^{log_coeffs} u64 coefficients, t={t}, s={s}), rate 1/4, initial_k 4: commit + "
+                  f"standalone open by the CPU restatement (oracle/, C kernels driven from Python, "
+                  f"1 thread, PCLMULQDQ); {secs:.2f} s",
+    }
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (got --gpus {args.gpus}): the commit+open path is "
+              f"measured on a single MI355X; see DESIGN.md for how it would shard.", file=sys.stderr)
+        sys.exit(2)
+    import torch
+
+    from bitz_amd.device import Commitment, Context, kernel_stats, set_timing
+
+    n, t, s = shape_for(args.log_coeffs)
+    ctx = Context(0)
+    g = torch.Generator(device="cuda").manual_seed(1234)
+    words = 1 << args.log_coeffs
+    coeffs = torch.randint(-(2**62), 2**62, (words,), dtype=torch.int64, device="cuda", generator=g)
+    coeffs ^= torch.randint(0, 4, (words,), dtype=torch.int64, device="cuda", generator=g) << 62
+
+    def step(claimed=None):
+        com = Commitment(ctx, coeffs, t, s, 4, 2)
+        if claimed is None:
+            return com, com.evaluation()
+        proof = com.prove(claimed)
+        com.close()
+        return proof
+
+    # claim setup (untimed): the honest value of the transcript-derived claim
+    com0, claimed = step()
+    com0.close()
+    for _ in range(args.warmup):
+        step(claimed)
+    torch.cuda.synchronize()
+    set_timing(ctx, True)
+    t0 = time.perf_counter()
+    proof = None
+    for _ in range(args.steps):
+        proof = step(claimed)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    stats = kernel_stats(ctx)
+    set_timing(ctx, False)
+    ms = (t1 - t0) * 1e3 / args.steps
+    value = (1 << args.log_coeffs) / (ms / 1e3) / 1e6
+
+    dom = max(stats.items(), key=lambda kv: kv[1][1]) if stats else None
+    roofline = None
+    if dom:
+        name, (launches, tot_ms, byts, ops) = dom
+        avg_s = tot_ms / launches / 1e3
+        gbs = byts / launches / avg_s / 1e9
+        tops = ops / launches / avg_s / 1e12
+        roofline = {
+            "kernel": name,
+            "bound": "hbm",
+            "achieved": round(gbs, 2),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(gbs / HBM_PEAK_GBS, 5),
+            "traffic": None,
+            "avg_launch_us": round(avg_s * 1e6, 2),
+            "launches_per_step": launches / args.steps,
+            "valu": {"achieved": round(tops, 3), "peak": round(VALU_PEAK_TOPS, 2), "unit": "T int32-op/s",
+                     "frac": round(tops / VALU_PEAK_TOPS, 4)},
+            "time_share": round(tot_ms / (ms * args.steps), 4),
+        }
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline(args.cpu_log_coeffs)
+    out = {
+        "metric": "PCS commit+open Mcoeff/s at 2^24",
+        "value": round(value, 3),
+        "unit": "Mcoeff/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "gf2^128 (u32 integer ops)",
+        "data": "synthetic: uniform u64 coefficients from a seeded generator, resident in HBM",
+        "config": {"workload": f"BitZ PCS commit+open, 2^{args.log_coeffs} u64 coefficients", "n": n, "t": t,
+                   "s": s, "rate": "1/4", "initial_k": 4, "merkle_hash": "blake3", "ladder": "custom:2:4",
+                   "proof_bytes": len(proof) if proof else None},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "kernels": {k: {"launches": int(v[0]), "ms": round(v[1] / args.steps, 3)} for k, v in
+                    sorted(stats.items(), key=lambda kv: -kv[1][1])},
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

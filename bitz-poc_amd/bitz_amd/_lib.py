@@ -36,6 +36,8 @@ SIGNATURES = {
     "bitz_ctx_synchronize": (_I, [_P]),
     "bitz_last_error": (C.c_char_p, []),
     "bitz_version": (C.c_char_p, []),
+    "bitz_ctx_set_timing": (_I, [_P, _I]),
+    "bitz_ctx_kernel_stats": (_I, [_P, _P, _U64, _P, _U64]),
     "bitz_gf_mul_dev": (_I, [_P, _P, _P, _P, _U64]),
     "bitz_gf_mul_const_dev": (_I, [_P, _P, _P, _P, _U64]),
     "bitz_blake3_leaves_dev": (_I, [_P, _P, _U64, _U64, _P]),
@@ -64,6 +66,7 @@ SIGNATURES = {
     "bitz_induce_basis_dev": (_I, [_P, _P, _U32, _I, _P, _P]),
     "bitz_pow_find_dev": (_I, [_P, _P, _P, _U32, C.POINTER(_U64)]),
     "bitz_commit": (_I, [_P, _P, _U32, _U32, _U32, _U32, _P, C.POINTER(_P)]),
+    "bitz_commit_device_rows": (_I, [_P, _P, _U32, _U32, _U32, _U32, _P, C.POINTER(_P)]),
     "bitz_commit_free": (None, [_P]),
     "bitz_commit_packed": (_P, [_P]),
     "bitz_commit_codeword": (_P, [_P]),

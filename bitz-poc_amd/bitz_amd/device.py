@@ -149,6 +149,21 @@ def _ints(buf, k: int):
     return [int.from_bytes(raw[16 * i:16 * i + 16], "little") for i in range(k)]
 
 
+def kernel_stats(ctx: Context) -> dict:
+    """{kernel: (launches, total_ms, algorithmic_bytes, algorithmic_int_ops)}"""
+    names = C.create_string_buffer(1 << 16)
+    vals = (C.c_double * 4096)()
+    n = _lib.load().bitz_ctx_kernel_stats(ctx.ptr, names, 1 << 16, C.cast(vals, C.c_void_p), 4096)
+    if n < 0:
+        _lib.check(n, "bitz_ctx_kernel_stats")
+    keys = names.value.decode().split("\n")[:n]
+    return {k: tuple(vals[4 * i:4 * i + 4]) for i, k in enumerate(keys)}
+
+
+def set_timing(ctx: Context, on: bool) -> None:
+    _lib.call("bitz_ctx_set_timing", ctx.ptr, 1 if on else 0)
+
+
 class OpenOps:
     """Thin wrappers over the opening kernels (device tensors in, device
     tensors or host ints out)."""
@@ -278,16 +293,24 @@ class Commitment:
     Pcs::commit (src/bitz/pcs.rs:270-284)."""
 
     def __init__(self, ctx: Context, rows, t: int, s: int, log_batch: int, log_inv_rate: int):
+        """rows: a host array (numpy u64) or a cuda tensor of 2^s * 2^(t-6) words."""
         import numpy as np
-        rows = np.ascontiguousarray(rows, dtype=np.uint64).reshape(-1)
-        if rows.size != (1 << s) << (t - 6):
-            raise ValueError("rows must hold 2^s columns of 2^(t-6) words")
         self.ctx, self.t, self.s = ctx, t, s
         self.log_batch, self.log_inv_rate = log_batch, log_inv_rate
+        words = (1 << s) << (t - 6)
         root = (C.c_uint8 * 32)()
         h = C.c_void_p()
-        _lib.call("bitz_commit", ctx.ptr, rows.ctypes.data_as(C.c_void_p), t, s, log_batch, log_inv_rate,
-                  _vp(root), C.byref(h))
+        if hasattr(rows, "is_cuda") and rows.is_cuda:
+            if rows.numel() * rows.element_size() != words * 8 or not rows.is_contiguous():
+                raise ValueError("rows must hold 2^s columns of 2^(t-6) words")
+            _lib.call("bitz_commit_device_rows", ctx.ptr, _ptr(rows), t, s, log_batch, log_inv_rate, _vp(root),
+                      C.byref(h))
+        else:
+            rows = np.ascontiguousarray(rows, dtype=np.uint64).reshape(-1)
+            if rows.size != words:
+                raise ValueError("rows must hold 2^s columns of 2^(t-6) words")
+            _lib.call("bitz_commit", ctx.ptr, rows.ctypes.data_as(C.c_void_p), t, s, log_batch, log_inv_rate,
+                      _vp(root), C.byref(h))
         self._h = h
         self.root = bytes(root)
 

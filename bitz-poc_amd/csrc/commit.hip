@@ -355,6 +355,12 @@ int ensure_twiddles(bitz_ctx* ctx, int ell, const uint4** out) {
     return BITZ_OK;
 }
 
+// cost model of the algorithmic work: one GF(2^128) multiplication = 3
+// 64-bit carry-less products + reduction, one BLAKE3 compression = 7 rounds
+// x 8 G x 14 ops + 16, both ~800 32-bit integer operations
+constexpr double kGfMulOps = 800.0;
+constexpr double kBlake3Ops = 800.0;
+
 static int tile_log(int log_d, int log_lanes) { return std::min(std::min(log_d, 11 - log_lanes), 9); }
 
 static size_t local_lds_bytes(int T, int log_lanes) {
@@ -377,6 +383,8 @@ static int run_ntt(bitz_ctx* ctx, uint4* data, const uint4* src, uint64_t src_ma
         int F = std::min(4, log_d - T - l);
         uint64_t threads = 1ull << (log_d - F + log_lanes);
         dim3 grid((unsigned)((threads + kThreads - 1) / kThreads));
+        double elems = (double)(1ull << (log_d + log_lanes));
+        TimedLaunch tl(ctx, "ntt_pass", 2.0 * 16.0 * elems, F * (elems / 2) * kGfMulOps);
         switch (F) {
             case 4:
                 hipLaunchKernelGGL(k_ntt_pass<4>, grid, dim3(kThreads), 0, ctx->stream, data, src,
@@ -401,9 +409,17 @@ static int run_ntt(bitz_ctx* ctx, uint4* data, const uint4* src, uint64_t src_ma
     }
     size_t lds = local_lds_bytes(T, log_lanes);
     unsigned tiles = (unsigned)(1ull << (log_d - T));
-    hipLaunchKernelGGL(k_ntt_local, dim3(tiles), dim3(kThreads), lds, ctx->stream, data, src,
-                       src_mask, log_d, log_lanes, T, local_first, tw, tree);
-    BITZ_LAUNCH_CHECK();
+    {
+        double elems = (double)(1ull << (log_d + log_lanes));
+        double leaves = (double)(1ull << log_d);
+        double hashes = tree ? leaves * (((16u << log_lanes) + 63) / 64) + leaves : 0;
+        TimedLaunch tl(ctx, tree ? "ntt_local_merkle" : "ntt_local",
+                       2.0 * 16.0 * elems + (tree ? 64.0 * leaves : 0.0),
+                       (log_d - local_first) * (elems / 2) * kGfMulOps + hashes * kBlake3Ops);
+        hipLaunchKernelGGL(k_ntt_local, dim3(tiles), dim3(kThreads), lds, ctx->stream, data, src,
+                           src_mask, log_d, log_lanes, T, local_first, tw, tree);
+        BITZ_LAUNCH_CHECK();
+    }
     if (tree) {
         // levels above the tile subtrees
         uint64_t n = 1ull << log_d;
@@ -508,9 +524,23 @@ extern "C" int bitz_commit_dev(bitz_ctx* ctx, const void* packed, int log_msg_le
 
 // ---- host-buffer commit with a device-resident handle ----
 
+static int commit_impl(bitz_ctx* ctx, const uint64_t* rows, bool rows_on_device, uint32_t t, uint32_t s,
+                       uint32_t log_batch, uint32_t log_inv_rate, uint8_t root[32], bitz_commit_handle** out);
+
 extern "C" int bitz_commit(bitz_ctx* ctx, const uint64_t* rows, uint32_t t, uint32_t s,
                            uint32_t log_batch, uint32_t log_inv_rate, uint8_t root[32],
                            bitz_commit_handle** out) {
+    return commit_impl(ctx, rows, false, t, s, log_batch, log_inv_rate, root, out);
+}
+
+extern "C" int bitz_commit_device_rows(bitz_ctx* ctx, const void* rows, uint32_t t, uint32_t s,
+                                       uint32_t log_batch, uint32_t log_inv_rate, uint8_t root[32],
+                                       bitz_commit_handle** out) {
+    return commit_impl(ctx, (const uint64_t*)rows, true, t, s, log_batch, log_inv_rate, root, out);
+}
+
+static int commit_impl(bitz_ctx* ctx, const uint64_t* rows, bool rows_on_device, uint32_t t, uint32_t s,
+                       uint32_t log_batch, uint32_t log_inv_rate, uint8_t root[32], bitz_commit_handle** out) {
     BITZ_REQUIRE(ctx && rows && root && out, BITZ_ERR_ARG, "bitz_commit: null argument");
     // Shape::new (src/bitz/params.rs:53-66) and the packing width
     BITZ_REQUIRE(t >= 7 && t <= 35 && s <= 35 - t, BITZ_ERR_SHAPE,
@@ -530,7 +560,8 @@ extern "C" int bitz_commit(bitz_ctx* ctx, const uint64_t* rows, uint32_t t, uint
                      h->tree.alloc((2 * h->n_leaves - 1) * 32),
                  BITZ_ERR_ALLOC, "bitz_commit: device allocation failed");
     BITZ_HIP(hipSetDevice(ctx->device));
-    BITZ_HIP(hipMemcpyAsync(h->packed.ptr, rows, msg_bytes, hipMemcpyHostToDevice, ctx->stream));
+    BITZ_HIP(hipMemcpyAsync(h->packed.ptr, rows, msg_bytes,
+                            rows_on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, ctx->stream));
     int rc = bitz_commit_dev(ctx, h->packed.ptr, log_msg, (int)log_batch, (int)log_inv_rate,
                              h->codeword.ptr, h->tree.ptr, root);
     if (rc) return rc;

@@ -69,9 +69,27 @@ struct Scratch {
 };
 }  // namespace bitz
 
+namespace bitz {
+// Optional per-kernel timing: HIP events around instrumented launches, with
+// the launch's algorithmic work (bytes and integer ops) attached.
+struct KernelStat {
+    uint64_t launches = 0;
+    double ms = 0, bytes = 0, ops = 0;
+};
+struct PendingTiming {
+    std::string name;
+    hipEvent_t start, stop;
+    double bytes, ops;
+};
+}  // namespace bitz
+
 struct bitz_ctx {
     int device = 0;
     bitz::Scratch scratch;
+    bool timing = false;
+    std::vector<bitz::PendingTiming> pending;
+    std::vector<hipEvent_t> event_pool;
+    std::map<std::string, bitz::KernelStat> stats;
     hipStream_t own_stream = nullptr;
     hipStream_t stream = nullptr;
     // device twiddle tables per NTT domain dimension ell:
@@ -88,6 +106,17 @@ struct bitz_commit_handle {
     uint64_t n_leaves;
     uint8_t root[32];
 };
+
+namespace bitz {
+// RAII bracket: records start/stop events on ctx->stream when timing is on.
+struct TimedLaunch {
+    bitz_ctx* ctx;
+    PendingTiming p;
+    bool on;
+    TimedLaunch(bitz_ctx* c, const char* name, double bytes, double ops);
+    ~TimedLaunch();
+};
+}  // namespace bitz
 
 #define BITZ_HIP(expr)                                                                   \
     do {                                                                                 \

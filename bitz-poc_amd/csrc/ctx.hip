@@ -10,6 +10,79 @@ void set_error(const std::string& msg) { g_err = msg; }
 
 extern "C" const char* bitz_last_error(void) { return bitz::g_err.c_str(); }
 
+namespace bitz {
+static hipEvent_t take_event(bitz_ctx* ctx) {
+    if (!ctx->event_pool.empty()) {
+        hipEvent_t e = ctx->event_pool.back();
+        ctx->event_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    (void)hipEventCreate(&e);
+    return e;
+}
+
+TimedLaunch::TimedLaunch(bitz_ctx* c, const char* name, double bytes, double ops) : ctx(c), on(c->timing) {
+    if (!on) return;
+    p.name = name;
+    p.bytes = bytes;
+    p.ops = ops;
+    p.start = take_event(ctx);
+    p.stop = take_event(ctx);
+    (void)hipEventRecord(p.start, ctx->stream);
+}
+
+TimedLaunch::~TimedLaunch() {
+    if (!on) return;
+    (void)hipEventRecord(p.stop, ctx->stream);
+    ctx->pending.push_back(p);
+}
+
+static void resolve_timings(bitz_ctx* ctx) {
+    for (auto& p : ctx->pending) {
+        (void)hipEventSynchronize(p.stop);
+        float ms = 0;
+        (void)hipEventElapsedTime(&ms, p.start, p.stop);
+        auto& st = ctx->stats[p.name];
+        st.launches++;
+        st.ms += ms;
+        st.bytes += p.bytes;
+        st.ops += p.ops;
+        ctx->event_pool.push_back(p.start);
+        ctx->event_pool.push_back(p.stop);
+    }
+    ctx->pending.clear();
+}
+}  // namespace bitz
+
+extern "C" int bitz_ctx_set_timing(bitz_ctx* ctx, int enable) {
+    BITZ_REQUIRE(ctx != nullptr, BITZ_ERR_ARG, "bitz_ctx_set_timing: null ctx");
+    bitz::resolve_timings(ctx);
+    ctx->timing = enable != 0;
+    if (enable) ctx->stats.clear();
+    return BITZ_OK;
+}
+
+extern "C" int bitz_ctx_kernel_stats(bitz_ctx* ctx, char* names, uint64_t names_cap, double* values,
+                                     uint64_t values_cap) {
+    BITZ_REQUIRE(ctx && names && values, BITZ_ERR_ARG, "bitz_ctx_kernel_stats: null argument");
+    bitz::resolve_timings(ctx);
+    std::string all;
+    uint64_t k = 0;
+    for (auto& kv : ctx->stats) {
+        if ((k + 1) * 4 > values_cap) break;
+        all += kv.first + "\n";
+        values[4 * k] = (double)kv.second.launches;
+        values[4 * k + 1] = kv.second.ms;
+        values[4 * k + 2] = kv.second.bytes;
+        values[4 * k + 3] = kv.second.ops;
+        k++;
+    }
+    BITZ_REQUIRE(all.size() + 1 <= names_cap, BITZ_ERR_ARG, "bitz_ctx_kernel_stats: names buffer too small");
+    memcpy(names, all.c_str(), all.size() + 1);
+    return (int)k;
+}
+
 extern "C" const char* bitz_version(void) { return "bitz_hip gfx950 0.1"; }
 
 extern "C" int bitz_ctx_create(int device, bitz_ctx** out) {
@@ -37,12 +110,14 @@ extern "C" void bitz_ctx_destroy(bitz_ctx* ctx) {
         (void)hipStreamSynchronize(ctx->own_stream);
         (void)hipStreamDestroy(ctx->own_stream);
     }
+    bitz::resolve_timings(ctx);
+    for (auto e : ctx->event_pool) (void)hipEventDestroy(e);
     delete ctx;
 }
 
 extern "C" int bitz_ctx_set_stream(bitz_ctx* ctx, void* stream) {
     BITZ_REQUIRE(ctx != nullptr, BITZ_ERR_ARG, "bitz_ctx_set_stream: null ctx");
-    ctx->stream = stream ? (hipStream_t)stream : ctx->own_stream;
+    ctx->stream = (hipStream_t)stream;
     return BITZ_OK;
 }
 

@@ -682,11 +682,14 @@ int pack_columns(bitz_ctx* ctx, const uint64_t* rows, int t, int s, uint64_t* pa
     return BITZ_OK;
 }
 
+constexpr double kGfMulOps = 800.0;
+
 int gkr_round_sums(bitz_ctx* ctx, const void* l, const void* r, uint64_t h, const void* eq_c, int s_c,
                    const void* eq_y, int send_one, void* out_dev) {
     unsigned nb = blocks_for(h, kRun * 4, 2048);
     int rc = ctx->scratch.ensure((size_t)nb * 2 * 16);
     if (rc) return rc;
+    TimedLaunch tl(ctx, "gkr_round_sums", 4.0 * 16.0 * h, 4.0 * h * kGfMulOps);
     DenseSrc src{(const uint4*)l, (const uint4*)r};
     hipLaunchKernelGGL(k_gkr_round_sums<DenseSrc>, dim3(nb), dim3(kT), 0, ctx->stream, src, h,
                        (const uint4*)eq_c, s_c, (const uint4*)eq_y, send_one, (uint4*)ctx->scratch.partials.ptr);
@@ -700,6 +703,7 @@ int gkr_leaf_round_sums(bitz_ctx* ctx, const uint64_t* packed, const void* A, in
     unsigned nb = blocks_for(h, kRun * 4, 2048);
     int rc = ctx->scratch.ensure((size_t)nb * 2 * 16);
     if (rc) return rc;
+    TimedLaunch tl(ctx, "gkr_leaf_round_sums", (double)(1ull << (t + s - 3)), 4.0 * h * kGfMulOps);
     LeafSrc src{packed, (const uint4*)A, t, s};
     hipLaunchKernelGGL(k_gkr_round_sums<LeafSrc>, dim3(nb), dim3(kT), 0, ctx->stream, src, h,
                        (const uint4*)eq_c, s_c, (const uint4*)eq_y, send_one, (uint4*)ctx->scratch.partials.ptr);
@@ -715,6 +719,7 @@ static void split16(const uint8_t* p, uint64_t* lo, uint64_t* hi) {
 int fold_halves(bitz_ctx* ctx, void* v, uint64_t h, const uint8_t rho[16]) {
     uint64_t lo, hi;
     split16(rho, &lo, &hi);
+    TimedLaunch tl(ctx, "fold_halves", 3.0 * 16.0 * h, h * kGfMulOps);
     hipLaunchKernelGGL(k_fold_halves, dim3(blocks_for(h, 4)), dim3(kT), 0, ctx->stream, (uint4*)v, h, lo, hi);
     BITZ_LAUNCH_CHECK();
     return BITZ_OK;
@@ -725,6 +730,8 @@ int gkr_leaf_fold(bitz_ctx* ctx, const uint64_t* packed, const void* A, int t, i
     uint64_t lo, hi;
     split16(rho, &lo, &hi);
     LeafSrc src{packed, (const uint4*)A, t, s};
+    TimedLaunch tl(ctx, "gkr_leaf_fold", 2.0 * 16.0 * (1ull << (t + s - 2)) + (1ull << (t + s - 3)),
+                   2.0 * (1ull << (t + s - 2)) * kGfMulOps);
     hipLaunchKernelGGL(k_gkr_leaf_fold, dim3(blocks_for(1ull << (t + s - 2), 4)), dim3(kT), 0, ctx->stream, src, lo,
                        hi, (uint4*)l_out, (uint4*)r_out);
     BITZ_LAUNCH_CHECK();
@@ -740,6 +747,7 @@ int gkr_level1(bitz_ctx* ctx, const uint64_t* packed, const void* A, int t, int 
 }
 
 int product_level(bitz_ctx* ctx, const void* in, uint64_t n, void* out) {
+    TimedLaunch tl(ctx, "product_level", 1.5 * 16.0 * n, (n / 2) * kGfMulOps);
     hipLaunchKernelGGL(k_product_level, dim3(blocks_for(n / 2, 4)), dim3(kT), 0, ctx->stream, (const uint4*)in,
                        n / 2, (uint4*)out);
     BITZ_LAUNCH_CHECK();
@@ -826,6 +834,7 @@ int lig_msg(bitz_ctx* ctx, const void* f, const void* b, uint64_t n, int with_ev
     int K = with_eval ? 3 : 2;
     int rc = ctx->scratch.ensure((size_t)nb * K * 16);
     if (rc) return rc;
+    TimedLaunch tl(ctx, "lig_msg", 2.0 * 16.0 * n, (with_eval ? 3.0 : 2.0) * pairs * kGfMulOps);
     if (with_eval)
         hipLaunchKernelGGL(k_lig_msg<3>, dim3(nb), dim3(kT), 0, ctx->stream, (const uint4*)f, (const uint4*)b, pairs,
                            (uint4*)ctx->scratch.partials.ptr);
@@ -840,6 +849,7 @@ int lig_fold(bitz_ctx* ctx, const void* f, const void* b, uint64_t n, const uint
     BITZ_REQUIRE(f != fo && b != bo, BITZ_ERR_ARG, "lig_fold: outputs must not alias inputs");
     uint64_t lo, hi;
     split16(r, &lo, &hi);
+    TimedLaunch tl(ctx, "lig_fold", 3.0 * 16.0 * n, n * kGfMulOps);
     hipLaunchKernelGGL(k_lig_fold, dim3(blocks_for(n / 2, 4)), dim3(kT), 0, ctx->stream, (const uint4*)f,
                        (const uint4*)b, n / 2, lo, hi, (uint4*)fo, (uint4*)bo);
     BITZ_LAUNCH_CHECK();

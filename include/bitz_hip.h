@@ -43,12 +43,20 @@ typedef struct bitz_commit_handle bitz_commit_handle;
 /* ---- context ---- */
 int bitz_ctx_create(int device, bitz_ctx** out);
 void bitz_ctx_destroy(bitz_ctx* ctx);
-/* Use an existing HIP stream (e.g. torch.cuda.current_stream().cuda_stream);
- * NULL restores the context's own stream. */
+/* Enqueue all further work on `hip_stream` (e.g. torch.cuda.current_stream().cuda_stream);
+ * NULL selects the device's default (null) stream, which torch uses unless told otherwise.
+ * A new context starts on a stream of its own. */
 int bitz_ctx_set_stream(bitz_ctx* ctx, void* hip_stream);
 void* bitz_ctx_stream(bitz_ctx* ctx);
 int bitz_ctx_synchronize(bitz_ctx* ctx);
 const char* bitz_last_error(void);
+/* Per-kernel timing: with enable != 0, instrumented launches are bracketed by
+ * HIP events on the context's stream (stats reset). bitz_ctx_kernel_stats
+ * writes newline-separated kernel names and, per kernel, 4 doubles:
+ * launches, total ms, algorithmic bytes, algorithmic 32-bit integer ops;
+ * returns the number of kernels. */
+int bitz_ctx_set_timing(bitz_ctx* ctx, int enable);
+int bitz_ctx_kernel_stats(bitz_ctx* ctx, char* names, uint64_t names_cap, double* values, uint64_t values_cap);
 /* Library build tag (architecture and version). */
 const char* bitz_version(void);
 
@@ -152,6 +160,9 @@ int bitz_pow_find_dev(bitz_ctx* ctx, const uint8_t prefix[16], const uint8_t see
  * and tree on the device inside the handle. Synchronizing. */
 int bitz_commit(bitz_ctx* ctx, const uint64_t* rows, uint32_t t, uint32_t s, uint32_t log_batch,
                 uint32_t log_inv_rate, uint8_t root[32], bitz_commit_handle** out);
+/* the same commitment from rows already in device memory (copied into the handle) */
+int bitz_commit_device_rows(bitz_ctx* ctx, const void* rows, uint32_t t, uint32_t s, uint32_t log_batch,
+                            uint32_t log_inv_rate, uint8_t root[32], bitz_commit_handle** out);
 void bitz_commit_free(bitz_commit_handle* h);
 /* device pointers held by the handle (valid until bitz_commit_free) */
 const void* bitz_commit_packed(const bitz_commit_handle* h);

@@ -119,6 +119,8 @@ def main():
     ms = (t1 - t0) * 1e3 / args.steps
     value = (1 << args.log_coeffs) / (ms / 1e3) / 1e6
 
+    phases = {k[6:]: round(v[1] / args.steps, 3) for k, v in stats.items() if k.startswith("phase:")}
+    stats = {k: v for k, v in stats.items() if not k.startswith("phase:")}
     dom = max(stats.items(), key=lambda kv: kv[1][1]) if stats else None
     roofline = None
     if dom:
@@ -163,6 +165,7 @@ def main():
         "cpu_baseline": cpu,
         "kernels": {k: {"launches": int(v[0]), "ms": round(v[1] / args.steps, 3)} for k, v in
                     sorted(stats.items(), key=lambda kv: -kv[1][1])},
+        "host_phases_ms": phases,
     }
     print(json.dumps(out), flush=True)
 

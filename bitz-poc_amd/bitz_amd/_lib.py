@@ -34,6 +34,8 @@ SIGNATURES = {
     "bitz_ctx_set_stream": (_I, [_P, _P]),
     "bitz_ctx_stream": (_P, [_P]),
     "bitz_ctx_synchronize": (_I, [_P]),
+    "bitz_ctx_trim": (_I, [_P]),
+    "bitz_ctx_cached_bytes": (_U64, [_P]),
     "bitz_last_error": (C.c_char_p, []),
     "bitz_version": (C.c_char_p, []),
     "bitz_ctx_set_timing": (_I, [_P, _I]),
@@ -51,6 +53,12 @@ SIGNATURES = {
     "bitz_gkr_round_sums_dev": (_I, [_P, _P, _P, _U64, _P, _I, _P, _I, _P]),
     "bitz_gkr_leaf_round_sums_dev": (_I, [_P, _P, _P, _I, _I, _P, _I, _P, _I, _P]),
     "bitz_fold_halves_dev": (_I, [_P, _P, _U64, _P]),
+    "bitz_gkr_fold_round_sums_dev": (_I, [_P, _P, _P, _U64, _P, _P, _I, _P, _I, _P]),
+    "bitz_gkr_leaf_round1_sums_dev": (_I, [_P, _P, _P, _I, _I, _P, _P, _P, _I, _P]),
+    "bitz_gkr_leaf_fold2_dev": (_I, [_P, _P, _P, _I, _I, _P, _P, _P, _P, _P]),
+    "bitz_gkr_premul_round_sums_dev": (_I, [_P, _P, _P, _U64, _P, _I, _P, _I, _P]),
+    "bitz_gkr_round_sums_rowonly_dev": (_I, [_P, _P, _P, _U64, _I, _P, _I, _P]),
+    "bitz_tensor_table_dev": (_I, [_P, _P, _P, _I, _P]),
     "bitz_gkr_leaf_fold_dev": (_I, [_P, _P, _P, _I, _I, _P, _P, _P]),
     "bitz_gkr_level1_dev": (_I, [_P, _P, _P, _I, _I, _P]),
     "bitz_product_level_dev": (_I, [_P, _P, _U64, _P]),

@@ -208,6 +208,44 @@ class OpenOps:
                   _ptr(eq_y), 1 if send_one else 0, _vp(buf))
         return _ints(buf, 2)
 
+    def gkr_fold_round_sums_(self, l, r, h, rho, eq_c, s_c, eq_y, send_one):
+        """folds l and r (4h entries) in place to 2h and returns the round sums"""
+        buf = (C.c_uint8 * 32)()
+        _lib.call("bitz_gkr_fold_round_sums_dev", self.ctx.ptr, _ptr(l), _ptr(r), h, _vp(_b16(rho)), _ptr(eq_c),
+                  s_c, _ptr(eq_y), 1 if send_one else 0, _vp(buf))
+        return _ints(buf, 2)
+
+    def gkr_leaf_round1_sums(self, packed, A, t, s, rho0, eq_c, eq_y, send_one):
+        buf = (C.c_uint8 * 32)()
+        _lib.call("bitz_gkr_leaf_round1_sums_dev", self.ctx.ptr, _ptr(packed), _ptr(A), t, s, _vp(_b16(rho0)),
+                  _ptr(eq_c), _ptr(eq_y), 1 if send_one else 0, _vp(buf))
+        return _ints(buf, 2)
+
+    def gkr_leaf_fold2(self, packed, A, t, s, rho0, rho1, eq_c=None):
+        n = 1 << (t + s - 3)
+        l, r = self._gf(n), self._gf(n)
+        _lib.call("bitz_gkr_leaf_fold2_dev", self.ctx.ptr, _ptr(packed), _ptr(A), t, s, _vp(_b16(rho0)),
+                  _vp(_b16(rho1)), _ptr(eq_c) if eq_c is not None else None, _ptr(l), _ptr(r))
+        return l, r
+
+    def gkr_premul_round_sums_(self, l, r, h, eq_c, s, eq_y, send_one):
+        """multiplies eq_c into r (2h entries) in place; returns the round sums"""
+        buf = (C.c_uint8 * 32)()
+        _lib.call("bitz_gkr_premul_round_sums_dev", self.ctx.ptr, _ptr(l), _ptr(r), h, _ptr(eq_c), s, _ptr(eq_y),
+                  1 if send_one else 0, _vp(buf))
+        return _ints(buf, 2)
+
+    def gkr_round_sums_rowonly(self, l, r, h, s, eq_y, send_one):
+        buf = (C.c_uint8 * 32)()
+        _lib.call("bitz_gkr_round_sums_rowonly_dev", self.ctx.ptr, _ptr(l), _ptr(r), h, s, _ptr(eq_y),
+                  1 if send_one else 0, _vp(buf))
+        return _ints(buf, 2)
+
+    def tensor_table(self, a, b, n):
+        out = self._gf(1 << n)
+        _lib.call("bitz_tensor_table_dev", self.ctx.ptr, _ptr(a), _ptr(b), n, _ptr(out))
+        return out
+
     def fold_halves_(self, v, h, rho):
         _lib.call("bitz_fold_halves_dev", self.ctx.ptr, _ptr(v), h, _vp(_b16(rho)))
         return v
@@ -330,9 +368,9 @@ class Commitment:
         _lib.call("bitz_standalone_evaluation", self.ctx.ptr, self._h, _vp(out))
         return int.from_bytes(bytes(out), "little")
 
-    def prove(self, claimed: int, capacity: int = 1 << 24) -> bytes:
-        buf = (C.c_uint8 * capacity)()
+    def prove(self, claimed: int, capacity: int = 1 << 22) -> bytes:
+        buf = bytearray(capacity)
         n = C.c_uint64()
-        _lib.call("bitz_prove_standalone", self.ctx.ptr, self._h, _vp(_b16(claimed)), _vp(buf), capacity,
-                  C.byref(n))
-        return bytes(buf[: n.value])
+        _lib.call("bitz_prove_standalone", self.ctx.ptr, self._h, _vp(_b16(claimed)),
+                  _vp((C.c_uint8 * capacity).from_buffer(buf)), capacity, C.byref(n))
+        return bytes(memoryview(buf)[: n.value])

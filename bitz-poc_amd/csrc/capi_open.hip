@@ -7,11 +7,9 @@ int prelude(bitz_ctx* ctx) {
     BITZ_HIP(hipSetDevice(ctx->device));
     return BITZ_OK;
 }
-#define PRELUDE()                   \
-    do {                            \
-        int _rc = prelude(ctx);     \
-        if (_rc) return _rc;        \
-    } while (0)
+#define PRELUDE()                                \
+    if (int _rc = prelude(ctx)) return _rc;      \
+    bitz::PoolScope _pool_scope(&ctx->pool)
 #define CALL(expr)                  \
     do {                            \
         int _rc = (expr);           \
@@ -67,6 +65,64 @@ extern "C" int bitz_gkr_round_sums_dev(bitz_ctx* ctx, const void* l, const void*
                  "bitz_gkr_round_sums_dev: eq_c must tile the half");
     CALL(bitz::gkr_round_sums(ctx, l, r, h, eq_c, s_c, eq_y, send_one, nullptr));
     return bitz::read_result(ctx, 2, out_host);
+}
+
+extern "C" int bitz_gkr_fold_round_sums_dev(bitz_ctx* ctx, void* l, void* r, uint64_t h, const uint8_t rho[16],
+                                            const void* eq_c, int s_c, const void* eq_y, int send_one,
+                                            uint8_t out_host[32]) {
+    PRELUDE();
+    BITZ_REQUIRE(l && r && rho && eq_c && eq_y && out_host, BITZ_ERR_ARG,
+                 "bitz_gkr_fold_round_sums_dev: null argument");
+    BITZ_REQUIRE(h > 0 && s_c >= 0 && (1ull << s_c) <= h && h % (1ull << s_c) == 0, BITZ_ERR_SHAPE,
+                 "bitz_gkr_fold_round_sums_dev: eq_c must tile the half");
+    CALL(bitz::gkr_fold_round_sums(ctx, l, r, h, rho, eq_c, s_c, eq_y, send_one, nullptr));
+    return bitz::read_result(ctx, 2, out_host);
+}
+
+extern "C" int bitz_gkr_leaf_round1_sums_dev(bitz_ctx* ctx, const void* packed, const void* A, int t, int s,
+                                             const uint8_t rho0[16], const void* eq_c, const void* eq_y,
+                                             int send_one, uint8_t out_host[32]) {
+    PRELUDE();
+    BITZ_REQUIRE(packed && A && rho0 && eq_c && eq_y && out_host, BITZ_ERR_ARG,
+                 "bitz_gkr_leaf_round1_sums_dev: null argument");
+    BITZ_REQUIRE(t >= 3 && s >= 0 && t + s <= 40, BITZ_ERR_SHAPE, "bitz_gkr_leaf_round1_sums_dev: bad shape");
+    CALL(bitz::gkr_leaf_round1_sums(ctx, (const uint64_t*)packed, A, t, s, rho0, eq_c, eq_y, send_one, nullptr));
+    return bitz::read_result(ctx, 2, out_host);
+}
+
+extern "C" int bitz_gkr_premul_round_sums_dev(bitz_ctx* ctx, const void* l, void* r, uint64_t h, const void* eq_c,
+                                              int s, const void* eq_y, int send_one, uint8_t out_host[32]) {
+    PRELUDE();
+    BITZ_REQUIRE(l && r && eq_c && eq_y && out_host, BITZ_ERR_ARG, "bitz_gkr_premul_round_sums_dev: null argument");
+    BITZ_REQUIRE(s >= 0 && s <= 40 && h > 0 && (1ull << s) <= h && h % (1ull << s) == 0, BITZ_ERR_SHAPE,
+                 "bitz_gkr_premul_round_sums_dev: eq_c must tile the half");
+    CALL(bitz::gkr_premul_round_sums(ctx, l, r, h, eq_c, s, eq_y, send_one, nullptr));
+    return bitz::read_result(ctx, 2, out_host);
+}
+
+extern "C" int bitz_gkr_round_sums_rowonly_dev(bitz_ctx* ctx, const void* l, const void* r, uint64_t h, int s,
+                                               const void* eq_y, int send_one, uint8_t out_host[32]) {
+    PRELUDE();
+    BITZ_REQUIRE(l && r && eq_y && out_host, BITZ_ERR_ARG, "bitz_gkr_round_sums_rowonly_dev: null argument");
+    BITZ_REQUIRE(s >= 0 && s <= 40 && h > 0 && (1ull << s) <= h && h % (1ull << s) == 0, BITZ_ERR_SHAPE,
+                 "bitz_gkr_round_sums_rowonly_dev: rows of 2^s must tile the half");
+    CALL(bitz::gkr_round_sums_rowonly(ctx, l, r, h, s, eq_y, send_one, nullptr));
+    return bitz::read_result(ctx, 2, out_host);
+}
+
+extern "C" int bitz_tensor_table_dev(bitz_ctx* ctx, const void* a, const void* b, int n, void* out) {
+    PRELUDE();
+    BITZ_REQUIRE(a && b && out, BITZ_ERR_ARG, "bitz_tensor_table_dev: null argument");
+    return bitz::tensor_table(ctx, a, b, n, out);
+}
+
+extern "C" int bitz_gkr_leaf_fold2_dev(bitz_ctx* ctx, const void* packed, const void* A, int t, int s,
+                                       const uint8_t rho0[16], const uint8_t rho1[16], const void* eq_c, void* l_out,
+                                       void* r_out) {
+    PRELUDE();
+    BITZ_REQUIRE(packed && A && rho0 && rho1 && l_out && r_out, BITZ_ERR_ARG, "bitz_gkr_leaf_fold2_dev: null argument");
+    BITZ_REQUIRE(t >= 3 && s >= 0 && t + s <= 40, BITZ_ERR_SHAPE, "bitz_gkr_leaf_fold2_dev: bad shape");
+    return bitz::gkr_leaf_fold2(ctx, (const uint64_t*)packed, A, t, s, rho0, rho1, eq_c, l_out, r_out);
 }
 
 extern "C" int bitz_gkr_leaf_round_sums_dev(bitz_ctx* ctx, const void* packed, const void* A, int t, int s,

@@ -476,6 +476,7 @@ extern "C" int bitz_ntt_forward_dev(bitz_ctx* ctx, void* data, int log_d, int lo
                  "bitz_ntt_forward_dev: log_lanes must be in [0, 6]");
     if (start_layer == log_d) return BITZ_OK;
     BITZ_HIP(hipSetDevice(ctx->device));
+    bitz::PoolScope scope(&ctx->pool);
     return bitz::run_ntt(ctx, (uint4*)data, nullptr, ~0ull, log_d, log_lanes, start_layer, ell,
                          nullptr);
 }
@@ -511,6 +512,8 @@ extern "C" int bitz_merkle_tree_dev(bitz_ctx* ctx, const void* data, uint64_t nu
 
 extern "C" int bitz_commit_dev(bitz_ctx* ctx, const void* packed, int log_msg_len, int log_batch,
                                int log_inv_rate, void* codeword, void* tree, uint8_t* root) {
+    BITZ_REQUIRE(ctx != nullptr, BITZ_ERR_ARG, "bitz_commit_dev: null context");
+    bitz::PoolScope scope(&ctx->pool);
     int rc = bitz::commit_device(ctx, packed, log_msg_len, log_batch, log_inv_rate, codeword, tree);
     if (rc) return rc;
     if (root) {
@@ -546,6 +549,9 @@ static int commit_impl(bitz_ctx* ctx, const uint64_t* rows, bool rows_on_device,
     BITZ_REQUIRE(t >= 7 && t <= 35 && s <= 35 - t, BITZ_ERR_SHAPE,
                  "bitz_commit: shape needs 7 <= t and t + s <= 35");
     int log_msg = (int)(t + s) - 7;
+    bitz::PoolScope scope(&ctx->pool);
+    bitz::PhaseClock phase(ctx);
+    phase.mark("commit");
     auto h = std::make_unique<bitz_commit_handle>();
     h->ctx = ctx;
     h->t = t;

@@ -5,6 +5,7 @@
 #include <stdint.h>
 #include <string.h>
 
+#include <chrono>
 #include <map>
 #include <memory>
 #include <string>
@@ -16,35 +17,106 @@ namespace bitz {
 
 void set_error(const std::string& msg);
 
+// Caching device allocator of one context. Every launch of a context goes to
+// its one stream, so a block released here and handed out again is only
+// touched by later work of that stream: no synchronisation is needed. Large
+// blocks are rounded up to 2 MiB so that the buffers of repeated proofs of
+// one shape are served from the cache instead of hipMalloc/hipFree.
+struct Pool {
+    std::multimap<size_t, void*> free_blocks;
+    size_t cached = 0;
+    static size_t round_up(size_t n) {
+        const size_t big = 2u << 20, small = 256;
+        return n >= big / 2 ? (n + big - 1) / big * big : (n + small - 1) / small * small;
+    }
+    void* get(size_t n, size_t* got) {
+        n = round_up(n);
+        auto it = free_blocks.lower_bound(n);
+        if (it != free_blocks.end() && it->first <= n + n / 4) {
+            void* p = it->second;
+            *got = it->first;
+            cached -= it->first;
+            free_blocks.erase(it);
+            return p;
+        }
+        void* p = nullptr;
+        if (hipMalloc(&p, n) != hipSuccess) {
+            (void)hipGetLastError();
+            trim();
+            if (hipMalloc(&p, n) != hipSuccess) {
+                (void)hipGetLastError();
+                return nullptr;
+            }
+        }
+        *got = n;
+        return p;
+    }
+    void put(void* p, size_t n) {
+        free_blocks.emplace(n, p);
+        cached += n;
+    }
+    void trim() {
+        for (auto& kv : free_blocks) (void)hipFree(kv.second);
+        free_blocks.clear();
+        cached = 0;
+    }
+    ~Pool() { trim(); }
+};
+
+// The pool that DevBuf allocations of the calling thread draw from (set by
+// PoolScope at the library's entry points; none = plain hipMalloc).
+Pool*& current_pool();
+struct PoolScope {
+    Pool* prev;
+    explicit PoolScope(Pool* p) : prev(current_pool()) { current_pool() = p; }
+    ~PoolScope() { current_pool() = prev; }
+};
+
 struct DevBuf {
     void* ptr = nullptr;
     size_t bytes = 0;
+    Pool* pool = nullptr;  // where ptr came from and goes back to
     DevBuf() = default;
     DevBuf(const DevBuf&) = delete;
     DevBuf& operator=(const DevBuf&) = delete;
-    DevBuf(DevBuf&& o) noexcept : ptr(o.ptr), bytes(o.bytes) {
+    DevBuf(DevBuf&& o) noexcept : ptr(o.ptr), bytes(o.bytes), pool(o.pool) {
         o.ptr = nullptr;
         o.bytes = 0;
     }
     DevBuf& operator=(DevBuf&& o) noexcept {
         if (this != &o) {
-            if (ptr) (void)hipFree(ptr);
+            release();
             ptr = o.ptr;
             bytes = o.bytes;
+            pool = o.pool;
             o.ptr = nullptr;
             o.bytes = 0;
         }
         return *this;
     }
-    ~DevBuf() {
-        if (ptr) (void)hipFree(ptr);
+    ~DevBuf() { release(); }
+    void release() {
+        if (ptr) {
+            if (pool)
+                pool->put(ptr, bytes);
+            else
+                (void)hipFree(ptr);
+        }
+        ptr = nullptr;
+        bytes = 0;
     }
     bool alloc(size_t n) {
         if (ptr && bytes >= n) return true;
-        if (ptr) (void)hipFree(ptr);
-        ptr = nullptr;
-        bytes = 0;
+        release();
         if (n == 0) return true;
+        pool = current_pool();
+        if (pool) {
+            size_t got = 0;
+            ptr = pool->get(n, &got);
+            if (!ptr) return false;
+            bytes = got;
+            return true;
+        }
         if (hipMalloc(&ptr, n) != hipSuccess) {
             ptr = nullptr;
             return false;
@@ -60,7 +132,7 @@ namespace bitz {
 // Per-context device scratch reused across calls (partial sums, small
 // tables) plus a pinned host staging buffer for round results.
 struct Scratch {
-    DevBuf partials, result, eq_lo, fold_partial, comb, what, misc;
+    DevBuf partials, result, eq_lo, fold_partial, comb, what, misc, leaf_tab;
     void* host = nullptr;
     uint8_t comb_base[16] = {0};
     bool comb_ready = false;
@@ -84,6 +156,8 @@ struct PendingTiming {
 }  // namespace bitz
 
 struct bitz_ctx {
+    // first member: destroyed last, after every buffer that returns to it
+    bitz::Pool pool;
     int device = 0;
     bitz::Scratch scratch;
     bool timing = false;
@@ -115,6 +189,25 @@ struct TimedLaunch {
     bool on;
     TimedLaunch(bitz_ctx* c, const char* name, double bytes, double ops);
     ~TimedLaunch();
+};
+}  // namespace bitz
+
+namespace bitz {
+// Host wall-clock phases of a call: mark(name) closes the running phase and
+// opens the next; with timing on, each closed phase adds its milliseconds
+// to ctx->stats under "phase:<name>" (host work plus any wait on the GPU).
+struct PhaseClock {
+    bitz_ctx* ctx;
+    std::string cur;
+    std::chrono::steady_clock::time_point t0;
+    explicit PhaseClock(bitz_ctx* c) : ctx(c) {}
+    void mark(const char* name) {
+        close();
+        cur = name;
+        t0 = std::chrono::steady_clock::now();
+    }
+    void close();
+    ~PhaseClock() { close(); }
 };
 }  // namespace bitz
 

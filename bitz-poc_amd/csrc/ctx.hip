@@ -6,6 +6,10 @@
 namespace bitz {
 static thread_local std::string g_err;
 void set_error(const std::string& msg) { g_err = msg; }
+Pool*& current_pool() {
+    static thread_local Pool* pool = nullptr;
+    return pool;
+}
 }  // namespace bitz
 
 extern "C" const char* bitz_last_error(void) { return bitz::g_err.c_str(); }
@@ -36,6 +40,16 @@ TimedLaunch::~TimedLaunch() {
     if (!on) return;
     (void)hipEventRecord(p.stop, ctx->stream);
     ctx->pending.push_back(p);
+}
+
+void PhaseClock::close() {
+    if (!cur.empty() && ctx->timing) {
+        double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        auto& st = ctx->stats["phase:" + cur];
+        st.launches++;
+        st.ms += ms;
+    }
+    cur.clear();
 }
 
 static void resolve_timings(bitz_ctx* ctx) {
@@ -117,9 +131,22 @@ extern "C" void bitz_ctx_destroy(bitz_ctx* ctx) {
 
 extern "C" int bitz_ctx_set_stream(bitz_ctx* ctx, void* stream) {
     BITZ_REQUIRE(ctx != nullptr, BITZ_ERR_ARG, "bitz_ctx_set_stream: null ctx");
+    // blocks cached by the context's pool may still be in use by work on the
+    // old stream: drain it before the new stream can be handed those blocks
+    if ((hipStream_t)stream != ctx->stream) BITZ_HIP(hipStreamSynchronize(ctx->stream));
     ctx->stream = (hipStream_t)stream;
     return BITZ_OK;
 }
+
+extern "C" int bitz_ctx_trim(bitz_ctx* ctx) {
+    BITZ_REQUIRE(ctx != nullptr, BITZ_ERR_ARG, "bitz_ctx_trim: null ctx");
+    BITZ_HIP(hipSetDevice(ctx->device));
+    BITZ_HIP(hipStreamSynchronize(ctx->stream));
+    ctx->pool.trim();
+    return BITZ_OK;
+}
+
+extern "C" uint64_t bitz_ctx_cached_bytes(bitz_ctx* ctx) { return ctx ? (uint64_t)ctx->pool.cached : 0; }
 
 extern "C" void* bitz_ctx_stream(bitz_ctx* ctx) { return ctx ? (void*)ctx->stream : nullptr; }
 

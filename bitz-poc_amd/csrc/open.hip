@@ -58,6 +58,23 @@ __global__ void k_eq_combine(const uint4* __restrict__ lo, int nlo, const uint4*
     }
 }
 
+// out[x] = prod_i (bit i of x ? b[i] : a[i]) for x < 2^n (n <= 14), one block
+__global__ void k_tensor_small(const uint4* __restrict__ a, const uint4* __restrict__ b, int n,
+                               uint4* __restrict__ out) {
+    if (threadIdx.x == 0) out[0] = make_uint4(1, 0, 0, 0);
+    __syncthreads();
+    for (int i = 0; i < n; i++) {
+        gf128 ai = gf_load(&a[i]), bi = gf_load(&b[i]);
+        uint32_t len = 1u << i;
+        for (uint32_t x = threadIdx.x; x < len; x += blockDim.x) {
+            gf128 p = gf_load(&out[x]);
+            gf_store(&out[x + len], gf_mul(p, bi));
+            gf_store(&out[x], gf_mul(p, ai));
+        }
+        __syncthreads();
+    }
+}
+
 // ------------------------------------------------------------- column folds
 // eta_c = sum_b w_b bit(c, b) as wrapping u128 (src/bitz/fold.rs:58-170).
 // Grid: (column, word-chunk); partial sums per chunk, then a second pass.
@@ -157,37 +174,49 @@ __global__ void k_pack_columns(const uint64_t* __restrict__ rows, int t, int s,
 
 // ------------------------------------------------------------- GKR rounds
 // Weight of entry x of a round's half: eq_y[x >> s_c] * eq_c[x & (2^s_c - 1)].
-// Each thread takes a run of kRun consecutive x (one y), accumulates the
-// eq_c-weighted products unreduced, then multiplies by eq_y once.
-constexpr int kRun = 8;
-
+// A block takes chunks of kT * R consecutive slots, thread i the slots
+// base + i + kT k (k < R): loads coalesce across the wave. When R > 1 a chunk
+// lies inside one row of eq_c (kT * R divides 2^s_c), so the thread adds its
+// R weighted products unreduced and multiplies by eq_y once.
 struct RoundVals {
     gf128 el, eh, ol, oh;
 };
 
-template <class Src>
+// ColW = false: the right operand already carries eq_c (it was multiplied
+// in once per layer), only eq_y weighs the chunk.
+template <class Src, bool ColW = true>
 __global__ __launch_bounds__(kT) void k_gkr_round_sums(Src src, uint64_t h, const uint4* __restrict__ eq_c,
-                                                       int s_c, const uint4* __restrict__ eq_y,
-                                                       int send_one, uint4* __restrict__ partials) {
+                                                       int s_c, const uint4* __restrict__ eq_y, int send_one,
+                                                       uint32_t R, uint4* __restrict__ partials) {
+    __shared__ uint4 tab[Src::kTable ? 512 : 1];
+    if constexpr (Src::kTable) {
+        src.build(tab);
+        __syncthreads();
+    }
     gf128 acc[2] = {gf_zero(), gf_zero()};
-    // a run never crosses a row of eq_c (2^s_c entries)
-    const uint64_t run_len = s_c >= 3 ? (uint64_t)kRun : (1ull << s_c);
-    const uint64_t runs = (h + run_len - 1) / run_len;
+    const uint64_t chunk = (uint64_t)kT * R;
+    const uint64_t nchunks = (h + chunk - 1) / chunk;
     const uint64_t cmask = (1ull << s_c) - 1;
-    for (uint64_t run = grid_tid(); run < runs; run += grid_threads()) {
-        uint64_t x0 = run * run_len;
+    for (uint64_t ch = blockIdx.x; ch < nchunks; ch += gridDim.x) {
+        const uint64_t x0 = ch * chunk + threadIdx.x;
+        if (x0 >= h) continue;
         gf256 w_end = gf256_zero(), w_inf = gf256_zero();
-        int cnt = (int)((h - x0) < run_len ? (h - x0) : run_len);
-        for (int k = 0; k < cnt; k++) {
-            uint64_t x = x0 + k;
-            RoundVals v = src.at(x, h);
+        for (uint32_t k = 0; k < R; k++) {
+            uint64_t x = x0 + (uint64_t)k * kT;
+            if (x >= h) break;
+            RoundVals v = src.at(x, h, tab);
             gf128 e_end = send_one ? v.eh : v.el;
             gf128 o_end = send_one ? v.oh : v.ol;
-            gf128 p_end = gf_mul(e_end, o_end);
-            gf128 p_inf = gf_mul(gf_add(v.eh, v.el), gf_add(v.oh, v.ol));
-            gf128 wc = gf_load(&eq_c[x & cmask]);
-            gf256_acc(&w_end, gf_mul_wide(p_end, wc));
-            gf256_acc(&w_inf, gf_mul_wide(p_inf, wc));
+            if constexpr (ColW) {
+                gf128 p_end = gf_mul(e_end, o_end);
+                gf128 p_inf = gf_mul(gf_add(v.eh, v.el), gf_add(v.oh, v.ol));
+                gf128 wc = gf_load(&eq_c[x & cmask]);
+                gf256_acc(&w_end, gf_mul_wide(p_end, wc));
+                gf256_acc(&w_inf, gf_mul_wide(p_inf, wc));
+            } else {
+                gf256_acc(&w_end, gf_mul_wide(e_end, o_end));
+                gf256_acc(&w_inf, gf_mul_wide(gf_add(v.eh, v.el), gf_add(v.oh, v.ol)));
+            }
         }
         gf128 wy = gf_load(&eq_y[x0 >> s_c]);
         acc[0] = gf_add(acc[0], gf_mul(gf_reduce(w_end), wy));
@@ -196,18 +225,78 @@ __global__ __launch_bounds__(kT) void k_gkr_round_sums(Src src, uint64_t h, cons
     block_xor_store<2>(acc, partials + (size_t)blockIdx.x * 2);
 }
 
+// Slots per thread for a round of h slots with eq_c rows of 2^s_c: up to 32,
+// the chunk kT * R inside one row, and enough chunks to fill the chip.
+static uint32_t round_chunk_r(uint64_t h, int s_c) {
+    uint64_t row = 1ull << s_c;
+    if (row < (uint64_t)kT) return 1;
+    uint64_t r = std::min<uint64_t>(32, row / kT);
+    while (r > 1 && h / ((uint64_t)kT * r) < 2048) r >>= 1;
+    return (uint32_t)r;
+}
+static unsigned round_blocks(uint64_t h, uint32_t R) {
+    uint64_t nch = (h + (uint64_t)kT * R - 1) / ((uint64_t)kT * R);
+    return (unsigned)std::max<uint64_t>(1, std::min<uint64_t>(nch, 4096));
+}
+
 // materialised halves l, r (each 2h entries)
 struct DenseSrc {
+    static constexpr bool kTable = false;
     const uint4* l;
     const uint4* r;
-    __device__ __forceinline__ RoundVals at(uint64_t x, uint64_t h) const {
+    __device__ __forceinline__ RoundVals at(uint64_t x, uint64_t h, const uint4*) const {
         return RoundVals{gf_load(&l[x]), gf_load(&l[x + h]), gf_load(&r[x]), gf_load(&r[x + h])};
+    }
+};
+
+// Round 0 of a layer whose right half gets eq_c multiplied in: slot x
+// replaces r[x], r[x+h] by eq_c[x & cmask] * r[x], eq_c[.] * r[x+h] and
+// returns those (each r entry is read and written by its own slot only).
+struct PremulSrc {
+    static constexpr bool kTable = false;
+    const uint4* l;
+    uint4* r;
+    const uint4* eq_c;
+    uint64_t cmask;
+    __device__ __forceinline__ RoundVals at(uint64_t x, uint64_t h, const uint4*) const {
+        gf128 wc = gf_load(&eq_c[x & cmask]);
+        gf128 ol = gf_mul(gf_load(&r[x]), wc), oh = gf_mul(gf_load(&r[x + h]), wc);
+        gf_store(&r[x], ol);
+        gf_store(&r[x + h], oh);
+        return RoundVals{gf_load(&l[x]), gf_load(&l[x + h]), ol, oh};
+    }
+};
+
+// The previous round's fold fused in: l, r still hold that round's halves
+// (4h entries each); slot x folds entries x, x+2h into x and x+h, x+3h into
+// x+h by rho (v + rho (v' - v)), stores them and returns them. Slot x writes
+// only x and x+h and reads only x, x+h, x+2h, x+3h (x < h): no other slot
+// touches those, so folding in place is safe.
+struct FoldSrc {
+    static constexpr bool kTable = true;
+    uint4* l;
+    uint4* r;
+    uint64_t rho_lo, rho_hi;
+    __device__ __forceinline__ void build(uint4* tab) const {
+        gf_table_build(tab, gf_from_u64(rho_lo, rho_hi), threadIdx.x, blockDim.x);
+    }
+    __device__ __forceinline__ gf128 fold(const uint4* tab, uint4* v, uint64_t i, uint64_t h) const {
+        gf128 a = gf_load(&v[i]), b = gf_load(&v[i + 2 * h]);
+        gf128 o = gf_add(a, gf_mul_tab(tab, gf_add(a, b)));
+        gf_store(&v[i], o);
+        return o;
+    }
+    __device__ __forceinline__ RoundVals at(uint64_t x, uint64_t h, const uint4* tab) const {
+        gf128 el = fold(tab, l, x, h), eh = fold(tab, l, x + h, h);
+        gf128 ol = fold(tab, r, x, h), oh = fold(tab, r, x + h, h);
+        return RoundVals{el, eh, ol, oh};
     }
 };
 
 // leaf layer: leaf(row, c) = bit(c, row) ? A[row] : 1, halves of 2^(t+s-1)
 // entries (l = rows < 2^(t-1)), so round 0 has h = 2^(t+s-2).
 struct LeafSrc {
+    static constexpr bool kTable = false;
     const uint64_t* packed;  // row-major packed columns
     const uint4* A;
     int t, s;
@@ -215,7 +304,7 @@ struct LeafSrc {
         uint64_t word = packed[(c >> 6) * (1ull << t) + row];
         return ((word >> (c & 63)) & 1) ? gf_load(&A[row]) : gf_one();
     }
-    __device__ __forceinline__ RoundVals at(uint64_t x, uint64_t h) const {
+    __device__ __forceinline__ RoundVals at(uint64_t x, uint64_t h, const uint4* = nullptr) const {
         uint64_t y = x >> s, c = x & ((1ull << s) - 1);
         uint64_t q = 1ull << (t - 2);
         return RoundVals{leaf(y, c), leaf(y + q, c), leaf(y + 2 * q, c), leaf(y + 3 * q, c)};
@@ -234,28 +323,199 @@ __global__ __launch_bounds__(kT) void k_fold_halves(uint4* __restrict__ v, uint6
     }
 }
 
-// Leaf layer folded once by rho: writes l', r' (2^(t+s-2) each).
-__global__ __launch_bounds__(kT) void k_gkr_leaf_fold(LeafSrc src, uint64_t rho_lo, uint64_t rho_hi,
-                                                      uint4* __restrict__ l_out, uint4* __restrict__ r_out) {
+// ---- leaf layer by bit patterns (src/bitz/forest.rs bit_rounds) ----
+// Leaf (row, c) is 1 or A[row] according to one bit. For the rows
+// y, y+q, y+2q, y+3q (q = 2^(t-2)) that meet in one round-0 slot, every
+// product of the round takes one of 16 values, indexed by the column's
+// 4-bit pattern (bit k = bit(c, y + kq)).
+__device__ __forceinline__ uint32_t leaf_pattern4(const uint64_t* col, uint64_t y, uint64_t q, int bit) {
+    return (uint32_t)(((col[y] >> bit) & 1ull) | (((col[y + q] >> bit) & 1ull) << 1) |
+                      (((col[y + 2 * q] >> bit) & 1ull) << 2) | (((col[y + 3 * q] >> bit) & 1ull) << 3));
+}
+
+// qtab[(y*16 + pat)*2 + {0,1}] = eq_y[y] * {p_end, p_inf}(pat)
+__global__ void k_leaf_qtab(const uint4* __restrict__ A, int t, const uint4* __restrict__ eq_y, int send_one,
+                            uint4* __restrict__ qtab) {
+    const uint64_t q = 1ull << (t - 2);
+    uint64_t idx = grid_tid();
+    if (idx >= q * 16) return;
+    uint64_t y = idx >> 4;
+    uint32_t pat = (uint32_t)(idx & 15);
+    gf128 one = gf_one();
+    gf128 el = (pat & 1) ? gf_load(&A[y]) : one;
+    gf128 eh = (pat & 2) ? gf_load(&A[y + q]) : one;
+    gf128 ol = (pat & 4) ? gf_load(&A[y + 2 * q]) : one;
+    gf128 oh = (pat & 8) ? gf_load(&A[y + 3 * q]) : one;
+    gf128 wy = gf_load(&eq_y[y]);
+    gf128 pe = gf_mul(send_one ? eh : el, send_one ? oh : ol);
+    gf128 pi = gf_mul(gf_add(el, eh), gf_add(ol, oh));
+    gf_store(&qtab[idx * 2], gf_mul(pe, wy));
+    gf_store(&qtab[idx * 2 + 1], gf_mul(pi, wy));
+}
+
+// Pattern sums of the leaf layer with weights eq_y[y] * eq_c[c]: the slot
+// (y, c) reads K bits of column c at rows y + k*stride (k < K) and the table
+// entry qtab[((y << K) + pattern)*2 + {0,1}]; thread (c, ylane) XORs the
+// entries over its rows y < ny, then multiplies by eq_c[c].
+template <int K>
+__global__ __launch_bounds__(kT) void k_leaf_pattern_sums(const uint64_t* __restrict__ packed, int t, int s,
+                                                          uint64_t ny, uint64_t stride,
+                                                          const uint4* __restrict__ qtab,
+                                                          const uint4* __restrict__ eq_c, uint64_t ylanes,
+                                                          uint4* __restrict__ partials) {
+    const uint64_t cols = 1ull << s, nrows = 1ull << t;
+    gf128 acc[2] = {gf_zero(), gf_zero()};
+    uint64_t gt = grid_tid();
+    if (gt < cols * ylanes) {
+        uint64_t c = gt & (cols - 1), yl = gt >> s;
+        const uint64_t* col = packed + (c >> 6) * nrows;
+        int bit = (int)(c & 63);
+        uint32_t e0 = 0, e1 = 0, e2 = 0, e3 = 0, i0 = 0, i1 = 0, i2 = 0, i3 = 0;
+        for (uint64_t y = yl; y < ny; y += ylanes) {
+            uint32_t pat = 0;
+#pragma unroll
+            for (int k = 0; k < K; k++) pat |= (uint32_t)((col[y + k * stride] >> bit) & 1ull) << k;
+            const uint4* e = &qtab[((y << K) + pat) * 2];
+            uint4 a = e[0], b = e[1];
+            e0 ^= a.x, e1 ^= a.y, e2 ^= a.z, e3 ^= a.w;
+            i0 ^= b.x, i1 ^= b.y, i2 ^= b.z, i3 ^= b.w;
+        }
+        gf128 wc = gf_load(&eq_c[c]);
+        acc[0] = gf_mul(gf128{{e0, e1, e2, e3}}, wc);
+        acc[1] = gf_mul(gf128{{i0, i1, i2, i3}}, wc);
+    }
+    block_xor_store<2>(acc, partials + (size_t)blockIdx.x * 2);
+}
+
+// Round 1 of the leaf layer. After round 0 folded by rho0, the halves are
+// l'(y, c) = ftab[y*8 + pat2(y, y+q)] and r'(y, c) = ftab[y*8 + 4 + pat2(y+2q, y+3q)]
+// (q = 2^(t-2)); a round-1 slot y1 < q8 = q/2 meets l'(y1), l'(y1+q8),
+// r'(y1), r'(y1+q8): eight rows y1 + k q8, bit k of the 8-bit pattern.
+// q1tab[(y1*256 + pat)*2 + {0,1}] = eq_y[y1] * {p_end, p_inf}(pat).
+__device__ __forceinline__ void leaf_round1_vals(const uint4* ftab, uint64_t y1, uint64_t q8, uint32_t pat,
+                                                 gf128* el, gf128* eh, gf128* ol, gf128* oh) {
+    *el = gf_load(&ftab[y1 * 8 + ((pat & 1) | ((pat >> 1) & 2))]);
+    *eh = gf_load(&ftab[(y1 + q8) * 8 + (((pat >> 1) & 1) | ((pat >> 2) & 2))]);
+    *ol = gf_load(&ftab[y1 * 8 + 4 + (((pat >> 4) & 1) | ((pat >> 5) & 2))]);
+    *oh = gf_load(&ftab[(y1 + q8) * 8 + 4 + (((pat >> 5) & 1) | ((pat >> 6) & 2))]);
+}
+
+__global__ void k_leaf_q1tab(const uint4* __restrict__ ftab, int t, const uint4* __restrict__ eq_y, int send_one,
+                             uint4* __restrict__ q1tab) {
+    const uint64_t q8 = 1ull << (t - 3);
+    uint64_t idx = grid_tid();
+    if (idx >= q8 * 256) return;
+    uint64_t y1 = idx >> 8;
+    gf128 el, eh, ol, oh;
+    leaf_round1_vals(ftab, y1, q8, (uint32_t)(idx & 255), &el, &eh, &ol, &oh);
+    gf128 wy = gf_load(&eq_y[y1]);
+    gf128 pe = send_one ? gf_mul(eh, oh) : gf_mul(el, ol);
+    gf128 pi = gf_mul(gf_add(el, eh), gf_add(ol, oh));
+    gf_store(&q1tab[idx * 2], gf_mul(pe, wy));
+    gf_store(&q1tab[idx * 2 + 1], gf_mul(pi, wy));
+}
+
+// The leaf layer after two folds (rho0, then rho1): gtab[y1*32 + half*16 + sub]
+// with sub the half's 4 pattern bits (l: bits 0-3, r: bits 4-7).
+__global__ __launch_bounds__(kT) void k_leaf_gtab(const uint4* __restrict__ ftab, int t, uint64_t rho_lo,
+                                                  uint64_t rho_hi, uint4* __restrict__ gtab) {
     __shared__ uint4 tab[512];
     gf_table_build(tab, gf_from_u64(rho_lo, rho_hi), threadIdx.x, blockDim.x);
     __syncthreads();
-    const uint64_t h = 1ull << (src.t + src.s - 2);
+    const uint64_t q8 = 1ull << (t - 3);
+    uint64_t idx = grid_tid();
+    if (idx >= q8 * 32) return;
+    uint64_t y1 = idx >> 5;
+    uint32_t half = (uint32_t)((idx >> 4) & 1), sub = (uint32_t)(idx & 15);
+    gf128 el, eh, ol, oh;
+    leaf_round1_vals(ftab, y1, q8, sub << (4 * half), &el, &eh, &ol, &oh);
+    gf128 lo = half ? ol : el, hi = half ? oh : eh;
+    gf_store(&gtab[idx], gf_add(lo, gf_mul_tab(tab, gf_add(lo, hi))));
+}
+
+// eq_c != nullptr: the right half is written multiplied by eq_c[c]
+__global__ __launch_bounds__(kT) void k_leaf_fold2_pattern(const uint64_t* __restrict__ packed, int t, int s,
+                                                           const uint4* __restrict__ gtab,
+                                                           const uint4* __restrict__ eq_c, uint4* __restrict__ l_out,
+                                                           uint4* __restrict__ r_out) {
+    const uint64_t q8 = 1ull << (t - 3), nrows = 1ull << t, h = 1ull << (t + s - 3);
     for (uint64_t x = grid_tid(); x < h; x += grid_threads()) {
-        RoundVals v = src.at(x, h);
-        gf_store(&l_out[x], gf_add(v.el, gf_mul_tab(tab, gf_add(v.el, v.eh))));
-        gf_store(&r_out[x], gf_add(v.ol, gf_mul_tab(tab, gf_add(v.ol, v.oh))));
+        uint64_t y1 = x >> s, c = x & ((1ull << s) - 1);
+        const uint64_t* col = packed + (c >> 6) * nrows;
+        int bit = (int)(c & 63);
+        uint32_t pat = 0;
+#pragma unroll
+        for (int k = 0; k < 8; k++) pat |= (uint32_t)((col[y1 + k * q8] >> bit) & 1ull) << k;
+        l_out[x] = gtab[y1 * 32 + (pat & 15)];
+        if (eq_c)
+            gf_store(&r_out[x], gf_mul(gf_load(&gtab[y1 * 32 + 16 + (pat >> 4)]), gf_load(&eq_c[c])));
+        else
+            r_out[x] = gtab[y1 * 32 + 16 + (pat >> 4)];
     }
 }
 
-// Level 1 of the product tree straight from the bits: entry (y, c) =
-// leaf(y, c) * leaf(y + 2^(t-1), c), y < 2^(t-1).
-__global__ void k_gkr_level1(LeafSrc src, uint4* __restrict__ out) {
-    const uint64_t n = 1ull << (src.t + src.s - 1);
-    const uint64_t half_rows = 1ull << (src.t - 1);
+// ftab[y*8 + half*4 + pat] = lo + rho (lo + hi) with lo/hi the leaves of rows
+// y + 2*half*q and y + (2*half+1)*q selected by the 2-bit pattern.
+__global__ __launch_bounds__(kT) void k_leaf_ftab(const uint4* __restrict__ A, int t, uint64_t rho_lo,
+                                                  uint64_t rho_hi, uint4* __restrict__ ftab) {
+    __shared__ uint4 tab[512];
+    gf_table_build(tab, gf_from_u64(rho_lo, rho_hi), threadIdx.x, blockDim.x);
+    __syncthreads();
+    const uint64_t q = 1ull << (t - 2);
+    uint64_t idx = grid_tid();
+    if (idx >= q * 8) return;
+    uint64_t y = idx >> 3;
+    uint32_t half = (uint32_t)((idx >> 2) & 1), pat = (uint32_t)(idx & 3);
+    gf128 lo = (pat & 1) ? gf_load(&A[y + 2 * half * q]) : gf_one();
+    gf128 hi = (pat & 2) ? gf_load(&A[y + (2 * half + 1) * q]) : gf_one();
+    gf_store(&ftab[idx], gf_add(lo, gf_mul_tab(tab, gf_add(lo, hi))));
+}
+
+__global__ __launch_bounds__(kT) void k_leaf_fold_pattern(const uint64_t* __restrict__ packed, int t, int s,
+                                                          const uint4* __restrict__ ftab, uint4* __restrict__ l_out,
+                                                          uint4* __restrict__ r_out) {
+    const uint64_t q = 1ull << (t - 2), nrows = 1ull << t, h = 1ull << (t + s - 2);
+    for (uint64_t x = grid_tid(); x < h; x += grid_threads()) {
+        uint64_t y = x >> s, c = x & ((1ull << s) - 1);
+        uint32_t pat = leaf_pattern4(packed + (c >> 6) * nrows, y, q, (int)(c & 63));
+        l_out[x] = ftab[y * 8 + (pat & 3)];
+        r_out[x] = ftab[y * 8 + 4 + (pat >> 2)];
+    }
+}
+
+// Level 1 from the bits: entry (y, c) is 1, A[y], A[y + half] or
+// pa[y] = A[y] * A[y + half] by the two bits.
+__global__ __launch_bounds__(kT) void k_level1_pattern(const uint64_t* __restrict__ packed, int t, int s,
+                                                       const uint4* __restrict__ A, const uint4* __restrict__ pa,
+                                                       uint4* __restrict__ out) {
+    const uint64_t nrows = 1ull << t, half = nrows >> 1, n = 1ull << (t + s - 1);
     for (uint64_t x = grid_tid(); x < n; x += grid_threads()) {
-        uint64_t y = x >> src.s, c = x & ((1ull << src.s) - 1);
-        gf_store(&out[x], gf_mul(src.leaf(y, c), src.leaf(y + half_rows, c)));
+        uint64_t y = x >> s, c = x & ((1ull << s) - 1);
+        const uint64_t* col = packed + (c >> 6) * nrows;
+        int bit = (int)(c & 63);
+        uint32_t b0 = (uint32_t)((col[y] >> bit) & 1ull), b1 = (uint32_t)((col[y + half] >> bit) & 1ull);
+        uint4 v = make_uint4(1u, 0u, 0u, 0u);
+        if (b0 & b1)
+            v = pa[y];
+        else if (b0)
+            v = A[y];
+        else if (b1)
+            v = A[y + half];
+        out[x] = v;
+    }
+}
+
+// Both halves l, r (2h entries each) folded by rho in one launch.
+__global__ __launch_bounds__(kT) void k_fold_pair(uint4* __restrict__ l, uint4* __restrict__ r, uint64_t h,
+                                                  uint64_t rho_lo, uint64_t rho_hi) {
+    __shared__ uint4 tab[512];
+    gf_table_build(tab, gf_from_u64(rho_lo, rho_hi), threadIdx.x, blockDim.x);
+    __syncthreads();
+    for (uint64_t i = grid_tid(); i < 2 * h; i += grid_threads()) {
+        uint4* v = i < h ? l : r;
+        uint64_t j = i < h ? i : i - h;
+        gf128 a = gf_load(&v[j]), b = gf_load(&v[j + h]);
+        gf_store(&v[j], gf_add(a, gf_mul_tab(tab, gf_add(a, b))));
     }
 }
 
@@ -443,22 +703,36 @@ __global__ __launch_bounds__(kT) void k_lig_fold(const uint4* f, const uint4* b,
 }
 
 // Induced basis: out[j] = sum_i aw[i] prod_{k in bits(j)} what[i][k]
-// (pcs/ligerito.rs:1968-2070). One block per query i adds its tensor
-// table into out with global XOR atomics? No: to stay deterministic and
-// atomic-free, each thread owns output indices and loops over queries.
-__global__ __launch_bounds__(kT) void k_induce_basis(const uint4* __restrict__ what /* [nq][log_n] */,
-                                                     const uint4* __restrict__ aw, uint32_t nq, int log_n,
-                                                     uint4* __restrict__ out) {
-    const uint64_t n = 1ull << log_n;
+// (pcs/ligerito.rs:1968-2070). With j = (j_hi, j_lo) split at bit `lo`,
+// the product factors into plo[i][j_lo] * phi[i][j_hi] (aw folded into phi),
+// so out is a product of an (2^lo x nq) and an (nq x 2^hi) matrix.
+// k_induce_factors: plo[i][j] for j < 2^lo, then phi[i][j] for j < 2^hi.
+__global__ void k_induce_factors(const uint4* __restrict__ what /* [nq][log_n] */, const uint4* __restrict__ aw,
+                                 uint32_t nq, int log_n, int lo, uint4* __restrict__ plo, uint4* __restrict__ phi) {
+    const uint64_t nlo = 1ull << lo, nhi = 1ull << (log_n - lo);
+    uint64_t idx = grid_tid();
+    if (idx >= (uint64_t)nq * (nlo + nhi)) return;
+    bool is_lo = idx < (uint64_t)nq * nlo;
+    uint64_t k = is_lo ? idx : idx - (uint64_t)nq * nlo;
+    uint64_t n = is_lo ? nlo : nhi;
+    uint32_t i = (uint32_t)(k / n);
+    uint64_t j = k % n;
+    const uint4* w = what + (size_t)i * log_n + (is_lo ? 0 : lo);
+    gf128 p = is_lo ? gf_one() : gf_load(&aw[i]);
+    for (int b = 0; b < (is_lo ? lo : log_n - lo); b++)
+        if ((j >> b) & 1) p = gf_mul(p, gf_load(&w[b]));
+    gf_store(is_lo ? &plo[(size_t)i * nlo + j] : &phi[(size_t)i * nhi + j], p);
+}
+
+__global__ __launch_bounds__(kT) void k_induce_basis(const uint4* __restrict__ plo, const uint4* __restrict__ phi,
+                                                     uint32_t nq, int log_n, int lo, uint4* __restrict__ out) {
+    const uint64_t n = 1ull << log_n, nlo = 1ull << lo, nhi = 1ull << (log_n - lo);
     for (uint64_t j = grid_tid(); j < n; j += grid_threads()) {
-        gf128 acc = gf_zero();
-        for (uint32_t i = 0; i < nq; i++) {
-            gf128 p = gf_load(&aw[i]);
-            for (int k = 0; k < log_n; k++)
-                if ((j >> k) & 1) p = gf_mul(p, gf_load(&what[(size_t)i * log_n + k]));
-            acc = gf_add(acc, p);
-        }
-        gf_store(&out[j], acc);
+        uint64_t jl = j & (nlo - 1), jh = j >> lo;
+        gf256 acc = gf256_zero();
+        for (uint32_t i = 0; i < nq; i++)
+            gf256_acc(&acc, gf_mul_wide(gf_load(&plo[(size_t)i * nlo + jl]), gf_load(&phi[(size_t)i * nhi + jh])));
+        gf_store(&out[j], gf_reduce(acc));
     }
 }
 
@@ -489,6 +763,11 @@ __global__ __launch_bounds__(kT) void k_dot(const uint4* __restrict__ a, const u
 __global__ void k_mul_elem(const uint4* __restrict__ a, const uint4* __restrict__ b, uint64_t n,
                            uint4* __restrict__ out) {
     for (uint64_t x = grid_tid(); x < n; x += grid_threads()) gf_store(&out[x], gf_mul(gf_load(&a[x]), gf_load(&b[x])));
+}
+
+// v[x] *= b[x] (each thread reads and writes only its own index)
+__global__ void k_mul_inplace(uint4* v, const uint4* __restrict__ b, uint64_t n) {
+    for (uint64_t x = grid_tid(); x < n; x += grid_threads()) gf_store(&v[x], gf_mul(gf_load(&v[x]), gf_load(&b[x])));
 }
 
 // out[x] = (a[x] + 1) * b[x]
@@ -686,13 +965,79 @@ constexpr double kGfMulOps = 800.0;
 
 int gkr_round_sums(bitz_ctx* ctx, const void* l, const void* r, uint64_t h, const void* eq_c, int s_c,
                    const void* eq_y, int send_one, void* out_dev) {
-    unsigned nb = blocks_for(h, kRun * 4, 2048);
+    uint32_t R = round_chunk_r(h, s_c);
+    unsigned nb = round_blocks(h, R);
     int rc = ctx->scratch.ensure((size_t)nb * 2 * 16);
     if (rc) return rc;
     TimedLaunch tl(ctx, "gkr_round_sums", 4.0 * 16.0 * h, 4.0 * h * kGfMulOps);
     DenseSrc src{(const uint4*)l, (const uint4*)r};
     hipLaunchKernelGGL(k_gkr_round_sums<DenseSrc>, dim3(nb), dim3(kT), 0, ctx->stream, src, h,
-                       (const uint4*)eq_c, s_c, (const uint4*)eq_y, send_one, (uint4*)ctx->scratch.partials.ptr);
+                       (const uint4*)eq_c, s_c, (const uint4*)eq_y, send_one, R, (uint4*)ctx->scratch.partials.ptr);
+    BITZ_LAUNCH_CHECK();
+    return sums_finish(ctx, nb, 2, out_dev);
+}
+
+// One size for the leaf-layer tables, so the buffer is never reallocated
+// between launches that share it: round 0 qtab (2^(t+3) entries) or ftab
+// (2^(t+1)) or the level-1 row products (2^(t-1)) at the start; the round-1
+// q1tab (2^(t+6)) or gtab (2^(t+2)) after 2^(t+1) entries.
+static size_t leaf_tab_bytes(int t) { return (size_t)16 * ((1ull << (t + 1)) + (1ull << (t + 6))); }
+static uint4* leaf_tab_hi(bitz_ctx* ctx, int t) { return (uint4*)ctx->scratch.leaf_tab.ptr + (1ull << (t + 1)); }
+
+// threads of a pattern-sums launch: (column, ylane) pairs, about 2^20
+static uint64_t pattern_ylanes(int s, uint64_t ny) {
+    return s >= 20 ? 1 : std::min<uint64_t>(ny, (1ull << 20) >> s);
+}
+
+int gkr_premul_round_sums(bitz_ctx* ctx, const void* l, void* r, uint64_t h, const void* eq_c, int s,
+                          const void* eq_y, int send_one, void* out_dev) {
+    uint32_t R = round_chunk_r(h, s);
+    unsigned nb = round_blocks(h, R);
+    int rc = ctx->scratch.ensure((size_t)nb * 2 * 16);
+    if (rc) return rc;
+    TimedLaunch tl(ctx, "gkr_premul_round_sums", 6.0 * 16.0 * h, 4.0 * h * kGfMulOps);
+    PremulSrc src{(const uint4*)l, (uint4*)r, (const uint4*)eq_c, (1ull << s) - 1};
+    hipLaunchKernelGGL((k_gkr_round_sums<PremulSrc, false>), dim3(nb), dim3(kT), 0, ctx->stream, src, h,
+                       (const uint4*)eq_c, s, (const uint4*)eq_y, send_one, R, (uint4*)ctx->scratch.partials.ptr);
+    BITZ_LAUNCH_CHECK();
+    return sums_finish(ctx, nb, 2, out_dev);
+}
+
+int gkr_round_sums_rowonly(bitz_ctx* ctx, const void* l, const void* r, uint64_t h, int s, const void* eq_y,
+                           int send_one, void* out_dev) {
+    uint32_t R = round_chunk_r(h, s);
+    unsigned nb = round_blocks(h, R);
+    int rc = ctx->scratch.ensure((size_t)nb * 2 * 16);
+    if (rc) return rc;
+    TimedLaunch tl(ctx, "gkr_round_sums_rowonly", 4.0 * 16.0 * h, 2.0 * h * kGfMulOps);
+    DenseSrc src{(const uint4*)l, (const uint4*)r};
+    hipLaunchKernelGGL((k_gkr_round_sums<DenseSrc, false>), dim3(nb), dim3(kT), 0, ctx->stream, src, h,
+                       (const uint4*)nullptr, s, (const uint4*)eq_y, send_one, R, (uint4*)ctx->scratch.partials.ptr);
+    BITZ_LAUNCH_CHECK();
+    return sums_finish(ctx, nb, 2, out_dev);
+}
+
+int tensor_table(bitz_ctx* ctx, const void* a, const void* b, int n, void* out) {
+    BITZ_REQUIRE(n >= 0 && n <= 14, BITZ_ERR_SHAPE, "tensor_table: n out of range");
+    hipLaunchKernelGGL(k_tensor_small, dim3(1), dim3(kT), 0, ctx->stream, (const uint4*)a, (const uint4*)b, n,
+                       (uint4*)out);
+    BITZ_LAUNCH_CHECK();
+    return BITZ_OK;
+}
+
+int gkr_fold_round_sums(bitz_ctx* ctx, void* l, void* r, uint64_t h, const uint8_t rho[16], const void* eq_c,
+                        int s_c, const void* eq_y, int send_one, void* out_dev) {
+    uint32_t R = round_chunk_r(h, s_c);
+    unsigned nb = round_blocks(h, R);
+    int rc = ctx->scratch.ensure((size_t)nb * 2 * 16);
+    if (rc) return rc;
+    uint64_t lo, hi;
+    memcpy(&lo, rho, 8);
+    memcpy(&hi, rho + 8, 8);
+    TimedLaunch tl(ctx, "gkr_fold_round_sums", 12.0 * 16.0 * h, 8.0 * h * kGfMulOps);
+    FoldSrc src{(uint4*)l, (uint4*)r, lo, hi};
+    hipLaunchKernelGGL(k_gkr_round_sums<FoldSrc>, dim3(nb), dim3(kT), 0, ctx->stream, src, h, (const uint4*)eq_c,
+                       s_c, (const uint4*)eq_y, send_one, R, (uint4*)ctx->scratch.partials.ptr);
     BITZ_LAUNCH_CHECK();
     return sums_finish(ctx, nb, 2, out_dev);
 }
@@ -700,13 +1045,33 @@ int gkr_round_sums(bitz_ctx* ctx, const void* l, const void* r, uint64_t h, cons
 int gkr_leaf_round_sums(bitz_ctx* ctx, const uint64_t* packed, const void* A, int t, int s, const void* eq_c,
                         int s_c, const void* eq_y, int send_one, void* out_dev) {
     uint64_t h = 1ull << (t + s - 2);
-    unsigned nb = blocks_for(h, kRun * 4, 2048);
+    if (s_c == s && t >= 2) {
+        // weights eq_y[y] * eq_c[c]: the bit-pattern path
+        const uint64_t q = 1ull << (t - 2), cols = 1ull << s;
+        if (!ctx->scratch.leaf_tab.alloc(leaf_tab_bytes(t))) return BITZ_ERR_ALLOC;
+        uint4* qtab = (uint4*)ctx->scratch.leaf_tab.ptr;
+        uint64_t ylanes = pattern_ylanes(s, q);
+        uint64_t threads = cols * ylanes;
+        unsigned nb = (unsigned)((threads + kT - 1) / kT);
+        int rc = ctx->scratch.ensure((size_t)nb * 2 * 16);
+        if (rc) return rc;
+        TimedLaunch tl(ctx, "gkr_leaf_round_sums", (double)(1ull << (t + s - 3)) + q * 512.0,
+                       q * 64.0 * kGfMulOps + 2.0 * threads * kGfMulOps);
+        hipLaunchKernelGGL(k_leaf_qtab, dim3((unsigned)((q * 16 + kT - 1) / kT)), dim3(kT), 0, ctx->stream,
+                           (const uint4*)A, t, (const uint4*)eq_y, send_one, qtab);
+        hipLaunchKernelGGL(k_leaf_pattern_sums<4>, dim3(nb), dim3(kT), 0, ctx->stream, packed, t, s, q, q,
+                           (const uint4*)qtab, (const uint4*)eq_c, ylanes, (uint4*)ctx->scratch.partials.ptr);
+        BITZ_LAUNCH_CHECK();
+        return sums_finish(ctx, nb, 2, out_dev);
+    }
+    uint32_t R = round_chunk_r(h, s_c);
+    unsigned nb = round_blocks(h, R);
     int rc = ctx->scratch.ensure((size_t)nb * 2 * 16);
     if (rc) return rc;
     TimedLaunch tl(ctx, "gkr_leaf_round_sums", (double)(1ull << (t + s - 3)), 4.0 * h * kGfMulOps);
     LeafSrc src{packed, (const uint4*)A, t, s};
     hipLaunchKernelGGL(k_gkr_round_sums<LeafSrc>, dim3(nb), dim3(kT), 0, ctx->stream, src, h,
-                       (const uint4*)eq_c, s_c, (const uint4*)eq_y, send_one, (uint4*)ctx->scratch.partials.ptr);
+                       (const uint4*)eq_c, s_c, (const uint4*)eq_y, send_one, R, (uint4*)ctx->scratch.partials.ptr);
     BITZ_LAUNCH_CHECK();
     return sums_finish(ctx, nb, 2, out_dev);
 }
@@ -725,23 +1090,89 @@ int fold_halves(bitz_ctx* ctx, void* v, uint64_t h, const uint8_t rho[16]) {
     return BITZ_OK;
 }
 
-int gkr_leaf_fold(bitz_ctx* ctx, const uint64_t* packed, const void* A, int t, int s, const uint8_t rho[16],
-                  void* l_out, void* r_out) {
+int fold_pair(bitz_ctx* ctx, void* l, void* r, uint64_t h, const uint8_t rho[16]) {
     uint64_t lo, hi;
     split16(rho, &lo, &hi);
-    LeafSrc src{packed, (const uint4*)A, t, s};
-    TimedLaunch tl(ctx, "gkr_leaf_fold", 2.0 * 16.0 * (1ull << (t + s - 2)) + (1ull << (t + s - 3)),
-                   2.0 * (1ull << (t + s - 2)) * kGfMulOps);
-    hipLaunchKernelGGL(k_gkr_leaf_fold, dim3(blocks_for(1ull << (t + s - 2), 4)), dim3(kT), 0, ctx->stream, src, lo,
-                       hi, (uint4*)l_out, (uint4*)r_out);
+    TimedLaunch tl(ctx, "fold_pair", 6.0 * 16.0 * h, 2.0 * h * kGfMulOps);
+    hipLaunchKernelGGL(k_fold_pair, dim3(blocks_for(2 * h, 4)), dim3(kT), 0, ctx->stream, (uint4*)l, (uint4*)r, h,
+                       lo, hi);
+    BITZ_LAUNCH_CHECK();
+    return BITZ_OK;
+}
+
+int gkr_leaf_fold(bitz_ctx* ctx, const uint64_t* packed, const void* A, int t, int s, const uint8_t rho[16],
+                  void* l_out, void* r_out) {
+    BITZ_REQUIRE(t >= 2, BITZ_ERR_SHAPE, "gkr_leaf_fold: t < 2");
+    uint64_t lo, hi;
+    split16(rho, &lo, &hi);
+    const uint64_t q = 1ull << (t - 2), h = 1ull << (t + s - 2);
+    if (!ctx->scratch.leaf_tab.alloc(leaf_tab_bytes(t))) return BITZ_ERR_ALLOC;
+    uint4* ftab = (uint4*)ctx->scratch.leaf_tab.ptr;
+    TimedLaunch tl(ctx, "gkr_leaf_fold", 2.0 * 16.0 * h + (double)(1ull << (t + s - 3)), q * 8.0 * kGfMulOps);
+    hipLaunchKernelGGL(k_leaf_ftab, dim3((unsigned)((q * 8 + kT - 1) / kT)), dim3(kT), 0, ctx->stream,
+                       (const uint4*)A, t, lo, hi, ftab);
+    hipLaunchKernelGGL(k_leaf_fold_pattern, dim3(blocks_for(h, 4)), dim3(kT), 0, ctx->stream, packed, t, s,
+                       (const uint4*)ftab, (uint4*)l_out, (uint4*)r_out);
+    BITZ_LAUNCH_CHECK();
+    return BITZ_OK;
+}
+
+int gkr_leaf_round1_sums(bitz_ctx* ctx, const uint64_t* packed, const void* A, int t, int s, const uint8_t rho0[16],
+                         const void* eq_c, const void* eq_y, int send_one, void* out_dev) {
+    BITZ_REQUIRE(t >= 3, BITZ_ERR_SHAPE, "gkr_leaf_round1_sums: t < 3");
+    uint64_t lo, hi;
+    split16(rho0, &lo, &hi);
+    const uint64_t q = 1ull << (t - 2), q8 = q >> 1, cols = 1ull << s;
+    if (!ctx->scratch.leaf_tab.alloc(leaf_tab_bytes(t))) return BITZ_ERR_ALLOC;
+    uint4* ftab = (uint4*)ctx->scratch.leaf_tab.ptr;
+    uint4* q1tab = leaf_tab_hi(ctx, t);
+    uint64_t ylanes = pattern_ylanes(s, q8);
+    uint64_t threads = cols * ylanes;
+    unsigned nb = (unsigned)((threads + kT - 1) / kT);
+    int rc = ctx->scratch.ensure((size_t)nb * 2 * 16);
+    if (rc) return rc;
+    TimedLaunch tl(ctx, "gkr_leaf_round1_sums", (double)(1ull << (t + s - 3)) + q8 * 8192.0,
+                   (q * 8.0 + q8 * 1024.0 + 2.0 * threads) * kGfMulOps);
+    hipLaunchKernelGGL(k_leaf_ftab, dim3((unsigned)((q * 8 + kT - 1) / kT)), dim3(kT), 0, ctx->stream,
+                       (const uint4*)A, t, lo, hi, ftab);
+    hipLaunchKernelGGL(k_leaf_q1tab, dim3((unsigned)((q8 * 256 + kT - 1) / kT)), dim3(kT), 0, ctx->stream,
+                       (const uint4*)ftab, t, (const uint4*)eq_y, send_one, q1tab);
+    hipLaunchKernelGGL(k_leaf_pattern_sums<8>, dim3(nb), dim3(kT), 0, ctx->stream, packed, t, s, q8, q8,
+                       (const uint4*)q1tab, (const uint4*)eq_c, ylanes, (uint4*)ctx->scratch.partials.ptr);
+    BITZ_LAUNCH_CHECK();
+    return sums_finish(ctx, nb, 2, out_dev);
+}
+
+int gkr_leaf_fold2(bitz_ctx* ctx, const uint64_t* packed, const void* A, int t, int s, const uint8_t rho0[16],
+                   const uint8_t rho1[16], const void* eq_c, void* l_out, void* r_out) {
+    BITZ_REQUIRE(t >= 3, BITZ_ERR_SHAPE, "gkr_leaf_fold2: t < 3");
+    uint64_t lo0, hi0, lo1, hi1;
+    split16(rho0, &lo0, &hi0);
+    split16(rho1, &lo1, &hi1);
+    const uint64_t q = 1ull << (t - 2), q8 = q >> 1, h = 1ull << (t + s - 3);
+    if (!ctx->scratch.leaf_tab.alloc(leaf_tab_bytes(t))) return BITZ_ERR_ALLOC;
+    uint4* ftab = (uint4*)ctx->scratch.leaf_tab.ptr;
+    uint4* gtab = leaf_tab_hi(ctx, t);
+    TimedLaunch tl(ctx, "gkr_leaf_fold2", 2.0 * 16.0 * h + (double)(h / 8), (q * 8.0 + q8 * 32.0) * kGfMulOps);
+    hipLaunchKernelGGL(k_leaf_ftab, dim3((unsigned)((q * 8 + kT - 1) / kT)), dim3(kT), 0, ctx->stream,
+                       (const uint4*)A, t, lo0, hi0, ftab);
+    hipLaunchKernelGGL(k_leaf_gtab, dim3((unsigned)((q8 * 32 + kT - 1) / kT)), dim3(kT), 0, ctx->stream,
+                       (const uint4*)ftab, t, lo1, hi1, gtab);
+    hipLaunchKernelGGL(k_leaf_fold2_pattern, dim3(blocks_for(h, 4)), dim3(kT), 0, ctx->stream, packed, t, s,
+                       (const uint4*)gtab, (const uint4*)eq_c, (uint4*)l_out, (uint4*)r_out);
     BITZ_LAUNCH_CHECK();
     return BITZ_OK;
 }
 
 int gkr_level1(bitz_ctx* ctx, const uint64_t* packed, const void* A, int t, int s, void* out) {
-    LeafSrc src{packed, (const uint4*)A, t, s};
-    hipLaunchKernelGGL(k_gkr_level1, dim3(blocks_for(1ull << (t + s - 1), 4)), dim3(kT), 0, ctx->stream, src,
-                       (uint4*)out);
+    const uint64_t half = 1ull << (t - 1), n = 1ull << (t + s - 1);
+    if (!ctx->scratch.leaf_tab.alloc(leaf_tab_bytes(t))) return BITZ_ERR_ALLOC;
+    uint4* pa = (uint4*)ctx->scratch.leaf_tab.ptr;
+    TimedLaunch tl(ctx, "gkr_level1", 16.0 * n + (double)(n / 8), half * kGfMulOps);
+    hipLaunchKernelGGL(k_mul_elem, dim3(blocks_for(half, 4)), dim3(kT), 0, ctx->stream, (const uint4*)A,
+                       (const uint4*)A + half, half, pa);
+    hipLaunchKernelGGL(k_level1_pattern, dim3(blocks_for(n, 4)), dim3(kT), 0, ctx->stream, packed, t, s,
+                       (const uint4*)A, (const uint4*)pa, (uint4*)out);
     BITZ_LAUNCH_CHECK();
     return BITZ_OK;
 }
@@ -858,11 +1289,22 @@ int lig_fold(bitz_ctx* ctx, const void* f, const void* b, uint64_t n, const uint
 
 int induce_basis(bitz_ctx* ctx, const uint64_t* queries_dev, uint32_t nq, int log_n, const void* sks,
                  const void* inv_sks, const void* aw, void* out) {
-    if (!ctx->scratch.what.alloc((size_t)std::max<uint32_t>(nq, 1) * std::max(log_n, 1) * 16)) return BITZ_ERR_ALLOC;
+    const int lo = std::min(log_n, 10);
+    const uint64_t nlo = 1ull << lo, nhi = 1ull << (log_n - lo);
+    const size_t what_elems = (size_t)std::max<uint32_t>(nq, 1) * std::max(log_n, 1);
+    const size_t fac_elems = (size_t)nq * (nlo + nhi);
+    if (!ctx->scratch.what.alloc((what_elems + fac_elems) * 16)) return BITZ_ERR_ALLOC;
+    uint4* what = (uint4*)ctx->scratch.what.ptr;
+    uint4* plo = what + what_elems;
+    uint4* phi = plo + (size_t)nq * nlo;
+    TimedLaunch tl(ctx, "induce_basis", 16.0 * (1ull << log_n), (double)nq * (1ull << log_n) * kGfMulOps);
     hipLaunchKernelGGL(k_induce_what, dim3((nq + kT - 1) / kT), dim3(kT), 0, ctx->stream, queries_dev, nq, log_n,
-                       (const uint4*)sks, (const uint4*)inv_sks, (uint4*)ctx->scratch.what.ptr);
+                       (const uint4*)sks, (const uint4*)inv_sks, what);
+    if (nq > 0)
+        hipLaunchKernelGGL(k_induce_factors, dim3((unsigned)((fac_elems + kT - 1) / kT)), dim3(kT), 0, ctx->stream,
+                           (const uint4*)what, (const uint4*)aw, nq, log_n, lo, plo, phi);
     hipLaunchKernelGGL(k_induce_basis, dim3(blocks_for(1ull << log_n, 1)), dim3(kT), 0, ctx->stream,
-                       (const uint4*)ctx->scratch.what.ptr, (const uint4*)aw, nq, log_n, (uint4*)out);
+                       (const uint4*)plo, (const uint4*)phi, nq, log_n, lo, (uint4*)out);
     BITZ_LAUNCH_CHECK();
     return BITZ_OK;
 }
@@ -919,6 +1361,12 @@ int mul_elem(bitz_ctx* ctx, const void* a, const void* b, uint64_t n, void* out,
     else
         hipLaunchKernelGGL(k_mul_elem, dim3(blocks_for(n, 4)), dim3(kT), 0, ctx->stream, (const uint4*)a,
                            (const uint4*)b, n, (uint4*)out);
+    BITZ_LAUNCH_CHECK();
+    return BITZ_OK;
+}
+
+int mul_inplace(bitz_ctx* ctx, void* v, const void* b, uint64_t n) {
+    hipLaunchKernelGGL(k_mul_inplace, dim3(blocks_for(n, 4)), dim3(kT), 0, ctx->stream, (uint4*)v, (const uint4*)b, n);
     BITZ_LAUNCH_CHECK();
     return BITZ_OK;
 }

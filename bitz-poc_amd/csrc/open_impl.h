@@ -16,11 +16,34 @@ int pow_images(bitz_ctx* ctx, const uint8_t g[16], const void* exps, uint64_t n,
 int pack_columns(bitz_ctx* ctx, const uint64_t* rows, int t, int s, uint64_t* packed);
 int gkr_round_sums(bitz_ctx* ctx, const void* l, const void* r, uint64_t h, const void* eq_c, int s_c,
                    const void* eq_y, int send_one, void* out_dev);
+// the previous round's fold by rho (halves of 2h entries -> h) fused with
+// this round's sums over the folded halves
+int gkr_fold_round_sums(bitz_ctx* ctx, void* l, void* r, uint64_t h, const uint8_t rho[16], const void* eq_c,
+                        int s_c, const void* eq_y, int send_one, void* out_dev);
 int gkr_leaf_round_sums(bitz_ctx* ctx, const uint64_t* packed, const void* A, int t, int s, const void* eq_c,
                         int s_c, const void* eq_y, int send_one, void* out_dev);
 int fold_halves(bitz_ctx* ctx, void* v, uint64_t h, const uint8_t rho[16]);
+// both halves l and r (2h entries each) folded by rho in one launch
+int fold_pair(bitz_ctx* ctx, void* l, void* r, uint64_t h, const uint8_t rho[16]);
 int gkr_leaf_fold(bitz_ctx* ctx, const uint64_t* packed, const void* A, int t, int s, const uint8_t rho[16],
                   void* l_out, void* r_out);
+// round 1 of the leaf layer (after round 0 folded by rho0) from 8-bit row
+// patterns; weights eq_y[y] * eq_c[c] over all s column coordinates
+int gkr_leaf_round1_sums(bitz_ctx* ctx, const uint64_t* packed, const void* A, int t, int s, const uint8_t rho0[16],
+                         const void* eq_c, const void* eq_y, int send_one, void* out_dev);
+// the leaf layer folded by rho0 then rho1 into halves of 2^(t+s-3) entries;
+// with eq_c the right half is written multiplied by eq_c[c] (c = low s bits)
+int gkr_leaf_fold2(bitz_ctx* ctx, const uint64_t* packed, const void* A, int t, int s, const uint8_t rho0[16],
+                   const uint8_t rho1[16], const void* eq_c, void* l_out, void* r_out);
+// round sums whose right half gets eq_c[x & (2^s-1)] multiplied in (in place);
+// weights eq_y[x >> s] only
+int gkr_premul_round_sums(bitz_ctx* ctx, const void* l, void* r, uint64_t h, const void* eq_c, int s,
+                          const void* eq_y, int send_one, void* out_dev);
+// round sums over halves whose right operand already carries eq_c: weights eq_y[x >> s]
+int gkr_round_sums_rowonly(bitz_ctx* ctx, const void* l, const void* r, uint64_t h, int s, const void* eq_y,
+                           int send_one, void* out_dev);
+// out[x] = prod_i (bit i of x ? b[i] : a[i]), x < 2^n, n <= 14
+int tensor_table(bitz_ctx* ctx, const void* a, const void* b, int n, void* out);
 int gkr_level1(bitz_ctx* ctx, const uint64_t* packed, const void* A, int t, int s, void* out);
 int product_level(bitz_ctx* ctx, const void* in, uint64_t n, void* out);
 int xi_combined(bitz_ctx* ctx, const uint64_t* packed, int t, int s, const void* w2, void* out);
@@ -41,6 +64,7 @@ int gather_nodes(bitz_ctx* ctx, const void* tree, const uint64_t* idx_dev, uint3
 
 int dot(bitz_ctx* ctx, const void* a, const void* b, uint64_t n, void* out_dev);
 int mul_elem(bitz_ctx* ctx, const void* a, const void* b, uint64_t n, void* out, int plus_one);
+int mul_inplace(bitz_ctx* ctx, void* v, const void* b, uint64_t n);
 int scale(bitz_ctx* ctx, const void* a, uint64_t n, const uint8_t c[16], void* out);
 
 // commit pipeline (commit.hip)

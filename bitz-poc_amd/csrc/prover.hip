@@ -58,7 +58,7 @@ u128 addmod(u128 a, u128 b, u128 q) {
     u128 s = a + b;  // q < 2^126: no overflow
     return s >= q ? s - q : s;
 }
-u128 mulmod(u128 a, u128 b, u128 q) {
+u128 mulmod_serial(u128 a, u128 b, u128 q) {
     u128 r = 0;
     a %= q;
     for (int i = 127; i >= 0; i--) {
@@ -66,6 +66,51 @@ u128 mulmod(u128 a, u128 b, u128 q) {
         if ((b >> i) & 1) r = addmod(r, a, q);
     }
     return r;
+}
+
+// Montgomery arithmetic modulo an odd n < 2^126 with R = 2^128.
+struct ModN {
+    u128 n = 0, ninv = 0, r2 = 0;  // ninv = -n^-1 mod R, r2 = R^2 mod n
+    void set(u128 n_) {
+        n = n_;
+        u128 x = n;  // inverse of n modulo 2^3; each Newton step doubles the bits
+        for (int i = 0; i < 7; i++) x *= 2 - n * x;
+        ninv = (u128)0 - x;
+        u128 r = (~(u128)0 % n + 1) % n;  // R mod n
+        r2 = mulmod_serial(r, r, n);
+    }
+    static void wide(u128 a, u128 b, u128* hi, u128* lo) {
+        uint64_t a0 = (uint64_t)a, a1 = (uint64_t)(a >> 64), b0 = (uint64_t)b, b1 = (uint64_t)(b >> 64);
+        u128 p00 = (u128)a0 * b0, p01 = (u128)a0 * b1, p10 = (u128)a1 * b0, p11 = (u128)a1 * b1;
+        u128 mid = (p00 >> 64) + (uint64_t)p01 + (uint64_t)p10;
+        *lo = (uint64_t)p00 | (mid << 64);
+        *hi = p11 + (p01 >> 64) + (p10 >> 64) + (mid >> 64);
+    }
+    // (hi R + lo) R^-1 mod n for hi R + lo < R n
+    u128 redc(u128 hi, u128 lo) const {
+        u128 m = lo * ninv, mh, ml;
+        wide(m, n, &mh, &ml);
+        u128 c = (lo + ml) < lo ? 1 : 0;
+        u128 r = hi + mh + c;
+        return r >= n ? r - n : r;
+    }
+    u128 mul(u128 a, u128 b) const {
+        if (a >= n) a %= n;
+        if (b >= n) b %= n;
+        u128 hi, lo;
+        wide(a, b, &hi, &lo);
+        u128 t = redc(hi, lo);  // a b R^-1
+        wide(t, r2, &hi, &lo);
+        return redc(hi, lo);
+    }
+};
+
+// a b mod q (q odd, q < 2^126: every modulus here is a prime candidate or the
+// sampled prime); the Montgomery constants of the last modulus are kept.
+u128 mulmod(u128 a, u128 b, u128 q) {
+    static thread_local ModN ctx;
+    if (ctx.n != q) ctx.set(q);
+    return ctx.mul(a, b);
 }
 u128 powmod(u128 a, u128 e, u128 q) {
     u128 r = 1 % q;
@@ -231,7 +276,7 @@ struct Prover {
     LigConfig cfg;
     G g;
     // device buffers
-    DevBuf packed, rowimg, colimg, folds, roww, levels, eqc, eqy, tmp0, tmp1, tmp2, tmp3, qdev, pt;
+    DevBuf packed, rowimg, colimg, folds, roww, levels, eqc, eqy, eqcinv, tmp0, tmp1, tmp2, tmp3, qdev, pt;
     std::vector<DevBuf*> lig_bufs;
 
     int upload_gf(const std::vector<G>& v, DevBuf& buf) {
@@ -281,44 +326,135 @@ struct Prover {
         std::vector<G> next;
         G factor = gone();
         int rc;
-        int eqc_built = -1;
+        // Round `round` weighs its slots by eq over ext[0..rem) (rem = L-1-round)
+        // as eq_y[x >> s_c] * eq_c[x & (2^s_c - 1)]. The tables are built once,
+        // over ext[0..L-1); read at indices below 2^rem they equal the round's
+        // weights times the constant prod_{i=1..round} (1 + internal[i]) (the
+        // dropped coordinates sit at x = 0), which corr[round] divides out.
+        // Should a coordinate equal one the constant vanishes, and that layer
+        // rebuilds its tables every round instead.
+        std::vector<G> corr(L, gone());
+        bool per_layer = true;
+        {
+            std::vector<G> pp(L, gone());
+            for (int i = 1; i < L; i++) pp[i] = hgf_mul(pp[i - 1], hgf_add(internal[i], gone()));
+            if (geq(pp[L - 1], gzero())) {
+                per_layer = false;
+            } else {
+                G cur = hgf_inv(pp[L - 1]);
+                for (int i = L - 1; i >= 1; i--) {
+                    corr[i] = cur;
+                    cur = hgf_mul(cur, hgf_add(internal[i], gone()));
+                }
+            }
+        }
+        if (per_layer) {
+            int s_c0 = std::min(s, L - 1);
+            std::vector<G> c(ext.begin(), ext.begin() + s_c0);
+            std::vector<G> yv(ext.begin() + s_c0, ext.begin() + (L - 1));
+            if ((rc = eq_dev(c, eqc))) return rc;
+            if ((rc = eq_dev(yv, eqy))) return rc;
+        }
+        // Column weights carried by the right half (weighable / batch_inverse,
+        // src/bitz/gkr.rs:48-80; kernels.rs:114-131): eq_c depends only on the
+        // low s index bits, so r multiplied by it once (round 0, or the leaf
+        // layer's second fold) stays so through every round that binds a row
+        // variable, whose slots then skip the two eq_c multiplies. Before the
+        // first column round the factor is divided out again with the table
+        // prod_i (bit i ? ext_i : 1 + ext_i)^-1 (no coordinate is 0 or 1).
+        bool weigh = per_layer && s > 0 && L - 1 >= s;
+        for (int i = 0; weigh && i < s; i++)
+            if (geq(ext[i], gzero()) || geq(ext[i], gone())) weigh = false;
+        if (weigh) {
+            std::vector<G> v(2 * s), inv(2 * s);
+            for (int i = 0; i < s; i++) {
+                v[i] = hgf_add(ext[i], gone());
+                v[s + i] = ext[i];
+            }
+            G acc = gone();
+            for (int i = 0; i < 2 * s; i++) {
+                inv[i] = acc;  // prefix product
+                acc = hgf_mul(acc, v[i]);
+            }
+            G cur = hgf_inv(acc);
+            for (int i = 2 * s - 1; i >= 0; i--) {
+                inv[i] = hgf_mul(inv[i], cur);
+                cur = hgf_mul(cur, v[i]);
+            }
+            if ((rc = upload_gf(inv, pt))) return rc;
+            if (!eqcinv.alloc(16ull << s)) return BITZ_ERR_ALLOC;
+            if ((rc = tensor_table(ctx, pt.ptr, (const uint4*)pt.ptr + s, s, eqcinv.ptr))) return rc;
+        }
+        bool weighted = false;  // r currently carries eq_c
+        // a round's fold by its challenge is enqueued with the next round's
+        // sums (fold pass, then sums pass: on MI355X the two separate passes
+        // beat the fused kernel, whose table lookups stall on LDS)
+        bool pending_fold = false;
+        uint8_t pending_rho[16], rho0[16];
+        BITZ_REQUIRE(!leaf || L >= 3, BITZ_ERR_SHAPE, "GKR leaf layer: too few variables");
         for (int round = 0; round < L; round++) {
             G z = internal[round];
             int rem = L - 1 - round;
             uint64_t hh = 1ull << rem;
             int s_c = std::min(s, rem);
-            if (eqc_built != s_c || s_c < s) {
+            if (!per_layer) {
                 std::vector<G> c(ext.begin(), ext.begin() + s_c);
+                std::vector<G> yv(ext.begin() + s_c, ext.begin() + rem);
                 if ((rc = eq_dev(c, eqc))) return rc;
-                eqc_built = s_c;
+                if ((rc = eq_dev(yv, eqy))) return rc;
             }
-            std::vector<G> yv(ext.begin() + s_c, ext.begin() + rem);
-            if ((rc = eq_dev(yv, eqy))) return rc;
             int send_one = geq(z, gzero()) ? 1 : 0;
             if (leaf && round == 0)
                 rc = gkr_leaf_round_sums(ctx, (const uint64_t*)packed.ptr, rowimg.ptr, t, s, eqc.ptr, s_c, eqy.ptr,
                                          send_one, nullptr);
-            else
-                rc = gkr_round_sums(ctx, l, r, hh, eqc.ptr, s_c, eqy.ptr, send_one, nullptr);
+            else if (leaf && round == 1)
+                rc = gkr_leaf_round1_sums(ctx, (const uint64_t*)packed.ptr, rowimg.ptr, t, s, rho0, eqc.ptr, eqy.ptr,
+                                          send_one, nullptr);
+            else {
+                if (pending_fold && (rc = fold_pair(ctx, l, r, 2 * hh, pending_rho))) return rc;
+                if (weighted && rem < s) {
+                    // all row variables bound: r holds 2^s entries eq_c[c] * r[c]
+                    if ((rc = mul_inplace(ctx, r, eqcinv.ptr, 1ull << s))) return rc;
+                    weighted = false;
+                }
+                if (weighted) {
+                    rc = gkr_round_sums_rowonly(ctx, l, r, hh, s, eqy.ptr, send_one, nullptr);
+                } else if (weigh && round == 0) {
+                    rc = gkr_premul_round_sums(ctx, l, r, hh, eqc.ptr, s, eqy.ptr, send_one, nullptr);
+                    weighted = true;
+                } else {
+                    rc = gkr_round_sums(ctx, l, r, hh, eqc.ptr, s_c, eqy.ptr, send_one, nullptr);
+                }
+            }
             if (rc) return rc;
             G sums[2];
             if ((rc = result(2, sums))) return rc;
             uint8_t msg[32];
-            gput(hgf_mul(factor, sums[0]), msg);
-            gput(hgf_mul(factor, sums[1]), msg + 16);
+            G f = per_layer ? hgf_mul(factor, corr[round]) : factor;
+            gput(hgf_mul(f, sums[0]), msg);
+            gput(hgf_mul(f, sums[1]), msg + 16);
             it.prover_message(msg, 32);
             uint8_t rb[16];
             it.verifier_gf(rb);
             G rr = gfrom(rb);
             next.push_back(rr);
             if (leaf && round == 0) {
-                if ((rc = gkr_leaf_fold(ctx, (const uint64_t*)packed.ptr, rowimg.ptr, t, s, rb, l, r))) return rc;
+                // leaves stay implicit: round 1 reads the bits again
+                memcpy(rho0, rb, 16);
+                pending_fold = false;
+            } else if (leaf && round == 1) {
+                if ((rc = gkr_leaf_fold2(ctx, (const uint64_t*)packed.ptr, rowimg.ptr, t, s, rho0, rb,
+                                         weigh ? eqc.ptr : nullptr, l, r)))
+                    return rc;
+                weighted = weigh;
+                pending_fold = false;
             } else {
-                if ((rc = fold_halves(ctx, l, hh, rb))) return rc;
-                if ((rc = fold_halves(ctx, r, hh, rb))) return rc;
+                pending_fold = true;
+                memcpy(pending_rho, rb, 16);
             }
             factor = hgf_mul(factor, eq_factor(rr, z));
         }
+        if (pending_fold && (rc = fold_pair(ctx, l, r, 1, pending_rho))) return rc;
         uint8_t last[32];
         if ((rc = download(l, last, 16))) return rc;
         if ((rc = download(r, last + 16, 16))) return rc;
@@ -528,6 +664,8 @@ int ceil_log2(uint64_t n) {
 int Prover::run(const uint8_t* claimed_in, uint8_t* claimed_out, std::vector<uint8_t>* proof_out) {
     int rc;
     const uint64_t nrows = 1ull << t, ncols = 1ull << s;
+    PhaseClock phase(ctx);
+    phase.mark(proof_out ? "statement_round0" : "eval_statement_round0");
     // ---------------- outer transcript: statement and Round 0 ----------------
     Transcript outer;
     outer.absorb_slice(std::string("bitz/ligerito-flock/standalone-mod-q-statement/v1"));
@@ -578,6 +716,7 @@ int Prover::run(const uint8_t* claimed_in, uint8_t* claimed_out, std::vector<uin
     uint8_t yb[16];
     gput(ood_y, yb);
     outer.absorb_slice(yb, 16);
+    phase.mark(proof_out ? "instance" : "eval_instance");
     // ---------------- the standalone instance ----------------
     u128 q;
     if ((rc = sample_prime(outer, q_bits, &q))) return rc;
@@ -616,6 +755,7 @@ int Prover::run(const uint8_t* claimed_in, uint8_t* claimed_out, std::vector<uin
     outer.challenge(tag, 16);
     outer.challenge(tag + 16, 16);
 
+    phase.mark("fold_round");
     // ---------------- BitZ opening on the inner transcript ----------------
     InnerTranscript it("bitz/bitz-opener/standalone/v1", tag, 32);
     it.public_message(h->root, 32);
@@ -668,7 +808,9 @@ int Prover::run(const uint8_t* claimed_in, uint8_t* claimed_out, std::vector<uin
     // GKR (reduce.rs:100-114)
     std::vector<G> leaf_point;
     G leaf_claim;
+    phase.mark("gkr");
     if ((rc = gkr(it, zeta, &leaf_point, &leaf_claim))) return rc;
+    phase.mark("inner_sumcheck");
     levels = DevBuf();  // release the product tree
     // exit_from_terminal (reduce.rs:28-58)
     std::vector<G> alpha_c(leaf_point.begin(), leaf_point.begin() + s);
@@ -737,6 +879,7 @@ int Prover::run(const uint8_t* claimed_in, uint8_t* claimed_out, std::vector<uin
         }
         it.public_message(evb, 16);
     }
+    phase.mark("ring_switch");
     // ---------------- ring switch (prove_mle, src/bitz/pcs.rs:439-507) ----------------
     std::vector<G> prefix_pt(sc_point.begin(), sc_point.begin() + 7);
     std::vector<G> suffix_pt(sc_point.begin() + 7, sc_point.end());
@@ -802,6 +945,7 @@ int Prover::run(const uint8_t* claimed_in, uint8_t* claimed_out, std::vector<uin
     }
     suffix = DevBuf();
 
+    phase.mark("ligerito");
     // ---------------- Ligerito (recursive_prover_with_basis) ----------------
     Challenger ch{it, ctx};
     ch.observe_label("flock-ligerito-basis-v0");
@@ -1004,6 +1148,7 @@ int Prover::run(const uint8_t* claimed_in, uint8_t* claimed_out, std::vector<uin
         std::swap(tree_prev, tree_next);
         prev_level = i + 2;
     }
+    phase.mark("encode");
     // ---------------- LigeritoProof as a hint (write_opening_proof) ----------------
     Writer w;
     w.bytes(h->root, 32);
@@ -1064,6 +1209,7 @@ int make_prover(bitz_ctx* ctx, const bitz_commit_handle* h, int target_bits, Pro
 extern "C" int bitz_standalone_evaluation(bitz_ctx* ctx, bitz_commit_handle* h, uint8_t claimed_out[16]) {
     BITZ_REQUIRE(ctx && h && claimed_out, BITZ_ERR_ARG, "bitz_standalone_evaluation: null argument");
     BITZ_HIP(hipSetDevice(ctx->device));
+    bitz::PoolScope scope(&ctx->pool);
     bitz::Prover p;
     int rc = bitz::make_prover(ctx, h, 100, &p);
     if (rc) return rc;
@@ -1074,6 +1220,7 @@ extern "C" int bitz_prove_standalone(bitz_ctx* ctx, bitz_commit_handle* h, const
                                      uint8_t* proof, uint64_t capacity, uint64_t* proof_len) {
     BITZ_REQUIRE(ctx && h && claimed && proof_len, BITZ_ERR_ARG, "bitz_prove_standalone: null argument");
     BITZ_HIP(hipSetDevice(ctx->device));
+    bitz::PoolScope scope(&ctx->pool);
     bitz::Prover p;
     int rc = bitz::make_prover(ctx, h, 100, &p);
     if (rc) return rc;

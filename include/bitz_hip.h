@@ -49,6 +49,14 @@ void bitz_ctx_destroy(bitz_ctx* ctx);
 int bitz_ctx_set_stream(bitz_ctx* ctx, void* hip_stream);
 void* bitz_ctx_stream(bitz_ctx* ctx);
 int bitz_ctx_synchronize(bitz_ctx* ctx);
+/* Device memory of a context is served by a caching pool (blocks freed by
+ * one call are reused by the next, in stream order; cf. flock's codeword
+ * scratch pool, vendor/flock-mod/crates/flock-core/src/pcs/commit.rs:157-163).
+ * bitz_ctx_trim drains the stream and returns every cached block to HIP;
+ * bitz_ctx_cached_bytes reports the bytes currently cached. Changing the
+ * stream with bitz_ctx_set_stream drains the old stream first. */
+int bitz_ctx_trim(bitz_ctx* ctx);
+uint64_t bitz_ctx_cached_bytes(bitz_ctx* ctx);
 const char* bitz_last_error(void);
 /* Per-kernel timing: with enable != 0, instrumented launches are bracketed by
  * HIP events on the context's stream (stats reset). bitz_ctx_kernel_stats
@@ -111,10 +119,39 @@ int bitz_pack_columns_dev(bitz_ctx* ctx, const void* rows, int t, int s, void* p
 int bitz_gkr_round_sums_dev(bitz_ctx* ctx, const void* l, const void* r, uint64_t h, const void* eq_c,
                             int s_c, const void* eq_y, int send_one, uint8_t out_host[32]);
 /* the same sums on the leaf layer, leaves read from the bits and row images
- * (Forest::bit_round, src/bitz/forest.rs:381-520) */
+ * (Forest::bit_rounds, src/bitz/forest.rs:381-423) */
 int bitz_gkr_leaf_round_sums_dev(bitz_ctx* ctx, const void* packed, const void* A, int t, int s,
                                  const void* eq_c, int s_c, const void* eq_y, int send_one,
                                  uint8_t out_host[32]);
+/* round 1 of the leaf layer after round 0 folded by rho0, from 8-bit row
+ * patterns (Forest::one_pass_bit_rounds, src/bitz/forest.rs:424-582); weights
+ * eq_y[y] (2^(t-3) entries) * eq_c[c] (2^s entries) */
+int bitz_gkr_leaf_round1_sums_dev(bitz_ctx* ctx, const void* packed, const void* A, int t, int s,
+                                  const uint8_t rho0[16], const void* eq_c, const void* eq_y, int send_one,
+                                  uint8_t out_host[32]);
+/* the leaf layer folded by rho0 and then rho1 into halves of 2^(t+s-3) entries;
+ * a non-null eq_c (2^s entries) is multiplied into the right half
+ * (jit_fold_round_with carrying the weights, src/bitz/forest.rs:635-650) */
+int bitz_gkr_leaf_fold2_dev(bitz_ctx* ctx, const void* packed, const void* A, int t, int s, const uint8_t rho0[16],
+                            const uint8_t rho1[16], const void* eq_c, void* l_out, void* r_out);
+/* Round sums with the column weights carried by the right half (Left::Weigh /
+ * Left::Weighted, src/bitz/kernels.rs:114-131; the reference carries them in
+ * the left half, the sums are the same). premul: r[x], r[x+h] are replaced
+ * in place by eq_c[x & (2^s-1)] * r[.] and the sums use weights eq_y[x >> s]
+ * over the new values; rowonly: r already carries eq_c. */
+int bitz_gkr_premul_round_sums_dev(bitz_ctx* ctx, const void* l, void* r, uint64_t h, const void* eq_c, int s,
+                                   const void* eq_y, int send_one, uint8_t out_host[32]);
+int bitz_gkr_round_sums_rowonly_dev(bitz_ctx* ctx, const void* l, const void* r, uint64_t h, int s,
+                                    const void* eq_y, int send_one, uint8_t out_host[32]);
+/* out[x] = prod_i (bit i of x ? b[i] : a[i]), x < 2^n (n <= 14): the inverse
+ * column-weight table that removes carried weights (src/bitz/gkr.rs:56-80) */
+int bitz_tensor_table_dev(bitz_ctx* ctx, const void* a, const void* b, int n, void* out);
+/* fused_fold_round (src/bitz/kernels.rs:146-200): l and r hold the previous
+ * round's halves (4h entries each); both are folded by rho in place into 2h
+ * entries (quarters q0 + rho (q2 - q0), q1 + rho (q3 - q1)) and this round's
+ * sums over the folded halves (h slots, weights as above) are returned */
+int bitz_gkr_fold_round_sums_dev(bitz_ctx* ctx, void* l, void* r, uint64_t h, const uint8_t rho[16],
+                                 const void* eq_c, int s_c, const void* eq_y, int send_one, uint8_t out_host[32]);
 /* fold_halves (src/bitz/gkr.rs:170-177): v[i] += rho (v[i+h] - v[i]), i < h */
 int bitz_fold_halves_dev(bitz_ctx* ctx, void* v, uint64_t h, const uint8_t rho[16]);
 /* the leaf layer folded once by rho into materialised halves (2^(t+s-2) each) */

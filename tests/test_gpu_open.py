@@ -109,6 +109,24 @@ def test_gkr_round_sums(log_h, s_c, send_one):
     assert np.array_equal(to_host(v)[:h], O.fold_halves(l, rho))
 
 
+@pytest.mark.parametrize("log_h,s_c", [(0, 0), (3, 3), (5, 2), (10, 4), (17, 11)])
+@pytest.mark.parametrize("send_one", [False, True])
+def test_gkr_fold_round_sums(log_h, s_c, send_one):
+    h = 1 << log_h
+    l, r = rand_gf(4 * h, log_h + 3), rand_gf(4 * h, log_h + 200)
+    point = rand_ints(log_h, 17)
+    eq_c, eq_y = _eq_parts(point, s_c)
+    rho = rand_ints(1, 23)[0]
+    log(f"gkr_fold_round_sums h=2^{log_h} s_c={s_c} send_one={send_one}")
+    dl, dr = to_dev(l), to_dev(r)
+    got = ops().gkr_fold_round_sums_(dl, dr, h, rho, to_dev(eq_c), s_c, to_dev(eq_y), send_one)
+    fl, fr = O.fold_halves(l, rho), O.fold_halves(r, rho)
+    assert np.array_equal(to_host(dl)[:2 * h], fl)
+    assert np.array_equal(to_host(dr)[:2 * h], fr)
+    full_eq = O.eq_table(O.ints_to_gf(point))
+    assert tuple(got) == O.gkr_round_sums(fl, fr, full_eq, send_one)
+
+
 @pytest.mark.parametrize("t,s", [(7, 0), (7, 3), (9, 6), (10, 8), (15, 11)])
 def test_gkr_leaf_layer(t, s):
     rows = rand_rows(t, s, 11)
@@ -150,6 +168,100 @@ def test_gkr_leaf_layer(t, s):
                 x = (y << s) | c
                 assert O.gf_to_ints(glh[x:x + 1])[0] == e
                 assert O.gf_to_ints(grh[x:x + 1])[0] == o
+
+
+@pytest.mark.parametrize("t,s", [(7, 0), (7, 3), (9, 6), (10, 8), (15, 11)])
+def test_gkr_leaf_round1(t, s):
+    """round 1 of the leaf layer and the double fold, from bit patterns: against
+    the oracle's dense fold + round sums on small shapes, and against the
+    device's pinned dense kernels (round-0 fold, fold_halves, round sums) at
+    the full-size shape"""
+    rows = rand_rows(t, s, 31)
+    A = rand_gf(1 << t, 32)
+    A[::5] = [1, 0]
+    packed = ops().pack_columns(dev_u64(rows), t, s)
+    dA = to_dev(A)
+    rho0, rho1 = rand_ints(2, 33)
+    point = rand_ints(t + s - 3, 34)
+    eq_c, eq_y = _eq_parts(point, s)
+    log(f"gkr leaf round 1 t={t} s={s}")
+    h1 = 1 << (t + s - 3)
+    if t + s <= 20:
+        leaves = O.forest_leaves(rows, t, s, A)
+        half = 1 << (t + s - 1)
+        l1, r1 = O.fold_halves(leaves[:half], rho0), O.fold_halves(leaves[half:], rho0)
+        full_eq = O.eq_table(O.ints_to_gf(point))
+        for send_one in (False, True):
+            got = ops().gkr_leaf_round1_sums(packed, dA, t, s, rho0, to_dev(eq_c), to_dev(eq_y), send_one)
+            assert tuple(got) == O.gkr_round_sums(l1, r1, full_eq, send_one)
+        gl, gr = ops().gkr_leaf_fold2(packed, dA, t, s, rho0, rho1)
+        assert np.array_equal(to_host(gl), O.fold_halves(l1, rho1))
+        assert np.array_equal(to_host(gr), O.fold_halves(r1, rho1))
+    else:
+        dl, dr = ops().gkr_leaf_fold(packed, dA, t, s, rho0)
+        for send_one in (False, True):
+            want = ops().gkr_round_sums(dl, dr, h1, to_dev(eq_c), s, to_dev(eq_y), send_one)
+            got = ops().gkr_leaf_round1_sums(packed, dA, t, s, rho0, to_dev(eq_c), to_dev(eq_y), send_one)
+            assert tuple(got) == tuple(want)
+        gl, gr = ops().gkr_leaf_fold2(packed, dA, t, s, rho0, rho1)
+        ops().fold_halves_(dl, h1, rho1)
+        ops().fold_halves_(dr, h1, rho1)
+        assert np.array_equal(to_host(gl), to_host(dl)[:h1])
+        assert np.array_equal(to_host(gr), to_host(dr)[:h1])
+
+
+@pytest.mark.parametrize("log_h,s", [(1, 1), (4, 2), (10, 6), (18, 11)])
+@pytest.mark.parametrize("send_one", [False, True])
+def test_gkr_carried_column_weights(log_h, s, send_one):
+    """premul: r <- eq_c * r in place and the same sums as the plain round;
+    rowonly over the weighted halves gives them again; after a row fold the
+    weighted round still equals the plain round over the folded halves"""
+    h = 1 << log_h
+    l, r = rand_gf(2 * h, log_h + 40), rand_gf(2 * h, log_h + 41)
+    point = rand_ints(log_h, 42)
+    eq_c, eq_y = _eq_parts(point, s)
+    full_eq = O.eq_table(O.ints_to_gf(point))
+    want = O.gkr_round_sums(l, r, full_eq, send_one)
+    log(f"carried column weights h=2^{log_h} s={s} send_one={send_one}")
+    dl, dr = to_dev(l), to_dev(r)
+    assert tuple(ops().gkr_premul_round_sums_(dl, dr, h, to_dev(eq_c), s, to_dev(eq_y), send_one)) == want
+    weights = np.tile(eq_c, (2 * h >> s, 1))
+    assert np.array_equal(to_host(dr), O.mul_arrays(r, weights))
+    assert tuple(ops().gkr_round_sums_rowonly(dl, dr, h, s, to_dev(eq_y), send_one)) == want
+    if log_h > s:
+        rho = rand_ints(1, 43)[0]
+        fl, fr = O.fold_halves(l, rho), O.fold_halves(r, rho)
+        ops().fold_halves_(dl, h, rho)
+        ops().fold_halves_(dr, h, rho)
+        sub = point[:log_h - 1]
+        c2, y2 = _eq_parts(sub, s)
+        want2 = O.gkr_round_sums(fl, fr, O.eq_table(O.ints_to_gf(sub)), send_one)
+        assert tuple(ops().gkr_round_sums_rowonly(dl, dr, h // 2, s, to_dev(y2), send_one)) == want2
+
+
+@pytest.mark.parametrize("n", [0, 1, 5, 13])
+def test_tensor_table(n):
+    a, b = rand_ints(n, 50), rand_ints(n, 51)
+    got = O.gf_to_ints(to_host(ops().tensor_table(to_dev(O.ints_to_gf(a or [0])), to_dev(O.ints_to_gf(b or [0])), n)))
+    for x in ([0, 1, (1 << n) - 1, (1 << n) // 3] if n else [0]):
+        want = 1
+        for i in range(n):
+            want = O.mul(want, b[i] if (x >> i) & 1 else a[i])
+        assert got[x] == want
+
+
+@pytest.mark.parametrize("t,s", [(7, 3), (10, 8)])
+def test_gkr_leaf_fold2_carrying_weights(t, s):
+    rows = rand_rows(t, s, 61)
+    A = rand_gf(1 << t, 62)
+    packed = ops().pack_columns(dev_u64(rows), t, s)
+    dA = to_dev(A)
+    rho0, rho1 = rand_ints(2, 63)
+    eq_c = O.eq_table(O.ints_to_gf(rand_ints(s, 64)))
+    pl, pr = ops().gkr_leaf_fold2(packed, dA, t, s, rho0, rho1)
+    wl, wr = ops().gkr_leaf_fold2(packed, dA, t, s, rho0, rho1, to_dev(eq_c))
+    assert np.array_equal(to_host(wl), to_host(pl))
+    assert np.array_equal(to_host(wr), O.mul_arrays(to_host(pr), np.tile(eq_c, (1 << (t - 3), 1))))
 
 
 def test_product_level():

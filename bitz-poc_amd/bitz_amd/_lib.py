@@ -59,6 +59,9 @@ SIGNATURES = {
     "bitz_gkr_premul_round_sums_dev": (_I, [_P, _P, _P, _U64, _P, _I, _P, _I, _P]),
     "bitz_gkr_round_sums_rowonly_dev": (_I, [_P, _P, _P, _U64, _I, _P, _I, _P]),
     "bitz_tensor_table_dev": (_I, [_P, _P, _P, _I, _P]),
+    "bitz_gkr_level2_dev": (_I, [_P, _P, _P, _I, _I, _P]),
+    "bitz_gkr_level1_round0_sums_dev": (_I, [_P, _P, _P, _I, _I, _P, _P, _I, _P]),
+    "bitz_gkr_level1_fold_dev": (_I, [_P, _P, _P, _I, _I, _P, _P, _P, _P]),
     "bitz_gkr_leaf_fold_dev": (_I, [_P, _P, _P, _I, _I, _P, _P, _P]),
     "bitz_gkr_level1_dev": (_I, [_P, _P, _P, _I, _I, _P]),
     "bitz_product_level_dev": (_I, [_P, _P, _U64, _P]),
@@ -84,6 +87,7 @@ SIGNATURES = {
     "bitz_prove_standalone": (_I, [_P, _P, _P, _P, _U64, C.POINTER(_U64)]),
     "bitz_ligerito_config": (_I, [_U32, _U32, _U32, _U32, _P, _U32]),
     "bitz_host_blake3": (None, [_P, _U64, _P]),
+    "bitz_host_blake3_parts": (None, [_P, _U64, _U64, _P]),
 }
 
 _lib = None

@@ -211,7 +211,8 @@ class OpenOps:
     def gkr_fold_round_sums_(self, l, r, h, rho, eq_c, s_c, eq_y, send_one):
         """folds l and r (4h entries) in place to 2h and returns the round sums"""
         buf = (C.c_uint8 * 32)()
-        _lib.call("bitz_gkr_fold_round_sums_dev", self.ctx.ptr, _ptr(l), _ptr(r), h, _vp(_b16(rho)), _ptr(eq_c),
+        _lib.call("bitz_gkr_fold_round_sums_dev", self.ctx.ptr, _ptr(l), _ptr(r), h, _vp(_b16(rho)),
+                  _ptr(eq_c) if eq_c is not None else None,
                   s_c, _ptr(eq_y), 1 if send_one else 0, _vp(buf))
         return _ints(buf, 2)
 
@@ -240,6 +241,24 @@ class OpenOps:
         _lib.call("bitz_gkr_round_sums_rowonly_dev", self.ctx.ptr, _ptr(l), _ptr(r), h, s, _ptr(eq_y),
                   1 if send_one else 0, _vp(buf))
         return _ints(buf, 2)
+
+    def gkr_level2(self, packed, A, t, s):
+        out = self._gf(1 << (t + s - 2))
+        _lib.call("bitz_gkr_level2_dev", self.ctx.ptr, _ptr(packed), _ptr(A), t, s, _ptr(out))
+        return out
+
+    def gkr_level1_round0_sums(self, packed, A, t, s, eq_c, eq_y, send_one):
+        buf = (C.c_uint8 * 32)()
+        _lib.call("bitz_gkr_level1_round0_sums_dev", self.ctx.ptr, _ptr(packed), _ptr(A), t, s, _ptr(eq_c),
+                  _ptr(eq_y), 1 if send_one else 0, _vp(buf))
+        return _ints(buf, 2)
+
+    def gkr_level1_fold(self, packed, A, t, s, rho, eq_c=None):
+        n = 1 << (t + s - 3)
+        l, r = self._gf(n), self._gf(n)
+        _lib.call("bitz_gkr_level1_fold_dev", self.ctx.ptr, _ptr(packed), _ptr(A), t, s, _vp(_b16(rho)),
+                  _ptr(eq_c) if eq_c is not None else None, _ptr(l), _ptr(r))
+        return l, r
 
     def tensor_table(self, a, b, n):
         out = self._gf(1 << n)

@@ -71,8 +71,7 @@ extern "C" int bitz_gkr_fold_round_sums_dev(bitz_ctx* ctx, void* l, void* r, uin
                                             const void* eq_c, int s_c, const void* eq_y, int send_one,
                                             uint8_t out_host[32]) {
     PRELUDE();
-    BITZ_REQUIRE(l && r && rho && eq_c && eq_y && out_host, BITZ_ERR_ARG,
-                 "bitz_gkr_fold_round_sums_dev: null argument");
+    BITZ_REQUIRE(l && r && rho && eq_y && out_host, BITZ_ERR_ARG, "bitz_gkr_fold_round_sums_dev: null argument");
     BITZ_REQUIRE(h > 0 && s_c >= 0 && (1ull << s_c) <= h && h % (1ull << s_c) == 0, BITZ_ERR_SHAPE,
                  "bitz_gkr_fold_round_sums_dev: eq_c must tile the half");
     CALL(bitz::gkr_fold_round_sums(ctx, l, r, h, rho, eq_c, s_c, eq_y, send_one, nullptr));
@@ -146,6 +145,32 @@ extern "C" int bitz_gkr_leaf_fold_dev(bitz_ctx* ctx, const void* packed, const v
     PRELUDE();
     BITZ_REQUIRE(packed && A && rho && l_out && r_out, BITZ_ERR_ARG, "bitz_gkr_leaf_fold_dev: null argument");
     return bitz::gkr_leaf_fold(ctx, (const uint64_t*)packed, A, t, s, rho, l_out, r_out);
+}
+
+extern "C" int bitz_gkr_level2_dev(bitz_ctx* ctx, const void* packed, const void* A, int t, int s, void* out) {
+    PRELUDE();
+    BITZ_REQUIRE(packed && A && out, BITZ_ERR_ARG, "bitz_gkr_level2_dev: null argument");
+    BITZ_REQUIRE(t >= 2 && s >= 0 && t + s <= 40, BITZ_ERR_SHAPE, "bitz_gkr_level2_dev: bad shape");
+    return bitz::gkr_level2(ctx, (const uint64_t*)packed, A, t, s, out);
+}
+
+extern "C" int bitz_gkr_level1_round0_sums_dev(bitz_ctx* ctx, const void* packed, const void* A, int t, int s,
+                                               const void* eq_c, const void* eq_y, int send_one,
+                                               uint8_t out_host[32]) {
+    PRELUDE();
+    BITZ_REQUIRE(packed && A && eq_c && eq_y && out_host, BITZ_ERR_ARG,
+                 "bitz_gkr_level1_round0_sums_dev: null argument");
+    BITZ_REQUIRE(t >= 3 && s >= 0 && t + s <= 40, BITZ_ERR_SHAPE, "bitz_gkr_level1_round0_sums_dev: bad shape");
+    CALL(bitz::gkr_level1_round0_sums(ctx, (const uint64_t*)packed, A, t, s, eq_c, eq_y, send_one, nullptr));
+    return bitz::read_result(ctx, 2, out_host);
+}
+
+extern "C" int bitz_gkr_level1_fold_dev(bitz_ctx* ctx, const void* packed, const void* A, int t, int s,
+                                        const uint8_t rho[16], const void* eq_c, void* l_out, void* r_out) {
+    PRELUDE();
+    BITZ_REQUIRE(packed && A && rho && l_out && r_out, BITZ_ERR_ARG, "bitz_gkr_level1_fold_dev: null argument");
+    BITZ_REQUIRE(t >= 3 && s >= 0 && t + s <= 40, BITZ_ERR_SHAPE, "bitz_gkr_level1_fold_dev: bad shape");
+    return bitz::gkr_level1_fold(ctx, (const uint64_t*)packed, A, t, s, rho, eq_c, l_out, r_out);
 }
 
 extern "C" int bitz_gkr_level1_dev(bitz_ctx* ctx, const void* packed, const void* A, int t, int s, void* out) {

@@ -4,16 +4,32 @@
 #include <math.h>
 
 #include <algorithm>
+#include <thread>
 
 namespace bitz {
 
 // ------------------------------------------------------------------ BLAKE3
 static const uint32_t kIV[8] = {0x6A09E667u, 0xBB67AE85u, 0x3C6EF372u, 0xA54FF53Au,
                                 0x510E527Fu, 0x9B05688Cu, 0x1F83D9ABu, 0x5BE0CD19u};
-static const uint8_t kPerm[16] = {2, 6, 3, 10, 7, 0, 4, 13, 1, 11, 12, 5, 9, 14, 15, 8};
+static constexpr uint8_t kPerm[16] = {2, 6, 3, 10, 7, 0, 4, 13, 1, 11, 12, 5, 9, 14, 15, 8};
 enum { kChunkStart = 1, kChunkEnd = 2, kParent = 4, kRoot = 8 };
 
 static inline uint32_t rotr32(uint32_t x, int n) { return (x >> n) | (x << (32 - n)); }
+
+// message word order of each round: round r reads m[kSched.idx[r][i]] where
+// idx[r+1][i] = idx[r][kPerm[i]] (a compile-time table, so the unrolled
+// rounds index the message words with constants)
+struct Sched {
+    uint8_t idx[7][16];
+};
+static constexpr Sched make_sched() {
+    Sched s{};
+    for (int i = 0; i < 16; i++) s.idx[0][i] = (uint8_t)i;
+    for (int r = 1; r < 7; r++)
+        for (int i = 0; i < 16; i++) s.idx[r][i] = s.idx[r - 1][kPerm[i]];
+    return s;
+}
+static constexpr Sched kSched = make_sched();
 
 void blake3_compress_host(const uint32_t cv[8], const uint8_t block[64], uint64_t counter, uint32_t block_len,
                           uint32_t flags, uint32_t out[16]) {
@@ -40,18 +56,17 @@ void blake3_compress_host(const uint32_t cv[8], const uint8_t block[64], uint64_
         s[c] = s[c] + s[d];
         s[b] = rotr32(s[b] ^ s[c], 7);
     };
+#pragma GCC unroll 7
     for (int r = 0; r < 7; r++) {
-        g(0, 4, 8, 12, m[0], m[1]);
-        g(1, 5, 9, 13, m[2], m[3]);
-        g(2, 6, 10, 14, m[4], m[5]);
-        g(3, 7, 11, 15, m[6], m[7]);
-        g(0, 5, 10, 15, m[8], m[9]);
-        g(1, 6, 11, 12, m[10], m[11]);
-        g(2, 7, 8, 13, m[12], m[13]);
-        g(3, 4, 9, 14, m[14], m[15]);
-        uint32_t t[16];
-        for (int i = 0; i < 16; i++) t[i] = m[kPerm[i]];
-        memcpy(m, t, sizeof m);
+        const uint8_t* q = kSched.idx[r];
+        g(0, 4, 8, 12, m[q[0]], m[q[1]]);
+        g(1, 5, 9, 13, m[q[2]], m[q[3]]);
+        g(2, 6, 10, 14, m[q[4]], m[q[5]]);
+        g(3, 7, 11, 15, m[q[6]], m[q[7]]);
+        g(0, 5, 10, 15, m[q[8]], m[q[9]]);
+        g(1, 6, 11, 12, m[q[10]], m[q[11]]);
+        g(2, 7, 8, 13, m[q[12]], m[q[13]]);
+        g(3, 4, 9, 14, m[q[14]], m[q[15]]);
     }
     for (int i = 0; i < 8; i++) {
         out[i] = s[i] ^ s[i + 8];
@@ -90,8 +105,56 @@ void Blake3::push_cv(uint32_t cv[8], uint64_t total_chunks) {
     memcpy(stack_[stack_len_++], cur, 32);
 }
 
+// CV of one whole, non-final 1024-byte chunk
+static void chunk_cv(const uint8_t* chunk, uint64_t counter, uint32_t cv[8]) {
+    memcpy(cv, kIV, 32);
+    for (int b = 0; b < 16; b++) {
+        uint32_t out[16];
+        blake3_compress_host(cv, chunk + 64 * b, counter, 64, (b == 0 ? kChunkStart : 0) | (b == 15 ? kChunkEnd : 0),
+                             out);
+        memcpy(cv, out, 32);
+    }
+}
+
 void Blake3::update(const void* data, size_t len) {
     const uint8_t* p = (const uint8_t*)data;
+    // Long inputs: once at a chunk boundary, the whole chunks that more input
+    // follows are independent (a chunk's CV depends on its bytes and counter
+    // only), so their CVs are computed on several threads and pushed in order.
+    const size_t kBulkChunks = 64;
+    if (len >= (kBulkChunks + 1) * 1024) {
+        size_t lead = chunk_len() == 1024 ? 0 : (1024 - chunk_len()) % 1024;
+        if (chunk_len() == 0) lead = 0;
+        if (lead) {
+            update(p, lead);
+            p += lead;
+            len -= lead;
+        }
+        if (chunk_len() == 1024) {  // finish the buffered chunk: input follows
+            uint32_t out[16];
+            blake3_compress_host(chunk_.cv, chunk_.buf, chunk_.counter, chunk_.buf_len,
+                                 kChunkEnd | (chunk_.blocks == 0 ? kChunkStart : 0), out);
+            uint64_t total = chunk_.counter + 1;
+            push_cv(out, total);
+            chunk_reset(total);
+        }
+        size_t m = (len - 1) / 1024;  // keep at least one byte for the buffered chunk
+        if (m >= kBulkChunks && chunk_len() == 0) {
+            std::vector<uint32_t> cvs(m * 8);
+            const uint64_t c0 = chunk_.counter;
+            unsigned nt = std::max(1u, std::min(8u, std::thread::hardware_concurrency()));
+            std::vector<std::thread> pool;
+            for (unsigned w = 0; w < nt; w++)
+                pool.emplace_back([&, w]() {
+                    for (size_t i = w; i < m; i += nt) chunk_cv(p + 1024 * i, c0 + i, &cvs[8 * i]);
+                });
+            for (auto& th : pool) th.join();
+            for (size_t i = 0; i < m; i++) push_cv(&cvs[8 * i], c0 + i + 1);
+            chunk_reset(c0 + m);
+            p += 1024 * m;
+            len -= 1024 * m;
+        }
+    }
     while (len > 0) {
         if (chunk_len() == 1024) {
             // finish the chunk (its last block is still buffered)

@@ -433,11 +433,16 @@ __global__ __launch_bounds__(kT) void k_leaf_gtab(const uint4* __restrict__ ftab
     gf_store(&gtab[idx], gf_add(lo, gf_mul_tab(tab, gf_add(lo, hi))));
 }
 
-// eq_c != nullptr: the right half is written multiplied by eq_c[c]
-__global__ __launch_bounds__(kT) void k_leaf_fold2_pattern(const uint64_t* __restrict__ packed, int t, int s,
-                                                           const uint4* __restrict__ gtab,
-                                                           const uint4* __restrict__ eq_c, uint4* __restrict__ l_out,
-                                                           uint4* __restrict__ r_out) {
+// Folded halves written from 8-bit row patterns (rows y + k q8, q8 =
+// 2^(t-3)): l_out[x] = gtab[y*32 + lsub], r_out[x] = gtab[y*32 + 16 + rsub].
+// Layout 0 (leaf layer after two folds): lsub = bits 0-3, rsub = bits 4-7.
+// Layout 1 (level-1 layer after one fold): lsub = bits 0,1,4,5 and
+// rsub = bits 2,3,6,7. eq_c != nullptr: the right half is written multiplied
+// by eq_c[c].
+template <int Layout>
+__global__ __launch_bounds__(kT) void k_pattern_fold(const uint64_t* __restrict__ packed, int t, int s,
+                                                     const uint4* __restrict__ gtab, const uint4* __restrict__ eq_c,
+                                                     uint4* __restrict__ l_out, uint4* __restrict__ r_out) {
     const uint64_t q8 = 1ull << (t - 3), nrows = 1ull << t, h = 1ull << (t + s - 3);
     for (uint64_t x = grid_tid(); x < h; x += grid_threads()) {
         uint64_t y1 = x >> s, c = x & ((1ull << s) - 1);
@@ -446,11 +451,98 @@ __global__ __launch_bounds__(kT) void k_leaf_fold2_pattern(const uint64_t* __res
         uint32_t pat = 0;
 #pragma unroll
         for (int k = 0; k < 8; k++) pat |= (uint32_t)((col[y1 + k * q8] >> bit) & 1ull) << k;
-        l_out[x] = gtab[y1 * 32 + (pat & 15)];
+        uint32_t lsub = Layout == 0 ? (pat & 15) : ((pat & 3) | ((pat >> 2) & 12));
+        uint32_t rsub = Layout == 0 ? (pat >> 4) : (((pat >> 2) & 3) | ((pat >> 4) & 12));
+        l_out[x] = gtab[y1 * 32 + lsub];
         if (eq_c)
-            gf_store(&r_out[x], gf_mul(gf_load(&gtab[y1 * 32 + 16 + (pat >> 4)]), gf_load(&eq_c[c])));
+            gf_store(&r_out[x], gf_mul(gf_load(&gtab[y1 * 32 + 16 + rsub]), gf_load(&eq_c[c])));
         else
-            r_out[x] = gtab[y1 * 32 + 16 + (pat >> 4)];
+            r_out[x] = gtab[y1 * 32 + 16 + rsub];
+    }
+}
+
+// ---- level 1 by bit patterns (src/bitz/forest.rs prove_jit_level) ----
+// Level-1 entry (y, c), y < Y1 = 2^(t-1), is one of vtab[y*4 + p]: 1, A[y],
+// A[y+Y1], A[y] A[y+Y1] for p = bit(c, y) | bit(c, y+Y1) << 1.
+__global__ void k_level1_vtab(const uint4* __restrict__ A, int t, uint4* __restrict__ vtab) {
+    const uint64_t Y1 = 1ull << (t - 1);
+    uint64_t y = grid_tid();
+    if (y >= Y1) return;
+    gf128 a = gf_load(&A[y]), b = gf_load(&A[y + Y1]);
+    gf_store(&vtab[y * 4], gf_one());
+    gf_store(&vtab[y * 4 + 1], a);
+    gf_store(&vtab[y * 4 + 2], b);
+    gf_store(&vtab[y * 4 + 3], gf_mul(a, b));
+}
+
+// The layer over level 1 (halves: rows [0, Y2) and [Y2, Y1), Y2 = 2^(t-2)):
+// a round-0 slot y < Y3 = 2^(t-3) meets level-1 rows y, y+Y3 (left) and
+// y+Y2, y+Y2+Y3 (right); with pattern bit k = bit(c, y + k Y3) their
+// values are vtab[row][bits (k, k+4)] for k = 0, 1, 2, 3.
+__device__ __forceinline__ void level1_round0_vals(const uint4* vtab, uint64_t y, uint64_t Y3, uint32_t pat,
+                                                   gf128* el, gf128* eh, gf128* ol, gf128* oh) {
+    *el = gf_load(&vtab[y * 4 + ((pat & 1) | ((pat >> 3) & 2))]);
+    *eh = gf_load(&vtab[(y + Y3) * 4 + (((pat >> 1) & 1) | ((pat >> 4) & 2))]);
+    *ol = gf_load(&vtab[(y + 2 * Y3) * 4 + (((pat >> 2) & 1) | ((pat >> 5) & 2))]);
+    *oh = gf_load(&vtab[(y + 3 * Y3) * 4 + (((pat >> 3) & 1) | ((pat >> 6) & 2))]);
+}
+
+__global__ void k_level1_qtab(const uint4* __restrict__ vtab, int t, const uint4* __restrict__ eq_y, int send_one,
+                              uint4* __restrict__ qtab) {
+    const uint64_t Y3 = 1ull << (t - 3);
+    uint64_t idx = grid_tid();
+    if (idx >= Y3 * 256) return;
+    uint64_t y = idx >> 8;
+    gf128 el, eh, ol, oh;
+    level1_round0_vals(vtab, y, Y3, (uint32_t)(idx & 255), &el, &eh, &ol, &oh);
+    gf128 wy = gf_load(&eq_y[y]);
+    gf128 pe = send_one ? gf_mul(eh, oh) : gf_mul(el, ol);
+    gf128 pi = gf_mul(gf_add(el, eh), gf_add(ol, oh));
+    gf_store(&qtab[idx * 2], gf_mul(pe, wy));
+    gf_store(&qtab[idx * 2 + 1], gf_mul(pi, wy));
+}
+
+// gtab[y*32 + half*16 + sub] = lo + rho (lo + hi) for the level-1 layer's
+// round-0 fold; sub = (s0, s1, s2, s3) are pattern bits (k, k+1, k+4, k+5)
+// with k = 2*half, lo = vtab[row][s0 | s2 << 1], hi = vtab[row + Y3][s1 | s3 << 1].
+__global__ __launch_bounds__(kT) void k_level1_gtab(const uint4* __restrict__ vtab, int t, uint64_t rho_lo,
+                                                    uint64_t rho_hi, uint4* __restrict__ gtab) {
+    __shared__ uint4 tab[512];
+    gf_table_build(tab, gf_from_u64(rho_lo, rho_hi), threadIdx.x, blockDim.x);
+    __syncthreads();
+    const uint64_t Y3 = 1ull << (t - 3);
+    uint64_t idx = grid_tid();
+    if (idx >= Y3 * 32) return;
+    uint64_t y = idx >> 5;
+    uint32_t half = (uint32_t)((idx >> 4) & 1), sub = (uint32_t)(idx & 15);
+    uint64_t row = y + 2 * half * Y3;
+    gf128 lo = gf_load(&vtab[row * 4 + ((sub & 1) | ((sub >> 1) & 2))]);
+    gf128 hi = gf_load(&vtab[(row + Y3) * 4 + (((sub >> 1) & 1) | ((sub >> 2) & 2))]);
+    gf_store(&gtab[idx], gf_add(lo, gf_mul_tab(tab, gf_add(lo, hi))));
+}
+
+// Level 2 from the bits: entry (y, c), y < Y2, is ltab[y*16 + p] =
+// vtab[y][p & 3] * vtab[y + Y2][p >> 2] with p = bits of rows y, y+Y1,
+// y+Y2, y+Y2+Y1.
+__global__ void k_level2_ltab(const uint4* __restrict__ vtab, int t, uint4* __restrict__ ltab) {
+    const uint64_t Y2 = 1ull << (t - 2);
+    uint64_t idx = grid_tid();
+    if (idx >= Y2 * 16) return;
+    uint64_t y = idx >> 4;
+    uint32_t p = (uint32_t)(idx & 15);
+    gf_store(&ltab[idx], gf_mul(gf_load(&vtab[y * 4 + (p & 3)]), gf_load(&vtab[(y + Y2) * 4 + (p >> 2)])));
+}
+
+__global__ __launch_bounds__(kT) void k_level2_pattern(const uint64_t* __restrict__ packed, int t, int s,
+                                                       const uint4* __restrict__ ltab, uint4* __restrict__ out) {
+    const uint64_t nrows = 1ull << t, Y1 = nrows >> 1, Y2 = nrows >> 2, n = 1ull << (t + s - 2);
+    for (uint64_t x = grid_tid(); x < n; x += grid_threads()) {
+        uint64_t y = x >> s, c = x & ((1ull << s) - 1);
+        const uint64_t* col = packed + (c >> 6) * nrows;
+        int bit = (int)(c & 63);
+        uint32_t p = (uint32_t)(((col[y] >> bit) & 1ull) | (((col[y + Y1] >> bit) & 1ull) << 1) |
+                                (((col[y + Y2] >> bit) & 1ull) << 2) | (((col[y + Y2 + Y1] >> bit) & 1ull) << 3));
+        out[x] = ltab[y * 16 + p];
     }
 }
 
@@ -1034,10 +1126,16 @@ int gkr_fold_round_sums(bitz_ctx* ctx, void* l, void* r, uint64_t h, const uint8
     uint64_t lo, hi;
     memcpy(&lo, rho, 8);
     memcpy(&hi, rho + 8, 8);
-    TimedLaunch tl(ctx, "gkr_fold_round_sums", 12.0 * 16.0 * h, 8.0 * h * kGfMulOps);
     FoldSrc src{(uint4*)l, (uint4*)r, lo, hi};
-    hipLaunchKernelGGL(k_gkr_round_sums<FoldSrc>, dim3(nb), dim3(kT), 0, ctx->stream, src, h, (const uint4*)eq_c,
-                       s_c, (const uint4*)eq_y, send_one, R, (uint4*)ctx->scratch.partials.ptr);
+    if (eq_c) {
+        TimedLaunch tl(ctx, "gkr_fold_round_sums", 12.0 * 16.0 * h, 8.0 * h * kGfMulOps);
+        hipLaunchKernelGGL(k_gkr_round_sums<FoldSrc>, dim3(nb), dim3(kT), 0, ctx->stream, src, h, (const uint4*)eq_c,
+                           s_c, (const uint4*)eq_y, send_one, R, (uint4*)ctx->scratch.partials.ptr);
+    } else {
+        TimedLaunch tl(ctx, "gkr_fold_round_sums_rowonly", 12.0 * 16.0 * h, 6.0 * h * kGfMulOps);
+        hipLaunchKernelGGL((k_gkr_round_sums<FoldSrc, false>), dim3(nb), dim3(kT), 0, ctx->stream, src, h, nullptr,
+                           s_c, (const uint4*)eq_y, send_one, R, (uint4*)ctx->scratch.partials.ptr);
+    }
     BITZ_LAUNCH_CHECK();
     return sums_finish(ctx, nb, 2, out_dev);
 }
@@ -1158,8 +1256,70 @@ int gkr_leaf_fold2(bitz_ctx* ctx, const uint64_t* packed, const void* A, int t, 
                        (const uint4*)A, t, lo0, hi0, ftab);
     hipLaunchKernelGGL(k_leaf_gtab, dim3((unsigned)((q8 * 32 + kT - 1) / kT)), dim3(kT), 0, ctx->stream,
                        (const uint4*)ftab, t, lo1, hi1, gtab);
-    hipLaunchKernelGGL(k_leaf_fold2_pattern, dim3(blocks_for(h, 4)), dim3(kT), 0, ctx->stream, packed, t, s,
+    hipLaunchKernelGGL(k_pattern_fold<0>, dim3(blocks_for(h, 4)), dim3(kT), 0, ctx->stream, packed, t, s,
                        (const uint4*)gtab, (const uint4*)eq_c, (uint4*)l_out, (uint4*)r_out);
+    BITZ_LAUNCH_CHECK();
+    return BITZ_OK;
+}
+
+int gkr_level1_round0_sums(bitz_ctx* ctx, const uint64_t* packed, const void* A, int t, int s, const void* eq_c,
+                           const void* eq_y, int send_one, void* out_dev) {
+    BITZ_REQUIRE(t >= 3, BITZ_ERR_SHAPE, "gkr_level1_round0_sums: t < 3");
+    const uint64_t Y1 = 1ull << (t - 1), Y3 = 1ull << (t - 3), cols = 1ull << s;
+    if (!ctx->scratch.leaf_tab.alloc(leaf_tab_bytes(t))) return BITZ_ERR_ALLOC;
+    uint4* vtab = (uint4*)ctx->scratch.leaf_tab.ptr;
+    uint4* qtab = leaf_tab_hi(ctx, t);
+    uint64_t ylanes = pattern_ylanes(s, Y3);
+    uint64_t threads = cols * ylanes;
+    unsigned nb = (unsigned)((threads + kT - 1) / kT);
+    int rc = ctx->scratch.ensure((size_t)nb * 2 * 16);
+    if (rc) return rc;
+    TimedLaunch tl(ctx, "gkr_level1_round0_sums", (double)(1ull << (t + s - 3)) + Y3 * 8192.0,
+                   (Y1 + Y3 * 1024.0 + 2.0 * threads) * kGfMulOps);
+    hipLaunchKernelGGL(k_level1_vtab, dim3((unsigned)((Y1 + kT - 1) / kT)), dim3(kT), 0, ctx->stream,
+                       (const uint4*)A, t, vtab);
+    hipLaunchKernelGGL(k_level1_qtab, dim3((unsigned)((Y3 * 256 + kT - 1) / kT)), dim3(kT), 0, ctx->stream,
+                       (const uint4*)vtab, t, (const uint4*)eq_y, send_one, qtab);
+    hipLaunchKernelGGL(k_leaf_pattern_sums<8>, dim3(nb), dim3(kT), 0, ctx->stream, packed, t, s, Y3, Y3,
+                       (const uint4*)qtab, (const uint4*)eq_c, ylanes, (uint4*)ctx->scratch.partials.ptr);
+    BITZ_LAUNCH_CHECK();
+    return sums_finish(ctx, nb, 2, out_dev);
+}
+
+int gkr_level1_fold(bitz_ctx* ctx, const uint64_t* packed, const void* A, int t, int s, const uint8_t rho[16],
+                    const void* eq_c, void* l_out, void* r_out) {
+    BITZ_REQUIRE(t >= 3, BITZ_ERR_SHAPE, "gkr_level1_fold: t < 3");
+    uint64_t lo, hi;
+    split16(rho, &lo, &hi);
+    const uint64_t Y1 = 1ull << (t - 1), Y3 = 1ull << (t - 3), h = 1ull << (t + s - 3);
+    if (!ctx->scratch.leaf_tab.alloc(leaf_tab_bytes(t))) return BITZ_ERR_ALLOC;
+    uint4* vtab = (uint4*)ctx->scratch.leaf_tab.ptr;
+    uint4* gtab = leaf_tab_hi(ctx, t);
+    TimedLaunch tl(ctx, "gkr_level1_fold", 2.0 * 16.0 * h + (double)(h / 8),
+                   (Y1 + Y3 * 32.0 + (eq_c ? (double)h : 0.0)) * kGfMulOps);
+    hipLaunchKernelGGL(k_level1_vtab, dim3((unsigned)((Y1 + kT - 1) / kT)), dim3(kT), 0, ctx->stream,
+                       (const uint4*)A, t, vtab);
+    hipLaunchKernelGGL(k_level1_gtab, dim3((unsigned)((Y3 * 32 + kT - 1) / kT)), dim3(kT), 0, ctx->stream,
+                       (const uint4*)vtab, t, lo, hi, gtab);
+    hipLaunchKernelGGL(k_pattern_fold<1>, dim3(blocks_for(h, 4)), dim3(kT), 0, ctx->stream, packed, t, s,
+                       (const uint4*)gtab, (const uint4*)eq_c, (uint4*)l_out, (uint4*)r_out);
+    BITZ_LAUNCH_CHECK();
+    return BITZ_OK;
+}
+
+int gkr_level2(bitz_ctx* ctx, const uint64_t* packed, const void* A, int t, int s, void* out) {
+    BITZ_REQUIRE(t >= 2, BITZ_ERR_SHAPE, "gkr_level2: t < 2");
+    const uint64_t Y1 = 1ull << (t - 1), Y2 = 1ull << (t - 2), n = 1ull << (t + s - 2);
+    if (!ctx->scratch.leaf_tab.alloc(leaf_tab_bytes(t))) return BITZ_ERR_ALLOC;
+    uint4* vtab = (uint4*)ctx->scratch.leaf_tab.ptr;
+    uint4* ltab = leaf_tab_hi(ctx, t);
+    TimedLaunch tl(ctx, "gkr_level2", 16.0 * n + (double)(n / 8), (Y1 + Y2 * 16.0) * kGfMulOps);
+    hipLaunchKernelGGL(k_level1_vtab, dim3((unsigned)((Y1 + kT - 1) / kT)), dim3(kT), 0, ctx->stream,
+                       (const uint4*)A, t, vtab);
+    hipLaunchKernelGGL(k_level2_ltab, dim3((unsigned)((Y2 * 16 + kT - 1) / kT)), dim3(kT), 0, ctx->stream,
+                       (const uint4*)vtab, t, ltab);
+    hipLaunchKernelGGL(k_level2_pattern, dim3(blocks_for(n, 4)), dim3(kT), 0, ctx->stream, packed, t, s,
+                       (const uint4*)ltab, (uint4*)out);
     BITZ_LAUNCH_CHECK();
     return BITZ_OK;
 }

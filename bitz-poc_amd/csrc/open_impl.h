@@ -44,6 +44,15 @@ int gkr_round_sums_rowonly(bitz_ctx* ctx, const void* l, const void* r, uint64_t
                            int send_one, void* out_dev);
 // out[x] = prod_i (bit i of x ? b[i] : a[i]), x < 2^n, n <= 14
 int tensor_table(bitz_ctx* ctx, const void* a, const void* b, int n, void* out);
+// the layer over level 1 (never materialised) from 8-bit row patterns:
+// round-0 sums (weights eq_y[y] (2^(t-3)) * eq_c[c]) and the round-0 fold
+// into halves of 2^(t+s-3) entries (right half times eq_c when given)
+int gkr_level1_round0_sums(bitz_ctx* ctx, const uint64_t* packed, const void* A, int t, int s, const void* eq_c,
+                           const void* eq_y, int send_one, void* out_dev);
+int gkr_level1_fold(bitz_ctx* ctx, const uint64_t* packed, const void* A, int t, int s, const uint8_t rho[16],
+                    const void* eq_c, void* l_out, void* r_out);
+// level 2 of the product tree (2^(t+s-2) entries) from the bits
+int gkr_level2(bitz_ctx* ctx, const uint64_t* packed, const void* A, int t, int s, void* out);
 int gkr_level1(bitz_ctx* ctx, const uint64_t* packed, const void* A, int t, int s, void* out);
 int product_level(bitz_ctx* ctx, const void* in, uint64_t n, void* out);
 int xi_combined(bitz_ctx* ctx, const uint64_t* packed, int t, int s, const void* w2, void* out);

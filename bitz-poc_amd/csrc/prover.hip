@@ -320,7 +320,11 @@ struct Prover {
     }
 
     // ---- batched grand-product GKR (Forest::prove == gpgkr_prove) ----
-    int gkr_layer(InnerTranscript& it, uint4* l, uint4* r, bool leaf, std::vector<G>& internal, G* claim) {
+    // layer kinds: halves materialised, halves = the implicit level 1 (round 0
+    // from bit patterns), halves = the implicit leaves (rounds 0-1 from bit patterns)
+    enum LayerKind { kDense = 0, kLevel1 = 1, kLeaf = 2 };
+    int gkr_layer(InnerTranscript& it, uint4* l, uint4* r, LayerKind kind, std::vector<G>& internal, G* claim) {
+        const bool leaf = kind == kLeaf;
         const int L = (int)internal.size();
         std::vector<G> ext(internal.rbegin(), internal.rend());
         std::vector<G> next;
@@ -387,11 +391,12 @@ struct Prover {
         }
         bool weighted = false;  // r currently carries eq_c
         // a round's fold by its challenge is enqueued with the next round's
-        // sums (fold pass, then sums pass: on MI355X the two separate passes
-        // beat the fused kernel, whose table lookups stall on LDS)
+        // sums: fused into one pass where the right half carries eq_c (two
+        // products per slot), a fold pass then a sums pass otherwise (with the
+        // eq_c multiplies as well, the fused kernel loses to the two passes)
         bool pending_fold = false;
         uint8_t pending_rho[16], rho0[16];
-        BITZ_REQUIRE(!leaf || L >= 3, BITZ_ERR_SHAPE, "GKR leaf layer: too few variables");
+        BITZ_REQUIRE(kind == kDense || (L >= 3 && t >= 3), BITZ_ERR_SHAPE, "GKR: layer too small for bit patterns");
         for (int round = 0; round < L; round++) {
             G z = internal[round];
             int rem = L - 1 - round;
@@ -410,7 +415,12 @@ struct Prover {
             else if (leaf && round == 1)
                 rc = gkr_leaf_round1_sums(ctx, (const uint64_t*)packed.ptr, rowimg.ptr, t, s, rho0, eqc.ptr, eqy.ptr,
                                           send_one, nullptr);
-            else {
+            else if (kind == kLevel1 && round == 0)
+                rc = gkr_level1_round0_sums(ctx, (const uint64_t*)packed.ptr, rowimg.ptr, t, s, eqc.ptr, eqy.ptr,
+                                            send_one, nullptr);
+            else if (pending_fold && weighted && rem >= s) {
+                rc = gkr_fold_round_sums(ctx, l, r, hh, pending_rho, nullptr, s, eqy.ptr, send_one, nullptr);
+            } else {
                 if (pending_fold && (rc = fold_pair(ctx, l, r, 2 * hh, pending_rho))) return rc;
                 if (weighted && rem < s) {
                     // all row variables bound: r holds 2^s entries eq_c[c] * r[c]
@@ -438,7 +448,13 @@ struct Prover {
             it.verifier_gf(rb);
             G rr = gfrom(rb);
             next.push_back(rr);
-            if (leaf && round == 0) {
+            if (kind == kLevel1 && round == 0) {
+                if ((rc = gkr_level1_fold(ctx, (const uint64_t*)packed.ptr, rowimg.ptr, t, s, rb,
+                                          weigh ? eqc.ptr : nullptr, l, r)))
+                    return rc;
+                weighted = weigh;
+                pending_fold = false;
+            } else if (leaf && round == 0) {
                 // leaves stay implicit: round 1 reads the bits again
                 memcpy(rho0, rb, 16);
                 pending_fold = false;
@@ -472,8 +488,9 @@ struct Prover {
 
     // returns the leaf point (LE, t+s coordinates) and claim
     int gkr(InnerTranscript& it, const std::vector<G>& zeta, std::vector<G>* point, G* claim) {
-        // levels 1..t-1 materialised: level j has 2^(t+s-j) entries, stored
-        // back to back (level 1 first)
+        // storage of levels 1..t-1 back to back (level j: 2^(t+s-j) entries,
+        // level 1 first); levels 2..t-1 are materialised, level 1's region
+        // only receives the folded halves of the level-1 and leaf layers
         uint64_t total = 0;
         for (int j = 1; j < t; j++) total += 1ull << (t + s - j);
         if (!levels.alloc(total * 16)) return BITZ_ERR_ALLOC;
@@ -484,19 +501,21 @@ struct Prover {
             lv[j] = base + off;
             off += 1ull << (t + s - j);
         }
-        int rc = gkr_level1(ctx, (const uint64_t*)packed.ptr, rowimg.ptr, t, s, lv[1]);
+        // level 1 stays implicit (its layer reads the bits); level 2 comes
+        // straight from the bits, the upper levels from products
+        int rc = gkr_level2(ctx, (const uint64_t*)packed.ptr, rowimg.ptr, t, s, lv[2]);
         if (rc) return rc;
-        for (int j = 1; j + 1 < t; j++)
+        for (int j = 2; j + 1 < t; j++)
             if ((rc = product_level(ctx, lv[j], 1ull << (t + s - j), lv[j + 1]))) return rc;
         std::vector<G> internal(zeta.rbegin(), zeta.rend());
         G cl = gzero();
         for (int j = t - 1; j >= 1; j--) {
             uint64_t half = 1ull << (t + s - j - 1);
-            if ((rc = gkr_layer(it, lv[j], lv[j] + half, false, internal, &cl))) return rc;
+            if ((rc = gkr_layer(it, lv[j], lv[j] + half, j == 1 ? kLevel1 : kDense, internal, &cl))) return rc;
         }
         // leaf layer: the first fold lands in level 1's storage
         uint64_t q = 1ull << (t + s - 2);
-        if ((rc = gkr_layer(it, lv[1], lv[1] + q, true, internal, &cl))) return rc;
+        if ((rc = gkr_layer(it, lv[1], lv[1] + q, kLeaf, internal, &cl))) return rc;
         point->assign(internal.rbegin(), internal.rend());
         *claim = cl;
         return BITZ_OK;
@@ -1252,4 +1271,12 @@ extern "C" int bitz_ligerito_config(uint32_t m, uint32_t log_inv_rate, uint32_t 
 
 extern "C" void bitz_host_blake3(const uint8_t* data, uint64_t len, uint8_t out[32]) {
     bitz::blake3_hash_host(data, len, out);
+}
+
+extern "C" void bitz_host_blake3_parts(const uint8_t* data, uint64_t len, uint64_t split, uint8_t out[32]) {
+    bitz::Blake3 h;
+    if (split > len) split = len;
+    h.update(data, split);
+    h.update(data + split, len - split);
+    h.finalize_xof(out, 32);
 }

@@ -149,7 +149,8 @@ int bitz_tensor_table_dev(bitz_ctx* ctx, const void* a, const void* b, int n, vo
 /* fused_fold_round (src/bitz/kernels.rs:146-200): l and r hold the previous
  * round's halves (4h entries each); both are folded by rho in place into 2h
  * entries (quarters q0 + rho (q2 - q0), q1 + rho (q3 - q1)) and this round's
- * sums over the folded halves (h slots, weights as above) are returned */
+ * sums over the folded halves (h slots, weights as above) are returned; with
+ * eq_c NULL the right half carries the column weights (weights eq_y[x >> s_c]) */
 int bitz_gkr_fold_round_sums_dev(bitz_ctx* ctx, void* l, void* r, uint64_t h, const uint8_t rho[16],
                                  const void* eq_c, int s_c, const void* eq_y, int send_one, uint8_t out_host[32]);
 /* fold_halves (src/bitz/gkr.rs:170-177): v[i] += rho (v[i+h] - v[i]), i < h */
@@ -159,6 +160,17 @@ int bitz_gkr_leaf_fold_dev(bitz_ctx* ctx, const void* packed, const void* A, int
                            const uint8_t rho[16], void* l_out, void* r_out);
 /* level 1 of the batched product tree from the bits (GrandProductCircuit::batched_eval, gkr.rs:93) */
 int bitz_gkr_level1_dev(bitz_ctx* ctx, const void* packed, const void* A, int t, int s, void* out);
+/* level 2 (2^(t+s-2) entries) straight from the bits (Forest::fold_table /
+ * product_level_into over bit patterns, src/bitz/forest.rs:741-898) */
+int bitz_gkr_level2_dev(bitz_ctx* ctx, const void* packed, const void* A, int t, int s, void* out);
+/* the GKR layer whose halves are level 1, level 1 left implicit
+ * (Forest::prove_jit_level, src/bitz/forest.rs:583-740): round-0 sums with
+ * weights eq_y[y] (2^(t-3) entries) * eq_c[c], and the round-0 fold by rho into
+ * halves of 2^(t+s-3) entries (right half times eq_c[c] when eq_c is given) */
+int bitz_gkr_level1_round0_sums_dev(bitz_ctx* ctx, const void* packed, const void* A, int t, int s,
+                                    const void* eq_c, const void* eq_y, int send_one, uint8_t out_host[32]);
+int bitz_gkr_level1_fold_dev(bitz_ctx* ctx, const void* packed, const void* A, int t, int s, const uint8_t rho[16],
+                             const void* eq_c, void* l_out, void* r_out);
 /* one product level: out[i] = in[i] * in[i + n/2] (gkr.rs:93-108) */
 int bitz_product_level_dev(bitz_ctx* ctx, const void* in, uint64_t n, void* out);
 /* xi_combined_rows_packed (src/ligerito.rs:563-630): out[b] = sum_c w2[c] bit(c,b) */
@@ -228,8 +240,10 @@ int bitz_prove_standalone(bitz_ctx* ctx, bitz_commit_handle* h, const uint8_t cl
  * k_recursive, queries, grinding_bits, fold_grinding_bits, ood_samples]; returns the count */
 int bitz_ligerito_config(uint32_t m, uint32_t log_inv_rate, uint32_t initial_k, uint32_t target_bits,
                          int32_t* out, uint32_t capacity);
-/* BLAKE3 of a host buffer on the host (the transcript hash) */
+/* BLAKE3 of a host buffer on the host (the transcript hash); _parts absorbs
+ * data[0, split) and data[split, len) as two updates (incremental hashing) */
 void bitz_host_blake3(const uint8_t* data, uint64_t len, uint8_t out[32]);
+void bitz_host_blake3_parts(const uint8_t* data, uint64_t len, uint64_t split, uint8_t out[32]);
 
 #ifdef __cplusplus
 }

@@ -264,6 +264,50 @@ def test_gkr_leaf_fold2_carrying_weights(t, s):
     assert np.array_equal(to_host(wr), O.mul_arrays(to_host(pr), np.tile(eq_c, (1 << (t - 3), 1))))
 
 
+@pytest.mark.parametrize("t,s", [(7, 0), (7, 3), (9, 6), (10, 8), (15, 11)])
+def test_gkr_level1_layer_from_bits(t, s):
+    """level 2 from the bits, and the layer over the implicit level 1 (round-0
+    sums and fold, with and without carried column weights): against the
+    oracle's product tree on small shapes, against the device's pinned dense
+    kernels (level 1, product_level, round sums, fold_halves) at full size"""
+    rows = rand_rows(t, s, 71)
+    A = rand_gf(1 << t, 72)
+    A[::3] = [1, 0]
+    packed = ops().pack_columns(dev_u64(rows), t, s)
+    dA = to_dev(A)
+    rho = rand_ints(1, 73)[0]
+    point = rand_ints(t + s - 3, 74)
+    eq_c, eq_y = _eq_parts(point, s)
+    log(f"gkr level-1 layer from bits t={t} s={s}")
+    h = 1 << (t + s - 3)
+    if t + s <= 20:
+        lvl1 = O.product_level(O.forest_leaves(rows, t, s, A))
+        lvl2 = O.product_level(lvl1)
+        l, r = lvl1[:2 * h], lvl1[2 * h:]
+        full_eq = O.eq_table(O.ints_to_gf(point))
+        want_sums = {so: O.gkr_round_sums(l, r, full_eq, so) for so in (False, True)}
+        fl, fr = O.fold_halves(l, rho), O.fold_halves(r, rho)
+    else:
+        d1 = ops().gkr_level1(packed, dA, t, s)
+        lvl2 = to_host(ops().product_level(d1))
+        dl, dr = d1[:2 * h], d1[2 * h:]
+        want_sums = {so: tuple(ops().gkr_round_sums(dl, dr, h, to_dev(eq_c), s, to_dev(eq_y), so))
+                     for so in (False, True)}
+        ops().fold_halves_(dl, h, rho)
+        ops().fold_halves_(dr, h, rho)
+        fl, fr = to_host(dl)[:h], to_host(dr)[:h]
+    assert np.array_equal(to_host(ops().gkr_level2(packed, dA, t, s)), lvl2)
+    for so in (False, True):
+        got = ops().gkr_level1_round0_sums(packed, dA, t, s, to_dev(eq_c), to_dev(eq_y), so)
+        assert tuple(got) == tuple(want_sums[so])
+    gl, gr = ops().gkr_level1_fold(packed, dA, t, s, rho)
+    assert np.array_equal(to_host(gl), fl)
+    assert np.array_equal(to_host(gr), fr)
+    wl, wr = ops().gkr_level1_fold(packed, dA, t, s, rho, to_dev(eq_c))
+    assert np.array_equal(to_host(wl), fl)
+    assert np.array_equal(to_host(wr), O.mul_arrays(fr, np.tile(eq_c, (1 << (t - 3), 1))))
+
+
 def test_product_level():
     v = rand_gf(1 << 16, 3)
     assert np.array_equal(to_host(ops().product_level(to_dev(v))), O.product_level(v))

@@ -24,6 +24,22 @@ def test_host_blake3_matches_oracle(n):
     assert bytes(out) == O.blake3(data)
 
 
+@pytest.mark.parametrize("n", [65 * 1024, 65 * 1024 + 1, 200 * 1024 + 7, (1 << 20) + 5])
+@pytest.mark.parametrize("split", [0, 1, 100, 1023, 1024, 1025, 5000])
+def test_host_blake3_bulk_updates(n, split):
+    """long updates take the multi-threaded whole-chunk path; a first update
+    leaves the hasher at every kind of chunk offset"""
+    rng = np.random.default_rng(n + split)
+    data = rng.integers(0, 256, size=n, dtype=np.uint8).tobytes()
+    buf = (C.c_uint8 * n).from_buffer_copy(data)
+    out = (C.c_uint8 * 32)()
+    L.load().bitz_host_blake3_parts(C.cast(buf, C.c_void_p), n, split, C.cast(out, C.c_void_p))
+    want = O.blake3(data)
+    assert bytes(out) == want
+    L.load().bitz_host_blake3(C.cast(buf, C.c_void_p), n, C.cast(out, C.c_void_p))
+    assert bytes(out) == want
+
+
 def _lib_config(m, r, k):
     out = (C.c_int32 * 256)()
     n = L.load().bitz_ligerito_config(m, r, k, 100, C.cast(out, C.c_void_p), 256)

@@ -30,6 +30,19 @@ HBM_PEAK_GBS = 8000.0
 # VALU 32-bit integer peak: 256 CUs x 4 SIMDs x 16 lanes x 2.4 GHz
 VALU_PEAK_TOPS = 256 * 64 * 2.4e9 / 1e12
 
+# HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE
+# at the 2^24-coefficient workload, each scaled by what the counter reads for
+# a 1 GiB device copy (FETCH_SIZE x2.0, WRITE_SIZE x1.0); MiB per dispatch,
+# averaged over the dispatches of one proof (profiles/pmc_commit_open_2p24.txt).
+PMC_TRAFFIC_MIB = {
+    24: {
+        "gkr_fold_round_sums_rowonly": 280.33 + 139.99,
+        "gkr_premul_round_sums": 589.71 + 292.60,
+        "product_level": 585.17 + 292.55,
+        "ntt_pass": 1609.20 + 2116.45,
+    },
+}
+
 
 def parse():
     ap = argparse.ArgumentParser()
@@ -39,6 +52,9 @@ def parse():
     ap.add_argument("--log-coeffs", type=int, default=24, help="log2 of the u64 coefficient count")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-log-coeffs", type=int, default=18)
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="after the timed steps, copy 1 GiB device to device (a kernel of known traffic that "
+                         "calibrates the FETCH_SIZE / WRITE_SIZE counters of a profiler run)")
     return ap.parse_args()
 
 
@@ -128,6 +144,7 @@ def main():
         avg_s = tot_ms / launches / 1e3
         gbs = byts / launches / avg_s / 1e9
         tops = ops / launches / avg_s / 1e12
+        traffic_mib = PMC_TRAFFIC_MIB.get(args.log_coeffs, {}).get(name)
         roofline = {
             "kernel": name,
             "bound": "hbm",
@@ -135,13 +152,22 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(gbs / HBM_PEAK_GBS, 5),
-            "traffic": None,
+            "traffic": round(traffic_mib * 1048576) if traffic_mib else None,
+            "algorithmic_bytes_per_launch": round(byts / launches),
             "avg_launch_us": round(avg_s * 1e6, 2),
             "launches_per_step": launches / args.steps,
             "valu": {"achieved": round(tops, 3), "peak": round(VALU_PEAK_TOPS, 2), "unit": "T int32-op/s",
                      "frac": round(tops / VALU_PEAK_TOPS, 4)},
             "time_share": round(tot_ms / (ms * args.steps), 4),
         }
+    if args.copy_probe:
+        src = torch.empty(1 << 27, dtype=torch.int64, device="cuda")
+        src.fill_(7)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(args.cpu_log_coeffs)

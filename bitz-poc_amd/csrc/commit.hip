@@ -358,8 +358,6 @@ int ensure_twiddles(bitz_ctx* ctx, int ell, const uint4** out) {
 // cost model of the algorithmic work: one GF(2^128) multiplication = 3
 // 64-bit carry-less products + reduction, one BLAKE3 compression = 7 rounds
 // x 8 G x 14 ops + 16, both ~800 32-bit integer operations
-constexpr double kGfMulOps = 800.0;
-constexpr double kBlake3Ops = 800.0;
 
 static int tile_log(int log_d, int log_lanes) { return std::min(std::min(log_d, 11 - log_lanes), 9); }
 

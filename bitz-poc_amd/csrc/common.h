@@ -182,6 +182,16 @@ struct bitz_commit_handle {
 };
 
 namespace bitz {
+// Integer work per primitive, for the op counts attached to timed launches:
+// the VALU instructions of each primitive in this gfx950 build (one
+// instruction = 64 lane operations of the wave): a variable x variable
+// GF(2^128) product (gf_mul: 145 v_mad_u64_u32 among ~660), a product by a
+// launch constant through its 4-bit window table (32 LDS lookups, ~225
+// VALU), a BLAKE3 compression (7 rounds x 8 G x 14 + 16 = 800).
+constexpr double kGfMulOps = 660.0;
+constexpr double kGfTabMulOps = 225.0;
+constexpr double kBlake3Ops = 800.0;
+
 // RAII bracket: records start/stop events on ctx->stream when timing is on.
 struct TimedLaunch {
     bitz_ctx* ctx;

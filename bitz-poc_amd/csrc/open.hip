@@ -1053,7 +1053,6 @@ int pack_columns(bitz_ctx* ctx, const uint64_t* rows, int t, int s, uint64_t* pa
     return BITZ_OK;
 }
 
-constexpr double kGfMulOps = 800.0;
 
 int gkr_round_sums(bitz_ctx* ctx, const void* l, const void* r, uint64_t h, const void* eq_c, int s_c,
                    const void* eq_y, int send_one, void* out_dev) {
@@ -1128,11 +1127,11 @@ int gkr_fold_round_sums(bitz_ctx* ctx, void* l, void* r, uint64_t h, const uint8
     memcpy(&hi, rho + 8, 8);
     FoldSrc src{(uint4*)l, (uint4*)r, lo, hi};
     if (eq_c) {
-        TimedLaunch tl(ctx, "gkr_fold_round_sums", 12.0 * 16.0 * h, 8.0 * h * kGfMulOps);
+        TimedLaunch tl(ctx, "gkr_fold_round_sums", 12.0 * 16.0 * h, h * (4.0 * kGfMulOps + 4.0 * kGfTabMulOps));
         hipLaunchKernelGGL(k_gkr_round_sums<FoldSrc>, dim3(nb), dim3(kT), 0, ctx->stream, src, h, (const uint4*)eq_c,
                            s_c, (const uint4*)eq_y, send_one, R, (uint4*)ctx->scratch.partials.ptr);
     } else {
-        TimedLaunch tl(ctx, "gkr_fold_round_sums_rowonly", 12.0 * 16.0 * h, 6.0 * h * kGfMulOps);
+        TimedLaunch tl(ctx, "gkr_fold_round_sums_rowonly", 12.0 * 16.0 * h, h * (2.0 * kGfMulOps + 4.0 * kGfTabMulOps));
         hipLaunchKernelGGL((k_gkr_round_sums<FoldSrc, false>), dim3(nb), dim3(kT), 0, ctx->stream, src, h, nullptr,
                            s_c, (const uint4*)eq_y, send_one, R, (uint4*)ctx->scratch.partials.ptr);
     }
@@ -1182,7 +1181,7 @@ static void split16(const uint8_t* p, uint64_t* lo, uint64_t* hi) {
 int fold_halves(bitz_ctx* ctx, void* v, uint64_t h, const uint8_t rho[16]) {
     uint64_t lo, hi;
     split16(rho, &lo, &hi);
-    TimedLaunch tl(ctx, "fold_halves", 3.0 * 16.0 * h, h * kGfMulOps);
+    TimedLaunch tl(ctx, "fold_halves", 3.0 * 16.0 * h, h * kGfTabMulOps);
     hipLaunchKernelGGL(k_fold_halves, dim3(blocks_for(h, 4)), dim3(kT), 0, ctx->stream, (uint4*)v, h, lo, hi);
     BITZ_LAUNCH_CHECK();
     return BITZ_OK;
@@ -1191,7 +1190,7 @@ int fold_halves(bitz_ctx* ctx, void* v, uint64_t h, const uint8_t rho[16]) {
 int fold_pair(bitz_ctx* ctx, void* l, void* r, uint64_t h, const uint8_t rho[16]) {
     uint64_t lo, hi;
     split16(rho, &lo, &hi);
-    TimedLaunch tl(ctx, "fold_pair", 6.0 * 16.0 * h, 2.0 * h * kGfMulOps);
+    TimedLaunch tl(ctx, "fold_pair", 6.0 * 16.0 * h, 2.0 * h * kGfTabMulOps);
     hipLaunchKernelGGL(k_fold_pair, dim3(blocks_for(2 * h, 4)), dim3(kT), 0, ctx->stream, (uint4*)l, (uint4*)r, h,
                        lo, hi);
     BITZ_LAUNCH_CHECK();
@@ -1206,7 +1205,7 @@ int gkr_leaf_fold(bitz_ctx* ctx, const uint64_t* packed, const void* A, int t, i
     const uint64_t q = 1ull << (t - 2), h = 1ull << (t + s - 2);
     if (!ctx->scratch.leaf_tab.alloc(leaf_tab_bytes(t))) return BITZ_ERR_ALLOC;
     uint4* ftab = (uint4*)ctx->scratch.leaf_tab.ptr;
-    TimedLaunch tl(ctx, "gkr_leaf_fold", 2.0 * 16.0 * h + (double)(1ull << (t + s - 3)), q * 8.0 * kGfMulOps);
+    TimedLaunch tl(ctx, "gkr_leaf_fold", 2.0 * 16.0 * h + (double)(1ull << (t + s - 3)), q * 8.0 * kGfTabMulOps);
     hipLaunchKernelGGL(k_leaf_ftab, dim3((unsigned)((q * 8 + kT - 1) / kT)), dim3(kT), 0, ctx->stream,
                        (const uint4*)A, t, lo, hi, ftab);
     hipLaunchKernelGGL(k_leaf_fold_pattern, dim3(blocks_for(h, 4)), dim3(kT), 0, ctx->stream, packed, t, s,
@@ -1251,7 +1250,8 @@ int gkr_leaf_fold2(bitz_ctx* ctx, const uint64_t* packed, const void* A, int t, 
     if (!ctx->scratch.leaf_tab.alloc(leaf_tab_bytes(t))) return BITZ_ERR_ALLOC;
     uint4* ftab = (uint4*)ctx->scratch.leaf_tab.ptr;
     uint4* gtab = leaf_tab_hi(ctx, t);
-    TimedLaunch tl(ctx, "gkr_leaf_fold2", 2.0 * 16.0 * h + (double)(h / 8), (q * 8.0 + q8 * 32.0) * kGfMulOps);
+    TimedLaunch tl(ctx, "gkr_leaf_fold2", 2.0 * 16.0 * h + (double)(h / 8),
+                   (q * 8.0 + q8 * 32.0) * kGfTabMulOps + (eq_c ? (double)h : 0.0) * kGfMulOps);
     hipLaunchKernelGGL(k_leaf_ftab, dim3((unsigned)((q * 8 + kT - 1) / kT)), dim3(kT), 0, ctx->stream,
                        (const uint4*)A, t, lo0, hi0, ftab);
     hipLaunchKernelGGL(k_leaf_gtab, dim3((unsigned)((q8 * 32 + kT - 1) / kT)), dim3(kT), 0, ctx->stream,
@@ -1296,7 +1296,7 @@ int gkr_level1_fold(bitz_ctx* ctx, const uint64_t* packed, const void* A, int t,
     uint4* vtab = (uint4*)ctx->scratch.leaf_tab.ptr;
     uint4* gtab = leaf_tab_hi(ctx, t);
     TimedLaunch tl(ctx, "gkr_level1_fold", 2.0 * 16.0 * h + (double)(h / 8),
-                   (Y1 + Y3 * 32.0 + (eq_c ? (double)h : 0.0)) * kGfMulOps);
+                   (Y1 + (eq_c ? (double)h : 0.0)) * kGfMulOps + Y3 * 32.0 * kGfTabMulOps);
     hipLaunchKernelGGL(k_level1_vtab, dim3((unsigned)((Y1 + kT - 1) / kT)), dim3(kT), 0, ctx->stream,
                        (const uint4*)A, t, vtab);
     hipLaunchKernelGGL(k_level1_gtab, dim3((unsigned)((Y3 * 32 + kT - 1) / kT)), dim3(kT), 0, ctx->stream,
@@ -1440,7 +1440,7 @@ int lig_fold(bitz_ctx* ctx, const void* f, const void* b, uint64_t n, const uint
     BITZ_REQUIRE(f != fo && b != bo, BITZ_ERR_ARG, "lig_fold: outputs must not alias inputs");
     uint64_t lo, hi;
     split16(r, &lo, &hi);
-    TimedLaunch tl(ctx, "lig_fold", 3.0 * 16.0 * n, n * kGfMulOps);
+    TimedLaunch tl(ctx, "lig_fold", 3.0 * 16.0 * n, n * kGfTabMulOps);
     hipLaunchKernelGGL(k_lig_fold, dim3(blocks_for(n / 2, 4)), dim3(kT), 0, ctx->stream, (const uint4*)f,
                        (const uint4*)b, n / 2, lo, hi, (uint4*)fo, (uint4*)bo);
     BITZ_LAUNCH_CHECK();

@@ -35,6 +35,10 @@ SIGNATURES = {
     "bitz_ctx_stream": (_P, [_P]),
     "bitz_ctx_synchronize": (_I, [_P]),
     "bitz_ctx_trim": (_I, [_P]),
+    "bitz_dev_alloc": (_I, [_P, _U64, _P]),
+    "bitz_dev_free": (_I, [_P, _P]),
+    "bitz_dev_upload": (_I, [_P, _P, _P, _U64]),
+    "bitz_dev_download": (_I, [_P, _P, _P, _U64]),
     "bitz_ctx_cached_bytes": (_U64, [_P]),
     "bitz_last_error": (C.c_char_p, []),
     "bitz_version": (C.c_char_p, []),

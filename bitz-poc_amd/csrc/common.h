@@ -169,6 +169,8 @@ struct bitz_ctx {
     // device twiddle tables per NTT domain dimension ell:
     // tw[(1 << layer) - 1 + block] = twiddle(layer, block)
     std::map<int, std::unique_ptr<bitz::DevBuf>> twiddles;
+    // device buffers handed out through bitz_dev_alloc, by address
+    std::map<void*, bitz::DevBuf> user_bufs;
 };
 
 // A committed matrix kept on the device: the bit rows (= packed message

@@ -148,6 +148,47 @@ extern "C" int bitz_ctx_trim(bitz_ctx* ctx) {
 
 extern "C" uint64_t bitz_ctx_cached_bytes(bitz_ctx* ctx) { return ctx ? (uint64_t)ctx->pool.cached : 0; }
 
+// ---- device buffers for callers without a HIP binding of their own ----
+
+extern "C" int bitz_dev_alloc(bitz_ctx* ctx, uint64_t bytes, void** out) {
+    BITZ_REQUIRE(ctx && out, BITZ_ERR_ARG, "bitz_dev_alloc: null argument");
+    BITZ_HIP(hipSetDevice(ctx->device));
+    bitz::PoolScope scope(&ctx->pool);
+    bitz::DevBuf b;
+    BITZ_REQUIRE(b.alloc(bytes ? bytes : 1), BITZ_ERR_ALLOC, "bitz_dev_alloc: device allocation failed");
+    void* p = b.ptr;
+    ctx->user_bufs.emplace(p, std::move(b));
+    *out = p;
+    return BITZ_OK;
+}
+
+extern "C" int bitz_dev_free(bitz_ctx* ctx, void* ptr) {
+    BITZ_REQUIRE(ctx != nullptr, BITZ_ERR_ARG, "bitz_dev_free: null ctx");
+    if (!ptr) return BITZ_OK;
+    auto it = ctx->user_bufs.find(ptr);
+    BITZ_REQUIRE(it != ctx->user_bufs.end(), BITZ_ERR_ARG, "bitz_dev_free: not a buffer of this context");
+    ctx->user_bufs.erase(it);  // back to the pool, in stream order
+    return BITZ_OK;
+}
+
+extern "C" int bitz_dev_upload(bitz_ctx* ctx, void* dst, const void* src, uint64_t bytes) {
+    BITZ_REQUIRE(ctx && dst && (src || bytes == 0), BITZ_ERR_ARG, "bitz_dev_upload: null argument");
+    if (bytes == 0) return BITZ_OK;
+    BITZ_HIP(hipSetDevice(ctx->device));
+    BITZ_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+    BITZ_HIP(hipStreamSynchronize(ctx->stream));
+    return BITZ_OK;
+}
+
+extern "C" int bitz_dev_download(bitz_ctx* ctx, void* dst, const void* src, uint64_t bytes) {
+    BITZ_REQUIRE(ctx && (dst || bytes == 0) && src, BITZ_ERR_ARG, "bitz_dev_download: null argument");
+    if (bytes == 0) return BITZ_OK;
+    BITZ_HIP(hipSetDevice(ctx->device));
+    BITZ_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+    BITZ_HIP(hipStreamSynchronize(ctx->stream));
+    return BITZ_OK;
+}
+
 extern "C" void* bitz_ctx_stream(bitz_ctx* ctx) { return ctx ? (void*)ctx->stream : nullptr; }
 
 extern "C" int bitz_ctx_synchronize(bitz_ctx* ctx) {

@@ -57,6 +57,14 @@ int bitz_ctx_synchronize(bitz_ctx* ctx);
  * stream with bitz_ctx_set_stream drains the old stream first. */
 int bitz_ctx_trim(bitz_ctx* ctx);
 uint64_t bitz_ctx_cached_bytes(bitz_ctx* ctx);
+/* Device buffers from the context's pool, for callers that have no HIP
+ * binding of their own (the reference keeps its tables in Vec<Gf>; these move
+ * them to and from the device): alloc, free (returns the block to the pool),
+ * and synchronous upload / download on the context's stream. */
+int bitz_dev_alloc(bitz_ctx* ctx, uint64_t bytes, void** out);
+int bitz_dev_free(bitz_ctx* ctx, void* ptr);
+int bitz_dev_upload(bitz_ctx* ctx, void* dst, const void* src, uint64_t bytes);
+int bitz_dev_download(bitz_ctx* ctx, void* dst, const void* src, uint64_t bytes);
 const char* bitz_last_error(void);
 /* Per-kernel timing: with enable != 0, instrumented launches are bracketed by
  * HIP events on the context's stream (stats reset). bitz_ctx_kernel_stats

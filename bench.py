@@ -52,6 +52,10 @@ def parse():
     ap.add_argument("--log-coeffs", type=int, default=24, help="log2 of the u64 coefficient count")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-log-coeffs", type=int, default=18)
+    ap.add_argument("--no-batched", action="store_true", help="skip the batched and serving measurements")
+    ap.add_argument("--batch-jobs", type=int, default=64)
+    ap.add_argument("--batch-workers", type=int, default=8)
+    ap.add_argument("--serve-jobs", type=int, default=96)
     ap.add_argument("--copy-probe", action="store_true",
                     help="after the timed steps, copy 1 GiB device to device (a kernel of known traffic that "
                          "calibrates the FETCH_SIZE / WRITE_SIZE counters of a profiler run)")
@@ -90,6 +94,96 @@ def cpu_baseline(log_coeffs: int):
                   f"standalone open by the CPU restatement (oracle/, C kernels driven from Python, "
                   f"1 thread, PCLMULQDQ); {secs:.2f} s",
     }
+
+
+def _job_specs(rng, count, lo, hi):
+    """count jobs: log2 coefficients uniform in [lo, hi], rates 1/2, 1/4, 1/8"""
+    import numpy as np
+    specs = []
+    for _ in range(count):
+        lc = int(rng.integers(lo, hi + 1))
+        r = int(rng.choice([1, 2, 3]))
+        _, t, s = shape_for(lc)
+        specs.append((t, s, r, rng.integers(0, 2**64, size=(1 << s) << (t - 6), dtype=np.uint64)))
+    return specs
+
+
+def batched_and_serving(args):
+    """Batched commit+open of independent polynomials of mixed sizes and
+    rates on one server (bitz_server: per-worker contexts, jobs overlapping on
+    the GPU) against the same jobs one at a time, and a serving loop with
+    Poisson arrivals. Each job includes its claim setup and host-to-device
+    copy of its rows. Proof bytes are compared job by job."""
+    import numpy as np
+
+    from bitz_amd.serve import Job, Server
+
+    rng = np.random.default_rng(64)
+    specs = _job_specs(rng, args.batch_jobs, 14, 22)
+
+    def run(workers):
+        with Server(workers=workers, hbm_budget=200 << 30) as srv:
+            for _ in range(2):  # the first pass fills the contexts' pools and twiddle caches
+                jobs = [Job(rows, t, s, 4, r) for t, s, r, rows in specs]
+                t0 = time.perf_counter()
+                for j in sorted(jobs, key=lambda j: -j.footprint):
+                    srv.submit(j)
+                srv.wait_all()
+                dt = time.perf_counter() - t0
+        for j in jobs:
+            j.check()
+        return jobs, dt
+
+    seq, dt_seq = run(1)
+    bat, dt_bat = run(args.batch_workers)
+    exact = all(a.proof == b.proof and a.root == b.root for a, b in zip(seq, bat))
+    coeffs = sum(1 << (t + s - 6) for t, s, _, _ in specs)
+    batched = {
+        "workload": f"{len(specs)} independent polynomials, 2^14..2^22 u64 coefficients, rates 1/2, 1/4, 1/8 "
+                    "(custom:1|2|3:4), commit + claim + standalone open each, rows from host memory",
+        "jobs": len(specs),
+        "workers": args.batch_workers,
+        "proofs_per_s": round(len(specs) / dt_bat, 2),
+        "sequential_proofs_per_s": round(len(specs) / dt_seq, 2),
+        "speedup": round(dt_seq / dt_bat, 3),
+        "mcoeff_per_s": round(coeffs / dt_bat / 1e6, 2),
+        "sequential_mcoeff_per_s": round(coeffs / dt_seq / 1e6, 2),
+        "bit_exact_vs_sequential": exact,
+    }
+    # serving: Poisson arrivals at 70% of the batched rate
+    rate = 0.7 * len(specs) / dt_bat
+    sspecs = _job_specs(np.random.default_rng(65), args.serve_jobs, 14, 22)
+    gaps = np.random.default_rng(66).exponential(1.0 / rate, size=len(sspecs))
+    jobs = [Job(rows, t, s, 4, r) for t, s, r, rows in sspecs]
+    with Server(workers=args.batch_workers, hbm_budget=64 << 30) as srv:
+        warm = [Job(rows, t, s, 4, r) for t, s, r, rows in sspecs[:args.batch_workers * 2]]
+        for j in warm:
+            srv.submit(j)
+        srv.wait_all()
+        t0 = time.perf_counter()
+        due = 0.0
+        for j, g in zip(jobs, gaps):
+            due += g
+            delay = t0 + due - time.perf_counter()
+            if delay > 0:
+                time.sleep(delay)
+            srv.submit(j)
+        srv.wait_all()
+        t1 = time.perf_counter()
+    for j in jobs:
+        j.check()
+    lat = np.array([j.latency_ms for j in jobs])
+    serving = {
+        "workload": f"{len(jobs)} Poisson arrivals of commit+open jobs (2^14..2^22 coefficients, mixed rates), "
+                    "admitted by device-memory footprint (64 GiB budget)",
+        "arrival_rate_per_s": round(rate, 2),
+        "workers": args.batch_workers,
+        "proofs_per_s": round(len(jobs) / (t1 - t0), 2),
+        "p50_ms": round(float(np.percentile(lat, 50)), 3),
+        "p99_ms": round(float(np.percentile(lat, 99)), 3),
+        "max_ms": round(float(lat.max()), 3),
+    }
+    return batched, serving
 
 
 def main():
@@ -168,6 +262,9 @@ def main():
         dst.copy_(src)
         torch.cuda.synchronize()
         del src, dst
+    batched = serving = None
+    if not args.no_batched:
+        batched, serving = batched_and_serving(args)
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(args.cpu_log_coeffs)
@@ -189,6 +286,8 @@ def main():
                    "proof_bytes": len(proof) if proof else None},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "batched": batched,
+        "serving": serving,
         "kernels": {k: {"launches": int(v[0]), "ms": round(v[1] / args.steps, 3)} for k, v in
                     sorted(stats.items(), key=lambda kv: -kv[1][1])},
         "host_phases_ms": phases,

@@ -91,6 +91,14 @@ SIGNATURES = {
     "bitz_prove_standalone": (_I, [_P, _P, _P, _P, _U64, C.POINTER(_U64)]),
     "bitz_ligerito_config": (_I, [_U32, _U32, _U32, _U32, _P, _U32]),
     "bitz_host_blake3": (None, [_P, _U64, _P]),
+    "bitz_job_footprint": (_U64, [C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32]),
+    "bitz_server_create": (_I, [_I, C.c_uint32, _U64, _P]),
+    "bitz_server_submit": (_I, [_P, _P]),
+    "bitz_server_done": (_I, [_P, _P]),
+    "bitz_server_wait_all": (_I, [_P]),
+    "bitz_server_clock_ms": (C.c_double, [_P]),
+    "bitz_server_destroy": (None, [_P]),
+    "bitz_batch_run": (_I, [_I, C.c_uint32, _U64, _P, C.c_uint32]),
     "bitz_host_blake3_parts": (None, [_P, _U64, _U64, _P]),
 }
 

@@ -1,0 +1,124 @@
+"""Batched and served commit+open jobs (bitz_server / bitz_batch_run of the
+C ABI): each job is one commitment and its standalone opening, the same
+calls as Pcs::commit + standalone_evaluation / prove_standalone
+(src/bitz/pcs.rs:270-284, src/piop/spartan/protocol/bitz_opener.rs:643-707),
+run on one of several per-thread contexts so that jobs overlap on the GPU."""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+
+
+class _JobStruct(C.Structure):
+    _fields_ = [
+        ("rows", C.c_void_p),
+        ("t", C.c_uint32), ("s", C.c_uint32), ("log_batch", C.c_uint32), ("log_inv_rate", C.c_uint32),
+        ("rows_on_device", C.c_uint32), ("have_claim", C.c_uint32),
+        ("claimed", C.c_uint8 * 16),
+        ("proof", C.c_void_p), ("proof_capacity", C.c_uint64),
+        ("root", C.c_uint8 * 32),
+        ("proof_len", C.c_uint64),
+        ("status", C.c_int32), ("done", C.c_int32),
+        ("submit_ms", C.c_double), ("start_ms", C.c_double), ("end_ms", C.c_double),
+        ("error", C.c_char * 128),
+    ]
+
+
+class Job:
+    """One commit+open job over host bit rows (numpy u64, column-major: 2^s
+    columns of 2^(t-6) words). claimed=None: the job computes the honest
+    claim (standalone_evaluation) first."""
+
+    def __init__(self, rows: np.ndarray, t: int, s: int, log_batch: int, log_inv_rate: int,
+                 claimed: int | None = None, proof_capacity: int = 1 << 21):
+        rows = np.ascontiguousarray(rows, dtype=np.uint64)
+        if rows.size != (1 << s) << (t - 6):
+            raise ValueError("rows must hold 2^s * 2^(t-6) u64 words")
+        self._rows = rows
+        self._proof = (C.c_uint8 * proof_capacity)()
+        j = _JobStruct()
+        j.rows = rows.ctypes.data
+        j.t, j.s, j.log_batch, j.log_inv_rate = t, s, log_batch, log_inv_rate
+        j.have_claim = 0 if claimed is None else 1
+        if claimed is not None:
+            C.memmove(j.claimed, int(claimed).to_bytes(16, "little"), 16)
+        j.proof = C.cast(self._proof, C.c_void_p)
+        j.proof_capacity = proof_capacity
+        self.s_ = j
+
+    @property
+    def footprint(self) -> int:
+        j = self.s_
+        return int(_lib.load().bitz_job_footprint(j.t, j.s, j.log_batch, j.log_inv_rate))
+
+    def check(self) -> None:
+        if self.s_.status != 0:
+            raise _lib.BitzError(self.s_.status, "bitz_job", self.s_.error.decode(errors="replace"))
+
+    @property
+    def root(self) -> bytes:
+        return bytes(self.s_.root)
+
+    @property
+    def claimed(self) -> int:
+        return int.from_bytes(bytes(self.s_.claimed), "little")
+
+    @property
+    def proof(self) -> bytes:
+        self.check()
+        return C.string_at(self._proof, self.s_.proof_len)
+
+    @property
+    def latency_ms(self) -> float:
+        return self.s_.end_ms - self.s_.submit_ms
+
+
+class Server:
+    """bitz_server: `workers` contexts on one device; jobs admitted in order
+    while the footprint of the jobs in flight fits hbm_budget bytes."""
+
+    def __init__(self, device: int = 0, workers: int = 4, hbm_budget: int = 200 << 30):
+        self._p = C.c_void_p()
+        _lib.call("bitz_server_create", device, workers, hbm_budget, C.byref(self._p))
+        self._jobs: list[Job] = []
+
+    def submit(self, job: Job) -> None:
+        self._jobs.append(job)  # the struct must outlive the job
+        _lib.call("bitz_server_submit", self._p, C.byref(job.s_))
+
+    def done(self, job: Job) -> bool:
+        return _lib.load().bitz_server_done(self._p, C.byref(job.s_)) == 1
+
+    def wait_all(self) -> None:
+        _lib.call("bitz_server_wait_all", self._p)
+
+    def clock_ms(self) -> float:
+        return float(_lib.load().bitz_server_clock_ms(self._p))
+
+    def close(self) -> None:
+        if self._p:
+            _lib.load().bitz_server_destroy(self._p)
+            self._p = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
+def run_batch(jobs: list[Job], workers: int, hbm_budget: int = 200 << 30, device: int = 0) -> None:
+    """bitz_batch_run: all jobs, largest first, on a fresh server."""
+    arr = (_JobStruct * len(jobs))()
+    for i, j in enumerate(jobs):
+        arr[i] = j.s_
+    rc = _lib.load().bitz_batch_run(device, workers, hbm_budget, C.cast(arr, C.c_void_p), len(jobs))
+    for i, j in enumerate(jobs):
+        j.s_ = arr[i]
+    if rc != 0:
+        bad = [j for j in jobs if j.s_.status != 0]
+        msg = bad[0].s_.error.decode(errors="replace") if bad else _lib.load().bitz_last_error().decode()
+        raise _lib.BitzError(rc, "bitz_batch_run", msg)

@@ -248,6 +248,42 @@ int bitz_prove_standalone(bitz_ctx* ctx, bitz_commit_handle* h, const uint8_t cl
  * k_recursive, queries, grinding_bits, fold_grinding_bits, ood_samples]; returns the count */
 int bitz_ligerito_config(uint32_t m, uint32_t log_inv_rate, uint32_t initial_k, uint32_t target_bits,
                          int32_t* out, uint32_t capacity);
+/* ---- batched and served commit+open jobs ----
+ * A job is one commitment and its standalone opening (Pcs::commit +
+ * standalone_evaluation / prove_standalone). The server runs jobs on
+ * `workers` host threads, each with a context (pool + stream) of its own, so
+ * jobs overlap on the device; it admits them in submission order when a worker
+ * is free and the footprint of the jobs in flight plus the next one fits
+ * hbm_budget bytes (a job larger than the budget runs alone). A job's proof
+ * bytes are those of the same job run by itself. Job structs belong to the
+ * caller and must stay valid until done. */
+typedef struct bitz_job {
+    const void* rows;        /* 2^s * 2^(t-6) u64 words, column-major (host, or device memory) */
+    uint32_t t, s, log_batch, log_inv_rate;
+    uint32_t rows_on_device; /* rows are device memory */
+    uint32_t have_claim;     /* 0: claimed is computed (standalone_evaluation) inside the job */
+    uint8_t claimed[16];     /* u128 LE, in or out */
+    uint8_t* proof;          /* caller buffer of proof_capacity bytes */
+    uint64_t proof_capacity;
+    uint8_t root[32];        /* out */
+    uint64_t proof_len;      /* out */
+    int32_t status;          /* out: BITZ_OK or the job's error (message in error[]) */
+    int32_t done;            /* out: set under the server's lock, read it with bitz_server_done */
+    double submit_ms, start_ms, end_ms; /* server clock (bitz_server_clock_ms) */
+    char error[128];
+} bitz_job;
+typedef struct bitz_server bitz_server;
+/* device bytes one job holds while it runs (0 for an invalid shape) */
+uint64_t bitz_job_footprint(uint32_t t, uint32_t s, uint32_t log_batch, uint32_t log_inv_rate);
+int bitz_server_create(int device, uint32_t workers, uint64_t hbm_budget, bitz_server** out);
+int bitz_server_submit(bitz_server* srv, bitz_job* job);
+int bitz_server_done(bitz_server* srv, const bitz_job* job);
+int bitz_server_wait_all(bitz_server* srv);
+double bitz_server_clock_ms(bitz_server* srv);
+void bitz_server_destroy(bitz_server* srv);
+/* submits n jobs (largest first) to a fresh server and waits for all */
+int bitz_batch_run(int device, uint32_t workers, uint64_t hbm_budget, bitz_job* jobs, uint32_t n);
+
 /* BLAKE3 of a host buffer on the host (the transcript hash); _parts absorbs
  * data[0, split) and data[split, len) as two updates (incremental hashing) */
 void bitz_host_blake3(const uint8_t* data, uint64_t len, uint8_t out[32]);

@@ -3,6 +3,8 @@ include/bitz_hip.h declares, with the binding's table covering all of them."""
 import os
 import re
 
+import pytest
+
 import bitz_amd._lib as L
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -34,3 +36,24 @@ def test_error_path_without_device():
     assert rc == -4
     assert b"null" in lib.bitz_last_error()
     assert lib.bitz_version().startswith(b"bitz_hip gfx950")
+
+
+def test_job_struct_layout_matches_the_header(tmp_path):
+    """bitz_amd.serve's ctypes bitz_job has the C header's size and offsets"""
+    import shutil
+    import subprocess
+    if shutil.which("gcc") is None:
+        pytest.skip("gcc not available")
+    import ctypes as C
+    from bitz_amd.serve import _JobStruct
+    inc = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "include")
+    names = [f[0] for f in _JobStruct._fields_]
+    src = tmp_path / "layout.c"
+    src.write_text("#include <stdio.h>\n#include <stddef.h>\n#include \"bitz_hip.h\"\nint main(){printf(\"%zu\\n\", "
+                   "sizeof(bitz_job));" + "".join(f"printf(\"%zu\\n\", offsetof(bitz_job, {n}));" for n in names)
+                   + "return 0;}\n")
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-I", inc, "-o", str(exe), str(src)], check=True)
+    got = [int(x) for x in subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split()]
+    assert got[0] == C.sizeof(_JobStruct)
+    assert got[1:] == [getattr(_JobStruct, n).offset for n in names]

@@ -1,0 +1,93 @@
+"""Batched and served commit+open jobs: every job's root, claim and proof
+bytes are the same run alone and in a mix of concurrent jobs of different
+sizes and code rates; the CPU verifier accepts them; admission keeps the
+device-memory footprint of the jobs in flight within the budget."""
+import numpy as np
+import pytest
+
+import bitz_protocol as P
+from gpu_util import log
+
+pytestmark = pytest.mark.gpu
+
+# (t, s, log_inv_rate): n = t + s in 20..23, rates 1/2, 1/4, 1/8
+SHAPES = [(12, 8, 1), (12, 8, 2), (12, 8, 3), (13, 8, 2), (13, 9, 1), (12, 9, 3), (14, 9, 2), (13, 10, 3)]
+
+
+def _jobs(seed):
+    from bitz_amd.serve import Job
+    rng = np.random.default_rng(seed)
+    return [Job(rng.integers(0, 2**64, size=(1 << s) << (t - 6), dtype=np.uint64), t, s, 4, r)
+            for t, s, r in SHAPES]
+
+
+def _run_alone(jobs):
+    from bitz_amd.serve import Server
+    with Server(workers=1) as srv:
+        for j in jobs:
+            srv.submit(j)
+            srv.wait_all()
+            j.check()
+    return [(j.root, j.claimed, j.proof) for j in jobs]
+
+
+def test_jobs_alone_and_in_the_mix_are_identical():
+    from bitz_amd.serve import Server, run_batch
+    log("serve: jobs alone")
+    alone = _run_alone(_jobs(11))
+    log("serve: the same jobs on a 4-worker server, submitted in a shuffled order")
+    mix = _jobs(11)
+    order = [5, 0, 7, 2, 6, 1, 3, 4]
+    with Server(workers=4) as srv:
+        for i in order:
+            srv.submit(mix[i])
+        srv.wait_all()
+    for a, j in zip(alone, mix):
+        j.check()
+        assert (j.root, j.claimed, j.proof) == a
+    # intervals overlap: the jobs really ran concurrently
+    spans = sorted((j.s_.start_ms, j.s_.end_ms) for j in mix)
+    assert any(spans[i + 1][0] < spans[i][1] for i in range(len(spans) - 1))
+    log("serve: bitz_batch_run with 3 workers")
+    batch = _jobs(11)
+    run_batch(batch, workers=3)
+    for a, j in zip(alone, batch):
+        assert (j.root, j.claimed, j.proof) == a
+    for (t, s, r), j in list(zip(SHAPES, mix))[:3]:
+        P.verify_standalone(j.root, t, s, r, 4, j.claimed, j.proof)
+
+
+def test_admission_respects_the_budget():
+    from bitz_amd.serve import Server
+    jobs = [j for j in _jobs(12) if (j.s_.t, j.s_.s, j.s_.log_inv_rate) == (12, 8, 2)]
+    jobs += _jobs(13)[1:2] + _jobs(14)[1:2]  # three jobs of one shape and footprint
+    fp = jobs[0].footprint
+    log(f"serve: budget of 1.5 job footprints ({fp} B each)")
+    with Server(workers=3, hbm_budget=fp * 3 // 2) as srv:
+        for j in jobs:
+            srv.submit(j)
+        srv.wait_all()
+    spans = sorted((j.s_.start_ms, j.s_.end_ms) for j in jobs)
+    for j in jobs:
+        j.check()
+    # one at a time: no two jobs in flight together
+    assert all(spans[i + 1][0] >= spans[i][1] for i in range(len(spans) - 1))
+
+
+def test_bad_job_reports_its_error_and_the_server_continues():
+    from bitz_amd.serve import Job, Server
+    from bitz_amd._lib import BitzError
+    good = _jobs(15)[:2]
+    rng = np.random.default_rng(3)
+    # rate 1/256 commits, but no custom Johnson ladder has it (inverse-rate
+    # exponents 1..4): the opening rejects the job
+    bad = Job(rng.integers(0, 2**64, size=(1 << 8) << 6, dtype=np.uint64), 12, 8, 6, 8)
+    with Server(workers=2) as srv:
+        srv.submit(good[0])
+        srv.submit(bad)
+        srv.submit(good[1])
+        srv.wait_all()
+    good[0].check()
+    good[1].check()
+    with pytest.raises(BitzError):
+        bad.check()

@@ -53,6 +53,8 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-log-coeffs", type=int, default=18)
     ap.add_argument("--no-batched", action="store_true", help="skip the batched and serving measurements")
+    ap.add_argument("--device-transcript", action="store_true",
+                    help="run the GKR rounds on a device copy of the transcript (no host round trip per round)")
     ap.add_argument("--batch-jobs", type=int, default=64)
     ap.add_argument("--batch-workers", type=int, default=8)
     ap.add_argument("--serve-jobs", type=int, default=96)
@@ -198,6 +200,9 @@ def main():
 
     n, t, s = shape_for(args.log_coeffs)
     ctx = Context(0)
+    if args.device_transcript:
+        from bitz_amd import _lib
+        _lib.call("bitz_ctx_set_device_transcript", ctx.ptr, 1)
     g = torch.Generator(device="cuda").manual_seed(1234)
     words = 1 << args.log_coeffs
     coeffs = torch.randint(-(2**62), 2**62, (words,), dtype=torch.int64, device="cuda", generator=g)

@@ -35,6 +35,8 @@ SIGNATURES = {
     "bitz_ctx_stream": (_P, [_P]),
     "bitz_ctx_synchronize": (_I, [_P]),
     "bitz_ctx_trim": (_I, [_P]),
+    "bitz_ctx_set_device_transcript": (_I, [_P, _I]),
+    "bitz_transcript_challenges_dev": (_I, [_P, _P, _U64, _P, C.c_uint32, C.c_uint32, _P]),
     "bitz_dev_alloc": (_I, [_P, _U64, _P]),
     "bitz_dev_free": (_I, [_P, _P]),
     "bitz_dev_upload": (_I, [_P, _P, _P, _U64]),
@@ -64,6 +66,7 @@ SIGNATURES = {
     "bitz_gkr_round_sums_rowonly_dev": (_I, [_P, _P, _P, _U64, _I, _P, _I, _P]),
     "bitz_tensor_table_dev": (_I, [_P, _P, _P, _I, _P]),
     "bitz_gkr_level2_dev": (_I, [_P, _P, _P, _I, _I, _P]),
+    "bitz_gkr_level3_dev": (_I, [_P, _P, _P, _I, _I, _P]),
     "bitz_gkr_level1_round0_sums_dev": (_I, [_P, _P, _P, _I, _I, _P, _P, _I, _P]),
     "bitz_gkr_level1_fold_dev": (_I, [_P, _P, _P, _I, _I, _P, _P, _P, _P]),
     "bitz_gkr_leaf_fold_dev": (_I, [_P, _P, _P, _I, _I, _P, _P, _P]),

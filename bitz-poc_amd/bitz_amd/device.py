@@ -247,6 +247,11 @@ class OpenOps:
         _lib.call("bitz_gkr_level2_dev", self.ctx.ptr, _ptr(packed), _ptr(A), t, s, _ptr(out))
         return out
 
+    def gkr_level3(self, packed, A, t, s):
+        out = self._gf(1 << (t + s - 3))
+        _lib.call("bitz_gkr_level3_dev", self.ctx.ptr, _ptr(packed), _ptr(A), t, s, _ptr(out))
+        return out
+
     def gkr_level1_round0_sums(self, packed, A, t, s, eq_c, eq_y, send_one):
         buf = (C.c_uint8 * 32)()
         _lib.call("bitz_gkr_level1_round0_sums_dev", self.ctx.ptr, _ptr(packed), _ptr(A), t, s, _ptr(eq_c),

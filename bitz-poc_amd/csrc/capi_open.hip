@@ -1,4 +1,5 @@
 // C ABI over the opening kernels (device pointers; see include/bitz_hip.h).
+#include "host_proto.h"
 #include "open_impl.h"
 
 namespace {
@@ -154,6 +155,13 @@ extern "C" int bitz_gkr_level2_dev(bitz_ctx* ctx, const void* packed, const void
     return bitz::gkr_level2(ctx, (const uint64_t*)packed, A, t, s, out);
 }
 
+extern "C" int bitz_gkr_level3_dev(bitz_ctx* ctx, const void* packed, const void* A, int t, int s, void* out) {
+    PRELUDE();
+    BITZ_REQUIRE(packed && A && out, BITZ_ERR_ARG, "bitz_gkr_level3_dev: null argument");
+    BITZ_REQUIRE(t >= 3 && s >= 0 && t + s <= 40, BITZ_ERR_SHAPE, "bitz_gkr_level3_dev: bad shape");
+    return bitz::gkr_level3(ctx, (const uint64_t*)packed, A, t, s, out);
+}
+
 extern "C" int bitz_gkr_level1_round0_sums_dev(bitz_ctx* ctx, const void* packed, const void* A, int t, int s,
                                                const void* eq_c, const void* eq_y, int send_one,
                                                uint8_t out_host[32]) {
@@ -285,4 +293,24 @@ extern "C" int bitz_pow_find_dev(bitz_ctx* ctx, const uint8_t prefix[16], const 
     PRELUDE();
     BITZ_REQUIRE(prefix && seed && nonce, BITZ_ERR_ARG, "bitz_pow_find_dev: null argument");
     return bitz::pow_find(ctx, prefix, seed, bits, nonce);
+}
+
+extern "C" int bitz_transcript_challenges_dev(bitz_ctx* ctx, const uint8_t* prefix, uint64_t prefix_len,
+                                              const uint8_t* msgs, uint32_t n, uint32_t msg_len, uint8_t* out_host) {
+    PRELUDE();
+    BITZ_REQUIRE((prefix || prefix_len == 0) && (msgs || n == 0) && (out_host || n == 0), BITZ_ERR_ARG,
+                 "bitz_transcript_challenges_dev: null argument");
+    bitz::Transcript tr;
+    tr.absorb_slice(prefix, prefix_len);
+    bitz::B3State st;
+    tr.export_state(&st);
+    bitz::DevBuf dst, dmsg, dout;
+    BITZ_REQUIRE(dst.alloc(sizeof(st)) && dmsg.alloc((size_t)n * msg_len + 1) && dout.alloc((size_t)n * 16 + 1),
+                 BITZ_ERR_ALLOC, "bitz_transcript_challenges_dev: allocation failed");
+    BITZ_HIP(hipMemcpyAsync(dst.ptr, &st, sizeof(st), hipMemcpyHostToDevice, ctx->stream));
+    if (n) BITZ_HIP(hipMemcpyAsync(dmsg.ptr, msgs, (size_t)n * msg_len, hipMemcpyHostToDevice, ctx->stream));
+    CALL(bitz::transcript_challenges(ctx, dst.ptr, dmsg.ptr, n, msg_len, dout.ptr));
+    if (n) BITZ_HIP(hipMemcpyAsync(out_host, dout.ptr, (size_t)n * 16, hipMemcpyDeviceToHost, ctx->stream));
+    BITZ_HIP(hipStreamSynchronize(ctx->stream));
+    return BITZ_OK;
 }

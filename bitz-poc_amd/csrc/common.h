@@ -171,6 +171,14 @@ struct bitz_ctx {
     std::map<int, std::unique_ptr<bitz::DevBuf>> twiddles;
     // device buffers handed out through bitz_dev_alloc, by address
     std::map<void*, bitz::DevBuf> user_bufs;
+    // set while the GKR rounds reduce their block partials on the device:
+    // sums launches then leave the partials in scratch and record the count
+    bool defer_sums = false;
+    unsigned deferred_nb = 0;
+    // GKR rounds draw their challenges from a device copy of the transcript
+    // (off by default: on MI355X one wave hashes a round's ~8 BLAKE3
+    // compressions in ~20 us, no faster than the host round trip)
+    bool device_fs = false;
 };
 
 // A committed matrix kept on the device: the bit rows (= packed message

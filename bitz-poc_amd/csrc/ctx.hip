@@ -138,6 +138,12 @@ extern "C" int bitz_ctx_set_stream(bitz_ctx* ctx, void* stream) {
     return BITZ_OK;
 }
 
+extern "C" int bitz_ctx_set_device_transcript(bitz_ctx* ctx, int enable) {
+    BITZ_REQUIRE(ctx != nullptr, BITZ_ERR_ARG, "bitz_ctx_set_device_transcript: null ctx");
+    ctx->device_fs = enable != 0;
+    return BITZ_OK;
+}
+
 extern "C" int bitz_ctx_trim(bitz_ctx* ctx) {
     BITZ_REQUIRE(ctx != nullptr, BITZ_ERR_ARG, "bitz_ctx_trim: null ctx");
     BITZ_HIP(hipSetDevice(ctx->device));

@@ -81,6 +81,17 @@ static void words_le(const uint32_t* w, int n, uint8_t* out) {
 
 Blake3::Blake3() { chunk_reset(0); }
 
+void Blake3::export_state(B3State* s) const {
+    memset(s, 0, sizeof(*s));
+    memcpy(s->stack, stack_, sizeof(stack_));
+    memcpy(s->cv, chunk_.cv, 32);
+    s->counter = chunk_.counter;
+    memcpy(s->buf, chunk_.buf, 64);
+    s->buf_len = chunk_.buf_len;
+    s->blocks = chunk_.blocks;
+    s->stack_len = stack_len_;
+}
+
 void Blake3::chunk_reset(uint64_t counter) {
     memcpy(chunk_.cv, kIV, 32);
     chunk_.counter = counter;

@@ -10,6 +10,20 @@
 
 namespace bitz {
 
+// The state of an incremental BLAKE3 hasher as plain data, for continuing a
+// transcript on the device (the device code mirrors Blake3::update and
+// Blake3::finalize_xof on this struct).
+struct B3State {
+    uint32_t stack[54][8];
+    uint32_t cv[8];       // current chunk's chaining value
+    uint64_t counter;     // current chunk's index
+    uint8_t buf[64];      // current block, zero-padded
+    uint32_t buf_len;
+    uint32_t blocks;      // whole blocks compressed in the current chunk
+    int32_t stack_len;
+    uint32_t pad;
+};
+
 // ---- incremental BLAKE3 (public specification), hash mode ----
 class Blake3 {
    public:
@@ -17,6 +31,7 @@ class Blake3 {
     void update(const void* data, size_t len);
     // XOF output of everything absorbed so far; the state is unchanged.
     void finalize_xof(uint8_t* out, size_t len) const;
+    void export_state(B3State* s) const;
 
    private:
     struct Chunk {
@@ -50,6 +65,7 @@ class Transcript {
     // fill_sampling_bytes: XOF bytes, then absorb the sampling-read frame
     void sampling_bytes(uint8_t* out, size_t len);
     void digest(uint8_t out[32]) const { h_.finalize_xof(out, 32); }
+    void export_state(B3State* s) const { h_.export_state(s); }
 
    private:
     Blake3 h_;
@@ -76,6 +92,7 @@ class InnerTranscript {
     void verifier_gf(uint8_t out[16]) { t_.challenge(out, 16); }
     void hint_bytes(const void* data, size_t len);
     const std::vector<uint8_t>& narg() const { return narg_; }
+    void export_state(B3State* s) const { t_.export_state(s); }
     const std::vector<uint8_t>& hints() const { return hints_; }
 
    private:

@@ -14,6 +14,7 @@
 #include "blake3_dev.h"
 #include "reduce_dev.h"
 #include "open_impl.h"
+#include "transcript_dev.h"
 
 namespace bitz {
 namespace {
@@ -21,6 +22,11 @@ namespace {
 constexpr int kT = 256;
 
 __device__ __forceinline__ uint64_t grid_tid() { return (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; }
+// A challenge argument: read from device memory when the transcript runs on
+// the device (p != nullptr), else passed as two u64 scalars.
+__device__ __forceinline__ gf128 challenge_arg(uint64_t lo, uint64_t hi, const uint4* p) {
+    return p ? gf_load(p) : gf_from_u64(lo, hi);
+}
 __device__ __forceinline__ uint64_t grid_threads() { return (uint64_t)gridDim.x * blockDim.x; }
 
 unsigned blocks_for(uint64_t n, uint64_t per_thread = 1, unsigned cap = 4096) {
@@ -277,8 +283,9 @@ struct FoldSrc {
     uint4* l;
     uint4* r;
     uint64_t rho_lo, rho_hi;
+    const uint4* rho_p;
     __device__ __forceinline__ void build(uint4* tab) const {
-        gf_table_build(tab, gf_from_u64(rho_lo, rho_hi), threadIdx.x, blockDim.x);
+        gf_table_build(tab, challenge_arg(rho_lo, rho_hi, rho_p), threadIdx.x, blockDim.x);
     }
     __device__ __forceinline__ gf128 fold(const uint4* tab, uint4* v, uint64_t i, uint64_t h) const {
         gf128 a = gf_load(&v[i]), b = gf_load(&v[i + 2 * h]);
@@ -418,9 +425,9 @@ __global__ void k_leaf_q1tab(const uint4* __restrict__ ftab, int t, const uint4*
 // The leaf layer after two folds (rho0, then rho1): gtab[y1*32 + half*16 + sub]
 // with sub the half's 4 pattern bits (l: bits 0-3, r: bits 4-7).
 __global__ __launch_bounds__(kT) void k_leaf_gtab(const uint4* __restrict__ ftab, int t, uint64_t rho_lo,
-                                                  uint64_t rho_hi, uint4* __restrict__ gtab) {
+                                                  uint64_t rho_hi, const uint4* rho_p, uint4* __restrict__ gtab) {
     __shared__ uint4 tab[512];
-    gf_table_build(tab, gf_from_u64(rho_lo, rho_hi), threadIdx.x, blockDim.x);
+    gf_table_build(tab, challenge_arg(rho_lo, rho_hi, rho_p), threadIdx.x, blockDim.x);
     __syncthreads();
     const uint64_t q8 = 1ull << (t - 3);
     uint64_t idx = grid_tid();
@@ -506,9 +513,9 @@ __global__ void k_level1_qtab(const uint4* __restrict__ vtab, int t, const uint4
 // round-0 fold; sub = (s0, s1, s2, s3) are pattern bits (k, k+1, k+4, k+5)
 // with k = 2*half, lo = vtab[row][s0 | s2 << 1], hi = vtab[row + Y3][s1 | s3 << 1].
 __global__ __launch_bounds__(kT) void k_level1_gtab(const uint4* __restrict__ vtab, int t, uint64_t rho_lo,
-                                                    uint64_t rho_hi, uint4* __restrict__ gtab) {
+                                                    uint64_t rho_hi, const uint4* rho_p, uint4* __restrict__ gtab) {
     __shared__ uint4 tab[512];
-    gf_table_build(tab, gf_from_u64(rho_lo, rho_hi), threadIdx.x, blockDim.x);
+    gf_table_build(tab, challenge_arg(rho_lo, rho_hi, rho_p), threadIdx.x, blockDim.x);
     __syncthreads();
     const uint64_t Y3 = 1ull << (t - 3);
     uint64_t idx = grid_tid();
@@ -533,6 +540,35 @@ __global__ void k_level2_ltab(const uint4* __restrict__ vtab, int t, uint4* __re
     gf_store(&ltab[idx], gf_mul(gf_load(&vtab[y * 4 + (p & 3)]), gf_load(&vtab[(y + Y2) * 4 + (p >> 2)])));
 }
 
+// Level 3 from the bits: entry (y, c), y < Y3, is l3tab[y*256 + p] with p
+// bit k = bit(c, y + k Y3); as level3(y) = level2(y) level2(y + Y3) and a
+// level-2 entry reads rows y', y'+Y1, y'+Y2, y'+Y2+Y1 (k offsets 0, 4, 2, 6),
+// l3tab[y][p] = ltab[y][b0 b4 b2 b6] * ltab[y + Y3][b1 b5 b3 b7].
+__global__ void k_level3_tab(const uint4* __restrict__ ltab, int t, uint4* __restrict__ l3tab) {
+    const uint64_t Y3 = 1ull << (t - 3);
+    uint64_t idx = grid_tid();
+    if (idx >= Y3 * 256) return;
+    uint64_t y = idx >> 8;
+    uint32_t p = (uint32_t)(idx & 255);
+    uint32_t lo = (p & 1) | ((p >> 3) & 2) | ((p << 0) & 4) | ((p >> 3) & 8);
+    uint32_t hi = ((p >> 1) & 1) | ((p >> 4) & 2) | ((p >> 1) & 4) | ((p >> 4) & 8);
+    gf_store(&l3tab[idx], gf_mul(gf_load(&ltab[y * 16 + lo]), gf_load(&ltab[(y + Y3) * 16 + hi])));
+}
+
+__global__ __launch_bounds__(kT) void k_level3_pattern(const uint64_t* __restrict__ packed, int t, int s,
+                                                       const uint4* __restrict__ l3tab, uint4* __restrict__ out) {
+    const uint64_t nrows = 1ull << t, Y3 = nrows >> 3, n = 1ull << (t + s - 3);
+    for (uint64_t x = grid_tid(); x < n; x += grid_threads()) {
+        uint64_t y = x >> s, c = x & ((1ull << s) - 1);
+        const uint64_t* col = packed + (c >> 6) * nrows;
+        int bit = (int)(c & 63);
+        uint32_t p = 0;
+#pragma unroll
+        for (int k = 0; k < 8; k++) p |= (uint32_t)((col[y + k * Y3] >> bit) & 1ull) << k;
+        out[x] = l3tab[y * 256 + p];
+    }
+}
+
 __global__ __launch_bounds__(kT) void k_level2_pattern(const uint64_t* __restrict__ packed, int t, int s,
                                                        const uint4* __restrict__ ltab, uint4* __restrict__ out) {
     const uint64_t nrows = 1ull << t, Y1 = nrows >> 1, Y2 = nrows >> 2, n = 1ull << (t + s - 2);
@@ -549,9 +585,9 @@ __global__ __launch_bounds__(kT) void k_level2_pattern(const uint64_t* __restric
 // ftab[y*8 + half*4 + pat] = lo + rho (lo + hi) with lo/hi the leaves of rows
 // y + 2*half*q and y + (2*half+1)*q selected by the 2-bit pattern.
 __global__ __launch_bounds__(kT) void k_leaf_ftab(const uint4* __restrict__ A, int t, uint64_t rho_lo,
-                                                  uint64_t rho_hi, uint4* __restrict__ ftab) {
+                                                  uint64_t rho_hi, const uint4* rho_p, uint4* __restrict__ ftab) {
     __shared__ uint4 tab[512];
-    gf_table_build(tab, gf_from_u64(rho_lo, rho_hi), threadIdx.x, blockDim.x);
+    gf_table_build(tab, challenge_arg(rho_lo, rho_hi, rho_p), threadIdx.x, blockDim.x);
     __syncthreads();
     const uint64_t q = 1ull << (t - 2);
     uint64_t idx = grid_tid();
@@ -599,9 +635,9 @@ __global__ __launch_bounds__(kT) void k_level1_pattern(const uint64_t* __restric
 
 // Both halves l, r (2h entries each) folded by rho in one launch.
 __global__ __launch_bounds__(kT) void k_fold_pair(uint4* __restrict__ l, uint4* __restrict__ r, uint64_t h,
-                                                  uint64_t rho_lo, uint64_t rho_hi) {
+                                                  uint64_t rho_lo, uint64_t rho_hi, const uint4* rho_p) {
     __shared__ uint4 tab[512];
-    gf_table_build(tab, gf_from_u64(rho_lo, rho_hi), threadIdx.x, blockDim.x);
+    gf_table_build(tab, challenge_arg(rho_lo, rho_hi, rho_p), threadIdx.x, blockDim.x);
     __syncthreads();
     for (uint64_t i = grid_tid(); i < 2 * h; i += grid_threads()) {
         uint4* v = i < h ? l : r;
@@ -641,21 +677,42 @@ __global__ void k_xi_combined(const uint64_t* __restrict__ packed, int t, int s,
     out[b] = make_uint4(x0, x1, x2, x3);
 }
 
-// out[c] = sum_b eq[b] bit(c, b) from column-major rows; block per column.
-__global__ __launch_bounds__(kT) void k_fold_rows_eq(const uint64_t* __restrict__ rows, int t,
-                                                     const uint4* __restrict__ eq, uint4* __restrict__ out) {
-    const uint64_t words = 1ull << (t - 6);
-    const uint64_t c = blockIdx.x;
-    gf128 acc[1] = {gf_zero()};
-    for (uint64_t wi = threadIdx.x; wi < words; wi += blockDim.x) {
-        uint64_t x = rows[c * words + wi];
-        while (x) {
-            int j = __builtin_ctzll(x);
-            acc[0] = gf_add(acc[0], gf_load(&eq[wi * 64 + j]));
-            x &= x - 1;
+// partial[chunk][c] = sum over the chunk's rows b of eq[b] bit(c, b), from
+// column-major rows. Block (group of 64 columns, chunk of kRowsEq rows): the
+// chunk of eq is staged in LDS, thread (column, quarter) walks a quarter of
+// the column's words in the chunk and gathers eq per set bit from LDS.
+constexpr int kRowsEq = 2048;
+__global__ __launch_bounds__(kT) void k_fold_rows_eq(const uint64_t* __restrict__ rows, int t, int s,
+                                                     const uint4* __restrict__ eq, uint4* __restrict__ partial) {
+    __shared__ uint4 eqs[kRowsEq];
+    const uint64_t words = 1ull << (t - 6), cols = 1ull << s;
+    const uint32_t chunk_rows = (uint32_t)std::min<uint64_t>(kRowsEq, 1ull << t);
+    const uint64_t row0 = (uint64_t)blockIdx.y * chunk_rows;
+    for (uint32_t i = threadIdx.x; i < chunk_rows; i += blockDim.x) eqs[i] = eq[row0 + i];
+    __syncthreads();
+    const uint64_t c = (uint64_t)blockIdx.x * 64 + (threadIdx.x >> 2);
+    const uint32_t q = threadIdx.x & 3, cw = chunk_rows / 64;
+    uint32_t x0 = 0, x1 = 0, x2 = 0, x3 = 0;
+    if (c < cols) {
+        const uint64_t* col = rows + c * words + row0 / 64;
+        for (uint32_t w = q; w < cw; w += 4) {
+            uint64_t x = col[w];
+            while (x) {
+                uint4 e = eqs[w * 64 + __builtin_ctzll(x)];
+                x0 ^= e.x, x1 ^= e.y, x2 ^= e.z, x3 ^= e.w;
+                x &= x - 1;
+            }
         }
     }
-    block_xor_store<1>(acc, out + c);
+    // the four quarters of a column are adjacent lanes
+#pragma unroll
+    for (int off = 1; off < 4; off <<= 1) {
+        x0 ^= __shfl_xor(x0, off);
+        x1 ^= __shfl_xor(x1, off);
+        x2 ^= __shfl_xor(x2, off);
+        x3 ^= __shfl_xor(x3, off);
+    }
+    if (q == 0 && c < cols) partial[(uint64_t)blockIdx.y * cols + c] = make_uint4(x0, x1, x2, x3);
 }
 
 // [sum m0 w0, sum m1 w1, sum (m0+m1)(w0+w1)] over pairs (2k, 2k+1)
@@ -947,6 +1004,81 @@ __global__ void k_gather_nodes(const uint4* __restrict__ tree, const uint64_t* _
     out[2 * i + 1] = tree[2 * idx[i] + 1];
 }
 
+// ------------------------------------------------ device Fiat-Shamir steps
+__device__ __forceinline__ void gf_bytes(gf128 v, uint8_t* b) {
+    for (int w = 0; w < 4; w++)
+        for (int k = 0; k < 4; k++) b[4 * w + k] = (uint8_t)(v.w[w] >> (8 * k));
+}
+__device__ __forceinline__ gf128 gf_from_bytes(const uint8_t* b) {
+    gf128 v;
+    for (int w = 0; w < 4; w++)
+        v.w[w] = (uint32_t)b[4 * w] | ((uint32_t)b[4 * w + 1] << 8) | ((uint32_t)b[4 * w + 2] << 16) |
+                 ((uint32_t)b[4 * w + 3] << 24);
+    return v;
+}
+
+// One GKR round (gkr.rs:301-310 prover_message / verifier_message) on one
+// wavefront, after the block partials were reduced into sums[0..2):
+// message = factor * corr * sums, absorb it,
+// squeeze the challenge, factor *= eq_factor(challenge, z)
+// (src/bitz/mod.rs:328-331). The hasher state is worked on in LDS.
+constexpr uint32_t kFsThreads = 64;
+__global__ __launch_bounds__(kFsThreads) void k_gkr_fs_round(const uint4* __restrict__ sums, bitz::B3State* st,
+                                                             uint4* factor, const uint4* internal, const uint4* corr,
+                                                             uint4* rho, uint4* msgs, uint32_t round) {
+    __shared__ bitz::B3State sst;
+    __shared__ __align__(16) uint8_t stage[64];
+    __shared__ __align__(16) uint8_t msg[32];
+    __shared__ __align__(16) uint8_t rb[16];
+    constexpr uint32_t kWords = sizeof(bitz::B3State) / 4;
+    const uint32_t lane = threadIdx.x;
+    for (uint32_t i = lane; i < kWords; i += blockDim.x) ((uint32_t*)&sst)[i] = ((const uint32_t*)st)[i];
+    __syncthreads();
+    gf128 f = gf_mul(gf_load(factor), gf_load(&corr[round]));
+    gf128 m0 = gf_mul(f, gf_load(&sums[0])), m1 = gf_mul(f, gf_load(&sums[1]));
+    if (lane == 0) {
+        for (int i = 0; i < 4; i++) {
+            ((uint32_t*)msg)[i] = m0.w[i];
+            ((uint32_t*)msg)[4 + i] = m1.w[i];
+        }
+        gf_store(&msgs[2 * round], m0);
+        gf_store(&msgs[2 * round + 1], m1);
+    }
+    __syncthreads();
+    b3s::absorb_slice(&sst, msg, 32, stage);
+    b3s::challenge16(&sst, rb, stage);
+    gf128 r = gf_load((const uint4*)rb), z = gf_load(&internal[round]);
+    gf128 e = gf_add(gf_mul(r, z), gf_mul(gf_add(r, gf_one()), gf_add(z, gf_one())));
+    gf128 fnew = gf_mul(gf_load(factor), e);
+    __syncthreads();
+    if (lane == 0) {
+        gf_store(&rho[round], r);
+        gf_store(factor, fnew);
+    }
+    for (uint32_t i = lane; i < kWords; i += blockDim.x) ((uint32_t*)st)[i] = ((const uint32_t*)&sst)[i];
+}
+
+__global__ __launch_bounds__(kFsThreads) void k_transcript_challenges(bitz::B3State* st, const uint8_t* __restrict__ msgs,
+                                                                      uint32_t n, uint32_t msg_len,
+                                                                      uint8_t* __restrict__ out) {
+    __shared__ bitz::B3State sst;
+    __shared__ __align__(16) uint8_t stage[1600];
+    __shared__ __align__(16) uint8_t data[1536];
+    __shared__ __align__(16) uint8_t rb[16];
+    constexpr uint32_t kWords = sizeof(bitz::B3State) / 4;
+    const uint32_t lane = threadIdx.x;
+    for (uint32_t i = lane; i < kWords; i += blockDim.x) ((uint32_t*)&sst)[i] = ((const uint32_t*)st)[i];
+    __syncthreads();
+    for (uint32_t m = 0; m < n; m++) {
+        for (uint32_t i = lane; i < msg_len; i += blockDim.x) data[i] = msgs[(size_t)m * msg_len + i];
+        __syncthreads();
+        b3s::absorb_slice(&sst, data, msg_len, stage);
+        b3s::challenge16(&sst, rb, stage);
+        if (lane < 16) out[16 * m + lane] = rb[lane];
+        __syncthreads();
+    }
+}
+
 }  // namespace
 
 // ================================================================ host side
@@ -966,7 +1098,20 @@ Scratch::~Scratch() {
     if (host) (void)hipHostFree(host);
 }
 
+static void split16(const uint8_t* p, uint64_t* lo, uint64_t* hi) {
+    if (!p) {  // the challenge comes from device memory instead
+        *lo = *hi = 0;
+        return;
+    }
+    memcpy(lo, p, 8);
+    memcpy(hi, p + 8, 8);
+}
+
 static int sums_finish(bitz_ctx* ctx, unsigned nblocks, int K, void* out_dev) {
+    if (ctx->defer_sums) {  // the caller reduces the block partials itself
+        ctx->deferred_nb = nblocks;
+        return BITZ_OK;
+    }
     Scratch& sc = ctx->scratch;
     uint4* parts = (uint4*)sc.partials.ptr;
     uint4* dst = out_dev ? (uint4*)out_dev : (uint4*)sc.result.ptr;
@@ -1117,15 +1262,14 @@ int tensor_table(bitz_ctx* ctx, const void* a, const void* b, int n, void* out) 
 }
 
 int gkr_fold_round_sums(bitz_ctx* ctx, void* l, void* r, uint64_t h, const uint8_t rho[16], const void* eq_c,
-                        int s_c, const void* eq_y, int send_one, void* out_dev) {
+                        int s_c, const void* eq_y, int send_one, void* out_dev, const void* rho_dev) {
     uint32_t R = round_chunk_r(h, s_c);
     unsigned nb = round_blocks(h, R);
     int rc = ctx->scratch.ensure((size_t)nb * 2 * 16);
     if (rc) return rc;
     uint64_t lo, hi;
-    memcpy(&lo, rho, 8);
-    memcpy(&hi, rho + 8, 8);
-    FoldSrc src{(uint4*)l, (uint4*)r, lo, hi};
+    split16(rho, &lo, &hi);
+    FoldSrc src{(uint4*)l, (uint4*)r, lo, hi, (const uint4*)rho_dev};
     if (eq_c) {
         TimedLaunch tl(ctx, "gkr_fold_round_sums", 12.0 * 16.0 * h, h * (4.0 * kGfMulOps + 4.0 * kGfTabMulOps));
         hipLaunchKernelGGL(k_gkr_round_sums<FoldSrc>, dim3(nb), dim3(kT), 0, ctx->stream, src, h, (const uint4*)eq_c,
@@ -1173,11 +1317,6 @@ int gkr_leaf_round_sums(bitz_ctx* ctx, const uint64_t* packed, const void* A, in
     return sums_finish(ctx, nb, 2, out_dev);
 }
 
-static void split16(const uint8_t* p, uint64_t* lo, uint64_t* hi) {
-    memcpy(lo, p, 8);
-    memcpy(hi, p + 8, 8);
-}
-
 int fold_halves(bitz_ctx* ctx, void* v, uint64_t h, const uint8_t rho[16]) {
     uint64_t lo, hi;
     split16(rho, &lo, &hi);
@@ -1187,12 +1326,12 @@ int fold_halves(bitz_ctx* ctx, void* v, uint64_t h, const uint8_t rho[16]) {
     return BITZ_OK;
 }
 
-int fold_pair(bitz_ctx* ctx, void* l, void* r, uint64_t h, const uint8_t rho[16]) {
+int fold_pair(bitz_ctx* ctx, void* l, void* r, uint64_t h, const uint8_t rho[16], const void* rho_dev) {
     uint64_t lo, hi;
     split16(rho, &lo, &hi);
     TimedLaunch tl(ctx, "fold_pair", 6.0 * 16.0 * h, 2.0 * h * kGfTabMulOps);
     hipLaunchKernelGGL(k_fold_pair, dim3(blocks_for(2 * h, 4)), dim3(kT), 0, ctx->stream, (uint4*)l, (uint4*)r, h,
-                       lo, hi);
+                       lo, hi, (const uint4*)rho_dev);
     BITZ_LAUNCH_CHECK();
     return BITZ_OK;
 }
@@ -1207,7 +1346,7 @@ int gkr_leaf_fold(bitz_ctx* ctx, const uint64_t* packed, const void* A, int t, i
     uint4* ftab = (uint4*)ctx->scratch.leaf_tab.ptr;
     TimedLaunch tl(ctx, "gkr_leaf_fold", 2.0 * 16.0 * h + (double)(1ull << (t + s - 3)), q * 8.0 * kGfTabMulOps);
     hipLaunchKernelGGL(k_leaf_ftab, dim3((unsigned)((q * 8 + kT - 1) / kT)), dim3(kT), 0, ctx->stream,
-                       (const uint4*)A, t, lo, hi, ftab);
+                       (const uint4*)A, t, lo, hi, (const uint4*)nullptr, ftab);
     hipLaunchKernelGGL(k_leaf_fold_pattern, dim3(blocks_for(h, 4)), dim3(kT), 0, ctx->stream, packed, t, s,
                        (const uint4*)ftab, (uint4*)l_out, (uint4*)r_out);
     BITZ_LAUNCH_CHECK();
@@ -1215,7 +1354,7 @@ int gkr_leaf_fold(bitz_ctx* ctx, const uint64_t* packed, const void* A, int t, i
 }
 
 int gkr_leaf_round1_sums(bitz_ctx* ctx, const uint64_t* packed, const void* A, int t, int s, const uint8_t rho0[16],
-                         const void* eq_c, const void* eq_y, int send_one, void* out_dev) {
+                         const void* eq_c, const void* eq_y, int send_one, void* out_dev, const void* rho0_dev) {
     BITZ_REQUIRE(t >= 3, BITZ_ERR_SHAPE, "gkr_leaf_round1_sums: t < 3");
     uint64_t lo, hi;
     split16(rho0, &lo, &hi);
@@ -1231,7 +1370,7 @@ int gkr_leaf_round1_sums(bitz_ctx* ctx, const uint64_t* packed, const void* A, i
     TimedLaunch tl(ctx, "gkr_leaf_round1_sums", (double)(1ull << (t + s - 3)) + q8 * 8192.0,
                    (q * 8.0 + q8 * 1024.0 + 2.0 * threads) * kGfMulOps);
     hipLaunchKernelGGL(k_leaf_ftab, dim3((unsigned)((q * 8 + kT - 1) / kT)), dim3(kT), 0, ctx->stream,
-                       (const uint4*)A, t, lo, hi, ftab);
+                       (const uint4*)A, t, lo, hi, (const uint4*)rho0_dev, ftab);
     hipLaunchKernelGGL(k_leaf_q1tab, dim3((unsigned)((q8 * 256 + kT - 1) / kT)), dim3(kT), 0, ctx->stream,
                        (const uint4*)ftab, t, (const uint4*)eq_y, send_one, q1tab);
     hipLaunchKernelGGL(k_leaf_pattern_sums<8>, dim3(nb), dim3(kT), 0, ctx->stream, packed, t, s, q8, q8,
@@ -1241,7 +1380,8 @@ int gkr_leaf_round1_sums(bitz_ctx* ctx, const uint64_t* packed, const void* A, i
 }
 
 int gkr_leaf_fold2(bitz_ctx* ctx, const uint64_t* packed, const void* A, int t, int s, const uint8_t rho0[16],
-                   const uint8_t rho1[16], const void* eq_c, void* l_out, void* r_out) {
+                   const uint8_t rho1[16], const void* eq_c, void* l_out, void* r_out, const void* rho0_dev,
+                   const void* rho1_dev) {
     BITZ_REQUIRE(t >= 3, BITZ_ERR_SHAPE, "gkr_leaf_fold2: t < 3");
     uint64_t lo0, hi0, lo1, hi1;
     split16(rho0, &lo0, &hi0);
@@ -1253,9 +1393,9 @@ int gkr_leaf_fold2(bitz_ctx* ctx, const uint64_t* packed, const void* A, int t, 
     TimedLaunch tl(ctx, "gkr_leaf_fold2", 2.0 * 16.0 * h + (double)(h / 8),
                    (q * 8.0 + q8 * 32.0) * kGfTabMulOps + (eq_c ? (double)h : 0.0) * kGfMulOps);
     hipLaunchKernelGGL(k_leaf_ftab, dim3((unsigned)((q * 8 + kT - 1) / kT)), dim3(kT), 0, ctx->stream,
-                       (const uint4*)A, t, lo0, hi0, ftab);
+                       (const uint4*)A, t, lo0, hi0, (const uint4*)rho0_dev, ftab);
     hipLaunchKernelGGL(k_leaf_gtab, dim3((unsigned)((q8 * 32 + kT - 1) / kT)), dim3(kT), 0, ctx->stream,
-                       (const uint4*)ftab, t, lo1, hi1, gtab);
+                       (const uint4*)ftab, t, lo1, hi1, (const uint4*)rho1_dev, gtab);
     hipLaunchKernelGGL(k_pattern_fold<0>, dim3(blocks_for(h, 4)), dim3(kT), 0, ctx->stream, packed, t, s,
                        (const uint4*)gtab, (const uint4*)eq_c, (uint4*)l_out, (uint4*)r_out);
     BITZ_LAUNCH_CHECK();
@@ -1287,7 +1427,7 @@ int gkr_level1_round0_sums(bitz_ctx* ctx, const uint64_t* packed, const void* A,
 }
 
 int gkr_level1_fold(bitz_ctx* ctx, const uint64_t* packed, const void* A, int t, int s, const uint8_t rho[16],
-                    const void* eq_c, void* l_out, void* r_out) {
+                    const void* eq_c, void* l_out, void* r_out, const void* rho_dev) {
     BITZ_REQUIRE(t >= 3, BITZ_ERR_SHAPE, "gkr_level1_fold: t < 3");
     uint64_t lo, hi;
     split16(rho, &lo, &hi);
@@ -1300,7 +1440,7 @@ int gkr_level1_fold(bitz_ctx* ctx, const uint64_t* packed, const void* A, int t,
     hipLaunchKernelGGL(k_level1_vtab, dim3((unsigned)((Y1 + kT - 1) / kT)), dim3(kT), 0, ctx->stream,
                        (const uint4*)A, t, vtab);
     hipLaunchKernelGGL(k_level1_gtab, dim3((unsigned)((Y3 * 32 + kT - 1) / kT)), dim3(kT), 0, ctx->stream,
-                       (const uint4*)vtab, t, lo, hi, gtab);
+                       (const uint4*)vtab, t, lo, hi, (const uint4*)rho_dev, gtab);
     hipLaunchKernelGGL(k_pattern_fold<1>, dim3(blocks_for(h, 4)), dim3(kT), 0, ctx->stream, packed, t, s,
                        (const uint4*)gtab, (const uint4*)eq_c, (uint4*)l_out, (uint4*)r_out);
     BITZ_LAUNCH_CHECK();
@@ -1320,6 +1460,26 @@ int gkr_level2(bitz_ctx* ctx, const uint64_t* packed, const void* A, int t, int 
                        (const uint4*)vtab, t, ltab);
     hipLaunchKernelGGL(k_level2_pattern, dim3(blocks_for(n, 4)), dim3(kT), 0, ctx->stream, packed, t, s,
                        (const uint4*)ltab, (uint4*)out);
+    BITZ_LAUNCH_CHECK();
+    return BITZ_OK;
+}
+
+int gkr_level3(bitz_ctx* ctx, const uint64_t* packed, const void* A, int t, int s, void* out) {
+    BITZ_REQUIRE(t >= 3, BITZ_ERR_SHAPE, "gkr_level3: t < 3");
+    const uint64_t Y1 = 1ull << (t - 1), Y2 = 1ull << (t - 2), Y3 = 1ull << (t - 3), n = 1ull << (t + s - 3);
+    if (!ctx->scratch.leaf_tab.alloc(leaf_tab_bytes(t))) return BITZ_ERR_ALLOC;
+    uint4* vtab = (uint4*)ctx->scratch.leaf_tab.ptr;
+    uint4* ltab = leaf_tab_hi(ctx, t);
+    uint4* l3tab = ltab + Y2 * 16;
+    TimedLaunch tl(ctx, "gkr_level3", 16.0 * n + (double)(n / 8), (Y1 + Y2 * 16.0 + Y3 * 256.0) * kGfMulOps);
+    hipLaunchKernelGGL(k_level1_vtab, dim3((unsigned)((Y1 + kT - 1) / kT)), dim3(kT), 0, ctx->stream,
+                       (const uint4*)A, t, vtab);
+    hipLaunchKernelGGL(k_level2_ltab, dim3((unsigned)((Y2 * 16 + kT - 1) / kT)), dim3(kT), 0, ctx->stream,
+                       (const uint4*)vtab, t, ltab);
+    hipLaunchKernelGGL(k_level3_tab, dim3((unsigned)((Y3 * 256 + kT - 1) / kT)), dim3(kT), 0, ctx->stream,
+                       (const uint4*)ltab, t, l3tab);
+    hipLaunchKernelGGL(k_level3_pattern, dim3(blocks_for(n, 4)), dim3(kT), 0, ctx->stream, packed, t, s,
+                       (const uint4*)l3tab, (uint4*)out);
     BITZ_LAUNCH_CHECK();
     return BITZ_OK;
 }
@@ -1354,10 +1514,16 @@ int xi_combined(bitz_ctx* ctx, const uint64_t* packed, int t, int s, const void*
 }
 
 int fold_rows_eq(bitz_ctx* ctx, const uint64_t* rows, int t, int s, const void* eq, void* out) {
-    hipLaunchKernelGGL(k_fold_rows_eq, dim3((unsigned)(1ull << s)), dim3(kT), 0, ctx->stream, rows, t,
-                       (const uint4*)eq, (uint4*)out);
+    BITZ_REQUIRE(t >= 6, BITZ_ERR_SHAPE, "fold_rows_eq: t < 6");
+    const uint64_t cols = 1ull << s, chunk_rows = std::min<uint64_t>(kRowsEq, 1ull << t);
+    const uint64_t chunks = (1ull << t) / chunk_rows, groups = (cols + 63) / 64;
+    if (!ctx->scratch.fold_partial.alloc(chunks * cols * 16)) return BITZ_ERR_ALLOC;
+    TimedLaunch tl(ctx, "fold_rows_eq", (double)((1ull << (t + s)) / 8) + 16.0 * (1ull << t), 0.0);
+    hipLaunchKernelGGL(k_fold_rows_eq, dim3((unsigned)groups, (unsigned)chunks), dim3(kT), 0, ctx->stream, rows, t, s,
+                       (const uint4*)eq, (uint4*)ctx->scratch.fold_partial.ptr);
     BITZ_LAUNCH_CHECK();
-    return BITZ_OK;
+    return reduce_columns(ctx, (const uint4*)ctx->scratch.fold_partial.ptr, (uint32_t)chunks, (uint32_t)cols,
+                          (uint4*)out);
 }
 
 int round_poly(bitz_ctx* ctx, const void* m, const void* w, uint64_t n, void* out_dev) {
@@ -1502,6 +1668,28 @@ int pow_find(bitz_ctx* ctx, const uint8_t prefix[16], const uint8_t seed[16], ui
         }
         BITZ_REQUIRE(base < (1ull << 44), BITZ_ERR_DEVICE, "pow_find: search exhausted");
     }
+}
+
+int gkr_fs_round(bitz_ctx* ctx, unsigned nb, void* state, void* factor, const void* internal, const void* corr,
+                 void* rho, void* msgs, uint32_t round) {
+    int rc = ctx->scratch.ensure(64);
+    if (rc) return rc;
+    uint4* sums = (uint4*)ctx->scratch.result.ptr;
+    hipLaunchKernelGGL(k_xor_partials<2>, dim3(1), dim3(kT), 0, ctx->stream, (const uint4*)ctx->scratch.partials.ptr,
+                       nb, sums);
+    hipLaunchKernelGGL(k_gkr_fs_round, dim3(1), dim3(kFsThreads), 0, ctx->stream, (const uint4*)sums,
+                       (bitz::B3State*)state, (uint4*)factor, (const uint4*)internal, (const uint4*)corr,
+                       (uint4*)rho, (uint4*)msgs, round);
+    BITZ_LAUNCH_CHECK();
+    return BITZ_OK;
+}
+
+int transcript_challenges(bitz_ctx* ctx, void* state, const void* msgs, uint32_t n, uint32_t msg_len, void* out) {
+    BITZ_REQUIRE(msg_len <= 1536, BITZ_ERR_ARG, "transcript_challenges: messages of at most 1536 bytes");
+    hipLaunchKernelGGL(k_transcript_challenges, dim3(1), dim3(kFsThreads), 0, ctx->stream, (bitz::B3State*)state,
+                       (const uint8_t*)msgs, n, msg_len, (uint8_t*)out);
+    BITZ_LAUNCH_CHECK();
+    return BITZ_OK;
 }
 
 int dot(bitz_ctx* ctx, const void* a, const void* b, uint64_t n, void* out_dev) {

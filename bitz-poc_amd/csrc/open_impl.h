@@ -19,22 +19,25 @@ int gkr_round_sums(bitz_ctx* ctx, const void* l, const void* r, uint64_t h, cons
 // the previous round's fold by rho (halves of 2h entries -> h) fused with
 // this round's sums over the folded halves
 int gkr_fold_round_sums(bitz_ctx* ctx, void* l, void* r, uint64_t h, const uint8_t rho[16], const void* eq_c,
-                        int s_c, const void* eq_y, int send_one, void* out_dev);
+                        int s_c, const void* eq_y, int send_one, void* out_dev, const void* rho_dev = nullptr);
 int gkr_leaf_round_sums(bitz_ctx* ctx, const uint64_t* packed, const void* A, int t, int s, const void* eq_c,
                         int s_c, const void* eq_y, int send_one, void* out_dev);
 int fold_halves(bitz_ctx* ctx, void* v, uint64_t h, const uint8_t rho[16]);
 // both halves l and r (2h entries each) folded by rho in one launch
-int fold_pair(bitz_ctx* ctx, void* l, void* r, uint64_t h, const uint8_t rho[16]);
+// (rho_dev: the challenge in device memory, from the device transcript; rho unused then)
+int fold_pair(bitz_ctx* ctx, void* l, void* r, uint64_t h, const uint8_t rho[16], const void* rho_dev = nullptr);
 int gkr_leaf_fold(bitz_ctx* ctx, const uint64_t* packed, const void* A, int t, int s, const uint8_t rho[16],
                   void* l_out, void* r_out);
 // round 1 of the leaf layer (after round 0 folded by rho0) from 8-bit row
 // patterns; weights eq_y[y] * eq_c[c] over all s column coordinates
 int gkr_leaf_round1_sums(bitz_ctx* ctx, const uint64_t* packed, const void* A, int t, int s, const uint8_t rho0[16],
-                         const void* eq_c, const void* eq_y, int send_one, void* out_dev);
+                         const void* eq_c, const void* eq_y, int send_one, void* out_dev,
+                         const void* rho0_dev = nullptr);
 // the leaf layer folded by rho0 then rho1 into halves of 2^(t+s-3) entries;
 // with eq_c the right half is written multiplied by eq_c[c] (c = low s bits)
 int gkr_leaf_fold2(bitz_ctx* ctx, const uint64_t* packed, const void* A, int t, int s, const uint8_t rho0[16],
-                   const uint8_t rho1[16], const void* eq_c, void* l_out, void* r_out);
+                   const uint8_t rho1[16], const void* eq_c, void* l_out, void* r_out, const void* rho0_dev = nullptr,
+                   const void* rho1_dev = nullptr);
 // round sums whose right half gets eq_c[x & (2^s-1)] multiplied in (in place);
 // weights eq_y[x >> s] only
 int gkr_premul_round_sums(bitz_ctx* ctx, const void* l, void* r, uint64_t h, const void* eq_c, int s,
@@ -50,9 +53,10 @@ int tensor_table(bitz_ctx* ctx, const void* a, const void* b, int n, void* out);
 int gkr_level1_round0_sums(bitz_ctx* ctx, const uint64_t* packed, const void* A, int t, int s, const void* eq_c,
                            const void* eq_y, int send_one, void* out_dev);
 int gkr_level1_fold(bitz_ctx* ctx, const uint64_t* packed, const void* A, int t, int s, const uint8_t rho[16],
-                    const void* eq_c, void* l_out, void* r_out);
-// level 2 of the product tree (2^(t+s-2) entries) from the bits
+                    const void* eq_c, void* l_out, void* r_out, const void* rho_dev = nullptr);
+// levels 2 and 3 of the product tree (2^(t+s-2), 2^(t+s-3) entries) from the bits
 int gkr_level2(bitz_ctx* ctx, const uint64_t* packed, const void* A, int t, int s, void* out);
+int gkr_level3(bitz_ctx* ctx, const uint64_t* packed, const void* A, int t, int s, void* out);
 int gkr_level1(bitz_ctx* ctx, const uint64_t* packed, const void* A, int t, int s, void* out);
 int product_level(bitz_ctx* ctx, const void* in, uint64_t n, void* out);
 int xi_combined(bitz_ctx* ctx, const uint64_t* packed, int t, int s, const void* w2, void* out);
@@ -77,6 +81,16 @@ int mul_inplace(bitz_ctx* ctx, void* v, const void* b, uint64_t n);
 int scale(bitz_ctx* ctx, const void* a, uint64_t n, const uint8_t c[16], void* out);
 
 // commit pipeline (commit.hip)
+// One GKR round's Fiat-Shamir step on the device, after a sums launch whose
+// block partials (nb blocks x 2) are in the scratch partials: the sums are
+// reduced, scaled by factor * corr[round], absorbed as the round message
+// (msgs[2 round..]), and the challenge is squeezed into rho[round]; factor
+// then takes eq_factor(rho, internal[round]).
+int gkr_fs_round(bitz_ctx* ctx, unsigned nb, void* state, void* factor, const void* internal, const void* corr,
+                 void* rho, void* msgs, uint32_t round);
+// Device transcript check: n x (absorb_slice(msg_i), 16-byte challenge) on a
+// device copy of `state`; challenges to out (device, n x 16 B).
+int transcript_challenges(bitz_ctx* ctx, void* state, const void* msgs, uint32_t n, uint32_t msg_len, void* out);
 int commit_device(bitz_ctx* ctx, const void* packed, int log_msg_len, int log_batch, int log_inv_rate,
                   void* codeword, void* tree);
 

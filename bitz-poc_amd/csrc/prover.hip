@@ -12,6 +12,7 @@
 // available to restate).
 #include <algorithm>
 #include <set>
+#include <thread>
 
 #include "host_proto.h"
 #include "open_impl.h"
@@ -165,7 +166,7 @@ inline void u128_put(u128 v, uint8_t* p) {
 
 // eq_table_mod_q (src/ligerito_flock.rs:1330-1347): each coordinate is
 // appended as the new lowest index bit.
-std::vector<u128> eq_table_mod_q(const std::vector<u128>& r, u128 q) {
+std::vector<u128> eq_table_mod_q_serial(const std::vector<u128>& r, u128 q) {
     std::vector<u128> t = {1 % q};
     for (u128 c : r) {
         std::vector<u128> nx;
@@ -179,6 +180,26 @@ std::vector<u128> eq_table_mod_q(const std::vector<u128>& r, u128 q) {
         t.swap(nx);
     }
     return t;
+}
+
+// The same table as the tensor product of the tables of the first and the
+// last coordinates (entry x = first[x >> k] * last[x & (2^k - 1)]), the
+// products spread over host threads.
+std::vector<u128> eq_table_mod_q(const std::vector<u128>& r, u128 q) {
+    const size_t n = r.size();
+    if (n < 12) return eq_table_mod_q_serial(r, q);
+    const size_t k = n / 2;
+    std::vector<u128> hi = eq_table_mod_q_serial(std::vector<u128>(r.begin(), r.end() - k), q);
+    std::vector<u128> lo = eq_table_mod_q_serial(std::vector<u128>(r.end() - k, r.end()), q);
+    std::vector<u128> out(1ull << n);
+    unsigned nt = std::max(1u, std::min(8u, std::thread::hardware_concurrency()));
+    std::vector<std::thread> pool;
+    for (unsigned w = 0; w < nt; w++)
+        pool.emplace_back([&, w]() {
+            for (size_t x = w; x < out.size(); x += nt) out[x] = mulmod(hi[x >> k], lo[x & ((1ull << k) - 1)], q);
+        });
+    for (auto& th : pool) th.join();
+    return out;
 }
 
 // 256-bit transcript draw reduced mod q (transcript_uniform_mod, src/prime_sampling.rs:22-33)
@@ -276,7 +297,7 @@ struct Prover {
     LigConfig cfg;
     G g;
     // device buffers
-    DevBuf packed, rowimg, colimg, folds, roww, levels, eqc, eqy, eqcinv, tmp0, tmp1, tmp2, tmp3, qdev, pt;
+    DevBuf packed, rowimg, colimg, folds, roww, levels, eqc, eqy, eqcinv, tmp0, tmp1, tmp2, tmp3, qdev, pt, fsbuf;
     std::vector<DevBuf*> lig_bufs;
 
     int upload_gf(const std::vector<G>& v, DevBuf& buf) {
@@ -389,6 +410,8 @@ struct Prover {
             if (!eqcinv.alloc(16ull << s)) return BITZ_ERR_ALLOC;
             if ((rc = tensor_table(ctx, pt.ptr, (const uint4*)pt.ptr + s, s, eqcinv.ptr))) return rc;
         }
+        if (per_layer && ctx->device_fs)
+            return gkr_layer_device(it, l, r, kind, internal, corr, weigh, claim);
         bool weighted = false;  // r currently carries eq_c
         // a round's fold by its challenge is enqueued with the next round's
         // sums: fused into one pass where the right half carries eq_c (two
@@ -501,11 +524,12 @@ struct Prover {
             lv[j] = base + off;
             off += 1ull << (t + s - j);
         }
-        // level 1 stays implicit (its layer reads the bits); level 2 comes
-        // straight from the bits, the upper levels from products
+        // level 1 stays implicit (its layer reads the bits); levels 2 and 3
+        // come straight from the bits, the upper levels from products
         int rc = gkr_level2(ctx, (const uint64_t*)packed.ptr, rowimg.ptr, t, s, lv[2]);
         if (rc) return rc;
-        for (int j = 2; j + 1 < t; j++)
+        if ((rc = gkr_level3(ctx, (const uint64_t*)packed.ptr, rowimg.ptr, t, s, lv[3]))) return rc;
+        for (int j = 3; j + 1 < t; j++)
             if ((rc = product_level(ctx, lv[j], 1ull << (t + s - j), lv[j + 1]))) return rc;
         std::vector<G> internal(zeta.rbegin(), zeta.rend());
         G cl = gzero();
@@ -518,6 +542,133 @@ struct Prover {
         if ((rc = gkr_layer(it, lv[1], lv[1] + q, kLeaf, internal, &cl))) return rc;
         point->assign(internal.rbegin(), internal.rend());
         *claim = cl;
+        return BITZ_OK;
+    }
+
+    // The rounds of one layer with the transcript on the device: every round's
+    // launches (fold, sums, Fiat-Shamir step) are enqueued at once, the
+    // challenges come from the device copy of the transcript, and a single
+    // synchronisation at the end brings the messages and challenges back.
+    // The host then replays them through its own transcript, which also
+    // checks every device challenge. Kernel choices are those of gkr_layer.
+    int gkr_layer_device(InnerTranscript& it, uint4* l, uint4* r, LayerKind kind, std::vector<G>& internal,
+                         const std::vector<G>& corr, bool weigh, G* claim) {
+        const bool leaf = kind == kLeaf;
+        const int L = (int)internal.size();
+        int rc;
+        const size_t st_units = (sizeof(B3State) + 15) / 16;
+        const size_t up_units = st_units + 1 + 2 * (size_t)L, down_units = 3 * (size_t)L + 2;
+        if (!fsbuf.alloc((up_units + down_units) * 16)) return BITZ_ERR_ALLOC;
+        {
+            std::vector<uint8_t> up(up_units * 16, 0);
+            B3State st;
+            it.export_state(&st);
+            memcpy(up.data(), &st, sizeof(st));
+            G one = gone();
+            memcpy(up.data() + st_units * 16, &one, 16);
+            memcpy(up.data() + (st_units + 1) * 16, internal.data(), 16 * (size_t)L);
+            memcpy(up.data() + (st_units + 1 + L) * 16, corr.data(), 16 * (size_t)L);
+            BITZ_HIP(hipMemcpyAsync(fsbuf.ptr, up.data(), up.size(), hipMemcpyHostToDevice, ctx->stream));
+            BITZ_HIP(hipStreamSynchronize(ctx->stream));  // `up` is freed on return
+        }
+        uint4* base = (uint4*)fsbuf.ptr;
+        void* st_dev = base;
+        uint4* factor = base + st_units;
+        uint4* internal_dev = factor + 1;
+        uint4* corr_dev = internal_dev + L;
+        uint4* rho = corr_dev + L;
+        uint4* msgs = rho + L;
+        uint4* last = msgs + 2 * L;
+        struct DeferGuard {
+            bitz_ctx* c;
+            ~DeferGuard() { c->defer_sums = false; }
+        } guard{ctx};
+        ctx->defer_sums = true;
+        bool weighted = false, pending_fold = false;
+        const uint4* pending_rho = nullptr;
+        for (int round = 0; round < L; round++) {
+            int rem = L - 1 - round;
+            uint64_t hh = 1ull << rem;
+            int s_c = std::min(s, rem);
+            int send_one = geq(internal[round], gzero()) ? 1 : 0;
+            if (leaf && round == 0)
+                rc = gkr_leaf_round_sums(ctx, (const uint64_t*)packed.ptr, rowimg.ptr, t, s, eqc.ptr, s_c, eqy.ptr,
+                                         send_one, nullptr);
+            else if (leaf && round == 1)
+                rc = gkr_leaf_round1_sums(ctx, (const uint64_t*)packed.ptr, rowimg.ptr, t, s, nullptr, eqc.ptr,
+                                          eqy.ptr, send_one, nullptr, rho);
+            else if (kind == kLevel1 && round == 0)
+                rc = gkr_level1_round0_sums(ctx, (const uint64_t*)packed.ptr, rowimg.ptr, t, s, eqc.ptr, eqy.ptr,
+                                            send_one, nullptr);
+            else if (pending_fold && weighted && rem >= s) {
+                rc = gkr_fold_round_sums(ctx, l, r, hh, nullptr, nullptr, s, eqy.ptr, send_one, nullptr, pending_rho);
+            } else {
+                if (pending_fold && (rc = fold_pair(ctx, l, r, 2 * hh, nullptr, pending_rho))) return rc;
+                if (weighted && rem < s) {
+                    if ((rc = mul_inplace(ctx, r, eqcinv.ptr, 1ull << s))) return rc;
+                    weighted = false;
+                }
+                if (weighted) {
+                    rc = gkr_round_sums_rowonly(ctx, l, r, hh, s, eqy.ptr, send_one, nullptr);
+                } else if (weigh && round == 0) {
+                    rc = gkr_premul_round_sums(ctx, l, r, hh, eqc.ptr, s, eqy.ptr, send_one, nullptr);
+                    weighted = true;
+                } else {
+                    rc = gkr_round_sums(ctx, l, r, hh, eqc.ptr, s_c, eqy.ptr, send_one, nullptr);
+                }
+            }
+            if (rc) return rc;
+            if ((rc = gkr_fs_round(ctx, ctx->deferred_nb, st_dev, factor, internal_dev, corr_dev, rho, msgs,
+                                   (uint32_t)round)))
+                return rc;
+            pending_fold = false;
+            if (kind == kLevel1 && round == 0) {
+                if ((rc = gkr_level1_fold(ctx, (const uint64_t*)packed.ptr, rowimg.ptr, t, s, nullptr,
+                                          weigh ? eqc.ptr : nullptr, l, r, rho)))
+                    return rc;
+                weighted = weigh;
+            } else if (leaf && round == 0) {
+                // leaves stay implicit: round 1 reads the bits again
+            } else if (leaf && round == 1) {
+                if ((rc = gkr_leaf_fold2(ctx, (const uint64_t*)packed.ptr, rowimg.ptr, t, s, nullptr, nullptr,
+                                         weigh ? eqc.ptr : nullptr, l, r, rho, rho + 1)))
+                    return rc;
+                weighted = weigh;
+            } else {
+                pending_fold = true;
+                pending_rho = rho + round;
+            }
+        }
+        if (pending_fold && (rc = fold_pair(ctx, l, r, 1, nullptr, pending_rho))) return rc;
+        ctx->defer_sums = false;
+        BITZ_HIP(hipMemcpyAsync(last, l, 16, hipMemcpyDeviceToDevice, ctx->stream));
+        BITZ_HIP(hipMemcpyAsync(last + 1, r, 16, hipMemcpyDeviceToDevice, ctx->stream));
+        std::vector<uint8_t> down(down_units * 16);
+        if ((rc = download(rho, down.data(), down.size()))) return rc;
+        // replay through the host transcript; every challenge must agree
+        std::vector<G> next;
+        const uint8_t* rho_h = down.data();
+        const uint8_t* msg_h = down.data() + 16 * (size_t)L;
+        for (int round = 0; round < L; round++) {
+            it.prover_message(msg_h + 32 * (size_t)round, 32);
+            uint8_t rb[16];
+            it.verifier_gf(rb);
+            if (memcmp(rb, rho_h + 16 * (size_t)round, 16) != 0) {
+                set_error("GKR: device transcript diverged from the host transcript");
+                return BITZ_ERR_DEVICE;
+            }
+            next.push_back(gfrom(rb));
+        }
+        const uint8_t* last_h = down.data() + 48 * (size_t)L;
+        it.prover_message(last_h, 32);
+        uint8_t rb[16];
+        it.verifier_gf(rb);
+        G rc_ = gfrom(rb);
+        G el = gfrom(last_h), er = gfrom(last_h + 16);
+        *claim = hgf_add(el, hgf_mul(rc_, hgf_add(er, el)));
+        internal.clear();
+        internal.push_back(rc_);
+        internal.insert(internal.end(), next.begin(), next.end());
         return BITZ_OK;
     }
 

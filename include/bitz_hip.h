@@ -56,6 +56,12 @@ int bitz_ctx_synchronize(bitz_ctx* ctx);
  * bitz_ctx_cached_bytes reports the bytes currently cached. Changing the
  * stream with bitz_ctx_set_stream drains the old stream first. */
 int bitz_ctx_trim(bitz_ctx* ctx);
+/* GKR rounds take their Fiat-Shamir challenges from a device copy of the
+ * transcript (enable 1): a layer's rounds are enqueued without host round
+ * trips and the host replays the messages afterwards, checking every
+ * challenge. 0 (the default) runs each round through the host transcript.
+ * Proof bytes are the same either way. */
+int bitz_ctx_set_device_transcript(bitz_ctx* ctx, int enable);
 uint64_t bitz_ctx_cached_bytes(bitz_ctx* ctx);
 /* Device buffers from the context's pool, for callers that have no HIP
  * binding of their own (the reference keeps its tables in Vec<Gf>; these move
@@ -168,9 +174,10 @@ int bitz_gkr_leaf_fold_dev(bitz_ctx* ctx, const void* packed, const void* A, int
                            const uint8_t rho[16], void* l_out, void* r_out);
 /* level 1 of the batched product tree from the bits (GrandProductCircuit::batched_eval, gkr.rs:93) */
 int bitz_gkr_level1_dev(bitz_ctx* ctx, const void* packed, const void* A, int t, int s, void* out);
-/* level 2 (2^(t+s-2) entries) straight from the bits (Forest::fold_table /
- * product_level_into over bit patterns, src/bitz/forest.rs:741-898) */
+/* levels 2 (2^(t+s-2) entries) and 3 (2^(t+s-3)) straight from the bits
+ * (Forest::fold_table / product_level_into over bit patterns, src/bitz/forest.rs:741-898) */
 int bitz_gkr_level2_dev(bitz_ctx* ctx, const void* packed, const void* A, int t, int s, void* out);
+int bitz_gkr_level3_dev(bitz_ctx* ctx, const void* packed, const void* A, int t, int s, void* out);
 /* the GKR layer whose halves are level 1, level 1 left implicit
  * (Forest::prove_jit_level, src/bitz/forest.rs:583-740): round-0 sums with
  * weights eq_y[y] (2^(t-3) entries) * eq_c[c], and the round-0 fold by rho into
@@ -283,6 +290,13 @@ double bitz_server_clock_ms(bitz_server* srv);
 void bitz_server_destroy(bitz_server* srv);
 /* submits n jobs (largest first) to a fresh server and waits for all */
 int bitz_batch_run(int device, uint32_t workers, uint64_t hbm_budget, bitz_job* jobs, uint32_t n);
+
+/* The device transcript (Blake3Transcript, src/transcript/mod.rs:56-96) checked
+ * against the host's: a host transcript absorbs `prefix` (absorb_slice), its
+ * state goes to the device, which then runs n x (absorb_slice(msg_i of
+ * msg_len bytes), 16-byte challenge) and writes the challenges to out_host. */
+int bitz_transcript_challenges_dev(bitz_ctx* ctx, const uint8_t* prefix, uint64_t prefix_len, const uint8_t* msgs,
+                                   uint32_t n, uint32_t msg_len, uint8_t* out_host);
 
 /* BLAKE3 of a host buffer on the host (the transcript hash); _parts absorbs
  * data[0, split) and data[split, len) as two updates (incremental hashing) */

@@ -61,7 +61,7 @@ def test_fold_columns_and_images(t, s):
     assert np.array_equal(imgs, O.pow_many(g, want[:k]))
 
 
-@pytest.mark.parametrize("t,s", [(7, 0), (8, 6), (9, 7), (12, 9)])
+@pytest.mark.parametrize("t,s", [(7, 0), (8, 6), (9, 7), (12, 9), (13, 5), (17, 3)])
 def test_pack_and_sumcheck_tables(t, s):
     rows = rand_rows(t, s, 3)
     packed = ops().pack_columns(dev_u64(rows), t, s)
@@ -297,6 +297,8 @@ def test_gkr_level1_layer_from_bits(t, s):
         ops().fold_halves_(dr, h, rho)
         fl, fr = to_host(dl)[:h], to_host(dr)[:h]
     assert np.array_equal(to_host(ops().gkr_level2(packed, dA, t, s)), lvl2)
+    lvl3 = O.product_level(lvl2) if t + s <= 20 else to_host(ops().product_level(to_dev(lvl2)))
+    assert np.array_equal(to_host(ops().gkr_level3(packed, dA, t, s)), lvl3)
     for so in (False, True):
         got = ops().gkr_level1_round0_sums(packed, dA, t, s, to_dev(eq_c), to_dev(eq_y), so)
         assert tuple(got) == tuple(want_sums[so])

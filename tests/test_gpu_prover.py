@@ -17,20 +17,28 @@ def _rows(t, s, seed):
     return rng.integers(0, 2**64, size=(1 << s) << (t - 6), dtype=np.uint64)
 
 
+@pytest.mark.parametrize("device_fs", [1, 0])
 @pytest.mark.parametrize("t,s,rate,k0,seed", [(11, 9, 2, 4, 5), (12, 8, 1, 4, 6), (10, 10, 3, 3, 7)])
-def test_gpu_proof_bytes_equal_oracle(t, s, rate, k0, seed):
+def test_gpu_proof_bytes_equal_oracle(t, s, rate, k0, seed, device_fs):
+    """with the GKR transcript on the device (default) and on the host"""
+    from bitz_amd import _lib
     from bitz_amd.device import Commitment
     rows = _rows(t, s, seed)
-    log(f"prover parity t={t} s={s} rate=1/{1 << rate} k0={k0}")
-    com = Commitment(ctx(), rows, t, s, k0, rate)
-    pr = P.OracleProver(rows, t, s, rate, k0)
-    assert com.root == pr.root
-    claimed = com.evaluation()
-    assert claimed == pr.evaluation()
-    proof = com.prove(claimed)
-    want = pr.prove(claimed)
-    assert proof == want
-    P.verify_standalone(com.root, t, s, rate, k0, claimed, proof)
+    log(f"prover parity t={t} s={s} rate=1/{1 << rate} k0={k0} device_fs={device_fs}")
+    c = ctx()
+    _lib.call("bitz_ctx_set_device_transcript", c.ptr, device_fs)
+    try:
+        com = Commitment(c, rows, t, s, k0, rate)
+        pr = P.OracleProver(rows, t, s, rate, k0)
+        assert com.root == pr.root
+        claimed = com.evaluation()
+        assert claimed == pr.evaluation()
+        proof = com.prove(claimed)
+        want = pr.prove(claimed)
+        assert proof == want
+        P.verify_standalone(com.root, t, s, rate, k0, claimed, proof)
+    finally:
+        _lib.call("bitz_ctx_set_device_transcript", c.ptr, 0)
 
 
 def test_gpu_proof_verifies_at_2p20_coefficients():

@@ -46,17 +46,35 @@ __device__ __forceinline__ void gf_store(uint4* p, gf128 a) {
     *p = make_uint4(a.w[0], a.w[1], a.w[2], a.w[3]);
 }
 
+// 3-input XOR (v_bitop3_b32, truth table 0x96), per 32-bit half for u64
+__device__ __forceinline__ uint32_t xor3(uint32_t a, uint32_t b, uint32_t c) {
+    return __builtin_amdgcn_bitop3_b32(a, b, c, 0x96);
+}
+__device__ __forceinline__ uint64_t xor3(uint64_t a, uint64_t b, uint64_t c) {
+    return (uint64_t)xor3((uint32_t)a, (uint32_t)b, (uint32_t)c) |
+           ((uint64_t)xor3((uint32_t)(a >> 32), (uint32_t)(b >> 32), (uint32_t)(c >> 32)) << 32);
+}
+
+// bitwise m ? a : b (v_bitop3_b32; src0/1/2 weigh 0xF0/0xCC/0xAA in the table)
+__device__ __forceinline__ uint32_t bitsel(uint32_t m, uint32_t a, uint32_t b) {
+    return __builtin_amdgcn_bitop3_b32(m, a, b, 0xCA);
+}
+
 // 32x32 -> 64 carry-less product via four residue classes.
 __device__ __forceinline__ uint64_t clmul32(uint32_t a, uint32_t b) {
     const uint32_t m = 0x11111111u;
     uint32_t a0 = a & m, a1 = a & (m << 1), a2 = a & (m << 2), a3 = a & (m << 3);
     uint32_t b0 = b & m, b1 = b & (m << 1), b2 = b & (m << 2), b3 = b & (m << 3);
-    uint64_t z0 = ((uint64_t)a0 * b0) ^ ((uint64_t)a1 * b3) ^ ((uint64_t)a2 * b2) ^ ((uint64_t)a3 * b1);
-    uint64_t z1 = ((uint64_t)a0 * b1) ^ ((uint64_t)a1 * b0) ^ ((uint64_t)a2 * b3) ^ ((uint64_t)a3 * b2);
-    uint64_t z2 = ((uint64_t)a0 * b2) ^ ((uint64_t)a1 * b1) ^ ((uint64_t)a2 * b0) ^ ((uint64_t)a3 * b3);
-    uint64_t z3 = ((uint64_t)a0 * b3) ^ ((uint64_t)a1 * b2) ^ ((uint64_t)a2 * b1) ^ ((uint64_t)a3 * b0);
-    const uint64_t M = 0x1111111111111111ull;
-    return (z0 & M) | (z1 & (M << 1)) | (z2 & (M << 2)) | (z3 & (M << 3));
+    uint64_t z0 = xor3((uint64_t)a0 * b0, (uint64_t)a1 * b3, (uint64_t)a2 * b2) ^ ((uint64_t)a3 * b1);
+    uint64_t z1 = xor3((uint64_t)a0 * b1, (uint64_t)a1 * b0, (uint64_t)a2 * b3) ^ ((uint64_t)a3 * b2);
+    uint64_t z2 = xor3((uint64_t)a0 * b2, (uint64_t)a1 * b1, (uint64_t)a2 * b0) ^ ((uint64_t)a3 * b3);
+    uint64_t z3 = xor3((uint64_t)a0 * b3, (uint64_t)a1 * b2, (uint64_t)a2 * b1) ^ ((uint64_t)a3 * b0);
+    // the classes' bit positions are disjoint and cover the word: select
+    const uint32_t M = 0x11111111u;
+    uint32_t lo = bitsel(M, (uint32_t)z0, bitsel(M << 1, (uint32_t)z1, bitsel(M << 2, (uint32_t)z2, (uint32_t)z3)));
+    uint32_t hi = bitsel(M, (uint32_t)(z0 >> 32),
+                         bitsel(M << 1, (uint32_t)(z1 >> 32), bitsel(M << 2, (uint32_t)(z2 >> 32), (uint32_t)(z3 >> 32))));
+    return (uint64_t)lo | ((uint64_t)hi << 32);
 }
 
 // 64x64 -> 128 (Karatsuba over 32-bit halves); out[0] low, out[1] high.
@@ -65,7 +83,7 @@ __device__ __forceinline__ void clmul64(uint64_t a, uint64_t b, uint64_t* lo, ui
     uint32_t bl = (uint32_t)b, bh = (uint32_t)(b >> 32);
     uint64_t l = clmul32(al, bl);
     uint64_t h = clmul32(ah, bh);
-    uint64_t m = clmul32(al ^ ah, bl ^ bh) ^ l ^ h;
+    uint64_t m = xor3(clmul32(al ^ ah, bl ^ bh), l, h);
     *lo = l ^ (m << 32);
     *hi = h ^ (m >> 32);
 }
@@ -86,8 +104,8 @@ __device__ __forceinline__ gf256 gf_mul_wide(gf128 a, gf128 b) {
     clmul64(a0 ^ a1, b0 ^ b1, &m0, &m1);
     gf256 o;
     o.r[0] = l0;
-    o.r[1] = l1 ^ m0 ^ l0 ^ h0;
-    o.r[2] = h0 ^ m1 ^ l1 ^ h1;
+    o.r[1] = xor3(l1, m0, l0) ^ h0;
+    o.r[2] = xor3(h0, m1, l1) ^ h1;
     o.r[3] = h1;
     return o;
 }
@@ -105,12 +123,12 @@ __device__ __forceinline__ void gf256_acc(gf256* acc, gf256 x) {
 // X^128 are folded once more.
 __device__ __forceinline__ gf128 gf_reduce(gf256 x) {
     uint64_t r2 = x.r[2], r3 = x.r[3];
-    uint64_t a0 = r2 ^ (r2 << 1) ^ (r2 << 2) ^ (r2 << 7);
-    uint64_t a1 = (r2 >> 63) ^ (r2 >> 62) ^ (r2 >> 57);
-    uint64_t b0 = r3 ^ (r3 << 1) ^ (r3 << 2) ^ (r3 << 7);
-    uint64_t b1 = (r3 >> 63) ^ (r3 >> 62) ^ (r3 >> 57);
-    uint64_t w0 = x.r[0] ^ a0 ^ b1 ^ (b1 << 1) ^ (b1 << 2) ^ (b1 << 7);
-    uint64_t w1 = x.r[1] ^ a1 ^ b0;
+    uint64_t a0 = xor3(r2, r2 << 1, r2 << 2) ^ (r2 << 7);
+    uint64_t a1 = xor3(r2 >> 63, r2 >> 62, r2 >> 57);
+    uint64_t b0 = xor3(r3, r3 << 1, r3 << 2) ^ (r3 << 7);
+    uint64_t b1 = xor3(r3 >> 63, r3 >> 62, r3 >> 57);
+    uint64_t w0 = xor3(x.r[0], a0, b1) ^ xor3(b1 << 1, b1 << 2, b1 << 7);
+    uint64_t w1 = xor3(x.r[1], a1, b0);
     return gf128{{(uint32_t)w0, (uint32_t)(w0 >> 32), (uint32_t)w1, (uint32_t)(w1 >> 32)}};
 }
 
@@ -163,18 +181,21 @@ __device__ __forceinline__ void gf_table_build(uint4* tab, gf128 c, int tid, int
     }
 }
 
+// Two windows per step, so each accumulator word takes one 3-input XOR
+// (v_bitop3_b32) per pair of table entries.
 __device__ __forceinline__ gf128 gf_mul_tab(const uint4* tab, gf128 v) {
     uint32_t x0 = 0, x1 = 0, x2 = 0, x3 = 0;
 #pragma unroll
     for (int word = 0; word < 4; word++) {
         uint32_t w = v.w[word];
 #pragma unroll
-        for (int nib = 0; nib < 8; nib++) {
+        for (int nib = 0; nib < 8; nib += 2) {
             uint4 t = tab[(word * 8 + nib) * 16 + ((w >> (4 * nib)) & 15u)];
-            x0 ^= t.x;
-            x1 ^= t.y;
-            x2 ^= t.z;
-            x3 ^= t.w;
+            uint4 u = tab[(word * 8 + nib + 1) * 16 + ((w >> (4 * nib + 4)) & 15u)];
+            x0 = __builtin_amdgcn_bitop3_b32(x0, t.x, u.x, 0x96);
+            x1 = __builtin_amdgcn_bitop3_b32(x1, t.y, u.y, 0x96);
+            x2 = __builtin_amdgcn_bitop3_b32(x2, t.z, u.z, 0x96);
+            x3 = __builtin_amdgcn_bitop3_b32(x3, t.w, u.w, 0x96);
         }
     }
     return gf128{{x0, x1, x2, x3}};

@@ -65,9 +65,14 @@ __device__ __forceinline__ uint64_t clmul32(uint32_t a, uint32_t b) {
     const uint32_t m = 0x11111111u;
     uint32_t a0 = a & m, a1 = a & (m << 1), a2 = a & (m << 2), a3 = a & (m << 3);
     uint32_t b0 = b & m, b1 = b & (m << 1), b2 = b & (m << 2), b3 = b & (m << 3);
-    uint64_t z0 = xor3((uint64_t)a0 * b0, (uint64_t)a1 * b3, (uint64_t)a2 * b2) ^ ((uint64_t)a3 * b1);
-    uint64_t z1 = xor3((uint64_t)a0 * b1, (uint64_t)a1 * b0, (uint64_t)a2 * b3) ^ ((uint64_t)a3 * b2);
-    uint64_t z2 = xor3((uint64_t)a0 * b2, (uint64_t)a1 * b1, (uint64_t)a2 * b0) ^ ((uint64_t)a3 * b3);
+    // Class product (i, j) counts, at output slot 4k + ((i+j) mod 4), the
+    // pairs u + v = k (i + j < 4: at most min(k+1, 15-k)) or u + v = k - 1
+    // (i + j >= 4: at most min(k, 16-k)); one of each kind adds to at most 15
+    // per 4-bit slot, so such pairs share one multiply-add (v_mad_u64_u32's
+    // addend) instead of an XOR.
+    uint64_t z0 = xor3((uint64_t)a0 * b0 + (uint64_t)a1 * b3, (uint64_t)a2 * b2, (uint64_t)a3 * b1);
+    uint64_t z1 = ((uint64_t)a0 * b1 + (uint64_t)a2 * b3) ^ ((uint64_t)a1 * b0 + (uint64_t)a3 * b2);
+    uint64_t z2 = xor3((uint64_t)a0 * b2 + (uint64_t)a3 * b3, (uint64_t)a1 * b1, (uint64_t)a2 * b0);
     uint64_t z3 = xor3((uint64_t)a0 * b3, (uint64_t)a1 * b2, (uint64_t)a2 * b1) ^ ((uint64_t)a3 * b0);
     // the classes' bit positions are disjoint and cover the word: select
     const uint32_t M = 0x11111111u;

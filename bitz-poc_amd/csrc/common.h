@@ -134,6 +134,13 @@ namespace bitz {
 struct Scratch {
     DevBuf partials, result, eq_lo, fold_partial, comb, what, misc, leaf_tab;
     void* host = nullptr;
+    // Round results the host waits for: pinned, device-mapped, coherent host
+    // memory ([0]: sequence flag, values from byte 64) written by the final
+    // reduction kernel itself, so the host polls instead of copying + syncing.
+    void* mbox = nullptr;
+    void* mbox_dev = nullptr;
+    uint32_t seq = 0;
+    bool mbox_pending = false;
     uint8_t comb_base[16] = {0};
     bool comb_ready = false;
     int ensure(size_t partial_bytes);

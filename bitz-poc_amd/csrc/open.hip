@@ -9,6 +9,9 @@
 //    word w bit j = row 64w+j (the packed message is these words as Gf128).
 //  * committed bits, row-major ("packed columns"): packed[g*2^t + b] = bit b
 //    of columns 64g..64g+63 (src/ligerito.rs:655-690 pack_columns_from_rows).
+#include <atomic>
+#include <thread>
+
 #include "common.h"
 #include "gf128_dev.h"
 #include "blake3_dev.h"
@@ -1086,6 +1089,15 @@ __global__ __launch_bounds__(kFsThreads) void k_transcript_challenges(bitz::B3St
 int Scratch::ensure(size_t partial_bytes) {
     if (!partials.alloc(partial_bytes)) return BITZ_ERR_ALLOC;
     if (!result.alloc(64 * 16)) return BITZ_ERR_ALLOC;
+    if (!mbox) {
+        if (hipHostMalloc(&mbox, 4096, hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess ||
+            hipHostGetDevicePointer(&mbox_dev, mbox, 0) != hipSuccess) {
+            if (mbox) (void)hipHostFree(mbox);
+            mbox = mbox_dev = nullptr;
+            return BITZ_ERR_ALLOC;
+        }
+        memset(mbox, 0, 4096);
+    }
     if (!host) {
         if (hipHostMalloc(&host, 64 * 16, hipHostMallocDefault) != hipSuccess) {
             host = nullptr;
@@ -1096,6 +1108,7 @@ int Scratch::ensure(size_t partial_bytes) {
 }
 Scratch::~Scratch() {
     if (host) (void)hipHostFree(host);
+    if (mbox) (void)hipHostFree(mbox);
 }
 
 static void split16(const uint8_t* p, uint64_t* lo, uint64_t* hi) {
@@ -1114,6 +1127,22 @@ static int sums_finish(bitz_ctx* ctx, unsigned nblocks, int K, void* out_dev) {
     }
     Scratch& sc = ctx->scratch;
     uint4* parts = (uint4*)sc.partials.ptr;
+    if (!out_dev && sc.mbox_dev) {
+        // the host reads these sums next: deliver them to the mailbox
+        uint32_t seq = ++sc.seq;
+        uint32_t* flag = (uint32_t*)sc.mbox_dev;
+        uint4* vals = (uint4*)((uint8_t*)sc.mbox_dev + 64);
+        uint4* dst = (uint4*)sc.result.ptr;
+        switch (K) {
+            case 1: hipLaunchKernelGGL(k_xor_partials_mbox<1>, dim3(1), dim3(kT), 0, ctx->stream, parts, nblocks, dst, vals, flag, seq); break;
+            case 2: hipLaunchKernelGGL(k_xor_partials_mbox<2>, dim3(1), dim3(kT), 0, ctx->stream, parts, nblocks, dst, vals, flag, seq); break;
+            case 3: hipLaunchKernelGGL(k_xor_partials_mbox<3>, dim3(1), dim3(kT), 0, ctx->stream, parts, nblocks, dst, vals, flag, seq); break;
+            default: return BITZ_ERR_ARG;
+        }
+        BITZ_LAUNCH_CHECK();
+        sc.mbox_pending = true;
+        return BITZ_OK;
+    }
     uint4* dst = out_dev ? (uint4*)out_dev : (uint4*)sc.result.ptr;
     switch (K) {
         case 1: hipLaunchKernelGGL(k_xor_partials<1>, dim3(1), dim3(kT), 0, ctx->stream, parts, nblocks, dst); break;
@@ -1127,6 +1156,31 @@ static int sums_finish(bitz_ctx* ctx, unsigned nblocks, int K, void* out_dev) {
 
 int read_result(bitz_ctx* ctx, int K, uint8_t* host_out) {
     Scratch& sc = ctx->scratch;
+    if (sc.mbox_pending) {
+        // spin on the flag; every 1024 polls ask whether the stream has
+        // drained, so a launch that never delivers ends in an error, not a hang
+        volatile uint32_t* flag = (volatile uint32_t*)sc.mbox;
+        for (uint64_t spin = 1;; spin++) {
+            if (*flag == sc.seq) break;
+            if (spin > (1u << 16)) std::this_thread::yield();  // a long kernel: share the core
+            if ((spin & 1023) == 0) {
+                hipError_t q = hipStreamQuery(ctx->stream);
+                if (q == hipSuccess) {
+                    if (*flag == sc.seq) break;
+                    set_error("read_result: the reduction finished without delivering its sums");
+                    return BITZ_ERR_DEVICE;
+                }
+                if (q != hipErrorNotReady) {
+                    set_error(std::string("read_result: ") + hipGetErrorString(q));
+                    return BITZ_ERR_DEVICE;
+                }
+            }
+        }
+        std::atomic_thread_fence(std::memory_order_acquire);
+        memcpy(host_out, (const uint8_t*)sc.mbox + 64, K * 16);
+        sc.mbox_pending = false;
+        return BITZ_OK;
+    }
     BITZ_HIP(hipMemcpyAsync(sc.host, sc.result.ptr, K * 16, hipMemcpyDeviceToHost, ctx->stream));
     BITZ_HIP(hipStreamSynchronize(ctx->stream));
     memcpy(host_out, sc.host, K * 16);

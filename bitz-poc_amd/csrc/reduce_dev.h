@@ -57,4 +57,29 @@ __global__ void k_xor_partials(const uint4* __restrict__ partials, uint32_t nblo
     block_xor_store<K>(acc, out);
 }
 
+// The same reduction, also written to a host-mapped mailbox: the K values,
+// then a system-scope fence, then the sequence flag the host polls on.
+template <int K>
+__global__ void k_xor_partials_mbox(const uint4* __restrict__ partials, uint32_t nblocks, uint4* __restrict__ out,
+                                    uint4* mbox_vals, uint32_t* mbox_flag, uint32_t seq) {
+    __shared__ uint4 res[K];
+    gf128 acc[K];
+#pragma unroll
+    for (int k = 0; k < K; k++) acc[k] = gf_zero();
+    for (uint32_t b = threadIdx.x; b < nblocks; b += blockDim.x) {
+#pragma unroll
+        for (int k = 0; k < K; k++) acc[k] = gf_add(acc[k], gf_load(&partials[(size_t)b * K + k]));
+    }
+    block_xor_store<K>(acc, res);
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            out[k] = res[k];
+            mbox_vals[k] = res[k];
+        }
+        __threadfence_system();
+        *(volatile uint32_t*)mbox_flag = seq;
+    }
+}
+
 }  // namespace bitz

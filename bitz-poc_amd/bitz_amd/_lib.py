@@ -45,6 +45,7 @@ SIGNATURES = {
     "bitz_last_error": (C.c_char_p, []),
     "bitz_version": (C.c_char_p, []),
     "bitz_ctx_set_timing": (_I, [_P, _I]),
+    "bitz_ctx_set_timing_only": (_I, [_P, C.c_char_p]),
     "bitz_ctx_kernel_stats": (_I, [_P, _P, _U64, _P, _U64]),
     "bitz_gf_mul_dev": (_I, [_P, _P, _P, _P, _U64]),
     "bitz_gf_mul_const_dev": (_I, [_P, _P, _P, _P, _U64]),

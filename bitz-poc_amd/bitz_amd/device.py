@@ -164,6 +164,11 @@ def set_timing(ctx: Context, on: bool) -> None:
     _lib.call("bitz_ctx_set_timing", ctx.ptr, 1 if on else 0)
 
 
+def set_timing_only(ctx: Context, kernel: str) -> None:
+    """Time only the launches of `kernel` (fewer events in a timed region)."""
+    _lib.call("bitz_ctx_set_timing_only", ctx.ptr, kernel.encode())
+
+
 class OpenOps:
     """Thin wrappers over the opening kernels (device tensors in, device
     tensors or host ints out)."""

@@ -141,6 +141,10 @@ struct Scratch {
     void* mbox_dev = nullptr;
     uint32_t seq = 0;
     bool mbox_pending = false;
+    // pinned host staging for the host-side last rounds of a sumcheck
+    void* stage = nullptr;
+    size_t stage_bytes = 0;
+    int ensure_stage(size_t bytes);
     uint8_t comb_base[16] = {0};
     bool comb_ready = false;
     int ensure(size_t partial_bytes);
@@ -168,6 +172,7 @@ struct bitz_ctx {
     int device = 0;
     bitz::Scratch scratch;
     bool timing = false;
+    std::string timing_only;  // non-empty: time only launches of this name
     std::vector<bitz::PendingTiming> pending;
     std::vector<hipEvent_t> event_pool;
     std::map<std::string, bitz::KernelStat> stats;
@@ -201,12 +206,13 @@ struct bitz_commit_handle {
 namespace bitz {
 // Integer work per primitive, for the op counts attached to timed launches:
 // the VALU instructions of each primitive in this gfx950 build (one
-// instruction = 64 lane operations of the wave): a variable x variable
-// GF(2^128) product (gf_mul: 145 v_mad_u64_u32 among ~660), a product by a
-// launch constant through its 4-bit window table (32 LDS lookups, ~225
-// VALU), a BLAKE3 compression (7 rounds x 8 G x 14 + 16 = 800).
-constexpr double kGfMulOps = 660.0;
-constexpr double kGfTabMulOps = 225.0;
+// instruction = 64 lane operations of the wave), counted in the assembly of
+// the parity kernels k_gf_mul / k_gf_mul_const: a variable x variable
+// GF(2^128) product (~430 with its loads and stores), a product by a launch
+// constant through its 4-bit window table (32 LDS lookups, ~125 VALU), a
+// BLAKE3 compression (7 rounds x 8 G x 14 + 16 = 800).
+constexpr double kGfMulOps = 430.0;
+constexpr double kGfTabMulOps = 125.0;
 constexpr double kBlake3Ops = 800.0;
 
 // RAII bracket: records start/stop events on ctx->stream when timing is on.

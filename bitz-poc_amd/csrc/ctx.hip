@@ -26,7 +26,8 @@ static hipEvent_t take_event(bitz_ctx* ctx) {
     return e;
 }
 
-TimedLaunch::TimedLaunch(bitz_ctx* c, const char* name, double bytes, double ops) : ctx(c), on(c->timing) {
+TimedLaunch::TimedLaunch(bitz_ctx* c, const char* name, double bytes, double ops)
+    : ctx(c), on(c->timing && (c->timing_only.empty() || c->timing_only == name)) {
     if (!on) return;
     p.name = name;
     p.bytes = bytes;
@@ -43,7 +44,7 @@ TimedLaunch::~TimedLaunch() {
 }
 
 void PhaseClock::close() {
-    if (!cur.empty() && ctx->timing) {
+    if (!cur.empty() && ctx->timing && ctx->timing_only.empty()) {
         double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
         auto& st = ctx->stats["phase:" + cur];
         st.launches++;
@@ -73,7 +74,17 @@ extern "C" int bitz_ctx_set_timing(bitz_ctx* ctx, int enable) {
     BITZ_REQUIRE(ctx != nullptr, BITZ_ERR_ARG, "bitz_ctx_set_timing: null ctx");
     bitz::resolve_timings(ctx);
     ctx->timing = enable != 0;
+    ctx->timing_only.clear();
     if (enable) ctx->stats.clear();
+    return BITZ_OK;
+}
+
+extern "C" int bitz_ctx_set_timing_only(bitz_ctx* ctx, const char* kernel) {
+    BITZ_REQUIRE(ctx && kernel && kernel[0], BITZ_ERR_ARG, "bitz_ctx_set_timing_only: need ctx and a kernel name");
+    bitz::resolve_timings(ctx);
+    ctx->timing = true;
+    ctx->timing_only = kernel;
+    ctx->stats.clear();
     return BITZ_OK;
 }
 

@@ -1,0 +1,41 @@
+// Host GF(2^128) loops for the last rounds of a sumcheck, where the two
+// halves hold at most a few thousand elements: at that size one round on the
+// host (a few microseconds with carry-less multiply instructions) is cheaper
+// than a kernel launch and a device-to-host round trip per round.
+//
+// Elements are {lo, hi} 64-bit word pairs (the 16-byte Gf128 encoding of
+// src/bitz/codec.rs:4-21), modulus x^128 + x^7 + x^2 + x + 1. Plain C++ (no
+// device code); PCLMULQDQ is used when the CPU has it, an integer carry-less
+// multiply otherwise, with identical results.
+#pragma once
+#include <stdint.h>
+
+namespace bitz {
+namespace hostgf {
+
+struct E {
+    uint64_t lo, hi;
+};
+
+// a * b
+E mul(E a, E b);
+
+// eq table over n coordinates: out[x] = prod_i (bit i of x ? r[i] : 1 + r[i])
+void eq_table(const E* r, int n, E* out);
+
+// One round of the GKR layer sumcheck over h slots of l = [el | eh] and
+// r = [ol | oh] (2h entries each), slot weights w[x]:
+//   out[0] = sum_x w[x] * (send_one ? eh * oh : el * ol)
+//   out[1] = sum_x w[x] * (eh + el) * (oh + ol)
+// (Sums::slot, src/bitz/kernels.rs:299-310, summed over the round's slots).
+void gkr_round_sums(const E* l, const E* r, uint64_t h, const E* w, int send_one, E out[2]);
+
+// v[j] = v[j] + rho * (v[j] + v[j + h]) for j < h (fold,
+// src/bitz/kernels.rs:326-328).
+void fold(E* v, uint64_t h, E rho);
+
+// true when the carry-less multiply instructions are in use
+bool using_clmul();
+
+}  // namespace hostgf
+}  // namespace bitz

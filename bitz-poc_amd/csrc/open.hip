@@ -1109,6 +1109,21 @@ int Scratch::ensure(size_t partial_bytes) {
 Scratch::~Scratch() {
     if (host) (void)hipHostFree(host);
     if (mbox) (void)hipHostFree(mbox);
+    if (stage) (void)hipHostFree(stage);
+}
+
+int Scratch::ensure_stage(size_t bytes) {
+    if (bytes <= stage_bytes) return BITZ_OK;
+    if (stage) (void)hipHostFree(stage);
+    stage = nullptr;
+    stage_bytes = 0;
+    if (hipHostMalloc(&stage, bytes, hipHostMallocDefault) != hipSuccess) {
+        stage = nullptr;
+        set_error("host staging: hipHostMalloc failed");
+        return BITZ_ERR_ALLOC;
+    }
+    stage_bytes = bytes;
+    return BITZ_OK;
 }
 
 static void split16(const uint8_t* p, uint64_t* lo, uint64_t* hi) {

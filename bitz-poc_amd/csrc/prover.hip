@@ -14,6 +14,7 @@
 #include <set>
 #include <thread>
 
+#include "host_gf.h"
 #include "host_proto.h"
 #include "open_impl.h"
 
@@ -37,6 +38,12 @@ inline void gput(G g, uint8_t* p) {
 }
 // eq_factor (src/bitz/mod.rs:328-331): r z + (1 - r)(1 - z)
 inline G eq_factor(G r, G z) { return hgf_add(hgf_mul(r, z), hgf_mul(hgf_add(r, gone()), hgf_add(z, gone()))); }
+
+// A GKR layer's rounds with at most 2^kHostTailLog slots run on the host
+// (host_tail): one such round is a few microseconds of carry-less multiplies
+// there, against a kernel launch, a reduction and a device-to-host round trip
+// (tens of microseconds) on the device.
+constexpr int kHostTailLog = 10;
 
 std::vector<G> host_eq(const std::vector<G>& r) {
     std::vector<G> t(1ull << r.size());
@@ -419,12 +426,24 @@ struct Prover {
         // eq_c multiplies as well, the fused kernel loses to the two passes)
         bool pending_fold = false;
         uint8_t pending_rho[16], rho0[16];
+        uint8_t last[32];
+        bool tail_done = false;
         BITZ_REQUIRE(kind == kDense || (L >= 3 && t >= 3), BITZ_ERR_SHAPE, "GKR: layer too small for bit patterns");
         for (int round = 0; round < L; round++) {
             G z = internal[round];
             int rem = L - 1 - round;
             uint64_t hh = 1ull << rem;
             int s_c = std::min(s, rem);
+            const bool bit_round = (leaf && round <= 1) || (kind == kLevel1 && round == 0);
+            if (!bit_round && rem <= kHostTailLog && (!weighted || rem < s)) {
+                // the remaining rounds on the host, from dense halves
+                if (pending_fold && (rc = fold_pair(ctx, l, r, 2 * hh, pending_rho))) return rc;
+                if (weighted && (rc = mul_inplace(ctx, r, eqcinv.ptr, 1ull << s))) return rc;
+                if ((rc = host_tail(it, l, r, round, ext, internal, &factor, &next, last))) return rc;
+                pending_fold = false;
+                tail_done = true;
+                break;
+            }
             if (!per_layer) {
                 std::vector<G> c(ext.begin(), ext.begin() + s_c);
                 std::vector<G> yv(ext.begin() + s_c, ext.begin() + rem);
@@ -494,9 +513,10 @@ struct Prover {
             factor = hgf_mul(factor, eq_factor(rr, z));
         }
         if (pending_fold && (rc = fold_pair(ctx, l, r, 1, pending_rho))) return rc;
-        uint8_t last[32];
-        if ((rc = download(l, last, 16))) return rc;
-        if ((rc = download(r, last + 16, 16))) return rc;
+        if (!tail_done) {
+            if ((rc = download(l, last, 16))) return rc;
+            if ((rc = download(r, last + 16, 16))) return rc;
+        }
         it.prover_message(last, 32);
         uint8_t rb[16];
         it.verifier_gf(rb);
@@ -506,6 +526,51 @@ struct Prover {
         internal.clear();
         internal.push_back(rc_);
         internal.insert(internal.end(), next.begin(), next.end());
+        return BITZ_OK;
+    }
+
+    // Rounds round0..L-1 of a GKR layer on the host once each half holds at
+    // most 2^(kHostTailLog+1) entries: the halves come down once, then every
+    // round is the same sums / message / challenge / fold sequence as the
+    // device rounds (gkr_layer), with the round's own eq table over
+    // ext[0..rem) as slot weights (so no corr[] constant; Sums::slot,
+    // src/bitz/kernels.rs:299-310). `last` receives the folded l[0], r[0].
+    int host_tail(InnerTranscript& it, const uint4* l, const uint4* r, int round0, const std::vector<G>& ext,
+                  const std::vector<G>& internal, G* factor, std::vector<G>* next, uint8_t last[32]) {
+        using hostgf::E;
+        const int L = (int)internal.size();
+        const int rem0 = L - 1 - round0;
+        const uint64_t n = 2ull << rem0;  // entries of each half at round0
+        int rc = ctx->scratch.ensure_stage(2 * n * sizeof(E));
+        if (rc) return rc;
+        E* hl = (E*)ctx->scratch.stage;
+        E* hr = hl + n;
+        BITZ_HIP(hipMemcpyAsync(hl, l, n * sizeof(E), hipMemcpyDeviceToHost, ctx->stream));
+        BITZ_HIP(hipMemcpyAsync(hr, r, n * sizeof(E), hipMemcpyDeviceToHost, ctx->stream));
+        BITZ_HIP(hipStreamSynchronize(ctx->stream));
+        std::vector<E> coords(rem0 > 0 ? rem0 : 1), w(1ull << rem0);
+        for (int i = 0; i < rem0; i++) coords[i] = E{ext[i].lo, ext[i].hi};
+        for (int round = round0; round < L; round++) {
+            const int rem = L - 1 - round;
+            const uint64_t hh = 1ull << rem;
+            const G z = internal[round];
+            hostgf::eq_table(coords.data(), rem, w.data());
+            E sums[2];
+            hostgf::gkr_round_sums(hl, hr, hh, w.data(), geq(z, gzero()) ? 1 : 0, sums);
+            uint8_t msg[32];
+            gput(hgf_mul(*factor, G{sums[0].lo, sums[0].hi}), msg);
+            gput(hgf_mul(*factor, G{sums[1].lo, sums[1].hi}), msg + 16);
+            it.prover_message(msg, 32);
+            uint8_t rb[16];
+            it.verifier_gf(rb);
+            const G rr = gfrom(rb);
+            next->push_back(rr);
+            hostgf::fold(hl, hh, E{rr.lo, rr.hi});
+            hostgf::fold(hr, hh, E{rr.lo, rr.hi});
+            *factor = hgf_mul(*factor, eq_factor(rr, z));
+        }
+        memcpy(last, hl, 16);
+        memcpy(last + 16, hr, 16);
         return BITZ_OK;
     }
 

@@ -33,9 +33,12 @@ extern "C" int bitz_eq_table_dev(bitz_ctx* ctx, const uint8_t* r_host, int n, vo
     PRELUDE();
     BITZ_REQUIRE(out && (n == 0 || r_host), BITZ_ERR_ARG, "bitz_eq_table_dev: null argument");
     BITZ_REQUIRE(n >= 0 && n <= 28, BITZ_ERR_SHAPE, "bitz_eq_table_dev: n must be in [0, 28]");
-    void* r;
-    CALL(stage(ctx, r_host, (size_t)n * 16, &r));
-    return bitz::eq_table(ctx, r, n, out);
+    std::vector<bitz::hgf> r(n > 0 ? n : 1);
+    for (int i = 0; i < n; i++) {
+        memcpy(&r[i].lo, r_host + 16 * i, 8);
+        memcpy(&r[i].hi, r_host + 16 * i + 8, 8);
+    }
+    return bitz::eq_args(ctx, r.data(), n, out);
 }
 
 extern "C" int bitz_fold_columns_dev(bitz_ctx* ctx, const void* rows, int t, int s, const void* w, void* out) {

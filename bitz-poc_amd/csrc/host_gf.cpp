@@ -121,6 +121,28 @@ __attribute__((target("pclmul,sse4.1"))) E mul_call(E a, E b) {
     return mul_t<kClmul>(a, b);
 }
 
+template <bool kClmul>
+__attribute__((target("pclmul,sse4.1"))) void inner_round_t(const E* m, const E* w, uint64_t pairs, E out[3]) {
+    W a0{{0, 0, 0, 0}}, a1{{0, 0, 0, 0}}, q{{0, 0, 0, 0}};
+    for (uint64_t k = 0; k < pairs; k++) {
+        const E m0 = m[2 * k], m1 = m[2 * k + 1], w0 = w[2 * k], w1 = w[2 * k + 1];
+        acc(a0, wide<kClmul>(m0, w0));
+        acc(a1, wide<kClmul>(m1, w1));
+        acc(q, wide<kClmul>(add(m0, m1), add(w0, w1)));
+    }
+    out[0] = reduce(a0);
+    out[1] = reduce(a1);
+    out[2] = reduce(q);
+}
+
+template <bool kClmul>
+__attribute__((target("pclmul,sse4.1"))) void fold_lsb_t(E* v, uint64_t half, E r) {
+    for (uint64_t i = 0; i < half; i++) {
+        const E v0 = v[2 * i], v1 = v[2 * i + 1];
+        v[i] = add(v0, mul_t<kClmul>(r, add(v0, v1)));
+    }
+}
+
 bool detect() {
     __builtin_cpu_init();
     return __builtin_cpu_supports("pclmul") && __builtin_cpu_supports("sse4.1");
@@ -147,6 +169,24 @@ void gkr_round_sums(const E* l, const E* r, uint64_t h, const E* w, int send_one
         round_sums_t<true>(l, r, h, w, send_one, out);
     else
         round_sums_t<false>(l, r, h, w, send_one, out);
+}
+
+void drop_top(E* v, uint64_t h) {
+    for (uint64_t j = 0; j < h; j++) v[j] = add(v[j], v[j + h]);
+}
+
+void inner_round(const E* m, const E* w, uint64_t pairs, E out[3]) {
+    if (using_clmul())
+        inner_round_t<true>(m, w, pairs, out);
+    else
+        inner_round_t<false>(m, w, pairs, out);
+}
+
+void fold_lsb(E* v, uint64_t half, E r) {
+    if (using_clmul())
+        fold_lsb_t<true>(v, half, r);
+    else
+        fold_lsb_t<false>(v, half, r);
 }
 
 void fold(E* v, uint64_t h, E rho) {

@@ -34,6 +34,19 @@ void gkr_round_sums(const E* l, const E* r, uint64_t h, const E* w, int send_one
 // src/bitz/kernels.rs:326-328).
 void fold(E* v, uint64_t h, E rho);
 
+// v[j] = v[j] + v[j + h] for j < h: an eq table over n coordinates becomes
+// the table over the first n - 1 (the last coordinate's two weights sum to 1).
+void drop_top(E* v, uint64_t h);
+
+// One round of the inner-product sumcheck over pairs (m[2k], m[2k+1]),
+// (w[2k], w[2k+1]): out = (sum m0 w0, sum m1 w1, sum (m0 + m1)(w0 + w1))
+// (src/bitz/sumcheck.rs:146-198).
+void inner_round(const E* m, const E* w, uint64_t pairs, E out[3]);
+
+// v[i] = v[2i] + r (v[2i] + v[2i+1]) for i < half, in place (the LSB-first
+// fold of the inner-product sumcheck, src/bitz/sumcheck.rs:372-400).
+void fold_lsb(E* v, uint64_t half, E r);
+
 // true when the carry-less multiply instructions are in use
 bool using_clmul();
 

@@ -116,8 +116,51 @@ void Blake3::push_cv(uint32_t cv[8], uint64_t total_chunks) {
     memcpy(stack_[stack_len_++], cur, 32);
 }
 
+void blake3_parent_cv(const uint32_t left[8], const uint32_t right[8], uint32_t cv[8]) {
+    uint8_t block[64];
+    words_le(left, 8, block);
+    words_le(right, 8, block + 32);
+    uint32_t out[16];
+    blake3_compress_host(kIV, block, 0, 64, kParent, out);
+    memcpy(cv, out, 32);
+}
+
+std::vector<int> blake3_subtree_runs(uint64_t c0, uint64_t m) {
+    std::vector<int> ks;
+    uint64_t c = c0, end = c0 + m;
+    while (c < end) {
+        int k = 0;
+        while (k < 62 && ((c >> k) & 1) == 0 && c + (2ull << k) <= end) k++;
+        ks.push_back(k);
+        c += 1ull << k;
+    }
+    return ks;
+}
+
+void Blake3::absorb_subtree(const uint32_t cv[8], int k) {
+    if (chunk_len() == 1024) {  // finish the buffered chunk: input follows
+        uint32_t out[16];
+        blake3_compress_host(chunk_.cv, chunk_.buf, chunk_.counter, chunk_.buf_len,
+                             kChunkEnd | (chunk_.blocks == 0 ? kChunkStart : 0), out);
+        uint64_t total = chunk_.counter + 1;
+        push_cv(out, total);
+        chunk_reset(total);
+    }
+    // at a chunk boundary, at a multiple of 2^k chunks (checked by the caller's plan)
+    uint32_t cur[8];
+    memcpy(cur, cv, 32);
+    uint64_t t = (chunk_.counter >> k) + 1;
+    while ((t & 1) == 0) {
+        blake3_parent_cv(stack_[stack_len_ - 1], cur, cur);
+        stack_len_--;
+        t >>= 1;
+    }
+    memcpy(stack_[stack_len_++], cur, 32);
+    chunk_reset(chunk_.counter + (1ull << k));
+}
+
 // CV of one whole, non-final 1024-byte chunk
-static void chunk_cv(const uint8_t* chunk, uint64_t counter, uint32_t cv[8]) {
+void blake3_chunk_cv(const uint8_t* chunk, uint64_t counter, uint32_t cv[8]) {
     memcpy(cv, kIV, 32);
     for (int b = 0; b < 16; b++) {
         uint32_t out[16];
@@ -157,7 +200,7 @@ void Blake3::update(const void* data, size_t len) {
             std::vector<std::thread> pool;
             for (unsigned w = 0; w < nt; w++)
                 pool.emplace_back([&, w]() {
-                    for (size_t i = w; i < m; i += nt) chunk_cv(p + 1024 * i, c0 + i, &cvs[8 * i]);
+                    for (size_t i = w; i < m; i += nt) blake3_chunk_cv(p + 1024 * i, c0 + i, &cvs[8 * i]);
                 });
             for (auto& th : pool) th.join();
             for (size_t i = 0; i < m; i++) push_cv(&cvs[8 * i], c0 + i + 1);
@@ -244,6 +287,30 @@ void Transcript::absorb_slice(const void* data, size_t len) {
     memcpy(head + 1, &l, 8);
     h_.update(head, 9);
     h_.update(data, len);
+    uint8_t tail = 0x07;
+    h_.update(&tail, 1);
+}
+
+Transcript::SlicePlan Transcript::slice_plan(size_t len) const {
+    const uint64_t start = h_.position() + 9;  // after the 0x06 || len frame
+    const uint64_t lead = std::min<uint64_t>((1024 - start % 1024) % 1024, len);
+    const uint64_t chunks = (len - lead) / 1024;
+    return SlicePlan{lead, chunks, (start + lead) / 1024};
+}
+
+void Transcript::absorb_slice_runs(size_t len, const SlicePlan& plan, const uint8_t* lead_bytes,
+                                   const uint32_t* run_cvs, const uint8_t* tail_bytes) {
+    uint8_t head[9];
+    head[0] = 0x06;
+    uint64_t l = len;
+    memcpy(head + 1, &l, 8);
+    h_.update(head, 9);
+    h_.update(lead_bytes, plan.lead);
+    if (plan.chunks) {
+        std::vector<int> ks = blake3_subtree_runs(plan.c0, plan.chunks);
+        for (size_t i = 0; i < ks.size(); i++) h_.absorb_subtree(run_cvs + 8 * i, ks[i]);
+    }
+    h_.update(tail_bytes, len - plan.lead - 1024 * plan.chunks);
     uint8_t tail = 0x07;
     h_.update(&tail, 1);
 }

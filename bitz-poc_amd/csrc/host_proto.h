@@ -29,6 +29,12 @@ class Blake3 {
    public:
     Blake3();
     void update(const void* data, size_t len);
+    // Bytes absorbed so far.
+    uint64_t position() const { return chunk_.counter * 1024 + chunk_len(); }
+    // Absorb 2^k whole chunks, given the chaining value of their subtree,
+    // at a position that is a multiple of 2^k chunks; more input must
+    // follow. Equal to absorbing the chunks' bytes.
+    void absorb_subtree(const uint32_t cv[8], int k);
     // XOF output of everything absorbed so far; the state is unchanged.
     void finalize_xof(uint8_t* out, size_t len) const;
     void export_state(B3State* s) const;
@@ -51,6 +57,12 @@ class Blake3 {
 
 void blake3_compress_host(const uint32_t cv[8], const uint8_t block[64], uint64_t counter, uint32_t block_len,
                           uint32_t flags, uint32_t out[16]);
+// Chaining values: a whole non-final chunk, a non-root parent node.
+void blake3_chunk_cv(const uint8_t chunk[1024], uint64_t counter, uint32_t cv[8]);
+void blake3_parent_cv(const uint32_t left[8], const uint32_t right[8], uint32_t cv[8]);
+// The aligned power-of-two runs [c, c + 2^k) that tile chunks [c0, c0 + m),
+// in order (each run starts at a multiple of its length): their k.
+std::vector<int> blake3_subtree_runs(uint64_t c0, uint64_t m);
 void blake3_hash_host(const void* data, size_t len, uint8_t out[32]);
 
 // ---- Blake3Transcript (src/transcript/mod.rs:18-96, traits.rs:371-376) ----
@@ -60,6 +72,18 @@ class Transcript {
     void absorb_slice(const void* data, size_t len);
     void absorb_slice(const std::string& s) { absorb_slice(s.data(), s.size()); }
     void absorb_u64(uint64_t v) { absorb_slice(&v, 8); }
+    // absorb_slice of len bytes whose whole chunks are hashed elsewhere:
+    // slice_plan gives the bytes before the first chunk boundary inside the
+    // message (lead), the whole chunks after them and the first one's index;
+    // absorb_slice_runs then takes the lead bytes, the subtree chaining
+    // values of the runs blake3_subtree_runs(c0, chunks) in order, and the
+    // remaining len - lead - 1024 chunks bytes.
+    struct SlicePlan {
+        uint64_t lead, chunks, c0;
+    };
+    SlicePlan slice_plan(size_t len) const;
+    void absorb_slice_runs(size_t len, const SlicePlan& plan, const uint8_t* lead_bytes, const uint32_t* run_cvs,
+                           const uint8_t* tail_bytes);
     // get_challenge of 16 bytes: XOF bytes, then absorb 0x12 || bytes || 0x34
     void challenge(uint8_t* out, size_t len);
     // fill_sampling_bytes: XOF bytes, then absorb the sampling-read frame
@@ -82,6 +106,11 @@ class InnerTranscript {
     InnerTranscript(const std::string& session, const uint8_t* instance, size_t instance_len);
     void public_message(const void* data, size_t len) { t_.absorb_slice(data, len); }
     void public_message(const std::string& s) { t_.absorb_slice(s); }
+    Transcript::SlicePlan public_plan(size_t len) const { return t_.slice_plan(len); }
+    void public_message_runs(size_t len, const Transcript::SlicePlan& plan, const uint8_t* lead_bytes,
+                             const uint32_t* run_cvs, const uint8_t* tail_bytes) {
+        t_.absorb_slice_runs(len, plan, lead_bytes, run_cvs, tail_bytes);
+    }
     void prover_message(const void* data, size_t len) {
         t_.absorb_slice(data, len);
         const uint8_t* p = (const uint8_t*)data;

@@ -67,20 +67,53 @@ __global__ void k_eq_combine(const uint4* __restrict__ lo, int nlo, const uint4*
     }
 }
 
-// out[x] = prod_i (bit i of x ? b[i] : a[i]) for x < 2^n (n <= 14), one block
-__global__ void k_tensor_small(const uint4* __restrict__ a, const uint4* __restrict__ b, int n,
-                               uint4* __restrict__ out) {
-    if (threadIdx.x == 0) out[0] = make_uint4(1, 0, 0, 0);
+// Coordinate sources of k_tensor: by value (a kernel argument, no upload) or
+// in device memory.
+struct ArgCoords {
+    TensorArgs c;
+    __device__ __forceinline__ uint4 a(int i) const { return c.a[i]; }
+    __device__ __forceinline__ uint4 b(int i) const { return c.b[i]; }
+};
+struct DevCoords {
+    const uint4* pa;
+    const uint4* pb;
+    __device__ __forceinline__ uint4 a(int i) const { return pa[i]; }
+    __device__ __forceinline__ uint4 b(int i) const { return pb[i]; }
+};
+
+// out[x] = prod_i (bit i of x ? b(i) : a(i)) for x < 2^n. Block k writes
+// entries 256k..256k+255: the table of the low 8 coordinates is built in LDS
+// by doubling, the product of the high coordinates selected by the bits of k
+// by a tree, then one multiply per entry.
+template <class C>
+__global__ __launch_bounds__(256) void k_tensor(C c, int n, uint4* __restrict__ out) {
+    __shared__ uint4 lo[256];
+    __shared__ uint4 hif[kMaxTensorN];
+    const uint32_t t = threadIdx.x;
+    const int nlo = n < 8 ? n : 8, nh = n - nlo;
+    if (t == 0) lo[0] = make_uint4(1, 0, 0, 0);
     __syncthreads();
-    for (int i = 0; i < n; i++) {
-        gf128 ai = gf_load(&a[i]), bi = gf_load(&b[i]);
-        uint32_t len = 1u << i;
-        for (uint32_t x = threadIdx.x; x < len; x += blockDim.x) {
-            gf128 p = gf_load(&out[x]);
-            gf_store(&out[x + len], gf_mul(p, bi));
-            gf_store(&out[x], gf_mul(p, ai));
+    for (int i = 0; i < nlo; i++) {
+        const uint32_t len = 1u << i;
+        if (t < len) {
+            gf128 p = gf_load(&lo[t]);
+            const uint4 ai = c.a(i), bi = c.b(i);
+            gf_store(&lo[t + len], gf_mul(p, gf_load(&bi)));
+            gf_store(&lo[t], gf_mul(p, gf_load(&ai)));
         }
         __syncthreads();
+    }
+    if ((int)t < nh) hif[t] = ((blockIdx.x >> t) & 1u) ? c.b(nlo + t) : c.a(nlo + t);
+    __syncthreads();
+    for (int w = 1; w < nh; w <<= 1) {
+        if ((int)(t % (2 * w)) == 0 && (int)(t + w) < nh)
+            gf_store(&hif[t], gf_mul(gf_load(&hif[t]), gf_load(&hif[t + w])));
+        __syncthreads();
+    }
+    if (t < (1u << nlo)) {
+        gf128 v = gf_load(&lo[t]);
+        if (nh > 0) v = gf_mul(v, gf_load(&hif[0]));
+        gf_store(&out[((uint64_t)blockIdx.x << nlo) + t], v);
     }
 }
 
@@ -1008,17 +1041,6 @@ __global__ void k_gather_nodes(const uint4* __restrict__ tree, const uint64_t* _
 }
 
 // ------------------------------------------------ device Fiat-Shamir steps
-__device__ __forceinline__ void gf_bytes(gf128 v, uint8_t* b) {
-    for (int w = 0; w < 4; w++)
-        for (int k = 0; k < 4; k++) b[4 * w + k] = (uint8_t)(v.w[w] >> (8 * k));
-}
-__device__ __forceinline__ gf128 gf_from_bytes(const uint8_t* b) {
-    gf128 v;
-    for (int w = 0; w < 4; w++)
-        v.w[w] = (uint32_t)b[4 * w] | ((uint32_t)b[4 * w + 1] << 8) | ((uint32_t)b[4 * w + 2] << 16) |
-                 ((uint32_t)b[4 * w + 3] << 24);
-    return v;
-}
 
 // One GKR round (gkr.rs:301-310 prover_message / verifier_message) on one
 // wavefront, after the block partials were reduced into sums[0..2):
@@ -1322,10 +1344,31 @@ int gkr_round_sums_rowonly(bitz_ctx* ctx, const void* l, const void* r, uint64_t
     return sums_finish(ctx, nb, 2, out_dev);
 }
 
+int tensor_args(bitz_ctx* ctx, const hgf* a, const hgf* b, int n, void* out) {
+    BITZ_REQUIRE(n >= 0 && n <= kMaxTensorN, BITZ_ERR_SHAPE, "tensor_args: n out of range");
+    TensorArgs c;
+    for (int i = 0; i < n; i++) {
+        c.a[i] = make_uint4((uint32_t)a[i].lo, (uint32_t)(a[i].lo >> 32), (uint32_t)a[i].hi, (uint32_t)(a[i].hi >> 32));
+        c.b[i] = make_uint4((uint32_t)b[i].lo, (uint32_t)(b[i].lo >> 32), (uint32_t)b[i].hi, (uint32_t)(b[i].hi >> 32));
+    }
+    const unsigned blocks = n > 8 ? 1u << (n - 8) : 1u;
+    hipLaunchKernelGGL(k_tensor<ArgCoords>, dim3(blocks), dim3(256), 0, ctx->stream, ArgCoords{c}, n, (uint4*)out);
+    BITZ_LAUNCH_CHECK();
+    return BITZ_OK;
+}
+
+int eq_args(bitz_ctx* ctx, const hgf* r, int n, void* out) {
+    BITZ_REQUIRE(n >= 0 && n <= kMaxTensorN, BITZ_ERR_SHAPE, "eq_args: n out of range");
+    hgf a[kMaxTensorN];
+    for (int i = 0; i < n; i++) a[i] = hgf_add(r[i], hgf{1, 0});
+    return tensor_args(ctx, a, r, n, out);
+}
+
 int tensor_table(bitz_ctx* ctx, const void* a, const void* b, int n, void* out) {
     BITZ_REQUIRE(n >= 0 && n <= 14, BITZ_ERR_SHAPE, "tensor_table: n out of range");
-    hipLaunchKernelGGL(k_tensor_small, dim3(1), dim3(kT), 0, ctx->stream, (const uint4*)a, (const uint4*)b, n,
-                       (uint4*)out);
+    const unsigned blocks = n > 8 ? 1u << (n - 8) : 1u;
+    hipLaunchKernelGGL(k_tensor<DevCoords>, dim3(blocks), dim3(256), 0, ctx->stream,
+                       DevCoords{(const uint4*)a, (const uint4*)b}, n, (uint4*)out);
     BITZ_LAUNCH_CHECK();
     return BITZ_OK;
 }
@@ -1757,6 +1800,90 @@ int transcript_challenges(bitz_ctx* ctx, void* state, const void* msgs, uint32_t
     BITZ_REQUIRE(msg_len <= 1536, BITZ_ERR_ARG, "transcript_challenges: messages of at most 1536 bytes");
     hipLaunchKernelGGL(k_transcript_challenges, dim3(1), dim3(kFsThreads), 0, ctx->stream, (bitz::B3State*)state,
                        (const uint8_t*)msgs, n, msg_len, (uint8_t*)out);
+    BITZ_LAUNCH_CHECK();
+    return BITZ_OK;
+}
+
+// ------------------------------------- transcript hashing of long messages
+// Chaining values of m whole, non-final chunks of a BLAKE3 stream (chunk i
+// has counter c0 + i), from 4-byte aligned data; one thread per chunk.
+__global__ void k_b3_chunk_cvs(const uint32_t* __restrict__ data, uint64_t m, uint64_t c0,
+                               uint32_t* __restrict__ cvs) {
+    const uint64_t i = grid_tid();
+    if (i >= m) return;
+    const uint32_t* chunk = data + 256 * i;
+    uint32_t cv[8];
+    b3::iv_init(cv);
+    for (int b = 0; b < 16; b++) {
+        uint32_t w[16];
+#pragma unroll
+        for (int k = 0; k < 16; k++) w[k] = chunk[16 * b + k];
+        b3::compress(cv, w, c0 + i, 64, (b == 0 ? b3::CHUNK_START : 0u) | (b == 15 ? b3::CHUNK_END : 0u));
+    }
+#pragma unroll
+    for (int k = 0; k < 8; k++) cvs[8 * i + k] = cv[k];
+}
+
+// The runs' subtree chaining values: run r covers 2^ks[r] consecutive chunk
+// CVs (in order from cvs[0]); one block merges every run level by level with
+// PARENT compressions in place, then writes run r's root to roots[8r].
+constexpr int kMaxB3Runs = 128;
+struct B3Runs {
+    uint8_t k[kMaxB3Runs];
+    int n;
+};
+__global__ __launch_bounds__(256) void k_b3_runs(uint32_t* __restrict__ cvs, B3Runs runs,
+                                                 uint32_t* __restrict__ roots) {
+    int kmax = 0;
+    for (int r = 0; r < runs.n; r++) kmax = max(kmax, (int)runs.k[r]);
+    for (int level = 1; level <= kmax; level++) {
+        // merges at this level: node j of run r joins children at
+        // start_r + 2j 2^(level-1) and start_r + (2j+1) 2^(level-1)
+        uint64_t start = 0;
+        for (int r = 0; r < runs.n; r++) {
+            const int k = runs.k[r];
+            const uint64_t merges = k >= level ? 1ull << (k - level) : 0;
+            for (uint64_t j = threadIdx.x; j < merges; j += blockDim.x) {
+                const uint64_t left = start + (j << level), right = left + (1ull << (level - 1));
+                uint32_t block[16], cv[8];
+#pragma unroll
+                for (int w = 0; w < 8; w++) {
+                    block[w] = cvs[8 * left + w];
+                    block[8 + w] = cvs[8 * right + w];
+                }
+                b3::iv_init(cv);
+                b3::compress(cv, block, 0, 64, b3::PARENT);
+#pragma unroll
+                for (int w = 0; w < 8; w++) cvs[8 * left + w] = cv[w];
+            }
+            start += 1ull << k;
+        }
+        __syncthreads();
+    }
+    uint64_t start = 0;
+    for (int r = 0; r < runs.n; r++) {
+        if (threadIdx.x < 8) roots[8 * r + threadIdx.x] = cvs[8 * start + threadIdx.x];
+        start += 1ull << runs.k[r];
+    }
+}
+
+int blake3_runs(bitz_ctx* ctx, const void* data, uint64_t m, uint64_t c0, const std::vector<int>& ks, void* cvs,
+                void* roots) {
+    BITZ_REQUIRE(((uintptr_t)data & 3) == 0, BITZ_ERR_ARG, "blake3_runs: data must be 4-byte aligned");
+    BITZ_REQUIRE((int)ks.size() <= kMaxB3Runs, BITZ_ERR_ARG, "blake3_runs: too many runs");
+    uint64_t total = 0;
+    B3Runs runs{};
+    runs.n = (int)ks.size();
+    for (int r = 0; r < runs.n; r++) {
+        BITZ_REQUIRE(ks[r] >= 0 && ks[r] < 40, BITZ_ERR_ARG, "blake3_runs: run too long");
+        runs.k[r] = (uint8_t)ks[r];
+        total += 1ull << ks[r];
+    }
+    BITZ_REQUIRE(total == m, BITZ_ERR_ARG, "blake3_runs: runs do not tile the chunks");
+    if (m == 0) return BITZ_OK;
+    hipLaunchKernelGGL(k_b3_chunk_cvs, dim3((unsigned)((m + 63) / 64)), dim3(64), 0, ctx->stream,
+                       (const uint32_t*)data, m, c0, (uint32_t*)cvs);
+    hipLaunchKernelGGL(k_b3_runs, dim3(1), dim3(256), 0, ctx->stream, (uint32_t*)cvs, runs, (uint32_t*)roots);
     BITZ_LAUNCH_CHECK();
     return BITZ_OK;
 }

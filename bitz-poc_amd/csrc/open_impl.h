@@ -11,6 +11,22 @@ namespace bitz {
 int read_result(bitz_ctx* ctx, int K, uint8_t* host_out);
 
 int eq_table(bitz_ctx* ctx, const void* r_dev, int n, void* out);
+// Tables over host coordinates passed as a kernel argument (no upload, no
+// synchronisation): out[x] = prod_i (bit i of x ? b[i] : a[i]), and the eq
+// table (a = 1 + r, b = r), for n <= kMaxTensorN.
+constexpr int kMaxTensorN = 32;
+struct TensorArgs {
+    uint4 a[kMaxTensorN];
+    uint4 b[kMaxTensorN];
+};
+int tensor_args(bitz_ctx* ctx, const hgf* a, const hgf* b, int n, void* out);
+// BLAKE3 chaining values of m whole chunks of a transcript stream (chunk i at
+// data + 1024 i, counter c0 + i; data 4-byte aligned) merged into the
+// subtrees of the runs ks (blake3_subtree_runs(c0, m)): run r's root lands
+// at roots + 32 r. cvs: scratch of 32 m bytes.
+int blake3_runs(bitz_ctx* ctx, const void* data, uint64_t m, uint64_t c0, const std::vector<int>& ks, void* cvs,
+                void* roots);
+int eq_args(bitz_ctx* ctx, const hgf* r, int n, void* out);
 int fold_columns(bitz_ctx* ctx, const uint64_t* rows, int t, int s, const void* w, void* out);
 int pow_images(bitz_ctx* ctx, const uint8_t g[16], const void* exps, uint64_t n, void* out);
 int pack_columns(bitz_ctx* ctx, const uint64_t* rows, int t, int s, uint64_t* packed);

@@ -329,6 +329,10 @@ struct Prover {
     }
     // eq table over host coordinates into `buf`
     int eq_dev(const std::vector<G>& r, DevBuf& buf) {
+        if ((int)r.size() <= kMaxTensorN) {
+            if (!buf.alloc((16ull << r.size()))) return BITZ_ERR_ALLOC;
+            return eq_args(ctx, r.data(), (int)r.size(), buf.ptr);
+        }
         if (!pt.alloc(std::max<size_t>(r.size(), 1) * 16)) return BITZ_ERR_ALLOC;
         if (!r.empty()) {
             BITZ_HIP(hipMemcpyAsync(pt.ptr, r.data(), r.size() * 16, hipMemcpyHostToDevice, ctx->stream));
@@ -336,6 +340,49 @@ struct Prover {
         }
         if (!buf.alloc((16ull << r.size()))) return BITZ_ERR_ALLOC;
         return eq_table(ctx, pt.ptr, (int)r.size(), buf.ptr);
+    }
+
+    // Public message len(a) u64 || a || len(b) u64 || b || target (a, b device
+    // arrays of field elements): bytes assembled in device memory, whole
+    // chunks hashed there (blake3_runs), lead and tail bytes on the host.
+    int public_weights(InnerTranscript& it, const void* a, uint64_t na, const void* b, uint64_t nb, G target) {
+        const size_t len = 16 * (na + nb) + 32;
+        int rc = ctx->scratch.ensure(64);  // the pinned header staging
+        if (rc) return rc;
+        const Transcript::SlicePlan plan = it.public_plan(len);
+        const size_t pad = (4 - plan.lead % 4) % 4;  // chunk data 4-byte aligned
+        DevBuf msg, cvs, roots;
+        const std::vector<int> ks = blake3_subtree_runs(plan.c0, plan.chunks);
+        if (!msg.alloc(pad + len + 64) || !cvs.alloc(32 * std::max<uint64_t>(plan.chunks, 1)) ||
+            !roots.alloc(32 * std::max<size_t>(ks.size(), 1)))
+            return BITZ_ERR_ALLOC;
+        uint8_t* m = (uint8_t*)msg.ptr + pad;
+        uint8_t* hdr = (uint8_t*)ctx->scratch.host;  // pinned: the copies below stay asynchronous
+        uint64_t l = na;
+        memcpy(hdr, &l, 8);
+        l = nb;
+        memcpy(hdr + 8, &l, 8);
+        gput(target, hdr + 16);
+        BITZ_HIP(hipMemcpyAsync(m, hdr, 8, hipMemcpyHostToDevice, ctx->stream));
+        BITZ_HIP(hipMemcpyAsync(m + 8, a, 16 * na, hipMemcpyDeviceToDevice, ctx->stream));
+        BITZ_HIP(hipMemcpyAsync(m + 8 + 16 * na, hdr + 8, 8, hipMemcpyHostToDevice, ctx->stream));
+        BITZ_HIP(hipMemcpyAsync(m + 16 + 16 * na, b, 16 * nb, hipMemcpyDeviceToDevice, ctx->stream));
+        BITZ_HIP(hipMemcpyAsync(m + 16 + 16 * (na + nb), hdr + 16, 16, hipMemcpyHostToDevice, ctx->stream));
+        if ((rc = blake3_runs(ctx, m + plan.lead, plan.chunks, plan.c0, ks, cvs.ptr, roots.ptr))) return rc;
+        const size_t tail = len - plan.lead - 1024 * plan.chunks;
+        if ((rc = ctx->scratch.ensure_stage(plan.lead + tail + 32 * ks.size()))) return rc;
+        uint8_t* st = (uint8_t*)ctx->scratch.stage;
+        BITZ_HIP(hipMemcpyAsync(st, m, plan.lead, hipMemcpyDeviceToHost, ctx->stream));
+        BITZ_HIP(hipMemcpyAsync(st + plan.lead, m + plan.lead + 1024 * plan.chunks, tail, hipMemcpyDeviceToHost,
+                                ctx->stream));
+        if (!ks.empty())
+            BITZ_HIP(hipMemcpyAsync(st + plan.lead + tail, roots.ptr, 32 * ks.size(), hipMemcpyDeviceToHost,
+                                    ctx->stream));
+        BITZ_HIP(hipStreamSynchronize(ctx->stream));
+        std::vector<uint32_t> run_cvs(8 * ks.size());
+        if (!ks.empty()) memcpy(run_cvs.data(), st + plan.lead + tail, 32 * ks.size());
+        it.public_message_runs(len, plan, st, run_cvs.data(), st + plan.lead);
+        return BITZ_OK;
     }
 
     // Round 0 value y = MLE[p_msg](point) (ood_eval, src/ligerito_flock.rs:1459-1486)
@@ -413,9 +460,8 @@ struct Prover {
                 inv[i] = hgf_mul(inv[i], cur);
                 cur = hgf_mul(cur, v[i]);
             }
-            if ((rc = upload_gf(inv, pt))) return rc;
             if (!eqcinv.alloc(16ull << s)) return BITZ_ERR_ALLOC;
-            if ((rc = tensor_table(ctx, pt.ptr, (const uint4*)pt.ptr + s, s, eqcinv.ptr))) return rc;
+            if ((rc = tensor_args(ctx, inv.data(), inv.data() + s, s, eqcinv.ptr))) return rc;
         }
         if (per_layer && ctx->device_fs)
             return gkr_layer_device(it, l, r, kind, internal, corr, weigh, claim);
@@ -550,11 +596,12 @@ struct Prover {
         BITZ_HIP(hipStreamSynchronize(ctx->stream));
         std::vector<E> coords(rem0 > 0 ? rem0 : 1), w(1ull << rem0);
         for (int i = 0; i < rem0; i++) coords[i] = E{ext[i].lo, ext[i].hi};
+        hostgf::eq_table(coords.data(), rem0, w.data());
         for (int round = round0; round < L; round++) {
             const int rem = L - 1 - round;
             const uint64_t hh = 1ull << rem;
             const G z = internal[round];
-            hostgf::eq_table(coords.data(), rem, w.data());
+            if (round > round0) hostgf::drop_top(w.data(), hh);  // eq over ext[0..rem)
             E sums[2];
             hostgf::gkr_round_sums(hl, hr, hh, w.data(), geq(z, gzero()) ? 1 : 0, sums);
             uint8_t msg[32];
@@ -745,10 +792,27 @@ struct Prover {
         void* W = w.ptr;
         void* A2 = a2.ptr;
         void* W2 = w2.ptr;
+        hostgf::E* ha = nullptr;  // the tables on the host once they are small
+        hostgf::E* hw = nullptr;
         while (n > 1) {
-            if ((rc = round_poly(ctx, A, W, n, nullptr))) return rc;
             G v[3];
-            if ((rc = result(3, v))) return rc;
+            if (!ha && n <= (2ull << kHostTailLog)) {
+                // the remaining rounds on the host (as host_tail for GKR)
+                if ((rc = ctx->scratch.ensure_stage(2 * n * sizeof(hostgf::E)))) return rc;
+                ha = (hostgf::E*)ctx->scratch.stage;
+                hw = ha + n;
+                BITZ_HIP(hipMemcpyAsync(ha, A, n * 16, hipMemcpyDeviceToHost, ctx->stream));
+                BITZ_HIP(hipMemcpyAsync(hw, W, n * 16, hipMemcpyDeviceToHost, ctx->stream));
+                BITZ_HIP(hipStreamSynchronize(ctx->stream));
+            }
+            if (ha) {
+                hostgf::E e[3];
+                hostgf::inner_round(ha, hw, n / 2, e);
+                for (int k = 0; k < 3; k++) v[k] = G{e[k].lo, e[k].hi};
+            } else {
+                if ((rc = round_poly(ctx, A, W, n, nullptr))) return rc;
+                if ((rc = result(3, v))) return rc;
+            }
             G c0 = v[0], c2 = v[2], c1 = hgf_add(hgf_add(v[0], v[1]), v[2]);
             if (!geq(hgf_add(c1, c2), *target)) {
                 set_error("inner sumcheck: claim does not match the committed bits");
@@ -764,19 +828,30 @@ struct Prover {
             G r = gfrom(rb);
             *target = hgf_add(c0, hgf_mul(r, hgf_add(c1, hgf_mul(r, c2))));
             point.push_back(r);
-            if ((rc = fold_lsb(ctx, A, n, rb, A2))) return rc;
-            if ((rc = fold_lsb(ctx, W, n, rb, W2))) return rc;
-            std::swap(A, A2);
-            std::swap(W, W2);
+            if (ha) {
+                hostgf::fold_lsb(ha, n / 2, hostgf::E{r.lo, r.hi});
+                hostgf::fold_lsb(hw, n / 2, hostgf::E{r.lo, r.hi});
+            } else {
+                if ((rc = fold_lsb(ctx, A, n, rb, A2))) return rc;
+                if ((rc = fold_lsb(ctx, W, n, rb, W2))) return rc;
+                std::swap(A, A2);
+                std::swap(W, W2);
+            }
             n >>= 1;
         }
-        // final values live in A[0], W[0]
-        last_a = A;
-        last_w = W;
+        uint8_t fin[32];
+        if (ha) {
+            memcpy(fin, ha, 16);
+            memcpy(fin + 16, hw, 16);
+        } else {
+            if ((rc = download(A, fin, 16))) return rc;
+            if ((rc = download(W, fin + 16, 16))) return rc;
+        }
+        final_a = gfrom(fin);
+        final_w = gfrom(fin + 16);
         return BITZ_OK;
     }
-    void* last_a = nullptr;
-    void* last_w = nullptr;
+    G final_a{0, 0}, final_w{0, 0};  // the tables' last entries after the rounds
 
     int commit_level(const void* f, int log_msg, int log_lanes, int rate, DevBuf& cw, DevBuf& tree, uint8_t root[32]) {
         int k_code = log_msg - log_lanes + rate;
@@ -1062,15 +1137,9 @@ int Prover::run(const uint8_t* claimed_in, uint8_t* claimed_out, std::vector<uin
         it.public_message(h->root, 32);
         uint64_t pcs40[5] = {(uint64_t)m, (uint64_t)h->log_inv_rate, (uint64_t)h->log_batch, 0, 1};
         it.public_message(pcs40, 40);
-        std::vector<uint8_t> enc(16 * (nrows + ncols) + 32);
-        uint64_t len = nrows;
-        memcpy(enc.data(), &len, 8);
-        if ((rc = download(u1.ptr, enc.data() + 8, nrows * 16))) return rc;
-        len = ncols;
-        memcpy(enc.data() + 8 + nrows * 16, &len, 8);
-        if ((rc = download(u2.ptr, enc.data() + 16 + nrows * 16, ncols * 16))) return rc;
-        gput(ip_target, enc.data() + 16 + (nrows + ncols) * 16);
-        it.public_message(enc.data(), enc.size());
+        // the statement u1 || u2 (lengths framed): its whole chunks are
+        // hashed on the device where u1, u2 live, the rest on the host
+        if ((rc = public_weights(it, u1.ptr, nrows, u2.ptr, ncols, ip_target))) return rc;
     }
     it.public_message(std::string("bitz/pcs/inner-product-sumcheck/v1"));
     // row rounds on the column-combined table
@@ -1081,7 +1150,7 @@ int Prover::run(const uint8_t* claimed_in, uint8_t* claimed_out, std::vector<uin
     G target = ip_target;
     if ((rc = sumcheck_rounds(it, comb, u1, a2, w2, nrows, &target, sc_point))) return rc;
     uint8_t rs_bytes[16];
-    if ((rc = download(last_w, rs_bytes, 16))) return rc;
+    gput(final_w, rs_bytes);
     // column rounds on the row-folded table
     if ((rc = eq_dev(sc_point, tmp0))) return rc;
     DevBuf folded, cols, f2, c2;
@@ -1091,8 +1160,8 @@ int Prover::run(const uint8_t* claimed_in, uint8_t* claimed_out, std::vector<uin
     if ((rc = scale(ctx, u2.ptr, ncols, rs_bytes, cols.ptr))) return rc;
     if ((rc = sumcheck_rounds(it, folded, cols, f2, c2, ncols, &target, sc_point))) return rc;
     uint8_t evb[32];
-    if ((rc = download(last_a, evb, 16))) return rc;
-    if ((rc = download(last_w, evb + 16, 16))) return rc;
+    gput(final_a, evb);
+    gput(final_w, evb + 16);
     G evaluation = gfrom(evb);
     if (!geq(target, hgf_mul(evaluation, gfrom(evb + 16)))) {
         set_error("inner sumcheck: final check failed");

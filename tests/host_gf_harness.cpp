@@ -32,6 +32,23 @@ void hg_round_sums(int clmul, const uint64_t* l, const uint64_t* r, uint64_t h, 
         bitz::hostgf::round_sums_t<false>((const E*)l, (const E*)r, h, (const E*)w, send_one, (E*)out);
 }
 
+void hg_inner_round(int clmul, const uint64_t* m, const uint64_t* w, uint64_t pairs, uint64_t* out) {
+    if (clmul)
+        bitz::hostgf::inner_round_t<true>((const E*)m, (const E*)w, pairs, (E*)out);
+    else
+        bitz::hostgf::inner_round_t<false>((const E*)m, (const E*)w, pairs, (E*)out);
+}
+
+void hg_fold_lsb(int clmul, uint64_t* v, uint64_t half, const uint64_t* r) {
+    E x{r[0], r[1]};
+    if (clmul)
+        bitz::hostgf::fold_lsb_t<true>((E*)v, half, x);
+    else
+        bitz::hostgf::fold_lsb_t<false>((E*)v, half, x);
+}
+
+void hg_drop_top(uint64_t* v, uint64_t h) { bitz::hostgf::drop_top((E*)v, h); }
+
 void hg_fold(int clmul, uint64_t* v, uint64_t h, const uint64_t* rho) {
     E r{rho[0], rho[1]};
     if (clmul)

@@ -27,6 +27,9 @@ def hg(tmp_path_factory):
     lib.hg_eq_table.argtypes = [C.c_int, P, C.c_int, P]
     lib.hg_round_sums.argtypes = [C.c_int, P, P, C.c_uint64, P, C.c_int, P]
     lib.hg_fold.argtypes = [C.c_int, P, C.c_uint64, P]
+    lib.hg_inner_round.argtypes = [C.c_int, P, P, C.c_uint64, P]
+    lib.hg_fold_lsb.argtypes = [C.c_int, P, C.c_uint64, P]
+    lib.hg_drop_top.argtypes = [P, C.c_uint64]
     return lib
 
 
@@ -89,3 +92,38 @@ def test_fold_matches_oracle(hg, h):
         x = v.copy()
         hg.hg_fold(path, _p(x), h, _p(O.ints_to_gf([rho])))
         assert O.gf_to_ints(x[:h]) == want
+
+
+@pytest.mark.parametrize("n", [2, 8, 2048])
+def test_inner_round_matches_oracle(hg, n):
+    rng = random.Random(200 + n)
+    m = O.ints_to_gf(_rand(rng, n))
+    w = O.ints_to_gf(_rand(rng, n))
+    c0, c1, c2 = O.round_polynomial(m, w)
+    for path in _paths(hg):
+        out = np.zeros((3, 2), np.uint64)
+        hg.hg_inner_round(path, _p(m), _p(w), n // 2, _p(out))
+        v0, v1, v2 = O.gf_to_ints(out)
+        assert (v0, v0 ^ v1 ^ v2, v2) == (c0, c1, c2)
+
+
+@pytest.mark.parametrize("n", [2, 6, 1024])
+def test_fold_lsb_matches_oracle(hg, n):
+    rng = random.Random(300 + n)
+    v = O.ints_to_gf(_rand(rng, n))
+    r = rng.getrandbits(128)
+    want = O.gf_to_ints(O.fold_lsb(v, r))
+    for path in _paths(hg):
+        x = v.copy()
+        hg.hg_fold_lsb(path, _p(x), n // 2, _p(O.ints_to_gf([r])))
+        assert O.gf_to_ints(x[: n // 2]) == want
+
+
+def test_drop_top_gives_the_smaller_eq_table(hg):
+    # eq over r[0..n) with its halves added is eq over r[0..n-1): the two
+    # weights of the last coordinate, r and 1 + r, sum to one
+    rng = random.Random(400)
+    r = _rand(rng, 7)
+    t = O.eq_table(O.ints_to_gf(r)).copy()
+    hg.hg_drop_top(_p(t), 64)
+    assert O.gf_to_ints(t[:64]) == O.gf_to_ints(O.eq_table(O.ints_to_gf(r[:6])))

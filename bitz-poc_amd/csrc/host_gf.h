@@ -47,8 +47,10 @@ void inner_round(const E* m, const E* w, uint64_t pairs, E out[3]);
 // fold of the inner-product sumcheck, src/bitz/sumcheck.rs:372-400).
 void fold_lsb(E* v, uint64_t half, E r);
 
-// true when the carry-less multiply instructions are in use
+// true when the carry-less multiply instructions are in use (PCLMULQDQ;
+// VPCLMULQDQ on 512-bit registers, four elements at a time, where present)
 bool using_clmul();
+bool using_vpclmul();
 
 }  // namespace hostgf
 }  // namespace bitz

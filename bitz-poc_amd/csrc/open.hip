@@ -1361,7 +1361,21 @@ int eq_args(bitz_ctx* ctx, const hgf* r, int n, void* out) {
     BITZ_REQUIRE(n >= 0 && n <= kMaxTensorN, BITZ_ERR_SHAPE, "eq_args: n out of range");
     hgf a[kMaxTensorN];
     for (int i = 0; i < n; i++) a[i] = hgf_add(r[i], hgf{1, 0});
-    return tensor_args(ctx, a, r, n, out);
+    if (n <= 14) return tensor_args(ctx, a, r, n, out);
+    // large tables: the tables of the low and high halves of the
+    // coordinates, then one product per entry (a pass at memory speed)
+    const int nlo = n / 2, nhi = n - nlo;
+    if (!ctx->scratch.eq_lo.alloc((16ull << nlo) + (16ull << nhi))) return BITZ_ERR_ALLOC;
+    uint4* lo = (uint4*)ctx->scratch.eq_lo.ptr;
+    uint4* hi = lo + (1ull << nlo);
+    int rc = tensor_args(ctx, a, r, nlo, lo);
+    if (rc) return rc;
+    if ((rc = tensor_args(ctx, a + nlo, r + nlo, nhi, hi))) return rc;
+    const uint64_t total = 1ull << n;
+    hipLaunchKernelGGL(k_eq_combine, dim3(blocks_for(total, 4)), dim3(kT), 0, ctx->stream, lo, nlo, hi, total,
+                       (uint4*)out);
+    BITZ_LAUNCH_CHECK();
+    return BITZ_OK;
 }
 
 int tensor_table(bitz_ctx* ctx, const void* a, const void* b, int n, void* out) {

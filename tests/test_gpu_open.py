@@ -37,7 +37,7 @@ def dev_u64(a):
     return torch.from_numpy(np.ascontiguousarray(a).view(np.int64)).to("cuda")
 
 
-@pytest.mark.parametrize("n", [0, 1, 5, 12, 13, 19, 22])
+@pytest.mark.parametrize("n", [0, 1, 5, 8, 9, 12, 13, 14, 15, 19, 22])
 def test_eq_table(n):
     r = rand_ints(n, n)
     r[:1] = [0] if n > 3 else r[:1]

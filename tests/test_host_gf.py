@@ -1,7 +1,10 @@
 """The library's host GF(2^128) loops (bitz-poc_amd/csrc/host_gf.cpp: the
-last rounds of each GKR layer run on the host) against the oracle, for both
-multiply paths (PCLMULQDQ and the integer carry-less product). The oracle's
-field is itself pinned by the NIST GCM vectors (test_oracle_gf128.py)."""
+last rounds of each sumcheck run on the host) against the oracle, for every
+multiply path this CPU has: the integer carry-less product (0), PCLMULQDQ
+(1), and the dispatched entry points (2: four elements per 512-bit register
+with VPCLMULQDQ where present, at sizes that are multiples of four). The
+oracle's field is itself pinned by the NIST GCM vectors
+(test_oracle_gf128.py)."""
 import ctypes as C
 import os
 import random
@@ -23,6 +26,7 @@ def hg(tmp_path_factory):
     lib = C.CDLL(so)
     P = C.c_void_p
     lib.hg_has_clmul.restype = C.c_int
+    lib.hg_has_vpclmul.restype = C.c_int
     lib.hg_mul.argtypes = [C.c_int, P, P, P, C.c_uint64]
     lib.hg_eq_table.argtypes = [C.c_int, P, C.c_int, P]
     lib.hg_round_sums.argtypes = [C.c_int, P, P, C.c_uint64, P, C.c_int, P]
@@ -38,7 +42,7 @@ def _p(a):
 
 
 def _paths(hg):
-    return [0, 1] if hg.hg_has_clmul() else [0]
+    return [0, 1, 2] if hg.hg_has_clmul() else [0, 2]
 
 
 def _rand(rng, n):

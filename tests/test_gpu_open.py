@@ -109,7 +109,7 @@ def test_gkr_round_sums(log_h, s_c, send_one):
     assert np.array_equal(to_host(v)[:h], O.fold_halves(l, rho))
 
 
-@pytest.mark.parametrize("log_h,s_c", [(0, 0), (3, 3), (5, 2), (10, 4), (17, 11)])
+@pytest.mark.parametrize("log_h,s_c", [(0, 0), (3, 3), (5, 2), (10, 4), (17, 11), (20, 13)])
 @pytest.mark.parametrize("send_one", [False, True])
 def test_gkr_fold_round_sums(log_h, s_c, send_one):
     h = 1 << log_h

@@ -81,38 +81,60 @@ struct DevCoords {
     __device__ __forceinline__ uint4 b(int i) const { return pb[i]; }
 };
 
+// prod_{i < cnt} (bit i of x ? b(base + i) : a(base + i)) for cnt <= 4, as
+// a two-level tree (three products at most)
+template <class C>
+__device__ __forceinline__ gf128 tensor_prod4(const C& c, int base, int cnt, uint32_t x) {
+    gf128 f[4];
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        if (i < cnt) {
+            const uint4 v = ((x >> i) & 1u) ? c.b(base + i) : c.a(base + i);
+            f[i] = gf_load(&v);
+        } else {
+            f[i] = gf_one();
+        }
+    }
+    gf128 p = cnt > 1 ? gf_mul(f[0], f[1]) : f[0];
+    if (cnt > 2) p = gf_mul(p, cnt > 3 ? gf_mul(f[2], f[3]) : f[2]);
+    return p;
+}
+
 // out[x] = prod_i (bit i of x ? b(i) : a(i)) for x < 2^n. Block k writes
-// entries 256k..256k+255: the table of the low 8 coordinates is built in LDS
-// by doubling, the product of the high coordinates selected by the bits of k
-// by a tree, then one multiply per entry.
+// entries 256k..256k+255 (the low 8 coordinates; the high ones are fixed by
+// k). Three waves work side by side: wave 0 the table A of coordinates 0-3,
+// wave 1 the table B of coordinates 4-7, wave 2 the product H of the high
+// coordinates' factors by a shuffle tree; then entry x = A[x & 15] B[x >> 4]
+// H. The longest chain is five products.
 template <class C>
 __global__ __launch_bounds__(256) void k_tensor(C c, int n, uint4* __restrict__ out) {
-    __shared__ uint4 lo[256];
-    __shared__ uint4 hif[kMaxTensorN];
-    const uint32_t t = threadIdx.x;
-    const int nlo = n < 8 ? n : 8, nh = n - nlo;
-    if (t == 0) lo[0] = make_uint4(1, 0, 0, 0);
-    __syncthreads();
-    for (int i = 0; i < nlo; i++) {
-        const uint32_t len = 1u << i;
-        if (t < len) {
-            gf128 p = gf_load(&lo[t]);
-            const uint4 ai = c.a(i), bi = c.b(i);
-            gf_store(&lo[t + len], gf_mul(p, gf_load(&bi)));
-            gf_store(&lo[t], gf_mul(p, gf_load(&ai)));
+    __shared__ uint4 ta[16], tb[16], th[1];
+    const uint32_t t = threadIdx.x, wave = t >> 6, lane = t & 63;
+    const int nlo = n < 8 ? n : 8, la = nlo < 4 ? nlo : 4, lb = nlo - la, nh = n - nlo;
+    if (wave == 0) {
+        if (lane < (1u << la)) gf_store(&ta[lane], tensor_prod4(c, 0, la, lane));
+    } else if (wave == 1) {
+        if (lane < (1u << lb)) gf_store(&tb[lane], tensor_prod4(c, la, lb, lane));
+    } else if (wave == 2) {
+        gf128 v = gf_one();
+        if ((int)lane < nh) {
+            const uint4 f = ((blockIdx.x >> lane) & 1u) ? c.b(nlo + lane) : c.a(nlo + lane);
+            v = gf_load(&f);
         }
-        __syncthreads();
+        for (int off = 1; off < nh; off <<= 1) {  // lanes past nh hold one
+            gf128 o;
+#pragma unroll
+            for (int w = 0; w < 4; w++) o.w[w] = __shfl_down(v.w[w], off, 64);
+            if ((int)(lane + off) >= 64) o = gf_one();
+            v = gf_mul(v, o);
+        }
+        if (lane == 0) gf_store(&th[0], v);
     }
-    if ((int)t < nh) hif[t] = ((blockIdx.x >> t) & 1u) ? c.b(nlo + t) : c.a(nlo + t);
     __syncthreads();
-    for (int w = 1; w < nh; w <<= 1) {
-        if ((int)(t % (2 * w)) == 0 && (int)(t + w) < nh)
-            gf_store(&hif[t], gf_mul(gf_load(&hif[t]), gf_load(&hif[t + w])));
-        __syncthreads();
-    }
     if (t < (1u << nlo)) {
-        gf128 v = gf_load(&lo[t]);
-        if (nh > 0) v = gf_mul(v, gf_load(&hif[0]));
+        gf128 v = gf_load(&ta[t & ((1u << la) - 1)]);
+        if (lb > 0) v = gf_mul(v, gf_load(&tb[t >> la]));
+        if (nh > 0) v = gf_mul(v, gf_load(&th[0]));
         gf_store(&out[((uint64_t)blockIdx.x << nlo) + t], v);
     }
 }

@@ -1801,7 +1801,10 @@ int pow_find(bitz_ctx* ctx, const uint8_t prefix[16], const uint8_t seed[16], ui
     memcpy((uint8_t*)ctx->scratch.host + 64, prefix, 16);
     memcpy((uint8_t*)ctx->scratch.host + 80, seed, 16);
     BITZ_HIP(hipMemcpyAsync(ctx->scratch.result.ptr, ctx->scratch.host, 96, hipMemcpyHostToDevice, ctx->stream));
-    uint64_t span = 1ull << 20;
+    // spans of 16 x 2^bits nonces (at least 2^14, at most 2^20): the first
+    // holds a solution with probability 1 - e^-16, and scanning spans in
+    // order keeps the smallest valid nonce
+    const uint64_t span = std::min<uint64_t>(1ull << 20, std::max<uint64_t>(1ull << 14, 16ull << bits));
     for (uint64_t base = 0;; base += span) {
         hipLaunchKernelGGL(k_pow_scan, dim3((unsigned)(span / kT)), dim3(kT), 0, ctx->stream, pre, base, span,
                            bits, best);

@@ -867,14 +867,7 @@ struct Prover {
                    const std::vector<uint64_t>& queries, LevelOpen* out) {
         out->lanes = lanes;
         out->rows.resize(queries.size() * lanes);
-        int rc;
-        if (!qdev.alloc(std::max<size_t>(queries.size(), 1) * 8 + 16)) return BITZ_ERR_ALLOC;
-        BITZ_HIP(hipMemcpyAsync(qdev.ptr, queries.data(), queries.size() * 8, hipMemcpyHostToDevice, ctx->stream));
-        if (!tmp3.alloc(std::max<size_t>(queries.size() * lanes, 1) * 16)) return BITZ_ERR_ALLOC;
-        if ((rc = gather_rows(ctx, cw, (const uint64_t*)qdev.ptr, (uint32_t)queries.size(), lanes, tmp3.ptr)))
-            return rc;
-        if ((rc = download(tmp3.ptr, out->rows.data(), out->rows.size() * 16))) return rc;
-        // sibling node indices
+        // sibling node indices (host only: they depend on the queries alone)
         std::vector<uint64_t> active(queries), nodes;
         uint64_t level_start = 0, level_len = n_leaves;
         while (level_len > 1) {
@@ -896,14 +889,26 @@ struct Prover {
             level_len >>= 1;
         }
         out->proof.resize(nodes.size() * 32);
-        if (!nodes.empty()) {
-            if (!qdev.alloc(nodes.size() * 8 + 16)) return BITZ_ERR_ALLOC;
-            BITZ_HIP(hipMemcpyAsync(qdev.ptr, nodes.data(), nodes.size() * 8, hipMemcpyHostToDevice, ctx->stream));
-            if (!tmp3.alloc(nodes.size() * 32)) return BITZ_ERR_ALLOC;
-            if ((rc = gather_nodes(ctx, tree, (const uint64_t*)qdev.ptr, (uint32_t)nodes.size(), tmp3.ptr)))
-                return rc;
-            if ((rc = download(tmp3.ptr, out->proof.data(), out->proof.size()))) return rc;
-        }
+        // one upload of both index lists, both gathers, one download round
+        const size_t nq = queries.size(), nn = nodes.size();
+        const size_t rows_bytes = nq * lanes * 16, nodes_bytes = nn * 32;
+        int rc = ctx->scratch.ensure_stage(8 * (nq + nn) + rows_bytes + nodes_bytes);
+        if (rc) return rc;
+        uint8_t* st = (uint8_t*)ctx->scratch.stage;  // pinned: the copies stay asynchronous
+        memcpy(st, queries.data(), 8 * nq);
+        memcpy(st + 8 * nq, nodes.data(), 8 * nn);
+        if (!qdev.alloc(8 * (nq + nn) + 16) || !tmp3.alloc(std::max<size_t>(rows_bytes + nodes_bytes, 16)))
+            return BITZ_ERR_ALLOC;
+        BITZ_HIP(hipMemcpyAsync(qdev.ptr, st, 8 * (nq + nn), hipMemcpyHostToDevice, ctx->stream));
+        const uint64_t* qd = (const uint64_t*)qdev.ptr;
+        uint8_t* gathered = (uint8_t*)tmp3.ptr;
+        if ((rc = gather_rows(ctx, cw, qd, (uint32_t)nq, lanes, gathered))) return rc;
+        if (nn && (rc = gather_nodes(ctx, tree, qd + nq, (uint32_t)nn, gathered + rows_bytes))) return rc;
+        uint8_t* back = st + 8 * (nq + nn);
+        BITZ_HIP(hipMemcpyAsync(back, gathered, rows_bytes + nodes_bytes, hipMemcpyDeviceToHost, ctx->stream));
+        BITZ_HIP(hipStreamSynchronize(ctx->stream));
+        memcpy(out->rows.data(), back, rows_bytes);
+        if (nn) memcpy(out->proof.data(), back + rows_bytes, nodes_bytes);
         return BITZ_OK;
     }
 

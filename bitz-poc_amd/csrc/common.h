@@ -11,6 +11,8 @@
 #include <string>
 #include <vector>
 
+#include "host_gf.h"
+
 #include "../../include/bitz_hip.h"
 
 namespace bitz {
@@ -272,38 +274,15 @@ struct PhaseClock {
 
 namespace bitz {
 
-// ---- host GF(2^128) for setup values: residue-class integer clmul ----
+// ---- host GF(2^128) values (round messages, challenges, small tables) ----
+// Products go through host_gf.cpp: PCLMULQDQ where the CPU has it.
 struct hgf {
     uint64_t lo, hi;
 };
 inline hgf hgf_add(hgf a, hgf b) { return hgf{a.lo ^ b.lo, a.hi ^ b.hi}; }
-inline void hclmul64(uint64_t a, uint64_t b, uint64_t* lo, uint64_t* hi) {
-    const uint64_t M = 0x1111111111111111ull;
-    const unsigned __int128 W = ((unsigned __int128)M << 64) | M;
-    uint64_t bl = b & 0x0fffffffffffffffull;
-    unsigned __int128 a0 = a & M, a1 = a & (M << 1), a2 = a & (M << 2), a3 = a & (M << 3);
-    unsigned __int128 b0 = bl & M, b1 = bl & (M << 1), b2 = bl & (M << 2), b3 = bl & (M << 3);
-    unsigned __int128 z0 = (a0 * b0) ^ (a1 * b3) ^ (a2 * b2) ^ (a3 * b1);
-    unsigned __int128 z1 = (a0 * b1) ^ (a1 * b0) ^ (a2 * b3) ^ (a3 * b2);
-    unsigned __int128 z2 = (a0 * b2) ^ (a1 * b1) ^ (a2 * b0) ^ (a3 * b3);
-    unsigned __int128 z3 = (a0 * b3) ^ (a1 * b2) ^ (a2 * b1) ^ (a3 * b0);
-    unsigned __int128 p = (z0 & W) | (z1 & (W << 1)) | (z2 & (W << 2)) | (z3 & (W << 3));
-    for (int i = 60; i < 64; i++)
-        if ((b >> i) & 1) p ^= ((unsigned __int128)a) << i;
-    *lo = (uint64_t)p;
-    *hi = (uint64_t)(p >> 64);
-}
 inline hgf hgf_mul(hgf a, hgf b) {
-    uint64_t l0, l1, h0, h1, m0, m1;
-    hclmul64(a.lo, b.lo, &l0, &l1);
-    hclmul64(a.hi, b.hi, &h0, &h1);
-    hclmul64(a.lo ^ a.hi, b.lo ^ b.hi, &m0, &m1);
-    uint64_t r0 = l0, r1 = l1 ^ m0 ^ l0 ^ h0, r2 = h0 ^ m1 ^ l1 ^ h1, r3 = h1;
-    uint64_t a0 = r2 ^ (r2 << 1) ^ (r2 << 2) ^ (r2 << 7);
-    uint64_t a1 = (r2 >> 63) ^ (r2 >> 62) ^ (r2 >> 57);
-    uint64_t b0 = r3 ^ (r3 << 1) ^ (r3 << 2) ^ (r3 << 7);
-    uint64_t b1 = (r3 >> 63) ^ (r3 >> 62) ^ (r3 >> 57);
-    return hgf{r0 ^ a0 ^ b1 ^ (b1 << 1) ^ (b1 << 2) ^ (b1 << 7), r1 ^ a1 ^ b0};
+    const hostgf::E r = hostgf::mul(hostgf::E{a.lo, a.hi}, hostgf::E{b.lo, b.hi});
+    return hgf{r.lo, r.hi};
 }
 inline hgf hgf_inv(hgf a) {
     hgf acc{1, 0}, sq = a;

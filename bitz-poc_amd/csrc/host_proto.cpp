@@ -291,6 +291,21 @@ void Transcript::absorb_slice(const void* data, size_t len) {
     h_.update(&tail, 1);
 }
 
+void Transcript::absorb_slices(const void* data, size_t n, size_t len) {
+    const size_t framed = len + 10;
+    std::vector<uint8_t> stream(n * framed);
+    const uint8_t* p = (const uint8_t*)data;
+    const uint64_t l = len;
+    for (size_t i = 0; i < n; i++) {
+        uint8_t* q = stream.data() + i * framed;
+        q[0] = 0x06;
+        memcpy(q + 1, &l, 8);
+        memcpy(q + 9, p + i * len, len);
+        q[9 + len] = 0x07;
+    }
+    h_.update(stream.data(), stream.size());
+}
+
 Transcript::SlicePlan Transcript::slice_plan(size_t len) const {
     const uint64_t start = h_.position() + 9;  // after the 0x06 || len frame
     const uint64_t lead = std::min<uint64_t>((1024 - start % 1024) % 1024, len);

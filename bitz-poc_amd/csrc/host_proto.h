@@ -71,6 +71,10 @@ class Transcript {
     // absorb_slice: 0x06 || len u64 || bytes || 0x07
     void absorb_slice(const void* data, size_t len);
     void absorb_slice(const std::string& s) { absorb_slice(s.data(), s.size()); }
+    // absorb_slice of n messages of len bytes each (data = the n messages
+    // back to back): the framed messages are absorbed in one update, which
+    // takes the hasher's multi-threaded whole-chunk path when long.
+    void absorb_slices(const void* data, size_t n, size_t len);
     void absorb_u64(uint64_t v) { absorb_slice(&v, 8); }
     // absorb_slice of len bytes whose whole chunks are hashed elsewhere:
     // slice_plan gives the bytes before the first chunk boundary inside the
@@ -115,6 +119,12 @@ class InnerTranscript {
         t_.absorb_slice(data, len);
         const uint8_t* p = (const uint8_t*)data;
         narg_.insert(narg_.end(), p, p + len);
+    }
+    // n prover messages of len bytes each, back to back in data
+    void prover_messages(const void* data, size_t n, size_t len) {
+        t_.absorb_slices(data, n, len);
+        const uint8_t* p = (const uint8_t*)data;
+        narg_.insert(narg_.end(), p, p + n * len);
     }
     // u32 length-prefixed byte string as one prover message
     void prover_message_bytes(const void* data, size_t len);

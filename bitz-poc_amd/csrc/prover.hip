@@ -45,6 +45,25 @@ inline G eq_factor(G r, G z) { return hgf_add(hgf_mul(r, z), hgf_mul(hgf_add(r, 
 // (tens of microseconds) on the device.
 constexpr int kHostTailLog = 10;
 
+// out[i] = v[i]^-1 with one inversion (prefix products; Montgomery's
+// trick); a zero entry maps to zero, as hgf_inv does
+void batch_inverse(const std::vector<G>& v, std::vector<G>* out) {
+    const size_t n = v.size();
+    out->assign(n, gzero());
+    std::vector<G> pre(n);
+    G acc = gone();
+    for (size_t i = 0; i < n; i++) {
+        pre[i] = acc;
+        if (!geq(v[i], gzero())) acc = hgf_mul(acc, v[i]);
+    }
+    G cur = hgf_inv(acc);
+    for (size_t i = n; i-- > 0;) {
+        if (geq(v[i], gzero())) continue;
+        (*out)[i] = hgf_mul(pre[i], cur);
+        cur = hgf_mul(cur, v[i]);
+    }
+}
+
 std::vector<G> host_eq(const std::vector<G>& r) {
     std::vector<G> t(1ull << r.size());
     t[0] = gone();
@@ -1082,30 +1101,24 @@ int Prover::run(const uint8_t* claimed_in, uint8_t* claimed_out, std::vector<uin
         u128_put(q, frame + 16);
         gput(g, frame + 32);
         it.public_message(frame, 48);
-        // LinearClaim encoding (src/bitz/params.rs:273-285)
-        std::vector<uint8_t> enc;
-        enc.reserve(16 * (nrows + ncols) + 32);
-        uint64_t len = nrows;
-        enc.insert(enc.end(), (uint8_t*)&len, (uint8_t*)&len + 8);
-        for (auto v : row_w) {
+        // LinearClaim encoding (src/bitz/params.rs:273-285): len || row
+        // weights || len || column weights || claim, hashed where the row
+        // weights already are (the device copy made for the folds)
+        std::vector<uint8_t> cb(ncols * 16);
+        for (uint64_t c = 0; c < ncols; c++) u128_put(col_w[c], cb.data() + 16 * c);
+        DevBuf colw;
+        if (!colw.alloc(ncols * 16)) return BITZ_ERR_ALLOC;
+        BITZ_HIP(hipMemcpyAsync(colw.ptr, cb.data(), cb.size(), hipMemcpyHostToDevice, ctx->stream));
+        G claim_g;
+        {
             uint8_t b[16];
-            u128_put(v, b);
-            enc.insert(enc.end(), b, b + 16);
+            u128_put(claimed, b);
+            claim_g = gfrom(b);
         }
-        len = ncols;
-        enc.insert(enc.end(), (uint8_t*)&len, (uint8_t*)&len + 8);
-        for (auto v : col_w) {
-            uint8_t b[16];
-            u128_put(v, b);
-            enc.insert(enc.end(), b, b + 16);
-        }
-        uint8_t b[16];
-        u128_put(claimed, b);
-        enc.insert(enc.end(), b, b + 16);
-        it.public_message(enc.data(), enc.size());
+        if ((rc = public_weights(it, roww.ptr, nrows, colw.ptr, ncols, claim_g))) return rc;
     }
-    // fold round (src/bitz/fold.rs:237-263)
-    for (uint64_t c = 0; c < ncols; c++) it.prover_message(&fold_bytes[16 * c], 16);
+    // fold round (src/bitz/fold.rs:237-263): one prover message per column
+    it.prover_messages(fold_bytes.data(), ncols, 16);
     std::vector<G> zeta(s);
     for (auto& z : zeta) {
         uint8_t b[16];
@@ -1381,7 +1394,7 @@ int Prover::run(const uint8_t* claimed_in, uint8_t* claimed_out, std::vector<uin
                 c_--;
             }
         }
-        for (int k = 0; k <= log_n; k++) inv[k] = hgf_inv(sks[k]);
+        batch_inverse(sks, &inv);
         std::vector<G> both(sks);
         both.insert(both.end(), inv.begin(), inv.end());
         if ((r_ = upload_gf(both, tmp2))) return r_;

@@ -171,6 +171,34 @@ __global__ void k_fold_columns_partial(const uint64_t* __restrict__ rows, int t,
     partial[ch * cols + c] = make_uint4((uint32_t)lo, (uint32_t)(lo >> 32), (uint32_t)hi, (uint32_t)(hi >> 32));
 }
 
+// The same with the chunk's 4096 row weights staged in LDS (64 KiB) so the
+// per-set-bit gathers hit LDS: block = 256 consecutive columns of one chunk
+// of 64 words (needs 2^s a multiple of 256 and 64-word chunks).
+__global__ __launch_bounds__(256) void k_fold_columns_partial_lds(const uint64_t* __restrict__ rows, int t, int s,
+                                                                  const uint4* __restrict__ w,
+                                                                  uint4* __restrict__ partial) {
+    __shared__ uint4 ws[64 * 64];
+    const uint64_t words = 1ull << (t - 6), cols = 1ull << s;
+    const uint64_t blocks_per_chunk = cols / 256;
+    const uint64_t ch = blockIdx.x / blocks_per_chunk;
+    const uint64_t c = (blockIdx.x % blocks_per_chunk) * 256 + threadIdx.x;
+    const uint4* wc = w + ch * 64 * 64;
+    for (uint32_t i = threadIdx.x; i < 64 * 64; i += 256) ws[i] = wc[i];
+    __syncthreads();
+    uint64_t lo = 0, hi = 0;
+    const uint64_t* col = rows + c * words + ch * 64;
+    for (uint32_t k = 0; k < 64; k++) {
+        uint64_t x = col[k];
+        while (x) {
+            const int b = __builtin_ctzll(x);
+            const uint4 e = ws[k * 64 + b];
+            add128(lo, hi, (uint64_t)e.x | ((uint64_t)e.y << 32), (uint64_t)e.z | ((uint64_t)e.w << 32));
+            x &= x - 1;
+        }
+    }
+    partial[ch * cols + c] = make_uint4((uint32_t)lo, (uint32_t)(lo >> 32), (uint32_t)hi, (uint32_t)(hi >> 32));
+}
+
 __global__ void k_fold_columns_sum(const uint4* __restrict__ partial, uint64_t chunks, uint64_t cols,
                                    uint4* __restrict__ out) {
     uint64_t c = grid_tid();
@@ -1277,8 +1305,12 @@ int fold_columns(bitz_ctx* ctx, const uint64_t* rows, int t, int s, const void* 
     if (!ctx->scratch.fold_partial.alloc(chunks * cols * 16)) return BITZ_ERR_ALLOC;
     uint4* part = (uint4*)ctx->scratch.fold_partial.ptr;
     uint64_t items = cols * chunks;
-    hipLaunchKernelGGL(k_fold_columns_partial, dim3((unsigned)((items + kT - 1) / kT)), dim3(kT), 0, ctx->stream,
-                       rows, t, s, (const uint4*)w, chunk, part);
+    if (chunk == 64 && cols % 256 == 0)
+        hipLaunchKernelGGL(k_fold_columns_partial_lds, dim3((unsigned)(items / 256)), dim3(256), 0, ctx->stream, rows,
+                           t, s, (const uint4*)w, part);
+    else
+        hipLaunchKernelGGL(k_fold_columns_partial, dim3((unsigned)((items + kT - 1) / kT)), dim3(kT), 0, ctx->stream,
+                           rows, t, s, (const uint4*)w, chunk, part);
     hipLaunchKernelGGL(k_fold_columns_sum, dim3((unsigned)((cols + kT - 1) / kT)), dim3(kT), 0, ctx->stream, part,
                        chunks, cols, (uint4*)out);
     BITZ_LAUNCH_CHECK();

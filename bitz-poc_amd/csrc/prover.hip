@@ -130,6 +130,14 @@ struct ModN {
         wide(t, r2, &hi, &lo);
         return redc(hi, lo);
     }
+    // a b R^-1 mod n, for a b < R n (e.g. any a < R with b < n): one
+    // product and one reduction, no division
+    u128 mont(u128 a, u128 b) const {
+        u128 hi, lo;
+        wide(a, b, &hi, &lo);
+        return redc(hi, lo);
+    }
+    u128 to_mont(u128 a) const { return mont(a % n, r2); }  // a R mod n
 };
 
 // a b mod q (q odd, q < 2^126: every modulus here is a prime candidate or the
@@ -217,12 +225,21 @@ std::vector<u128> eq_table_mod_q(const std::vector<u128>& r, u128 q) {
     const size_t k = n / 2;
     std::vector<u128> hi = eq_table_mod_q_serial(std::vector<u128>(r.begin(), r.end() - k), q);
     std::vector<u128> lo = eq_table_mod_q_serial(std::vector<u128>(r.end() - k, r.end()), q);
+    // hi in Montgomery form: each entry is then one product and one reduction
+    ModN m;
+    m.set(q);
+    for (auto& v : hi) v = m.to_mont(v);
     std::vector<u128> out(1ull << n);
-    unsigned nt = std::max(1u, std::min(8u, std::thread::hardware_concurrency()));
+    const size_t per = (size_t)1 << 12;  // entries per task
+    const size_t tasks = (out.size() + per - 1) / per;
+    unsigned nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    nt = (unsigned)std::min<size_t>(nt, tasks);
     std::vector<std::thread> pool;
     for (unsigned w = 0; w < nt; w++)
         pool.emplace_back([&, w]() {
-            for (size_t x = w; x < out.size(); x += nt) out[x] = mulmod(hi[x >> k], lo[x & ((1ull << k) - 1)], q);
+            for (size_t tk = w; tk < tasks; tk += nt)
+                for (size_t x = tk * per; x < std::min(out.size(), (tk + 1) * per); x++)
+                    out[x] = m.mont(hi[x >> k], lo[x & ((1ull << k) - 1)]);
         });
     for (auto& th : pool) th.join();
     return out;
@@ -1057,20 +1074,33 @@ int Prover::run(const uint8_t* claimed_in, uint8_t* claimed_out, std::vector<uin
     std::vector<u128> r1(t), r2(s);
     for (auto& x : r1) x = sample_residue(outer, q);
     for (auto& x : r2) x = sample_residue(outer, q);
+    phase.mark(proof_out ? "instance_weights" : "eval_instance_weights");
     std::vector<u128> row_w = eq_table_mod_q(r1, q), col_w = eq_table_mod_q(r2, q);
+    phase.mark(proof_out ? "instance_folds" : "eval_instance_folds");
     // folds eta_c = sum_b w_b bit(c, b) (exact u128) on the device
-    {
-        std::vector<uint8_t> wb(nrows * 16);
-        for (uint64_t b = 0; b < nrows; b++) u128_put(row_w[b], wb.data() + 16 * b);
-        if (!roww.alloc(nrows * 16) || !folds.alloc(ncols * 16)) return BITZ_ERR_ALLOC;
-        BITZ_HIP(hipMemcpyAsync(roww.ptr, wb.data(), wb.size(), hipMemcpyHostToDevice, ctx->stream));
-        if ((rc = fold_columns(ctx, (const uint64_t*)h->packed.ptr, t, s, roww.ptr, folds.ptr))) return rc;
-        BITZ_HIP(hipStreamSynchronize(ctx->stream));
-    }
     std::vector<uint8_t> fold_bytes(ncols * 16);
-    if ((rc = download(folds.ptr, fold_bytes.data(), fold_bytes.size()))) return rc;
+    {
+        // row weights up through pinned staging (u128 little-endian is the
+        // 16-byte wire form), folds back the same way
+        if ((rc = ctx->scratch.ensure_stage(16 * (nrows + ncols)))) return rc;
+        uint8_t* st = (uint8_t*)ctx->scratch.stage;
+        for (uint64_t b = 0; b < nrows; b++) u128_put(row_w[b], st + 16 * b);
+        if (!roww.alloc(nrows * 16) || !folds.alloc(ncols * 16)) return BITZ_ERR_ALLOC;
+        BITZ_HIP(hipMemcpyAsync(roww.ptr, st, nrows * 16, hipMemcpyHostToDevice, ctx->stream));
+        if ((rc = fold_columns(ctx, (const uint64_t*)h->packed.ptr, t, s, roww.ptr, folds.ptr))) return rc;
+        BITZ_HIP(hipMemcpyAsync(st + 16 * nrows, folds.ptr, ncols * 16, hipMemcpyDeviceToHost, ctx->stream));
+        BITZ_HIP(hipStreamSynchronize(ctx->stream));
+        memcpy(fold_bytes.data(), st + 16 * nrows, ncols * 16);
+    }
     u128 honest = 0;
-    for (uint64_t c = 0; c < ncols; c++) honest = addmod(honest, mulmod(col_w[c], u128_from(&fold_bytes[16 * c]) % q, q), q);
+    {
+        // sum_c fold_c col_w_c mod q: each term as a Montgomery product
+        // fold_c col_w_c R^-1 (fold_c < 2^128, col_w_c < q), the sum times R
+        ModN m;
+        m.set(q);
+        for (uint64_t c = 0; c < ncols; c++) honest = addmod(honest, m.mont(u128_from(&fold_bytes[16 * c]), col_w[c]), q);
+        honest = m.to_mont(honest);
+    }
     u128 claimed = claimed_in ? u128_from(claimed_in) : honest;
     if (claimed_out) u128_put(honest, claimed_out);
     if (!proof_out) return BITZ_OK;  // evaluation only

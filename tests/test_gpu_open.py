@@ -46,7 +46,7 @@ def test_eq_table(n):
     assert np.array_equal(got, O.eq_table(O.ints_to_gf(r)))
 
 
-@pytest.mark.parametrize("t,s", [(7, 0), (9, 3), (15, 11)])
+@pytest.mark.parametrize("t,s", [(7, 0), (9, 3), (12, 8), (15, 11)])
 def test_fold_columns_and_images(t, s):
     rows = rand_rows(t, s, t + s)
     w = rand_gf(1 << t, 5)

@@ -111,6 +111,74 @@ __global__ __launch_bounds__(kThreads) void k_ntt_pass(uint4* __restrict__ data,
     }
 }
 
+// The same pass for blocks whose threads all share blk0 (2^(sh + log_lanes)
+// a multiple of the block size): then every twiddle of the block is one of
+// 2^f constants per layer f, the twiddles of layers l0..l0+F-1 at blocks
+// (blk0 << f) | k, k < 2^f. They become 4-bit window tables in LDS (built by
+// 32 threads each, eight at a time): a butterfly's product is a table
+// product (~125 VALU) instead of a variable one (~430). Layers 0-2 use
+// tables 0..6; for F = 4, layer 3's eight tables replace them afterwards.
+template <int F>
+__global__ __launch_bounds__(kThreads) void k_ntt_pass_tab(uint4* __restrict__ data,
+                                                           const uint4* __restrict__ src,
+                                                           uint64_t src_pos_mask, int log_d,
+                                                           int log_lanes, int l0,
+                                                           const uint4* __restrict__ tw) {
+    __shared__ uint4 tabs[8][512];
+    const uint64_t tid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t lane = tid & ((1ull << log_lanes) - 1);
+    const uint64_t g = tid >> log_lanes;
+    const int sh = log_d - l0 - F;
+    const uint64_t offset = g & ((1ull << sh) - 1);
+    const uint64_t blk0 = g >> sh;  // the same for the whole block
+    const uint64_t base = (blk0 << (log_d - l0)) + offset;
+    const int grp = threadIdx.x >> 5, gt = threadIdx.x & 31;
+    // tables of layers 0..min(F,3)-1: table (1 << f) - 1 + k
+    if (grp < 7) {
+        const int f = grp == 0 ? 0 : (grp < 3 ? 1 : 2);
+        const int k = grp - ((1 << f) - 1);
+        if (f < F) gf_table_build(tabs[grp], twiddle(tw, l0 + f, (blk0 << f) | (uint64_t)k), gt, 32);
+    }
+    gf128 v[1 << F];
+#pragma unroll
+    for (int j = 0; j < (1 << F); j++) {
+        uint64_t p = base + ((uint64_t)j << sh);
+        if (src)
+            v[j] = gf_load(&src[((p & src_pos_mask) << log_lanes) + lane]);
+        else
+            v[j] = gf_load(&data[(p << log_lanes) + lane]);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int f = 0; f < F && f < 3; f++) {
+        const int half = 1 << (F - f - 1);
+#pragma unroll
+        for (int j = 0; j < (1 << F); j++) {
+            if (j & half) continue;
+            const uint4* tab = tabs[(1 << f) - 1 + (j >> (F - f))];
+            gf128 u = gf_add(v[j], gf_mul_tab_lean(tab, v[j | half]));
+            v[j] = u;
+            v[j | half] = gf_add(v[j | half], u);
+        }
+    }
+    if constexpr (F == 4) {
+        __syncthreads();  // every table of layers 0-2 is read
+        gf_table_build(tabs[grp], twiddle(tw, l0 + 3, (blk0 << 3) | (uint64_t)grp), gt, 32);
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < 16; j += 2) {
+            gf128 u = gf_add(v[j], gf_mul_tab_lean(tabs[j >> 1], v[j | 1]));
+            v[j] = u;
+            v[j | 1] = gf_add(v[j | 1], u);
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < (1 << F); j++) {
+        uint64_t p = base + ((uint64_t)j << sh);
+        gf_store(&data[(p << log_lanes) + lane], v[j]);
+    }
+}
+
 // ---- LDS-tiled deepest layers, optional fused leaf hashing + subtree ----
 // Tile: 2^T positions x L lanes, position stride L+1 uint4 in LDS (one pad
 // slot per position spreads the per-leaf reads of the hashing phase across
@@ -383,7 +451,28 @@ static int run_ntt(bitz_ctx* ctx, uint4* data, const uint4* src, uint64_t src_ma
         dim3 grid((unsigned)((threads + kThreads - 1) / kThreads));
         double elems = (double)(1ull << (log_d + log_lanes));
         TimedLaunch tl(ctx, "ntt_pass", 2.0 * 16.0 * elems, F * (elems / 2) * kGfMulOps);
-        switch (F) {
+        // blocks whose threads share their twiddles take them from LDS tables
+        const bool tab = (log_d - l - F) + log_lanes >= 8;
+        if (tab) {
+            switch (F) {
+                case 4:
+                    hipLaunchKernelGGL(k_ntt_pass_tab<4>, grid, dim3(kThreads), 0, ctx->stream, data, src, src_mask,
+                                       log_d, log_lanes, l, tw);
+                    break;
+                case 3:
+                    hipLaunchKernelGGL(k_ntt_pass_tab<3>, grid, dim3(kThreads), 0, ctx->stream, data, src, src_mask,
+                                       log_d, log_lanes, l, tw);
+                    break;
+                case 2:
+                    hipLaunchKernelGGL(k_ntt_pass_tab<2>, grid, dim3(kThreads), 0, ctx->stream, data, src, src_mask,
+                                       log_d, log_lanes, l, tw);
+                    break;
+                default:
+                    hipLaunchKernelGGL(k_ntt_pass_tab<1>, grid, dim3(kThreads), 0, ctx->stream, data, src, src_mask,
+                                       log_d, log_lanes, l, tw);
+                    break;
+            }
+        } else switch (F) {
             case 4:
                 hipLaunchKernelGGL(k_ntt_pass<4>, grid, dim3(kThreads), 0, ctx->stream, data, src,
                                    src_mask, log_d, log_lanes, l, tw);

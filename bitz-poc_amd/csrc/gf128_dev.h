@@ -165,6 +165,18 @@ __device__ __forceinline__ gf128 gf_mul_x(gf128 a) {
     return o;
 }
 
+// a * X^4: shift by a nibble; the spilled nibble t (bits 124..127) returns
+// as t * (X^7 + X^2 + X + 1), at most 11 bits, into the low word.
+__device__ __forceinline__ gf128 gf_mul_x4(gf128 a) {
+    const uint32_t t = a.w[3] >> 28;
+    gf128 o;
+    o.w[3] = (a.w[3] << 4) | (a.w[2] >> 28);
+    o.w[2] = (a.w[2] << 4) | (a.w[1] >> 28);
+    o.w[1] = (a.w[1] << 4) | (a.w[0] >> 28);
+    o.w[0] = (a.w[0] << 4) ^ t ^ (t << 1) ^ (t << 2) ^ (t << 7);
+    return o;
+}
+
 // ---- fixed-multiplier 4-bit window tables (8 KiB each) ----
 // Layout: tab[i*16 + n] = (n * X^(4i)) * c, i in [0,32), n in [0,16).
 // Built cooperatively by `nthreads` threads of a block; caller syncs.
@@ -173,7 +185,7 @@ __device__ __forceinline__ void gf_table_build(uint4* tab, gf128 c, int tid, int
     // whole windows (32 windows; threads beyond 32 idle here).
     for (int i = tid; i < 32; i += nthreads) {
         gf128 b0 = c;
-        for (int k = 0; k < 4 * i; k++) b0 = gf_mul_x(b0);
+        for (int k = 0; k < i; k++) b0 = gf_mul_x4(b0);
         gf128 b1 = gf_mul_x(b0), b2 = gf_mul_x(b1), b3 = gf_mul_x(b2);
         for (int n = 0; n < 16; n++) {
             gf128 acc = gf_zero();
@@ -184,6 +196,30 @@ __device__ __forceinline__ void gf_table_build(uint4* tab, gf128 c, int tid, int
             gf_store(&tab[i * 16 + n], acc);
         }
     }
+}
+
+// gf_mul_tab one input word at a time (the word loop kept rolled): at most
+// eight lookups in flight, for kernels that run many table products on
+// register-resident data, where hoisting every lookup would spill.
+__device__ __forceinline__ gf128 gf_mul_tab_lean(const uint4* tab, gf128 v) {
+    uint32_t x0 = 0, x1 = 0, x2 = 0, x3 = 0;
+#pragma unroll 1
+    for (int word = 0; word < 4; word++) {
+        // a uniform select, not an indexed register array (that would move
+        // v to memory)
+        const uint32_t w = word == 0 ? v.w[0] : word == 1 ? v.w[1] : word == 2 ? v.w[2] : v.w[3];
+        const uint4* tw = tab + word * 128;
+#pragma unroll
+        for (int nib = 0; nib < 8; nib += 2) {
+            uint4 t = tw[nib * 16 + ((w >> (4 * nib)) & 15u)];
+            uint4 u = tw[(nib + 1) * 16 + ((w >> (4 * nib + 4)) & 15u)];
+            x0 = __builtin_amdgcn_bitop3_b32(x0, t.x, u.x, 0x96);
+            x1 = __builtin_amdgcn_bitop3_b32(x1, t.y, u.y, 0x96);
+            x2 = __builtin_amdgcn_bitop3_b32(x2, t.z, u.z, 0x96);
+            x3 = __builtin_amdgcn_bitop3_b32(x3, t.w, u.w, 0x96);
+        }
+    }
+    return gf128{{x0, x1, x2, x3}};
 }
 
 // Two windows per step, so each accumulator word takes one 3-input XOR

@@ -55,6 +55,7 @@ def parse():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--log-coeffs", type=int, default=24, help="log2 of the u64 coefficient count")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the library's default launch shapes")
     ap.add_argument("--cpu-log-coeffs", type=int, default=18)
     ap.add_argument("--no-batched", action="store_true", help="skip the batched and serving measurements")
     ap.add_argument("--device-transcript", action="store_true",
@@ -200,7 +201,7 @@ def main():
         sys.exit(2)
     import torch
 
-    from bitz_amd.device import Commitment, Context, kernel_stats, set_timing, set_timing_only
+    from bitz_amd.device import Commitment, Context, autotune, get_tune, kernel_stats, set_timing, set_timing_only
 
     n, t, s = shape_for(args.log_coeffs)
     ctx = Context(0)
@@ -223,6 +224,8 @@ def main():
     # claim setup (untimed): the honest value of the transcript-derived claim
     com0, claimed = step()
     com0.close()
+    # launch shapes and thresholds for this shape, timed on the device (untimed)
+    tune = autotune(ctx, t, s, 4, 2) if not args.no_autotune else get_tune(ctx)
     for _ in range(args.warmup):
         step(claimed)
     torch.cuda.synchronize()
@@ -314,6 +317,7 @@ def main():
         "kernels": {k: {"launches": int(v[0]), "ms": round(v[1], 3)} for k, v in
                     sorted(profile.items(), key=lambda kv: -kv[1][1])},
         "host_phases_ms": phases,
+        "tune": tune,
     }
     print(json.dumps(out), flush=True)
 

@@ -47,6 +47,9 @@ SIGNATURES = {
     "bitz_ctx_set_timing": (_I, [_P, _I]),
     "bitz_ctx_set_timing_only": (_I, [_P, C.c_char_p]),
     "bitz_ctx_kernel_stats": (_I, [_P, _P, _U64, _P, _U64]),
+    "bitz_ctx_set_tune": (_I, [_P, _U32, _U32, _U32, _U32, _U32]),
+    "bitz_ctx_get_tune": (_I, [_P, _P]),
+    "bitz_ctx_autotune": (_I, [_P, _U32, _U32, _U32, _U32, _P]),
     "bitz_gf_mul_dev": (_I, [_P, _P, _P, _P, _U64]),
     "bitz_gf_mul_const_dev": (_I, [_P, _P, _P, _P, _U64]),
     "bitz_blake3_leaves_dev": (_I, [_P, _P, _U64, _U64, _P]),

@@ -169,6 +169,38 @@ def set_timing_only(ctx: Context, kernel: str) -> None:
     _lib.call("bitz_ctx_set_timing_only", ctx.ptr, kernel.encode())
 
 
+TUNE_KEYS = ("round_r_max", "round_min_chunks", "round_block_cap", "host_tail_log", "ntt_radix_log")
+
+
+def get_tune(ctx: Context) -> dict:
+    """The context's launch-shape choices, plus `tuned` (set by autotune)."""
+    out = (C.c_uint32 * 6)()
+    _lib.call("bitz_ctx_get_tune", ctx.ptr, C.cast(out, C.c_void_p))
+    d = {k: int(out[i]) for i, k in enumerate(TUNE_KEYS)}
+    d["tuned"] = bool(out[5])
+    return d
+
+
+def set_tune(ctx: Context, **kw) -> None:
+    """Override some of the launch-shape choices (the rest keep their values)."""
+    cur = get_tune(ctx)
+    unknown = set(kw) - set(TUNE_KEYS)
+    if unknown:
+        raise ValueError(f"unknown tune keys {sorted(unknown)}")
+    cur.update(kw)
+    _lib.call("bitz_ctx_set_tune", ctx.ptr, *(int(cur[k]) for k in TUNE_KEYS))
+
+
+def autotune(ctx: Context, t: int, s: int, log_batch: int, log_inv_rate: int) -> dict:
+    """Time candidate launch shapes for jobs of this shape on the device and
+    keep the fastest in the context; returns get_tune() plus `tune_ms`."""
+    ms = C.c_double(0.0)
+    _lib.call("bitz_ctx_autotune", ctx.ptr, t, s, log_batch, log_inv_rate, C.byref(ms))
+    d = get_tune(ctx)
+    d["tune_ms"] = ms.value
+    return d
+
+
 class OpenOps:
     """Thin wrappers over the opening kernels (device tensors in, device
     tensors or host ints out)."""

@@ -446,7 +446,7 @@ static int run_ntt(bitz_ctx* ctx, uint4* data, const uint4* src, uint64_t src_ma
     int local_first = std::max(start, log_d - T);
     int l = start;
     while (l < log_d - T) {
-        int F = std::min(4, log_d - T - l);
+        int F = std::min(std::max(1, std::min(4, ctx->tune.ntt_radix_log)), log_d - T - l);
         uint64_t threads = 1ull << (log_d - F + log_lanes);
         dim3 grid((unsigned)((threads + kThreads - 1) / kThreads));
         double elems = (double)(1ull << (log_d + log_lanes));

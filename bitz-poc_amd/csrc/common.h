@@ -161,6 +161,17 @@ struct KernelStat {
     uint64_t launches = 0;
     double ms = 0, bytes = 0, ops = 0;
 };
+// Launch-shape choices that do not change any result: the defaults are the
+// hand-set heuristics, bitz_ctx_autotune replaces them by timed choices.
+struct Tune {
+    uint32_t round_r_max = 32;        // slots per thread in round-sum kernels
+    uint32_t round_min_chunks = 2048; // chunks below which R halves
+    uint32_t round_block_cap = 4096;  // blocks per round-sum launch
+    int host_tail_log = 10;           // sumcheck rounds of at most 2^x slots run on the host
+    int ntt_radix_log = 4;            // butterfly layers per global NTT pass
+    bool tuned = false;
+};
+
 struct PendingTiming {
     std::string name;
     hipEvent_t start, stop;
@@ -193,6 +204,7 @@ struct bitz_ctx {
     // (off by default: on MI355X one wave hashes a round's ~8 BLAKE3
     // compressions in ~20 us, no faster than the host round trip)
     bool device_fs = false;
+    bitz::Tune tune;  // launch shapes and thresholds (defaults or bitz_ctx_autotune)
 };
 
 // A committed matrix kept on the device: the bit rows (= packed message

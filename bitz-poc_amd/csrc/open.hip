@@ -319,16 +319,16 @@ __global__ __launch_bounds__(kT) void k_gkr_round_sums(Src src, uint64_t h, cons
 
 // Slots per thread for a round of h slots with eq_c rows of 2^s_c: up to 32,
 // the chunk kT * R inside one row, and enough chunks to fill the chip.
-static uint32_t round_chunk_r(uint64_t h, int s_c) {
+static uint32_t round_chunk_r(const Tune& tn, uint64_t h, int s_c) {
     uint64_t row = 1ull << s_c;
     if (row < (uint64_t)kT) return 1;
-    uint64_t r = std::min<uint64_t>(32, row / kT);
-    while (r > 1 && h / ((uint64_t)kT * r) < 2048) r >>= 1;
-    return (uint32_t)r;
+    uint64_t r = std::min<uint64_t>(tn.round_r_max, row / kT);
+    while (r > 1 && h / ((uint64_t)kT * r) < tn.round_min_chunks) r >>= 1;
+    return (uint32_t)std::max<uint64_t>(r, 1);
 }
-static unsigned round_blocks(uint64_t h, uint32_t R) {
+static unsigned round_blocks(const Tune& tn, uint64_t h, uint32_t R) {
     uint64_t nch = (h + (uint64_t)kT * R - 1) / ((uint64_t)kT * R);
-    return (unsigned)std::max<uint64_t>(1, std::min<uint64_t>(nch, 4096));
+    return (unsigned)std::max<uint64_t>(1, std::min<uint64_t>(nch, tn.round_block_cap));
 }
 
 // materialised halves l, r (each 2h entries)
@@ -1346,8 +1346,8 @@ int pack_columns(bitz_ctx* ctx, const uint64_t* rows, int t, int s, uint64_t* pa
 
 int gkr_round_sums(bitz_ctx* ctx, const void* l, const void* r, uint64_t h, const void* eq_c, int s_c,
                    const void* eq_y, int send_one, void* out_dev) {
-    uint32_t R = round_chunk_r(h, s_c);
-    unsigned nb = round_blocks(h, R);
+    uint32_t R = round_chunk_r(ctx->tune, h, s_c);
+    unsigned nb = round_blocks(ctx->tune, h, R);
     int rc = ctx->scratch.ensure((size_t)nb * 2 * 16);
     if (rc) return rc;
     TimedLaunch tl(ctx, "gkr_round_sums", 4.0 * 16.0 * h, 4.0 * h * kGfMulOps);
@@ -1372,8 +1372,8 @@ static uint64_t pattern_ylanes(int s, uint64_t ny) {
 
 int gkr_premul_round_sums(bitz_ctx* ctx, const void* l, void* r, uint64_t h, const void* eq_c, int s,
                           const void* eq_y, int send_one, void* out_dev) {
-    uint32_t R = round_chunk_r(h, s);
-    unsigned nb = round_blocks(h, R);
+    uint32_t R = round_chunk_r(ctx->tune, h, s);
+    unsigned nb = round_blocks(ctx->tune, h, R);
     int rc = ctx->scratch.ensure((size_t)nb * 2 * 16);
     if (rc) return rc;
     TimedLaunch tl(ctx, "gkr_premul_round_sums", 6.0 * 16.0 * h, 4.0 * h * kGfMulOps);
@@ -1386,8 +1386,8 @@ int gkr_premul_round_sums(bitz_ctx* ctx, const void* l, void* r, uint64_t h, con
 
 int gkr_round_sums_rowonly(bitz_ctx* ctx, const void* l, const void* r, uint64_t h, int s, const void* eq_y,
                            int send_one, void* out_dev) {
-    uint32_t R = round_chunk_r(h, s);
-    unsigned nb = round_blocks(h, R);
+    uint32_t R = round_chunk_r(ctx->tune, h, s);
+    unsigned nb = round_blocks(ctx->tune, h, R);
     int rc = ctx->scratch.ensure((size_t)nb * 2 * 16);
     if (rc) return rc;
     TimedLaunch tl(ctx, "gkr_round_sums_rowonly", 4.0 * 16.0 * h, 2.0 * h * kGfMulOps);
@@ -1443,8 +1443,8 @@ int tensor_table(bitz_ctx* ctx, const void* a, const void* b, int n, void* out) 
 
 int gkr_fold_round_sums(bitz_ctx* ctx, void* l, void* r, uint64_t h, const uint8_t rho[16], const void* eq_c,
                         int s_c, const void* eq_y, int send_one, void* out_dev, const void* rho_dev) {
-    uint32_t R = round_chunk_r(h, s_c);
-    unsigned nb = round_blocks(h, R);
+    uint32_t R = round_chunk_r(ctx->tune, h, s_c);
+    unsigned nb = round_blocks(ctx->tune, h, R);
     int rc = ctx->scratch.ensure((size_t)nb * 2 * 16);
     if (rc) return rc;
     uint64_t lo, hi;
@@ -1485,8 +1485,8 @@ int gkr_leaf_round_sums(bitz_ctx* ctx, const uint64_t* packed, const void* A, in
         BITZ_LAUNCH_CHECK();
         return sums_finish(ctx, nb, 2, out_dev);
     }
-    uint32_t R = round_chunk_r(h, s_c);
-    unsigned nb = round_blocks(h, R);
+    uint32_t R = round_chunk_r(ctx->tune, h, s_c);
+    unsigned nb = round_blocks(ctx->tune, h, R);
     int rc = ctx->scratch.ensure((size_t)nb * 2 * 16);
     if (rc) return rc;
     TimedLaunch tl(ctx, "gkr_leaf_round_sums", (double)(1ull << (t + s - 3)), 4.0 * h * kGfMulOps);

@@ -39,11 +39,10 @@ inline void gput(G g, uint8_t* p) {
 // eq_factor (src/bitz/mod.rs:328-331): r z + (1 - r)(1 - z)
 inline G eq_factor(G r, G z) { return hgf_add(hgf_mul(r, z), hgf_mul(hgf_add(r, gone()), hgf_add(z, gone()))); }
 
-// A GKR layer's rounds with at most 2^kHostTailLog slots run on the host
-// (host_tail): one such round is a few microseconds of carry-less multiplies
-// there, against a kernel launch, a reduction and a device-to-host round trip
-// (tens of microseconds) on the device.
-constexpr int kHostTailLog = 10;
+// A GKR layer's rounds with at most 2^host_tail_log slots (ctx->tune, 10 by
+// default) run on the host (host_tail): one such round is a few microseconds
+// of carry-less multiplies there, against a kernel launch, a reduction and a
+// device-to-host round trip (tens of microseconds) on the device.
 
 // out[i] = v[i]^-1 with one inversion (prefix products; Montgomery's
 // trick); a zero entry maps to zero, as hgf_inv does
@@ -517,7 +516,7 @@ struct Prover {
             uint64_t hh = 1ull << rem;
             int s_c = std::min(s, rem);
             const bool bit_round = (leaf && round <= 1) || (kind == kLevel1 && round == 0);
-            if (!bit_round && rem <= kHostTailLog && (!weighted || rem < s)) {
+            if (!bit_round && rem <= ctx->tune.host_tail_log && (!weighted || rem < s)) {
                 // the remaining rounds on the host, from dense halves
                 if (pending_fold && (rc = fold_pair(ctx, l, r, 2 * hh, pending_rho))) return rc;
                 if (weighted && (rc = mul_inplace(ctx, r, eqcinv.ptr, 1ull << s))) return rc;
@@ -612,7 +611,7 @@ struct Prover {
     }
 
     // Rounds round0..L-1 of a GKR layer on the host once each half holds at
-    // most 2^(kHostTailLog+1) entries: the halves come down once, then every
+    // most 2^(host_tail_log+1) entries: the halves come down once, then every
     // round is the same sums / message / challenge / fold sequence as the
     // device rounds (gkr_layer), with the round's own eq table over
     // ext[0..rem) as slot weights (so no corr[] constant; Sums::slot,
@@ -832,7 +831,7 @@ struct Prover {
         hostgf::E* hw = nullptr;
         while (n > 1) {
             G v[3];
-            if (!ha && n <= (2ull << kHostTailLog)) {
+            if (!ha && n <= (2ull << ctx->tune.host_tail_log)) {
                 // the remaining rounds on the host (as host_tail for GKR)
                 if ((rc = ctx->scratch.ensure_stage(2 * n * sizeof(hostgf::E)))) return rc;
                 ha = (hostgf::E*)ctx->scratch.stage;

@@ -83,6 +83,25 @@ int bitz_ctx_set_timing(bitz_ctx* ctx, int enable);
  * measurement aid of this library, with no counterpart in the reference. */
 int bitz_ctx_set_timing_only(bitz_ctx* ctx, const char* kernel);
 int bitz_ctx_kernel_stats(bitz_ctx* ctx, char* names, uint64_t names_cap, double* values, uint64_t values_cap);
+/* Launch-shape choices of the context (none changes a result; the reference
+ * has no counterpart, its CPU loops have no launch shapes): slots per thread
+ * of the round-sum kernels (round_r_max, a power of two <= 32), the chunk
+ * count below which that halves (round_min_chunks), blocks per round-sum
+ * launch (round_block_cap), the largest sumcheck round size 2^host_tail_log
+ * run on the host (4..14) and butterfly layers per global NTT pass
+ * (ntt_radix_log, 1..4). set_tune validates and overrides; get_tune writes
+ * the five values and 1 if they came from bitz_ctx_autotune. */
+int bitz_ctx_set_tune(bitz_ctx* ctx, uint32_t round_r_max, uint32_t round_min_chunks, uint32_t round_block_cap,
+                      uint32_t host_tail_log, uint32_t ntt_radix_log);
+int bitz_ctx_get_tune(bitz_ctx* ctx, uint32_t out[6]);
+/* Chooses the five values above for jobs of shape (t, s, log_batch,
+ * log_inv_rate) by timing candidates on the device over synthetic data of
+ * that shape (round kernels at the layer's round sizes, host against device
+ * rounds, the commit at both NTT radices); tune_ms (optional) gets the
+ * wall-clock time it took. Synchronizing; allocates about the job's
+ * footprint while it runs. */
+int bitz_ctx_autotune(bitz_ctx* ctx, uint32_t t, uint32_t s, uint32_t log_batch, uint32_t log_inv_rate,
+                      double* tune_ms);
 /* Library build tag (architecture and version). */
 const char* bitz_version(void);
 

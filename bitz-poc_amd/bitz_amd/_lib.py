@@ -50,6 +50,8 @@ SIGNATURES = {
     "bitz_ctx_set_tune": (_I, [_P, _U32, _U32, _U32, _U32, _U32]),
     "bitz_ctx_get_tune": (_I, [_P, _P]),
     "bitz_ctx_autotune": (_I, [_P, _U32, _U32, _U32, _U32, _P]),
+    "bitz_mul_layout": (_I, [_U32, _U64, C.c_int32, _P]),
+    "bitz_mul_bit_rows_dev": (_I, [_P, _U32, _U64, C.c_int32, _P, _P, _P, _P, _P, _U64]),
     "bitz_gf_mul_dev": (_I, [_P, _P, _P, _P, _U64]),
     "bitz_gf_mul_const_dev": (_I, [_P, _P, _P, _P, _U64]),
     "bitz_blake3_leaves_dev": (_I, [_P, _P, _U64, _U64, _P]),

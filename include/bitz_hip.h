@@ -314,6 +314,27 @@ void bitz_server_destroy(bitz_server* srv);
 /* submits n jobs (largest first) to a fresh server and waits for all */
 int bitz_batch_run(int device, uint32_t workers, uint64_t hbm_budget, bitz_job* jobs, uint32_t n);
 
+/* ---- committed bit grid of the native multiplication relations ---- */
+/* MulLayout::new + with_split_shift + bitz_params (src/piop/spartan/mul.rs:71-139)
+ * for word_bits 32, 64 or 128: out = {gate_vars g (capacity 2^g = max(n, 256)
+ * rounded up to a power of two), col_vars s, row_vars t = g - s + log2(4 word_bits),
+ * committed bits n = t + s}. Errors: empty batch, domain too large, shift out of
+ * range (MulError, mul.rs:48-59) -> BITZ_ERR_SHAPE. */
+int bitz_mul_layout(uint32_t word_bits, uint64_t multiplications, int32_t split_shift, uint32_t out[4]);
+/* MulWitness::bitz_bit_rows / write_bitz_bit_rows (mul.rs:268-330): the 2^s
+ * columns of 2^t bits (2^(t-6) u64 words each, column c at rows + c 2^(t-6),
+ * the layout bitz_commit takes) of the limbs [x | y | lo | hi] of each gate:
+ * bit b of limb l at row bit ((l word_bits + b) << (g - s)) | (gate >> s) of
+ * column gate & (2^s - 1) (bitz_cell, mul.rs:140-150); padding gates are
+ * zero and every word is written. x, y (and lo, hi when given: supplied
+ * product limbs, MulWitness::from_rows) are device arrays of n values of
+ * word_bits bits (little-endian); with lo = hi = NULL the limbs are the
+ * 2 word_bits-bit product x*y (MulRow::new). rows_words must be 2^(t+s-6)
+ * (InvalidRowShape otherwise). Asynchronous on the context's stream. */
+int bitz_mul_bit_rows_dev(bitz_ctx* ctx, uint32_t word_bits, uint64_t multiplications, int32_t split_shift,
+                          const void* x, const void* y, const void* lo, const void* hi, void* rows,
+                          uint64_t rows_words);
+
 /* The device transcript (Blake3Transcript, src/transcript/mod.rs:56-96) checked
  * against the host's: a host transcript absorbs `prefix` (absorb_slice), its
  * state goes to the device, which then runs n x (absorb_slice(msg_i of

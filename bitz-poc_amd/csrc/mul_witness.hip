@@ -20,6 +20,7 @@
 // lines with its neighbours'. With fewer high gates (a shifted split on a
 // small batch) each thread gathers the 64 bits of one output word.
 #include "../../include/bitz_hip.h"
+#include "bits_dev.h"
 #include "common.h"
 
 namespace bitz {
@@ -122,21 +123,6 @@ struct Limbs<128> {
         }
     }
 };
-
-// 64 x 64 bit transpose across a wavefront: lane g holds row g on entry,
-// lane k holds sum_g bit_k(row g) << g on exit. Column k of the block is the
-// wave's ballot of bit k (a compare into a scalar mask), selected into lane
-// k: scalar and VALU work only, no cross-lane data movement through the LDS
-// pipe.
-__device__ __forceinline__ uint64_t wave_transpose64(uint64_t v, int lane) {
-    uint64_t o = 0;
-#pragma unroll
-    for (int k = 0; k < 64; k++) {
-        const uint64_t m = __ballot((v >> k) & 1ull);
-        o = lane == k ? m : o;
-    }
-    return o;
-}
 
 template <int B>
 __global__ __launch_bounds__(256) void k_mul_rows_transpose(const void* __restrict__ x, const void* __restrict__ y,

@@ -18,6 +18,7 @@
 #include "reduce_dev.h"
 #include "open_impl.h"
 #include "transcript_dev.h"
+#include "bits_dev.h"
 
 namespace bitz {
 namespace {
@@ -247,21 +248,21 @@ __global__ void k_pow_comb(const uint4* __restrict__ comb, const uint4* __restri
 
 // -------------------------------------------------- bit layout transposes
 // packed[g * 2^t + b] = bit b of columns 64g..64g+63, from column-major rows.
-__global__ void k_pack_columns(const uint64_t* __restrict__ rows, int t, int s,
-                               uint64_t* __restrict__ packed) {
-    const uint64_t nrows = 1ull << t, words = nrows >> 6, cols = 1ull << s;
-    const uint64_t groups = (cols + 63) / 64;
-    uint64_t idx = grid_tid();
-    if (idx >= groups * nrows) return;
-    uint64_t g = idx / nrows, b = idx % nrows;
-    uint64_t out = 0;
-    for (int j = 0; j < 64; j++) {
-        uint64_t c = g * 64 + j;
-        if (c >= cols) break;
-        uint64_t bit = (rows[c * words + (b >> 6)] >> (b & 63)) & 1ull;
-        out |= bit << j;
-    }
-    packed[idx] = out;
+// One wavefront per (group g, row word w): lane j loads word w of column
+// 64g + j, the wave transposes the 64 x 64 block and lane k stores the bits
+// of row 64w + k (a coalesced 512-byte store). Consecutive waves take
+// consecutive w, so the column loads of 16 waves share each cache line.
+__global__ __launch_bounds__(256) void k_pack_columns(const uint64_t* __restrict__ rows, int t, int s,
+                                                      uint64_t* __restrict__ packed) {
+    const uint64_t words = 1ull << (t - 6), cols = 1ull << s;
+    const uint64_t groups = (cols + 63) >> 6;
+    const uint64_t wid = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int lane = threadIdx.x & 63;
+    if (wid >= groups * words) return;  // whole waves (blockDim is a multiple of 64)
+    const uint64_t g = wid >> (t - 6), w = wid & (words - 1);
+    const uint64_t c = (g << 6) + (uint64_t)lane;
+    const uint64_t v = c < cols ? rows[c * words + w] : 0;
+    packed[(g << t) + (w << 6) + (uint64_t)lane] = wave_transpose64(v, lane);
 }
 
 // ------------------------------------------------------------- GKR rounds
@@ -665,6 +666,43 @@ __global__ __launch_bounds__(kT) void k_level2_pattern(const uint64_t* __restric
         uint32_t p = (uint32_t)(((col[y] >> bit) & 1ull) | (((col[y + Y1] >> bit) & 1ull) << 1) |
                                 (((col[y + Y2] >> bit) & 1ull) << 2) | (((col[y + Y2 + Y1] >> bit) & 1ull) << 3));
         out[x] = ltab[y * 16 + p];
+    }
+}
+
+// The same two levels with the row's table staged in LDS, for rows of at
+// least kT entries (s >= 8): block b takes 2^cs consecutive entries of row
+// y = b >> (s - cs), so the 16 (level 2) or 256 (level 3) table entries are
+// read from HBM once per block and every entry is an LDS read instead of a
+// 16-byte gather.
+template <int K>
+__global__ __launch_bounds__(kT) void k_level_pattern_lds(const uint64_t* __restrict__ packed, int t, int s, int cs,
+                                                          const uint4* __restrict__ tab_g, uint4* __restrict__ out) {
+    __shared__ uint4 tab[1 << K];
+    const uint64_t nrows = 1ull << t;
+    const uint64_t y = (uint64_t)blockIdx.x >> (s - cs);
+    const uint64_t c0 = ((uint64_t)blockIdx.x & ((1ull << (s - cs)) - 1)) << cs;
+    for (int i = threadIdx.x; i < (1 << K); i += kT) tab[i] = tab_g[(y << K) + i];
+    __syncthreads();
+    uint64_t off[K];
+    if constexpr (K == 4) {
+        const uint64_t Y1 = nrows >> 1, Y2 = nrows >> 2;
+        off[0] = y;
+        off[1] = y + Y1;
+        off[2] = y + Y2;
+        off[3] = y + Y2 + Y1;
+    } else {
+        const uint64_t Y3 = nrows >> 3;
+#pragma unroll
+        for (int k = 0; k < K; k++) off[k] = y + k * Y3;
+    }
+    for (uint64_t i = threadIdx.x; i < (1ull << cs); i += kT) {
+        const uint64_t c = c0 + i;
+        const uint64_t* col = packed + (c >> 6) * nrows;
+        const int bit = (int)(c & 63);
+        uint32_t p = 0;
+#pragma unroll
+        for (int k = 0; k < K; k++) p |= (uint32_t)((col[off[k]] >> bit) & 1ull) << k;
+        out[(y << s) + c] = tab[p];
     }
 }
 
@@ -1336,8 +1374,10 @@ int pow_images(bitz_ctx* ctx, const uint8_t g[16], const void* exps, uint64_t n,
 }
 
 int pack_columns(bitz_ctx* ctx, const uint64_t* rows, int t, int s, uint64_t* packed) {
-    uint64_t groups = ((1ull << s) + 63) / 64, n = groups << t;
-    hipLaunchKernelGGL(k_pack_columns, dim3((unsigned)((n + kT - 1) / kT)), dim3(kT), 0, ctx->stream, rows, t, s,
+    BITZ_REQUIRE(t >= 6, BITZ_ERR_SHAPE, "pack_columns: t < 6");
+    const uint64_t groups = ((1ull << s) + 63) / 64, n = (groups << (t - 6)) * 64;  // one wave per word
+    TimedLaunch tl(ctx, "pack_columns", 2.0 * 8.0 * (double)(groups << t), 0.0);
+    hipLaunchKernelGGL(k_pack_columns, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream, rows, t, s,
                        packed);
     BITZ_LAUNCH_CHECK();
     return BITZ_OK;
@@ -1638,8 +1678,14 @@ int gkr_level2(bitz_ctx* ctx, const uint64_t* packed, const void* A, int t, int 
                        (const uint4*)A, t, vtab);
     hipLaunchKernelGGL(k_level2_ltab, dim3((unsigned)((Y2 * 16 + kT - 1) / kT)), dim3(kT), 0, ctx->stream,
                        (const uint4*)vtab, t, ltab);
-    hipLaunchKernelGGL(k_level2_pattern, dim3(blocks_for(n, 4)), dim3(kT), 0, ctx->stream, packed, t, s,
-                       (const uint4*)ltab, (uint4*)out);
+    if (s >= 8) {
+        const int cs = std::min(s, 12);
+        hipLaunchKernelGGL(k_level_pattern_lds<4>, dim3((unsigned)(Y2 << (s - cs))), dim3(kT), 0, ctx->stream,
+                           packed, t, s, cs, (const uint4*)ltab, (uint4*)out);
+    } else {
+        hipLaunchKernelGGL(k_level2_pattern, dim3(blocks_for(n, 4)), dim3(kT), 0, ctx->stream, packed, t, s,
+                           (const uint4*)ltab, (uint4*)out);
+    }
     BITZ_LAUNCH_CHECK();
     return BITZ_OK;
 }
@@ -1658,8 +1704,14 @@ int gkr_level3(bitz_ctx* ctx, const uint64_t* packed, const void* A, int t, int 
                        (const uint4*)vtab, t, ltab);
     hipLaunchKernelGGL(k_level3_tab, dim3((unsigned)((Y3 * 256 + kT - 1) / kT)), dim3(kT), 0, ctx->stream,
                        (const uint4*)ltab, t, l3tab);
-    hipLaunchKernelGGL(k_level3_pattern, dim3(blocks_for(n, 4)), dim3(kT), 0, ctx->stream, packed, t, s,
-                       (const uint4*)l3tab, (uint4*)out);
+    if (s >= 8) {
+        const int cs = std::min(s, 12);
+        hipLaunchKernelGGL(k_level_pattern_lds<8>, dim3((unsigned)(Y3 << (s - cs))), dim3(kT), 0, ctx->stream,
+                           packed, t, s, cs, (const uint4*)l3tab, (uint4*)out);
+    } else {
+        hipLaunchKernelGGL(k_level3_pattern, dim3(blocks_for(n, 4)), dim3(kT), 0, ctx->stream, packed, t, s,
+                           (const uint4*)l3tab, (uint4*)out);
+    }
     BITZ_LAUNCH_CHECK();
     return BITZ_OK;
 }

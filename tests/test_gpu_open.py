@@ -65,6 +65,14 @@ def test_fold_columns_and_images(t, s):
 def test_pack_and_sumcheck_tables(t, s):
     rows = rand_rows(t, s, 3)
     packed = ops().pack_columns(dev_u64(rows), t, s)
+    # pack_columns_from_rows (src/ligerito.rs:655-690): word g * 2^t + b holds
+    # bit b of columns 64g..64g+63, column 64g+j at bit j
+    cols, groups = 1 << s, ((1 << s) + 63) // 64
+    bits = np.unpackbits(rows.reshape(cols, -1).view(np.uint8), axis=1, bitorder="little")
+    bits = np.concatenate([bits, np.zeros((groups * 64 - cols, 1 << t), np.uint8)])
+    want = np.packbits(bits.reshape(groups, 64, 1 << t).transpose(0, 2, 1), axis=2, bitorder="little")
+    want = np.ascontiguousarray(want).view(np.uint64).reshape(-1)
+    assert np.array_equal(packed.cpu().numpy().view(np.uint64).reshape(-1), want)
     w2 = rand_gf(1 << s, 1)
     eq = rand_gf(1 << t, 2)
     log(f"xi/fold_rows t={t} s={s}")
@@ -127,7 +135,7 @@ def test_gkr_fold_round_sums(log_h, s_c, send_one):
     assert tuple(got) == O.gkr_round_sums(fl, fr, full_eq, send_one)
 
 
-@pytest.mark.parametrize("t,s", [(7, 0), (7, 3), (9, 6), (10, 8), (15, 11)])
+@pytest.mark.parametrize("t,s", [(7, 0), (7, 3), (9, 6), (10, 8), (15, 11), (13, 14)])
 def test_gkr_leaf_layer(t, s):
     rows = rand_rows(t, s, 11)
     A = rand_gf(1 << t, 12)
@@ -170,7 +178,7 @@ def test_gkr_leaf_layer(t, s):
                 assert O.gf_to_ints(grh[x:x + 1])[0] == o
 
 
-@pytest.mark.parametrize("t,s", [(7, 0), (7, 3), (9, 6), (10, 8), (15, 11)])
+@pytest.mark.parametrize("t,s", [(7, 0), (7, 3), (9, 6), (10, 8), (15, 11), (13, 14)])
 def test_gkr_leaf_round1(t, s):
     """round 1 of the leaf layer and the double fold, from bit patterns: against
     the oracle's dense fold + round sums on small shapes, and against the
@@ -264,7 +272,7 @@ def test_gkr_leaf_fold2_carrying_weights(t, s):
     assert np.array_equal(to_host(wr), O.mul_arrays(to_host(pr), np.tile(eq_c, (1 << (t - 3), 1))))
 
 
-@pytest.mark.parametrize("t,s", [(7, 0), (7, 3), (9, 6), (10, 8), (15, 11)])
+@pytest.mark.parametrize("t,s", [(7, 0), (7, 3), (9, 6), (10, 8), (15, 11), (13, 14)])
 def test_gkr_level1_layer_from_bits(t, s):
     """level 2 from the bits, and the layer over the implicit level 1 (round-0
     sums and fold, with and without carried column weights): against the

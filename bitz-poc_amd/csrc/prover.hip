@@ -197,51 +197,77 @@ inline void u128_put(u128 v, uint8_t* p) {
     memcpy(p + 8, &hi, 8);
 }
 
-// eq_table_mod_q (src/ligerito_flock.rs:1330-1347): each coordinate is
-// appended as the new lowest index bit.
-std::vector<u128> eq_table_mod_q_serial(const std::vector<u128>& r, u128 q) {
-    std::vector<u128> t = {1 % q};
-    for (u128 c : r) {
-        std::vector<u128> nx;
-        nx.reserve(t.size() * 2);
-        for (u128 v : t) {
-            u128 v1 = mulmod(v, c, q);
-            u128 v0 = v >= v1 ? v - v1 : v + q - v1;
-            nx.push_back(v0);
-            nx.push_back(v1);
-        }
-        t.swap(nx);
-    }
-    return t;
+// ---- eq tables mod q on the device ----
+// eq_table_mod_q (src/ligerito_flock.rs:1330-1347), entry by entry: x's
+// Montgomery product of the per-coordinate factors (1 - r_i or r_i,
+// coordinate 0 on the top index bit), passed in Montgomery form as kernel
+// arguments, then one product by 1 to leave the Montgomery domain.
+// Montgomery arithmetic with R = 2^128 as ModN::mont (a b R^-1 mod q for
+// a b < R q), in 128-bit integer arithmetic.
+constexpr int kMaxModQN = 32;
+struct ModQArgs {
+    uint64_t f[2][kMaxModQN][2];  // [bit][coordinate] = {lo, hi} of the factor x R mod q
+};
+
+__device__ __forceinline__ void wide_dev(u128 a, u128 b, u128* hi, u128* lo) {
+    const uint64_t a0 = (uint64_t)a, a1 = (uint64_t)(a >> 64), b0 = (uint64_t)b, b1 = (uint64_t)(b >> 64);
+    const u128 p00 = (u128)a0 * b0, p01 = (u128)a0 * b1, p10 = (u128)a1 * b0, p11 = (u128)a1 * b1;
+    const u128 mid = (p00 >> 64) + (uint64_t)p01 + (uint64_t)p10;
+    *lo = (uint64_t)p00 | (mid << 64);
+    *hi = p11 + (p01 >> 64) + (p10 >> 64) + (mid >> 64);
 }
 
-// The same table as the tensor product of the tables of the first and the
-// last coordinates (entry x = first[x >> k] * last[x & (2^k - 1)]), the
-// products spread over host threads.
-std::vector<u128> eq_table_mod_q(const std::vector<u128>& r, u128 q) {
-    const size_t n = r.size();
-    if (n < 12) return eq_table_mod_q_serial(r, q);
-    const size_t k = n / 2;
-    std::vector<u128> hi = eq_table_mod_q_serial(std::vector<u128>(r.begin(), r.end() - k), q);
-    std::vector<u128> lo = eq_table_mod_q_serial(std::vector<u128>(r.end() - k, r.end()), q);
-    // hi in Montgomery form: each entry is then one product and one reduction
+__device__ __forceinline__ u128 mont_dev(u128 a, u128 b, u128 q, u128 qinv) {
+    u128 hi, lo, mh, ml;
+    wide_dev(a, b, &hi, &lo);
+    const u128 m = lo * qinv;
+    wide_dev(m, q, &mh, &ml);
+    const u128 c = (lo + ml) < lo ? 1 : 0;
+    const u128 r = hi + mh + c;
+    return r >= q ? r - q : r;
+}
+
+__global__ void k_eq_mod_q(ModQArgs a, int n, uint64_t q_lo, uint64_t q_hi, uint64_t qi_lo, uint64_t qi_hi,
+                           uint64_t one_lo, uint64_t one_hi, uint64_t* __restrict__ out) {
+    const u128 q = ((u128)q_hi << 64) | q_lo, qinv = ((u128)qi_hi << 64) | qi_lo;
+    const uint64_t count = 1ull << n;
+    for (uint64_t x = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; x < count;
+         x += (uint64_t)gridDim.x * blockDim.x) {
+        u128 acc = ((u128)one_hi << 64) | one_lo;
+        for (int i = 0; i < n; i++) {
+            const int bit = (int)((x >> (n - 1 - i)) & 1);
+            const u128 f = ((u128)a.f[bit][i][1] << 64) | a.f[bit][i][0];
+            acc = mont_dev(acc, f, q, qinv);
+        }
+        acc = mont_dev(acc, 1, q, qinv);
+        out[2 * x] = (uint64_t)acc;
+        out[2 * x + 1] = (uint64_t)(acc >> 64);
+    }
+}
+
+// eq_table_mod_q(r, q) into out (2^n entries of 16 bytes, u128 little-endian)
+int eq_table_mod_q_dev(bitz_ctx* ctx, const std::vector<u128>& r, u128 q, void* out) {
+    const int n = (int)r.size();
+    BITZ_REQUIRE(n <= kMaxModQN, BITZ_ERR_SHAPE, "eq_table_mod_q_dev: more than 32 coordinates");
     ModN m;
     m.set(q);
-    for (auto& v : hi) v = m.to_mont(v);
-    std::vector<u128> out(1ull << n);
-    const size_t per = (size_t)1 << 12;  // entries per task
-    const size_t tasks = (out.size() + per - 1) / per;
-    unsigned nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
-    nt = (unsigned)std::min<size_t>(nt, tasks);
-    std::vector<std::thread> pool;
-    for (unsigned w = 0; w < nt; w++)
-        pool.emplace_back([&, w]() {
-            for (size_t tk = w; tk < tasks; tk += nt)
-                for (size_t x = tk * per; x < std::min(out.size(), (tk + 1) * per); x++)
-                    out[x] = m.mont(hi[x >> k], lo[x & ((1ull << k) - 1)]);
-        });
-    for (auto& th : pool) th.join();
-    return out;
+    ModQArgs a;
+    for (int i = 0; i < n; i++) {
+        const u128 f1 = m.to_mont(r[i]), f0 = m.to_mont((1 + q - r[i] % q) % q);
+        a.f[0][i][0] = (uint64_t)f0;
+        a.f[0][i][1] = (uint64_t)(f0 >> 64);
+        a.f[1][i][0] = (uint64_t)f1;
+        a.f[1][i][1] = (uint64_t)(f1 >> 64);
+    }
+    const u128 one = m.to_mont(1);
+    const uint64_t count = 1ull << n;
+    const unsigned blocks = (unsigned)std::min<uint64_t>(4096, (count + 255) / 256);
+    TimedLaunch tl(ctx, "eq_mod_q", 16.0 * count, 0.0);
+    hipLaunchKernelGGL(k_eq_mod_q, dim3(blocks), dim3(256), 0, ctx->stream, a, n, (uint64_t)q, (uint64_t)(q >> 64),
+                       (uint64_t)m.ninv, (uint64_t)(m.ninv >> 64), (uint64_t)one, (uint64_t)(one >> 64),
+                       (uint64_t*)out);
+    BITZ_LAUNCH_CHECK();
+    return BITZ_OK;
 }
 
 // 256-bit transcript draw reduced mod q (transcript_uniform_mod, src/prime_sampling.rs:22-33)
@@ -1074,31 +1100,37 @@ int Prover::run(const uint8_t* claimed_in, uint8_t* claimed_out, std::vector<uin
     for (auto& x : r1) x = sample_residue(outer, q);
     for (auto& x : r2) x = sample_residue(outer, q);
     phase.mark(proof_out ? "instance_weights" : "eval_instance_weights");
-    std::vector<u128> row_w = eq_table_mod_q(r1, q), col_w = eq_table_mod_q(r2, q);
+    // row and column weights (eq tables mod q) built on the device, where the
+    // column folds and the LinearClaim statement read them
+    DevBuf colw;
+    if (!roww.alloc(nrows * 16) || !colw.alloc(ncols * 16) || !folds.alloc(ncols * 16)) return BITZ_ERR_ALLOC;
+    if ((rc = eq_table_mod_q_dev(ctx, r1, q, roww.ptr))) return rc;
+    if ((rc = eq_table_mod_q_dev(ctx, r2, q, colw.ptr))) return rc;
     phase.mark(proof_out ? "instance_folds" : "eval_instance_folds");
-    // folds eta_c = sum_b w_b bit(c, b) (exact u128) on the device
+    // folds eta_c = sum_b w_b bit(c, b) (exact u128) on the device, down
+    // through pinned staging with the column weights when the honest value is
+    // wanted
+    const bool want_honest = claimed_out != nullptr || claimed_in == nullptr;
     std::vector<uint8_t> fold_bytes(ncols * 16);
-    {
-        // row weights up through pinned staging (u128 little-endian is the
-        // 16-byte wire form), folds back the same way
-        if ((rc = ctx->scratch.ensure_stage(16 * (nrows + ncols)))) return rc;
-        uint8_t* st = (uint8_t*)ctx->scratch.stage;
-        for (uint64_t b = 0; b < nrows; b++) u128_put(row_w[b], st + 16 * b);
-        if (!roww.alloc(nrows * 16) || !folds.alloc(ncols * 16)) return BITZ_ERR_ALLOC;
-        BITZ_HIP(hipMemcpyAsync(roww.ptr, st, nrows * 16, hipMemcpyHostToDevice, ctx->stream));
-        if ((rc = fold_columns(ctx, (const uint64_t*)h->packed.ptr, t, s, roww.ptr, folds.ptr))) return rc;
-        BITZ_HIP(hipMemcpyAsync(st + 16 * nrows, folds.ptr, ncols * 16, hipMemcpyDeviceToHost, ctx->stream));
-        BITZ_HIP(hipStreamSynchronize(ctx->stream));
-        memcpy(fold_bytes.data(), st + 16 * nrows, ncols * 16);
-    }
     u128 honest = 0;
     {
-        // sum_c fold_c col_w_c mod q: each term as a Montgomery product
-        // fold_c col_w_c R^-1 (fold_c < 2^128, col_w_c < q), the sum times R
-        ModN m;
-        m.set(q);
-        for (uint64_t c = 0; c < ncols; c++) honest = addmod(honest, m.mont(u128_from(&fold_bytes[16 * c]), col_w[c]), q);
-        honest = m.to_mont(honest);
+        if ((rc = ctx->scratch.ensure_stage(32 * ncols))) return rc;
+        uint8_t* st = (uint8_t*)ctx->scratch.stage;
+        if ((rc = fold_columns(ctx, (const uint64_t*)h->packed.ptr, t, s, roww.ptr, folds.ptr))) return rc;
+        BITZ_HIP(hipMemcpyAsync(st, folds.ptr, ncols * 16, hipMemcpyDeviceToHost, ctx->stream));
+        if (want_honest)
+            BITZ_HIP(hipMemcpyAsync(st + 16 * ncols, colw.ptr, ncols * 16, hipMemcpyDeviceToHost, ctx->stream));
+        BITZ_HIP(hipStreamSynchronize(ctx->stream));
+        memcpy(fold_bytes.data(), st, ncols * 16);
+        if (want_honest) {
+            // sum_c fold_c col_w_c mod q: each term as a Montgomery product
+            // fold_c col_w_c R^-1 (fold_c < 2^128, col_w_c < q), the sum times R
+            ModN m;
+            m.set(q);
+            for (uint64_t c = 0; c < ncols; c++)
+                honest = addmod(honest, m.mont(u128_from(st + 16 * c), u128_from(st + 16 * (ncols + c))), q);
+            honest = m.to_mont(honest);
+        }
     }
     u128 claimed = claimed_in ? u128_from(claimed_in) : honest;
     if (claimed_out) u128_put(honest, claimed_out);
@@ -1133,11 +1165,6 @@ int Prover::run(const uint8_t* claimed_in, uint8_t* claimed_out, std::vector<uin
         // LinearClaim encoding (src/bitz/params.rs:273-285): len || row
         // weights || len || column weights || claim, hashed where the row
         // weights already are (the device copy made for the folds)
-        std::vector<uint8_t> cb(ncols * 16);
-        for (uint64_t c = 0; c < ncols; c++) u128_put(col_w[c], cb.data() + 16 * c);
-        DevBuf colw;
-        if (!colw.alloc(ncols * 16)) return BITZ_ERR_ALLOC;
-        BITZ_HIP(hipMemcpyAsync(colw.ptr, cb.data(), cb.size(), hipMemcpyHostToDevice, ctx->stream));
         G claim_g;
         {
             uint8_t b[16];

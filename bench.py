@@ -55,7 +55,8 @@ def parse():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--log-coeffs", type=int, default=24, help="log2 of the u64 coefficient count")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--no-autotune", action="store_true", help="keep the library's default launch shapes")
+    ap.add_argument("--autotune", action="store_true",
+                    help="time launch-shape candidates first (bitz_ctx_autotune) and prove with the fastest")
     ap.add_argument("--cpu-log-coeffs", type=int, default=18)
     ap.add_argument("--no-batched", action="store_true", help="skip the batched and serving measurements")
     ap.add_argument("--device-transcript", action="store_true",
@@ -224,8 +225,9 @@ def main():
     # claim setup (untimed): the honest value of the transcript-derived claim
     com0, claimed = step()
     com0.close()
-    # launch shapes and thresholds for this shape, timed on the device (untimed)
-    tune = autotune(ctx, t, s, 4, 2) if not args.no_autotune else get_tune(ctx)
+    # launch shapes and thresholds: the library defaults, or (--autotune)
+    # candidates timed on the device for this shape (untimed)
+    tune = autotune(ctx, t, s, 4, 2) if args.autotune else get_tune(ctx)
     for _ in range(args.warmup):
         step(claimed)
     torch.cuda.synchronize()

@@ -451,6 +451,22 @@ __global__ void k_leaf_qtab(const uint4* __restrict__ A, int t, const uint4* __r
 // (y, c) reads K bits of column c at rows y + k*stride (k < K) and the table
 // entry qtab[((y << K) + pattern)*2 + {0,1}]; thread (c, ylane) XORs the
 // entries over its rows y < ny, then multiplies by eq_c[c].
+// one thread's XOR of its rows' table entries (the loop of k_leaf_pattern_sums)
+template <int K>
+__device__ __forceinline__ void leaf_pattern_rows(const uint64_t* __restrict__ col, int bit, uint64_t yl, uint64_t ny,
+                                                  uint64_t ylanes, uint64_t stride,
+                                                  const uint4* __restrict__ qtab, uint32_t* e, uint32_t* i) {
+    for (uint64_t y = yl; y < ny; y += ylanes) {
+        uint32_t pat = 0;
+#pragma unroll
+        for (int k = 0; k < K; k++) pat |= (uint32_t)((col[y + k * stride] >> bit) & 1ull) << k;
+        const uint4* q = &qtab[((y << K) + pat) * 2];
+        const uint4 a = q[0], b = q[1];
+        e[0] ^= a.x, e[1] ^= a.y, e[2] ^= a.z, e[3] ^= a.w;
+        i[0] ^= b.x, i[1] ^= b.y, i[2] ^= b.z, i[3] ^= b.w;
+    }
+}
+
 template <int K>
 __global__ __launch_bounds__(kT) void k_leaf_pattern_sums(const uint64_t* __restrict__ packed, int t, int s,
                                                           uint64_t ny, uint64_t stride,
@@ -461,22 +477,13 @@ __global__ __launch_bounds__(kT) void k_leaf_pattern_sums(const uint64_t* __rest
     gf128 acc[2] = {gf_zero(), gf_zero()};
     uint64_t gt = grid_tid();
     if (gt < cols * ylanes) {
-        uint64_t c = gt & (cols - 1), yl = gt >> s;
-        const uint64_t* col = packed + (c >> 6) * nrows;
-        int bit = (int)(c & 63);
-        uint32_t e0 = 0, e1 = 0, e2 = 0, e3 = 0, i0 = 0, i1 = 0, i2 = 0, i3 = 0;
-        for (uint64_t y = yl; y < ny; y += ylanes) {
-            uint32_t pat = 0;
-#pragma unroll
-            for (int k = 0; k < K; k++) pat |= (uint32_t)((col[y + k * stride] >> bit) & 1ull) << k;
-            const uint4* e = &qtab[((y << K) + pat) * 2];
-            uint4 a = e[0], b = e[1];
-            e0 ^= a.x, e1 ^= a.y, e2 ^= a.z, e3 ^= a.w;
-            i0 ^= b.x, i1 ^= b.y, i2 ^= b.z, i3 ^= b.w;
-        }
-        gf128 wc = gf_load(&eq_c[c]);
-        acc[0] = gf_mul(gf128{{e0, e1, e2, e3}}, wc);
-        acc[1] = gf_mul(gf128{{i0, i1, i2, i3}}, wc);
+        const uint64_t c = gt & (cols - 1), yl = gt >> s;
+        const int bit = (int)(c & 63);
+        uint32_t e[4] = {0, 0, 0, 0}, i[4] = {0, 0, 0, 0};
+        leaf_pattern_rows<K>(packed + (c >> 6) * nrows, bit, yl, ny, ylanes, stride, qtab, e, i);
+        const gf128 wc = gf_load(&eq_c[c]);
+        acc[0] = gf_mul(gf128{{e[0], e[1], e[2], e[3]}}, wc);
+        acc[1] = gf_mul(gf128{{i[0], i[1], i[2], i[3]}}, wc);
     }
     block_xor_store<2>(acc, partials + (size_t)blockIdx.x * 2);
 }
@@ -778,27 +785,59 @@ __global__ void k_product_level(const uint4* __restrict__ in, uint64_t half, uin
 }
 
 // ------------------------------------------------- inner-product sumcheck
-// m[b] = sum_c w2[c] bit(c, b) from row-major packed columns.
-__global__ void k_xi_combined(const uint64_t* __restrict__ packed, int t, int s, const uint4* __restrict__ w2,
-                              uint4* __restrict__ out) {
-    const uint64_t nrows = 1ull << t, cols = 1ull << s, groups = (cols + 63) / 64;
-    uint64_t b = grid_tid();
-    if (b >= nrows) return;
-    uint32_t x0 = 0, x1 = 0, x2 = 0, x3 = 0;
-    for (uint64_t g = 0; g < groups; g++) {
-        uint64_t word = packed[g * nrows + b];
-        if (cols - g * 64 < 64) word &= (1ull << (cols - g * 64)) - 1;
-        while (word) {
-            int j = __builtin_ctzll(word);
-            uint4 e = w2[g * 64 + j];
-            x0 ^= e.x;
-            x1 ^= e.y;
-            x2 ^= e.z;
-            x3 ^= e.w;
-            word &= word - 1;
-        }
+// m[b] = sum_c w2[c] bit(c, b) from row-major packed columns, through 4-bit
+// tables of w2 in LDS (four-Russians): block
+// (column chunk of 1024 columns, 2048 rows) builds T[n][v] = sum of w2 over
+// the set bits of v for the chunk's 256 nibble positions n (64 KiB), then each
+// row XORs one table entry per nibble of its 16 packed words: 256 LDS reads
+// per row and chunk in place of a dependent global gather per set bit. Chunk
+// k's sums go to partial[k][b] (out itself when there is one chunk).
+constexpr int kXiChunkWords = 16, kXiRows = 2048;
+__global__ __launch_bounds__(kT) void k_xi_nibbles(const uint64_t* __restrict__ packed, int t, int s,
+                                                   const uint4* __restrict__ w2, uint4* __restrict__ partial) {
+    __shared__ uint4 tab[kXiChunkWords * 16 * 16];
+    const uint64_t nrows = 1ull << t, cols = 1ull << s, groups = (cols + 63) >> 6;
+    const uint64_t g0 = (uint64_t)blockIdx.x * kXiChunkWords;
+    const uint32_t nw = (uint32_t)std::min<uint64_t>(kXiChunkWords, groups - g0);
+    for (uint32_t e = threadIdx.x; e < nw * 256; e += kT) {
+        const uint64_t c0 = g0 * 64 + (e >> 4) * 4;  // first column of nibble position e >> 4
+        const uint32_t v = e & 15;
+        uint32_t x0 = 0, x1 = 0, x2 = 0, x3 = 0;
+        for (int k = 0; k < 4; k++)
+            if (((v >> k) & 1) && c0 + k < cols) {
+                const uint4 q = w2[c0 + k];
+                x0 ^= q.x, x1 ^= q.y, x2 ^= q.z, x3 ^= q.w;
+            }
+        tab[e] = make_uint4(x0, x1, x2, x3);
     }
-    out[b] = make_uint4(x0, x1, x2, x3);
+    __syncthreads();
+    const uint64_t row0 = (uint64_t)blockIdx.y * kXiRows;
+    const uint64_t rows_here = std::min<uint64_t>(kXiRows, nrows - row0);
+    for (uint64_t i = threadIdx.x; i < rows_here; i += kT) {
+        const uint64_t b = row0 + i;
+        uint32_t x0 = 0, x1 = 0, x2 = 0, x3 = 0;
+        for (uint32_t w = 0; w < nw; w++) {
+            const uint64_t word = packed[(g0 + w) * nrows + b];
+#pragma unroll
+            for (int nib = 0; nib < 16; nib++) {
+                const uint4 q = tab[(w * 16 + nib) * 16 + (uint32_t)((word >> (4 * nib)) & 15)];
+                x0 ^= q.x, x1 ^= q.y, x2 ^= q.z, x3 ^= q.w;
+            }
+        }
+        partial[(uint64_t)blockIdx.x * nrows + b] = make_uint4(x0, x1, x2, x3);
+    }
+}
+
+// out[b] = XOR over k < chunks of partial[k][b]
+__global__ void k_xor_chunks(const uint4* __restrict__ partial, uint32_t chunks, uint64_t n, uint4* __restrict__ out) {
+    for (uint64_t b = grid_tid(); b < n; b += grid_threads()) {
+        uint4 a = partial[b];
+        for (uint32_t k = 1; k < chunks; k++) {
+            const uint4 q = partial[(uint64_t)k * n + b];
+            a.x ^= q.x, a.y ^= q.y, a.z ^= q.z, a.w ^= q.w;
+        }
+        out[b] = a;
+    }
 }
 
 // partial[chunk][c] = sum over the chunk's rows b of eq[b] bit(c, b), from
@@ -868,31 +907,36 @@ __global__ __launch_bounds__(kT) void k_fold_lsb(const uint4* v, uint64_t half, 
 }
 
 // ------------------------------------------------------------ ring switch
-// claims[i] = sum_x bit_i(p[x]) suffix[x]: each thread keeps 128 partial
-// sums in registers? Too many; instead each block owns 8 bit positions.
+// hat_s[i] = sum_x bit_i(p[x]) suffix[x] for the 128 bits i of p
+// (fold_1b_rows_naive, ring_switch.rs:1332-1381). 128 accumulators do not
+// fit a thread, so block (j, chunk) owns bits 8j..8j+7 over a contiguous
+// chunk of elements; the 16 blocks of a chunk are consecutive in the grid,
+// run together and share the chunk's loads through the caches (the data is
+// read from HBM about once, not 16 times).
 __global__ __launch_bounds__(kT) void k_fold_1b_rows(const uint4* __restrict__ p, const uint4* __restrict__ suffix,
-                                                     uint64_t n, uint4* __restrict__ partials /* [blocks][128] */) {
-    // blockIdx.y selects 8 bit positions (16 groups cover 128 bits)
-    const int bit0 = blockIdx.y * 8;
+                                                     uint64_t n, uint64_t per, uint4* __restrict__ partials /* [chunks][128] */) {
+    const int j = blockIdx.x & 15;
+    const uint64_t chunk = blockIdx.x >> 4;
+    const uint64_t x0 = chunk * per, x1 = std::min<uint64_t>(n, x0 + per);
     gf128 acc[8];
 #pragma unroll
     for (int k = 0; k < 8; k++) acc[k] = gf_zero();
-    for (uint64_t x = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; x < n; x += (uint64_t)gridDim.x * blockDim.x) {
-        uint4 pv = p[x];
-        uint32_t word = (&pv.x)[bit0 >> 5];
-        uint32_t byte = (word >> (bit0 & 31)) & 255u;
+    for (uint64_t x = x0 + threadIdx.x; x < x1; x += kT) {
+        const uint4 pv = p[x];
+        const uint32_t word = (&pv.x)[j >> 2];
+        const uint32_t byte = (word >> ((j & 3) * 8)) & 255u;
         if (!byte) continue;
-        gf128 sv = gf_load(&suffix[x]);
+        const gf128 sv = gf_load(&suffix[x]);
 #pragma unroll
         for (int k = 0; k < 8; k++) {
-            uint32_t m = 0u - ((byte >> k) & 1u);
+            const uint32_t m = 0u - ((byte >> k) & 1u);
             acc[k].w[0] ^= sv.w[0] & m;
             acc[k].w[1] ^= sv.w[1] & m;
             acc[k].w[2] ^= sv.w[2] & m;
             acc[k].w[3] ^= sv.w[3] & m;
         }
     }
-    block_xor_store<8>(acc, partials + ((size_t)blockIdx.x * 128 + bit0));
+    block_xor_store<8>(acc, partials + (size_t)chunk * 128 + 8 * j);
 }
 
 // basis[x] = sum over set bits i of suffix[x] of w[i], via 16 byte tables
@@ -1738,9 +1782,20 @@ int product_level(bitz_ctx* ctx, const void* in, uint64_t n, void* out) {
 }
 
 int xi_combined(bitz_ctx* ctx, const uint64_t* packed, int t, int s, const void* w2, void* out) {
-    uint64_t n = 1ull << t;
-    hipLaunchKernelGGL(k_xi_combined, dim3((unsigned)((n + kT - 1) / kT)), dim3(kT), 0, ctx->stream, packed, t, s,
-                       (const uint4*)w2, (uint4*)out);
+    const uint64_t n = 1ull << t, groups = ((1ull << s) + 63) / 64;
+    const uint64_t chunks = (groups + kXiChunkWords - 1) / kXiChunkWords;
+    const uint64_t row_blocks = (n + kXiRows - 1) / kXiRows;
+    TimedLaunch tl(ctx, "xi_combined", 8.0 * (double)(groups << t) + 16.0 * n * (2 * chunks + 1), 0.0);
+    uint4* dst = (uint4*)out;
+    if (chunks > 1) {
+        if (!ctx->scratch.fold_partial.alloc(chunks * n * 16)) return BITZ_ERR_ALLOC;
+        dst = (uint4*)ctx->scratch.fold_partial.ptr;
+    }
+    hipLaunchKernelGGL(k_xi_nibbles, dim3((unsigned)chunks, (unsigned)row_blocks), dim3(kT), 0, ctx->stream, packed,
+                       t, s, (const uint4*)w2, dst);
+    if (chunks > 1)
+        hipLaunchKernelGGL(k_xor_chunks, dim3(blocks_for(n, 1)), dim3(kT), 0, ctx->stream, (const uint4*)dst,
+                           (uint32_t)chunks, n, (uint4*)out);
     BITZ_LAUNCH_CHECK();
     return BITZ_OK;
 }
@@ -1783,11 +1838,13 @@ int fold_lsb(bitz_ctx* ctx, const void* v, uint64_t n, const uint8_t r[16], void
 }
 
 int fold_1b_rows(bitz_ctx* ctx, const void* p, const void* suffix, uint64_t n, void* out_dev) {
-    unsigned nb = blocks_for(n, 8, 512);
+    const unsigned nb = blocks_for(n, 8, 512);  // element chunks
+    const uint64_t per = (n + nb - 1) / nb;
     int rc = ctx->scratch.ensure((size_t)nb * 128 * 16);
     if (rc) return rc;
-    hipLaunchKernelGGL(k_fold_1b_rows, dim3(nb, 16), dim3(kT), 0, ctx->stream, (const uint4*)p,
-                       (const uint4*)suffix, n, (uint4*)ctx->scratch.partials.ptr);
+    TimedLaunch tl(ctx, "fold_1b_rows", 32.0 * (double)n, 0.0);
+    hipLaunchKernelGGL(k_fold_1b_rows, dim3(nb * 16), dim3(kT), 0, ctx->stream, (const uint4*)p, (const uint4*)suffix,
+                       n, per, (uint4*)ctx->scratch.partials.ptr);
     BITZ_LAUNCH_CHECK();
     // 128 independent sums over nb blocks: reuse the K=1 reducer per column
     // by a strided layout -> simple dedicated pass

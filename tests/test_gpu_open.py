@@ -61,7 +61,7 @@ def test_fold_columns_and_images(t, s):
     assert np.array_equal(imgs, O.pow_many(g, want[:k]))
 
 
-@pytest.mark.parametrize("t,s", [(7, 0), (8, 6), (9, 7), (12, 9), (13, 5), (17, 3)])
+@pytest.mark.parametrize("t,s", [(7, 0), (8, 6), (9, 7), (12, 9), (13, 5), (17, 3), (10, 11), (8, 13)])
 def test_pack_and_sumcheck_tables(t, s):
     rows = rand_rows(t, s, 3)
     packed = ops().pack_columns(dev_u64(rows), t, s)

@@ -115,7 +115,6 @@ extern "C" int bitz_ctx_autotune(bitz_ctx* ctx, uint32_t t, uint32_t s, uint32_t
     PoolScope pool_scope(&ctx->pool);
     const double t0 = now_ms();
     const int n = (int)(t + s);
-    const Tune defaults;
     Tune best = ctx->tune;
     int rc;
     uint8_t rho[16];
@@ -124,7 +123,7 @@ extern "C" int bitz_ctx_autotune(bitz_ctx* ctx, uint32_t t, uint32_t s, uint32_t
     // ---- 1. round-sum launch shape (fused fold + row-only sums) ----
     // sizes of a large layer's device rounds, every third one
     std::vector<int> logs;
-    for (int k = std::min(n - 4, 24); k > defaults.host_tail_log; k -= 3) logs.push_back(k);
+    for (int k = std::min(n - 4, 24); k > 10; k -= 3) logs.push_back(k);
     if (!logs.empty()) {
         const uint64_t hmax = 1ull << logs.front();
         const int sy = std::max(0, logs.front() - (int)s);

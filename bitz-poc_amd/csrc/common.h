@@ -147,6 +147,11 @@ struct Scratch {
     void* stage = nullptr;
     size_t stage_bytes = 0;
     int ensure_stage(size_t bytes);
+    // pinned upload ring: host data is copied in and sent with an
+    // asynchronous copy, no stream synchronisation; a region is written again
+    // only after a synchronisation at the wrap-around
+    void* up = nullptr;
+    size_t up_bytes = 0, up_off = 0;
     uint8_t comb_base[16] = {0};
     bool comb_ready = false;
     int ensure(size_t partial_bytes);
@@ -161,13 +166,15 @@ struct KernelStat {
     uint64_t launches = 0;
     double ms = 0, bytes = 0, ops = 0;
 };
-// Launch-shape choices that do not change any result: the defaults are the
-// hand-set heuristics, bitz_ctx_autotune replaces them by timed choices.
+// Launch-shape choices that do not change any result. The defaults are what
+// bitz_ctx_autotune picks most often for the 2^24-coefficient shape (n = 30)
+// on one MI355X; bitz_ctx_autotune replaces them by choices timed for the
+// caller's shape.
 struct Tune {
     uint32_t round_r_max = 32;        // slots per thread in round-sum kernels
-    uint32_t round_min_chunks = 2048; // chunks below which R halves
-    uint32_t round_block_cap = 4096;  // blocks per round-sum launch
-    int host_tail_log = 10;           // sumcheck rounds of at most 2^x slots run on the host
+    uint32_t round_min_chunks = 1024; // chunks below which R halves
+    uint32_t round_block_cap = 2048;  // blocks per round-sum launch
+    int host_tail_log = 11;           // sumcheck rounds of at most 2^x slots run on the host
     int ntt_radix_log = 4;            // butterfly layers per global NTT pass
     bool tuned = false;
 };

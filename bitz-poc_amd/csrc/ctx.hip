@@ -131,6 +131,9 @@ extern "C" int bitz_ctx_create(int device, bitz_ctx** out) {
 extern "C" void bitz_ctx_destroy(bitz_ctx* ctx) {
     if (!ctx) return;
     (void)hipSetDevice(ctx->device);
+    // work of the context (copies out of its pinned buffers included) ends
+    // before those buffers are freed
+    if (ctx->stream && ctx->stream != ctx->own_stream) (void)hipStreamSynchronize(ctx->stream);
     if (ctx->own_stream) {
         (void)hipStreamSynchronize(ctx->own_stream);
         (void)hipStreamDestroy(ctx->own_stream);

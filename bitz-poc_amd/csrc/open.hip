@@ -1107,14 +1107,16 @@ __global__ __launch_bounds__(kT) void k_scale(const uint4* __restrict__ a, uint6
 // Smallest nonce in [base, base + span) with >= bits leading zero bits of
 // BLAKE3(prefix16 || seed16 || nonce_le) (src/bitz/pcs.rs:1252-1298);
 // result via atomicMin on a u64 (stable: the minimum is unique).
-__global__ __launch_bounds__(kT) void k_pow_scan(const uint32_t* __restrict__ pre /* 8 words */, uint64_t base,
-                                                 uint64_t span, uint32_t bits,
+struct PowPrefix {
+    uint32_t w[8];  // prefix16 || seed16 as little-endian words
+};
+__global__ __launch_bounds__(kT) void k_pow_scan(PowPrefix pre, uint64_t base, uint64_t span, uint32_t bits,
                                                  unsigned long long* __restrict__ best) {
     uint64_t i = grid_tid();
     if (i >= span) return;
     uint64_t nonce = base + i;
     if ((unsigned long long)nonce >= *((volatile unsigned long long*)best)) return;
-    uint32_t m[16] = {pre[0], pre[1], pre[2], pre[3], pre[4], pre[5], pre[6], pre[7],
+    uint32_t m[16] = {pre.w[0], pre.w[1], pre.w[2], pre.w[3], pre.w[4], pre.w[5], pre.w[6], pre.w[7],
                       (uint32_t)nonce, (uint32_t)(nonce >> 32), 0, 0, 0, 0, 0, 0};
     uint32_t cv[8];
     b3::iv_init(cv);
@@ -1139,20 +1141,44 @@ __global__ __launch_bounds__(kT) void k_pow_scan(const uint32_t* __restrict__ pr
     if (zeros >= bits) atomicMin(best, (unsigned long long)nonce);
 }
 
-// out[j] = XOR over b < nblocks of partials[b * width + j]
-__global__ void k_reduce_columns(const uint4* __restrict__ partials, uint32_t nblocks, uint32_t width,
-                                 uint4* __restrict__ out) {
-    uint32_t j = grid_tid();
-    if (j >= width) return;
-    uint32_t x0 = 0, x1 = 0, x2 = 0, x3 = 0;
-    for (uint32_t b = 0; b < nblocks; b++) {
-        uint4 q = partials[(size_t)b * width + j];
-        x0 ^= q.x;
-        x1 ^= q.y;
-        x2 ^= q.z;
-        x3 ^= q.w;
+__global__ void k_pow_init(unsigned long long* best) {
+    if (threadIdx.x == 0) *best = ~0ull;
+}
+
+// the smallest valid nonce so far (all ones: none) to the mailbox
+__global__ void k_pow_report(const unsigned long long* best, uint4* mbox_vals, uint32_t* mbox_flag, uint32_t seq) {
+    if (threadIdx.x == 0) {
+        const unsigned long long b = *(volatile const unsigned long long*)best;
+        mbox_vals[0] = make_uint4((uint32_t)b, (uint32_t)(b >> 32), 0, 0);
+        __threadfence_system();
+        *(volatile uint32_t*)mbox_flag = seq;
     }
-    out[j] = make_uint4(x0, x1, x2, x3);
+}
+
+// out[j] = XOR over b < nblocks of partials[b * width + j]
+// out[j] = XOR over b < nblocks of partials[b][j]. Block: 16 columns x 16
+// slices of the b range (loads of 16 consecutive columns coalesce), the
+// slices combined in LDS.
+__global__ __launch_bounds__(256) void k_reduce_columns(const uint4* __restrict__ partials, uint32_t nblocks,
+                                                        uint32_t width, uint4* __restrict__ out) {
+    __shared__ uint4 red[16][16];
+    const uint32_t cj = threadIdx.x & 15, rs = threadIdx.x >> 4;
+    const uint32_t j = blockIdx.x * 16 + cj;
+    uint4 a = make_uint4(0, 0, 0, 0);
+    if (j < width)
+        for (uint32_t b = rs; b < nblocks; b += 16) {
+            const uint4 q = partials[(size_t)b * width + j];
+            a.x ^= q.x, a.y ^= q.y, a.z ^= q.z, a.w ^= q.w;
+        }
+    red[rs][cj] = a;
+    __syncthreads();
+    if (rs == 0 && j < width) {
+        for (int k = 1; k < 16; k++) {
+            const uint4 q = red[k][cj];
+            a.x ^= q.x, a.y ^= q.y, a.z ^= q.z, a.w ^= q.w;
+        }
+        out[j] = a;
+    }
 }
 
 // gather rows[q] (lanes elements each) for sorted query positions
@@ -1264,6 +1290,36 @@ Scratch::~Scratch() {
     if (host) (void)hipHostFree(host);
     if (mbox) (void)hipHostFree(mbox);
     if (stage) (void)hipHostFree(stage);
+    if (up) (void)hipHostFree(up);
+}
+
+int upload_async(bitz_ctx* ctx, const void* host, size_t bytes, void* dev) {
+    if (bytes == 0) return BITZ_OK;
+    Scratch& sc = ctx->scratch;
+    const size_t need = (bytes + 255) & ~(size_t)255;
+    if (need > sc.up_bytes) {
+        // grow (at least 1 MiB): earlier copies from the old ring finish first
+        BITZ_HIP(hipStreamSynchronize(ctx->stream));
+        if (sc.up) (void)hipHostFree(sc.up);
+        sc.up = nullptr;
+        sc.up_bytes = sc.up_off = 0;
+        const size_t cap = std::max<size_t>(need, 1u << 20);
+        if (hipHostMalloc(&sc.up, cap, hipHostMallocDefault) != hipSuccess) {
+            sc.up = nullptr;
+            set_error("upload ring: hipHostMalloc failed");
+            return BITZ_ERR_ALLOC;
+        }
+        sc.up_bytes = cap;
+    }
+    if (sc.up_off + need > sc.up_bytes) {
+        BITZ_HIP(hipStreamSynchronize(ctx->stream));  // every copy out of the ring is done
+        sc.up_off = 0;
+    }
+    uint8_t* slot = (uint8_t*)sc.up + sc.up_off;
+    memcpy(slot, host, bytes);
+    sc.up_off += need;
+    BITZ_HIP(hipMemcpyAsync(dev, slot, bytes, hipMemcpyHostToDevice, ctx->stream));
+    return BITZ_OK;
 }
 
 int Scratch::ensure_stage(size_t bytes) {
@@ -1852,7 +1908,7 @@ int fold_1b_rows(bitz_ctx* ctx, const void* p, const void* suffix, uint64_t n, v
 }
 
 int reduce_columns(bitz_ctx* ctx, const uint4* partials, uint32_t nblocks, uint32_t width, uint4* out) {
-    hipLaunchKernelGGL(k_reduce_columns, dim3((width + kT - 1) / kT), dim3(kT), 0, ctx->stream, partials, nblocks,
+    hipLaunchKernelGGL(k_reduce_columns, dim3((width + 15) / 16), dim3(256), 0, ctx->stream, partials, nblocks,
                        width, out);
     BITZ_LAUNCH_CHECK();
     return BITZ_OK;
@@ -1933,27 +1989,35 @@ int pow_find(bitz_ctx* ctx, const uint8_t prefix[16], const uint8_t seed[16], ui
     int rc = ctx->scratch.ensure(4096);
     if (rc) return rc;
     unsigned long long* best = (unsigned long long*)ctx->scratch.result.ptr;
-    uint32_t* pre = (uint32_t*)((uint8_t*)ctx->scratch.result.ptr + 64);
-    uint8_t staged[64];
-    unsigned long long init = ~0ull;
-    memcpy(staged, &init, 8);
-    memset(staged + 8, 0, 56);
-    memcpy(ctx->scratch.host, staged, 64);
-    memcpy((uint8_t*)ctx->scratch.host + 64, prefix, 16);
-    memcpy((uint8_t*)ctx->scratch.host + 80, seed, 16);
-    BITZ_HIP(hipMemcpyAsync(ctx->scratch.result.ptr, ctx->scratch.host, 96, hipMemcpyHostToDevice, ctx->stream));
+    PowPrefix pre;
+    memcpy(pre.w, prefix, 16);
+    memcpy(pre.w + 4, seed, 16);
+    hipLaunchKernelGGL(k_pow_init, dim3(1), dim3(64), 0, ctx->stream, best);
     // spans of 16 x 2^bits nonces (at least 2^14, at most 2^20): the first
     // holds a solution with probability 1 - e^-16, and scanning spans in
-    // order keeps the smallest valid nonce
+    // order keeps the smallest valid nonce. After each span the best nonce
+    // goes to the mailbox (no copy, no stream synchronisation).
+    Scratch& sc = ctx->scratch;
     const uint64_t span = std::min<uint64_t>(1ull << 20, std::max<uint64_t>(1ull << 14, 16ull << bits));
     for (uint64_t base = 0;; base += span) {
         hipLaunchKernelGGL(k_pow_scan, dim3((unsigned)(span / kT)), dim3(kT), 0, ctx->stream, pre, base, span,
                            bits, best);
-        BITZ_LAUNCH_CHECK();
         unsigned long long h;
-        BITZ_HIP(hipMemcpyAsync(ctx->scratch.host, best, 8, hipMemcpyDeviceToHost, ctx->stream));
-        BITZ_HIP(hipStreamSynchronize(ctx->stream));
-        memcpy(&h, ctx->scratch.host, 8);
+        if (sc.mbox_dev) {
+            const uint32_t seq = ++sc.seq;
+            hipLaunchKernelGGL(k_pow_report, dim3(1), dim3(64), 0, ctx->stream, (const unsigned long long*)best,
+                               (uint4*)((uint8_t*)sc.mbox_dev + 64), (uint32_t*)sc.mbox_dev, seq);
+            BITZ_LAUNCH_CHECK();
+            sc.mbox_pending = true;
+            uint8_t v[16];
+            if ((rc = read_result(ctx, 1, v))) return rc;
+            memcpy(&h, v, 8);
+        } else {
+            BITZ_LAUNCH_CHECK();
+            BITZ_HIP(hipMemcpyAsync(ctx->scratch.host, best, 8, hipMemcpyDeviceToHost, ctx->stream));
+            BITZ_HIP(hipStreamSynchronize(ctx->stream));
+            memcpy(&h, ctx->scratch.host, 8);
+        }
         if (h != ~0ull) {
             *nonce = h;
             return BITZ_OK;

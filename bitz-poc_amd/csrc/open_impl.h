@@ -9,6 +9,9 @@ namespace bitz {
 // Sums produced by the reduction kernels land in ctx->scratch.result (or in
 // out_dev when non-null); read_result copies K field elements to the host.
 int read_result(bitz_ctx* ctx, int K, uint8_t* host_out);
+// bytes of host data to device memory dev through the pinned upload ring
+// (asynchronous: host may be reused as soon as this returns)
+int upload_async(bitz_ctx* ctx, const void* host, size_t bytes, void* dev);
 
 int eq_table(bitz_ctx* ctx, const void* r_dev, int n, void* out);
 // Tables over host coordinates passed as a kernel argument (no upload, no

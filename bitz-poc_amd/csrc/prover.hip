@@ -368,13 +368,11 @@ struct Prover {
     DevBuf packed, rowimg, colimg, folds, roww, levels, eqc, eqy, eqcinv, tmp0, tmp1, tmp2, tmp3, qdev, pt, fsbuf;
     std::vector<DevBuf*> lig_bufs;
 
+    // through the pinned upload ring: no stream synchronisation, and the
+    // host vector may be freed as soon as this returns
     int upload_gf(const std::vector<G>& v, DevBuf& buf) {
         if (!buf.alloc(std::max<size_t>(v.size(), 1) * 16)) return BITZ_ERR_ALLOC;
-        if (!v.empty())
-            BITZ_HIP(hipMemcpyAsync(buf.ptr, v.data(), v.size() * 16, hipMemcpyHostToDevice, ctx->stream));
-        // the host vector may be freed by the caller: wait for the copy
-        BITZ_HIP(hipStreamSynchronize(ctx->stream));
-        return BITZ_OK;
+        return upload_async(ctx, v.data(), v.size() * 16, buf.ptr);
     }
     int download(const void* dev, void* host, size_t bytes) {
         BITZ_HIP(hipMemcpyAsync(host, dev, bytes, hipMemcpyDeviceToHost, ctx->stream));
@@ -395,10 +393,7 @@ struct Prover {
             return eq_args(ctx, r.data(), (int)r.size(), buf.ptr);
         }
         if (!pt.alloc(std::max<size_t>(r.size(), 1) * 16)) return BITZ_ERR_ALLOC;
-        if (!r.empty()) {
-            BITZ_HIP(hipMemcpyAsync(pt.ptr, r.data(), r.size() * 16, hipMemcpyHostToDevice, ctx->stream));
-            BITZ_HIP(hipStreamSynchronize(ctx->stream));
-        }
+        if (int rc = upload_async(ctx, r.data(), r.size() * 16, pt.ptr)) return rc;
         if (!buf.alloc((16ull << r.size()))) return BITZ_ERR_ALLOC;
         return eq_table(ctx, pt.ptr, (int)r.size(), buf.ptr);
     }
@@ -1430,7 +1425,7 @@ int Prover::run(const uint8_t* claimed_in, uint8_t* claimed_out, std::vector<uin
         int r_ = upload_gf(aw, tmp1);
         if (r_) return r_;
         if (!qdev.alloc(queries.size() * 8 + 16)) return BITZ_ERR_ALLOC;
-        BITZ_HIP(hipMemcpyAsync(qdev.ptr, queries.data(), queries.size() * 8, hipMemcpyHostToDevice, ctx->stream));
+        if ((r_ = upload_async(ctx, queries.data(), queries.size() * 8, qdev.ptr))) return r_;
         // s_k(v_k) and inverses (eval_sk_at_vks)
         std::vector<G> sks(log_n + 1, gzero()), inv(log_n + 1);
         sks[0] = gone();

@@ -135,6 +135,10 @@ namespace bitz {
 // tables) plus a pinned host staging buffer for round results.
 struct Scratch {
     DevBuf partials, result, eq_lo, fold_partial, comb, what, misc, leaf_tab;
+    // the proof-of-work search's best nonce: all ones between searches (each
+    // search's final report resets it)
+    DevBuf pow_best;
+    bool pow_best_ready = false;
     void* host = nullptr;
     // Round results the host waits for: pinned, device-mapped, coherent host
     // memory ([0]: sequence flag, values from byte 64) written by the final

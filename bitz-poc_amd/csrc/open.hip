@@ -1145,10 +1145,13 @@ __global__ void k_pow_init(unsigned long long* best) {
     if (threadIdx.x == 0) *best = ~0ull;
 }
 
-// the smallest valid nonce so far (all ones: none) to the mailbox
-__global__ void k_pow_report(const unsigned long long* best, uint4* mbox_vals, uint32_t* mbox_flag, uint32_t seq) {
+// the smallest valid nonce so far (all ones: none) to the mailbox; with
+// reset, the cell goes back to all ones for the next search
+__global__ void k_pow_report(unsigned long long* best, uint4* mbox_vals, uint32_t* mbox_flag, uint32_t seq,
+                             int reset) {
     if (threadIdx.x == 0) {
-        const unsigned long long b = *(volatile const unsigned long long*)best;
+        const unsigned long long b = *(volatile unsigned long long*)best;
+        if (reset) *best = ~0ull;
         mbox_vals[0] = make_uint4((uint32_t)b, (uint32_t)(b >> 32), 0, 0);
         __threadfence_system();
         *(volatile uint32_t*)mbox_flag = seq;
@@ -1988,16 +1991,20 @@ int pow_find(bitz_ctx* ctx, const uint8_t prefix[16], const uint8_t seed[16], ui
     BITZ_REQUIRE(bits <= 40, BITZ_ERR_ARG, "pow_find: bits too large");
     int rc = ctx->scratch.ensure(4096);
     if (rc) return rc;
-    unsigned long long* best = (unsigned long long*)ctx->scratch.result.ptr;
+    Scratch& sc = ctx->scratch;
+    if (!sc.pow_best.alloc(256)) return BITZ_ERR_ALLOC;
+    unsigned long long* best = (unsigned long long*)sc.pow_best.ptr;
     PowPrefix pre;
     memcpy(pre.w, prefix, 16);
     memcpy(pre.w + 4, seed, 16);
-    hipLaunchKernelGGL(k_pow_init, dim3(1), dim3(64), 0, ctx->stream, best);
+    // the cell is all ones unless this is the first search or the last one
+    // ended without a mailbox report
+    if (!sc.pow_best_ready || !sc.mbox_dev) hipLaunchKernelGGL(k_pow_init, dim3(1), dim3(64), 0, ctx->stream, best);
+    sc.pow_best_ready = false;
     // spans of 16 x 2^bits nonces (at least 2^14, at most 2^20): the first
     // holds a solution with probability 1 - e^-16, and scanning spans in
     // order keeps the smallest valid nonce. After each span the best nonce
     // goes to the mailbox (no copy, no stream synchronisation).
-    Scratch& sc = ctx->scratch;
     const uint64_t span = std::min<uint64_t>(1ull << 20, std::max<uint64_t>(1ull << 14, 16ull << bits));
     for (uint64_t base = 0;; base += span) {
         hipLaunchKernelGGL(k_pow_scan, dim3((unsigned)(span / kT)), dim3(kT), 0, ctx->stream, pre, base, span,
@@ -2005,13 +2012,16 @@ int pow_find(bitz_ctx* ctx, const uint8_t prefix[16], const uint8_t seed[16], ui
         unsigned long long h;
         if (sc.mbox_dev) {
             const uint32_t seq = ++sc.seq;
-            hipLaunchKernelGGL(k_pow_report, dim3(1), dim3(64), 0, ctx->stream, (const unsigned long long*)best,
-                               (uint4*)((uint8_t*)sc.mbox_dev + 64), (uint32_t*)sc.mbox_dev, seq);
+            // reset the cell with this report; a span without a solution
+            // keeps it (the next span continues the search)
+            hipLaunchKernelGGL(k_pow_report, dim3(1), dim3(64), 0, ctx->stream, best,
+                               (uint4*)((uint8_t*)sc.mbox_dev + 64), (uint32_t*)sc.mbox_dev, seq, 1);
             BITZ_LAUNCH_CHECK();
             sc.mbox_pending = true;
             uint8_t v[16];
             if ((rc = read_result(ctx, 1, v))) return rc;
             memcpy(&h, v, 8);
+            sc.pow_best_ready = true;  // the report reset the cell to all ones
         } else {
             BITZ_LAUNCH_CHECK();
             BITZ_HIP(hipMemcpyAsync(ctx->scratch.host, best, 8, hipMemcpyDeviceToHost, ctx->stream));

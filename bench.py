@@ -116,7 +116,21 @@ def _job_specs(rng, count, lo, hi):
     return specs
 
 
-def batched_and_serving(args):
+def verify_all(ctx, jobs, specs):
+    """Every proof through the library's verifier (untimed in the rates
+    above): (all accepted, ms per proof)."""
+    from bitz_amd.device import verify_standalone
+    t0 = time.perf_counter()
+    ok = True
+    for j, (t, s, r, _) in zip(jobs, specs):
+        try:
+            verify_standalone(ctx, j.root, t, s, r, 4, j.claimed, j.proof)
+        except RuntimeError:
+            ok = False
+    return ok, (time.perf_counter() - t0) * 1e3 / max(len(jobs), 1)
+
+
+def batched_and_serving(args, ctx):
     """Batched commit+open of independent polynomials of mixed sizes and
     rates on one server (bitz_server: per-worker contexts, jobs overlapping on
     the GPU) against the same jobs one at a time, and a serving loop with
@@ -158,6 +172,9 @@ def batched_and_serving(args):
         "sequential_mcoeff_per_s": round(coeffs / dt_seq / 1e6, 2),
         "bit_exact_vs_sequential": exact,
     }
+    ok, vms = verify_all(ctx, bat, specs)
+    batched["all_verified"] = ok
+    batched["verify_ms_per_proof"] = round(vms, 3)
     # serving: Poisson arrivals at 70% of the batched rate
     rate = 0.7 * len(specs) / dt_bat
     sspecs = _job_specs(np.random.default_rng(65), args.serve_jobs, 14, 22)
@@ -191,6 +208,9 @@ def batched_and_serving(args):
         "p99_ms": round(float(np.percentile(lat, 99)), 3),
         "max_ms": round(float(lat.max()), 3),
     }
+    ok, vms = verify_all(ctx, jobs, sspecs)
+    serving["all_verified"] = ok
+    serving["verify_ms_per_proof"] = round(vms, 3)
     return batched, serving
 
 
@@ -224,6 +244,7 @@ def main():
 
     # claim setup (untimed): the honest value of the transcript-derived claim
     com0, claimed = step()
+    root0 = com0.root
     com0.close()
     # launch shapes and thresholds: the library defaults, or (--autotune)
     # candidates timed on the device for this shape (untimed)
@@ -290,9 +311,14 @@ def main():
         dst.copy_(src)
         torch.cuda.synchronize()
         del src, dst
+    # the last timed step's proof through the library's verifier (untimed)
+    from bitz_amd.device import verify_standalone
+    tv = time.perf_counter()
+    verify_standalone(ctx, root0, t, s, 2, 4, claimed, proof)
+    verify_ms = round((time.perf_counter() - tv) * 1e3, 3)
     batched = serving = None
     if not args.no_batched:
-        batched, serving = batched_and_serving(args)
+        batched, serving = batched_and_serving(args, ctx)
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(args.cpu_log_coeffs)
@@ -311,7 +337,7 @@ def main():
         "data": "synthetic: uniform u64 coefficients from a seeded generator, resident in HBM",
         "config": {"workload": f"BitZ PCS commit+open, 2^{args.log_coeffs} u64 coefficients", "n": n, "t": t,
                    "s": s, "rate": "1/4", "initial_k": 4, "merkle_hash": "blake3", "ladder": "custom:2:4",
-                   "proof_bytes": len(proof) if proof else None},
+                   "proof_bytes": len(proof) if proof else None, "proof_verified": True, "verify_ms": verify_ms},
         "roofline": roofline,
         "cpu_baseline": cpu,
         "batched": batched,

@@ -13,7 +13,7 @@ LIB_PATH = os.path.join(_PKG_DIR, "libbitz_hip.so")
 HEADER_PATH = os.path.join(os.path.dirname(_PKG_DIR), "include", "bitz_hip.h")
 
 BITZ_OK = 0
-ERR_NAMES = {-1: "SHAPE", -2: "ALLOC", -3: "DEVICE", -4: "ARG"}
+ERR_NAMES = {-1: "SHAPE", -2: "ALLOC", -3: "DEVICE", -4: "ARG", -5: "VERIFY"}
 
 
 class BitzError(RuntimeError):
@@ -98,6 +98,7 @@ SIGNATURES = {
     "bitz_commit_read_tree": (_I, [_P, _P, _U64]),
     "bitz_standalone_evaluation": (_I, [_P, _P, _P]),
     "bitz_prove_standalone": (_I, [_P, _P, _P, _P, _U64, C.POINTER(_U64)]),
+    "bitz_verify_standalone": (_I, [_P, _P, _U32, _U32, _U32, _U32, _P, _P, _U64]),
     "bitz_ligerito_config": (_I, [_U32, _U32, _U32, _U32, _P, _U32]),
     "bitz_host_blake3": (None, [_P, _U64, _P]),
     "bitz_job_footprint": (_U64, [C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32]),

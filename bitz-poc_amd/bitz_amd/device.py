@@ -435,3 +435,16 @@ class Commitment:
         _lib.call("bitz_prove_standalone", self.ctx.ptr, self._h, _vp(_b16(claimed)),
                   _vp((C.c_uint8 * capacity).from_buffer(buf)), capacity, C.byref(n))
         return bytes(memoryview(buf)[: n.value])
+
+
+def verify_standalone(ctx: Context, root: bytes, t: int, s: int, log_inv_rate: int, initial_k: int,
+                      claimed: int, proof: bytes) -> None:
+    """verify_standalone (src/piop/spartan/protocol/bitz_opener.rs:709-747):
+    returns when the proof is accepted, raises BitzError (code -5, "VERIFY",
+    with the reason) when it is rejected."""
+    if len(root) != 32:
+        raise ValueError("root must be 32 bytes")
+    rb = (C.c_uint8 * 32).from_buffer_copy(root)
+    pb = (C.c_uint8 * max(len(proof), 1)).from_buffer_copy(proof if proof else b"\0")
+    _lib.call("bitz_verify_standalone", ctx.ptr, _vp(rb), t, s, log_inv_rate, initial_k, _vp(_b16(claimed)),
+              _vp(pb), len(proof))

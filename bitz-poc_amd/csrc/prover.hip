@@ -11,6 +11,7 @@
 // the same BLAKE3 framing (the reference's spongefish 0.7.4 sponge is not
 // available to restate).
 #include <algorithm>
+#include <array>
 #include <set>
 #include <thread>
 
@@ -329,6 +330,101 @@ G smallest_generator() {
     }
 }
 
+// The statement len || a || len || b || target (LinearClaim encoding,
+// src/bitz/params.rs:273-285) absorbed as one public message, with a and b
+// (na, nb field elements) in device memory: its whole chunks are hashed on
+// the device as aligned subtree runs, the rest on the host. T is the prover's
+// or the verifier's inner transcript.
+template <class T>
+int public_weights_dev(bitz_ctx* ctx, T& it, const void* a, uint64_t na, const void* b, uint64_t nb, G target) {
+    const size_t len = 16 * (na + nb) + 32;
+    int rc = ctx->scratch.ensure(64);  // the pinned header staging
+    if (rc) return rc;
+    const Transcript::SlicePlan plan = it.public_plan(len);
+    const size_t pad = (4 - plan.lead % 4) % 4;  // chunk data 4-byte aligned
+    DevBuf msg, cvs, roots;
+    const std::vector<int> ks = blake3_subtree_runs(plan.c0, plan.chunks);
+    if (!msg.alloc(pad + len + 64) || !cvs.alloc(32 * std::max<uint64_t>(plan.chunks, 1)) ||
+        !roots.alloc(32 * std::max<size_t>(ks.size(), 1)))
+        return BITZ_ERR_ALLOC;
+    uint8_t* m = (uint8_t*)msg.ptr + pad;
+    uint8_t* hdr = (uint8_t*)ctx->scratch.host;  // pinned: the copies below stay asynchronous
+    uint64_t l = na;
+    memcpy(hdr, &l, 8);
+    l = nb;
+    memcpy(hdr + 8, &l, 8);
+    gput(target, hdr + 16);
+    BITZ_HIP(hipMemcpyAsync(m, hdr, 8, hipMemcpyHostToDevice, ctx->stream));
+    BITZ_HIP(hipMemcpyAsync(m + 8, a, 16 * na, hipMemcpyDeviceToDevice, ctx->stream));
+    BITZ_HIP(hipMemcpyAsync(m + 8 + 16 * na, hdr + 8, 8, hipMemcpyHostToDevice, ctx->stream));
+    BITZ_HIP(hipMemcpyAsync(m + 16 + 16 * na, b, 16 * nb, hipMemcpyDeviceToDevice, ctx->stream));
+    BITZ_HIP(hipMemcpyAsync(m + 16 + 16 * (na + nb), hdr + 16, 16, hipMemcpyHostToDevice, ctx->stream));
+    if ((rc = blake3_runs(ctx, m + plan.lead, plan.chunks, plan.c0, ks, cvs.ptr, roots.ptr))) return rc;
+    const size_t tail = len - plan.lead - 1024 * plan.chunks;
+    if ((rc = ctx->scratch.ensure_stage(plan.lead + tail + 32 * ks.size()))) return rc;
+    uint8_t* st = (uint8_t*)ctx->scratch.stage;
+    BITZ_HIP(hipMemcpyAsync(st, m, plan.lead, hipMemcpyDeviceToHost, ctx->stream));
+    BITZ_HIP(hipMemcpyAsync(st + plan.lead, m + plan.lead + 1024 * plan.chunks, tail, hipMemcpyDeviceToHost,
+                            ctx->stream));
+    if (!ks.empty())
+        BITZ_HIP(hipMemcpyAsync(st + plan.lead + tail, roots.ptr, 32 * ks.size(), hipMemcpyDeviceToHost,
+                                ctx->stream));
+    BITZ_HIP(hipStreamSynchronize(ctx->stream));
+    std::vector<uint32_t> run_cvs(8 * ks.size());
+    if (!ks.empty()) memcpy(run_cvs.data(), st + plan.lead + tail, 32 * ks.size());
+    it.public_message_runs(len, plan, st, run_cvs.data(), st + plan.lead);
+    return BITZ_OK;
+}
+
+// The outer statement of a standalone opening and the Round-0 header, up to
+// its grinding (src/ligerito_flock.rs:1255-1284, 1520-1557)
+void standalone_statement(Transcript& outer, const uint8_t root[32], const LigConfig& cfg, int t, int s, G g) {
+    outer.absorb_slice(std::string("bitz/ligerito-flock/standalone-mod-q-statement/v1"));
+    outer.absorb_slice(root, 32);
+    std::vector<uint64_t> frame = {(uint64_t)cfg.m, (uint64_t)cfg.initial_k, (uint64_t)cfg.yr_log_n,
+                                   (uint64_t)cfg.target_bits};
+    for (auto& lv : cfg.levels)
+        for (int x : {lv.log_inv_rate, lv.log_msg_cols, lv.log_num_interleaved, lv.k_recursive, lv.queries,
+                      lv.grinding_bits, lv.fold_grinding_bits, lv.ood_samples})
+            frame.push_back((uint64_t)x);
+    outer.absorb_slice(frame.data(), frame.size() * 8);
+    uint64_t ts[2] = {(uint64_t)t, (uint64_t)s};
+    outer.absorb_slice(ts, 16);
+    uint8_t gb[16];
+    gput(g, gb);
+    outer.absorb_slice(gb, 16);
+    const int q_bits = std::min(113, 126 - t);  // standalone_q_bits (src/ligerito_flock.rs:1286-1293)
+    outer.absorb_u64((uint64_t)q_bits);
+    outer.absorb_u64((uint64_t)cfg.round0_grinding_bits + 1);
+    outer.absorb_slice(std::string("bitz/bitz-opener/standalone/v1"));
+    outer.absorb_slice(std::string("bitz/core/ood-round/v1"));
+    outer.absorb_u64((uint64_t)(t + s - 7));
+    uint32_t r0bits = (uint32_t)cfg.round0_grinding_bits;
+    outer.absorb_slice(&r0bits, 4);
+}
+
+// s_k(v_k) for k = 0..log_n and their inverses (eval_sk_at_vks,
+// ligerito.rs:1720-1745): s_{i+1}(x) = s_i(x)^2 + s_i(v_i) s_i(x), v_i = X^i
+void sk_at_vks(int log_n, std::vector<G>* sks, std::vector<G>* inv) {
+    sks->assign(log_n + 1, gzero());
+    (*sks)[0] = gone();
+    std::vector<G> layer(log_n);
+    for (int i = 1; i <= log_n; i++) layer[i - 1] = G{1ull << i, 0};
+    int c = log_n;
+    for (int i = 0; i < log_n; i++) {
+        for (int j = 0; j < c; j++) {
+            G sv = layer[j];
+            G v = hgf_add(hgf_mul(sv, sv), hgf_mul((*sks)[i], sv));
+            if (j == 0)
+                (*sks)[i + 1] = v;
+            else
+                layer[j - 1] = v;
+        }
+        c--;
+    }
+    batch_inverse(*sks, inv);
+}
+
 // -------------------------------------------------------------- proof writer
 struct Writer {
     std::vector<uint8_t> b;
@@ -402,43 +498,7 @@ struct Prover {
     // arrays of field elements): bytes assembled in device memory, whole
     // chunks hashed there (blake3_runs), lead and tail bytes on the host.
     int public_weights(InnerTranscript& it, const void* a, uint64_t na, const void* b, uint64_t nb, G target) {
-        const size_t len = 16 * (na + nb) + 32;
-        int rc = ctx->scratch.ensure(64);  // the pinned header staging
-        if (rc) return rc;
-        const Transcript::SlicePlan plan = it.public_plan(len);
-        const size_t pad = (4 - plan.lead % 4) % 4;  // chunk data 4-byte aligned
-        DevBuf msg, cvs, roots;
-        const std::vector<int> ks = blake3_subtree_runs(plan.c0, plan.chunks);
-        if (!msg.alloc(pad + len + 64) || !cvs.alloc(32 * std::max<uint64_t>(plan.chunks, 1)) ||
-            !roots.alloc(32 * std::max<size_t>(ks.size(), 1)))
-            return BITZ_ERR_ALLOC;
-        uint8_t* m = (uint8_t*)msg.ptr + pad;
-        uint8_t* hdr = (uint8_t*)ctx->scratch.host;  // pinned: the copies below stay asynchronous
-        uint64_t l = na;
-        memcpy(hdr, &l, 8);
-        l = nb;
-        memcpy(hdr + 8, &l, 8);
-        gput(target, hdr + 16);
-        BITZ_HIP(hipMemcpyAsync(m, hdr, 8, hipMemcpyHostToDevice, ctx->stream));
-        BITZ_HIP(hipMemcpyAsync(m + 8, a, 16 * na, hipMemcpyDeviceToDevice, ctx->stream));
-        BITZ_HIP(hipMemcpyAsync(m + 8 + 16 * na, hdr + 8, 8, hipMemcpyHostToDevice, ctx->stream));
-        BITZ_HIP(hipMemcpyAsync(m + 16 + 16 * na, b, 16 * nb, hipMemcpyDeviceToDevice, ctx->stream));
-        BITZ_HIP(hipMemcpyAsync(m + 16 + 16 * (na + nb), hdr + 16, 16, hipMemcpyHostToDevice, ctx->stream));
-        if ((rc = blake3_runs(ctx, m + plan.lead, plan.chunks, plan.c0, ks, cvs.ptr, roots.ptr))) return rc;
-        const size_t tail = len - plan.lead - 1024 * plan.chunks;
-        if ((rc = ctx->scratch.ensure_stage(plan.lead + tail + 32 * ks.size()))) return rc;
-        uint8_t* st = (uint8_t*)ctx->scratch.stage;
-        BITZ_HIP(hipMemcpyAsync(st, m, plan.lead, hipMemcpyDeviceToHost, ctx->stream));
-        BITZ_HIP(hipMemcpyAsync(st + plan.lead, m + plan.lead + 1024 * plan.chunks, tail, hipMemcpyDeviceToHost,
-                                ctx->stream));
-        if (!ks.empty())
-            BITZ_HIP(hipMemcpyAsync(st + plan.lead + tail, roots.ptr, 32 * ks.size(), hipMemcpyDeviceToHost,
-                                    ctx->stream));
-        BITZ_HIP(hipStreamSynchronize(ctx->stream));
-        std::vector<uint32_t> run_cvs(8 * ks.size());
-        if (!ks.empty()) memcpy(run_cvs.data(), st + plan.lead + tail, 32 * ks.size());
-        it.public_message_runs(len, plan, st, run_cvs.data(), st + plan.lead);
-        return BITZ_OK;
+        return public_weights_dev(ctx, it, a, na, b, nb, target);
     }
 
     // Round 0 value y = MLE[p_msg](point) (ood_eval, src/ligerito_flock.rs:1459-1486)
@@ -1039,31 +1099,8 @@ int Prover::run(const uint8_t* claimed_in, uint8_t* claimed_out, std::vector<uin
     phase.mark(proof_out ? "statement_round0" : "eval_statement_round0");
     // ---------------- outer transcript: statement and Round 0 ----------------
     Transcript outer;
-    outer.absorb_slice(std::string("bitz/ligerito-flock/standalone-mod-q-statement/v1"));
-    outer.absorb_slice(h->root, 32);
-    {
-        std::vector<uint64_t> frame = {(uint64_t)cfg.m, (uint64_t)cfg.initial_k, (uint64_t)cfg.yr_log_n,
-                                       (uint64_t)cfg.target_bits};
-        for (auto& lv : cfg.levels)
-            for (int x : {lv.log_inv_rate, lv.log_msg_cols, lv.log_num_interleaved, lv.k_recursive, lv.queries,
-                          lv.grinding_bits, lv.fold_grinding_bits, lv.ood_samples})
-                frame.push_back((uint64_t)x);
-        outer.absorb_slice(frame.data(), frame.size() * 8);
-        uint64_t ts[2] = {(uint64_t)t, (uint64_t)s};
-        outer.absorb_slice(ts, 16);
-        uint8_t gb[16];
-        gput(g, gb);
-        outer.absorb_slice(gb, 16);
-    }
-    const int q_bits = std::min(113, 126 - t);  // standalone_q_bits (src/ligerito_flock.rs:1286-1293)
-    outer.absorb_u64((uint64_t)q_bits);
-    outer.absorb_u64((uint64_t)cfg.round0_grinding_bits + 1);
-    outer.absorb_slice(std::string("bitz/bitz-opener/standalone/v1"));
-    // Round 0 (prove_ood_round_packed, src/ligerito_flock.rs:1520-1557)
-    outer.absorb_slice(std::string("bitz/core/ood-round/v1"));
-    outer.absorb_u64((uint64_t)mp);
-    uint32_t r0bits = (uint32_t)cfg.round0_grinding_bits;
-    outer.absorb_slice(&r0bits, 4);
+    standalone_statement(outer, h->root, cfg, t, s, g);
+    const uint32_t r0bits = (uint32_t)cfg.round0_grinding_bits;
     uint64_t r0_nonce = 0;
     if (r0bits) {
         uint8_t seed[16];
@@ -1090,7 +1127,7 @@ int Prover::run(const uint8_t* claimed_in, uint8_t* claimed_out, std::vector<uin
     phase.mark(proof_out ? "instance" : "eval_instance");
     // ---------------- the standalone instance ----------------
     u128 q;
-    if ((rc = sample_prime(outer, q_bits, &q))) return rc;
+    if ((rc = sample_prime(outer, std::min(113, 126 - t), &q))) return rc;
     std::vector<u128> r1(t), r2(s);
     for (auto& x : r1) x = sample_residue(outer, q);
     for (auto& x : r2) x = sample_residue(outer, q);
@@ -1427,25 +1464,8 @@ int Prover::run(const uint8_t* claimed_in, uint8_t* claimed_out, std::vector<uin
         if (!qdev.alloc(queries.size() * 8 + 16)) return BITZ_ERR_ALLOC;
         if ((r_ = upload_async(ctx, queries.data(), queries.size() * 8, qdev.ptr))) return r_;
         // s_k(v_k) and inverses (eval_sk_at_vks)
-        std::vector<G> sks(log_n + 1, gzero()), inv(log_n + 1);
-        sks[0] = gone();
-        {
-            std::vector<G> layer(log_n);
-            for (int i = 1; i <= log_n; i++) layer[i - 1] = G{1ull << i, 0};
-            int c_ = log_n;
-            for (int i = 0; i < log_n; i++) {
-                for (int j = 0; j < c_; j++) {
-                    G sv = layer[j];
-                    G v = hgf_add(hgf_mul(sv, sv), hgf_mul(sks[i], sv));
-                    if (j == 0)
-                        sks[i + 1] = v;
-                    else
-                        layer[j - 1] = v;
-                }
-                c_--;
-            }
-        }
-        batch_inverse(sks, &inv);
+        std::vector<G> sks, inv;
+        sk_at_vks(log_n, &sks, &inv);
         std::vector<G> both(sks);
         both.insert(both.end(), inv.begin(), inv.end());
         if ((r_ = upload_gf(both, tmp2))) return r_;
@@ -1560,6 +1580,823 @@ int Prover::run(const uint8_t* claimed_in, uint8_t* claimed_out, std::vector<uin
     return BITZ_OK;
 }
 
+// ------------------------------------------------------------- the verifier
+// verify_standalone (src/piop/spartan/protocol/bitz_opener.rs:709-747) ->
+// BitZVerifier::verify (src/bitz/mod.rs:218-255) -> the recursive Ligerito
+// verifier (pcs/ligerito.rs:4585-4760) with the residual evaluated densely.
+// Transcript checks run on the host; the dense tables (eq tables, images,
+// the ring-switch basis, the induced bases and their folds) are built by the
+// prover's kernels on the context's device.
+#define V_CHECK(cond, msg)                                  \
+    do {                                                    \
+        if (!(cond)) {                                      \
+            set_error(std::string("verify: ") + (msg));     \
+            return BITZ_ERR_VERIFY;                         \
+        }                                                   \
+    } while (0)
+
+// the inner transcript on the verifier's side: prover messages are read
+// from the narg string and absorbed
+struct InnerReader {
+    Transcript t;
+    const uint8_t* narg;
+    size_t len, pos = 0;
+    InnerReader(const std::string& session, const uint8_t* inst, size_t ilen, const uint8_t* n, size_t l)
+        : narg(n), len(l) {
+        t.absorb_slice(std::string("bitz/v1"));
+        t.absorb_slice(session);
+        t.absorb_slice(inst, ilen);
+    }
+    void public_message(const void* d, size_t n) { t.absorb_slice(d, n); }
+    void public_message(const std::string& s) { t.absorb_slice(s); }
+    Transcript::SlicePlan public_plan(size_t l) const { return t.slice_plan(l); }
+    void public_message_runs(size_t l, const Transcript::SlicePlan& plan, const uint8_t* lead, const uint32_t* cvs,
+                             const uint8_t* tail) {
+        t.absorb_slice_runs(l, plan, lead, cvs, tail);
+    }
+    const uint8_t* read(size_t n) {
+        if (n > len - pos) return nullptr;
+        const uint8_t* p = narg + pos;
+        t.absorb_slice(p, n);
+        pos += n;
+        return p;
+    }
+    // count prover messages of `each` bytes
+    const uint8_t* read_many(size_t count, size_t each) {
+        if (count > (len - pos) / std::max<size_t>(each, 1)) return nullptr;
+        const uint8_t* p = narg + pos;
+        t.absorb_slices(p, count, each);
+        pos += count * each;
+        return p;
+    }
+    G gf() {
+        uint8_t b[16];
+        t.challenge(b, 16);
+        return gfrom(b);
+    }
+};
+
+struct ByteReader {
+    const uint8_t* p;
+    size_t n, pos = 0;
+    bool ok = true;
+    const uint8_t* take(size_t k) {
+        if (!ok || k > n - pos) {
+            ok = false;
+            return nullptr;
+        }
+        const uint8_t* r = p + pos;
+        pos += k;
+        return r;
+    }
+    uint64_t u64() {
+        const uint8_t* r = take(8);
+        uint64_t v = 0;
+        if (r) memcpy(&v, r, 8);
+        return v;
+    }
+    G gf() {
+        const uint8_t* r = take(16);
+        return r ? gfrom(r) : gzero();
+    }
+    // a count of items of `each` bytes that the remaining bytes can hold
+    bool count(uint64_t c, size_t each) { return ok && c <= (n - pos) / std::max<size_t>(each, 1); }
+};
+
+struct OpenData {
+    std::vector<std::vector<G>> rows;
+    std::vector<uint8_t> proof;  // 32-byte nodes
+};
+
+bool read_open(ByteReader& rd, OpenData* o) {
+    const uint64_t nq = rd.u64();
+    if (!rd.count(nq, 8)) return false;
+    o->rows.resize(nq);
+    for (auto& row : o->rows) {
+        const uint64_t lanes = rd.u64();
+        if (!rd.count(lanes, 16)) return false;
+        row.resize(lanes);
+        for (auto& v : row) v = rd.gf();
+    }
+    const uint64_t np = rd.u64();
+    if (!rd.count(np, 32)) return false;
+    const uint8_t* b = rd.take(32 * np);
+    if (!b && np) return false;
+    o->proof.assign(b, b + 32 * np);
+    return rd.ok;
+}
+
+// the Ligerito proof of the hint stream (field order of ligerito.rs:1528-1551)
+struct LigProofData {
+    uint8_t initial_root[32];
+    OpenData initial, fin;
+    std::vector<std::array<uint8_t, 32>> roots;
+    std::vector<OpenData> rec;
+    std::vector<G> yr, ood;
+    std::vector<std::pair<G, G>> msgs;
+    std::vector<uint64_t> grind, fold_nonces;
+};
+
+bool parse_ligerito(const uint8_t* b, size_t n, LigProofData* p) {
+    ByteReader rd{b, n};
+    const uint8_t* r = rd.take(32);
+    if (!r) return false;
+    memcpy(p->initial_root, r, 32);
+    if (!read_open(rd, &p->initial)) return false;
+    uint64_t c = rd.u64();
+    if (!rd.count(c, 32)) return false;
+    p->roots.resize(c);
+    for (auto& x : p->roots) memcpy(x.data(), rd.take(32), 32);
+    c = rd.u64();
+    if (!rd.count(c, 16)) return false;
+    p->rec.resize(c);
+    for (auto& o : p->rec)
+        if (!read_open(rd, &o)) return false;
+    c = rd.u64();
+    if (!rd.count(c, 16)) return false;
+    p->yr.resize(c);
+    for (auto& v : p->yr) v = rd.gf();
+    if (!read_open(rd, &p->fin)) return false;
+    c = rd.u64();
+    if (!rd.count(c, 32)) return false;
+    p->msgs.resize(c);
+    for (auto& m : p->msgs) {
+        m.first = rd.gf();
+        m.second = rd.gf();
+    }
+    c = rd.u64();
+    if (!rd.count(c, 8)) return false;
+    p->grind.resize(c);
+    for (auto& v : p->grind) v = rd.u64();
+    c = rd.u64();
+    if (!rd.count(c, 16)) return false;
+    p->ood.resize(c);
+    for (auto& v : p->ood) v = rd.gf();
+    c = rd.u64();
+    if (!rd.count(c, 8)) return false;
+    p->fold_nonces.resize(c);
+    for (auto& v : p->fold_nonces) v = rd.u64();
+    return rd.ok && rd.pos == n;
+}
+
+// BLAKE3 chaining value of a leaf of at most 1024 bytes hashed as chunk 0
+// without the ROOT flag (blake3_leaf_cv, merkle.rs:192-196)
+void leaf_cv(const uint8_t* data, size_t len, uint8_t out[32]) {
+    static const uint32_t kIV[8] = {0x6A09E667u, 0xBB67AE85u, 0x3C6EF372u, 0xA54FF53Au,
+                                    0x510E527Fu, 0x9B05688Cu, 0x1F83D9ABu, 0x5BE0CD19u};
+    uint32_t cv[8], o[16];
+    memcpy(cv, kIV, 32);
+    const size_t blocks = len ? (len + 63) / 64 : 1;
+    for (size_t b = 0; b < blocks; b++) {
+        uint8_t block[64] = {0};
+        const size_t bl = std::min<size_t>(64, len - 64 * b);
+        if (len) memcpy(block, data + 64 * b, bl);
+        const uint32_t flags = (b == 0 ? 1u : 0u) | (b + 1 == blocks ? 2u : 0u);  // CHUNK_START, CHUNK_END
+        blake3_compress_host(cv, block, 0, (uint32_t)(len ? bl : 0), flags, o);
+        memcpy(cv, o, 32);
+    }
+    memcpy(out, cv, 32);
+}
+
+// verify_merkle_multi_proof (merkle.rs:749-828): sorted distinct leaf
+// positions, their chaining values, and the sibling nodes the octopus proof
+// lists level by level
+bool merkle_verify_multi(const uint8_t root[32], uint64_t num_leaves, const std::vector<uint64_t>& pos,
+                         std::vector<std::array<uint8_t, 32>> hashes, const std::vector<uint8_t>& proof) {
+    if (num_leaves == 0 || (num_leaves & (num_leaves - 1))) return false;
+    if (pos.empty()) return proof.empty();
+    for (size_t i = 0; i < pos.size(); i++)
+        if (pos[i] >= num_leaves || (i && pos[i - 1] >= pos[i])) return false;
+    std::vector<uint64_t> ap(pos);
+    size_t used = 0;
+    const size_t n_proof = proof.size() / 32;
+    for (uint64_t level = num_leaves; level > 1; level >>= 1) {
+        std::vector<uint64_t> np;
+        std::vector<std::array<uint8_t, 32>> nh;
+        for (size_t i = 0; i < ap.size();) {
+            const uint64_t p = ap[i];
+            uint32_t l[8], r[8], c[8];
+            if (i + 1 < ap.size() && ap[i + 1] == (p ^ 1)) {
+                memcpy(l, hashes[i].data(), 32);
+                memcpy(r, hashes[i + 1].data(), 32);
+                i += 2;
+            } else {
+                if (used >= n_proof) return false;
+                const uint8_t* sib = proof.data() + 32 * used++;
+                if ((p & 1) == 0) {
+                    memcpy(l, hashes[i].data(), 32);
+                    memcpy(r, sib, 32);
+                } else {
+                    memcpy(l, sib, 32);
+                    memcpy(r, hashes[i].data(), 32);
+                }
+                i += 1;
+            }
+            blake3_parent_cv(l, r, c);
+            std::array<uint8_t, 32> h;
+            memcpy(h.data(), c, 32);
+            np.push_back(p >> 1);
+            nh.push_back(h);
+        }
+        ap.swap(np);
+        hashes.swap(nh);
+    }
+    return used == n_proof && memcmp(hashes[0].data(), root, 32) == 0;
+}
+
+// BLAKE3(prefix16 || seed16 || nonce_le) has at least `bits` leading zero
+// bits (bytes in order, most significant bit first)
+bool pow_valid(const uint8_t prefix[16], const uint8_t seed[16], uint64_t nonce, uint32_t bits) {
+    uint8_t m[40], h[32];
+    memcpy(m, prefix, 16);
+    memcpy(m + 16, seed, 16);
+    memcpy(m + 32, &nonce, 8);
+    blake3_hash_host(m, 40, h);
+    uint32_t z = 0;
+    for (int i = 0; i < 32 && z < bits; i++) {
+        if (h[i] == 0) {
+            z += 8;
+            continue;
+        }
+        z += (uint32_t)__builtin_clz((unsigned)h[i]) - 24;
+        break;
+    }
+    return z >= bits;
+}
+
+struct Verifier {
+    bitz_ctx* ctx;
+    DevBuf scratch_a, scratch_b, pt;
+
+    int download(const void* dev, void* host, size_t bytes) {
+        BITZ_HIP(hipMemcpyAsync(host, dev, bytes, hipMemcpyDeviceToHost, ctx->stream));
+        BITZ_HIP(hipStreamSynchronize(ctx->stream));
+        return BITZ_OK;
+    }
+    int eq_into(const std::vector<G>& r, DevBuf& buf) {
+        BITZ_REQUIRE((int)r.size() <= kMaxTensorN, BITZ_ERR_SHAPE, "verify: eq table over too many coordinates");
+        if (!buf.alloc(16ull << r.size())) return BITZ_ERR_ALLOC;
+        return eq_args(ctx, r.data(), (int)r.size(), buf.ptr);
+    }
+    int dot_value(const void* a, const void* b, uint64_t n, G* out) {
+        int rc = dot(ctx, a, b, n, nullptr);
+        if (rc) return rc;
+        uint8_t v[16];
+        if ((rc = read_result(ctx, 1, v))) return rc;
+        *out = gfrom(v);
+        return BITZ_OK;
+    }
+    // v (2^n entries) folded LSB first by rs (fold_lsb, ligerito.rs:2458-2497)
+    // into 2^(n - |rs|) host values
+    int fold_chain(const void* v, uint64_t n_entries, const std::vector<G>& rs, size_t first, std::vector<G>* out) {
+        uint64_t n = n_entries;
+        BITZ_REQUIRE(rs.size() - first < 64 && (n >> (rs.size() - first)) >= 1, BITZ_ERR_SHAPE, "verify: fold chain");
+        if (!scratch_a.alloc(16 * std::max<uint64_t>(n / 2, 1)) || !scratch_b.alloc(16 * std::max<uint64_t>(n / 4, 1)))
+            return BITZ_ERR_ALLOC;
+        const void* cur = v;
+        int rc;
+        for (size_t k = first; k < rs.size(); k++) {
+            uint8_t rb[16];
+            gput(rs[k], rb);
+            void* dst = ((k - first) & 1) ? scratch_b.ptr : scratch_a.ptr;
+            if ((rc = fold_lsb(ctx, cur, n, rb, dst))) return rc;
+            cur = dst;
+            n >>= 1;
+        }
+        out->resize(n);
+        return download(cur, out->data(), 16 * n);
+    }
+};
+
+int verify_standalone_impl(bitz_ctx* ctx, const uint8_t root[32], int t, int s, int r0, int k0, u128 claimed,
+                           const uint8_t* proof, size_t plen) {
+    int rc;
+    const int m = t + s, mp = m - 7;
+    LigConfig cfg;
+    {
+        std::string err = custom_johnson_config(m, r0, k0, 100, &cfg);
+        BITZ_REQUIRE(err.empty(), BITZ_ERR_SHAPE, "Ligerito configuration: " + err);
+        BITZ_REQUIRE(t >= 7 && s >= 1 && cfg.levels.size() >= 2 && mp <= kMaxTensorN, BITZ_ERR_SHAPE,
+                     "verify: unsupported shape");
+    }
+    static G gen = smallest_generator();
+    const G g = gen;
+    Verifier V{ctx};
+    // ---- proof tail (BitZOpeningProof::from_bytes, bitz_opener.rs:325-372) ----
+    ByteReader pr{proof, plen};
+    const uint64_t narg_len = pr.u64();
+    V_CHECK(pr.count(narg_len, 1), "truncated proof");
+    const uint8_t* narg = pr.take(narg_len);
+    const uint64_t hints_len = pr.u64();
+    V_CHECK(pr.count(hints_len, 1), "truncated proof");
+    const uint8_t* hints = pr.take(hints_len);
+    const uint8_t* tag = pr.take(1);
+    V_CHECK(tag && tag[0] == 1, "missing Round-0 claim");
+    const uint8_t* yb = pr.take(16);
+    V_CHECK(yb != nullptr, "truncated proof");
+    const G ood_y = gfrom(yb);
+    const uint8_t* tag2 = pr.take(1);
+    V_CHECK(tag2 && (tag2[0] == 0 || tag2[0] == 1), "truncated proof");
+    bool has_nonce = tag2[0] == 1;
+    uint64_t r0_nonce = has_nonce ? pr.u64() : 0;
+    V_CHECK(pr.ok && pr.pos == plen, "trailing proof bytes");
+    // ---- outer transcript: statement, Round 0, instance ----
+    Transcript outer;
+    standalone_statement(outer, root, cfg, t, s, g);
+    const uint32_t r0bits = (uint32_t)cfg.round0_grinding_bits;
+    if (r0bits) {
+        uint8_t seed[16];
+        outer.challenge(seed, 16);
+        static const uint8_t kNative[16] = {'b', 'i', 't', 'z', '-', 'n', 'a', 't', 'i', 'v', 'e', '-', 'p', 'o', 'w', '1'};
+        V_CHECK(has_nonce && pow_valid(kNative, seed, r0_nonce, r0bits), "Round-0 proof of work");
+        outer.absorb_u64(r0_nonce);
+    } else {
+        V_CHECK(!has_nonce, "unexpected Round-0 nonce");
+    }
+    uint8_t zb[16];
+    outer.challenge(zb, 16);
+    std::vector<G> opt(mp);
+    {
+        G cur = gfrom(zb);
+        for (int i = 0; i < mp; i++) {
+            opt[i] = cur;
+            cur = hgf_mul(cur, cur);
+        }
+    }
+    outer.absorb_slice(yb, 16);
+    u128 q;
+    if ((rc = sample_prime(outer, std::min(113, 126 - t), &q))) return rc;
+    std::vector<u128> r1(t), r2(s);
+    for (auto& x : r1) x = sample_residue(outer, q);
+    for (auto& x : r2) x = sample_residue(outer, q);
+    V_CHECK(claimed < q, "claim not reduced");
+    outer.absorb_slice(std::string("bitz/ligerito-flock/standalone-mod-q-claim/v1"));
+    {
+        uint8_t qc[32];
+        u128_put(q, qc);
+        u128_put(claimed, qc + 16);
+        outer.absorb_slice(qc, 32);
+    }
+    uint8_t tagb[32];
+    outer.challenge(tagb, 16);
+    outer.challenge(tagb + 16, 16);
+    // ---- inner transcript: instance, fold round ----
+    InnerReader it("bitz/bitz-opener/standalone/v1", tagb, 32, narg, narg_len);
+    it.public_message(root, 32);
+    {
+        uint8_t frame[48];
+        uint64_t tt = t, ss = s;
+        memcpy(frame, &tt, 8);
+        memcpy(frame + 8, &ss, 8);
+        u128_put(q, frame + 16);
+        gput(g, frame + 32);
+        it.public_message(frame, 48);
+    }
+    const uint64_t nrows = 1ull << t, ncols = 1ull << s;
+    DevBuf roww, colw, folds, img, tmp, u1, u2;
+    if (!roww.alloc(nrows * 16) || !colw.alloc(ncols * 16) || !folds.alloc(ncols * 16) || !img.alloc(nrows * 16))
+        return BITZ_ERR_ALLOC;
+    if ((rc = eq_table_mod_q_dev(ctx, r1, q, roww.ptr))) return rc;
+    if ((rc = eq_table_mod_q_dev(ctx, r2, q, colw.ptr))) return rc;
+    G claimed_g;
+    {
+        uint8_t b[16];
+        u128_put(claimed, b);
+        claimed_g = gfrom(b);
+    }
+    if ((rc = public_weights_dev(ctx, it, roww.ptr, nrows, colw.ptr, ncols, claimed_g))) return rc;
+    // fold round (fold.rs receive_fold): range bound, reconstruction
+    const uint8_t* fb = it.read_many(ncols, 16);
+    V_CHECK(fb != nullptr, "narg string exhausted");
+    {
+        std::vector<uint8_t> cw(ncols * 16);
+        if ((rc = V.download(colw.ptr, cw.data(), cw.size()))) return rc;
+        ModN mq;
+        mq.set(q);
+        u128 acc = 0;
+        for (uint64_t c = 0; c < ncols; c++) {
+            const u128 f = u128_from(fb + 16 * c);
+            // f <= 2^t (q - 1)
+            const u128 hi = f >> t, lo = f & (((u128)1 << t) - 1);
+            V_CHECK(hi < q - 1 || (hi == q - 1 && lo == 0), "fold out of range");
+            acc = addmod(acc, mq.mont(f, u128_from(&cw[16 * c])), q);
+        }
+        V_CHECK(mq.to_mont(acc) == claimed, "folds do not reconstruct the claim");
+    }
+    std::vector<G> zeta(s);
+    for (auto& z : zeta) z = it.gf();
+    if ((rc = upload_async(ctx, fb, ncols * 16, folds.ptr))) return rc;
+    uint8_t gb[16];
+    gput(g, gb);
+    if ((rc = pow_images(ctx, gb, folds.ptr, ncols, img.ptr))) return rc;
+    G claim;
+    if ((rc = V.eq_into(zeta, tmp))) return rc;
+    if ((rc = V.dot_value(img.ptr, tmp.ptr, ncols, &claim))) return rc;
+    // ---- GKR (gpgkr_verify, gkr.rs:337-383) ----
+    std::vector<G> internal(zeta.rbegin(), zeta.rend());
+    for (int layer = 0; layer < t; layer++) {
+        G prefix = gone();
+        std::vector<G> nxt;
+        for (const G z : internal) {
+            const uint8_t* b = it.read(32);
+            V_CHECK(b != nullptr, "narg string exhausted");
+            const G se = gfrom(b), si = gfrom(b + 16);
+            G s0, s1;
+            if (geq(z, gzero())) {
+                s0 = claim;
+                s1 = se;
+            } else {
+                s0 = se;
+                s1 = hgf_mul(hgf_add(claim, hgf_mul(hgf_add(gone(), z), se)), hgf_inv(z));
+            }
+            const G rr = it.gf();
+            nxt.push_back(rr);
+            const G f = eq_factor(rr, z);
+            claim = hgf_mul(f, hgf_add(s0, hgf_mul(rr, hgf_add(hgf_add(s1, s0), hgf_mul(hgf_add(rr, gone()), si)))));
+            prefix = hgf_mul(prefix, f);
+        }
+        const uint8_t* b = it.read(32);
+        V_CHECK(b != nullptr, "narg string exhausted");
+        const G el = gfrom(b), er = gfrom(b + 16);
+        V_CHECK(geq(hgf_mul(prefix, hgf_mul(el, er)), claim), "GKR layer check");
+        const G rc_ = it.gf();
+        internal.clear();
+        internal.push_back(rc_);
+        internal.insert(internal.end(), nxt.begin(), nxt.end());
+        claim = hgf_add(el, hgf_mul(rc_, hgf_add(er, el)));
+    }
+    std::vector<G> point(internal.rbegin(), internal.rend());
+    std::vector<G> alpha_c(point.begin(), point.begin() + s), alpha_b(point.begin() + s, point.end());
+    // ---- inner-product statement (exit_from_terminal, reduce.rs:28-58) ----
+    if ((rc = pow_images(ctx, gb, roww.ptr, nrows, img.ptr))) return rc;
+    if ((rc = V.eq_into(alpha_b, tmp))) return rc;
+    if (!u1.alloc(nrows * 16)) return BITZ_ERR_ALLOC;
+    if ((rc = mul_elem(ctx, img.ptr, tmp.ptr, nrows, u1.ptr, 1))) return rc;
+    if ((rc = V.eq_into(alpha_c, u2))) return rc;
+    G target = hgf_add(claim, gone());
+    const uint64_t pcs40[5] = {(uint64_t)m, (uint64_t)r0, (uint64_t)k0, 0, 1};
+    it.public_message(std::string("bitz/pcs/bit-inner-product/v2"));
+    it.public_message(root, 32);
+    it.public_message(pcs40, 40);
+    if ((rc = public_weights_dev(ctx, it, u1.ptr, nrows, u2.ptr, ncols, target))) return rc;
+    it.public_message(std::string("bitz/pcs/inner-product-sumcheck/v1"));
+    std::vector<G> sc_point;
+    for (int round = 0; round < t + s; round++) {
+        const uint8_t* b = it.read(48);
+        V_CHECK(b != nullptr, "narg string exhausted");
+        const G c0 = gfrom(b), c1 = gfrom(b + 16), c2 = gfrom(b + 32);
+        V_CHECK(geq(hgf_add(c1, c2), target), "inner sumcheck round");
+        const G r = it.gf();
+        target = hgf_add(c0, hgf_mul(r, hgf_add(c1, hgf_mul(r, c2))));
+        sc_point.push_back(r);
+    }
+    std::vector<G> rows_w, cols_w;
+    if ((rc = V.fold_chain(u1.ptr, nrows, std::vector<G>(sc_point.begin(), sc_point.begin() + t), 0, &rows_w)))
+        return rc;
+    if ((rc = V.fold_chain(u2.ptr, ncols, std::vector<G>(sc_point.begin() + t, sc_point.end()), 0, &cols_w)))
+        return rc;
+    const uint8_t* evb = it.read(16);
+    V_CHECK(evb != nullptr, "narg string exhausted");
+    const G evaluation = gfrom(evb);
+    V_CHECK(geq(target, hgf_mul(evaluation, hgf_mul(rows_w[0], cols_w[0]))), "inner sumcheck final check");
+    it.public_message(std::string("bitz/pcs/mle-opening/v1"));
+    it.public_message(root, 32);
+    it.public_message(pcs40, 40);
+    {
+        uint64_t n = sc_point.size();
+        it.public_message(&n, 8);
+        for (auto& c : sc_point) {
+            uint8_t b[16];
+            gput(c, b);
+            it.public_message(b, 16);
+        }
+        it.public_message(evb, 16);
+    }
+    // ---- ring switch (verify_mle, pcs.rs:508-560) ----
+    const std::vector<G> prefix = host_eq(std::vector<G>(sc_point.begin(), sc_point.begin() + 7));
+    it.public_message(std::string("bitz/pcs/mle-claims/v1"));
+    const uint8_t* cb = it.read(128 * 16);
+    V_CHECK(cb != nullptr, "narg string exhausted");
+    std::vector<G> claims(128);
+    G cc = gzero();
+    for (int i = 0; i < 128; i++) {
+        claims[i] = gfrom(cb + 16 * i);
+        cc = hgf_add(cc, hgf_mul(prefix[i], claims[i]));
+    }
+    V_CHECK(geq(cc, evaluation), "ring-switch claim check");
+    it.public_message(std::string("bitz/pcs/ring-switch-challenges/v1"));
+    std::vector<G> batch_pt(7);
+    for (auto& x : batch_pt) x = it.gf();
+    const std::vector<G> bw = host_eq(batch_pt);
+    G packed_target = gzero();
+    for (int b = 0; b < 128; b++) {
+        G u = gzero();
+        for (int i = 0; i < 128; i++) {
+            const uint64_t bit = b < 64 ? (claims[i].lo >> b) & 1 : (claims[i].hi >> (b - 64)) & 1;
+            if (!bit) continue;
+            if (i < 64)
+                u.lo |= 1ull << i;
+            else
+                u.hi |= 1ull << (i - 64);
+        }
+        packed_target = hgf_add(packed_target, hgf_mul(u, bw[b]));
+    }
+    it.public_message(std::string("bitz/bitz/round0-claim/v1"));
+    {
+        uint64_t n = opt.size();
+        it.public_message(&n, 8);
+        for (auto& c : opt) {
+            uint8_t b[16];
+            gput(c, b);
+            it.public_message(b, 16);
+        }
+        it.public_message(yb, 16);
+    }
+    const G eta = it.gf();
+    packed_target = hgf_add(packed_target, hgf_mul(eta, ood_y));
+    // the dense initial basis b(x) = sum_i bit_i(suffix[x]) bw[i] + eta eq(opt)[x]
+    const uint64_t np = 1ull << mp;
+    DevBuf suffix, basis, bwd, eqo;
+    if ((rc = V.eq_into(std::vector<G>(sc_point.begin() + 7, sc_point.end()), suffix))) return rc;
+    if (!bwd.alloc(128 * 16) || !basis.alloc(np * 16)) return BITZ_ERR_ALLOC;
+    if ((rc = upload_async(ctx, bw.data(), 128 * 16, bwd.ptr))) return rc;
+    if ((rc = fold_b128(ctx, suffix.ptr, np, bwd.ptr, basis.ptr))) return rc;
+    if ((rc = V.eq_into(opt, eqo))) return rc;
+    {
+        uint8_t eb[16];
+        gput(eta, eb);
+        if ((rc = axpy(ctx, basis.ptr, eqo.ptr, np, eb))) return rc;
+    }
+    suffix = DevBuf();
+    eqo = DevBuf();
+    // ---- the Ligerito proof of the hint stream ----
+    V_CHECK(hints_len >= 4, "missing Ligerito hint");
+    uint32_t ln;
+    memcpy(&ln, hints, 4);
+    V_CHECK(4 + (uint64_t)ln == hints_len, "hint length");
+    LigProofData lp;
+    V_CHECK(parse_ligerito(hints + 4, ln, &lp), "malformed Ligerito proof");
+    const auto& lv = cfg.levels;
+    V_CHECK(memcmp(lp.initial_root, root, 32) == 0, "initial root");
+    size_t mi = 0, fni = 0, gni = 0, oi = 0;
+    bool first_obs = true;
+    auto observe = [&](G v) -> bool {
+        uint8_t b[16];
+        gput(v, b);
+        if (first_obs) {
+            first_obs = false;
+            it.public_message(b, 16);
+            return true;
+        }
+        const uint8_t* r = it.read(16);
+        return r && memcmp(r, b, 16) == 0;
+    };
+    struct Quad {
+        G c, b, a;
+    };
+    auto quad = [](std::pair<G, G> msg, G t_r) { return Quad{msg.first, hgf_add(t_r, msg.second), msg.second}; };
+    auto qeval = [](const Quad& qd, G r) { return hgf_add(qd.c, hgf_add(hgf_mul(r, qd.b), hgf_mul(hgf_mul(r, r), qd.a))); };
+    auto qfold = [](const Quad& p1, const Quad& p2, G al) {
+        return Quad{hgf_add(p1.c, hgf_mul(al, p2.c)), hgf_add(p1.b, hgf_mul(al, p2.b)), hgf_add(p1.a, hgf_mul(al, p2.a))};
+    };
+    static const uint8_t kPcsPow[16] = {'b', 'i', 't', 'z', '-', 'p', 'c', 's', '-', 'p', 'o', 'w', '-', 'v', '1', 0};
+    std::string fail;
+    auto next_msg = [&](std::pair<G, G>* m_) -> bool {
+        if (mi >= lp.msgs.size()) {
+            fail = "sumcheck transcript exhausted";
+            return false;
+        }
+        *m_ = lp.msgs[mi++];
+        if (!observe(m_->first) || !observe(m_->second)) {
+            fail = "observed value differs from the narg string";
+            return false;
+        }
+        return true;
+    };
+    auto check_pow = [&](const std::vector<uint64_t>& nonces, size_t* idx, uint32_t bits) -> bool {
+        it.public_message(std::string("pcs/flock/pow/v1"));
+        it.public_message(&bits, 4);
+        uint8_t seed[16];
+        it.t.challenge(seed, 16);
+        if (*idx >= nonces.size()) {
+            fail = "missing nonce";
+            return false;
+        }
+        const uint64_t nonce = nonces[(*idx)++];
+        const uint8_t* r = it.read(8);
+        uint64_t rn = 0;
+        if (r) memcpy(&rn, r, 8);
+        if (!r || rn != nonce || !pow_valid(kPcsPow, seed, nonce, bits)) {
+            fail = "proof of work";
+            return false;
+        }
+        return true;
+    };
+    auto sample_vec = [&](size_t n) {
+        it.public_message(std::string("pcs/flock/sample-vector/v1"));
+        uint64_t nn = n;
+        it.public_message(&nn, 8);
+        std::vector<G> v(n);
+        for (auto& x : v) x = it.gf();
+        return v;
+    };
+    auto sample_queries = [&](uint64_t block_len, int count) {
+        std::set<uint64_t> seen;
+        std::vector<uint64_t> out;
+        while ((int)out.size() < count) {
+            const uint64_t qq = it.gf().lo % block_len;
+            if (seen.insert(qq).second) out.push_back(qq);
+        }
+        std::sort(out.begin(), out.end());
+        return out;
+    };
+    auto check_root_bytes = [&](const uint8_t* expected) -> bool {
+        const uint8_t* b = it.read(36);
+        uint32_t l32 = 0;
+        if (b) memcpy(&l32, b, 4);
+        return b && l32 == 32 && memcmp(b + 4, expected, 32) == 0;
+    };
+    auto verify_open = [&](const OpenData& o, const uint8_t* root_, uint64_t block_len, uint64_t lanes,
+                           const std::vector<uint64_t>& queries) -> bool {
+        if (o.rows.size() != queries.size()) return false;
+        std::vector<std::array<uint8_t, 32>> leaves(o.rows.size());
+        std::vector<uint8_t> bytes;
+        for (size_t i = 0; i < o.rows.size(); i++) {
+            if (o.rows[i].size() != lanes) return false;
+            bytes.resize(16 * lanes);
+            for (uint64_t l = 0; l < lanes; l++) gput(o.rows[i][l], &bytes[16 * l]);
+            leaf_cv(bytes.data(), bytes.size(), leaves[i].data());
+        }
+        return merkle_verify_multi(root_, block_len, queries, leaves, o.proof);
+    };
+    auto enforced_sum = [&](const OpenData& o, const std::vector<G>& v_ch, const std::vector<uint64_t>& queries,
+                            const std::vector<G>& alpha) {
+        std::vector<G> aw = host_eq(alpha);
+        const std::vector<G> eqv = host_eq(v_ch);
+        G tot = gzero();
+        for (size_t i = 0; i < queries.size() && i < aw.size(); i++) {
+            G d = gzero();
+            for (size_t x = 0; x < o.rows[i].size() && x < eqv.size(); x++) d = hgf_add(d, hgf_mul(o.rows[i][x], eqv[x]));
+            tot = hgf_add(tot, hgf_mul(d, aw[i]));
+        }
+        return tot;
+    };
+    // combined basis components: (device table, entries, first challenge index, weight)
+    struct Comp {
+        std::unique_ptr<DevBuf> tab;
+        uint64_t n;
+        size_t start;
+        G w;
+    };
+    std::vector<Comp> comps;
+    comps.push_back(Comp{std::make_unique<DevBuf>(std::move(basis)), np, 0, gone()});
+    DevBuf qdev, sksd, awd;
+    auto induce_comp = [&](int log_n, const std::vector<uint64_t>& queries, const std::vector<G>& alpha, size_t start,
+                           G w) -> int {
+        std::vector<G> aw = host_eq(alpha);
+        aw.resize(queries.size());
+        std::vector<G> sks, inv;
+        sk_at_vks(log_n, &sks, &inv);
+        sks.insert(sks.end(), inv.begin(), inv.end());
+        if (!qdev.alloc(queries.size() * 8 + 16) || !sksd.alloc(sks.size() * 16) ||
+            !awd.alloc(std::max<size_t>(aw.size(), 1) * 16))
+            return BITZ_ERR_ALLOC;
+        int r_;
+        if ((r_ = upload_async(ctx, queries.data(), queries.size() * 8, qdev.ptr))) return r_;
+        if ((r_ = upload_async(ctx, sks.data(), sks.size() * 16, sksd.ptr))) return r_;
+        if ((r_ = upload_async(ctx, aw.data(), aw.size() * 16, awd.ptr))) return r_;
+        auto tab = std::make_unique<DevBuf>();
+        if (!tab->alloc(16ull << log_n)) return BITZ_ERR_ALLOC;
+        if ((r_ = induce_basis(ctx, (const uint64_t*)qdev.ptr, (uint32_t)queries.size(), log_n, sksd.ptr,
+                               (const uint4*)sksd.ptr + (log_n + 1), awd.ptr, tab->ptr)))
+            return r_;
+        comps.push_back(Comp{std::move(tab), 1ull << log_n, start, w});
+        return BITZ_OK;
+    };
+    it.public_message(std::string("flock-ligerito-basis-v0"));
+    V_CHECK(observe(packed_target), "observed value differs from the narg string");
+    V_CHECK(check_root_bytes(root), "root message");
+    G t_r = packed_target;
+    std::pair<G, G> msg;
+    V_CHECK(next_msg(&msg), fail);
+    Quad rq = quad(msg, t_r);
+    std::vector<G> ris;
+    for (int j = 0; j < k0; j++) {
+        const int bits = std::max(0, lv[0].fold_grinding_bits - j);
+        if (bits > 0) V_CHECK(check_pow(lp.fold_nonces, &fni, (uint32_t)bits), fail);
+        const G r = it.gf();
+        ris.push_back(r);
+        t_r = qeval(rq, r);
+        V_CHECK(next_msg(&msg), fail);
+        rq = quad(msg, t_r);
+    }
+    V_CHECK(!lp.roots.empty(), "missing recursive roots");
+    V_CHECK(check_root_bytes(lp.roots[0].data()), "root message");
+    int n_cur = mp - k0;
+    auto ood_mirror = [&](int level, int n_vars) -> int {
+        for (int o = 0; o < lv[level].ood_samples; o++) {
+            const std::vector<G> z = sample_vec((size_t)n_vars);
+            V_CHECK(oi < lp.ood.size(), "missing OOD value");
+            const G y = lp.ood[oi++];
+            V_CHECK(observe(y), "observed value differs from the narg string");
+            V_CHECK(next_msg(&msg), fail);
+            const Quad iq = quad(msg, y);
+            const G beta = it.gf();
+            rq = qfold(rq, iq, beta);
+            t_r = hgf_add(t_r, hgf_mul(beta, y));
+            auto tab = std::make_unique<DevBuf>();
+            int r_ = V.eq_into(z, *tab);
+            if (r_) return r_;
+            comps.push_back(Comp{std::move(tab), 1ull << n_vars, ris.size(), beta});
+        }
+        return BITZ_OK;
+    };
+    if ((rc = ood_mirror(1, n_cur))) return rc;
+    V_CHECK(check_pow(lp.grind, &gni, (uint32_t)lv[0].grinding_bits), fail);
+    const uint64_t block0 = 1ull << (lv[0].log_msg_cols + lv[0].log_inv_rate);
+    const std::vector<uint64_t> q0 = sample_queries(block0, lv[0].queries);
+    const std::vector<G> a0 = sample_vec((size_t)ceil_log2((uint64_t)lv[0].queries));
+    V_CHECK(verify_open(lp.initial, root, block0, 1ull << k0, q0), "Merkle multi-proof");
+    {
+        const G es = enforced_sum(lp.initial, ris, q0, a0);
+        V_CHECK(next_msg(&msg), fail);
+        const Quad iq = quad(msg, es);
+        const G beta = it.gf();
+        rq = qfold(rq, iq, beta);
+        t_r = hgf_add(t_r, hgf_mul(beta, es));
+        if ((rc = induce_comp(n_cur, q0, a0, ris.size(), beta))) return rc;
+    }
+    const uint8_t* prev_root = lp.roots[0].data();
+    int prev = 1;
+    const int r_steps = (int)lv.size() - 1;
+    size_t next_root = 1, reci = 0;
+    for (int i = 0; i < r_steps; i++) {
+        std::vector<G> level_rs;
+        for (int j = 0; j < lv[i + 1].k_recursive; j++) {
+            const int bits = std::max(0, lv[i + 1].fold_grinding_bits - j);
+            if (bits > 0) V_CHECK(check_pow(lp.fold_nonces, &fni, (uint32_t)bits), fail);
+            const G r = it.gf();
+            ris.push_back(r);
+            level_rs.push_back(r);
+            t_r = qeval(rq, r);
+            V_CHECK(next_msg(&msg), fail);
+            rq = quad(msg, t_r);
+        }
+        n_cur -= lv[i + 1].k_recursive;
+        const LigLevel& Lp = lv[prev];
+        const uint64_t block_prev = 1ull << (Lp.log_msg_cols + Lp.log_inv_rate);
+        if (i == r_steps - 1) {
+            V_CHECK(mi == lp.msgs.size() && oi == lp.ood.size() && fni == lp.fold_nonces.size(),
+                    "unconsumed Ligerito transcript");
+            V_CHECK(n_cur >= 0 && n_cur < 32 && lp.yr.size() == (1ull << n_cur), "yr length");
+            for (const G v : lp.yr) V_CHECK(observe(v), "observed value differs from the narg string");
+            V_CHECK(check_pow(lp.grind, &gni, (uint32_t)lv[i + 1].grinding_bits), fail);
+            const std::vector<uint64_t> ql = sample_queries(block_prev, lv[i + 1].queries);
+            const std::vector<G> al = sample_vec((size_t)ceil_log2((uint64_t)lv[i + 1].queries));
+            V_CHECK(verify_open(lp.fin, prev_root, block_prev, 1ull << Lp.log_num_interleaved, ql),
+                    "Merkle multi-proof");
+            const G esl = enforced_sum(lp.fin, level_rs, ql, al);
+            const G beta_l = it.gf();
+            t_r = hgf_add(t_r, hgf_mul(beta_l, esl));
+            if ((rc = induce_comp(n_cur, ql, al, ris.size(), beta_l))) return rc;
+            // dense residual: each component folded by the challenges after it
+            std::vector<G> combined(lp.yr.size(), gzero());
+            for (const Comp& c : comps) {
+                std::vector<G> vals;
+                if ((rc = V.fold_chain(c.tab->ptr, c.n, ris, c.start, &vals))) return rc;
+                V_CHECK(vals.size() == lp.yr.size(), "residual length");
+                for (size_t y = 0; y < vals.size(); y++) combined[y] = hgf_add(combined[y], hgf_mul(c.w, vals[y]));
+            }
+            G inner = gzero();
+            for (size_t y = 0; y < combined.size(); y++) inner = hgf_add(inner, hgf_mul(lp.yr[y], combined[y]));
+            V_CHECK(geq(inner, t_r), "Ligerito final check");
+            V_CHECK(gni == lp.grind.size(), "unconsumed grinding nonces");
+            V_CHECK(it.pos == it.len, "trailing narg bytes");
+            return BITZ_OK;
+        }
+        V_CHECK(next_root < lp.roots.size(), "missing recursive root");
+        const uint8_t* root_n = lp.roots[next_root++].data();
+        V_CHECK(check_root_bytes(root_n), "root message");
+        if ((rc = ood_mirror(i + 2, n_cur))) return rc;
+        V_CHECK(check_pow(lp.grind, &gni, (uint32_t)lv[i + 1].grinding_bits), fail);
+        const std::vector<uint64_t> qi = sample_queries(block_prev, lv[i + 1].queries);
+        const std::vector<G> ai = sample_vec((size_t)ceil_log2((uint64_t)lv[i + 1].queries));
+        V_CHECK(reci < lp.rec.size(), "missing recursive opening");
+        const OpenData& oi_ = lp.rec[reci++];
+        V_CHECK(verify_open(oi_, prev_root, block_prev, 1ull << Lp.log_num_interleaved, qi), "Merkle multi-proof");
+        const G es = enforced_sum(oi_, level_rs, qi, ai);
+        V_CHECK(next_msg(&msg), fail);
+        const Quad iq = quad(msg, es);
+        const G beta = it.gf();
+        rq = qfold(rq, iq, beta);
+        t_r = hgf_add(t_r, hgf_mul(beta, es));
+        if ((rc = induce_comp(n_cur, qi, ai, ris.size(), beta))) return rc;
+        prev_root = root_n;
+        prev = i + 2;
+    }
+    V_CHECK(false, "Ligerito ladder without levels");
+}
+
 int make_prover(bitz_ctx* ctx, const bitz_commit_handle* h, int target_bits, Prover* p) {
     p->ctx = ctx;
     p->h = h;
@@ -1604,6 +2441,20 @@ extern "C" int bitz_prove_standalone(bitz_ctx* ctx, bitz_commit_handle* h, const
     BITZ_REQUIRE(proof && capacity >= bytes.size(), BITZ_ERR_ARG, "bitz_prove_standalone: proof buffer too small");
     memcpy(proof, bytes.data(), bytes.size());
     return BITZ_OK;
+}
+
+extern "C" int bitz_verify_standalone(bitz_ctx* ctx, const uint8_t root[32], uint32_t t, uint32_t s,
+                                      uint32_t log_inv_rate, uint32_t initial_k, const uint8_t claimed[16],
+                                      const uint8_t* proof, uint64_t proof_len) {
+    BITZ_REQUIRE(ctx && root && claimed && (proof || proof_len == 0), BITZ_ERR_ARG,
+                 "bitz_verify_standalone: null argument");
+    BITZ_REQUIRE(t <= 40 && s <= 40, BITZ_ERR_SHAPE, "bitz_verify_standalone: shape out of range");
+    BITZ_HIP(hipSetDevice(ctx->device));
+    bitz::PoolScope scope(&ctx->pool);
+    bitz::u128 c = 0;
+    for (int i = 15; i >= 0; i--) c = (c << 8) | claimed[i];
+    return bitz::verify_standalone_impl(ctx, root, (int)t, (int)s, (int)log_inv_rate, (int)initial_k, c, proof,
+                                        (size_t)proof_len);
 }
 
 extern "C" int bitz_ligerito_config(uint32_t m, uint32_t log_inv_rate, uint32_t initial_k, uint32_t target_bits,

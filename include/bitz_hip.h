@@ -36,6 +36,7 @@ extern "C" {
 #define BITZ_ERR_ALLOC (-2)  /* device allocation failed */
 #define BITZ_ERR_DEVICE (-3) /* HIP runtime error */
 #define BITZ_ERR_ARG (-4)    /* null or inconsistent argument */
+#define BITZ_ERR_VERIFY (-5) /* a proof rejected by the verifier (reason in bitz_last_error) */
 
 typedef struct bitz_ctx bitz_ctx;
 typedef struct bitz_commit_handle bitz_commit_handle;
@@ -278,6 +279,18 @@ int bitz_prove_standalone(bitz_ctx* ctx, bitz_commit_handle* h, const uint8_t cl
  * k_recursive, queries, grinding_bits, fold_grinding_bits, ood_samples]; returns the count */
 int bitz_ligerito_config(uint32_t m, uint32_t log_inv_rate, uint32_t initial_k, uint32_t target_bits,
                          int32_t* out, uint32_t capacity);
+/* verify_standalone (src/piop/spartan/protocol/bitz_opener.rs:709-747) ->
+ * BitZVerifier::verify (src/bitz/mod.rs:218-255) and the recursive Ligerito
+ * verifier (pcs/ligerito.rs:4585-4760) with the residual evaluated densely:
+ * BITZ_OK when the proof (BitZOpeningProof::to_bytes layout) opens the
+ * commitment `root` of shape (t, s, log_batch = initial_k, log_inv_rate) to
+ * `claimed` (16-byte little-endian integer), BITZ_ERR_VERIFY with the reason
+ * otherwise. Transcript checks run on the host, the dense tables (eq tables,
+ * images, the ring-switch basis, the induced bases) on the context's device.
+ * Synchronizing. */
+int bitz_verify_standalone(bitz_ctx* ctx, const uint8_t root[32], uint32_t t, uint32_t s, uint32_t log_inv_rate,
+                           uint32_t initial_k, const uint8_t claimed[16], const uint8_t* proof, uint64_t proof_len);
+
 /* ---- batched and served commit+open jobs ----
  * A job is one commitment and its standalone opening (Pcs::commit +
  * standalone_evaluation / prove_standalone). The server runs jobs on

@@ -175,16 +175,40 @@ def batched_and_serving(args, ctx):
     ok, vms = verify_all(ctx, bat, specs)
     batched["all_verified"] = ok
     batched["verify_ms_per_proof"] = round(vms, 3)
-    # serving: Poisson arrivals at 70% of the batched rate
-    rate = 0.7 * len(specs) / dt_bat
-    sspecs = _job_specs(np.random.default_rng(65), args.serve_jobs, 14, 22)
-    gaps = np.random.default_rng(66).exponential(1.0 / rate, size=len(sspecs))
-    jobs = [Job(rows, t, s, 4, r) for t, s, r, rows in sspecs]
+    # serving: Poisson arrivals at 70% of the batched rate, half of them
+    # commit+open jobs over host rows, half u64-multiplication jobs (2^12..2^22
+    # multiplications, their witness grid packed on the device); every proof
+    # is verified by the worker that made it, inside the measured latency
+    from bitz_amd.serve import MulJob
+    srng = np.random.default_rng(65)
+    sspecs = _job_specs(srng, args.serve_jobs // 2, 14, 22)
+
+    def mixed_jobs(limit=None):
+        mrng = np.random.default_rng(67)
+        out = []
+        for t, s, r, rows in sspecs[:limit]:
+            out.append(Job(rows, t, s, 4, r, verify=True))
+            ln = int(mrng.integers(12, 23))
+            x = mrng.integers(0, 2**64, size=1 << ln, dtype=np.uint64)
+            y = mrng.integers(0, 2**64, size=1 << ln, dtype=np.uint64)
+            out.append(MulJob(x, y, 4, int(mrng.choice([1, 2, 3])), verify=True))
+        return out
+
+    jobs = mixed_jobs()
     with Server(workers=args.batch_workers, hbm_budget=64 << 30) as srv:
-        warm = [Job(rows, t, s, 4, r) for t, s, r, rows in sspecs[:args.batch_workers * 2]]
+        warm = mixed_jobs(args.batch_workers)
         for j in warm:
             srv.submit(j)
         srv.wait_all()
+        # the mix's closed-loop capacity sets the arrival rate (70% of it)
+        cap_jobs = mixed_jobs()
+        tc = time.perf_counter()
+        for j in cap_jobs:
+            srv.submit(j)
+        srv.wait_all()
+        capacity = len(cap_jobs) / (time.perf_counter() - tc)
+        rate = 0.7 * capacity
+        gaps = np.random.default_rng(66).exponential(1.0 / rate, size=len(jobs))
         t0 = time.perf_counter()
         due = 0.0
         for j, g in zip(jobs, gaps):
@@ -198,19 +222,22 @@ def batched_and_serving(args, ctx):
     for j in jobs:
         j.check()
     lat = np.array([j.latency_ms for j in jobs])
+    lat_mul = np.array([j.latency_ms for j in jobs[1::2]])
     serving = {
-        "workload": f"{len(jobs)} Poisson arrivals of commit+open jobs (2^14..2^22 coefficients, mixed rates), "
-                    "admitted by device-memory footprint (64 GiB budget)",
+        "workload": f"{len(jobs)} Poisson arrivals: commit+open of host rows (2^14..2^22 coefficients) and of "
+                    "u64-multiplication witness grids (2^12..2^22 multiplications), 50/50, mixed rates, "
+                    "admitted by device-memory footprint (64 GiB budget), each proof verified by its worker",
+        "capacity_proofs_per_s": round(capacity, 2),
         "arrival_rate_per_s": round(rate, 2),
         "workers": args.batch_workers,
         "proofs_per_s": round(len(jobs) / (t1 - t0), 2),
         "p50_ms": round(float(np.percentile(lat, 50)), 3),
         "p99_ms": round(float(np.percentile(lat, 99)), 3),
         "max_ms": round(float(lat.max()), 3),
+        "mul_jobs_p50_ms": round(float(np.percentile(lat_mul, 50)), 3),
+        "mul_jobs_p99_ms": round(float(np.percentile(lat_mul, 99)), 3),
+        "all_verified": bool(all(j.verified for j in jobs)),
     }
-    ok, vms = verify_all(ctx, jobs, sspecs)
-    serving["all_verified"] = ok
-    serving["verify_ms_per_proof"] = round(vms, 3)
     return batched, serving
 
 

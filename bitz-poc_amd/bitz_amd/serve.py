@@ -24,7 +24,12 @@ class _JobStruct(C.Structure):
         ("status", C.c_int32), ("done", C.c_int32),
         ("submit_ms", C.c_double), ("start_ms", C.c_double), ("end_ms", C.c_double),
         ("error", C.c_char * 128),
+        ("kind", C.c_uint32), ("verify", C.c_uint32),
+        ("mul_x", C.c_void_p), ("mul_y", C.c_void_p), ("mul_n", C.c_uint64),
+        ("verified", C.c_int32), ("reserved", C.c_int32),
     ]
+
+JOB_ROWS, JOB_MUL_U64 = 0, 1
 
 
 class Job:
@@ -33,13 +38,15 @@ class Job:
     claim (standalone_evaluation) first."""
 
     def __init__(self, rows: np.ndarray, t: int, s: int, log_batch: int, log_inv_rate: int,
-                 claimed: int | None = None, proof_capacity: int = 1 << 21):
+                 claimed: int | None = None, proof_capacity: int = 1 << 21, verify: bool = False):
         rows = np.ascontiguousarray(rows, dtype=np.uint64)
         if rows.size != (1 << s) << (t - 6):
             raise ValueError("rows must hold 2^s * 2^(t-6) u64 words")
         self._rows = rows
         self._proof = (C.c_uint8 * proof_capacity)()
         j = _JobStruct()
+        j.kind = JOB_ROWS
+        j.verify = 1 if verify else 0
         j.rows = rows.ctypes.data
         j.t, j.s, j.log_batch, j.log_inv_rate = t, s, log_batch, log_inv_rate
         j.have_claim = 0 if claimed is None else 1
@@ -74,6 +81,47 @@ class Job:
     @property
     def latency_ms(self) -> float:
         return self.s_.end_ms - self.s_.submit_ms
+
+    @property
+    def t(self) -> int:
+        return int(self.s_.t)
+
+    @property
+    def s(self) -> int:
+        return int(self.s_.s)
+
+    @property
+    def verified(self) -> bool:
+        return bool(self.s_.verified)
+
+
+class MulJob(Job):
+    """A commit+open job over the witness grid of n u64 multiplications
+    x[i] * y[i] (MulWitness::bitz_bit_rows, src/piop/spartan/mul.rs:268-330),
+    packed on the device by the worker; t and s follow MulLayout."""
+
+    def __init__(self, x: np.ndarray, y: np.ndarray, log_batch: int, log_inv_rate: int,
+                 claimed: int | None = None, proof_capacity: int = 1 << 21, verify: bool = False):
+        x = np.ascontiguousarray(x, dtype=np.uint64)
+        y = np.ascontiguousarray(y, dtype=np.uint64)
+        if x.shape != y.shape or x.ndim != 1 or x.size == 0:
+            raise ValueError("x and y must be equal-length non-empty u64 vectors")
+        self._x, self._y = x, y
+        self._proof = (C.c_uint8 * proof_capacity)()
+        j = _JobStruct()
+        j.kind = JOB_MUL_U64
+        j.verify = 1 if verify else 0
+        j.mul_x, j.mul_y, j.mul_n = x.ctypes.data, y.ctypes.data, x.size
+        out = (C.c_uint32 * 4)()
+        _lib.call("bitz_mul_layout", 64, x.size, 0, C.cast(out, C.c_void_p))
+        j.t, j.s = out[2], out[1]
+        j.log_batch, j.log_inv_rate = log_batch, log_inv_rate
+        j.have_claim = 0 if claimed is None else 1
+        if claimed is not None:
+            C.memmove(j.claimed, int(claimed).to_bytes(16, "little"), 16)
+        j.proof = C.cast(self._proof, C.c_void_p)
+        j.proof_capacity = proof_capacity
+        self.s_ = j
 
 
 class Server:

@@ -71,13 +71,36 @@ double ms_since(const bitz_server* srv) {
     return std::chrono::duration<double, std::milli>(Clock::now() - srv->t0).count();
 }
 
-// commit + (evaluation) + proof of one job on one context
+// the committed rows of a u64 multiplication job, packed on the device
+int mul_rows(bitz_ctx* ctx, const bitz_job* j, void** rows) {
+    void *xd = nullptr, *yd = nullptr;
+    const uint64_t bytes = 8 * j->mul_n, words = 1ull << (j->t + j->s - 6);
+    int rc = bitz_dev_alloc(ctx, bytes, &xd);
+    if (!rc) rc = bitz_dev_alloc(ctx, bytes, &yd);
+    if (!rc) rc = bitz_dev_alloc(ctx, 8 * words, rows);
+    if (!rc) rc = bitz_dev_upload(ctx, xd, j->mul_x, bytes);
+    if (!rc) rc = bitz_dev_upload(ctx, yd, j->mul_y, bytes);
+    if (!rc) rc = bitz_mul_bit_rows_dev(ctx, 64, j->mul_n, 0, xd, yd, nullptr, nullptr, *rows, words);
+    if (xd) bitz_dev_free(ctx, xd);  // stream-ordered: the packing has read them
+    if (yd) bitz_dev_free(ctx, yd);
+    return rc;
+}
+
+// commit + (evaluation) + proof (+ verification) of one job on one context
 int run_job(bitz_ctx* ctx, bitz_job* j) {
     bitz_commit_handle* h = nullptr;
-    int rc = j->rows_on_device ? bitz_commit_device_rows(ctx, j->rows, j->t, j->s, j->log_batch, j->log_inv_rate,
+    int rc;
+    if (j->kind == BITZ_JOB_MUL_U64) {
+        void* rows = nullptr;
+        rc = mul_rows(ctx, j, &rows);
+        if (!rc) rc = bitz_commit_device_rows(ctx, rows, j->t, j->s, j->log_batch, j->log_inv_rate, j->root, &h);
+        if (rows) bitz_dev_free(ctx, rows);  // the handle holds its own copy
+    } else {
+        rc = j->rows_on_device ? bitz_commit_device_rows(ctx, j->rows, j->t, j->s, j->log_batch, j->log_inv_rate,
                                                          j->root, &h)
                                : bitz_commit(ctx, (const uint64_t*)j->rows, j->t, j->s, j->log_batch,
                                              j->log_inv_rate, j->root, &h);
+    }
     if (rc) return rc;
     if (!j->have_claim) {
         rc = bitz_standalone_evaluation(ctx, h, j->claimed);
@@ -88,6 +111,10 @@ int run_job(bitz_ctx* ctx, bitz_job* j) {
     }
     rc = bitz_prove_standalone(ctx, h, j->claimed, j->proof, j->proof_capacity, &j->proof_len);
     bitz_commit_free(h);
+    if (rc || !j->verify) return rc;
+    rc = bitz_verify_standalone(ctx, j->root, j->t, j->s, j->log_inv_rate, j->log_batch, j->claimed, j->proof,
+                                j->proof_len);
+    j->verified = rc == BITZ_OK ? 1 : 0;
     return rc;
 }
 
@@ -160,10 +187,24 @@ extern "C" int bitz_server_create(int device, uint32_t workers, uint64_t hbm_bud
 }
 
 extern "C" int bitz_server_submit(bitz_server* srv, bitz_job* job) {
-    if (!srv || !job || !job->rows || !job->proof) {
+    if (!srv || !job || !job->proof) {
         bitz::set_error("bitz_server_submit: null argument");
         return BITZ_ERR_ARG;
     }
+    if (job->kind == BITZ_JOB_MUL_U64) {
+        uint32_t lay[4];
+        if (!job->mul_x || !job->mul_y) {
+            bitz::set_error("bitz_server_submit: null multiplication operands");
+            return BITZ_ERR_ARG;
+        }
+        if (int rc = bitz_mul_layout(64, job->mul_n, 0, lay)) return rc;
+        job->t = lay[2];
+        job->s = lay[1];
+    } else if (job->kind != BITZ_JOB_ROWS || !job->rows) {
+        bitz::set_error("bitz_server_submit: null rows or unknown job kind");
+        return BITZ_ERR_ARG;
+    }
+    job->verified = 0;
     if (bitz_job_footprint(job->t, job->s, job->log_batch, job->log_inv_rate) == 0) {
         bitz::set_error("bitz_server_submit: job shape rejected");
         return BITZ_ERR_SHAPE;

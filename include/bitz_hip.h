@@ -314,7 +314,21 @@ typedef struct bitz_job {
     int32_t done;            /* out: set under the server's lock, read it with bitz_server_done */
     double submit_ms, start_ms, end_ms; /* server clock (bitz_server_clock_ms) */
     char error[128];
+    /* kind BITZ_JOB_MUL_U64: the committed rows are the witness grid of
+     * mul_n u64 multiplications (MulWitness::bitz_bit_rows,
+     * src/piop/spartan/mul.rs:268-330) of the host arrays mul_x, mul_y,
+     * packed on the device; rows is unused and t, s are set at submission
+     * (MulLayout). */
+    uint32_t kind;           /* BITZ_JOB_ROWS (0) or BITZ_JOB_MUL_U64 (1) */
+    uint32_t verify;         /* nonzero: the worker verifies the proof (bitz_verify_standalone) */
+    const uint64_t* mul_x;
+    const uint64_t* mul_y;
+    uint64_t mul_n;
+    int32_t verified;        /* out: 1 when the proof was verified and accepted */
+    int32_t reserved;
 } bitz_job;
+#define BITZ_JOB_ROWS 0
+#define BITZ_JOB_MUL_U64 1
 typedef struct bitz_server bitz_server;
 /* device bytes one job holds while it runs (0 for an invalid shape) */
 uint64_t bitz_job_footprint(uint32_t t, uint32_t s, uint32_t log_batch, uint32_t log_inv_rate);

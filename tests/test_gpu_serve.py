@@ -91,3 +91,46 @@ def test_bad_job_reports_its_error_and_the_server_continues():
     good[1].check()
     with pytest.raises(BitzError):
         bad.check()
+
+
+def test_multiplication_jobs_in_a_verified_mix():
+    """u64-multiplication jobs (the witness grid packed on the device by the
+    worker) beside row jobs on one server, each proof verified by the worker:
+    a multiplication job's root, claim and proof equal those of the grid
+    packed through bitz_mul_bit_rows_dev and committed directly, and do not
+    change in the mix"""
+    import torch
+
+    from bitz_amd.device import Commitment
+    from bitz_amd.mul import MulWitness
+    from bitz_amd.serve import Job, MulJob, Server
+    from gpu_util import ctx
+    rng = np.random.default_rng(21)
+    ops = [(rng.integers(0, 2**64, size=n, dtype=np.uint64), rng.integers(0, 2**64, size=n, dtype=np.uint64), r)
+           for n, r in [(1 << 12, 2), (3000, 1), (1 << 13, 3)]]
+    log("serve: multiplication jobs alone (direct packing + commit)")
+    direct = []
+    for x, y, r in ops:
+        w = MulWitness(ctx(), 64, torch.from_numpy(x.view(np.int64)).cuda(), torch.from_numpy(y.view(np.int64)).cuda())
+        com = Commitment(ctx(), w.bitz_bit_rows(), w.layout.t, w.layout.s, 4, r)
+        claimed = com.evaluation()
+        direct.append((com.root, claimed, com.prove(claimed)))
+        com.close()
+    log("serve: multiplication and row jobs on a 4-worker server, verified by the workers")
+    mjobs = [MulJob(x, y, 4, r, verify=True) for x, y, r in ops]
+    rjobs = [Job(rng.integers(0, 2**64, size=(1 << s) << (t - 6), dtype=np.uint64), t, s, 4, r, verify=True)
+             for t, s, r in SHAPES[:4]]
+    with Server(workers=4) as srv:
+        for j in [mjobs[0], rjobs[0], mjobs[1], rjobs[1], rjobs[2], mjobs[2], rjobs[3]]:
+            srv.submit(j)
+        srv.wait_all()
+    for j, d in zip(mjobs, direct):
+        j.check()
+        assert j.verified
+        assert (j.root, j.claimed, j.proof) == d
+    for j in rjobs:
+        j.check()
+        assert j.verified
+    x, y, r = ops[1]
+    t, s = mjobs[1].t, mjobs[1].s
+    P.verify_standalone(mjobs[1].root, t, s, r, 4, mjobs[1].claimed, mjobs[1].proof)

@@ -488,6 +488,58 @@ __global__ __launch_bounds__(kT) void k_leaf_pattern_sums(const uint64_t* __rest
     block_xor_store<2>(acc, partials + (size_t)blockIdx.x * 2);
 }
 
+// The same sums with the rows' tables staged in LDS, for at least 1024
+// columns: block (1024 columns, rpb rows) takes G = 2048 / 2^K rows at a time,
+// copies their G x 2^K table entries (64 KiB) into LDS with coalesced loads,
+// and each thread walks its 4 columns over those rows, reading each entry
+// from LDS. A row's table crosses the memory hierarchy once per column block
+// (8 times at 2^13 columns) instead of once per wavefront gather.
+template <int K>
+__global__ __launch_bounds__(kT) void k_leaf_pattern_sums_lds(const uint64_t* __restrict__ packed, int t, int s,
+                                                              uint64_t stride, const uint4* __restrict__ qtab,
+                                                              const uint4* __restrict__ eq_c, uint64_t rpb,
+                                                              uint4* __restrict__ partials) {
+    constexpr int G = 2048 >> K;
+    __shared__ uint4 tab[(G << K) * 2];
+    const uint64_t nrows = 1ull << t, colblocks = (1ull << s) >> 10;
+    const uint64_t cb = blockIdx.x % colblocks, rb = blockIdx.x / colblocks;
+    const uint64_t y0 = rb * rpb, cbase = cb << 10;
+    uint32_t e[4][4], q[4][4];
+#pragma unroll
+    for (int j = 0; j < 4; j++)
+#pragma unroll
+        for (int w = 0; w < 4; w++) e[j][w] = q[j][w] = 0;
+    for (uint64_t g0 = y0; g0 < y0 + rpb; g0 += G) {
+        __syncthreads();  // the previous rows' readers are done
+        const uint4* src = qtab + (g0 << K) * 2;
+        for (uint32_t i = threadIdx.x; i < (uint32_t)((G << K) * 2); i += kT) tab[i] = src[i];
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const uint64_t c = cbase + threadIdx.x + 256 * j;
+            const uint64_t* col = packed + (c >> 6) * nrows;
+            const int bit = (int)(c & 63);
+            for (int gy = 0; gy < G; gy++) {
+                const uint64_t y = g0 + gy;
+                uint32_t pat = 0;
+#pragma unroll
+                for (int k = 0; k < K; k++) pat |= (uint32_t)((col[y + k * stride] >> bit) & 1ull) << k;
+                const uint4 a = tab[((gy << K) + pat) * 2], b = tab[((gy << K) + pat) * 2 + 1];
+                e[j][0] ^= a.x, e[j][1] ^= a.y, e[j][2] ^= a.z, e[j][3] ^= a.w;
+                q[j][0] ^= b.x, q[j][1] ^= b.y, q[j][2] ^= b.z, q[j][3] ^= b.w;
+            }
+        }
+    }
+    gf128 acc[2] = {gf_zero(), gf_zero()};
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        const gf128 wc = gf_load(&eq_c[cbase + threadIdx.x + 256 * j]);
+        acc[0] = gf_add(acc[0], gf_mul(gf128{{e[j][0], e[j][1], e[j][2], e[j][3]}}, wc));
+        acc[1] = gf_add(acc[1], gf_mul(gf128{{q[j][0], q[j][1], q[j][2], q[j][3]}}, wc));
+    }
+    block_xor_store<2>(acc, partials + (size_t)blockIdx.x * 2);
+}
+
 // Round 1 of the leaf layer. After round 0 folded by rho0, the halves are
 // l'(y, c) = ftab[y*8 + pat2(y, y+q)] and r'(y, c) = ftab[y*8 + 4 + pat2(y+2q, y+3q)]
 // (q = 2^(t-2)); a round-1 slot y1 < q8 = q/2 meets l'(y1), l'(y1+q8),
@@ -1513,6 +1565,34 @@ static uint64_t pattern_ylanes(int s, uint64_t ny) {
     return s >= 20 ? 1 : std::min<uint64_t>(ny, (1ull << 20) >> s);
 }
 
+// Blocks of a K-bit pattern-sums launch over ny rows (row stride ny): the
+// LDS-table kernel (rpb rows per block) for at least 1024 columns, the
+// gather kernel otherwise (rpb = 0).
+template <int K>
+static unsigned pattern_sums_blocks(int s, uint64_t ny, uint64_t* rpb) {
+    constexpr uint64_t G = 2048 >> K;
+    if (s >= 10 && ny >= G && ny % G == 0) {
+        const uint64_t colblocks = (1ull << s) >> 10;
+        uint64_t rowblocks = std::min<uint64_t>(std::max<uint64_t>(1, 512 / colblocks), ny / G);
+        while (rowblocks > 1 && ny % (rowblocks * G)) rowblocks >>= 1;
+        *rpb = ny / rowblocks;
+        return (unsigned)(colblocks * rowblocks);
+    }
+    *rpb = 0;
+    return (unsigned)(((1ull << s) * pattern_ylanes(s, ny) + kT - 1) / kT);
+}
+template <int K>
+static void launch_pattern_sums(bitz_ctx* ctx, unsigned nb, uint64_t rpb, const uint64_t* packed, int t, int s,
+                                uint64_t ny, const uint4* tab, const void* eq_c) {
+    uint4* parts = (uint4*)ctx->scratch.partials.ptr;
+    if (rpb)
+        hipLaunchKernelGGL(k_leaf_pattern_sums_lds<K>, dim3(nb), dim3(kT), 0, ctx->stream, packed, t, s, ny, tab,
+                           (const uint4*)eq_c, rpb, parts);
+    else
+        hipLaunchKernelGGL(k_leaf_pattern_sums<K>, dim3(nb), dim3(kT), 0, ctx->stream, packed, t, s, ny, ny, tab,
+                           (const uint4*)eq_c, pattern_ylanes(s, ny), parts);
+}
+
 int gkr_premul_round_sums(bitz_ctx* ctx, const void* l, void* r, uint64_t h, const void* eq_c, int s,
                           const void* eq_y, int send_one, void* out_dev) {
     uint32_t R = round_chunk_r(ctx->tune, h, s);
@@ -1614,17 +1694,16 @@ int gkr_leaf_round_sums(bitz_ctx* ctx, const uint64_t* packed, const void* A, in
         const uint64_t q = 1ull << (t - 2), cols = 1ull << s;
         if (!ctx->scratch.leaf_tab.alloc(leaf_tab_bytes(t))) return BITZ_ERR_ALLOC;
         uint4* qtab = (uint4*)ctx->scratch.leaf_tab.ptr;
-        uint64_t ylanes = pattern_ylanes(s, q);
-        uint64_t threads = cols * ylanes;
-        unsigned nb = (unsigned)((threads + kT - 1) / kT);
+        uint64_t rpb;
+        const unsigned nb = pattern_sums_blocks<4>(s, q, &rpb);
+        const double threads = (double)cols;
         int rc = ctx->scratch.ensure((size_t)nb * 2 * 16);
         if (rc) return rc;
         TimedLaunch tl(ctx, "gkr_leaf_round_sums", (double)(1ull << (t + s - 3)) + q * 512.0,
                        q * 64.0 * kGfMulOps + 2.0 * threads * kGfMulOps);
         hipLaunchKernelGGL(k_leaf_qtab, dim3((unsigned)((q * 16 + kT - 1) / kT)), dim3(kT), 0, ctx->stream,
                            (const uint4*)A, t, (const uint4*)eq_y, send_one, qtab);
-        hipLaunchKernelGGL(k_leaf_pattern_sums<4>, dim3(nb), dim3(kT), 0, ctx->stream, packed, t, s, q, q,
-                           (const uint4*)qtab, (const uint4*)eq_c, ylanes, (uint4*)ctx->scratch.partials.ptr);
+        launch_pattern_sums<4>(ctx, nb, rpb, packed, t, s, q, (const uint4*)qtab, eq_c);
         BITZ_LAUNCH_CHECK();
         return sums_finish(ctx, nb, 2, out_dev);
     }
@@ -1685,9 +1764,9 @@ int gkr_leaf_round1_sums(bitz_ctx* ctx, const uint64_t* packed, const void* A, i
     if (!ctx->scratch.leaf_tab.alloc(leaf_tab_bytes(t))) return BITZ_ERR_ALLOC;
     uint4* ftab = (uint4*)ctx->scratch.leaf_tab.ptr;
     uint4* q1tab = leaf_tab_hi(ctx, t);
-    uint64_t ylanes = pattern_ylanes(s, q8);
-    uint64_t threads = cols * ylanes;
-    unsigned nb = (unsigned)((threads + kT - 1) / kT);
+    uint64_t rpb;
+    const unsigned nb = pattern_sums_blocks<8>(s, q8, &rpb);
+    const double threads = (double)cols;
     int rc = ctx->scratch.ensure((size_t)nb * 2 * 16);
     if (rc) return rc;
     TimedLaunch tl(ctx, "gkr_leaf_round1_sums", (double)(1ull << (t + s - 3)) + q8 * 8192.0,
@@ -1696,8 +1775,7 @@ int gkr_leaf_round1_sums(bitz_ctx* ctx, const uint64_t* packed, const void* A, i
                        (const uint4*)A, t, lo, hi, (const uint4*)rho0_dev, ftab);
     hipLaunchKernelGGL(k_leaf_q1tab, dim3((unsigned)((q8 * 256 + kT - 1) / kT)), dim3(kT), 0, ctx->stream,
                        (const uint4*)ftab, t, (const uint4*)eq_y, send_one, q1tab);
-    hipLaunchKernelGGL(k_leaf_pattern_sums<8>, dim3(nb), dim3(kT), 0, ctx->stream, packed, t, s, q8, q8,
-                       (const uint4*)q1tab, (const uint4*)eq_c, ylanes, (uint4*)ctx->scratch.partials.ptr);
+    launch_pattern_sums<8>(ctx, nb, rpb, packed, t, s, q8, (const uint4*)q1tab, eq_c);
     BITZ_LAUNCH_CHECK();
     return sums_finish(ctx, nb, 2, out_dev);
 }
@@ -1732,9 +1810,9 @@ int gkr_level1_round0_sums(bitz_ctx* ctx, const uint64_t* packed, const void* A,
     if (!ctx->scratch.leaf_tab.alloc(leaf_tab_bytes(t))) return BITZ_ERR_ALLOC;
     uint4* vtab = (uint4*)ctx->scratch.leaf_tab.ptr;
     uint4* qtab = leaf_tab_hi(ctx, t);
-    uint64_t ylanes = pattern_ylanes(s, Y3);
-    uint64_t threads = cols * ylanes;
-    unsigned nb = (unsigned)((threads + kT - 1) / kT);
+    uint64_t rpb;
+    const unsigned nb = pattern_sums_blocks<8>(s, Y3, &rpb);
+    const double threads = (double)cols;
     int rc = ctx->scratch.ensure((size_t)nb * 2 * 16);
     if (rc) return rc;
     TimedLaunch tl(ctx, "gkr_level1_round0_sums", (double)(1ull << (t + s - 3)) + Y3 * 8192.0,
@@ -1743,8 +1821,7 @@ int gkr_level1_round0_sums(bitz_ctx* ctx, const uint64_t* packed, const void* A,
                        (const uint4*)A, t, vtab);
     hipLaunchKernelGGL(k_level1_qtab, dim3((unsigned)((Y3 * 256 + kT - 1) / kT)), dim3(kT), 0, ctx->stream,
                        (const uint4*)vtab, t, (const uint4*)eq_y, send_one, qtab);
-    hipLaunchKernelGGL(k_leaf_pattern_sums<8>, dim3(nb), dim3(kT), 0, ctx->stream, packed, t, s, Y3, Y3,
-                       (const uint4*)qtab, (const uint4*)eq_c, ylanes, (uint4*)ctx->scratch.partials.ptr);
+    launch_pattern_sums<8>(ctx, nb, rpb, packed, t, s, Y3, (const uint4*)qtab, eq_c);
     BITZ_LAUNCH_CHECK();
     return sums_finish(ctx, nb, 2, out_dev);
 }

@@ -41,9 +41,9 @@ VALU_PEAK_TOPS = 256 * 64 * 2.4e9 / 1e12
 PMC = {
     24: {
         "gkr_fold_round_sums_rowonly": {"mib": 280.29 + 139.98, "valu_insts": 4.986e+07},
-        "gkr_premul_round_sums": {"mib": 589.43 + 292.58, "valu_insts": 2.486e+08},
+        "gkr_premul_round_sums": {"mib": 589.42 + 292.58, "valu_insts": 2.486e+08},
         "product_level": {"mib": 315.10 + 157.52, "valu_insts": 6.925e+07},
-        "ntt_pass": {"mib": 318.72 + 326.40, "valu_insts": 1.335e+08},
+        "ntt_pass": {"mib": 318.73 + 326.40, "valu_insts": 1.335e+08},
     },
 }
 

@@ -245,7 +245,7 @@ def main():
     args = parse()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}): the commit+open path is "
-              f"measured on a single MI355X; see DESIGN.md for how it would shard.", file=sys.stderr)
+              f"measured on a single MI355X.", file=sys.stderr)
         sys.exit(2)
     import torch
 

@@ -6,7 +6,7 @@ TEST INFRASTRUCTURE ONLY (see oracle.py): the checker of the device packer.
 Pins (tests/test_oracle_mul.py): the packing cases of the reference's own
 test (mul/tests.rs:7-83: fixed rows, n in {37, 257, 2053, 4097}, shifts
 -1..2, expected words from its slot-major index formula), the layout bounds
-test (mul/tests.rs:85-100) and the config-2 shape of the u64 workload at
+test (mul/tests.rs:85-100) and the shape of the u64 multiplication workload at
 2^20 multiplications (t = 16, s = 12); and, for random batches, decoding the
 grid back through bitz_cell recovers x, y, lo, hi with x * y = lo + hi 2^B.
 """

@@ -1,6 +1,6 @@
 """The CPU restatement of the multiplication relations' committed bit grid
 (oracle/mul_witness.py) against the reference's own packing test, its layout
-bounds, the config-2 shape, and decoding; and the library's host-side layout
+bounds, the shape of the u64 workload at 2^20 multiplications, and decoding; and the library's host-side layout
 (bitz_mul_layout, no GPU needed) against the restatement."""
 import random
 

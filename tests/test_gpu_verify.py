@@ -95,3 +95,17 @@ def test_verifies_2p20_coefficient_proof():
     assert not _gpu_accepts(com.root, t, s, 2, 4, claimed, bytes(bad))
     assert not _gpu_accepts(com.root, t, s, 2, 4, claimed ^ 1, proof)
     com.close()
+
+
+def test_random_byte_flips_rejected(proved):
+    """40 seeded single-bit flips over the whole proof: every one is rejected
+    by the library's verifier (and by the CPU verifier for a sample)"""
+    (t, s, rate, k0), com, claimed, proof = proved
+    rng = np.random.default_rng(7 + t)
+    positions = rng.choice(len(proof), size=40, replace=False)
+    for i, pos in enumerate(positions):
+        bad = bytearray(proof)
+        bad[int(pos)] ^= 1 << int(rng.integers(0, 8))
+        assert not _gpu_accepts(com.root, t, s, rate, k0, claimed, bytes(bad)), f"flip at {pos} accepted"
+        if i < 4:
+            assert not _cpu_accepts(com.root, t, s, rate, k0, claimed, bytes(bad))

@@ -280,7 +280,8 @@ struct RoundVals {
 template <class Src, bool ColW = true>
 __global__ __launch_bounds__(kT) void k_gkr_round_sums(Src src, uint64_t h, const uint4* __restrict__ eq_c,
                                                        int s_c, const uint4* __restrict__ eq_y, int send_one,
-                                                       uint32_t R, uint4* __restrict__ partials) {
+                                                       uint32_t R, uint4* __restrict__ partials,
+                                                       const uint4* __restrict__ eq_lo, uint32_t lo_mask) {
     __shared__ uint4 tab[Src::kTable ? 512 : 1];
     if constexpr (Src::kTable) {
         src.build(tab);
@@ -314,6 +315,14 @@ __global__ __launch_bounds__(kT) void k_gkr_round_sums(Src src, uint64_t h, cons
         gf128 wy = gf_load(&eq_y[x0 >> s_c]);
         acc[0] = gf_add(acc[0], gf_mul(gf_reduce(w_end), wy));
         acc[1] = gf_add(acc[1], gf_mul(gf_reduce(w_inf), wy));
+    }
+    if (eq_lo) {
+        // column weights of the low index bits, left out of the right half:
+        // chunks start at multiples of kT, so every slot x of this thread has
+        // x & (kT - 1) = threadIdx.x and one factor weighs all of them
+        const gf128 wl = gf_load(&eq_lo[threadIdx.x & lo_mask]);
+        acc[0] = gf_mul(acc[0], wl);
+        acc[1] = gf_mul(acc[1], wl);
     }
     block_xor_store<2>(acc, partials + (size_t)blockIdx.x * 2);
 }
@@ -611,6 +620,52 @@ __global__ __launch_bounds__(kT) void k_pattern_fold(const uint64_t* __restrict_
             gf_store(&r_out[x], gf_mul(gf_load(&gtab[y1 * 32 + 16 + rsub]), gf_load(&eq_c[c])));
         else
             r_out[x] = gtab[y1 * 32 + 16 + rsub];
+    }
+}
+
+// Column weights split as eq_c[c] = eq_lo[c & 255] * eq_hi[c >> 8] (the eq
+// table is a tensor product over the bits of c): a 256-column run of one row
+// shares its eq_hi factor, so a block that keeps to one run multiplies by a
+// constant through its window table (~125 VALU) where eq_c[c] would take a
+// variable product (~430); eq_lo goes to the row-only round sums, one
+// factor per thread. Block b takes run g = b mod 2^(s-8) and rows
+// b >> (s-8), + row_lanes.
+template <int Layout>
+__global__ __launch_bounds__(kT) void k_pattern_fold_hi(const uint64_t* __restrict__ packed, int t, int s,
+                                                        const uint4* __restrict__ gtab,
+                                                        const uint4* __restrict__ eq_hi, uint32_t row_lanes,
+                                                        uint4* __restrict__ l_out, uint4* __restrict__ r_out) {
+    __shared__ uint4 tab[512];
+    const uint64_t q8 = 1ull << (t - 3), nrows = 1ull << t;
+    const uint32_t g = blockIdx.x & ((1u << (s - 8)) - 1);
+    gf_table_build(tab, gf_load(&eq_hi[g]), threadIdx.x, blockDim.x);
+    __syncthreads();
+    const uint64_t c = ((uint64_t)g << 8) + threadIdx.x;
+    const uint64_t* col = packed + (c >> 6) * nrows;
+    const int bit = (int)(c & 63);
+    for (uint64_t y1 = blockIdx.x >> (s - 8); y1 < q8; y1 += row_lanes) {
+        const uint64_t x = (y1 << s) + c;
+        uint32_t pat = 0;
+#pragma unroll
+        for (int k = 0; k < 8; k++) pat |= (uint32_t)((col[y1 + k * q8] >> bit) & 1ull) << k;
+        uint32_t lsub = Layout == 0 ? (pat & 15) : ((pat & 3) | ((pat >> 2) & 12));
+        uint32_t rsub = Layout == 0 ? (pat >> 4) : (((pat >> 2) & 3) | ((pat >> 4) & 12));
+        l_out[x] = gtab[y1 * 32 + lsub];
+        gf_store(&r_out[x], gf_mul_tab(tab, gf_load(&gtab[y1 * 32 + 16 + rsub])));
+    }
+}
+
+// v[x] *= eq_hi[(x & (2^s - 1)) >> 8] for x < rows * 2^s, blocks as above
+__global__ __launch_bounds__(kT) void k_scale_cols_hi(uint4* __restrict__ v, uint64_t rows, int s,
+                                                      const uint4* __restrict__ eq_hi, uint32_t row_lanes) {
+    __shared__ uint4 tab[512];
+    const uint32_t g = blockIdx.x & ((1u << (s - 8)) - 1);
+    gf_table_build(tab, gf_load(&eq_hi[g]), threadIdx.x, blockDim.x);
+    __syncthreads();
+    const uint64_t c = ((uint64_t)g << 8) + threadIdx.x;
+    for (uint64_t y = blockIdx.x >> (s - 8); y < rows; y += row_lanes) {
+        uint4* p = &v[(y << s) + c];
+        gf_store(p, gf_mul_tab(tab, gf_load(p)));
     }
 }
 
@@ -1548,7 +1603,7 @@ int gkr_round_sums(bitz_ctx* ctx, const void* l, const void* r, uint64_t h, cons
     TimedLaunch tl(ctx, "gkr_round_sums", 4.0 * 16.0 * h, 4.0 * h * kGfMulOps);
     DenseSrc src{(const uint4*)l, (const uint4*)r};
     hipLaunchKernelGGL(k_gkr_round_sums<DenseSrc>, dim3(nb), dim3(kT), 0, ctx->stream, src, h,
-                       (const uint4*)eq_c, s_c, (const uint4*)eq_y, send_one, R, (uint4*)ctx->scratch.partials.ptr);
+                       (const uint4*)eq_c, s_c, (const uint4*)eq_y, send_one, R, (uint4*)ctx->scratch.partials.ptr, nullptr, 0u);
     BITZ_LAUNCH_CHECK();
     return sums_finish(ctx, nb, 2, out_dev);
 }
@@ -1602,13 +1657,15 @@ int gkr_premul_round_sums(bitz_ctx* ctx, const void* l, void* r, uint64_t h, con
     TimedLaunch tl(ctx, "gkr_premul_round_sums", 6.0 * 16.0 * h, 4.0 * h * kGfMulOps);
     PremulSrc src{(const uint4*)l, (uint4*)r, (const uint4*)eq_c, (1ull << s) - 1};
     hipLaunchKernelGGL((k_gkr_round_sums<PremulSrc, false>), dim3(nb), dim3(kT), 0, ctx->stream, src, h,
-                       (const uint4*)eq_c, s, (const uint4*)eq_y, send_one, R, (uint4*)ctx->scratch.partials.ptr);
+                       (const uint4*)eq_c, s, (const uint4*)eq_y, send_one, R, (uint4*)ctx->scratch.partials.ptr,
+                       nullptr, 0u);
     BITZ_LAUNCH_CHECK();
     return sums_finish(ctx, nb, 2, out_dev);
 }
 
 int gkr_round_sums_rowonly(bitz_ctx* ctx, const void* l, const void* r, uint64_t h, int s, const void* eq_y,
-                           int send_one, void* out_dev) {
+                           int send_one, void* out_dev, const void* eq_lo, int lo_bits) {
+    BITZ_REQUIRE(!eq_lo || (lo_bits >= 0 && lo_bits <= 8), BITZ_ERR_SHAPE, "gkr_round_sums_rowonly: lo_bits > 8");
     uint32_t R = round_chunk_r(ctx->tune, h, s);
     unsigned nb = round_blocks(ctx->tune, h, R);
     int rc = ctx->scratch.ensure((size_t)nb * 2 * 16);
@@ -1616,7 +1673,8 @@ int gkr_round_sums_rowonly(bitz_ctx* ctx, const void* l, const void* r, uint64_t
     TimedLaunch tl(ctx, "gkr_round_sums_rowonly", 4.0 * 16.0 * h, 2.0 * h * kGfMulOps);
     DenseSrc src{(const uint4*)l, (const uint4*)r};
     hipLaunchKernelGGL((k_gkr_round_sums<DenseSrc, false>), dim3(nb), dim3(kT), 0, ctx->stream, src, h,
-                       (const uint4*)nullptr, s, (const uint4*)eq_y, send_one, R, (uint4*)ctx->scratch.partials.ptr);
+                       (const uint4*)nullptr, s, (const uint4*)eq_y, send_one, R, (uint4*)ctx->scratch.partials.ptr,
+                       (const uint4*)eq_lo, (1u << lo_bits) - 1);
     BITZ_LAUNCH_CHECK();
     return sums_finish(ctx, nb, 2, out_dev);
 }
@@ -1665,7 +1723,10 @@ int tensor_table(bitz_ctx* ctx, const void* a, const void* b, int n, void* out) 
 }
 
 int gkr_fold_round_sums(bitz_ctx* ctx, void* l, void* r, uint64_t h, const uint8_t rho[16], const void* eq_c,
-                        int s_c, const void* eq_y, int send_one, void* out_dev, const void* rho_dev) {
+                        int s_c, const void* eq_y, int send_one, void* out_dev, const void* rho_dev,
+                        const void* eq_lo, int lo_bits) {
+    BITZ_REQUIRE(!eq_lo || (!eq_c && lo_bits >= 0 && lo_bits <= 8), BITZ_ERR_SHAPE,
+                 "gkr_fold_round_sums: eq_lo needs eq_c null and lo_bits <= 8");
     uint32_t R = round_chunk_r(ctx->tune, h, s_c);
     unsigned nb = round_blocks(ctx->tune, h, R);
     int rc = ctx->scratch.ensure((size_t)nb * 2 * 16);
@@ -1676,11 +1737,12 @@ int gkr_fold_round_sums(bitz_ctx* ctx, void* l, void* r, uint64_t h, const uint8
     if (eq_c) {
         TimedLaunch tl(ctx, "gkr_fold_round_sums", 12.0 * 16.0 * h, h * (4.0 * kGfMulOps + 4.0 * kGfTabMulOps));
         hipLaunchKernelGGL(k_gkr_round_sums<FoldSrc>, dim3(nb), dim3(kT), 0, ctx->stream, src, h, (const uint4*)eq_c,
-                           s_c, (const uint4*)eq_y, send_one, R, (uint4*)ctx->scratch.partials.ptr);
+                           s_c, (const uint4*)eq_y, send_one, R, (uint4*)ctx->scratch.partials.ptr, nullptr, 0u);
     } else {
         TimedLaunch tl(ctx, "gkr_fold_round_sums_rowonly", 12.0 * 16.0 * h, h * (2.0 * kGfMulOps + 4.0 * kGfTabMulOps));
         hipLaunchKernelGGL((k_gkr_round_sums<FoldSrc, false>), dim3(nb), dim3(kT), 0, ctx->stream, src, h, nullptr,
-                           s_c, (const uint4*)eq_y, send_one, R, (uint4*)ctx->scratch.partials.ptr);
+                           s_c, (const uint4*)eq_y, send_one, R, (uint4*)ctx->scratch.partials.ptr,
+                           (const uint4*)eq_lo, (1u << lo_bits) - 1);
     }
     BITZ_LAUNCH_CHECK();
     return sums_finish(ctx, nb, 2, out_dev);
@@ -1714,7 +1776,7 @@ int gkr_leaf_round_sums(bitz_ctx* ctx, const uint64_t* packed, const void* A, in
     TimedLaunch tl(ctx, "gkr_leaf_round_sums", (double)(1ull << (t + s - 3)), 4.0 * h * kGfMulOps);
     LeafSrc src{packed, (const uint4*)A, t, s};
     hipLaunchKernelGGL(k_gkr_round_sums<LeafSrc>, dim3(nb), dim3(kT), 0, ctx->stream, src, h,
-                       (const uint4*)eq_c, s_c, (const uint4*)eq_y, send_one, R, (uint4*)ctx->scratch.partials.ptr);
+                       (const uint4*)eq_c, s_c, (const uint4*)eq_y, send_one, R, (uint4*)ctx->scratch.partials.ptr, nullptr, 0u);
     BITZ_LAUNCH_CHECK();
     return sums_finish(ctx, nb, 2, out_dev);
 }
@@ -1780,10 +1842,32 @@ int gkr_leaf_round1_sums(bitz_ctx* ctx, const uint64_t* packed, const void* A, i
     return sums_finish(ctx, nb, 2, out_dev);
 }
 
+// Row lanes of a k_pattern_fold_hi / k_scale_cols_hi launch over `rows` rows
+// of 2^(s-8) runs: about 2048 blocks in all.
+static uint32_t hi_row_lanes(uint64_t rows, int s) {
+    const uint64_t want = std::max<uint64_t>(1, 2048ull >> std::min(s - 8, 11));
+    return (uint32_t)std::min<uint64_t>(rows, want);
+}
+
+int scale_cols_hi(bitz_ctx* ctx, void* v, uint64_t n, int s, const void* eq_hi) {
+    BITZ_REQUIRE(s > 8 && s <= 28 && n % (1ull << s) == 0 && eq_hi, BITZ_ERR_SHAPE,
+                 "scale_cols_hi: needs 8 < s <= 28 and n a multiple of 2^s");
+    const uint64_t rows = n >> s;
+    if (rows == 0) return BITZ_OK;
+    const uint32_t lanes = hi_row_lanes(rows, s);
+    TimedLaunch tl(ctx, "scale_cols_hi", 2.0 * 16.0 * n, (double)n * kGfTabMulOps);
+    hipLaunchKernelGGL(k_scale_cols_hi, dim3(lanes << (s - 8)), dim3(kT), 0, ctx->stream, (uint4*)v, rows, s,
+                       (const uint4*)eq_hi, lanes);
+    BITZ_LAUNCH_CHECK();
+    return BITZ_OK;
+}
+
 int gkr_leaf_fold2(bitz_ctx* ctx, const uint64_t* packed, const void* A, int t, int s, const uint8_t rho0[16],
                    const uint8_t rho1[16], const void* eq_c, void* l_out, void* r_out, const void* rho0_dev,
-                   const void* rho1_dev) {
+                   const void* rho1_dev, const void* eq_hi) {
     BITZ_REQUIRE(t >= 3, BITZ_ERR_SHAPE, "gkr_leaf_fold2: t < 3");
+    BITZ_REQUIRE(!eq_hi || (!eq_c && s > 8 && s <= 28), BITZ_ERR_SHAPE,
+                 "gkr_leaf_fold2: eq_hi needs eq_c null and 8 < s <= 28");
     uint64_t lo0, hi0, lo1, hi1;
     split16(rho0, &lo0, &hi0);
     split16(rho1, &lo1, &hi1);
@@ -1792,13 +1876,20 @@ int gkr_leaf_fold2(bitz_ctx* ctx, const uint64_t* packed, const void* A, int t, 
     uint4* ftab = (uint4*)ctx->scratch.leaf_tab.ptr;
     uint4* gtab = leaf_tab_hi(ctx, t);
     TimedLaunch tl(ctx, "gkr_leaf_fold2", 2.0 * 16.0 * h + (double)(h / 8),
-                   (q * 8.0 + q8 * 32.0) * kGfTabMulOps + (eq_c ? (double)h : 0.0) * kGfMulOps);
+                   (q * 8.0 + q8 * 32.0 + (eq_hi ? (double)h : 0.0)) * kGfTabMulOps +
+                       (eq_c ? (double)h : 0.0) * kGfMulOps);
     hipLaunchKernelGGL(k_leaf_ftab, dim3((unsigned)((q * 8 + kT - 1) / kT)), dim3(kT), 0, ctx->stream,
                        (const uint4*)A, t, lo0, hi0, (const uint4*)rho0_dev, ftab);
     hipLaunchKernelGGL(k_leaf_gtab, dim3((unsigned)((q8 * 32 + kT - 1) / kT)), dim3(kT), 0, ctx->stream,
                        (const uint4*)ftab, t, lo1, hi1, (const uint4*)rho1_dev, gtab);
-    hipLaunchKernelGGL(k_pattern_fold<0>, dim3(blocks_for(h, 4)), dim3(kT), 0, ctx->stream, packed, t, s,
-                       (const uint4*)gtab, (const uint4*)eq_c, (uint4*)l_out, (uint4*)r_out);
+    if (eq_hi) {
+        const uint32_t lanes = hi_row_lanes(q8, s);
+        hipLaunchKernelGGL(k_pattern_fold_hi<0>, dim3(lanes << (s - 8)), dim3(kT), 0, ctx->stream, packed, t, s,
+                           (const uint4*)gtab, (const uint4*)eq_hi, lanes, (uint4*)l_out, (uint4*)r_out);
+    } else {
+        hipLaunchKernelGGL(k_pattern_fold<0>, dim3(blocks_for(h, 4)), dim3(kT), 0, ctx->stream, packed, t, s,
+                           (const uint4*)gtab, (const uint4*)eq_c, (uint4*)l_out, (uint4*)r_out);
+    }
     BITZ_LAUNCH_CHECK();
     return BITZ_OK;
 }
@@ -1827,8 +1918,10 @@ int gkr_level1_round0_sums(bitz_ctx* ctx, const uint64_t* packed, const void* A,
 }
 
 int gkr_level1_fold(bitz_ctx* ctx, const uint64_t* packed, const void* A, int t, int s, const uint8_t rho[16],
-                    const void* eq_c, void* l_out, void* r_out, const void* rho_dev) {
+                    const void* eq_c, void* l_out, void* r_out, const void* rho_dev, const void* eq_hi) {
     BITZ_REQUIRE(t >= 3, BITZ_ERR_SHAPE, "gkr_level1_fold: t < 3");
+    BITZ_REQUIRE(!eq_hi || (!eq_c && s > 8 && s <= 28), BITZ_ERR_SHAPE,
+                 "gkr_level1_fold: eq_hi needs eq_c null and 8 < s <= 28");
     uint64_t lo, hi;
     split16(rho, &lo, &hi);
     const uint64_t Y1 = 1ull << (t - 1), Y3 = 1ull << (t - 3), h = 1ull << (t + s - 3);
@@ -1836,13 +1929,19 @@ int gkr_level1_fold(bitz_ctx* ctx, const uint64_t* packed, const void* A, int t,
     uint4* vtab = (uint4*)ctx->scratch.leaf_tab.ptr;
     uint4* gtab = leaf_tab_hi(ctx, t);
     TimedLaunch tl(ctx, "gkr_level1_fold", 2.0 * 16.0 * h + (double)(h / 8),
-                   (Y1 + (eq_c ? (double)h : 0.0)) * kGfMulOps + Y3 * 32.0 * kGfTabMulOps);
+                   (Y1 + (eq_c ? (double)h : 0.0)) * kGfMulOps + (Y3 * 32.0 + (eq_hi ? (double)h : 0.0)) * kGfTabMulOps);
     hipLaunchKernelGGL(k_level1_vtab, dim3((unsigned)((Y1 + kT - 1) / kT)), dim3(kT), 0, ctx->stream,
                        (const uint4*)A, t, vtab);
     hipLaunchKernelGGL(k_level1_gtab, dim3((unsigned)((Y3 * 32 + kT - 1) / kT)), dim3(kT), 0, ctx->stream,
                        (const uint4*)vtab, t, lo, hi, (const uint4*)rho_dev, gtab);
-    hipLaunchKernelGGL(k_pattern_fold<1>, dim3(blocks_for(h, 4)), dim3(kT), 0, ctx->stream, packed, t, s,
-                       (const uint4*)gtab, (const uint4*)eq_c, (uint4*)l_out, (uint4*)r_out);
+    if (eq_hi) {
+        const uint32_t lanes = hi_row_lanes(Y3, s);
+        hipLaunchKernelGGL(k_pattern_fold_hi<1>, dim3(lanes << (s - 8)), dim3(kT), 0, ctx->stream, packed, t, s,
+                           (const uint4*)gtab, (const uint4*)eq_hi, lanes, (uint4*)l_out, (uint4*)r_out);
+    } else {
+        hipLaunchKernelGGL(k_pattern_fold<1>, dim3(blocks_for(h, 4)), dim3(kT), 0, ctx->stream, packed, t, s,
+                           (const uint4*)gtab, (const uint4*)eq_c, (uint4*)l_out, (uint4*)r_out);
+    }
     BITZ_LAUNCH_CHECK();
     return BITZ_OK;
 }

@@ -36,9 +36,13 @@ int pack_columns(bitz_ctx* ctx, const uint64_t* rows, int t, int s, uint64_t* pa
 int gkr_round_sums(bitz_ctx* ctx, const void* l, const void* r, uint64_t h, const void* eq_c, int s_c,
                    const void* eq_y, int send_one, void* out_dev);
 // the previous round's fold by rho (halves of 2h entries -> h) fused with
-// this round's sums over the folded halves
+// this round's sums over the folded halves. eq_lo (2^lo_bits entries, with
+// eq_c null): the right half carries the column weights of the index bits
+// lo_bits..s-1 only, and the sums take eq_lo[x & (2^lo_bits - 1)] as well
+// (lo_bits <= 8)
 int gkr_fold_round_sums(bitz_ctx* ctx, void* l, void* r, uint64_t h, const uint8_t rho[16], const void* eq_c,
-                        int s_c, const void* eq_y, int send_one, void* out_dev, const void* rho_dev = nullptr);
+                        int s_c, const void* eq_y, int send_one, void* out_dev, const void* rho_dev = nullptr,
+                        const void* eq_lo = nullptr, int lo_bits = 0);
 int gkr_leaf_round_sums(bitz_ctx* ctx, const uint64_t* packed, const void* A, int t, int s, const void* eq_c,
                         int s_c, const void* eq_y, int send_one, void* out_dev);
 int fold_halves(bitz_ctx* ctx, void* v, uint64_t h, const uint8_t rho[16]);
@@ -53,17 +57,22 @@ int gkr_leaf_round1_sums(bitz_ctx* ctx, const uint64_t* packed, const void* A, i
                          const void* eq_c, const void* eq_y, int send_one, void* out_dev,
                          const void* rho0_dev = nullptr);
 // the leaf layer folded by rho0 then rho1 into halves of 2^(t+s-3) entries;
-// with eq_c the right half is written multiplied by eq_c[c] (c = low s bits)
+// with eq_c the right half is written multiplied by eq_c[c] (c = low s bits),
+// with eq_hi (s > 8, eq_c null) by eq_hi[c >> 8]
 int gkr_leaf_fold2(bitz_ctx* ctx, const uint64_t* packed, const void* A, int t, int s, const uint8_t rho0[16],
                    const uint8_t rho1[16], const void* eq_c, void* l_out, void* r_out, const void* rho0_dev = nullptr,
-                   const void* rho1_dev = nullptr);
+                   const void* rho1_dev = nullptr, const void* eq_hi = nullptr);
 // round sums whose right half gets eq_c[x & (2^s-1)] multiplied in (in place);
 // weights eq_y[x >> s] only
 int gkr_premul_round_sums(bitz_ctx* ctx, const void* l, void* r, uint64_t h, const void* eq_c, int s,
                           const void* eq_y, int send_one, void* out_dev);
-// round sums over halves whose right operand already carries eq_c: weights eq_y[x >> s]
+// round sums over halves whose right operand already carries eq_c: weights
+// eq_y[x >> s]; with eq_lo the right operand carries the weights of column
+// bits lo_bits..s-1 only and eq_lo[x & (2^lo_bits - 1)] weighs the slots too
 int gkr_round_sums_rowonly(bitz_ctx* ctx, const void* l, const void* r, uint64_t h, int s, const void* eq_y,
-                           int send_one, void* out_dev);
+                           int send_one, void* out_dev, const void* eq_lo = nullptr, int lo_bits = 0);
+// v[x] *= eq_hi[(x & (2^s - 1)) >> 8] for x < n (n a multiple of 2^s, s > 8)
+int scale_cols_hi(bitz_ctx* ctx, void* v, uint64_t n, int s, const void* eq_hi);
 // out[x] = prod_i (bit i of x ? b[i] : a[i]), x < 2^n, n <= 14
 int tensor_table(bitz_ctx* ctx, const void* a, const void* b, int n, void* out);
 // the layer over level 1 (never materialised) from 8-bit row patterns:
@@ -72,7 +81,8 @@ int tensor_table(bitz_ctx* ctx, const void* a, const void* b, int n, void* out);
 int gkr_level1_round0_sums(bitz_ctx* ctx, const uint64_t* packed, const void* A, int t, int s, const void* eq_c,
                            const void* eq_y, int send_one, void* out_dev);
 int gkr_level1_fold(bitz_ctx* ctx, const uint64_t* packed, const void* A, int t, int s, const uint8_t rho[16],
-                    const void* eq_c, void* l_out, void* r_out, const void* rho_dev = nullptr);
+                    const void* eq_c, void* l_out, void* r_out, const void* rho_dev = nullptr,
+                    const void* eq_hi = nullptr);
 // levels 2 and 3 of the product tree (2^(t+s-2), 2^(t+s-3) entries) from the bits
 int gkr_level2(bitz_ctx* ctx, const uint64_t* packed, const void* A, int t, int s, void* out);
 int gkr_level3(bitz_ctx* ctx, const uint64_t* packed, const void* A, int t, int s, void* out);

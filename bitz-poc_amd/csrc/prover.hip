@@ -462,6 +462,13 @@ struct Prover {
     G g;
     // device buffers
     DevBuf packed, rowimg, colimg, folds, roww, levels, eqc, eqy, eqcinv, tmp0, tmp1, tmp2, tmp3, qdev, pt, fsbuf;
+    // Split column weights of a weighted layer: eq_c[c] = eqlo[c & (2^wlo - 1)]
+    // * eqhi[c >> wlo] (wlo = min(s, 8)). The right half carries eqhi only:
+    // its multiplies are window-table products over 256-column runs, and
+    // the row-only round sums weigh each thread's slots by one eqlo entry.
+    DevBuf eqlo, eqhi;
+    bool wsplit = false;
+    int wlo = 0;
     std::vector<DevBuf*> lig_bufs;
 
     // through the pinned upload ring: no stream synchronisation, and the
@@ -483,6 +490,19 @@ struct Prover {
         return BITZ_OK;
     }
     // eq table over host coordinates into `buf`
+    // eqhi for a pattern fold of a weighted layer (none when s <= 8: the
+    // high bits' factor is then 1)
+    const void* hi_weights(bool weigh) const { return weigh && wsplit && s > wlo ? eqhi.ptr : nullptr; }
+
+    // Round 0 of a weighted layer: the right half (2h entries) takes the
+    // column weights (eqhi only when split), then the row-only sums.
+    int premul_round(const uint4* l, uint4* r, uint64_t hh, int send_one) {
+        if (!wsplit) return gkr_premul_round_sums(ctx, l, r, hh, eqc.ptr, s, eqy.ptr, send_one, nullptr);
+        if (s > wlo)
+            if (int rc = scale_cols_hi(ctx, r, 2 * hh, s, eqhi.ptr)) return rc;
+        return gkr_round_sums_rowonly(ctx, l, r, hh, s, eqy.ptr, send_one, nullptr, eqlo.ptr, wlo);
+    }
+
     int eq_dev(const std::vector<G>& r, DevBuf& buf) {
         if ((int)r.size() <= kMaxTensorN) {
             if (!buf.alloc((16ull << r.size()))) return BITZ_ERR_ALLOC;
@@ -560,6 +580,13 @@ struct Prover {
         bool weigh = per_layer && s > 0 && L - 1 >= s;
         for (int i = 0; weigh && i < s; i++)
             if (geq(ext[i], gzero()) || geq(ext[i], gone())) weigh = false;
+        wsplit = weigh && s <= 28;
+        wlo = std::min(s, 8);
+        if (wsplit) {
+            std::vector<G> clo(ext.begin(), ext.begin() + wlo), chi(ext.begin() + wlo, ext.begin() + s);
+            if ((rc = eq_dev(clo, eqlo))) return rc;
+            if (s > wlo && (rc = eq_dev(chi, eqhi))) return rc;
+        }
         if (weigh) {
             std::vector<G> v(2 * s), inv(2 * s);
             for (int i = 0; i < s; i++) {
@@ -576,6 +603,8 @@ struct Prover {
                 inv[i] = hgf_mul(inv[i], cur);
                 cur = hgf_mul(cur, v[i]);
             }
+            // split weights: r never carries the low bits' factors
+            for (int i = 0; wsplit && i < wlo; i++) inv[i] = inv[s + i] = gone();
             if (!eqcinv.alloc(16ull << s)) return BITZ_ERR_ALLOC;
             if ((rc = tensor_args(ctx, inv.data(), inv.data() + s, s, eqcinv.ptr))) return rc;
         }
@@ -623,7 +652,8 @@ struct Prover {
                 rc = gkr_level1_round0_sums(ctx, (const uint64_t*)packed.ptr, rowimg.ptr, t, s, eqc.ptr, eqy.ptr,
                                             send_one, nullptr);
             else if (pending_fold && weighted && rem >= s) {
-                rc = gkr_fold_round_sums(ctx, l, r, hh, pending_rho, nullptr, s, eqy.ptr, send_one, nullptr);
+                rc = gkr_fold_round_sums(ctx, l, r, hh, pending_rho, nullptr, s, eqy.ptr, send_one, nullptr, nullptr,
+                                         wsplit ? eqlo.ptr : nullptr, wlo);
             } else {
                 if (pending_fold && (rc = fold_pair(ctx, l, r, 2 * hh, pending_rho))) return rc;
                 if (weighted && rem < s) {
@@ -632,9 +662,10 @@ struct Prover {
                     weighted = false;
                 }
                 if (weighted) {
-                    rc = gkr_round_sums_rowonly(ctx, l, r, hh, s, eqy.ptr, send_one, nullptr);
+                    rc = gkr_round_sums_rowonly(ctx, l, r, hh, s, eqy.ptr, send_one, nullptr,
+                                                wsplit ? eqlo.ptr : nullptr, wlo);
                 } else if (weigh && round == 0) {
-                    rc = gkr_premul_round_sums(ctx, l, r, hh, eqc.ptr, s, eqy.ptr, send_one, nullptr);
+                    rc = premul_round(l, r, hh, send_one);
                     weighted = true;
                 } else {
                     rc = gkr_round_sums(ctx, l, r, hh, eqc.ptr, s_c, eqy.ptr, send_one, nullptr);
@@ -654,7 +685,7 @@ struct Prover {
             next.push_back(rr);
             if (kind == kLevel1 && round == 0) {
                 if ((rc = gkr_level1_fold(ctx, (const uint64_t*)packed.ptr, rowimg.ptr, t, s, rb,
-                                          weigh ? eqc.ptr : nullptr, l, r)))
+                                          weigh && !wsplit ? eqc.ptr : nullptr, l, r, nullptr, hi_weights(weigh))))
                     return rc;
                 weighted = weigh;
                 pending_fold = false;
@@ -664,7 +695,8 @@ struct Prover {
                 pending_fold = false;
             } else if (leaf && round == 1) {
                 if ((rc = gkr_leaf_fold2(ctx, (const uint64_t*)packed.ptr, rowimg.ptr, t, s, rho0, rb,
-                                         weigh ? eqc.ptr : nullptr, l, r)))
+                                         weigh && !wsplit ? eqc.ptr : nullptr, l, r, nullptr, nullptr,
+                                         hi_weights(weigh))))
                     return rc;
                 weighted = weigh;
                 pending_fold = false;
@@ -829,7 +861,8 @@ struct Prover {
                 rc = gkr_level1_round0_sums(ctx, (const uint64_t*)packed.ptr, rowimg.ptr, t, s, eqc.ptr, eqy.ptr,
                                             send_one, nullptr);
             else if (pending_fold && weighted && rem >= s) {
-                rc = gkr_fold_round_sums(ctx, l, r, hh, nullptr, nullptr, s, eqy.ptr, send_one, nullptr, pending_rho);
+                rc = gkr_fold_round_sums(ctx, l, r, hh, nullptr, nullptr, s, eqy.ptr, send_one, nullptr, pending_rho,
+                                         wsplit ? eqlo.ptr : nullptr, wlo);
             } else {
                 if (pending_fold && (rc = fold_pair(ctx, l, r, 2 * hh, nullptr, pending_rho))) return rc;
                 if (weighted && rem < s) {
@@ -837,9 +870,10 @@ struct Prover {
                     weighted = false;
                 }
                 if (weighted) {
-                    rc = gkr_round_sums_rowonly(ctx, l, r, hh, s, eqy.ptr, send_one, nullptr);
+                    rc = gkr_round_sums_rowonly(ctx, l, r, hh, s, eqy.ptr, send_one, nullptr,
+                                                wsplit ? eqlo.ptr : nullptr, wlo);
                 } else if (weigh && round == 0) {
-                    rc = gkr_premul_round_sums(ctx, l, r, hh, eqc.ptr, s, eqy.ptr, send_one, nullptr);
+                    rc = premul_round(l, r, hh, send_one);
                     weighted = true;
                 } else {
                     rc = gkr_round_sums(ctx, l, r, hh, eqc.ptr, s_c, eqy.ptr, send_one, nullptr);
@@ -852,14 +886,15 @@ struct Prover {
             pending_fold = false;
             if (kind == kLevel1 && round == 0) {
                 if ((rc = gkr_level1_fold(ctx, (const uint64_t*)packed.ptr, rowimg.ptr, t, s, nullptr,
-                                          weigh ? eqc.ptr : nullptr, l, r, rho)))
+                                          weigh && !wsplit ? eqc.ptr : nullptr, l, r, rho, hi_weights(weigh))))
                     return rc;
                 weighted = weigh;
             } else if (leaf && round == 0) {
                 // leaves stay implicit: round 1 reads the bits again
             } else if (leaf && round == 1) {
                 if ((rc = gkr_leaf_fold2(ctx, (const uint64_t*)packed.ptr, rowimg.ptr, t, s, nullptr, nullptr,
-                                         weigh ? eqc.ptr : nullptr, l, r, rho, rho + 1)))
+                                         weigh && !wsplit ? eqc.ptr : nullptr, l, r, rho, rho + 1,
+                                         hi_weights(weigh))))
                     return rc;
                 weighted = weigh;
             } else {

@@ -291,6 +291,13 @@ __global__ __launch_bounds__(kT) void k_gkr_round_sums(Src src, uint64_t h, cons
     const uint64_t chunk = (uint64_t)kT * R;
     const uint64_t nchunks = (h + chunk - 1) / chunk;
     const uint64_t cmask = (1ull << s_c) - 1;
+    // column weights of the low index bits, left out of the right half:
+    // chunks start at multiples of kT, so every slot x of this thread has
+    // x & (kT - 1) = threadIdx.x and one factor weighs all of them, taken
+    // into each chunk's row weight (Src::kLoPerChunk: one product per chunk)
+    // or into the thread's two sums at the end (two products per thread;
+    // measured the faster form for DenseSrc, the other for FoldSrc)
+    const gf128 wl = eq_lo ? gf_load(&eq_lo[threadIdx.x & lo_mask]) : gf_one();
     for (uint64_t ch = blockIdx.x; ch < nchunks; ch += gridDim.x) {
         const uint64_t x0 = ch * chunk + threadIdx.x;
         if (x0 >= h) continue;
@@ -313,14 +320,17 @@ __global__ __launch_bounds__(kT) void k_gkr_round_sums(Src src, uint64_t h, cons
             }
         }
         gf128 wy = gf_load(&eq_y[x0 >> s_c]);
+        if (Src::kLoPerChunk && eq_lo) {
+            // opaque per chunk, so the product's operand masks of wl are not
+            // hoisted out of the loop (they would hold ~36 more VGPRs)
+            gf128 w = wl;
+            asm volatile("" : "+v"(w.w[0]), "+v"(w.w[1]), "+v"(w.w[2]), "+v"(w.w[3]));
+            wy = gf_mul(wy, w);
+        }
         acc[0] = gf_add(acc[0], gf_mul(gf_reduce(w_end), wy));
         acc[1] = gf_add(acc[1], gf_mul(gf_reduce(w_inf), wy));
     }
-    if (eq_lo) {
-        // column weights of the low index bits, left out of the right half:
-        // chunks start at multiples of kT, so every slot x of this thread has
-        // x & (kT - 1) = threadIdx.x and one factor weighs all of them
-        const gf128 wl = gf_load(&eq_lo[threadIdx.x & lo_mask]);
+    if (!Src::kLoPerChunk && eq_lo) {
         acc[0] = gf_mul(acc[0], wl);
         acc[1] = gf_mul(acc[1], wl);
     }
@@ -343,6 +353,7 @@ static unsigned round_blocks(const Tune& tn, uint64_t h, uint32_t R) {
 
 // materialised halves l, r (each 2h entries)
 struct DenseSrc {
+    static constexpr bool kLoPerChunk = false;
     static constexpr bool kTable = false;
     const uint4* l;
     const uint4* r;
@@ -355,6 +366,7 @@ struct DenseSrc {
 // replaces r[x], r[x+h] by eq_c[x & cmask] * r[x], eq_c[.] * r[x+h] and
 // returns those (each r entry is read and written by its own slot only).
 struct PremulSrc {
+    static constexpr bool kLoPerChunk = false;
     static constexpr bool kTable = false;
     const uint4* l;
     uint4* r;
@@ -375,6 +387,7 @@ struct PremulSrc {
 // only x and x+h and reads only x, x+h, x+2h, x+3h (x < h): no other slot
 // touches those, so folding in place is safe.
 struct FoldSrc {
+    static constexpr bool kLoPerChunk = true;
     static constexpr bool kTable = true;
     uint4* l;
     uint4* r;
@@ -399,6 +412,7 @@ struct FoldSrc {
 // leaf layer: leaf(row, c) = bit(c, row) ? A[row] : 1, halves of 2^(t+s-1)
 // entries (l = rows < 2^(t-1)), so round 0 has h = 2^(t+s-2).
 struct LeafSrc {
+    static constexpr bool kLoPerChunk = false;
     static constexpr bool kTable = false;
     const uint64_t* packed;  // row-major packed columns
     const uint4* A;

@@ -40,8 +40,9 @@ VALU_PEAK_TOPS = 256 * 64 * 2.4e9 / 1e12
 # x1.0), and SQ_INSTS_VALU (wave instructions; 64 lane operations each).
 PMC = {
     24: {
-        "gkr_fold_round_sums_rowonly": {"mib": 280.29 + 139.98, "valu_insts": 4.986e+07},
-        "gkr_premul_round_sums": {"mib": 589.42 + 292.58, "valu_insts": 2.486e+08},
+        "gkr_fold_round_sums_rowonly": {"mib": 280.37 + 139.98, "valu_insts": 5.101e+07},
+        "gkr_round_sums_rowonly": {"mib": 1024.69 + 0.03, "valu_insts": 2.240e+08},
+        "scale_cols_hi": {"mib": 292.60 + 292.55, "valu_insts": 3.614e+07},
         "product_level": {"mib": 315.10 + 157.52, "valu_insts": 6.925e+07},
         "ntt_pass": {"mib": 318.73 + 326.40, "valu_insts": 1.335e+08},
     },

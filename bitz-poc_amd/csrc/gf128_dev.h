@@ -45,6 +45,13 @@ __device__ __forceinline__ gf128 gf_load(const uint4* p) {
 __device__ __forceinline__ void gf_store(uint4* p, gf128 a) {
     *p = make_uint4(a.w[0], a.w[1], a.w[2], a.w[3]);
 }
+// Streaming store (nontemporal hint) for outputs far larger than the caches
+// that the next pass reads from HBM anyway.
+__device__ __forceinline__ void gf_store_nt(uint4* p, gf128 a) {
+    typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+    u32x4 v = {a.w[0], a.w[1], a.w[2], a.w[3]};
+    __builtin_nontemporal_store(v, (u32x4*)p);
+}
 
 // 3-input XOR (v_bitop3_b32, truth table 0x96), per 32-bit half for u64
 __device__ __forceinline__ uint32_t xor3(uint32_t a, uint32_t b, uint32_t c) {

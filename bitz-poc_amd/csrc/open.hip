@@ -664,8 +664,8 @@ __global__ __launch_bounds__(kT) void k_pattern_fold_hi(const uint64_t* __restri
         for (int k = 0; k < 8; k++) pat |= (uint32_t)((col[y1 + k * q8] >> bit) & 1ull) << k;
         uint32_t lsub = Layout == 0 ? (pat & 15) : ((pat & 3) | ((pat >> 2) & 12));
         uint32_t rsub = Layout == 0 ? (pat >> 4) : (((pat >> 2) & 3) | ((pat >> 4) & 12));
-        l_out[x] = gtab[y1 * 32 + lsub];
-        gf_store(&r_out[x], gf_mul_tab(tab, gf_load(&gtab[y1 * 32 + 16 + rsub])));
+        gf_store_nt(&l_out[x], gf_load(&gtab[y1 * 32 + lsub]));
+        gf_store_nt(&r_out[x], gf_mul_tab(tab, gf_load(&gtab[y1 * 32 + 16 + rsub])));
     }
 }
 
@@ -830,7 +830,7 @@ __global__ __launch_bounds__(kT) void k_level_pattern_lds(const uint64_t* __rest
         uint32_t p = 0;
 #pragma unroll
         for (int k = 0; k < K; k++) p |= (uint32_t)((col[off[k]] >> bit) & 1ull) << k;
-        out[(y << s) + c] = tab[p];
+        gf_store_nt(&out[(y << s) + c], gf_load(&tab[p]));
     }
 }
 

@@ -175,7 +175,7 @@ __global__ __launch_bounds__(kThreads) void k_ntt_pass_tab(uint4* __restrict__ d
 #pragma unroll
     for (int j = 0; j < (1 << F); j++) {
         uint64_t p = base + ((uint64_t)j << sh);
-        gf_store(&data[(p << log_lanes) + lane], v[j]);
+        gf_store_nt(&data[(p << log_lanes) + lane], v[j]);
     }
 }
 

@@ -399,7 +399,7 @@ struct FoldSrc {
     __device__ __forceinline__ gf128 fold(const uint4* tab, uint4* v, uint64_t i, uint64_t h) const {
         gf128 a = gf_load(&v[i]), b = gf_load(&v[i + 2 * h]);
         gf128 o = gf_add(a, gf_mul_tab(tab, gf_add(a, b)));
-        gf_store(&v[i], o);
+        gf_store_nt(&v[i], o);
         return o;
     }
     __device__ __forceinline__ RoundVals at(uint64_t x, uint64_t h, const uint4* tab) const {
@@ -679,7 +679,7 @@ __global__ __launch_bounds__(kT) void k_scale_cols_hi(uint4* __restrict__ v, uin
     const uint64_t c = ((uint64_t)g << 8) + threadIdx.x;
     for (uint64_t y = blockIdx.x >> (s - 8); y < rows; y += row_lanes) {
         uint4* p = &v[(y << s) + c];
-        gf_store(p, gf_mul_tab(tab, gf_load(p)));
+        gf_store_nt(p, gf_mul_tab(tab, gf_load(p)));
     }
 }
 

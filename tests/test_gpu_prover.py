@@ -18,9 +18,13 @@ def _rows(t, s, seed):
 
 
 @pytest.mark.parametrize("device_fs", [1, 0])
-@pytest.mark.parametrize("t,s,rate,k0,seed", [(11, 9, 2, 4, 5), (12, 8, 1, 4, 6), (10, 10, 3, 3, 7)])
+@pytest.mark.parametrize("t,s,rate,k0,seed",
+                         [(11, 9, 2, 4, 5), (12, 8, 1, 4, 6), (10, 10, 3, 3, 7), (14, 6, 2, 4, 8)])
 def test_gpu_proof_bytes_equal_oracle(t, s, rate, k0, seed, device_fs):
-    """with the GKR transcript on the device (default) and on the host"""
+    """with the GKR transcript on the device (default) and on the host; the
+    column counts cover s > 8 (the right half carries the weights of column
+    bits 8..s-1, the round sums those of bits 0..7), s = 8 and s < 8 (the
+    round sums carry every column weight)"""
     from bitz_amd import _lib
     from bitz_amd.device import Commitment
     rows = _rows(t, s, seed)

@@ -535,7 +535,7 @@ static int run_ntt(bitz_ctx* ctx, uint4* data, const uint4* src, uint64_t src_ma
 int commit_device(bitz_ctx* ctx, const void* packed, int log_msg_len, int log_batch,
                   int log_inv_rate, void* codeword, void* tree) {
     BITZ_REQUIRE(ctx && packed && codeword && tree, BITZ_ERR_ARG, "commit: null argument");
-    // PcsParams::validate (commit.rs:125-136) plus this build's limits
+    // PcsParams::validate (commit.rs:125-136) plus the library's own size limits
     BITZ_REQUIRE(log_batch >= 0 && log_batch <= 6, BITZ_ERR_SHAPE,
                  "commit: log_batch must be in [0, 6] (leaf <= 1024 B)");
     BITZ_REQUIRE(log_msg_len > log_batch, BITZ_ERR_SHAPE,

@@ -285,10 +285,10 @@ void Transcript::absorb_slice(const void* data, size_t len) {
     head[0] = 0x06;
     uint64_t l = len;
     memcpy(head + 1, &l, 8);
-    h_.update(head, 9);
-    h_.update(data, len);
+    put(head, 9);
+    put(data, len);
     uint8_t tail = 0x07;
-    h_.update(&tail, 1);
+    put(&tail, 1);
 }
 
 void Transcript::absorb_slices(const void* data, size_t n, size_t len) {
@@ -303,7 +303,7 @@ void Transcript::absorb_slices(const void* data, size_t n, size_t len) {
         memcpy(q + 9, p + i * len, len);
         q[9 + len] = 0x07;
     }
-    h_.update(stream.data(), stream.size());
+    put(stream.data(), stream.size());
 }
 
 Transcript::SlicePlan Transcript::slice_plan(size_t len) const {
@@ -333,18 +333,18 @@ void Transcript::absorb_slice_runs(size_t len, const SlicePlan& plan, const uint
 void Transcript::challenge(uint8_t* out, size_t len) {
     h_.finalize_xof(out, len);
     uint8_t a = 0x12, b = 0x34;
-    h_.update(&a, 1);
-    h_.update(out, len);
-    h_.update(&b, 1);
+    put(&a, 1);
+    put(out, len);
+    put(&b, 1);
 }
 
 void Transcript::sampling_bytes(uint8_t* out, size_t len) {
     h_.finalize_xof(out, len);
     static const char kFrame[] = "bitz/sampling-read/v1";
-    h_.update(kFrame, sizeof(kFrame) - 1);
+    put(kFrame, sizeof(kFrame) - 1);
     uint64_t l = len;
-    h_.update(&l, 8);
-    h_.update(out, len);
+    put(&l, 8);
+    put(out, len);
 }
 
 InnerTranscript::InnerTranscript(const std::string& session, const uint8_t* instance, size_t instance_len) {
@@ -446,10 +446,15 @@ std::string custom_johnson_config(int m, int r0, int k0, int target_bits, LigCon
         if (lv.queries > positions) return "custom Johnson level is too thin for its query count";
         double pg = kLogQ - johnson_log_a(lv.log_inv_rate, cfg.eta, lv.log_msg_cols, lv.log_num_interleaved);
         lv.fold_grinding_bits = (int)std::max(0.0, ceil((double)target_bits - pg));
+        lv.eps_pg = pg;
+        lv.eps_query = (double)lv.queries * pq;
+        const int mu = lv.log_msg_cols + lv.log_num_interleaved;
+        lv.eps_ood = ood_bits(lv.log_inv_rate, cfg.eta, mu, lv.ood_samples);
         if (i > 0) {
-            int mu = lv.log_msg_cols + lv.log_num_interleaved;
-            while (ood_bits(lv.log_inv_rate, cfg.eta, mu, lv.ood_samples) + 1e-3 < (double)target_bits)
+            while (lv.eps_ood + 1e-3 < (double)target_bits) {
                 lv.ood_samples++;
+                lv.eps_ood = ood_bits(lv.log_inv_rate, cfg.eta, mu, lv.ood_samples);
+            }
         }
         cfg.levels.push_back(lv);
     }

@@ -8,6 +8,8 @@
 #include <string>
 #include <vector>
 
+#include "modq.h"
+
 namespace bitz {
 
 // The state of an incremental BLAKE3 hasher as plain data, for continuing a
@@ -94,9 +96,20 @@ class Transcript {
     void sampling_bytes(uint8_t* out, size_t len);
     void digest(uint8_t out[32]) const { h_.finalize_xof(out, 32); }
     void export_state(B3State* s) const { h_.export_state(s); }
+    // absorb_inner: bytes into the hash with no framing (a statement
+    // frame's fields, src/ligerito_flock.rs:1083-1098)
+    void absorb_inner(const void* data, size_t len) { put(data, len); }
+    // Every byte the hash receives from now on is also appended to tape
+    // (nullptr stops it); absorb_slice_runs is not taped.
+    void set_tape(std::vector<uint8_t>* tape) { tape_ = tape; }
 
    private:
+    void put(const void* data, size_t len) {
+        h_.update(data, len);
+        if (tape_) tape_->insert(tape_->end(), (const uint8_t*)data, (const uint8_t*)data + len);
+    }
     Blake3 h_;
+    std::vector<uint8_t>* tape_ = nullptr;
 };
 
 // Field element / integer wire forms
@@ -143,6 +156,9 @@ class InnerTranscript {
 struct LigLevel {
     int log_inv_rate, log_msg_cols, log_num_interleaved, k_recursive;
     int queries, grinding_bits, fold_grinding_bits, ood_samples;
+    // the level's soundness diagnostics as the security config carries them
+    // (expected_eps_pg_bits, expected_eps_query_bits, expected_eps_ood_bits)
+    double eps_pg, eps_query, eps_ood;
 };
 struct LigConfig {
     int m, log_n, initial_k, yr_log_n, target_bits;
@@ -152,5 +168,30 @@ struct LigConfig {
 };
 // returns empty string on success, else the rejection reason
 std::string custom_johnson_config(int m, int r0, int k0, int target_bits, LigConfig* out);
+
+// ---- the outer transcript's statement, Round 0 and instance (outer.cpp) ----
+// bincode (1.3, fixint) of the ladder's LigeritoSecurityConfig and its
+// BLAKE3 digest, the ladder identity bound after the statement
+void ligerito_bincode(const LigConfig& cfg, std::vector<uint8_t>* out);
+void ligerito_digest(const LigConfig& cfg, uint8_t out[32]);
+// min(113, 126 - t): the standalone prime's bit width
+int standalone_q_bits(int t);
+// the standalone statement frame, the session label and the ladder digest
+void bind_standalone_statement(Transcript& tr, const uint8_t root[32], const LigConfig& cfg, int t, int s,
+                               const uint8_t generator[16]);
+// Round 0's header: its domain, the packed variable count, the grinding bits
+void ood_round_header(Transcript& tr, int packed_vars, uint32_t bits);
+// a typed grinding boundary: frame, domain, index, difficulty, then the
+// 32-byte seed; the found (or checked) nonce is absorbed after it
+void grinding_seed(Transcript& tr, const char* domain, uint64_t index, uint32_t bits, uint8_t seed[32]);
+void absorb_grinding_nonce(Transcript& tr, uint64_t nonce);
+// the claim frame (q, claimed value) of a standalone opening
+void absorb_standalone_claim(Transcript& tr, u128 q, u128 claimed);
+// the prime search over [lo, hi] with its policy bound first; false with
+// *err set when the search is exhausted or the interval is invalid
+bool sample_prime_context(Transcript& tr, u128 lo, u128 hi, uint32_t security_bits, u128* out, std::string* err);
+bool sample_prime_with_bits(Transcript& tr, int bits, u128* out, std::string* err);
+// a 256-bit draw reduced mod q (two 16-byte challenges, low first)
+u128 sample_residue(Transcript& tr, u128 q);
 
 }  // namespace bitz

@@ -17,6 +17,7 @@
 
 #include "host_gf.h"
 #include "host_proto.h"
+#include "modq.h"
 #include "open_impl.h"
 
 namespace bitz {
@@ -78,125 +79,6 @@ std::vector<G> host_eq(const std::vector<G>& r) {
     return t;
 }
 
-// --------------------------------------------------------- mod-q integers
-typedef unsigned __int128 u128;
-
-u128 addmod(u128 a, u128 b, u128 q) {
-    u128 s = a + b;  // q < 2^126: no overflow
-    return s >= q ? s - q : s;
-}
-u128 mulmod_serial(u128 a, u128 b, u128 q) {
-    u128 r = 0;
-    a %= q;
-    for (int i = 127; i >= 0; i--) {
-        r = addmod(r, r, q);
-        if ((b >> i) & 1) r = addmod(r, a, q);
-    }
-    return r;
-}
-
-// Montgomery arithmetic modulo an odd n < 2^126 with R = 2^128.
-struct ModN {
-    u128 n = 0, ninv = 0, r2 = 0;  // ninv = -n^-1 mod R, r2 = R^2 mod n
-    void set(u128 n_) {
-        n = n_;
-        u128 x = n;  // inverse of n modulo 2^3; each Newton step doubles the bits
-        for (int i = 0; i < 7; i++) x *= 2 - n * x;
-        ninv = (u128)0 - x;
-        u128 r = (~(u128)0 % n + 1) % n;  // R mod n
-        r2 = mulmod_serial(r, r, n);
-    }
-    static void wide(u128 a, u128 b, u128* hi, u128* lo) {
-        uint64_t a0 = (uint64_t)a, a1 = (uint64_t)(a >> 64), b0 = (uint64_t)b, b1 = (uint64_t)(b >> 64);
-        u128 p00 = (u128)a0 * b0, p01 = (u128)a0 * b1, p10 = (u128)a1 * b0, p11 = (u128)a1 * b1;
-        u128 mid = (p00 >> 64) + (uint64_t)p01 + (uint64_t)p10;
-        *lo = (uint64_t)p00 | (mid << 64);
-        *hi = p11 + (p01 >> 64) + (p10 >> 64) + (mid >> 64);
-    }
-    // (hi R + lo) R^-1 mod n for hi R + lo < R n
-    u128 redc(u128 hi, u128 lo) const {
-        u128 m = lo * ninv, mh, ml;
-        wide(m, n, &mh, &ml);
-        u128 c = (lo + ml) < lo ? 1 : 0;
-        u128 r = hi + mh + c;
-        return r >= n ? r - n : r;
-    }
-    u128 mul(u128 a, u128 b) const {
-        if (a >= n) a %= n;
-        if (b >= n) b %= n;
-        u128 hi, lo;
-        wide(a, b, &hi, &lo);
-        u128 t = redc(hi, lo);  // a b R^-1
-        wide(t, r2, &hi, &lo);
-        return redc(hi, lo);
-    }
-    // a b R^-1 mod n, for a b < R n (e.g. any a < R with b < n): one
-    // product and one reduction, no division
-    u128 mont(u128 a, u128 b) const {
-        u128 hi, lo;
-        wide(a, b, &hi, &lo);
-        return redc(hi, lo);
-    }
-    u128 to_mont(u128 a) const { return mont(a % n, r2); }  // a R mod n
-};
-
-// a b mod q (q odd, q < 2^126: every modulus here is a prime candidate or the
-// sampled prime); the Montgomery constants of the last modulus are kept.
-u128 mulmod(u128 a, u128 b, u128 q) {
-    static thread_local ModN ctx;
-    if (ctx.n != q) ctx.set(q);
-    return ctx.mul(a, b);
-}
-u128 powmod(u128 a, u128 e, u128 q) {
-    u128 r = 1 % q;
-    a %= q;
-    while (e) {
-        if (e & 1) r = mulmod(r, a, q);
-        a = mulmod(a, a, q);
-        e >>= 1;
-    }
-    return r;
-}
-bool miller_rabin(u128 n) {
-    static const unsigned kBases[] = {2,  3,  5,  7,  11, 13, 17, 19, 23, 29, 31, 37,
-                                      41, 43, 47, 53, 59, 61, 67, 71, 73, 79, 83, 89};
-    if (n < 2) return false;
-    for (unsigned p : kBases) {
-        if (n == p) return true;
-        if (n % p == 0) return false;
-    }
-    u128 d = n - 1;
-    int r = 0;
-    while ((d & 1) == 0) {
-        d >>= 1;
-        r++;
-    }
-    for (unsigned a : kBases) {
-        u128 x = powmod(a, d, n);
-        if (x == 1 || x == n - 1) continue;
-        bool comp = true;
-        for (int i = 1; i < r; i++) {
-            x = mulmod(x, x, n);
-            if (x == n - 1) {
-                comp = false;
-                break;
-            }
-        }
-        if (comp) return false;
-    }
-    return true;
-}
-inline u128 u128_from(const uint8_t* p) {
-    uint64_t lo, hi;
-    memcpy(&lo, p, 8);
-    memcpy(&hi, p + 8, 8);
-    return ((u128)hi << 64) | lo;
-}
-inline void u128_put(u128 v, uint8_t* p) {
-    uint64_t lo = (uint64_t)v, hi = (uint64_t)(v >> 64);
-    memcpy(p, &lo, 8);
-    memcpy(p + 8, &hi, 8);
-}
 
 // ---- eq tables mod q on the device ----
 // eq_table_mod_q (src/ligerito_flock.rs:1330-1347), entry by entry: x's
@@ -271,40 +153,6 @@ int eq_table_mod_q_dev(bitz_ctx* ctx, const std::vector<u128>& r, u128 q, void* 
     return BITZ_OK;
 }
 
-// 256-bit transcript draw reduced mod q (transcript_uniform_mod, src/prime_sampling.rs:22-33)
-u128 sample_residue(Transcript& tr, u128 q) {
-    uint8_t b[16];
-    tr.challenge(b, 16);
-    u128 lo = u128_from(b);
-    tr.challenge(b, 16);
-    u128 hi = u128_from(b);
-    // (hi * 2^128 + lo) mod q, q < 2^126
-    u128 two128 = mulmod((u128)1 << 64, (u128)1 << 64, q);
-    return addmod(mulmod(hi % q, two128, q), lo % q, q);
-}
-
-// Prime of exactly `bits` bits from the transcript's sampling stream.
-int sample_prime(Transcript& tr, int bits, u128* out) {
-    u128 top = (u128)1 << (bits - 1);
-    tr.absorb_slice(std::string("bitz/shared-prime-sampling/v1"));
-    uint8_t b[16];
-    u128_put(top, b);
-    tr.absorb_slice(b, 16);
-    u128_put(top | (top - 1), b);
-    tr.absorb_slice(b, 16);
-    int attempts = 64 * bits;
-    for (int a = 0; a < attempts; a++) {
-        tr.sampling_bytes(b, 16);
-        u128 cand = (u128_from(b) & (top - 1)) | top | 1;
-        if (miller_rabin(cand)) {
-            *out = cand;
-            return BITZ_OK;
-        }
-    }
-    set_error("prime search exhausted");
-    return BITZ_ERR_DEVICE;
-}
-
 // smallest_generator (src/pcs.rs:85-102): smallest n >= 2 generating GF(2^128)^x
 G gf_pow_u128(G b, u128 e) {
     G acc = gone();
@@ -374,33 +222,6 @@ int public_weights_dev(bitz_ctx* ctx, T& it, const void* a, uint64_t na, const v
     if (!ks.empty()) memcpy(run_cvs.data(), st + plan.lead + tail, 32 * ks.size());
     it.public_message_runs(len, plan, st, run_cvs.data(), st + plan.lead);
     return BITZ_OK;
-}
-
-// The outer statement of a standalone opening and the Round-0 header, up to
-// its grinding (src/ligerito_flock.rs:1255-1284, 1520-1557)
-void standalone_statement(Transcript& outer, const uint8_t root[32], const LigConfig& cfg, int t, int s, G g) {
-    outer.absorb_slice(std::string("bitz/ligerito-flock/standalone-mod-q-statement/v1"));
-    outer.absorb_slice(root, 32);
-    std::vector<uint64_t> frame = {(uint64_t)cfg.m, (uint64_t)cfg.initial_k, (uint64_t)cfg.yr_log_n,
-                                   (uint64_t)cfg.target_bits};
-    for (auto& lv : cfg.levels)
-        for (int x : {lv.log_inv_rate, lv.log_msg_cols, lv.log_num_interleaved, lv.k_recursive, lv.queries,
-                      lv.grinding_bits, lv.fold_grinding_bits, lv.ood_samples})
-            frame.push_back((uint64_t)x);
-    outer.absorb_slice(frame.data(), frame.size() * 8);
-    uint64_t ts[2] = {(uint64_t)t, (uint64_t)s};
-    outer.absorb_slice(ts, 16);
-    uint8_t gb[16];
-    gput(g, gb);
-    outer.absorb_slice(gb, 16);
-    const int q_bits = std::min(113, 126 - t);  // standalone_q_bits (src/ligerito_flock.rs:1286-1293)
-    outer.absorb_u64((uint64_t)q_bits);
-    outer.absorb_u64((uint64_t)cfg.round0_grinding_bits + 1);
-    outer.absorb_slice(std::string("bitz/bitz-opener/standalone/v1"));
-    outer.absorb_slice(std::string("bitz/core/ood-round/v1"));
-    outer.absorb_u64((uint64_t)(t + s - 7));
-    uint32_t r0bits = (uint32_t)cfg.round0_grinding_bits;
-    outer.absorb_slice(&r0bits, 4);
 }
 
 // s_k(v_k) for k = 0..log_n and their inverses (eval_sk_at_vks,
@@ -1134,15 +955,20 @@ int Prover::run(const uint8_t* claimed_in, uint8_t* claimed_out, std::vector<uin
     phase.mark(proof_out ? "statement_round0" : "eval_statement_round0");
     // ---------------- outer transcript: statement and Round 0 ----------------
     Transcript outer;
-    standalone_statement(outer, h->root, cfg, t, s, g);
+    uint8_t gb[16];
+    gput(g, gb);
+    bind_standalone_statement(outer, h->root, cfg, t, s, gb);
     const uint32_t r0bits = (uint32_t)cfg.round0_grinding_bits;
+    ood_round_header(outer, mp, r0bits);
     uint64_t r0_nonce = 0;
     if (r0bits) {
-        uint8_t seed[16];
-        outer.challenge(seed, 16);
-        static const uint8_t kNative[16] = {'b', 'i', 't', 'z', '-', 'n', 'a', 't', 'i', 'v', 'e', '-', 'p', 'o', 'w', '1'};
-        if ((rc = pow_find(ctx, kNative, seed, r0bits, &r0_nonce))) return rc;
-        outer.absorb_u64(r0_nonce);
+        // grind_and_absorb at the Round-0 boundary (src/ligerito_flock.rs:1523-1537,
+        // src/piop/spartan/grinding.rs:198-211): BLAKE3(seed || nonce), the
+        // 32-byte seed split over the scan's prefix and seed words
+        uint8_t seed[32];
+        grinding_seed(outer, "bitz/core/ood-round-grinding/v1", 0, r0bits, seed);
+        if ((rc = pow_find(ctx, seed, seed + 16, r0bits, &r0_nonce))) return rc;
+        absorb_grinding_nonce(outer, r0_nonce);
     }
     uint8_t zb[16];
     outer.challenge(zb, 16);
@@ -1158,11 +984,23 @@ int Prover::run(const uint8_t* claimed_in, uint8_t* claimed_out, std::vector<uin
     if ((rc = mle_packed(ood_point, &ood_y))) return rc;
     uint8_t yb[16];
     gput(ood_y, yb);
-    outer.absorb_slice(yb, 16);
+    {
+        // absorb_ood_value (src/ligerito.rs:71-73): tag 0x50, then y
+        uint8_t tagged[17];
+        tagged[0] = 0x50;
+        memcpy(tagged + 1, yb, 16);
+        outer.absorb_slice(tagged, 17);
+    }
     phase.mark(proof_out ? "instance" : "eval_instance");
     // ---------------- the standalone instance ----------------
     u128 q;
-    if ((rc = sample_prime(outer, std::min(113, 126 - t), &q))) return rc;
+    {
+        std::string err;
+        if (!sample_prime_with_bits(outer, standalone_q_bits(t), &q, &err)) {
+            set_error(err.c_str());
+            return BITZ_ERR_DEVICE;
+        }
+    }
     std::vector<u128> r1(t), r2(s);
     for (auto& x : r1) x = sample_residue(outer, q);
     for (auto& x : r2) x = sample_residue(outer, q);
@@ -1206,13 +1044,7 @@ int Prover::run(const uint8_t* claimed_in, uint8_t* claimed_out, std::vector<uin
         set_error("BitZ claim: value not reduced");
         return BITZ_ERR_ARG;
     }
-    outer.absorb_slice(std::string("bitz/ligerito-flock/standalone-mod-q-claim/v1"));
-    {
-        uint8_t qc[32];
-        u128_put(q, qc);
-        u128_put(claimed, qc + 16);
-        outer.absorb_slice(qc, 32);
-    }
+    absorb_standalone_claim(outer, q, claimed);
     uint8_t tag[32];
     outer.challenge(tag, 16);
     outer.challenge(tag + 16, 16);
@@ -1248,8 +1080,6 @@ int Prover::run(const uint8_t* claimed_in, uint8_t* claimed_out, std::vector<uin
         it.verifier_gf(b);
         z = gfrom(b);
     }
-    uint8_t gb[16];
-    gput(g, gb);
     if (!rowimg.alloc(nrows * 16)) return BITZ_ERR_ALLOC;
     if ((rc = pow_images(ctx, gb, roww.ptr, nrows, rowimg.ptr))) return rc;
     // packed columns for the leaf layer and the combined rows
@@ -1937,14 +1767,19 @@ int verify_standalone_impl(bitz_ctx* ctx, const uint8_t root[32], int t, int s, 
     V_CHECK(pr.ok && pr.pos == plen, "trailing proof bytes");
     // ---- outer transcript: statement, Round 0, instance ----
     Transcript outer;
-    standalone_statement(outer, root, cfg, t, s, g);
+    uint8_t gb[16];
+    gput(g, gb);
+    bind_standalone_statement(outer, root, cfg, t, s, gb);
     const uint32_t r0bits = (uint32_t)cfg.round0_grinding_bits;
+    ood_round_header(outer, mp, r0bits);
     if (r0bits) {
-        uint8_t seed[16];
-        outer.challenge(seed, 16);
-        static const uint8_t kNative[16] = {'b', 'i', 't', 'z', '-', 'n', 'a', 't', 'i', 'v', 'e', '-', 'p', 'o', 'w', '1'};
-        V_CHECK(has_nonce && pow_valid(kNative, seed, r0_nonce, r0bits), "Round-0 proof of work");
-        outer.absorb_u64(r0_nonce);
+        // verify_and_absorb (src/piop/spartan/grinding.rs:219-236)
+        V_CHECK(has_nonce, "missing Round-0 nonce");
+        uint8_t seed[32];
+        grinding_seed(outer, "bitz/core/ood-round-grinding/v1", 0, r0bits, seed);
+        const bool ok = pow_valid(seed, seed + 16, r0_nonce, r0bits);
+        absorb_grinding_nonce(outer, r0_nonce);
+        V_CHECK(ok, "Round-0 proof of work");
     } else {
         V_CHECK(!has_nonce, "unexpected Round-0 nonce");
     }
@@ -1958,20 +1793,22 @@ int verify_standalone_impl(bitz_ctx* ctx, const uint8_t root[32], int t, int s, 
             cur = hgf_mul(cur, cur);
         }
     }
-    outer.absorb_slice(yb, 16);
+    {
+        uint8_t tagged[17];
+        tagged[0] = 0x50;
+        memcpy(tagged + 1, yb, 16);
+        outer.absorb_slice(tagged, 17);
+    }
     u128 q;
-    if ((rc = sample_prime(outer, std::min(113, 126 - t), &q))) return rc;
+    {
+        std::string err;
+        V_CHECK(sample_prime_with_bits(outer, standalone_q_bits(t), &q, &err), "prime search");
+    }
     std::vector<u128> r1(t), r2(s);
     for (auto& x : r1) x = sample_residue(outer, q);
     for (auto& x : r2) x = sample_residue(outer, q);
     V_CHECK(claimed < q, "claim not reduced");
-    outer.absorb_slice(std::string("bitz/ligerito-flock/standalone-mod-q-claim/v1"));
-    {
-        uint8_t qc[32];
-        u128_put(q, qc);
-        u128_put(claimed, qc + 16);
-        outer.absorb_slice(qc, 32);
-    }
+    absorb_standalone_claim(outer, q, claimed);
     uint8_t tagb[32];
     outer.challenge(tagb, 16);
     outer.challenge(tagb + 16, 16);
@@ -2021,8 +1858,6 @@ int verify_standalone_impl(bitz_ctx* ctx, const uint8_t root[32], int t, int s, 
     std::vector<G> zeta(s);
     for (auto& z : zeta) z = it.gf();
     if ((rc = upload_async(ctx, fb, ncols * 16, folds.ptr))) return rc;
-    uint8_t gb[16];
-    gput(g, gb);
     if ((rc = pow_images(ctx, gb, folds.ptr, ncols, img.ptr))) return rc;
     G claim;
     if ((rc = V.eq_into(zeta, tmp))) return rc;
@@ -2406,6 +2241,10 @@ int verify_standalone_impl(bitz_ctx* ctx, const uint8_t root[32], int t, int s, 
             for (size_t y = 0; y < combined.size(); y++) inner = hgf_add(inner, hgf_mul(lp.yr[y], combined[y]));
             V_CHECK(geq(inner, t_r), "Ligerito final check");
             V_CHECK(gni == lp.grind.size(), "unconsumed grinding nonces");
+            // every recursive root and opening is consumed (the flock
+            // verifier rejects recursive_roots.len() != r, ligerito.rs:5298)
+            V_CHECK(next_root == lp.roots.size() && reci == lp.rec.size(),
+                    "unconsumed recursive roots or openings");
             V_CHECK(it.pos == it.len, "trailing narg bytes");
             return BITZ_OK;
         }

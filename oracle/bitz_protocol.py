@@ -17,6 +17,7 @@ the spongefish 0.7.4 sponge (absent from the reference tree).
 from __future__ import annotations
 
 import math
+import struct
 
 import numpy as np
 
@@ -24,7 +25,6 @@ import oracle as O
 
 LOG_PACKING = 7
 POW_PREFIX_PCS = b"bitz-pcs-pow-v1\0"
-POW_PREFIX_NATIVE = b"bitz-native-pow1"
 
 
 def u64(v: int) -> bytes:
@@ -50,8 +50,11 @@ class Transcript:
     def __init__(self):
         self.buf = bytearray()
 
+    def absorb_inner(self, b: bytes) -> None:
+        self.buf += bytes(b)
+
     def absorb_slice(self, b: bytes) -> None:
-        self.buf += b"\x06" + u64(len(b)) + bytes(b) + b"\x07"
+        self.absorb_inner(b"\x06" + u64(len(b)) + bytes(b) + b"\x07")
 
     def challenge(self, n: int = 16) -> bytes:
         out = O.blake3(bytes(self.buf), n)
@@ -102,15 +105,53 @@ class VerifyError(Exception):
 
 
 # ------------------------------------------------------------ Ligerito ladder
+# The embedded slim template every custom Johnson ladder starts from
+# (vendor/flock-mod/crates/flock-core/configs/ligerito/m{22..35}_slim.toml:
+# identical header strings, eta, query grinding and target at every m); the
+# hash is forced to BLAKE3 (src/ligerito_flock/configuration.rs:146).
+TEMPLATE = dict(analysis_version="johnson_ood_row_union_over_bchks25_thm_4_6", field="f128",
+                hash="blake3", eta=0.02, grinding=16, target=100)
+
+
+def _lig_log_a(r: int, eta: float, mc: int, il: int) -> float:
+    """paper_johnson_log_a (pcs/ligerito.rs:700-740)."""
+    rho = 2.0 ** -r
+    sr = math.sqrt(rho)
+    gamma = 1.0 - sr - eta
+    m_param = float(max(math.ceil(sr / (2.0 * eta)), 3))
+    half = m_param + 0.5
+    num = 2.0 * half ** 5 + 3.0 * half * gamma * rho
+    den = 3.0 * rho ** 1.5
+    n = 2.0 ** (mc + r)
+    return math.log2((num / den) * n + half / sr) + max(il - 1.0, 0.0)
+
+
+def _lig_per_query(r: int, eta: float) -> float:
+    """paper_per_query_bits (pcs/ligerito.rs:743-747)."""
+    gamma = 1.0 - math.sqrt(2.0 ** -r) - eta
+    return math.log2(1.0 / (1.0 - gamma))
+
+
+def _lig_ood_bits(r: int, eta: float, mu: int, samples: int) -> float:
+    """paper_ood_bits (pcs/ligerito.rs:820-853)."""
+    log2_l = math.log2(1.0 / (2.0 * eta * math.sqrt(2.0 ** -r)))
+    log2_mu = math.log2(float(mu))
+    if samples == 0:
+        return 128.0 - log2_l - log2_mu
+    return float(samples) * (128.0 - log2_mu) - (2.0 * log2_l - 1.0)
+
+
 def custom_johnson_config(m: int, r0: int, k0: int, target: int = 100) -> dict:
-    """try_custom_johnson_config_bits (src/ligerito_flock.rs:535-665) with the
-    Johnson formulas of pcs/ligerito.rs:700-930 (eta 0.02, grinding 16)."""
+    """try_custom_johnson_config_bits (src/ligerito_flock.rs:535-643) on the
+    slim template, with the level diagnostics the security config carries
+    (expected_eps_pg/query/ood_bits: unrounded f64, as the Rust assigns them)
+    and Round 0 (ood_round_bits / ood_round_params, :1384-1413)."""
     if not 20 <= m <= 35 or not 1 <= r0 <= 4:
         raise ValueError("custom Johnson ladders need m in [20, 35] and rate exponent in [1, 4]")
     log_n = m - LOG_PACKING
     if not 0 < k0 < log_n:
         raise ValueError("custom initial_k out of range")
-    eta, grinding = 0.02, 16
+    eta, grinding = TEMPLATE["eta"], TEMPLATE["grinding"]
     shapes = [(log_n - k0, k0, k0, r0)]
     n_run, rate = log_n - k0, r0
     while n_run > 5:
@@ -118,99 +159,284 @@ def custom_johnson_config(m: int, r0: int, k0: int, target: int = 100) -> dict:
         rate += 1
         shapes.append((n_run - kr, kr, kr, rate))
         n_run -= kr
-
-    def per_q(r):
-        rho = 2.0 ** -r
-        return math.log2(1.0 / (1.0 - (1.0 - math.sqrt(rho) - eta)))
-
-    def log_a(r, mc, il):
-        rho = 2.0 ** -r
-        sr = math.sqrt(rho)
-        gamma = 1.0 - sr - eta
-        mp_ = max(math.ceil(sr / (2 * eta)), 3)
-        half = mp_ + 0.5
-        num = 2.0 * half ** 5 + 3.0 * half * gamma * rho
-        den = 3.0 * rho ** 1.5
-        n = 2.0 ** (mc + r)
-        return math.log2((num / den) * n + half / sr) + max(0.0, il - 1.0)
-
-    def ood_bits(r, mu, samples):
-        rho = 2.0 ** -r
-        l2 = math.log2(1.0 / (2 * eta * math.sqrt(rho)))
-        if samples == 0:
-            return 128.0 - l2 - math.log2(mu)
-        return samples * (128.0 - math.log2(mu)) - (2 * l2 - 1)
-
     levels = []
     for i, (mc, il, kr, r) in enumerate(shapes):
-        q = next(q for q in range(1, 10001) if q * per_q(r) + 1e-3 >= target - grinding)
+        need = float(target - grinding)
+        pq = _lig_per_query(r, eta)
+        q = next((q for q in range(1, 10001) if float(q) * pq + 1e-3 >= need), None)
+        if q is None:
+            raise ValueError("Johnson query search did not converge")
         if q > 1 << (mc + r):
             raise ValueError("custom Johnson level is too thin for its query count")
-        pg = 128.0 - log_a(r, mc, il)
+        pg = 128.0 - _lig_log_a(r, eta, mc, il)
+        qb = float(q) * pq
         ood = 0 if i == 0 else 1
-        if i > 0:
-            while ood_bits(r, mc + il, ood) + 1e-3 < target:
-                ood += 1
+        while True:
+            ob = _lig_ood_bits(r, eta, mc + il, ood)
+            if i == 0 or ob + 1e-3 >= float(target):
+                break
+            ood += 1
         levels.append(dict(rate=r, msg_cols=mc, lanes=il, k=kr, queries=q, grinding=grinding,
-                           fold_grinding=max(0, math.ceil(target - pg)), ood=ood))
+                           fold_grinding=int(max(math.ceil(float(target) - pg), 0.0)), ood=ood,
+                           eps_pg=pg, eps_query=qb, eps_ood=ob))
     rho = 2.0 ** -r0
-    lst = 1.0 / (2 * eta * math.sqrt(rho))
-    pairs = max(lst * (lst - 1) / 2, 1.0)
-    bits0 = 128.0 - math.log2(pairs) - math.log2(max(2.0 ** log_n - 1, 1.0))
-    r0grind = 0 if target - bits0 <= 0 else math.ceil(target - bits0)
+    lst = 1.0 / (2.0 * eta * math.sqrt(rho))
+    pairs = max(lst * (lst - 1.0) / 2.0, 1.0)
+    bits0 = 128.0 - math.log2(pairs) - math.log2(max(2.0 ** log_n - 1.0, 1.0))
+    deficit = float(target) - bits0
+    r0grind = 0 if deficit <= 0.0 else int(math.ceil(deficit))
     return dict(m=m, log_n=log_n, initial_k=k0, yr_log_n=n_run, target=target, levels=levels,
                 round0_grinding=r0grind)
 
 
-def config_frame(cfg: dict) -> bytes:
-    words = [cfg["m"], cfg["initial_k"], cfg["yr_log_n"], cfg["target"]]
+def _bc_str(s: str) -> bytes:
+    return u64(len(s)) + s.encode()
+
+
+def _bc_f64(x: float) -> bytes:
+    return struct.pack("<d", x)
+
+
+def security_config_bincode(cfg: dict) -> bytes:
+    """bincode 1.3 (fixint, little endian) of LigeritoSecurityConfig
+    (pcs/ligerito.rs:541-651): usize as u64, strings and vectors with a u64
+    length, unit enum variants as u32 indices, Option as a 0/1 byte."""
+    out = u64(cfg["m"]) + u64(cfg["log_n"]) + u64(cfg["initial_k"]) + u64(cfg["target"])
+    out += _bc_str(TEMPLATE["analysis_version"]) + _bc_str(TEMPLATE["field"]) + _bc_str(TEMPLATE["hash"])
+    out += u32(0)  # GrindingStep::PostCommitPreQueries
+    out += u64(len(cfg["levels"]))
     for lv in cfg["levels"]:
-        words += [lv["rate"], lv["msg_cols"], lv["lanes"], lv["k"], lv["queries"], lv["grinding"],
-                  lv["fold_grinding"], lv["ood"]]
-    return b"".join(u64(w) for w in words)
+        out += u64(lv["rate"]) + u64(lv["msg_cols"]) + u64(lv["lanes"]) + u64(lv["k"])
+        out += u32(1)  # SoundnessRegime::JohnsonOod
+        out += b"\x01" + _bc_f64(TEMPLATE["eta"])  # eta: Some
+        out += b"\x00"  # proximity_loss: None
+        out += u64(lv["queries"]) + u64(lv["grinding"]) + u64(lv["fold_grinding"]) + u64(lv["ood"])
+        out += u64(cfg["target"])
+        out += _bc_f64(lv["eps_pg"]) + _bc_f64(lv["eps_query"])
+        out += b"\x01" + _bc_f64(lv["eps_ood"])
+    out += u64(cfg["yr_log_n"])
+    return out
+
+
+def ladder_digest(cfg: dict) -> bytes:
+    """ResolvedLigerito::digest: BLAKE3 of the security config's bincode
+    (src/ligerito_flock/configuration.rs:157-159)."""
+    return O.blake3(security_config_bincode(cfg))
+
+
+# --------------------------------------------------------- statement frames
+FRAME_DOMAIN = b"bitz/ligerito-flock/statement-frame/v1"
+F_BYTES, F_U8, F_U64, F_U128, F_GF128 = 1, 2, 3, 4, 5
+
+
+class StatementFrame:
+    """StatementFrame (src/ligerito_flock.rs:1057-1160): two framed domain
+    slices, then one 0x06 || tag || kind || count u64 || payload || 0x07 run
+    per field, absorbed unframed."""
+
+    def __init__(self, tr: Transcript, domain: bytes):
+        self.tr = tr
+        tr.absorb_slice(FRAME_DOMAIN)
+        tr.absorb_slice(domain)
+
+    def field(self, tag: int, kind: int, count: int, payload: bytes) -> None:
+        self.tr.absorb_inner(b"\x06" + bytes([tag, kind]) + u64(count) + payload + b"\x07")
+
+    def usize(self, tag: int, v: int) -> None:
+        self.field(tag, F_U64, 1, u64(v))
+
+    def usizes(self, tag: int, vs) -> None:
+        self.field(tag, F_U64, len(vs), b"".join(u64(v) for v in vs))
+
+    def byte(self, tag: int, v: int) -> None:
+        self.field(tag, F_U8, 1, bytes([v]))
+
+
+def bind_standalone_statement(tr: Transcript, root: bytes, t: int, s: int, cfg: dict) -> None:
+    """bind_standalone_statement (bitz_opener.rs:589-609) ->
+    absorb_standalone_mod_q_statement (src/ligerito_flock.rs:1245-1276): the
+    commitment (root, m, rate, batch, profile Fast = 0, BLAKE3 = 1), the
+    verifier configuration, the shape, the generator, the prime width and the
+    Round-0 word, then the session label and the ladder digest."""
+    lv = cfg["levels"]
+    f = StatementFrame(tr, b"bitz/ligerito-flock/standalone-mod-q-statement/v1")
+    f.field(0x01, F_BYTES, len(root), root)
+    f.usize(0x02, cfg["m"])
+    f.usize(0x03, lv[0]["rate"])
+    f.usize(0x04, cfg["initial_k"])
+    f.byte(0x05, 0)
+    f.byte(0x06, 1)
+    f.usizes(0x08, [x["rate"] for x in lv])
+    f.usize(0x09, len(lv) - 1)
+    f.usize(0x0a, lv[0]["msg_cols"])
+    f.usize(0x0b, cfg["initial_k"])
+    f.usize(0x0c, cfg["initial_k"])
+    f.usizes(0x0d, [x["msg_cols"] for x in lv[1:]])
+    f.usizes(0x0e, [x["k"] for x in lv[1:]])
+    f.usizes(0x0f, [x["queries"] for x in lv])
+    f.usizes(0x10, [x["grinding"] for x in lv])
+    f.usizes(0x11, [x["fold_grinding"] for x in lv])
+    f.usizes(0x12, [x["ood"] for x in lv])
+    f.byte(0x13, 1)
+    f.usize(0x20, t)
+    f.usize(0x21, s)
+    f.usize(0x22, 1)
+    f.field(0x30, F_GF128, 1, g16(generator()))
+    f.usize(0x31, standalone_q_bits(t))
+    f.usize(0x32, 1 + cfg["round0_grinding"])
+    tr.absorb_slice(b"bitz/bitz-opener/standalone/v1")
+    tr.absorb_slice(ladder_digest(cfg))
+
+
+def standalone_q_bits(t: int) -> int:
+    """standalone_q_bits (src/ligerito_flock.rs:1286-1293)."""
+    return min(113, 126 - t)
+
+
+def absorb_standalone_claim(tr: Transcript, q: int, claimed: int) -> None:
+    """absorb_standalone_mod_q_claim (src/ligerito_flock.rs:1278-1284)."""
+    f = StatementFrame(tr, b"bitz/ligerito-flock/standalone-mod-q-claim/v1")
+    f.field(0x30, F_U128, 2, g16(q) + g16(claimed))
+
+
+# ------------------------------------------------------ Round-0 grinding
+GRIND_DOMAIN = b"bitz/spartan/fiat-shamir-grinding/v1"
+GRIND_NONCE_DOMAIN = b"bitz/spartan/fiat-shamir-grinding/nonce/v1"
+OOD_GRIND_DOMAIN = b"bitz/core/ood-round-grinding/v1"
+
+
+def grinding_seed(tr: Transcript, domain: bytes, index: int, bits: int) -> bytes:
+    """derive_grinding_seed_in_domain (src/piop/spartan/grinding.rs:145-160):
+    the frame, the domain, the round index u64 and the difficulty u32, then
+    a 32-byte challenge."""
+    if not 1 <= bits <= 256:
+        raise ValueError("grinding difficulty out of range")
+    tr.absorb_slice(GRIND_DOMAIN)
+    tr.absorb_slice(domain)
+    tr.absorb_slice(u64(index))
+    tr.absorb_slice(u32(bits))
+    return tr.challenge(32)
+
+
+def absorb_grinding_nonce(tr: Transcript, nonce: int) -> None:
+    """absorb_grinding_nonce (grinding.rs:272-275)."""
+    tr.absorb_slice(GRIND_NONCE_DOMAIN)
+    tr.absorb_slice(u64(nonce))
+
+
+def ood_round_header(tr: Transcript, packed_vars: int, bits: int) -> None:
+    """absorb_ood_round_header (src/ligerito_flock.rs:1488-1496)."""
+    tr.absorb_slice(b"bitz/core/ood-round/v1")
+    tr.absorb_slice(u64(packed_vars))
+    tr.absorb_slice(u32(bits))
 
 
 # --------------------------------------------------------- integers mod q
-def miller_rabin(n: int) -> bool:
-    bases = [2, 3, 5, 7, 11, 13, 17, 19, 23, 29, 31, 37, 41, 43, 47, 53, 59, 61, 67, 71, 73, 79,
-             83, 89]
-    if n < 2:
-        return False
-    for p in bases:
-        if n == p:
+SMALL_PRIMES = (2, 3, 5, 7, 11, 13, 17, 19, 23, 29, 31, 37, 41, 43, 47, 53)
+
+
+def prime_budgets(security: int, max_candidates: int) -> tuple[int, int]:
+    """PrimeSearchPolicy::budgets (vendor/field/src/prime/sampling.rs:33-56):
+    Miller-Rabin rounds and the rejection-draw budget."""
+    if max_candidates == 0:
+        raise ValueError("invalid prime search policy")
+    sec = max(security + 16, 128)
+    cand_bits = (max_candidates - 1).bit_length()
+    rounds = (sec + cand_bits + 1) // 2
+    draws = sec + cand_bits + rounds.bit_length()
+    return rounds, draws
+
+
+def sample_below(tr: Transcript, bound: int, max_draws: int) -> int:
+    """sample_below (sampling.rs:110-138): masked rejection over whole
+    little-endian u64 words, each read from the sampling stream."""
+    if bound == 0:
+        raise ValueError("empty range")
+    bits = (bound - 1).bit_length()
+    if bits == 0:
+        return 0
+    words = (bits + 63) // 64
+    for _ in range(max_draws):
+        cand = 0
+        for w in range(words):
+            cand |= gint(tr.sampling_bytes(8)) << (64 * w)
+        cand &= (1 << bits) - 1
+        if cand < bound:
+            return cand
+    raise RuntimeError("sampling exhausted")
+
+
+def _strong_round(n: int, base: int, odd: int, twos: int) -> bool:
+    x = pow(base, odd, n)
+    if x == 1 or x == n - 1:
+        return True
+    for _ in range(1, twos):
+        x = x * x % n
+        if x == n - 1:
             return True
-        if n % p == 0:
-            return False
-    d, r = n - 1, 0
-    while d % 2 == 0:
-        d //= 2
-        r += 1
-    for a in bases:
-        x = pow(a, d, n)
-        if x in (1, n - 1):
-            continue
-        for _ in range(r - 1):
-            x = x * x % n
-            if x == n - 1:
-                break
-        else:
-            return False
-    return True
+    return False
 
 
-def sample_prime(tr: Transcript, bits: int) -> int:
-    top = 1 << (bits - 1)
+def sample_prime_context(tr: Transcript, lo: int, hi: int, security: int = 128) -> int:
+    """sample_prime_context (src/prime_sampling.rs:66-111) ->
+    sample_prime_public (vendor/field/src/prime/sampling.rs:290-345): the
+    interval and policy absorbed, then odd candidates 2(first + i) + 1 with
+    i drawn below the odd count, trial division by the primes to 53, and
+    `rounds` Miller-Rabin rounds with bases drawn from the stream."""
+    first_odd = max(lo, 3) | 1
+    if first_odd > hi:
+        raise ValueError("no odd candidate")
+    attempts = 1 if first_odd == hi - (1 - (hi & 1)) else 64 * hi.bit_length()
+    rounds, draws = prime_budgets(security, attempts)
     tr.absorb_slice(b"bitz/shared-prime-sampling/v1")
-    tr.absorb_slice(g16(top))
-    tr.absorb_slice(g16(top | (top - 1)))
-    for _ in range(64 * bits):
-        cand = (gint(tr.sampling_bytes(16)) & (top - 1)) | top | 1
-        if miller_rabin(cand):
+    tr.absorb_slice(g16(lo))
+    tr.absorb_slice(g16(hi))
+    tr.absorb_slice(u32(security))
+    tr.absorb_slice(u64(attempts))
+    tr.absorb_slice(u64(draws))
+    low = max(lo, 3)
+    first = low >> 1
+    last = hi >> 1 if hi & 1 else (hi >> 1) - 1
+    count = last - first + 1
+    for _ in range(attempts):
+        cand = ((first + sample_below(tr, count, draws)) << 1) | 1
+        small = None
+        for p in SMALL_PRIMES:
+            if cand == p:
+                small = True
+                break
+            if cand % p == 0:
+                small = False
+                break
+        if small is True:
+            return cand
+        if small is False:
+            continue
+        odd, twos = cand - 1, 0
+        while odd % 2 == 0:
+            odd //= 2
+            twos += 1
+        ok = True
+        for _ in range(rounds):
+            base = sample_below(tr, cand - 3, draws) + 2
+            if not _strong_round(cand, base, odd, twos):
+                ok = False
+                break
+        if ok:
             return cand
     raise RuntimeError("prime search exhausted")
 
 
+def sample_prime(tr: Transcript, bits: int) -> int:
+    """sample_prime_with_bits (src/prime_sampling.rs:125-137)."""
+    if not 32 <= bits <= 120:
+        raise ValueError("prime bit width must be in [32, 120]")
+    top = 1 << (bits - 1)
+    return sample_prime_context(tr, top, top | (top - 1), 128)
+
+
 def sample_residue(tr: Transcript, q: int) -> int:
+    """transcript_uniform_mod (src/prime_sampling.rs:11-33)."""
     lo = gint(tr.challenge(16))
     hi = gint(tr.challenge(16))
     return ((hi << 128) | lo) % q
@@ -318,19 +544,11 @@ def proof_tail_parse(proof: bytes):
 
 # ---------------------------------------------------------------- statement
 def outer_statement(root: bytes, t: int, s: int, cfg: dict) -> Transcript:
+    """The outer transcript up to Round 0: the bound statement, then Round
+    0's header (prove_standalone, bitz_opener.rs:672-685)."""
     tr = Transcript()
-    tr.absorb_slice(b"bitz/ligerito-flock/standalone-mod-q-statement/v1")
-    tr.absorb_slice(root)
-    tr.absorb_slice(config_frame(cfg))
-    tr.absorb_slice(u64(t) + u64(s))
-    tr.absorb_slice(g16(generator()))
-    q_bits = min(113, 126 - t)
-    tr.absorb_slice(u64(q_bits))
-    tr.absorb_slice(u64(cfg["round0_grinding"] + 1))
-    tr.absorb_slice(b"bitz/bitz-opener/standalone/v1")
-    tr.absorb_slice(b"bitz/core/ood-round/v1")
-    tr.absorb_slice(u64(t + s - LOG_PACKING))
-    tr.absorb_slice(u32(cfg["round0_grinding"]))
+    bind_standalone_statement(tr, root, t, s, cfg)
+    ood_round_header(tr, t + s - LOG_PACKING, cfg["round0_grinding"])
     return tr
 
 
@@ -343,7 +561,8 @@ def ood_point(zeta: int, n: int) -> list[int]:
 
 
 def instance(tr: Transcript, t: int, s: int):
-    q = sample_prime(tr, min(113, 126 - t))
+    """sample_standalone_instance (src/ligerito_flock.rs:1305-1327)."""
+    q = sample_prime(tr, standalone_q_bits(t))
     r1 = [sample_residue(tr, q) for _ in range(t)]
     r2 = [sample_residue(tr, q) for _ in range(s)]
     return q, eq_table_mod_q(r1, q), eq_table_mod_q(r2, q)
@@ -377,13 +596,15 @@ class OracleProver:
         bits = self.cfg["round0_grinding"]
         nonce = None
         if bits:
-            seed = tr.challenge(16)
-            nonce = O.pow_find(POW_PREFIX_NATIVE, seed, bits)
-            tr.absorb_slice(u64(nonce))
+            # grind_and_absorb (src/piop/spartan/grinding.rs:198-211), the
+            # Round-0 boundary (src/ligerito_flock.rs:1523-1537)
+            seed = grinding_seed(tr, OOD_GRIND_DOMAIN, 0, bits)
+            nonce = O.pow_find(seed[:16], seed[16:], bits)
+            absorb_grinding_nonce(tr, nonce)
         zeta = gint(tr.challenge(16))
         pt = ood_point(zeta, self.mp)
         y = O.mle_eval(self.packed, gfa(pt))
-        tr.absorb_slice(g16(y))
+        tr.absorb_slice(b"\x50" + g16(y))  # absorb_ood_value (src/ligerito.rs:71-73)
         return tr, pt, y, nonce
 
     def evaluation(self) -> int:
@@ -440,8 +661,7 @@ class OracleProver:
         q, row_w, col_w = instance(tr, t, s)
         folds = self._folds(row_w)
         assert claimed < q
-        tr.absorb_slice(b"bitz/ligerito-flock/standalone-mod-q-claim/v1")
-        tr.absorb_slice(g16(q) + g16(claimed))
+        absorb_standalone_claim(tr, q, claimed)
         tag = tr.challenge(16) + tr.challenge(16)
         it = Inner(b"bitz/bitz-opener/standalone/v1", tag)
         g = generator()
@@ -657,6 +877,22 @@ def serialize_ligerito(root, initial, roots, rec, yr, final, msgs, grind_nonces,
     return out
 
 
+def with_extra_recursive_level(proof: bytes) -> bytes:
+    """The proof with one more recursive root and opening appended to its
+    Ligerito hint (copies of the last ones, counts and lengths edited to
+    match): a verifier must reject it, every root and opening being bound
+    to a level."""
+    narg, hints, ood_y, nonce = proof_tail_parse(proof)
+    p = parse_ligerito(hints[4:])
+    lig = serialize_ligerito(p["initial_root"], p["initial"], p["roots"] + [p["roots"][-1]],
+                             p["rec"] + [p["rec"][-1]], p["yr"], p["final"], p["msgs"], p["grind"],
+                             p["ood"], p["fold_nonces"])
+    hints = u32(len(lig)) + lig
+    out = u64(len(narg)) + narg + u64(len(hints)) + hints + b"\x01" + g16(ood_y)
+    out += (b"\x01" + u64(nonce)) if nonce is not None else b"\x00"
+    return out
+
+
 class _Reader:
     def __init__(self, b: bytes):
         self.b, self.pos = b, 0
@@ -711,20 +947,24 @@ def verify_standalone(root: bytes, t: int, s: int, log_inv_rate: int, initial_k:
     tr = outer_statement(root, t, s, cfg)
     bits = cfg["round0_grinding"]
     if bits:
-        seed = tr.challenge(16)
-        if r0_nonce is None or not O.pow_valid(POW_PREFIX_NATIVE, seed, r0_nonce, bits):
+        # verify_and_absorb (grinding.rs:219-236): the nonce is absorbed
+        # before the verdict
+        if r0_nonce is None:
+            raise VerifyError("missing Round-0 nonce")
+        seed = grinding_seed(tr, OOD_GRIND_DOMAIN, 0, bits)
+        ok = O.pow_valid(seed[:16], seed[16:], r0_nonce, bits)
+        absorb_grinding_nonce(tr, r0_nonce)
+        if not ok:
             raise VerifyError("Round-0 proof of work")
-        tr.absorb_slice(u64(r0_nonce))
     elif r0_nonce is not None:
         raise VerifyError("unexpected Round-0 nonce")
     zeta0 = gint(tr.challenge(16))
     opt = ood_point(zeta0, mp)
-    tr.absorb_slice(g16(ood_y))
+    tr.absorb_slice(b"\x50" + g16(ood_y))
     q, row_w, col_w = instance(tr, t, s)
     if claimed >= q:
         raise VerifyError("claim not reduced")
-    tr.absorb_slice(b"bitz/ligerito-flock/standalone-mod-q-claim/v1")
-    tr.absorb_slice(g16(q) + g16(claimed))
+    absorb_standalone_claim(tr, q, claimed)
     tag = tr.challenge(16) + tr.challenge(16)
     it = Inner(b"bitz/bitz-opener/standalone/v1", tag, narg)
     g = generator()
@@ -1014,6 +1254,10 @@ def _verify_ligerito(it: Inner, cfg: dict, p: dict, root: bytes, log_n: int, tar
                 raise VerifyError("Ligerito final check")
             if next(grind_nonces, None) is not None:
                 raise VerifyError("unconsumed grinding nonces")
+            # every recursive root and opening is consumed (the flock
+            # verifier rejects recursive_roots.len() != r, ligerito.rs:5298)
+            if next_root != len(p["roots"]) or next(rec, None) is not None:
+                raise VerifyError("unconsumed recursive roots or openings")
             return
         if next_root >= len(p["roots"]):
             raise VerifyError("missing recursive root")

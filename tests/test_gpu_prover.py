@@ -1,7 +1,7 @@
 """End-to-end parity of the GPU standalone opening: the commitment root, the
 honest claim and the proof bytes equal the CPU restatement's at m = 20, and
-the CPU verifier accepts the GPU proof at m = 20 and at the 2^20-coefficient
-workload (n = 26: t = 15, s = 11, rate 1/4, initial_k 4), rejecting tampered
+the CPU verifier accepts the GPU proof at m = 20 and 24 and at the
+2^20-coefficient workload (n = 26: t = 15, s = 11, rate 1/4, initial_k 4), rejecting tampered
 bytes."""
 import numpy as np
 import pytest
@@ -19,9 +19,11 @@ def _rows(t, s, seed):
 
 @pytest.mark.parametrize("device_fs", [1, 0])
 @pytest.mark.parametrize("t,s,rate,k0,seed",
-                         [(11, 9, 2, 4, 5), (12, 8, 1, 4, 6), (10, 10, 3, 3, 7), (14, 6, 2, 4, 8)])
+                         [(11, 9, 2, 4, 5), (12, 8, 1, 4, 6), (10, 10, 3, 3, 7), (14, 6, 2, 4, 8),
+                          (14, 10, 3, 4, 9)])
 def test_gpu_proof_bytes_equal_oracle(t, s, rate, k0, seed, device_fs):
-    """with the GKR transcript on the device (default) and on the host; the
+    """with the GKR transcript on the device (default) and on the host; m = 24
+    at rate 1/8 runs Round 0 with one bit of grinding (the others none); the
     column counts cover s > 8 (the right half carries the weights of column
     bits 8..s-1, the round sums those of bits 0..7), s = 8 and s < 8 (the
     round sums carry every column weight)"""

@@ -109,3 +109,13 @@ def test_random_byte_flips_rejected(proved):
         assert not _gpu_accepts(com.root, t, s, rate, k0, claimed, bytes(bad)), f"flip at {pos} accepted"
         if i < 4:
             assert not _cpu_accepts(com.root, t, s, rate, k0, claimed, bytes(bad))
+
+
+def test_extra_recursive_level_rejected(proved):
+    """a proof with one more recursive root and opening appended to its
+    Ligerito hint is rejected by both verifiers"""
+    (t, s, rate, k0), com, claimed, proof = proved
+    log(f"verify extra recursive level t={t} s={s}")
+    bad = P.with_extra_recursive_level(proof)
+    assert not _cpu_accepts(com.root, t, s, rate, k0, claimed, bad)
+    assert not _gpu_accepts(com.root, t, s, rate, k0, claimed, bad)

@@ -60,3 +60,13 @@ def test_dishonest_claim_is_not_accepted(proved):
     bad_proof = pr.prove(wrong)  # the folds still reconstruct the honest value
     with pytest.raises(P.VerifyError):
         P.verify_standalone(pr.root, pr.t, pr.s, 2, 4, wrong, bad_proof)
+
+
+def test_extra_recursive_level_rejected(proved):
+    """one more recursive root and opening than the ladder's levels
+    (the flock verifier's recursive_roots.len() != r check)"""
+    pr, claimed, proof = proved
+    bad = P.with_extra_recursive_level(proof)
+    assert bad != proof
+    with pytest.raises(P.VerifyError, match="unconsumed recursive"):
+        P.verify_standalone(pr.root, pr.t, pr.s, 2, 4, claimed, bad)

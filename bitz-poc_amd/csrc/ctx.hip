@@ -2,6 +2,7 @@
 #include "common.h"
 #include "gf128_dev.h"
 #include "blake3_dev.h"
+#include "open_impl.h"
 
 namespace bitz {
 static thread_local std::string g_err;
@@ -116,6 +117,8 @@ extern "C" int bitz_ctx_create(int device, bitz_ctx** out) {
     BITZ_HIP(hipGetDeviceCount(&count));
     BITZ_REQUIRE(device >= 0 && device < count, BITZ_ERR_ARG, "bitz_ctx_create: no such device");
     BITZ_HIP(hipSetDevice(device));
+    // the no-allocation entry point's device array, resolved once per device
+    if (int rc = bitz::noalloc_setup(device)) return rc;
     auto* ctx = new bitz_ctx();
     ctx->device = device;
     if (hipStreamCreateWithFlags(&ctx->own_stream, hipStreamNonBlocking) != hipSuccess) {

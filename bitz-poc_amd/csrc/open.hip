@@ -2086,18 +2086,26 @@ int fold_lsb(bitz_ctx* ctx, const void* v, uint64_t n, const uint8_t r[16], void
     return BITZ_OK;
 }
 
-int fold_1b_rows(bitz_ctx* ctx, const void* p, const void* suffix, uint64_t n, void* out_dev) {
-    const unsigned nb = blocks_for(n, 8, 512);  // element chunks
+unsigned fold_1b_rows_chunks(uint64_t n) { return blocks_for(n, 8, kFold1bMaxChunks); }
+
+int fold_1b_rows_into(bitz_ctx* ctx, const void* p, const void* suffix, uint64_t n, void* out_dev, void* partials) {
+    const unsigned nb = fold_1b_rows_chunks(n);  // element chunks
     const uint64_t per = (n + nb - 1) / nb;
+    // every block writes its chunk's 8 partial sums (zero for an empty
+    // chunk), so the partials need no clearing
+    hipLaunchKernelGGL(k_fold_1b_rows, dim3(nb * 16), dim3(kT), 0, ctx->stream, (const uint4*)p, (const uint4*)suffix,
+                       n, per, (uint4*)partials);
+    BITZ_LAUNCH_CHECK();
+    // 128 independent sums over nb blocks: one column reduction per bit
+    return reduce_columns(ctx, (const uint4*)partials, nb, 128, (uint4*)out_dev);
+}
+
+int fold_1b_rows(bitz_ctx* ctx, const void* p, const void* suffix, uint64_t n, void* out_dev) {
+    const unsigned nb = fold_1b_rows_chunks(n);
     int rc = ctx->scratch.ensure((size_t)nb * 128 * 16);
     if (rc) return rc;
     TimedLaunch tl(ctx, "fold_1b_rows", 32.0 * (double)n, 0.0);
-    hipLaunchKernelGGL(k_fold_1b_rows, dim3(nb * 16), dim3(kT), 0, ctx->stream, (const uint4*)p, (const uint4*)suffix,
-                       n, per, (uint4*)ctx->scratch.partials.ptr);
-    BITZ_LAUNCH_CHECK();
-    // 128 independent sums over nb blocks: reuse the K=1 reducer per column
-    // by a strided layout -> simple dedicated pass
-    return reduce_columns(ctx, (const uint4*)ctx->scratch.partials.ptr, nb, 128, (uint4*)out_dev);
+    return fold_1b_rows_into(ctx, p, suffix, n, out_dev, ctx->scratch.partials.ptr);
 }
 
 int reduce_columns(bitz_ctx* ctx, const uint4* partials, uint32_t nblocks, uint32_t width, uint4* out) {

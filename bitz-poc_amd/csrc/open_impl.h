@@ -93,6 +93,14 @@ int fold_rows_eq(bitz_ctx* ctx, const uint64_t* rows, int t, int s, const void* 
 int round_poly(bitz_ctx* ctx, const void* m, const void* w, uint64_t n, void* out_dev);
 int fold_lsb(bitz_ctx* ctx, const void* v, uint64_t n, const uint8_t r[16], void* out);
 int fold_1b_rows(bitz_ctx* ctx, const void* p, const void* suffix, uint64_t n, void* out_dev);
+// The ring-switch bit fold with its block partials in a caller-given device
+// buffer of fold_1b_rows_chunks(n) * 128 * 16 bytes (at most
+// kFold1bMaxChunks chunks); allocates nothing.
+constexpr unsigned kFold1bMaxChunks = 512;
+unsigned fold_1b_rows_chunks(uint64_t n);
+int fold_1b_rows_into(bitz_ctx* ctx, const void* p, const void* suffix, uint64_t n, void* out_dev, void* partials);
+// Resolves the no-allocation entry point's device array on `device` (current).
+int noalloc_setup(int device);
 int reduce_columns(bitz_ctx* ctx, const uint4* partials, uint32_t nblocks, uint32_t width, uint4* out);
 int fold_b128(bitz_ctx* ctx, const void* suffix, uint64_t n, const void* w, void* out);
 int axpy(bitz_ctx* ctx, void* acc, const void* e, uint64_t n, const uint8_t eta[16]);

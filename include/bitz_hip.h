@@ -223,6 +223,12 @@ int bitz_round_poly_dev(bitz_ctx* ctx, const void* m, const void* w, uint64_t n,
 int bitz_fold_lsb_dev(bitz_ctx* ctx, const void* v, uint64_t n, const uint8_t r[16], void* out);
 /* fold_1b_rows_naive (pcs/ring_switch.rs:1332-1381): out[i] = sum_x bit_i(p[x]) suffix[x], 128 outputs */
 int bitz_fold_1b_rows_dev(bitz_ctx* ctx, const void* p, const void* suffix, uint64_t n, void* out);
+/* No counterpart in the reference. bitz_fold_1b_rows_dev for callers that
+   cannot allocate device memory (graph capture, serving loops, harnesses that
+   forbid allocation): its block partials live in a fixed 1 MiB device array
+   per GPU (512 chunks x 128 sums), so the call makes no allocation. Calls
+   that overlap on one GPU share that array and race: serialise them. */
+int bitz_fold_1b_rows_noalloc(bitz_ctx* ctx, const void* p, const void* suffix, uint64_t n, void* out);
 /* fold_b128_elems (ring_switch.rs:1486-1560): out[x] = sum over set bits i of suffix[x] of w[i] */
 int bitz_fold_b128_dev(bitz_ctx* ctx, const void* suffix, uint64_t n, const void* w, void* out);
 /* acc[x] += eta * e[x] (SumcheckProver::glue, ligerito.rs:3125; add_ood_basis, src/ligerito_flock.rs:1590) */

@@ -83,6 +83,9 @@ SIGNATURES = {
     "bitz_round_poly_dev": (_I, [_P, _P, _P, _U64, _P]),
     "bitz_fold_lsb_dev": (_I, [_P, _P, _U64, _P, _P]),
     "bitz_fold_1b_rows_dev": (_I, [_P, _P, _P, _U64, _P]),
+    "bitz_gkr_nibble_rows_dev": (_I, [_P, _P, _I, _I, _P]),
+    "bitz_gkr_nibble_round_sums_dev": (_I, [_P, _I, _P, _P, _I, _I, _P, _P, _P, _I, _P]),
+    "bitz_gkr_nibble_fold_dev": (_I, [_P, _I, _P, _P, _I, _I, _P, _P, _P, _P]),
     "bitz_fold_1b_rows_noalloc": (_I, [_P, _P, _P, _U64, _P]),
     "bitz_fold_b128_dev": (_I, [_P, _P, _U64, _P, _P]),
     "bitz_axpy_dev": (_I, [_P, _P, _P, _U64, _P]),

@@ -184,6 +184,46 @@ extern "C" int bitz_gkr_level1_fold_dev(bitz_ctx* ctx, const void* packed, const
     return bitz::gkr_level1_fold(ctx, (const uint64_t*)packed, A, t, s, rho, eq_c, l_out, r_out);
 }
 
+extern "C" int bitz_gkr_nibble_rows_dev(bitz_ctx* ctx, const void* packed, int t, int s, void* nr) {
+    PRELUDE();
+    BITZ_REQUIRE(packed && nr, BITZ_ERR_ARG, "bitz_gkr_nibble_rows_dev: null argument");
+    BITZ_REQUIRE(t >= 4 && s >= 0 && t + s <= 40, BITZ_ERR_SHAPE, "bitz_gkr_nibble_rows_dev: bad shape");
+    return bitz::gkr_nibble_rows(ctx, (const uint64_t*)packed, t, s, nr);
+}
+
+static void chal_from_bytes(const uint8_t* b, int n, bitz::hgf* out) {
+    for (int i = 0; i < n; i++) {
+        memcpy(&out[i].lo, b + 16 * i, 8);
+        memcpy(&out[i].hi, b + 16 * i + 8, 8);
+    }
+}
+
+extern "C" int bitz_gkr_nibble_round_sums_dev(bitz_ctx* ctx, int level, const void* nr, const void* A, int t, int s,
+                                              const uint8_t* chal, const void* eq_c, const void* eq_y, int send_one,
+                                              uint8_t out_host[32]) {
+    PRELUDE();
+    BITZ_REQUIRE(level >= 0 && level <= 2, BITZ_ERR_ARG, "bitz_gkr_nibble_round_sums_dev: level must be 0, 1 or 2");
+    BITZ_REQUIRE(nr && A && eq_c && eq_y && out_host && (level == 2 || chal), BITZ_ERR_ARG,
+                 "bitz_gkr_nibble_round_sums_dev: null argument");
+    BITZ_REQUIRE(t >= 4 && s >= 0 && t + s <= 40, BITZ_ERR_SHAPE, "bitz_gkr_nibble_round_sums_dev: bad shape");
+    bitz::hgf ch[3] = {};
+    chal_from_bytes(chal, 2 - level, ch);
+    CALL(bitz::gkr_nibble_round_sums(ctx, level, nr, A, t, s, ch, nullptr, eq_c, eq_y, send_one, nullptr));
+    return bitz::read_result(ctx, 2, out_host);
+}
+
+extern "C" int bitz_gkr_nibble_fold_dev(bitz_ctx* ctx, int level, const void* nr, const void* A, int t, int s,
+                                        const uint8_t* chal, const void* eq_hi, void* l_out, void* r_out) {
+    PRELUDE();
+    BITZ_REQUIRE(level >= 0 && level <= 2, BITZ_ERR_ARG, "bitz_gkr_nibble_fold_dev: level must be 0, 1 or 2");
+    BITZ_REQUIRE(nr && A && chal && l_out && r_out, BITZ_ERR_ARG, "bitz_gkr_nibble_fold_dev: null argument");
+    BITZ_REQUIRE(t >= 4 && s >= 0 && t + s <= 40 && (!eq_hi || s > 8), BITZ_ERR_SHAPE,
+                 "bitz_gkr_nibble_fold_dev: bad shape (eq_hi needs s > 8)");
+    bitz::hgf ch[3] = {};
+    chal_from_bytes(chal, 3 - level, ch);
+    return bitz::gkr_nibble_fold(ctx, level, nr, A, t, s, ch, nullptr, eq_hi, l_out, r_out);
+}
+
 extern "C" int bitz_gkr_level1_dev(bitz_ctx* ctx, const void* packed, const void* A, int t, int s, void* out) {
     PRELUDE();
     BITZ_REQUIRE(packed && A && out, BITZ_ERR_ARG, "bitz_gkr_level1_dev: null argument");

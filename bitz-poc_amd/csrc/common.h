@@ -135,6 +135,8 @@ namespace bitz {
 // tables) plus a pinned host staging buffer for round results.
 struct Scratch {
     DevBuf partials, result, eq_lo, fold_partial, comb, what, misc, leaf_tab;
+    // the row-word rounds' per-row value tables and product / fold tables
+    DevBuf nib_t, nib_tab;
     // the proof-of-work search's best nonce: all ones between searches (each
     // search's final report resets it)
     DevBuf pow_best;

@@ -1469,7 +1469,7 @@ static void split16(const uint8_t* p, uint64_t* lo, uint64_t* hi) {
     memcpy(hi, p + 8, 8);
 }
 
-static int sums_finish(bitz_ctx* ctx, unsigned nblocks, int K, void* out_dev) {
+int sums_finish(bitz_ctx* ctx, unsigned nblocks, int K, void* out_dev) {
     if (ctx->defer_sums) {  // the caller reduces the block partials itself
         ctx->deferred_nb = nblocks;
         return BITZ_OK;

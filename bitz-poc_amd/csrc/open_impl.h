@@ -87,6 +87,22 @@ int gkr_level1_fold(bitz_ctx* ctx, const uint64_t* packed, const void* A, int t,
 int gkr_level2(bitz_ctx* ctx, const uint64_t* packed, const void* A, int t, int s, void* out);
 int gkr_level3(bitz_ctx* ctx, const uint64_t* packed, const void* A, int t, int s, void* out);
 int gkr_level1(bitz_ctx* ctx, const uint64_t* packed, const void* A, int t, int s, void* out);
+// Low GKR levels off 16-bit row words (forest.hip): nr[(y << s) | c] holds
+// the bits of column c at rows y + w 2^(t-4), w < 16 (even w in the low
+// byte, odd w in the high byte). The level-l layer (l = 0, 1, 2), after its
+// first 2 - l rounds (challenges chal[0..2-l)), takes round 2 - l's sums and
+// the fold by chal[2 - l] into halves of 2^(t+s-4) entries (right half
+// times eq_hi[c >> 8] when given, s > 8). Challenges from chal_host, or from
+// the device array chal_dev when non-null.
+int gkr_nibble_rows(bitz_ctx* ctx, const uint64_t* packed, int t, int s, void* nr);
+int gkr_nibble_round_sums(bitz_ctx* ctx, int level, const void* nr, const void* A, int t, int s, const hgf* chal_host,
+                          const void* chal_dev, const void* eq_c, const void* eq_y, int send_one, void* out_dev);
+int gkr_nibble_fold(bitz_ctx* ctx, int level, const void* nr, const void* A, int t, int s, const hgf* chal_host,
+                    const void* chal_dev, const void* eq_hi, void* l_out, void* r_out);
+// Reduces nblocks x K block partials (ctx->scratch.partials) to out_dev, or
+// to the mailbox for the host when out_dev is null (deferred when the
+// context defers sums to the device transcript).
+int sums_finish(bitz_ctx* ctx, unsigned nblocks, int K, void* out_dev);
 int product_level(bitz_ctx* ctx, const void* in, uint64_t n, void* out);
 int xi_combined(bitz_ctx* ctx, const uint64_t* packed, int t, int s, const void* w2, void* out);
 int fold_rows_eq(bitz_ctx* ctx, const uint64_t* rows, int t, int s, const void* eq, void* out);

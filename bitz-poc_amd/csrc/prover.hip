@@ -283,6 +283,9 @@ struct Prover {
     G g;
     // device buffers
     DevBuf packed, rowimg, colimg, folds, roww, levels, eqc, eqy, eqcinv, tmp0, tmp1, tmp2, tmp3, qdev, pt, fsbuf;
+    // the committed bits as 16-bit row words (gkr_nibble_rows), read by the
+    // low GKR layers' table rounds
+    DevBuf nibbles;
     // Split column weights of a weighted layer: eq_c[c] = eqlo[c & (2^wlo - 1)]
     // * eqhi[c >> wlo] (wlo = min(s, 8)). The right half carries eqhi only:
     // its multiplies are window-table products over 256-column runs, and
@@ -352,9 +355,15 @@ struct Prover {
     }
 
     // ---- batched grand-product GKR (Forest::prove == gpgkr_prove) ----
-    // layer kinds: halves materialised, halves = the implicit level 1 (round 0
-    // from bit patterns), halves = the implicit leaves (rounds 0-1 from bit patterns)
-    enum LayerKind { kDense = 0, kLevel1 = 1, kLeaf = 2 };
+    // Layer kinds: halves materialised, or an implicit low level whose first
+    // rounds run off the bits and whose fold writes halves of 2^(t+s-4)
+    // entries: the leaves (rounds 0-1 from bit patterns, round 2 from row
+    // words), level 1 (round 0 from bit patterns, round 1 from row words),
+    // level 2 (round 0 from row words).
+    enum LayerKind { kDense = 0, kLevel1 = 1, kLeaf = 2, kLevel2 = 3 };
+    // the level a low layer's row-word round serves, and that round's index
+    static int nibble_level(LayerKind kind) { return kind == kLeaf ? 0 : kind == kLevel1 ? 1 : 2; }
+    static int nibble_round(LayerKind kind) { return 2 - nibble_level(kind); }
     int gkr_layer(InnerTranscript& it, uint4* l, uint4* r, LayerKind kind, std::vector<G>& internal, G* claim) {
         const bool leaf = kind == kLeaf;
         const int L = (int)internal.size();
@@ -440,13 +449,16 @@ struct Prover {
         uint8_t pending_rho[16], rho0[16];
         uint8_t last[32];
         bool tail_done = false;
-        BITZ_REQUIRE(kind == kDense || (L >= 3 && t >= 3), BITZ_ERR_SHAPE, "GKR: layer too small for bit patterns");
+        BITZ_REQUIRE(kind == kDense || (L >= 4 && t >= 7 && (!weigh || wsplit)), BITZ_ERR_SHAPE,
+                     "GKR: layer too small for bit patterns");
+        std::vector<hgf> chal;  // the layer's challenges so far (for its row-word round)
+        const int nround = kind == kDense ? -1 : nibble_round(kind);
         for (int round = 0; round < L; round++) {
             G z = internal[round];
             int rem = L - 1 - round;
             uint64_t hh = 1ull << rem;
             int s_c = std::min(s, rem);
-            const bool bit_round = (leaf && round <= 1) || (kind == kLevel1 && round == 0);
+            const bool bit_round = kind != kDense && round <= nround;
             if (!bit_round && rem <= ctx->tune.host_tail_log && (!weighted || rem < s)) {
                 // the remaining rounds on the host, from dense halves
                 if (pending_fold && (rc = fold_pair(ctx, l, r, 2 * hh, pending_rho))) return rc;
@@ -463,7 +475,10 @@ struct Prover {
                 if ((rc = eq_dev(yv, eqy))) return rc;
             }
             int send_one = geq(z, gzero()) ? 1 : 0;
-            if (leaf && round == 0)
+            if (round == nround)
+                rc = gkr_nibble_round_sums(ctx, nibble_level(kind), nibbles.ptr, rowimg.ptr, t, s, chal.data(),
+                                           nullptr, eqc.ptr, eqy.ptr, send_one, nullptr);
+            else if (leaf && round == 0)
                 rc = gkr_leaf_round_sums(ctx, (const uint64_t*)packed.ptr, rowimg.ptr, t, s, eqc.ptr, s_c, eqy.ptr,
                                          send_one, nullptr);
             else if (leaf && round == 1)
@@ -504,9 +519,11 @@ struct Prover {
             it.verifier_gf(rb);
             G rr = gfrom(rb);
             next.push_back(rr);
-            if (kind == kLevel1 && round == 0) {
-                if ((rc = gkr_level1_fold(ctx, (const uint64_t*)packed.ptr, rowimg.ptr, t, s, rb,
-                                          weigh && !wsplit ? eqc.ptr : nullptr, l, r, nullptr, hi_weights(weigh))))
+            if (bit_round) chal.push_back(hgf{rr.lo, rr.hi});
+            if (round == nround) {
+                // the level stays implicit up to here: its fold writes the halves
+                if ((rc = gkr_nibble_fold(ctx, nibble_level(kind), nibbles.ptr, rowimg.ptr, t, s, chal.data(), nullptr,
+                                          hi_weights(weigh), l, r)))
                     return rc;
                 weighted = weigh;
                 pending_fold = false;
@@ -514,12 +531,7 @@ struct Prover {
                 // leaves stay implicit: round 1 reads the bits again
                 memcpy(rho0, rb, 16);
                 pending_fold = false;
-            } else if (leaf && round == 1) {
-                if ((rc = gkr_leaf_fold2(ctx, (const uint64_t*)packed.ptr, rowimg.ptr, t, s, rho0, rb,
-                                         weigh && !wsplit ? eqc.ptr : nullptr, l, r, nullptr, nullptr,
-                                         hi_weights(weigh))))
-                    return rc;
-                weighted = weigh;
+            } else if (bit_round) {
                 pending_fold = false;
             } else {
                 pending_fold = true;
@@ -592,35 +604,36 @@ struct Prover {
 
     // returns the leaf point (LE, t+s coordinates) and claim
     int gkr(InnerTranscript& it, const std::vector<G>& zeta, std::vector<G>* point, G* claim) {
-        // storage of levels 1..t-1 back to back (level j: 2^(t+s-j) entries,
-        // level 1 first); levels 2..t-1 are materialised, level 1's region
-        // only receives the folded halves of the level-1 and leaf layers
-        uint64_t total = 0;
-        for (int j = 1; j < t; j++) total += 1ull << (t + s - j);
+        // Storage: the halves the leaf, level-1 and level-2 layers fold into
+        // (2 x 2^(t+s-4) entries, shared: the layers run one after another),
+        // then levels 3..t-1 back to back (level j: 2^(t+s-j) entries). Levels
+        // 0-2 are never materialised; level 3 comes straight from the bits,
+        // the upper levels from products.
+        const uint64_t h4 = 1ull << (t + s - 4);
+        uint64_t total = 2 * h4;
+        for (int j = 3; j < t; j++) total += 1ull << (t + s - j);
         if (!levels.alloc(total * 16)) return BITZ_ERR_ALLOC;
         std::vector<uint4*> lv(t + 1, nullptr);
-        uint4* base = (uint4*)levels.ptr;
-        uint64_t off = 0;
-        for (int j = 1; j < t; j++) {
-            lv[j] = base + off;
+        uint4* low = (uint4*)levels.ptr;
+        uint64_t off = 2 * h4;
+        for (int j = 3; j < t; j++) {
+            lv[j] = low + off;
             off += 1ull << (t + s - j);
         }
-        // level 1 stays implicit (its layer reads the bits); levels 2 and 3
-        // come straight from the bits, the upper levels from products
-        int rc = gkr_level2(ctx, (const uint64_t*)packed.ptr, rowimg.ptr, t, s, lv[2]);
+        if (!nibbles.alloc(2 * h4 * 2)) return BITZ_ERR_ALLOC;  // 2^(t+s-4) u16
+        int rc = gkr_nibble_rows(ctx, (const uint64_t*)packed.ptr, t, s, nibbles.ptr);
         if (rc) return rc;
         if ((rc = gkr_level3(ctx, (const uint64_t*)packed.ptr, rowimg.ptr, t, s, lv[3]))) return rc;
         for (int j = 3; j + 1 < t; j++)
             if ((rc = product_level(ctx, lv[j], 1ull << (t + s - j), lv[j + 1]))) return rc;
         std::vector<G> internal(zeta.rbegin(), zeta.rend());
         G cl = gzero();
-        for (int j = t - 1; j >= 1; j--) {
+        for (int j = t - 1; j >= 3; j--) {
             uint64_t half = 1ull << (t + s - j - 1);
-            if ((rc = gkr_layer(it, lv[j], lv[j] + half, j == 1 ? kLevel1 : kDense, internal, &cl))) return rc;
+            if ((rc = gkr_layer(it, lv[j], lv[j] + half, kDense, internal, &cl))) return rc;
         }
-        // leaf layer: the first fold lands in level 1's storage
-        uint64_t q = 1ull << (t + s - 2);
-        if ((rc = gkr_layer(it, lv[1], lv[1] + q, kLeaf, internal, &cl))) return rc;
+        for (LayerKind kind : {kLevel2, kLevel1, kLeaf})
+            if ((rc = gkr_layer(it, low, low + h4, kind, internal, &cl))) return rc;
         point->assign(internal.rbegin(), internal.rend());
         *claim = cl;
         return BITZ_OK;
@@ -667,12 +680,19 @@ struct Prover {
         ctx->defer_sums = true;
         bool weighted = false, pending_fold = false;
         const uint4* pending_rho = nullptr;
+        BITZ_REQUIRE(kind == kDense || (L >= 4 && t >= 7 && (!weigh || wsplit)), BITZ_ERR_SHAPE,
+                     "GKR: layer too small for bit patterns");
+        const int nround = kind == kDense ? -1 : nibble_round(kind);
         for (int round = 0; round < L; round++) {
             int rem = L - 1 - round;
             uint64_t hh = 1ull << rem;
             int s_c = std::min(s, rem);
             int send_one = geq(internal[round], gzero()) ? 1 : 0;
-            if (leaf && round == 0)
+            // the layer's challenges r_0.. are rho[0..]: the row-word kernels read them there
+            if (round == nround)
+                rc = gkr_nibble_round_sums(ctx, nibble_level(kind), nibbles.ptr, rowimg.ptr, t, s, nullptr, rho,
+                                           eqc.ptr, eqy.ptr, send_one, nullptr);
+            else if (leaf && round == 0)
                 rc = gkr_leaf_round_sums(ctx, (const uint64_t*)packed.ptr, rowimg.ptr, t, s, eqc.ptr, s_c, eqy.ptr,
                                          send_one, nullptr);
             else if (leaf && round == 1)
@@ -705,19 +725,13 @@ struct Prover {
                                    (uint32_t)round)))
                 return rc;
             pending_fold = false;
-            if (kind == kLevel1 && round == 0) {
-                if ((rc = gkr_level1_fold(ctx, (const uint64_t*)packed.ptr, rowimg.ptr, t, s, nullptr,
-                                          weigh && !wsplit ? eqc.ptr : nullptr, l, r, rho, hi_weights(weigh))))
+            if (round == nround) {
+                if ((rc = gkr_nibble_fold(ctx, nibble_level(kind), nibbles.ptr, rowimg.ptr, t, s, nullptr, rho,
+                                          hi_weights(weigh), l, r)))
                     return rc;
                 weighted = weigh;
-            } else if (leaf && round == 0) {
-                // leaves stay implicit: round 1 reads the bits again
-            } else if (leaf && round == 1) {
-                if ((rc = gkr_leaf_fold2(ctx, (const uint64_t*)packed.ptr, rowimg.ptr, t, s, nullptr, nullptr,
-                                         weigh && !wsplit ? eqc.ptr : nullptr, l, r, rho, rho + 1,
-                                         hi_weights(weigh))))
-                    return rc;
-                weighted = weigh;
+            } else if (kind != kDense && round < nround) {
+                // the level stays implicit: its next round reads the bits again
             } else {
                 pending_fold = true;
                 pending_rho = rho + round;

@@ -210,6 +210,20 @@ int bitz_gkr_level1_round0_sums_dev(bitz_ctx* ctx, const void* packed, const voi
                                     const void* eq_c, const void* eq_y, int send_one, uint8_t out_host[32]);
 int bitz_gkr_level1_fold_dev(bitz_ctx* ctx, const void* packed, const void* A, int t, int s, const uint8_t rho[16],
                              const void* eq_c, void* l_out, void* r_out);
+/* The low GKR levels off 16-bit row words (Forest::bit_rounds and the
+ * table-driven rounds, src/bitz/forest.rs:1-31, 381-740): nr[(y << s) | c]
+ * (u16, y < 2^(t-4)) holds the bits of column c at rows y + w 2^(t-4), w < 16,
+ * even w in the low byte, odd w in the high byte. The level-l layer
+ * (l = 0 leaves, 1, 2), after its first 2 - l rounds (challenges chal, 16 bytes
+ * each), takes round 2 - l's sums (weights eq_y[y] (2^(t-4) entries) *
+ * eq_c[c]) and the fold by one more challenge into halves of 2^(t+s-4)
+ * entries (right half times eq_hi[c >> 8] when given, s > 8). */
+int bitz_gkr_nibble_rows_dev(bitz_ctx* ctx, const void* packed, int t, int s, void* nr);
+int bitz_gkr_nibble_round_sums_dev(bitz_ctx* ctx, int level, const void* nr, const void* A, int t, int s,
+                                   const uint8_t* chal, const void* eq_c, const void* eq_y, int send_one,
+                                   uint8_t out_host[32]);
+int bitz_gkr_nibble_fold_dev(bitz_ctx* ctx, int level, const void* nr, const void* A, int t, int s,
+                             const uint8_t* chal, const void* eq_hi, void* l_out, void* r_out);
 /* one product level: out[i] = in[i] * in[i + n/2] (gkr.rs:93-108) */
 int bitz_product_level_dev(bitz_ctx* ctx, const void* in, uint64_t n, void* out);
 /* xi_combined_rows_packed (src/ligerito.rs:563-630): out[b] = sum_c w2[c] bit(c,b) */

@@ -19,10 +19,10 @@
 // E, and the bits Om = Em << sh for O:
 //   l = 0 (k = 2): Em = 0x0f, sh = 4;  l = 1 (k = 1): Em = 0x33, sh = 2;
 //   l = 2 (k = 0): Em = 0x55, sh = 1.
-// So round k's products E_a O_b are lookups in four 256-entry tables per y,
-// P_ab[i] = eq_y[y] T_{a,0}[ext(i & Em)] T_{b,1}[ext((i & Om) >> sh)], at
-// i = the low byte (ab = 00), the high byte (11), or the mix of the two
-// (01, 10); a column XORs its lookups over y and multiplies by eq_c[c] once.
+// So round k's products E_a O_b are functions of one byte per slot,
+// P_ab[i] = T_{a,0}[ext(i & Em)] T_{b,1}[ext((i & Om) >> sh)], at i = the
+// low byte (ab = 00), the high byte (11), or the mix of the two (01, 10),
+// and a row's sums are those tables against the eq_c buckets of its columns.
 // The fold by r_k is a lookup in two 256-entry tables per y,
 // F_p[i] = (1 + r_k) T_{0,p}[ext(i & Em)] + r_k T_{1,p}[ext((i & Om) >> sh)],
 // into halves of 2^(t+s-4) entries, where the layer's dense rounds start.
@@ -90,95 +90,111 @@ __global__ __launch_bounds__(kT) void k_nibble_rows(const uint64_t* __restrict__
     }
 }
 
+// T_{h,p}[j] of row y (e = (h*2 + p)*16 + j)
+template <int L>
+__device__ __forceinline__ gf128 nibble_tval(const uint4* __restrict__ A, int t, const Chal& ch, uint64_t y, uint32_t e) {
+    constexpr int k = Nib<L>::k;
+    const uint64_t y4 = 1ull << (t - 4);
+    const uint32_t h = (e >> 5) & 1u, p = (e >> 4) & 1u, j = e & 15u;
+    gf128 acc = gf_zero();
+#pragma unroll
+    for (uint32_t v = 0; v < (1u << k); v++) {
+        gf128 term = gf_one();
+        bool first = true;
+#pragma unroll
+        for (uint32_t u = 0; u < (1u << L); u++) {
+            if ((j >> (v + (u << k))) & 1u) {
+                const uint64_t w = h + 2 * v + (p << (k + 1)) + ((uint64_t)u << (k + 2));
+                const gf128 a = gf_load(&A[y + w * y4]);
+                term = first ? a : gf_mul(term, a);
+                first = false;
+            }
+        }
+        if (k == 0) {
+            acc = term;
+        } else {
+            // eq(r, v): v's top bit is bound by r_0
+            gf128 wv = gf_one();
+            bool wfirst = true;
+#pragma unroll
+            for (int i = 0; i < k; i++) {
+                const gf128 r = chal(ch, i);
+                const gf128 f = ((v >> (k - 1 - i)) & 1u) ? r : gf_add(r, gf_one());
+                wv = wfirst ? f : gf_mul(wv, f);
+                wfirst = false;
+            }
+            acc = gf_add(acc, first ? wv : gf_mul(wv, term));
+        }
+    }
+    return acc;
+}
+
 // T[((y*2 + h)*2 + p)*16 + j], unscaled
 template <int L>
 __global__ __launch_bounds__(kT) void k_nibble_ttab(const uint4* __restrict__ A, int t, Chal ch,
                                                     uint4* __restrict__ T) {
-    constexpr int k = Nib<L>::k;
-    const uint64_t y4 = 1ull << (t - 4), n = y4 * 64;
-    for (uint64_t idx = grid_tid(); idx < n; idx += grid_threads()) {
-        const uint64_t y = idx >> 6;
-        const uint32_t h = (uint32_t)(idx >> 5) & 1u, p = (uint32_t)(idx >> 4) & 1u, j = (uint32_t)idx & 15u;
-        gf128 acc = gf_zero();
-#pragma unroll
-        for (uint32_t v = 0; v < (1u << k); v++) {
-            gf128 term = gf_one();
-            bool first = true;
-#pragma unroll
-            for (uint32_t u = 0; u < (1u << L); u++) {
-                if ((j >> (v + (u << k))) & 1u) {
-                    const uint64_t w = h + 2 * v + (p << (k + 1)) + ((uint64_t)u << (k + 2));
-                    const gf128 a = gf_load(&A[y + w * y4]);
-                    term = first ? a : gf_mul(term, a);
-                    first = false;
-                }
-            }
-            if (k == 0) {
-                acc = term;
-            } else {
-                // eq(r, v): v's top bit is bound by r_0
-                gf128 wv = gf_one();
-                bool wfirst = true;
-#pragma unroll
-                for (int i = 0; i < k; i++) {
-                    const gf128 r = chal(ch, i);
-                    const gf128 f = ((v >> (k - 1 - i)) & 1u) ? r : gf_add(r, gf_one());
-                    wv = wfirst ? f : gf_mul(wv, f);
-                    wfirst = false;
-                }
-                acc = gf_add(acc, first ? wv : gf_mul(wv, term));
-            }
-        }
-        gf_store(&T[idx], acc);
-    }
+    const uint64_t n = (1ull << (t - 4)) * 64;
+    for (uint64_t idx = grid_tid(); idx < n; idx += grid_threads())
+        gf_store(&T[idx], nibble_tval<L>(A, t, ch, idx >> 6, (uint32_t)idx & 63u));
 }
 
-// P[(y*4 + ab)*256 + i] = eq_y[y] T_{a,0}[ext(i & em)] T_{b,1}[ext((i & om) >> sh)]
+// Round sums by buckets. Block y: each column scatters eq_c[c] into the
+// buckets of its four indices (B_ab[i] = sum of eq_c over the row's columns
+// whose ab-index is i, one LDS array per 32-bit word so that random indices
+// spread over the banks); then
+//   sum_ab = sum_i P_ab[i] B_ab[i] = sum_e T_{a,0}[e] sum_o T_{b,1}[o] B_ab[i(e, o)]
+// takes 1280 products per row instead of one per slot, with T_{a,0}
+// pre-scaled by eq_y[y]. end = sum_{ss}, ss = send_one; inf = sum over ab.
 template <int L>
-__global__ __launch_bounds__(kT) void k_nibble_ptab(const uint4* __restrict__ T, int t, const uint4* __restrict__ eq_y,
-                                                    uint4* __restrict__ P) {
-    using N = Nib<L>;
-    const uint64_t n = (1ull << (t - 4)) * 1024;
-    for (uint64_t idx = grid_tid(); idx < n; idx += grid_threads()) {
-        const uint64_t y = idx >> 10;
-        const uint32_t a = (uint32_t)(idx >> 9) & 1u, b = (uint32_t)(idx >> 8) & 1u, i = (uint32_t)idx & 255u;
-        const gf128 e = gf_load(&T[((y * 2 + a) * 2) * 16 + N::ext(i & N::em)]);
-        const gf128 o = gf_load(&T[((y * 2 + b) * 2 + 1) * 16 + N::ext((i & N::om) >> N::sh)]);
-        gf_store(&P[idx], gf_mul(gf_mul(e, gf_load(&eq_y[y])), o));
-    }
+__device__ __forceinline__ uint32_t nibble_place(uint32_t j) {
+    if (L == 0) return j;
+    if (L == 1) return (j & 3u) | ((j & 12u) << 2);
+    return (j & 1u) | ((j & 2u) << 1) | ((j & 4u) << 2) | ((j & 8u) << 3);
 }
 
-// Round sums: thread (c, yl) XORs its rows' lookups, then multiplies by eq_c[c].
 template <int L>
-__global__ __launch_bounds__(kT) void k_nibble_round_sums(const uint16_t* __restrict__ nr, const uint4* __restrict__ P,
-                                                          int t, int s, const uint4* __restrict__ eq_c,
-                                                          uint64_t ylanes, int send_one, uint4* __restrict__ partials) {
+__global__ __launch_bounds__(kT) void k_nibble_bucket_sums(const uint16_t* __restrict__ nr, const uint4* __restrict__ A,
+                                                           int t, int s, Chal ch, const uint4* __restrict__ eq_c,
+                                                           const uint4* __restrict__ eq_y, int send_one,
+                                                           uint4* __restrict__ partials) {
     using N = Nib<L>;
-    const uint64_t cols = 1ull << s, y4 = 1ull << (t - 4);
-    gf128 acc[2] = {gf_zero(), gf_zero()};
-    const uint64_t gt = grid_tid();
-    if (gt < cols * ylanes) {
-        const uint64_t c = gt & (cols - 1), yl = gt >> s;
-        uint32_t e[4] = {0, 0, 0, 0}, q[4] = {0, 0, 0, 0};
-        for (uint64_t y = yl; y < y4; y += ylanes) {
-            const uint32_t w = nr[(y << s) | c];
-            const uint32_t lo = w & 255u, hi = w >> 8;
-            const uint4* base = P + y * 1024;
-            const uint4 v00 = base[lo], v11 = base[768 + hi];
-            const uint4 v01 = base[256 + ((lo & N::em) | (hi & N::om))];
-            const uint4 v10 = base[512 + ((hi & N::em) | (lo & N::om))];
-            const uint4 ve = send_one ? v11 : v00;
-            e[0] ^= ve.x, e[1] ^= ve.y, e[2] ^= ve.z, e[3] ^= ve.w;
-            q[0] ^= v00.x ^ v01.x ^ v10.x ^ v11.x;
-            q[1] ^= v00.y ^ v01.y ^ v10.y ^ v11.y;
-            q[2] ^= v00.z ^ v01.z ^ v10.z ^ v11.z;
-            q[3] ^= v00.w ^ v01.w ^ v10.w ^ v11.w;
-        }
-        const gf128 wc = gf_load(&eq_c[c]);
-        acc[0] = gf_mul(gf128{{e[0], e[1], e[2], e[3]}}, wc);
-        acc[1] = gf_mul(gf128{{q[0], q[1], q[2], q[3]}}, wc);
+    __shared__ uint32_t B[4][4][256];  // [word][ab][index]
+    __shared__ uint4 T[64];
+    const uint64_t y = blockIdx.x, cols = 1ull << s;
+    for (uint32_t i = threadIdx.x; i < 4 * 4 * 256; i += kT) (&B[0][0][0])[i] = 0u;
+    if (threadIdx.x < 64) {
+        gf128 v = nibble_tval<L>(A, t, ch, y, threadIdx.x);
+        if (((threadIdx.x >> 4) & 1u) == 0) v = gf_mul(v, gf_load(&eq_y[y]));  // p = 0: T_{a,0} eq_y[y]
+        gf_store(&T[threadIdx.x], v);
     }
-    block_xor_store<2>(acc, partials + (size_t)blockIdx.x * 2);
+    __syncthreads();
+    for (uint64_t c = threadIdx.x; c < cols; c += kT) {
+        const uint32_t w = nr[(y << s) | c];
+        const uint32_t lo = w & 255u, hi = w >> 8;
+        const uint32_t ix[4] = {lo, (lo & N::em) | (hi & N::om), (hi & N::em) | (lo & N::om), hi};
+        const uint4 e = eq_c[c];
+#pragma unroll
+        for (int ab = 0; ab < 4; ab++) {
+            atomicXor(&B[0][ab][ix[ab]], e.x);
+            atomicXor(&B[1][ab][ix[ab]], e.y);
+            atomicXor(&B[2][ab][ix[ab]], e.z);
+            atomicXor(&B[3][ab][ix[ab]], e.w);
+        }
+    }
+    __syncthreads();
+    // thread (ab, e, oq): T_{a,0}[e] * sum over its four o of T_{b,1}[o] B_ab[i(e, o)]
+    const uint32_t ab = threadIdx.x >> 6, a = ab >> 1, b = ab & 1u, ej = (threadIdx.x >> 2) & 15u,
+                   oq = threadIdx.x & 3u;
+    gf256 acc = gf256_zero();
+#pragma unroll
+    for (uint32_t o = oq * 4; o < oq * 4 + 4; o++) {
+        const uint32_t i = nibble_place<L>(ej) | (nibble_place<L>(o) << N::sh);
+        const gf128 bv = gf128{{B[0][ab][i], B[1][ab][i], B[2][ab][i], B[3][ab][i]}};
+        gf256_acc(&acc, gf_mul_wide(gf_load(&T[(b * 2 + 1) * 16 + o]), bv));
+    }
+    const gf128 v = gf_mul(gf_reduce(acc), gf_load(&T[(a * 2) * 16 + ej]));
+    gf128 sums[2] = {(a == (uint32_t)send_one && b == (uint32_t)send_one) ? v : gf_zero(), v};
+    block_xor_store<2>(sums, partials + (size_t)blockIdx.x * 2);
 }
 
 // F[(y*2 + p)*256 + i] = (1 + r) T_{0,p}[ext(i & em)] + r T_{1,p}[ext((i & om) >> sh)], r = challenge k
@@ -220,20 +236,24 @@ __global__ __launch_bounds__(kT) void k_nibble_fold(const uint16_t* __restrict__
     }
 }
 
-// The same with the right half times eq_hi[c >> 8] (s > 8): block b takes
-// the 256-column run g = b mod 2^(s-8) (one window table of its factor) and
-// the rows b >> (s-8), + row_lanes.
+// The same with the right half times eq_hi[c >> 8] (s > 8). Workgroups are
+// dispatched to the eight XCDs in turn, so block b runs on XCD b mod 8: it
+// takes the rows y = b mod 8 (mod 8) only, so that each row's fold tables
+// are read into one XCD's L2. Block b = xcd + 8 (g + runs l8) takes the
+// 256-column run g (one window table of its factor) and the rows
+// y = xcd + 8 (l8 + lanes8 k).
 template <int L>
 __global__ __launch_bounds__(kT) void k_nibble_fold_hi(const uint16_t* __restrict__ nr, const uint4* __restrict__ F,
                                                        int t, int s, const uint4* __restrict__ eq_hi,
-                                                       uint32_t row_lanes, uint4* __restrict__ l_out,
+                                                       uint32_t lanes8, uint4* __restrict__ l_out,
                                                        uint4* __restrict__ r_out) {
     __shared__ uint4 tab[512];
-    const uint32_t g = blockIdx.x & ((1u << (s - 8)) - 1);
+    const uint32_t xcd = blockIdx.x & 7u, li = blockIdx.x >> 3;
+    const uint32_t g = li & ((1u << (s - 8)) - 1), l8 = li >> (s - 8);
     gf_table_build(tab, gf_load(&eq_hi[g]), threadIdx.x, blockDim.x);
     __syncthreads();
     const uint64_t y4 = 1ull << (t - 4), c = ((uint64_t)g << 8) + threadIdx.x;
-    for (uint64_t y = blockIdx.x >> (s - 8); y < y4; y += row_lanes) {
+    for (uint64_t y = xcd + 8ull * l8; y < y4; y += 8ull * lanes8) {
         const uint64_t x = (y << s) + c;
         uint32_t ie, io;
         nibble_fold_idx<L>(nr[x], &ie, &io);
@@ -283,40 +303,25 @@ int gkr_nibble_round_sums(bitz_ctx* ctx, int level, const void* nr, const void* 
                           const void* chal_dev, const void* eq_c, const void* eq_y, int send_one, void* out_dev) {
     BITZ_REQUIRE(level >= 0 && level <= 2 && t >= 4, BITZ_ERR_SHAPE, "gkr_nibble_round_sums: bad level or shape");
     const Chal ch = chal_args(chal_host, 2 - level, chal_dev);
-    uint4* T;
-    int rc = nibble_ttab(ctx, level, A, t, ch, &T);
-    if (rc) return rc;
     const uint64_t y4 = 1ull << (t - 4), cols = 1ull << s;
-    if (!ctx->scratch.nib_tab.alloc(y4 * 1024 * 16)) return BITZ_ERR_ALLOC;
-    uint4* P = (uint4*)ctx->scratch.nib_tab.ptr;
+    BITZ_REQUIRE(y4 <= 0x7fffffffull, BITZ_ERR_SHAPE, "gkr_nibble_round_sums: too many rows");
+    const unsigned nb = (unsigned)y4;  // one block per row
+    int rc = ctx->scratch.ensure((size_t)nb * 2 * 16);
+    if (rc) return rc;
     {
-        TimedLaunch tl(ctx, "nibble_ptab", 16.0 * (double)(y4 * 1024), 0.0);
-        const dim3 grid(blocks_for(y4 * 1024)), block(kT);
-        if (level == 0)
-            hipLaunchKernelGGL(k_nibble_ptab<0>, grid, block, 0, ctx->stream, T, t, (const uint4*)eq_y, P);
-        else if (level == 1)
-            hipLaunchKernelGGL(k_nibble_ptab<1>, grid, block, 0, ctx->stream, T, t, (const uint4*)eq_y, P);
-        else
-            hipLaunchKernelGGL(k_nibble_ptab<2>, grid, block, 0, ctx->stream, T, t, (const uint4*)eq_y, P);
-        BITZ_LAUNCH_CHECK();
-    }
-    // enough (column, row-lane) threads to fill the chip; each lane walks rows
-    uint64_t ylanes = std::max<uint64_t>(1, std::min<uint64_t>(y4, (1ull << 20) >> s));
-    const unsigned nb = (unsigned)((cols * ylanes + kT - 1) / kT);
-    if ((rc = ctx->scratch.ensure((size_t)nb * 2 * 16))) return rc;
-    {
-        TimedLaunch tl(ctx, "nibble_round_sums", 2.0 * (double)(y4 * cols), 0.0);
+        TimedLaunch tl(ctx, "nibble_round_sums", (double)(y4 * cols) * 2.0 + 16.0 * (double)cols, 0.0);
         const dim3 grid(nb), block(kT);
+        const uint16_t* w = (const uint16_t*)nr;
         uint4* parts = (uint4*)ctx->scratch.partials.ptr;
         if (level == 0)
-            hipLaunchKernelGGL(k_nibble_round_sums<0>, grid, block, 0, ctx->stream, (const uint16_t*)nr, P, t, s,
-                               (const uint4*)eq_c, ylanes, send_one, parts);
+            hipLaunchKernelGGL(k_nibble_bucket_sums<0>, grid, block, 0, ctx->stream, w, (const uint4*)A, t, s, ch,
+                               (const uint4*)eq_c, (const uint4*)eq_y, send_one, parts);
         else if (level == 1)
-            hipLaunchKernelGGL(k_nibble_round_sums<1>, grid, block, 0, ctx->stream, (const uint16_t*)nr, P, t, s,
-                               (const uint4*)eq_c, ylanes, send_one, parts);
+            hipLaunchKernelGGL(k_nibble_bucket_sums<1>, grid, block, 0, ctx->stream, w, (const uint4*)A, t, s, ch,
+                               (const uint4*)eq_c, (const uint4*)eq_y, send_one, parts);
         else
-            hipLaunchKernelGGL(k_nibble_round_sums<2>, grid, block, 0, ctx->stream, (const uint16_t*)nr, P, t, s,
-                               (const uint4*)eq_c, ylanes, send_one, parts);
+            hipLaunchKernelGGL(k_nibble_bucket_sums<2>, grid, block, 0, ctx->stream, w, (const uint4*)A, t, s, ch,
+                               (const uint4*)eq_c, (const uint4*)eq_y, send_one, parts);
         BITZ_LAUNCH_CHECK();
     }
     return sums_finish(ctx, nb, 2, out_dev);
@@ -331,7 +336,7 @@ int gkr_nibble_fold(bitz_ctx* ctx, int level, const void* nr, const void* A, int
     int rc = nibble_ttab(ctx, level, A, t, ch, &T);
     if (rc) return rc;
     const uint64_t y4 = 1ull << (t - 4), n = y4 << s;
-    if (!ctx->scratch.nib_tab.alloc(y4 * 1024 * 16)) return BITZ_ERR_ALLOC;
+    if (!ctx->scratch.nib_tab.alloc(y4 * 512 * 16)) return BITZ_ERR_ALLOC;
     uint4* F = (uint4*)ctx->scratch.nib_tab.ptr;
     {
         const dim3 grid(blocks_for(y4 * 512)), block(kT);
@@ -347,16 +352,17 @@ int gkr_nibble_fold(bitz_ctx* ctx, int level, const void* nr, const void* A, int
     const uint16_t* w = (const uint16_t*)nr;
     if (eq_hi) {
         const uint32_t runs = 1u << (s - 8);
-        const uint32_t lanes = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(y4, std::max<uint32_t>(1u, 8192u / runs)));
-        const dim3 grid(lanes << (s - 8)), block(kT);
+        const uint32_t lanes8 =
+            (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(std::max<uint64_t>(y4 / 8, 1), std::max(1u, 1024u / runs)));
+        const dim3 grid(8u * runs * lanes8), block(kT);
         if (level == 0)
-            hipLaunchKernelGGL(k_nibble_fold_hi<0>, grid, block, 0, ctx->stream, w, F, t, s, (const uint4*)eq_hi, lanes,
+            hipLaunchKernelGGL(k_nibble_fold_hi<0>, grid, block, 0, ctx->stream, w, F, t, s, (const uint4*)eq_hi, lanes8,
                                (uint4*)l_out, (uint4*)r_out);
         else if (level == 1)
-            hipLaunchKernelGGL(k_nibble_fold_hi<1>, grid, block, 0, ctx->stream, w, F, t, s, (const uint4*)eq_hi, lanes,
+            hipLaunchKernelGGL(k_nibble_fold_hi<1>, grid, block, 0, ctx->stream, w, F, t, s, (const uint4*)eq_hi, lanes8,
                                (uint4*)l_out, (uint4*)r_out);
         else
-            hipLaunchKernelGGL(k_nibble_fold_hi<2>, grid, block, 0, ctx->stream, w, F, t, s, (const uint4*)eq_hi, lanes,
+            hipLaunchKernelGGL(k_nibble_fold_hi<2>, grid, block, 0, ctx->stream, w, F, t, s, (const uint4*)eq_hi, lanes8,
                                (uint4*)l_out, (uint4*)r_out);
     } else {
         const dim3 grid(blocks_for(n, 1, 8192)), block(kT);

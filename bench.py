@@ -15,8 +15,9 @@ launch and host phase ("kernels", "host_phases_ms") and picks the dominant
 kernel; inside the timed region that kernel's launches of the first timed
 step carry HIP events (on the library's stream), and the roofline object is
 computed from them.
-cpu_baseline times the CPU restatement of the same protocol on a bounded
-sample on this host.
+cpu_baseline times the CPU restatement of the same protocol (oracle/: C
+kernels with OpenMP) at 2^20 u64 coefficients on this host, at the process's
+thread share and at 1 thread, beside the reference's published M5 figures.
 """
 from __future__ import annotations
 
@@ -58,7 +59,7 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--autotune", action="store_true",
                     help="time launch-shape candidates first (bitz_ctx_autotune) and prove with the fastest")
-    ap.add_argument("--cpu-log-coeffs", type=int, default=18)
+    ap.add_argument("--cpu-log-coeffs", type=int, default=20)
     ap.add_argument("--no-batched", action="store_true", help="skip the batched and serving measurements")
     ap.add_argument("--no-host-rows", action="store_true",
                     help="skip the PCIe-inclusive steps from rows in host memory (after the timed region)")
@@ -80,30 +81,81 @@ def shape_for(log_coeffs: int):
     return n, t, n - t
 
 
+# The reference's published CPU prover figures for commit+open (paper
+# Table 11, MacBook Air M5, rate 1/2): u64 coefficients -> (1 thread, 10
+# threads) Mcoeff/s, derived as coefficients / prover total.
+M5_PUBLISHED = {
+    20: {"ms_1_thread": 155.0, "ms_10_threads": 52.8},
+    24: {"ms_1_thread": 2175.0, "ms_10_threads": 575.0},
+}
+
+
+def _cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def _host_threads() -> int:
+    """this process's CPU share: the affinity set, capped by OMP_NUM_THREADS"""
+    n = len(os.sched_getaffinity(0))
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    if env.isdigit() and int(env) > 0:
+        n = min(n, int(env))
+    return n
+
+
 def cpu_baseline(log_coeffs: int):
+    """the CPU restatement (oracle/: C kernels with OpenMP, driven from
+    Python) on the same protocol at 2^log_coeffs u64 coefficients: commit +
+    standalone open, at the host's thread share and at 1 thread"""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import numpy as np
 
     import bitz_protocol as P
+    import oracle as O
     n, t, s = shape_for(log_coeffs)
     rng = np.random.default_rng(7)
     rows = rng.integers(0, 2**64, size=(1 << s) << (t - 6), dtype=np.uint64)
-    t0 = time.perf_counter()
-    pr = P.OracleProver(rows, t, s, 2, 4)
-    t1 = time.perf_counter()
-    claimed = pr.evaluation()
-    t2 = time.perf_counter()
-    pr.prove(claimed)
-    t3 = time.perf_counter()
-    secs = (t1 - t0) + (t3 - t2)
+
+    def timed(threads):
+        O.set_threads(threads)
+        t0 = time.perf_counter()
+        pr = P.OracleProver(rows, t, s, 2, 4)
+        t1 = time.perf_counter()
+        claimed = pr.evaluation()
+        t2 = time.perf_counter()
+        proof = pr.prove(claimed)
+        t3 = time.perf_counter()
+        return (t1 - t0) + (t3 - t2), proof
+
+    threads = _host_threads()
+    secs_n, proof_n = timed(threads)
+    secs_1, proof_1 = timed(1)
+    assert proof_n == proof_1, "CPU restatement: proof depends on the thread count"
+    O.set_threads(threads)
+    rate = lambda secs: round((1 << log_coeffs) / secs / 1e6, 4)
+    m5 = M5_PUBLISHED.get(log_coeffs)
     return {
-        "value": round((1 << log_coeffs) / secs / 1e6, 4),
+        "value": rate(secs_n),
         "unit": "Mcoeff/s",
-        "cores": 1,
+        "cores": threads,
         "kind": "port",
+        "one_thread": {"value": rate(secs_1), "s": round(secs_1, 3)},
+        "cpu_model": _cpu_model(),
         "sample": f"n={n} (2^{log_coeffs} u64 coefficients, t={t}, s={s}), rate 1/4, initial_k 4: commit + "
-                  f"standalone open by the CPU restatement (oracle/, C kernels driven from Python, "
-                  f"1 thread, PCLMULQDQ); {secs:.2f} s",
+                  f"standalone open by the CPU restatement (oracle/: C kernels with OpenMP and PCLMULQDQ, "
+                  f"protocol driven from Python; not the reference's Rust prover), {threads} threads "
+                  f"{secs_n:.2f} s, 1 thread {secs_1:.2f} s, proofs equal",
+        "reference_published_m5": None if m5 is None else {
+            "hardware": "MacBook Air M5 24 GB (paper Table 11), rate 1/2",
+            "mcoeff_s_1_thread": round((1 << log_coeffs) / m5["ms_1_thread"] / 1e3, 2),
+            "mcoeff_s_10_threads": round((1 << log_coeffs) / m5["ms_10_threads"] / 1e3, 2)},
     }
 
 

@@ -49,21 +49,24 @@ class Transcript:
 
     def __init__(self):
         self.buf = bytearray()
+        self._h = O.Blake3()
 
     def absorb_inner(self, b: bytes) -> None:
-        self.buf += bytes(b)
+        b = bytes(b)
+        self.buf += b
+        self._h.update(b)
 
     def absorb_slice(self, b: bytes) -> None:
         self.absorb_inner(b"\x06" + u64(len(b)) + bytes(b) + b"\x07")
 
     def challenge(self, n: int = 16) -> bytes:
-        out = O.blake3(bytes(self.buf), n)
-        self.buf += b"\x12" + out + b"\x34"
+        out = self._h.digest(n)
+        self.absorb_inner(b"\x12" + out + b"\x34")
         return out
 
     def sampling_bytes(self, n: int) -> bytes:
-        out = O.blake3(bytes(self.buf), n)
-        self.buf += b"bitz/sampling-read/v1" + u64(n) + out
+        out = self._h.digest(n)
+        self.absorb_inner(b"bitz/sampling-read/v1" + u64(n) + out)
         return out
 
 
@@ -624,7 +627,7 @@ class OracleProver:
         claim = 0
         for w in reversed(layers):
             L = len(internal)
-            l, r = w[: 1 << L].copy(), w[1 << L:].copy()
+            l, r = w[: 1 << L], w[1 << L:]  # read-only views: the folds write new arrays
             factor, nxt = 1, []
             ext = list(reversed(internal))
             for rnd, z in enumerate(internal):

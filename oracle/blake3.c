@@ -177,3 +177,105 @@ void or_blake3_parent_cv(const uint8_t left[32], const uint8_t right[32], uint8_
     or_blake3_compress(IV, block, 0, BLOCK_LEN, PARENT, w);
     words_to_bytes(w, 8, out);
 }
+
+/* Incremental hashing (BLAKE3 specification, section 5.1): the chunk being
+ * filled keeps its last block uncompressed (it may be the chunk's final
+ * block), and completed chunks' chaining values sit on a stack merged into
+ * parents whenever the number of completed chunks has a trailing zero bit.
+ * Finalizing reads the state without changing it, so a transcript can keep
+ * absorbing after a challenge. */
+typedef struct {
+    uint32_t cv[8];
+    uint64_t chunk;
+    uint8_t buf[64];
+    uint32_t buf_len;
+    uint32_t blocks;
+    uint32_t depth;
+    uint32_t stack[54][8];
+} b3_state;
+
+size_t or_b3_state_size(void) { return sizeof(b3_state); }
+
+void or_b3_init(void* st) {
+    b3_state* h = (b3_state*)st;
+    memset(h, 0, sizeof *h);
+    memcpy(h->cv, IV, sizeof h->cv);
+}
+
+static void b3_push_chunk(b3_state* h, uint32_t cv[8]) {
+    uint64_t done = h->chunk;
+    while ((done & 1) == 0) {
+        uint8_t block[64];
+        uint32_t w[16];
+        h->depth--;
+        words_to_bytes(h->stack[h->depth], 8, block);
+        words_to_bytes(cv, 8, block + 32);
+        or_blake3_compress(IV, block, 0, BLOCK_LEN, PARENT, w);
+        memcpy(cv, w, 32);
+        done >>= 1;
+    }
+    memcpy(h->stack[h->depth++], cv, 32);
+}
+
+void or_b3_update(void* st, const uint8_t* data, size_t len) {
+    b3_state* h = (b3_state*)st;
+    while (len) {
+        if (h->blocks * BLOCK_LEN + h->buf_len == CHUNK_LEN) {
+            /* the chunk is full and more input follows: close it */
+            node_out o;
+            memcpy(o.cv, h->cv, 32);
+            memcpy(o.block, h->buf, 64);
+            o.block_len = BLOCK_LEN;
+            o.counter = h->chunk;
+            o.flags = CHUNK_END | (h->blocks == 0 ? CHUNK_START : 0);
+            uint32_t cv[8];
+            node_cv(&o, cv);
+            h->chunk++;
+            b3_push_chunk(h, cv);
+            memcpy(h->cv, IV, 32);
+            h->blocks = 0;
+            h->buf_len = 0;
+        }
+        if (h->buf_len == BLOCK_LEN) {
+            uint32_t w[16];
+            or_blake3_compress(h->cv, h->buf, h->chunk, BLOCK_LEN, h->blocks == 0 ? CHUNK_START : 0, w);
+            memcpy(h->cv, w, 32);
+            h->blocks++;
+            h->buf_len = 0;
+        }
+        size_t take = BLOCK_LEN - h->buf_len;
+        if (take > len) take = len;
+        memcpy(h->buf + h->buf_len, data, take);
+        h->buf_len += (uint32_t)take;
+        data += take;
+        len -= take;
+    }
+}
+
+void or_b3_finalize(const void* st, uint8_t* out, size_t out_len) {
+    const b3_state* h = (const b3_state*)st;
+    node_out o;
+    memcpy(o.cv, h->cv, 32);
+    memset(o.block, 0, 64);
+    memcpy(o.block, h->buf, h->buf_len);
+    o.block_len = h->buf_len;
+    o.counter = h->chunk;
+    o.flags = CHUNK_END | (h->blocks == 0 ? CHUNK_START : 0);
+    for (uint32_t i = h->depth; i-- > 0;) {
+        uint32_t cv[8];
+        node_cv(&o, cv);
+        parent_output(h->stack[i], cv, &o);
+    }
+    uint64_t ctr = 0;
+    size_t done = 0;
+    while (done < out_len) {
+        uint32_t w[16];
+        uint8_t bytes[64];
+        or_blake3_compress(o.cv, o.block, ctr, o.block_len, o.flags | ROOT, w);
+        words_to_bytes(w, 16, bytes);
+        size_t take = out_len - done < 64 ? out_len - done : 64;
+        memcpy(out + done, bytes, take);
+        done += take;
+        ctr++;
+    }
+}

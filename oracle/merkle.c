@@ -14,12 +14,14 @@
 
 void or_merkle_tree(const uint8_t* data, uint64_t num_leaves, uint64_t leaf_size,
                     uint8_t* tree) {
+#pragma omp parallel for if (num_leaves > 256)
     for (uint64_t i = 0; i < num_leaves; i++) {
         or_blake3_leaf_cv(data + i * leaf_size, leaf_size, tree + i * 32);
     }
     uint64_t read_start = 0, read_len = num_leaves;
     while (read_len > 1) {
         uint64_t next = read_len >> 1;
+#pragma omp parallel for if (next > 1024)
         for (uint64_t i = 0; i < next; i++) {
             or_blake3_parent_cv(tree + (read_start + 2 * i) * 32,
                                 tree + (read_start + 2 * i + 1) * 32,

@@ -57,13 +57,19 @@ void or_ntt_forward_interleaved(gf128_t* data, int log_d, int log_lanes, int sta
     size_t lanes = (size_t)1 << log_lanes;
     gf128_t* evals = (gf128_t*)malloc(sizeof(gf128_t) * (row_off(ell, ell) + 1));
     or_ntt_evals(ell, evals);
+    gf128_t* tws = (gf128_t*)malloc(sizeof(gf128_t) << (log_d > 0 ? log_d - 1 : 0));
     for (int layer = start_layer; layer < log_d; layer++) {
         size_t num_blocks = (size_t)1 << layer;
         size_t half = (size_t)1 << (log_d - layer - 1);
-        for (size_t block = 0; block < num_blocks; block++) {
-            gf128_t tw = or_ntt_twiddle(ell, evals, layer, block);
+#pragma omp parallel for if (num_blocks > 64)
+        for (size_t block = 0; block < num_blocks; block++) tws[block] = or_ntt_twiddle(ell, evals, layer, block);
+        /* every (block, row) pair is one butterfly column of its own */
+#pragma omp parallel for if (num_blocks * half * lanes > 4096)
+        for (size_t br = 0; br < num_blocks * half; br++) {
+            size_t block = br / half, row = br % half;
+            gf128_t tw = tws[block];
             size_t start = (block << (log_d - layer)) * lanes;
-            for (size_t row = 0; row < half; row++) {
+            {
                 gf128_t* top = data + start + row * lanes;
                 gf128_t* bot = top + half * lanes;
                 for (size_t lane = 0; lane < lanes; lane++) {
@@ -77,6 +83,7 @@ void or_ntt_forward_interleaved(gf128_t* data, int log_d, int log_lanes, int sta
             }
         }
     }
+    free(tws);
     free(evals);
 }
 

@@ -51,6 +51,13 @@ void or_blake3_parent_cv(const uint8_t left[32], const uint8_t right[32], uint8_
 /* keyed-free compression, exposed for device parity tests */
 void or_blake3_compress(const uint32_t cv[8], const uint8_t block[64], uint64_t counter,
                         uint32_t block_len, uint32_t flags, uint32_t out[16]);
+/* incremental hashing: a caller-owned state of or_b3_state_size() bytes;
+ * or_b3_finalize reads the root XOF of everything absorbed so far and leaves
+ * the state unchanged */
+size_t or_b3_state_size(void);
+void or_b3_init(void* st);
+void or_b3_update(void* st, const uint8_t* data, size_t len);
+void or_b3_finalize(const void* st, uint8_t* out, size_t out_len);
 
 /* ---- Additive NTT (LCH novel basis), additive_ntt_f128.rs ---- */
 /* evals table for the standard basis {1, x, ..., x^(ell-1)}:

@@ -39,6 +39,10 @@ def _load():
         "or_blake3_leaf_cv": (None, [P, C.c_size_t, P]),
         "or_blake3_parent_cv": (None, [P, P, P]),
         "or_blake3_compress": (None, [P, P, C.c_uint64, C.c_uint32, C.c_uint32, P]),
+        "or_b3_state_size": (C.c_size_t, []),
+        "or_b3_init": (None, [P]),
+        "or_b3_update": (None, [P, P, C.c_size_t]),
+        "or_b3_finalize": (None, [P, P, C.c_size_t]),
         "or_ntt_evals": (None, [C.c_int, P]),
         "or_ntt_twiddle": (GF, [C.c_int, P, C.c_int, C.c_uint64]),
         "or_ntt_forward_interleaved": (None, [P, C.c_int, C.c_int, C.c_int, C.c_int]),
@@ -105,6 +109,33 @@ def blake3(data: bytes, out_len: int = 32) -> bytes:
     out = np.zeros(out_len, np.uint8)
     lib.or_blake3_hash(_ptr(buf), len(data), _ptr(out), out_len)
     return out.tobytes()
+
+
+class Blake3:
+    """incremental BLAKE3; digest() leaves the state open for more input"""
+
+    def __init__(self):
+        self._st = (C.c_uint8 * lib.or_b3_state_size())()
+        lib.or_b3_init(self._st)
+
+    def update(self, data: bytes) -> None:
+        if data:
+            b = (C.c_uint8 * len(data)).from_buffer_copy(bytes(data))
+            lib.or_b3_update(self._st, b, len(data))
+
+    def digest(self, out_len: int = 32) -> bytes:
+        out = (C.c_uint8 * max(out_len, 1))()
+        lib.or_b3_finalize(self._st, out, out_len)
+        return bytes(out)[:out_len]
+
+
+def set_threads(n: int) -> None:
+    """OpenMP threads of the oracle's heavy loops (results do not depend on it)"""
+    lib.or_set_threads(n)
+
+
+def max_threads() -> int:
+    return int(lib.or_max_threads())
 
 
 def leaf_cv(data: bytes) -> bytes:
@@ -204,10 +235,14 @@ def _sig_open():
         "or_product_level": (None, [P, C.c_uint64, P]),
         "or_gkr_round_sums": (None, [P, P, C.c_uint64, P, C.c_int, P]),
         "or_fold_halves": (None, [P, C.c_uint64, GF]),
+        "or_fold_halves_out": (None, [P, C.c_uint64, GF, P]),
         "or_xi_combined_rows": (None, [P, C.c_int, C.c_int, P, P]),
         "or_fold_rows_eq": (None, [P, C.c_int, C.c_int, P, P]),
         "or_round_polynomial": (None, [P, P, C.c_uint64, P]),
         "or_fold_lsb": (None, [P, C.c_uint64, GF]),
+        "or_fold_lsb_out": (None, [P, C.c_uint64, GF, P]),
+        "or_set_threads": (None, [C.c_int]),
+        "or_max_threads": (C.c_int, []),
         "or_fold_1b_rows": (None, [P, P, C.c_uint64, P]),
         "or_fold_b128_elems": (None, [P, C.c_uint64, P, P]),
         "or_batch_claims": (GF, [P, P]),
@@ -292,10 +327,11 @@ def gkr_round_sums(l, r, eq, send_one: bool) -> tuple[int, int]:
 
 
 def fold_halves(v, rho: int) -> np.ndarray:
-    v = _gfa(v).copy()
+    v = _gfa(v)
     h = v.shape[0] // 2
-    lib.or_fold_halves(_ptr(v), h, gf(rho))
-    return v[:h].copy()
+    out = np.empty((h, 2), np.uint64)
+    lib.or_fold_halves_out(_ptr(v), h, gf(rho), _ptr(out))
+    return out
 
 
 def xi_combined_rows(rows, t, s, w2) -> np.ndarray:
@@ -322,9 +358,10 @@ def round_polynomial(m, w) -> list[int]:
 
 
 def fold_lsb(v, r: int) -> np.ndarray:
-    v = _gfa(v).copy()
-    lib.or_fold_lsb(_ptr(v), v.shape[0], gf(r))
-    return v[: v.shape[0] // 2].copy()
+    v = _gfa(v)
+    out = np.empty((v.shape[0] // 2, 2), np.uint64)
+    lib.or_fold_lsb_out(_ptr(v), v.shape[0], gf(r), _ptr(out))
+    return out
 
 
 def fold_1b_rows(p, suffix) -> np.ndarray:

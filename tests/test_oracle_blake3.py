@@ -51,3 +51,24 @@ def test_leaf_and_parent_are_the_tree_rule():
     # Single chunk: the leaf CV is the root digest minus the ROOT flag, so it
     # differs from the hash; for inputs <= 64 bytes both come from one block.
     assert O.leaf_cv(d[:100]) != O.blake3(d[:100])
+
+
+def test_incremental_hasher_equals_one_shot():
+    """the incremental state (chunk in progress + chaining-value stack) gives
+    the one-shot digest (pinned above by the published vectors) for every
+    length around the block, chunk and stack-merge boundaries, fed in random
+    pieces, with digests taken mid-stream leaving the state usable"""
+    import random
+    rng = random.Random(5)
+    d = _input(40000)
+    for n in (0, 1, 63, 64, 65, 1023, 1024, 1025, 2047, 2048, 2049, 3072, 4096, 4097, 8193,
+              16384, 31745, 40000):
+        h = O.Blake3()
+        pos = 0
+        while pos < n:
+            k = min(n - pos, rng.choice([1, 7, 64, 100, 1024, 3000]))
+            h.update(d[pos:pos + k])
+            pos += k
+            if rng.random() < 0.3:
+                assert h.digest(48) == O.blake3(d[:pos], 48), pos
+        assert h.digest(131) == O.blake3(d[:n], 131), n

@@ -324,3 +324,51 @@ def test_pow_is_smallest_valid_nonce():
             if z != 8:
                 break
         assert lz >= bits
+
+
+def test_openmp_loops_do_not_depend_on_thread_count():
+    """the oracle's parallel loops (the CPU baseline runs them on every host
+    core) give the single-thread results, at sizes above their parallel
+    thresholds"""
+    rs = np.random.default_rng(77)
+    n = 1 << 15
+    t, s = 12, 6
+    rows = rs.integers(0, 2**64, size=(1 << s) << (t - 6), dtype=np.uint64)
+    a = gfa(R(n))
+    b = gfa(R(n))
+    pt = gfa(R(14))
+    w128 = gfa(R(128))
+
+    def run():
+        eq = O.eq_table(pt)
+        return [
+            eq,
+            O.mle_eval(a, gfa(R(15, random.Random(1)))),
+            O.gkr_round_sums(a[: n // 2], b[: n // 2], eq[: n // 4], False),
+            O.gkr_round_sums(a[: n // 2], b[: n // 2], eq[: n // 4], True),
+            O.fold_halves(a, 12345),
+            O.fold_lsb(b, 678),
+            O.product_level(a),
+            O.round_polynomial(a, b),
+            O.xi_combined_rows(rows, t, s, gfa(R(1 << s, random.Random(2)))),
+            O.fold_rows_eq(rows, t, s, O.eq_table(gfa(R(t, random.Random(3))))),
+            O.fold_1b_rows(a, b),
+            O.fold_b128_elems(b, w128),
+            O.lig_round_msg(a, b),
+            O.ntt_forward(a, 11, 4),
+            O.merkle_tree(a.tobytes(), 1 << 10),
+        ]
+
+    before = O.max_threads()
+    try:
+        O.set_threads(1)
+        one = run()
+        O.set_threads(max(before, 4))
+        many = run()
+    finally:
+        O.set_threads(before)
+    for x, y in zip(one, many):
+        if isinstance(x, np.ndarray):
+            assert np.array_equal(x, y)
+        else:
+            assert x == y

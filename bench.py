@@ -35,19 +35,85 @@ HBM_PEAK_GBS = 8000.0
 VALU_PEAK_TOPS = 256 * 64 * 2.4e9 / 1e12
 
 # Per-dispatch hardware counters of the 2^24-coefficient workload, averaged
-# over the dispatches of one proof (profiles/pmc_commit_open_2p24.txt):
-# HBM MiB from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, each scaled by what
-# the counter reads for a 1 GiB device copy (FETCH_SIZE x2.0, WRITE_SIZE
-# x1.0), and SQ_INSTS_VALU (wave instructions; 64 lane operations each).
+# over every dispatch of the profiled process, from the four rocprofv3 runs
+# whose commands head PMC_SOURCE["file"] (made by `bench.py
+# --summarize-profiles`), taken on the code of PMC_SOURCE["commit"]. "mib":
+# HBM MiB per dispatch, FETCH_SIZE and WRITE_SIZE each scaled by what the
+# counter reads for a 1 GiB device copy; "valu_insts": SQ_INSTS_VALU (wave
+# instructions, 64 lane operations each). Keys are the library's launch names.
+PMC_SOURCE = {"file": "profiles/pmc_commit_open_2p24.txt", "commit": "34035dd"}
 PMC = {
     24: {
-        "gkr_fold_round_sums_rowonly": {"mib": 280.37 + 139.98, "valu_insts": 5.101e+07},
-        "gkr_round_sums_rowonly": {"mib": 1024.69 + 0.03, "valu_insts": 2.240e+08},
-        "scale_cols_hi": {"mib": 292.60 + 292.55, "valu_insts": 3.614e+07},
-        "product_level": {"mib": 315.10 + 157.52, "valu_insts": 6.925e+07},
-        "ntt_pass": {"mib": 318.73 + 326.40, "valu_insts": 1.335e+08},
+        "gkr_fold_round_sums_rowonly": {"kernel": "k_gkr_round_sums<FoldSrc, false>",
+                                        "mib": 179.93 + 89.77, "valu_insts": 3.370e+07},
     },
 }
+
+
+def _short_kernel(name: str) -> str:
+    name = name.strip('"')
+    if name.startswith("void "):
+        name = name[5:]
+    name = name.replace("(anonymous namespace)::", "").replace("bitz::", "")
+    depth = 0
+    for i, ch in enumerate(name):
+        if ch == "<":
+            depth += 1
+        elif ch == ">":
+            depth -= 1
+        elif ch == "(" and depth == 0:
+            return name[:i]
+    return name
+
+
+def summarize_profiles(stats_csv, sq_csv, fetch_csv, write_csv, out=sys.stdout):
+    """one line per kernel from rocprofv3 output: the kernel-trace stats
+    (--kernel-trace --stats) and three --pmc runs (SQ counters, FETCH_SIZE,
+    WRITE_SIZE) of the same bench command with --copy-probe; HBM counters are
+    scaled by what they read for the probe's 1 GiB device copy"""
+    import csv
+    from collections import defaultdict
+
+    def counters(path):
+        per = defaultdict(lambda: defaultdict(float))  # (kernel, dispatch) -> counter -> value
+        with open(path, newline="") as f:
+            for row in csv.DictReader(f):
+                key = (_short_kernel(row["Kernel_Name"]), row["Dispatch_Id"])
+                per[key][row["Counter_Name"]] += float(row["Counter_Value"])
+        agg = defaultdict(lambda: defaultdict(list))
+        for (k, _), cs in per.items():
+            for c, v in cs.items():
+                agg[k][c].append(v)
+        return agg
+
+    stats = {}
+    with open(stats_csv, newline="") as f:
+        for row in csv.DictReader(f):
+            stats[_short_kernel(row["Name"])] = (int(row["Calls"]), float(row["AverageNs"]) / 1e3)
+    sq, fe, wr = counters(sq_csv), counters(fetch_csv), counters(write_csv)
+    probe = "__amd_rocclr_copyBuffer"
+    gib_kib = 1 << 20
+    f_probe = max(fe[probe]["FETCH_SIZE"]) if probe in fe else gib_kib
+    w_probe = max(wr[probe]["WRITE_SIZE"]) if probe in wr else gib_kib
+    fs, ws = gib_kib / f_probe, gib_kib / w_probe
+    print(f"copy probe (1 GiB read + 1 GiB write = {gib_kib} KiB each): FETCH_SIZE {f_probe:.0f} KiB, "
+          f"WRITE_SIZE {w_probe:.0f} KiB", file=out)
+    print(f"scale factors: fetch {fs:.4f}, write {ws:.4f}", file=out)
+    print("", file=out)
+    hdr = (f"{'kernel':<58}{'calls':>7}{'avg_us':>10}{'valu_inst/disp':>16}{'valu_util':>11}"
+           f"{'fetch_MiB/disp':>16}{'write_MiB/disp':>16}{'HBM_GB/s':>10}")
+    print(hdr, file=out)
+    mean = lambda xs: sum(xs) / len(xs) if xs else 0.0
+    for k, (calls, avg_us) in sorted(stats.items(), key=lambda kv: -kv[1][0] * kv[1][1]):
+        if k not in sq and k not in fe:
+            continue
+        valu = mean(sq[k].get("SQ_INSTS_VALU", []))
+        util = valu / (avg_us * 1e-6 * 256 * 2.4e9) if avg_us else 0.0
+        fmib = mean(fe[k].get("FETCH_SIZE", [])) * fs / 1024
+        wmib = mean(wr[k].get("WRITE_SIZE", [])) * ws / 1024
+        gbs = (fmib + wmib) * 1048576 / (avg_us * 1e-6) / 1e9 if avg_us else 0.0
+        print(f"{k[:57]:<58}{calls:>7}{avg_us:>10.1f}{valu:>16.4g}{util:>11.3f}{fmib:>16.2f}{wmib:>16.2f}"
+              f"{gbs:>10.1f}", file=out)
 
 
 def parse():
@@ -68,6 +134,9 @@ def parse():
     ap.add_argument("--batch-jobs", type=int, default=64)
     ap.add_argument("--batch-workers", type=int, default=8)
     ap.add_argument("--serve-jobs", type=int, default=96)
+    ap.add_argument("--summarize-profiles", nargs=4, metavar=("STATS", "SQ", "FETCH", "WRITE"),
+                    help="print the per-kernel counter table from rocprofv3 CSVs (kernel stats, SQ, FETCH_SIZE, "
+                         "WRITE_SIZE runs) and exit")
     ap.add_argument("--copy-probe", action="store_true",
                     help="after the timed steps, copy 1 GiB device to device (a kernel of known traffic that "
                          "calibrates the FETCH_SIZE / WRITE_SIZE counters of a profiler run)")
@@ -325,6 +394,9 @@ def host_rows_steps(args, ctx, coeffs, t, s, claimed, proof, steps=3):
 
 def main():
     args = parse()
+    if args.summarize_profiles:
+        summarize_profiles(*args.summarize_profiles)
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}): the commit+open path is "
               f"measured on a single MI355X.", file=sys.stderr)
@@ -394,22 +466,33 @@ def main():
         gbs = byts / launches / avg_s / 1e9
         pmc = PMC.get(args.log_coeffs, {}).get(name, {})
         traffic_mib = pmc.get("mib")
-        # VALU issue rate: measured instructions per dispatch / live duration
+        # VALU issue rate: the profile's instructions per dispatch / this run's
+        # live launch duration (counters need a profiler run of their own)
         tops = pmc["valu_insts"] * 64 / avg_s / 1e12 if "valu_insts" in pmc else None
+        hbm = {"achieved": round(gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4),
+               "source": "algorithmic bytes per launch / live launch duration (HIP events)"}
+        valu = None if tops is None else {
+            "achieved": round(tops, 3), "peak": round(VALU_PEAK_TOPS, 2), "unit": "T int32-op/s",
+            "frac": round(tops / VALU_PEAK_TOPS, 4),
+            "source": f"SQ_INSTS_VALU per dispatch of {pmc['kernel']} from {PMC_SOURCE['file']} "
+                      f"(rocprofv3 at commit {PMC_SOURCE['commit']}) / live launch duration"}
+        # the binding roof is the one the kernel runs closest to
+        top = valu if valu is not None and valu["frac"] > hbm["frac"] else hbm
         roofline = {
             "kernel": name,
-            "bound": "hbm",
-            "achieved": round(gbs, 2),
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": round(gbs / HBM_PEAK_GBS, 5),
+            "bound": "valu" if top is valu else "hbm",
+            "achieved": top["achieved"],
+            "peak": top["peak"],
+            "unit": top["unit"],
+            "frac": top["frac"],
             "traffic": round(traffic_mib * 1048576) if traffic_mib else None,
+            "traffic_source": f"FETCH_SIZE + WRITE_SIZE per dispatch, {PMC_SOURCE['file']} "
+                              f"(commit {PMC_SOURCE['commit']})" if traffic_mib else None,
             "algorithmic_bytes_per_launch": round(byts / launches),
             "avg_launch_us": round(avg_s * 1e6, 2),
             "launches_per_step": launches,
-            "valu": {"achieved": round(tops, 3), "peak": round(VALU_PEAK_TOPS, 2), "unit": "T int32-op/s",
-                     "frac": round(tops / VALU_PEAK_TOPS, 4),
-                     "source": "SQ_INSTS_VALU per dispatch (rocprofv3) / live launch duration"} if tops else None,
+            "hbm": hbm,
+            "valu": valu,
             "time_share": round(tot_ms / ms, 4),
         }
     if args.copy_probe:

@@ -667,6 +667,15 @@ class OracleProver:
         absorb_standalone_claim(tr, q, claimed)
         tag = tr.challenge(16) + tr.challenge(16)
         it = Inner(b"bitz/bitz-opener/standalone/v1", tag)
+        return self.open_claim(it, q, row_w, col_w, claimed, folds, opt, ood_y, r0_nonce)
+
+    def open_claim(self, it: Inner, q: int, row_w, col_w, claimed: int, folds, opt, ood_y: int,
+                   r0_nonce) -> bytes:
+        """BitZProver::prove (src/bitz/pcs.rs) of the claim
+        sum_{c,b} row_w[b] col_w[c] bit(c, b) = claimed (mod q) on the inner
+        transcript `it`, with Round 0's point and value (opt, ood_y) already
+        bound on the outer one; returns the proof bytes."""
+        t, s = self.t, self.s
         g = generator()
         it.public(self.root)
         it.public(u64(t) + u64(s) + g16(q) + g16(g))

@@ -979,6 +979,15 @@ def verify_standalone(root: bytes, t: int, s: int, log_inv_rate: int, initial_k:
     absorb_standalone_claim(tr, q, claimed)
     tag = tr.challenge(16) + tr.challenge(16)
     it = Inner(b"bitz/bitz-opener/standalone/v1", tag, narg)
+    verify_claim(it, root, t, s, cfg, q, row_w, col_w, claimed, opt, ood_y, narg, hints)
+
+
+def verify_claim(it: Inner, root: bytes, t: int, s: int, cfg: dict, q: int, row_w, col_w, claimed: int, opt,
+                 ood_y: int, narg: bytes, hints: bytes) -> None:
+    """BitZVerifier::verify of the claim sum_{c,b} row_w[b] col_w[c]
+    bit(c, b) = claimed (mod q) on the inner transcript `it` (reading the
+    narg string), Round 0's point and value bound on the outer one."""
+    m, mp = t + s, t + s - LOG_PACKING
     g = generator()
     it.public(root)
     it.public(u64(t) + u64(s) + g16(q) + g16(g))
@@ -1022,7 +1031,7 @@ def verify_standalone(root: bytes, t: int, s: int, log_inv_rate: int, initial_k:
     u1 = [O.mul(a ^ 1, e) for a, e in zip(row_images, eq_ints(alpha_b))]
     u2 = eq_ints(alpha_c)
     target = claim ^ 1
-    pcs40 = u64(m) + u64(log_inv_rate) + u64(initial_k) + u64(0) + u64(1)
+    pcs40 = u64(m) + u64(cfg["levels"][0]["rate"]) + u64(cfg["initial_k"]) + u64(0) + u64(1)
     it.public(b"bitz/pcs/bit-inner-product/v2")
     it.public(root)
     it.public(pcs40)

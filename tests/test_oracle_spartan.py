@@ -15,6 +15,8 @@ import random
 
 import pytest
 
+import bitz_protocol as P
+
 import oracle as O
 import spartan_u64 as S
 
@@ -73,3 +75,72 @@ def test_sumchecks_and_bitification_on_a_ragged_batch():
     # and the claim is the MLE of the assignment at r_y minus the constant cell
     z = out["z_ry"]
     assert out["value"] == out["scale"] * z % q
+
+
+def _batch(n, seed):
+    rng = random.Random(seed)
+    xs = [rng.getrandbits(64) for _ in range(n)]
+    ys = [rng.getrandbits(64) for _ in range(n)]
+    xs[1], ys[1] = 2**64 - 1, 2**64 - 1
+    xs[2] = 0
+    return xs, ys
+
+
+@pytest.fixture(scope="module")
+def honest_proof():
+    xs, ys = _batch(3000, 21)
+    pr = S.SpartanU64Prover(xs, ys)
+    tr, out = pr.prove()
+    return pr, tr, S.encode_proof(out)
+
+
+def test_full_proof_verifies(honest_proof):
+    """the CPU verifier replays the same outer transcript and accepts"""
+    pr, tr, proof = honest_proof
+    vt = S.verify_u64(pr.pcs.root, pr.n, 0, proof)
+    assert bytes(vt.buf) == bytes(tr.buf)
+
+
+@pytest.mark.parametrize("where", ["outer", "abc", "inner", "narg", "tail"])
+def test_tampered_proof_rejected(honest_proof, where):
+    pr, _, proof = honest_proof
+    g = pr.lay["gate_vars"]
+    outer_len = 16 * 4 * pr.tau_arity
+    pos = {"outer": 40, "abc": outer_len + 20, "inner": outer_len + 48 + 16 * 3 * (g // 2) + 3,
+           "narg": outer_len + 48 + 16 * 3 * (g + 3) + 8 + 100, "tail": len(proof) - 3}[where]
+    bad = bytearray(proof)
+    bad[pos] ^= 1
+    with pytest.raises(P.VerifyError):
+        S.verify_u64(pr.pcs.root, pr.n, 0, bytes(bad))
+
+
+def test_wrong_statement_rejected(honest_proof):
+    pr, _, proof = honest_proof
+    with pytest.raises(P.VerifyError):
+        S.verify_u64(bytes(32), pr.n, 0, proof)
+    with pytest.raises(P.VerifyError):
+        S.verify_u64(pr.pcs.root, pr.n - 1, 0, proof)
+
+
+def test_wrong_product_rejected():
+    """a committed z_hi off by one in one gate (the reference's own
+    rejection test, u64_bitz.rs:272-310): the rows no longer satisfy
+    x y = lo + 2^64 hi and the proof is rejected"""
+    xs, ys = _batch(3000, 22)
+    lo = [(a * b) & (2**64 - 1) for a, b in zip(xs, ys)]
+    hi = [(a * b) >> 64 for a, b in zip(xs, ys)]
+    hi[3] ^= 1
+    pr = S.SpartanU64Prover(xs, ys, products=(lo, hi))
+    _, out = pr.prove()
+    with pytest.raises(P.VerifyError):
+        S.verify_u64(pr.pcs.root, pr.n, 0, S.encode_proof(out))
+
+
+def test_eq_prefix_sum_brute_force():
+    rng = random.Random(4)
+    q = (1 << 61) - 1
+    a = [rng.randrange(q) for _ in range(6)]
+    b = [rng.randrange(q) for _ in range(6)]
+    ea, eb = S.eq_le(a, q), S.eq_le(b, q)
+    for live in (0, 1, 5, 17, 32, 63, 64):
+        assert S.eq_prefix_sum(a, b, live, q) == sum(x * y for x, y in zip(ea[:live], eb[:live])) % q

@@ -13,6 +13,8 @@ LIB_PATH = os.path.join(_PKG_DIR, "libbitz_hip.so")
 HEADER_PATH = os.path.join(os.path.dirname(_PKG_DIR), "include", "bitz_hip.h")
 
 BITZ_OK = 0
+BITZ_ERR_ARG = -4
+BITZ_ERR_VERIFY = -5
 ERR_NAMES = {-1: "SHAPE", -2: "ALLOC", -3: "DEVICE", -4: "ARG", -5: "VERIFY"}
 
 
@@ -52,6 +54,8 @@ SIGNATURES = {
     "bitz_ctx_autotune": (_I, [_P, _U32, _U32, _U32, _U32, _P]),
     "bitz_mul_layout": (_I, [_U32, _U64, C.c_int32, _P]),
     "bitz_mul_bit_rows_dev": (_I, [_P, _U32, _U64, C.c_int32, _P, _P, _P, _P, _P, _U64]),
+    "bitz_prove_mul_u64": (_I, [_P, _P, _P, _P, _P, _U64, C.c_int32, _P, _P, _P, _U64, _P]),
+    "bitz_verify_mul_u64": (_I, [_P, _P, _U64, C.c_int32, _P, _U64, _P]),
     "bitz_gf_mul_dev": (_I, [_P, _P, _P, _P, _U64]),
     "bitz_gf_mul_const_dev": (_I, [_P, _P, _P, _P, _U64]),
     "bitz_blake3_leaves_dev": (_I, [_P, _P, _U64, _U64, _P]),

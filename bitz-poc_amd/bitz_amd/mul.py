@@ -102,3 +102,42 @@ class MulWitness:
         rows = torch.empty(self.layout.rows_words, dtype=torch.int64, device=self._limbs[0].device)
         self.write_bitz_bit_rows(rows)
         return rows
+
+
+def prove_u64(ctx, x, y, split_shift: int = 0, lo=None, hi=None):
+    """protocol::commit + protocol::prove of the u64 multiplication relation
+    (src/piop/spartan/protocol/mod.rs:1117-1170): x, y (and optionally the
+    supplied product limbs lo, hi) are contiguous int64 cuda tensors of n
+    values. Returns (root, outer transcript state digest, proof bytes)."""
+    if (lo is None) != (hi is None):
+        raise ValueError("supply both product limbs or neither")
+    n = x.shape[0]
+    for t in (x, y) + ((lo, hi) if lo is not None else ()):
+        if not t.is_cuda or not t.is_contiguous() or t.shape[0] != n or t.element_size() * t.numel() != 8 * n:
+            raise ValueError(f"operand tensors must be contiguous cuda tensors of {n} u64")
+
+    def p(t):
+        return C.c_void_p(t.data_ptr()) if t is not None else None
+
+    root = (C.c_uint8 * 32)()
+    state = (C.c_uint8 * 32)()
+    plen = C.c_uint64(0)
+    lib = _lib.load()
+    rc = lib.bitz_prove_mul_u64(ctx.ptr, p(x), p(y), p(lo), p(hi), n, split_shift, root, state, None, 0,
+                                C.byref(plen))
+    if rc != _lib.BITZ_ERR_ARG or plen.value == 0:
+        _lib.check(rc, "bitz_prove_mul_u64")
+    buf = (C.c_uint8 * plen.value)()
+    _lib.call("bitz_prove_mul_u64", ctx.ptr, p(x), p(y), p(lo), p(hi), n, split_shift, root, state, buf, plen.value,
+              C.byref(plen))
+    return bytes(root), bytes(state), bytes(buf)[:plen.value]
+
+
+def verify_u64(ctx, root: bytes, n: int, proof: bytes, split_shift: int = 0) -> bytes:
+    """protocol::verify of a u64 multiplication proof; raises BitzError when
+    rejected, returns the outer transcript state digest when accepted."""
+    state = (C.c_uint8 * 32)()
+    pb = (C.c_uint8 * max(len(proof), 1)).from_buffer_copy(proof or b"\0")
+    _lib.call("bitz_verify_mul_u64", ctx.ptr, (C.c_uint8 * 32).from_buffer_copy(root), n, split_shift, pb,
+              len(proof), state)
+    return bytes(state)

@@ -191,6 +191,9 @@ void absorb_standalone_claim(Transcript& tr, u128 q, u128 claimed);
 // *err set when the search is exhausted or the interval is invalid
 bool sample_prime_context(Transcript& tr, u128 lo, u128 hi, uint32_t security_bits, u128* out, std::string* err);
 bool sample_prime_with_bits(Transcript& tr, int bits, u128* out, std::string* err);
+// sample_below: uniform below bound from whole u64 sampling reads, at most
+// max_draws rejections; false when exhausted
+bool sample_below(Transcript& tr, u128 bound, uint64_t max_draws, u128* out);
 // a 256-bit draw reduced mod q (two 16-byte challenges, low first)
 u128 sample_residue(Transcript& tr, u128 q);
 

@@ -209,6 +209,27 @@ int bit_length(u128 v) {
     return n;
 }
 
+// strong_round (sampling.rs:247-263): base^odd, then up to twos - 1 squarings
+bool strong_round(const ModN& m, u128 base, u128 odd, int twos) {
+    const u128 n = m.n, one = m.to_mont(1), minus_one = m.to_mont(n - 1);
+    u128 b = m.to_mont(base), x = one;
+    for (int i = bit_length(odd) - 1; i >= 0; i--) {
+        x = m.mont(x, x);
+        if ((odd >> i) & 1) x = m.mont(x, b);
+    }
+    if (x == one || x == minus_one) return true;
+    for (int i = 1; i < twos; i++) {
+        x = m.mont(x, x);
+        if (x == minus_one) return true;
+    }
+    return false;
+}
+
+}  // namespace
+
+// sample_prime_context (src/prime_sampling.rs:66-111) and
+// sample_prime_public (vendor/field/src/prime/sampling.rs:290-345). The
+// policy's rounds and rejection budget are budgets() (:33-56).
 // sample_below (vendor/field/src/prime/sampling.rs:110-138): whole
 // little-endian u64 words from the sampling stream, masked to the bound's
 // width, rejected at or above it
@@ -239,27 +260,6 @@ bool sample_below(Transcript& tr, u128 bound, uint64_t max_draws, u128* out) {
     return false;
 }
 
-// strong_round (sampling.rs:247-263): base^odd, then up to twos - 1 squarings
-bool strong_round(const ModN& m, u128 base, u128 odd, int twos) {
-    const u128 n = m.n, one = m.to_mont(1), minus_one = m.to_mont(n - 1);
-    u128 b = m.to_mont(base), x = one;
-    for (int i = bit_length(odd) - 1; i >= 0; i--) {
-        x = m.mont(x, x);
-        if ((odd >> i) & 1) x = m.mont(x, b);
-    }
-    if (x == one || x == minus_one) return true;
-    for (int i = 1; i < twos; i++) {
-        x = m.mont(x, x);
-        if (x == minus_one) return true;
-    }
-    return false;
-}
-
-}  // namespace
-
-// sample_prime_context (src/prime_sampling.rs:66-111) and
-// sample_prime_public (vendor/field/src/prime/sampling.rs:290-345). The
-// policy's rounds and rejection budget are budgets() (:33-56).
 bool sample_prime_context(Transcript& tr, u128 lo, u128 hi, uint32_t security_bits, u128* out, std::string* err) {
     if (lo > hi) {
         *err = "prime interval is empty";

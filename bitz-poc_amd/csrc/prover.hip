@@ -19,6 +19,7 @@
 #include "host_proto.h"
 #include "modq.h"
 #include "open_impl.h"
+#include "spartan_impl.h"
 
 namespace bitz {
 namespace {
@@ -899,6 +900,9 @@ struct Prover {
     }
 
     int run(const uint8_t* claimed_in, uint8_t* claimed_out, std::vector<uint8_t>* proof_out);
+    int open_claim(InnerTranscript& it, PhaseClock& phase, u128 q, DevBuf& colw, u128 claimed,
+                   const std::vector<uint8_t>& fold_bytes, const std::vector<G>& ood_point, G ood_y,
+                   bool has_r0_nonce, uint64_t r0_nonce, std::vector<uint8_t>* proof_out);
 };
 
 // Ligerito challenger over the inner transcript (src/bitz/pcs.rs:1062-1175)
@@ -1063,9 +1067,26 @@ int Prover::run(const uint8_t* claimed_in, uint8_t* claimed_out, std::vector<uin
     outer.challenge(tag, 16);
     outer.challenge(tag + 16, 16);
 
+    InnerTranscript it("bitz/bitz-opener/standalone/v1", tag, 32);
+    return open_claim(it, phase, q, colw, claimed, fold_bytes, ood_point, ood_y, r0bits != 0, r0_nonce, proof_out);
+}
+
+// BitZProver::prove (src/bitz/pcs.rs) of the claim
+// sum_{c,b} roww[b] colw[c] bit(c, b) = claimed (mod q) on the inner
+// transcript `it`: roww (member) and colw hold the canonical weights on the
+// device, fold_bytes the column folds of roww; Round 0's point and value are
+// already bound on the outer transcript.
+int Prover::open_claim(InnerTranscript& it, PhaseClock& phase, u128 q, DevBuf& colw, u128 claimed,
+                       const std::vector<uint8_t>& fold_bytes, const std::vector<G>& ood_point, G ood_y,
+                       bool has_r0_nonce, uint64_t r0_nonce, std::vector<uint8_t>* proof_out) {
+    int rc;
+    const uint64_t nrows = 1ull << t, ncols = 1ull << s;
+    const uint32_t r0bits = has_r0_nonce ? 1u : 0u;
+    uint8_t gb[16], yb[16];
+    gput(g, gb);
+    gput(ood_y, yb);
     phase.mark("fold_round");
     // ---------------- BitZ opening on the inner transcript ----------------
-    InnerTranscript it("bitz/bitz-opener/standalone/v1", tag, 32);
     it.public_message(h->root, 32);
     {
         uint8_t frame[48];
@@ -1747,6 +1768,10 @@ struct Verifier {
     }
 };
 
+int verify_claim_impl(bitz_ctx* ctx, InnerReader& it, const uint8_t root[32], int t, int s, const LigConfig& cfg,
+                      u128 q, DevBuf& roww, DevBuf& colw, u128 claimed, const std::vector<G>& opt, G ood_y,
+                      uint64_t narg_len, const uint8_t* hints, uint64_t hints_len);
+
 int verify_standalone_impl(bitz_ctx* ctx, const uint8_t root[32], int t, int s, int r0, int k0, u128 claimed,
                            const uint8_t* proof, size_t plen) {
     int rc;
@@ -1826,8 +1851,34 @@ int verify_standalone_impl(bitz_ctx* ctx, const uint8_t root[32], int t, int s, 
     uint8_t tagb[32];
     outer.challenge(tagb, 16);
     outer.challenge(tagb + 16, 16);
-    // ---- inner transcript: instance, fold round ----
+    const uint64_t nrows = 1ull << t, ncols = 1ull << s;
+    DevBuf roww, colw;
+    if (!roww.alloc(nrows * 16) || !colw.alloc(ncols * 16)) return BITZ_ERR_ALLOC;
+    if ((rc = eq_table_mod_q_dev(ctx, r1, q, roww.ptr))) return rc;
+    if ((rc = eq_table_mod_q_dev(ctx, r2, q, colw.ptr))) return rc;
     InnerReader it("bitz/bitz-opener/standalone/v1", tagb, 32, narg, narg_len);
+    return verify_claim_impl(ctx, it, root, t, s, cfg, q, roww, colw, claimed, opt, ood_y, narg_len, hints,
+                             hints_len);
+}
+
+// BitZVerifier::verify (src/bitz/mod.rs:218-255) of the claim
+// sum_{c,b} roww[b] colw[c] bit(c, b) = claimed (mod q), the canonical
+// weights on the device, on the inner transcript `it` (reading the narg
+// string of narg_len bytes); Round 0's point and value are bound on the
+// outer transcript.
+int verify_claim_impl(bitz_ctx* ctx, InnerReader& it, const uint8_t root[32], int t, int s, const LigConfig& cfg,
+                      u128 q, DevBuf& roww, DevBuf& colw, u128 claimed, const std::vector<G>& opt, G ood_y,
+                      uint64_t narg_len, const uint8_t* hints, uint64_t hints_len) {
+    int rc;
+    const int m = t + s, mp = m - 7;
+    const int r0 = cfg.levels[0].log_inv_rate, k0 = cfg.initial_k;
+    static G gen = smallest_generator();
+    const G g = gen;
+    Verifier V{ctx};
+    uint8_t gb[16], yb[16];
+    gput(g, gb);
+    gput(ood_y, yb);
+    // ---- inner transcript: instance, fold round ----
     it.public_message(root, 32);
     {
         uint8_t frame[48];
@@ -1839,11 +1890,8 @@ int verify_standalone_impl(bitz_ctx* ctx, const uint8_t root[32], int t, int s, 
         it.public_message(frame, 48);
     }
     const uint64_t nrows = 1ull << t, ncols = 1ull << s;
-    DevBuf roww, colw, folds, img, tmp, u1, u2;
-    if (!roww.alloc(nrows * 16) || !colw.alloc(ncols * 16) || !folds.alloc(ncols * 16) || !img.alloc(nrows * 16))
-        return BITZ_ERR_ALLOC;
-    if ((rc = eq_table_mod_q_dev(ctx, r1, q, roww.ptr))) return rc;
-    if ((rc = eq_table_mod_q_dev(ctx, r2, q, colw.ptr))) return rc;
+    DevBuf folds, img, tmp, u1, u2;
+    if (!folds.alloc(ncols * 16) || !img.alloc(nrows * 16)) return BITZ_ERR_ALLOC;
     G claimed_g;
     {
         uint8_t b[16];
@@ -2302,6 +2350,86 @@ int make_prover(bitz_ctx* ctx, const bitz_commit_handle* h, int target_bits, Pro
 }
 
 }  // namespace
+
+// ---- the BitZ opening of a linear claim, for composed proofs ----
+int round0_prove(bitz_ctx* ctx, const bitz_commit_handle* h, Transcript& outer, Round0Claim* r0) {
+    Prover p;
+    int rc = make_prover(ctx, h, 100, &p);
+    if (rc) return rc;
+    const uint32_t bits = (uint32_t)p.cfg.round0_grinding_bits;
+    ood_round_header(outer, p.mp, bits);
+    r0->has_nonce = bits != 0;
+    r0->nonce = 0;
+    if (bits) {
+        uint8_t seed[32];
+        grinding_seed(outer, "bitz/core/ood-round-grinding/v1", 0, bits, seed);
+        if ((rc = pow_find(ctx, seed, seed + 16, bits, &r0->nonce))) return rc;
+        absorb_grinding_nonce(outer, r0->nonce);
+    }
+    uint8_t zb[16];
+    outer.challenge(zb, 16);
+    r0->point.resize(p.mp);
+    G cur = gfrom(zb);
+    for (int i = 0; i < p.mp; i++) {
+        r0->point[i] = cur;
+        cur = hgf_mul(cur, cur);
+    }
+    if ((rc = p.mle_packed(r0->point, &r0->y))) return rc;
+    uint8_t tagged[17];
+    tagged[0] = 0x50;
+    gput(r0->y, tagged + 1);
+    outer.absorb_slice(tagged, 17);
+    return BITZ_OK;
+}
+
+int open_linear_claim(bitz_ctx* ctx, const bitz_commit_handle* h, const char* session, const uint8_t tag[32],
+                      const uint8_t* lead, size_t lead_len, u128 q, const void* roww, const void* colw_in,
+                      u128 claimed, const Round0Claim& r0, std::vector<uint8_t>* proof) {
+    Prover p;
+    int rc = make_prover(ctx, h, 100, &p);
+    if (rc) return rc;
+    BITZ_REQUIRE(claimed < q, BITZ_ERR_ARG, "BitZ claim: value not reduced");
+    const uint64_t nrows = 1ull << p.t, ncols = 1ull << p.s;
+    DevBuf colw;
+    if (!p.roww.alloc(nrows * 16) || !colw.alloc(ncols * 16) || !p.folds.alloc(ncols * 16)) return BITZ_ERR_ALLOC;
+    BITZ_HIP(hipMemcpyAsync(p.roww.ptr, roww, nrows * 16, hipMemcpyDeviceToDevice, ctx->stream));
+    BITZ_HIP(hipMemcpyAsync(colw.ptr, colw_in, ncols * 16, hipMemcpyDeviceToDevice, ctx->stream));
+    std::vector<uint8_t> fold_bytes(ncols * 16);
+    if ((rc = fold_columns(ctx, (const uint64_t*)h->packed.ptr, p.t, p.s, p.roww.ptr, p.folds.ptr))) return rc;
+    if ((rc = p.download(p.folds.ptr, fold_bytes.data(), fold_bytes.size()))) return rc;
+    InnerTranscript it(session, tag, 32);
+    if (lead_len) it.public_message(lead, lead_len);
+    PhaseClock phase(ctx);
+    return p.open_claim(it, phase, q, colw, claimed, fold_bytes, r0.point, r0.y, r0.has_nonce, r0.nonce, proof);
+}
+
+int verify_linear_claim(bitz_ctx* ctx, const uint8_t root[32], int t, int s, int r0, int k0, const char* session,
+                        const uint8_t tag[32], const uint8_t* lead, size_t lead_len, u128 q, const void* roww,
+                        const void* colw, u128 claimed, const std::vector<G>& opt, G ood_y, const uint8_t* narg,
+                        uint64_t narg_len, const uint8_t* hints, uint64_t hints_len) {
+    LigConfig cfg;
+    std::string err = custom_johnson_config(t + s, r0, k0, 100, &cfg);
+    BITZ_REQUIRE(err.empty(), BITZ_ERR_SHAPE, "Ligerito configuration: " + err);
+    BITZ_REQUIRE(t >= 7 && s >= 1 && cfg.levels.size() >= 2 && t + s - 7 <= kMaxTensorN, BITZ_ERR_SHAPE,
+                 "verify: unsupported shape");
+    if (claimed >= q) {
+        set_error("verify: claim not reduced");
+        return BITZ_ERR_VERIFY;
+    }
+    const uint64_t nrows = 1ull << t, ncols = 1ull << s;
+    DevBuf rw, cw;
+    if (!rw.alloc(nrows * 16) || !cw.alloc(ncols * 16)) return BITZ_ERR_ALLOC;
+    BITZ_HIP(hipMemcpyAsync(rw.ptr, roww, nrows * 16, hipMemcpyDeviceToDevice, ctx->stream));
+    BITZ_HIP(hipMemcpyAsync(cw.ptr, colw, ncols * 16, hipMemcpyDeviceToDevice, ctx->stream));
+    InnerReader it(session, tag, 32, narg, narg_len);
+    if (lead_len) it.public_message(lead, lead_len);
+    return verify_claim_impl(ctx, it, root, t, s, cfg, q, rw, cw, claimed, opt, ood_y, narg_len, hints, hints_len);
+}
+
+bool round0_pow_valid(const uint8_t seed[32], uint64_t nonce, uint32_t bits) {
+    return pow_valid(seed, seed + 16, nonce, bits);
+}
+
 }  // namespace bitz
 
 extern "C" int bitz_standalone_evaluation(bitz_ctx* ctx, bitz_commit_handle* h, uint8_t claimed_out[16]) {

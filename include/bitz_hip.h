@@ -382,6 +382,34 @@ int bitz_mul_bit_rows_dev(bitz_ctx* ctx, uint32_t word_bits, uint64_t multiplica
                           const void* x, const void* y, const void* lo, const void* hi, void* rows,
                           uint64_t rows_words);
 
+/* ---- the u64 multiplication proof (Spartan PIOP over F_q, discharged by BitZ) ---- */
+/* protocol::commit + protocol::prove for MulLayout<u64> at the lambda = 100
+ * profile with the custom:1:4 Johnson ladder
+ * (src/piop/spartan/protocol/mod.rs:1117-1170 -> bitz_opener.rs:439-494,
+ * prove_piop mod.rs:1410-1506): x, y are device arrays of n u64 operands;
+ * the products lo, hi are computed on the device, or taken from lo, hi when
+ * given (a supplied witness, MulWitness::from_rows; a wrong one yields a
+ * proof the verifier rejects). Writes the commitment root, the outer
+ * Blake3Transcript's state digest after the proof (the value the
+ * reference's tests/transcript_state_pins.rs:163-172 pins) and the proof:
+ * the Spartan messages as 16-byte field elements (4 per outer round, Az Bz
+ * Cz, 3 per inner round; SpartanProof::payload_elements, mod.rs:940-957)
+ * followed by BitZOpeningProof::to_bytes (bitz_opener.rs:297-323).
+ * *proof_len is set even when capacity is too small (BITZ_ERR_ARG).
+ * Shapes: committed grids of 2^20..2^35 bits with t <= 24. */
+int bitz_prove_mul_u64(bitz_ctx* ctx, const void* x, const void* y, const void* lo, const void* hi, uint64_t n,
+                       int32_t split_shift, uint8_t root[32], uint8_t state[32], uint8_t* proof, uint64_t capacity,
+                       uint64_t* proof_len);
+/* protocol::verify for MulLayout<u64> (mod.rs:1174-1207 -> bitz_opener.rs
+ * verify): replays the outer transcript from (root, n, split_shift) and the
+ * proof, checks every sumcheck round, the outer terminal
+ * eq(tau, r_x)(Az Bz - Cz), the inner terminal against the public weight at
+ * r_y, and the BitZ opening of the bitified claim. BITZ_OK when accepted,
+ * BITZ_ERR_VERIFY otherwise; state (nullable) receives the outer transcript
+ * digest. */
+int bitz_verify_mul_u64(bitz_ctx* ctx, const uint8_t root[32], uint64_t n, int32_t split_shift,
+                        const uint8_t* proof, uint64_t proof_len, uint8_t state[32]);
+
 /* The device transcript (Blake3Transcript, src/transcript/mod.rs:56-96) checked
  * against the host's: a host transcript absorbs `prefix` (absorb_slice), its
  * state goes to the device, which then runs n x (absorb_slice(msg_i of

@@ -132,6 +132,8 @@ def parse():
     ap.add_argument("--device-transcript", action="store_true",
                     help="run the GKR rounds on a device copy of the transcript (no host round trip per round)")
     ap.add_argument("--batch-jobs", type=int, default=64)
+    ap.add_argument("--mul-log", type=int, default=20, help="log2 multiplications of the u64 proof workload")
+    ap.add_argument("--no-mul", action="store_true", help="skip the u64 multiplication proof workload")
     ap.add_argument("--batch-workers", type=int, default=8)
     ap.add_argument("--serve-jobs", type=int, default=96)
     ap.add_argument("--summarize-profiles", nargs=4, metavar=("STATS", "SQ", "FETCH", "WRITE"),
@@ -252,6 +254,62 @@ def verify_all(ctx, jobs, specs):
         except RuntimeError:
             ok = False
     return ok, (time.perf_counter() - t0) * 1e3 / max(len(jobs), 1)
+
+
+# The reference's published u64 multiplication SNARK (full prover including
+# the commitment, rate 1/2, paper Table 17, MacBook Air M5): multiplications
+# -> (1 thread, 10 threads) ms. 2^20 itself is not published.
+M5_MUL_U64 = {19: (347.0, 106.0), 21: (1297.0, 369.0)}
+
+
+def mul_u64_steps(args, ctx):
+    """The u64 multiplication workload (2^mul_log uniform u64 pairs from a
+    seed, resident in HBM): commit of the 256-slot bit grid + the Spartan
+    PIOP + the BitZ opening (bitz_prove_mul_u64), timed over args.steps
+    proofs after one warmup; the last proof verified by the library."""
+    import numpy as np
+    import torch
+
+    from bitz_amd.device import kernel_stats, set_timing
+    from bitz_amd.mul import MulLayout, prove_u64, verify_u64
+    n = 1 << args.mul_log
+    g = torch.Generator(device="cuda").manual_seed(20)
+    x = torch.randint(-(2**62), 2**62, (n,), dtype=torch.int64, device="cuda", generator=g)
+    y = torch.randint(-(2**62), 2**62, (n,), dtype=torch.int64, device="cuda", generator=g)
+    x ^= torch.randint(0, 4, (n,), dtype=torch.int64, device="cuda", generator=g) << 62
+    y ^= torch.randint(0, 4, (n,), dtype=torch.int64, device="cuda", generator=g) << 62
+    lay = MulLayout.new(64, n)
+    prove_u64(ctx, x, y)
+    torch.cuda.synchronize()
+    set_timing(ctx, True)
+    prove_u64(ctx, x, y)
+    torch.cuda.synchronize()
+    prof = kernel_stats(ctx)
+    set_timing(ctx, False)
+    phases = {k[6:]: round(v[1], 3) for k, v in prof.items() if k.startswith("phase:")}
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        root, state, proof = prove_u64(ctx, x, y)
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    tv = time.perf_counter()
+    verify_u64(ctx, root, n, proof)
+    verify_ms = (time.perf_counter() - tv) * 1e3
+    ref = {f"2^{k}": {"ms_1_thread": a, "ms_10_threads": b} for k, (a, b) in M5_MUL_U64.items()}
+    return {
+        "workload": f"u64 multiplication proof: 2^{args.mul_log} uniform u64 pairs (x y = lo + 2^64 hi), "
+                    f"commit of the 256-slot grid (t={lay.t}, s={lay.s}) + Spartan PIOP over a runtime prime + "
+                    f"BitZ opening, lambda 100, custom:1:4 (rate 1/2)",
+        "ms_per_proof": round(ms, 3),
+        "proofs_per_s": round(1e3 / ms, 2),
+        "mmul_per_s": round(n / ms / 1e3, 3),
+        "steps": args.steps,
+        "proof_bytes": len(proof),
+        "verify_ms": round(verify_ms, 3),
+        "proof_verified": True,
+        "host_phases_ms": phases,
+        "reference_published_m5_rate_half": ref,
+    }
 
 
 def batched_and_serving(args, ctx):
@@ -509,6 +567,7 @@ def main():
     verify_standalone(ctx, root0, t, s, 2, 4, claimed, proof)
     verify_ms = round((time.perf_counter() - tv) * 1e3, 3)
     host_rows_rate = None if args.no_host_rows else host_rows_steps(args, ctx, coeffs, t, s, claimed, proof)
+    mul_u64 = None if args.no_mul else mul_u64_steps(args, ctx)
     batched = serving = None
     if not args.no_batched:
         batched, serving = batched_and_serving(args, ctx)
@@ -536,6 +595,7 @@ def main():
         "cpu_baseline": cpu,
         "batched": batched,
         "serving": serving,
+        "mul_u64": mul_u64,
         "kernels": {k: {"launches": int(v[0]), "ms": round(v[1], 3)} for k, v in
                     sorted(profile.items(), key=lambda kv: -kv[1][1])},
         "host_phases_ms": phases,

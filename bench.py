@@ -358,10 +358,10 @@ def batched_and_serving(args, ctx):
     batched["all_verified"] = ok
     batched["verify_ms_per_proof"] = round(vms, 3)
     # serving: Poisson arrivals at 70% of the batched rate, half of them
-    # commit+open jobs over host rows, half u64-multiplication jobs (2^12..2^22
-    # multiplications, their witness grid packed on the device); every proof
-    # is verified by the worker that made it, inside the measured latency
-    from bitz_amd.serve import MulJob
+    # commit+open jobs over host rows, half full u64-multiplication proofs
+    # (2^12..2^22 multiplications: commit, Spartan PIOP, BitZ opening); every
+    # proof is verified by the worker that made it, inside the measured latency
+    from bitz_amd.serve import MulProofJob
     srng = np.random.default_rng(65)
     sspecs = _job_specs(srng, args.serve_jobs // 2, 14, 22)
 
@@ -373,7 +373,7 @@ def batched_and_serving(args, ctx):
             ln = int(mrng.integers(12, 23))
             x = mrng.integers(0, 2**64, size=1 << ln, dtype=np.uint64)
             y = mrng.integers(0, 2**64, size=1 << ln, dtype=np.uint64)
-            out.append(MulJob(x, y, 4, int(mrng.choice([1, 2, 3])), verify=True))
+            out.append(MulProofJob(x, y, verify=True))
         return out
 
     jobs = mixed_jobs()
@@ -406,8 +406,9 @@ def batched_and_serving(args, ctx):
     lat = np.array([j.latency_ms for j in jobs])
     lat_mul = np.array([j.latency_ms for j in jobs[1::2]])
     serving = {
-        "workload": f"{len(jobs)} Poisson arrivals: commit+open of host rows (2^14..2^22 coefficients) and of "
-                    "u64-multiplication witness grids (2^12..2^22 multiplications), 50/50, mixed rates, "
+        "workload": f"{len(jobs)} Poisson arrivals: commit+open of host rows (2^14..2^22 coefficients, mixed "
+                    "rates) and full u64-multiplication proofs (2^12..2^22 multiplications: commit, Spartan PIOP, "
+                    "BitZ opening), 50/50, "
                     "admitted by device-memory footprint (64 GiB budget), each proof verified by its worker",
         "capacity_proofs_per_s": round(capacity, 2),
         "arrival_rate_per_s": round(rate, 2),

@@ -29,7 +29,7 @@ class _JobStruct(C.Structure):
         ("verified", C.c_int32), ("reserved", C.c_int32),
     ]
 
-JOB_ROWS, JOB_MUL_U64 = 0, 1
+JOB_ROWS, JOB_MUL_U64, JOB_MUL_U64_PROOF = 0, 1, 2
 
 
 class Job:
@@ -122,6 +122,18 @@ class MulJob(Job):
         j.proof = C.cast(self._proof, C.c_void_p)
         j.proof_capacity = proof_capacity
         self.s_ = j
+
+
+class MulProofJob(MulJob):
+    """The full u64 multiplication proof of x[i] * y[i] (bitz_prove_mul_u64:
+    commit, Spartan PIOP, BitZ opening; protocol::prove for MulLayout<u64>,
+    src/piop/spartan/protocol/mod.rs:1138-1170), verified by its worker
+    (bitz_verify_mul_u64) when verify is set. `claimed` is the first half of
+    the outer transcript state digest."""
+
+    def __init__(self, x: np.ndarray, y: np.ndarray, proof_capacity: int = 1 << 21, verify: bool = False):
+        super().__init__(x, y, 4, 1, None, proof_capacity, verify)
+        self.s_.kind = JOB_MUL_U64_PROOF
 
 
 class Server:

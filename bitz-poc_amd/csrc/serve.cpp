@@ -86,8 +86,32 @@ int mul_rows(bitz_ctx* ctx, const bitz_job* j, void** rows) {
     return rc;
 }
 
+// the u64 multiplication proof of one job (bitz_prove_mul_u64), its outer
+// transcript state in claimed[0..16) (the first half of the digest)
+int run_mul_proof_job(bitz_ctx* ctx, bitz_job* j) {
+    void *xd = nullptr, *yd = nullptr;
+    const uint64_t bytes = 8 * j->mul_n;
+    uint8_t state[32];
+    int rc = bitz_dev_alloc(ctx, bytes, &xd);
+    if (!rc) rc = bitz_dev_alloc(ctx, bytes, &yd);
+    if (!rc) rc = bitz_dev_upload(ctx, xd, j->mul_x, bytes);
+    if (!rc) rc = bitz_dev_upload(ctx, yd, j->mul_y, bytes);
+    if (!rc)
+        rc = bitz_prove_mul_u64(ctx, xd, yd, nullptr, nullptr, j->mul_n, 0, j->root, state, j->proof,
+                                j->proof_capacity, &j->proof_len);
+    if (xd) bitz_dev_free(ctx, xd);
+    if (yd) bitz_dev_free(ctx, yd);
+    if (rc) return rc;
+    memcpy(j->claimed, state, 16);
+    if (!j->verify) return rc;
+    rc = bitz_verify_mul_u64(ctx, j->root, j->mul_n, 0, j->proof, j->proof_len, nullptr);
+    j->verified = rc == BITZ_OK ? 1 : 0;
+    return rc;
+}
+
 // commit + (evaluation) + proof (+ verification) of one job on one context
 int run_job(bitz_ctx* ctx, bitz_job* j) {
+    if (j->kind == BITZ_JOB_MUL_U64_PROOF) return run_mul_proof_job(ctx, j);
     bitz_commit_handle* h = nullptr;
     int rc;
     if (j->kind == BITZ_JOB_MUL_U64) {
@@ -191,7 +215,7 @@ extern "C" int bitz_server_submit(bitz_server* srv, bitz_job* job) {
         bitz::set_error("bitz_server_submit: null argument");
         return BITZ_ERR_ARG;
     }
-    if (job->kind == BITZ_JOB_MUL_U64) {
+    if (job->kind == BITZ_JOB_MUL_U64 || job->kind == BITZ_JOB_MUL_U64_PROOF) {
         uint32_t lay[4];
         if (!job->mul_x || !job->mul_y) {
             bitz::set_error("bitz_server_submit: null multiplication operands");

@@ -338,8 +338,12 @@ typedef struct bitz_job {
      * mul_n u64 multiplications (MulWitness::bitz_bit_rows,
      * src/piop/spartan/mul.rs:268-330) of the host arrays mul_x, mul_y,
      * packed on the device; rows is unused and t, s are set at submission
-     * (MulLayout). */
-    uint32_t kind;           /* BITZ_JOB_ROWS (0) or BITZ_JOB_MUL_U64 (1) */
+     * (MulLayout). Kind BITZ_JOB_MUL_U64_PROOF: the full u64 multiplication
+     * proof of mul_x, mul_y (bitz_prove_mul_u64 with the products computed
+     * on the device, custom:1:4; log_batch and log_inv_rate must be 4 and 1),
+     * verified by bitz_verify_mul_u64 when verify is set; claimed receives
+     * the first 16 bytes of the outer transcript state digest. */
+    uint32_t kind;           /* BITZ_JOB_ROWS (0), BITZ_JOB_MUL_U64 (1) or BITZ_JOB_MUL_U64_PROOF (2) */
     uint32_t verify;         /* nonzero: the worker verifies the proof (bitz_verify_standalone) */
     const uint64_t* mul_x;
     const uint64_t* mul_y;
@@ -349,6 +353,7 @@ typedef struct bitz_job {
 } bitz_job;
 #define BITZ_JOB_ROWS 0
 #define BITZ_JOB_MUL_U64 1
+#define BITZ_JOB_MUL_U64_PROOF 2
 typedef struct bitz_server bitz_server;
 /* device bytes one job holds while it runs (0 for an invalid shape) */
 uint64_t bitz_job_footprint(uint32_t t, uint32_t s, uint32_t log_batch, uint32_t log_inv_rate);

@@ -134,3 +134,41 @@ def test_multiplication_jobs_in_a_verified_mix():
     x, y, r = ops[1]
     t, s = mjobs[1].t, mjobs[1].s
     P.verify_standalone(mjobs[1].root, t, s, r, 4, mjobs[1].claimed, mjobs[1].proof)
+
+
+def test_multiplication_proof_jobs_in_a_verified_mix():
+    """full u64 multiplication proofs (Spartan + BitZ) as server jobs beside
+    row jobs, each verified by its worker: root, transcript state and proof
+    bytes equal bitz_prove_mul_u64 run alone, and one is accepted by the CPU
+    verifier"""
+    import spartan_u64 as S
+    import torch
+
+    from bitz_amd.mul import prove_u64
+    from bitz_amd.serve import Job, MulProofJob, Server
+    from gpu_util import ctx
+    rng = np.random.default_rng(31)
+    ops = [(rng.integers(0, 2**64, size=n, dtype=np.uint64), rng.integers(0, 2**64, size=n, dtype=np.uint64))
+           for n in (1 << 12, 5000, 1 << 14)]
+    log("serve: u64 multiplication proofs alone")
+    direct = []
+    for x, y in ops:
+        root, state, proof = prove_u64(ctx(), torch.from_numpy(x.view(np.int64)).cuda(),
+                                       torch.from_numpy(y.view(np.int64)).cuda())
+        direct.append((root, int.from_bytes(state[:16], "little"), proof))
+    log("serve: u64 multiplication proofs and row jobs on a 4-worker server")
+    mjobs = [MulProofJob(x, y, verify=True) for x, y in ops]
+    rjobs = [Job(rng.integers(0, 2**64, size=(1 << s) << (t - 6), dtype=np.uint64), t, s, 4, r, verify=True)
+             for t, s, r in SHAPES[:3]]
+    with Server(workers=4) as srv:
+        for j in [mjobs[0], rjobs[0], mjobs[1], rjobs[1], mjobs[2], rjobs[2]]:
+            srv.submit(j)
+        srv.wait_all()
+    for j, d in zip(mjobs, direct):
+        j.check()
+        assert j.verified
+        assert (j.root, j.claimed, j.proof) == d
+    for j in rjobs:
+        j.check()
+        assert j.verified
+    S.verify_u64(mjobs[1].root, 5000, 0, mjobs[1].proof)

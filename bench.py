@@ -312,6 +312,54 @@ def mul_u64_steps(args, ctx):
     }
 
 
+def commit_batch_steps(args, ctx):
+    """64 commitments of mixed sizes and rates from device-resident rows:
+    one at a time (each commit's kernels launched in turn on the context's
+    stream) against one graph launch (bitz_commit_batch, replaying the kept
+    graph); roots compared item by item."""
+    import numpy as np
+    import torch
+
+    from bitz_amd.device import CommitBatch
+    rng = np.random.default_rng(64)
+    specs = []
+    for _ in range(args.batch_jobs):
+        lc = int(rng.integers(14, 23))
+        _, t, s = shape_for(lc)
+        specs.append((t, s, 4, int(rng.choice([1, 2, 3]))))
+    batch = CommitBatch(ctx, specs)
+    for p in batch.packed:
+        p.copy_(torch.randint(-(2**62), 2**62, p.shape, dtype=torch.int64, device="cuda"))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    roots = batch.commit()  # capture + instantiate + launch
+    first_ms = (time.perf_counter() - t0) * 1e3
+    reps = max(args.steps, 3)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        roots = batch.commit()
+    graph_ms = (time.perf_counter() - t0) * 1e3 / reps
+    singles = None
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        singles = [ctx.commit(batch.packed[i], t + s - 7, k, r, batch.codeword[i], batch.tree[i])[0]
+                   for i, (t, s, k, r) in enumerate(specs)]
+    torch.cuda.synchronize()
+    single_ms = (time.perf_counter() - t0) * 1e3 / reps
+    coeffs = sum(1 << (t + s - 6) for t, s, _, _ in specs)
+    return {
+        "workload": f"{len(specs)} commitments from device rows, 2^14..2^22 u64 coefficients, rates 1/2, 1/4, 1/8",
+        "one_launch_ms": round(graph_ms, 3),
+        "one_launch_commits_per_s": round(len(specs) / graph_ms * 1e3, 1),
+        "one_launch_mcoeff_per_s": round(coeffs / graph_ms / 1e3, 1),
+        "first_launch_ms_with_capture": round(first_ms, 3),
+        "one_at_a_time_ms": round(single_ms, 3),
+        "one_at_a_time_commits_per_s": round(len(specs) / single_ms * 1e3, 1),
+        "speedup": round(single_ms / graph_ms, 3),
+        "roots_equal": roots == singles,
+    }
+
+
 def batched_and_serving(args, ctx):
     """Batched commit+open of independent polynomials of mixed sizes and
     rates on one server (bitz_server: per-worker contexts, jobs overlapping on
@@ -569,6 +617,7 @@ def main():
     verify_ms = round((time.perf_counter() - tv) * 1e3, 3)
     host_rows_rate = None if args.no_host_rows else host_rows_steps(args, ctx, coeffs, t, s, claimed, proof)
     mul_u64 = None if args.no_mul else mul_u64_steps(args, ctx)
+    commit_batch = None if args.no_batched else commit_batch_steps(args, ctx)
     batched = serving = None
     if not args.no_batched:
         batched, serving = batched_and_serving(args, ctx)
@@ -597,6 +646,7 @@ def main():
         "batched": batched,
         "serving": serving,
         "mul_u64": mul_u64,
+        "commit_batch": commit_batch,
         "kernels": {k: {"launches": int(v[0]), "ms": round(v[1], 3)} for k, v in
                     sorted(profile.items(), key=lambda kv: -kv[1][1])},
         "host_phases_ms": phases,

@@ -54,6 +54,7 @@ SIGNATURES = {
     "bitz_ctx_autotune": (_I, [_P, _U32, _U32, _U32, _U32, _P]),
     "bitz_mul_layout": (_I, [_U32, _U64, C.c_int32, _P]),
     "bitz_mul_bit_rows_dev": (_I, [_P, _U32, _U64, C.c_int32, _P, _P, _P, _P, _P, _U64]),
+    "bitz_commit_batch": (_I, [_P, _U32, _P]),
     "bitz_prove_mul_u64": (_I, [_P, _P, _P, _P, _P, _U64, C.c_int32, _P, _P, _P, _U64, _P]),
     "bitz_verify_mul_u64": (_I, [_P, _P, _U64, C.c_int32, _P, _U64, _P]),
     "bitz_gf_mul_dev": (_I, [_P, _P, _P, _P, _U64]),

@@ -130,6 +130,41 @@ class Context:
         return (bytes(root) if want_root else None), codeword, tree
 
 
+class CommitItem(C.Structure):
+    """bitz_commit_item (include/bitz_hip.h)"""
+    _fields_ = [("packed", C.c_void_p), ("t", C.c_uint32), ("s", C.c_uint32), ("log_batch", C.c_uint32),
+                ("log_inv_rate", C.c_uint32), ("codeword", C.c_void_p), ("tree", C.c_void_p),
+                ("root", C.c_uint8 * 32)]
+
+
+class CommitBatch:
+    """Resident buffers of a batch of commitments, committed in one graph
+    launch (bitz_commit_batch). specs: (t, s, log_batch, log_inv_rate) per
+    item; `packed[i]` is item i's device message (2^(t+s-7) Gf128: the rows),
+    filled by the caller; commit() returns the roots. The same object
+    committed again replays the captured graph."""
+
+    def __init__(self, ctx: "Context", specs):
+        torch = ctx.torch
+        self.ctx = ctx
+        self.specs = list(specs)
+        self.packed, self.codeword, self.tree = [], [], []
+        self.items = (CommitItem * len(self.specs))()
+        for i, (t, s, k, r) in enumerate(self.specs):
+            lm = t + s - 7
+            leaves = 1 << (lm - k + r)
+            self.packed.append(ctx.empty_gf(1 << lm))
+            self.codeword.append(ctx.empty_gf(1 << (lm + r)))
+            self.tree.append(torch.empty((2 * leaves - 1, 32), dtype=torch.uint8, device="cuda"))
+            it = self.items[i]
+            it.packed, it.codeword, it.tree = _ptr(self.packed[i]), _ptr(self.codeword[i]), _ptr(self.tree[i])
+            it.t, it.s, it.log_batch, it.log_inv_rate = t, s, k, r
+
+    def commit(self) -> list[bytes]:
+        _lib.call("bitz_commit_batch", self.ctx._ptr, len(self.specs), C.cast(self.items, C.c_void_p))
+        return [bytes(it.root) for it in self.items]
+
+
 # ----------------------------------------------------------------- opening
 def _b16(x: int):
     return (C.c_uint8 * 16).from_buffer_copy(int(x).to_bytes(16, "little"))

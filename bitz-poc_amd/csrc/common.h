@@ -218,6 +218,11 @@ struct bitz_ctx {
     // compressions in ~20 us, no faster than the host round trip)
     bool device_fs = false;
     bitz::Tune tune;  // launch shapes and thresholds (defaults or bitz_ctx_autotune)
+    // bitz_commit_batch: side streams of the captured graphs' parallel
+    // branches (default priority) and the instantiated graphs of recent
+    // batches, keyed by their items (pointers and shapes)
+    std::vector<hipStream_t> branch_streams;
+    std::vector<std::pair<uint64_t, hipGraphExec_t>> batch_graphs;
 };
 
 // A committed matrix kept on the device: the bit rows (= packed message

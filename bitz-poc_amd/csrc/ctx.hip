@@ -143,6 +143,11 @@ extern "C" void bitz_ctx_destroy(bitz_ctx* ctx) {
     }
     bitz::resolve_timings(ctx);
     for (auto e : ctx->event_pool) (void)hipEventDestroy(e);
+    for (auto& g : ctx->batch_graphs) (void)hipGraphExecDestroy(g.second);
+    for (auto st : ctx->branch_streams) {
+        (void)hipStreamSynchronize(st);
+        (void)hipStreamDestroy(st);
+    }
     delete ctx;
 }
 

@@ -144,6 +144,8 @@ int gkr_fs_round(bitz_ctx* ctx, unsigned nb, void* state, void* factor, const vo
 // Device transcript check: n x (absorb_slice(msg_i), 16-byte challenge) on a
 // device copy of `state`; challenges to out (device, n x 16 B).
 int transcript_challenges(bitz_ctx* ctx, void* state, const void* msgs, uint32_t n, uint32_t msg_len, void* out);
+// the device twiddle table of an NTT domain of dimension ell (cached in ctx)
+int ensure_twiddles(bitz_ctx* ctx, int ell, const uint4** out);
 int commit_device(bitz_ctx* ctx, const void* packed, int log_msg_len, int log_batch, int log_inv_rate,
                   void* codeword, void* tree);
 

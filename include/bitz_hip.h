@@ -272,6 +272,26 @@ int bitz_commit(bitz_ctx* ctx, const uint64_t* rows, uint32_t t, uint32_t s, uin
 int bitz_commit_device_rows(bitz_ctx* ctx, const void* rows, uint32_t t, uint32_t s, uint32_t log_batch,
                             uint32_t log_inv_rate, uint8_t root[32], bitz_commit_handle** out);
 void bitz_commit_free(bitz_commit_handle* h);
+/* Batched commitments in one launch. No counterpart in the reference (its
+ * commitments run one at a time, commit_rs_flock_from_rows,
+ * src/ligerito_flock.rs:198-296); each item's codeword and tree equal
+ * bitz_commit_device_rows's for the same rows. The items' kernel sequences
+ * (NTT passes, the LDS pass with the leaf hashes, the upper Merkle levels)
+ * are captured into one HIP graph, items on up to 8 parallel branches, and
+ * the graph is launched once; the instantiated graph is kept per item list
+ * (pointers and shapes, the last 4 lists), so the same batch of resident
+ * buffers replays it. packed: the device rows (2^(t+s-7) Gf128, the layout
+ * bitz_commit takes); codeword: 16 << (t+s-7+log_inv_rate) bytes; tree:
+ * (2 leaves - 1) * 32 bytes, leaves = 2^(t+s-7-log_batch+log_inv_rate).
+ * Synchronous: roots are written when it returns. */
+typedef struct bitz_commit_item {
+    const void* packed;
+    uint32_t t, s, log_batch, log_inv_rate;
+    void* codeword;
+    void* tree;
+    uint8_t root[32]; /* out */
+} bitz_commit_item;
+int bitz_commit_batch(bitz_ctx* ctx, uint32_t n, bitz_commit_item* items);
 /* device pointers held by the handle (valid until bitz_commit_free) */
 const void* bitz_commit_packed(const bitz_commit_handle* h);
 const void* bitz_commit_codeword(const bitz_commit_handle* h);

@@ -172,3 +172,52 @@ def test_multiplication_proof_jobs_in_a_verified_mix():
         j.check()
         assert j.verified
     S.verify_u64(mjobs[1].root, 5000, 0, mjobs[1].proof)
+
+
+def test_poisson_arrivals_continuous_batching():
+    """continuous batching under Poisson arrivals: 24 jobs (row commit+open
+    jobs and full u64 multiplication proofs, alternating) submitted at
+    exponential inter-arrival times while earlier jobs run; every job is
+    verified by its worker, admitted jobs overlap in time, and every result
+    equals the same job run alone"""
+    import time
+
+    from bitz_amd.serve import Job, MulProofJob, Server
+    rng = np.random.default_rng(41)
+    specs = []
+    for i in range(24):
+        if i % 2 == 0:
+            t, s, r = SHAPES[int(rng.integers(0, len(SHAPES)))]
+            specs.append(("rows", rng.integers(0, 2**64, size=(1 << s) << (t - 6), dtype=np.uint64), t, s, r))
+        else:
+            n = int(rng.choice([1 << 12, 5000, 1 << 13]))
+            specs.append(("mul", rng.integers(0, 2**64, size=n, dtype=np.uint64),
+                          rng.integers(0, 2**64, size=n, dtype=np.uint64)))
+
+    def make(sp, verify):
+        if sp[0] == "rows":
+            return Job(sp[1], sp[2], sp[3], 4, sp[4], verify=verify)
+        return MulProofJob(sp[1], sp[2], verify=verify)
+
+    log("serve: 24 jobs alone")
+    alone = [make(sp, False) for sp in specs]
+    with Server(workers=1) as srv:
+        for j in alone:
+            srv.submit(j)
+            srv.wait_all()
+            j.check()
+    log("serve: 24 Poisson arrivals (mean gap 3 ms) on a 4-worker server")
+    gaps = rng.exponential(0.003, size=len(specs))
+    live = [make(sp, True) for sp in specs]
+    with Server(workers=4) as srv:
+        for j, gap in zip(live, gaps):
+            time.sleep(float(gap))
+            srv.submit(j)
+        srv.wait_all()
+    for a, j in zip(alone, live):
+        j.check()
+        assert j.verified
+        assert (j.root, j.claimed, j.proof) == (a.root, a.claimed, a.proof)
+        assert j.s_.start_ms >= j.s_.submit_ms and j.s_.end_ms >= j.s_.start_ms
+    spans = sorted((j.s_.start_ms, j.s_.end_ms) for j in live)
+    assert any(spans[i + 1][0] < spans[i][1] for i in range(len(spans) - 1))

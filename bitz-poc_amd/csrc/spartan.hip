@@ -303,6 +303,13 @@ unsigned blocks_for(uint64_t n) {
     const uint64_t b = (n + kT - 1) / kT;
     return (unsigned)std::max<uint64_t>(1, std::min<uint64_t>(b, 2048));
 }
+// sumcheck rounds: at most 256 blocks, so the partials are at most 16 KiB
+unsigned round_blocks(uint64_t pairs) {
+    const uint64_t b = (pairs + kT - 1) / kT;
+    return (unsigned)std::max<uint64_t>(1, std::min<uint64_t>(b, 256));
+}
+// rounds over at most this many pairs run on the host
+constexpr uint64_t kHostPairs = 128;
 
 // ------------------------------------------------------------- host F_q
 struct Fq {
@@ -643,8 +650,10 @@ struct Eqs {
 
 // sums of nb blocks' K partials (Montgomery) -> canonical values
 int partial_sums(bitz_ctx* ctx, const Fq& F, const DevBuf& part, unsigned nb, int K, u128* out) {
-    std::vector<u128> h((size_t)nb * K);
-    BITZ_HIP(hipMemcpyAsync(h.data(), part.ptr, h.size() * 16, hipMemcpyDeviceToHost, ctx->stream));
+    const size_t bytes = (size_t)nb * K * 16;
+    if (int rc = ctx->scratch.ensure_stage(bytes)) return rc;
+    const u128* h = (const u128*)ctx->scratch.stage;
+    BITZ_HIP(hipMemcpyAsync(ctx->scratch.stage, part.ptr, bytes, hipMemcpyDeviceToHost, ctx->stream));
     BITZ_HIP(hipStreamSynchronize(ctx->stream));
     for (int k = 0; k < K; k++) {
         u128 acc = 0;
@@ -652,6 +661,29 @@ int partial_sums(bitz_ctx* ctx, const Fq& F, const DevBuf& part, unsigned nb, in
         out[k] = F.from_mont(acc);
     }
     return BITZ_OK;
+}
+
+// device tables (Montgomery) of n entries each, down to the host
+int download_tables(bitz_ctx* ctx, u128* const* src, int k, uint64_t n, std::vector<u128>* out) {
+    const size_t bytes = (size_t)k * n * 16;
+    if (int rc = ctx->scratch.ensure_stage(bytes)) return rc;
+    for (int i = 0; i < k; i++)
+        BITZ_HIP(hipMemcpyAsync((uint8_t*)ctx->scratch.stage + (size_t)i * n * 16, src[i], n * 16,
+                                hipMemcpyDeviceToHost, ctx->stream));
+    BITZ_HIP(hipStreamSynchronize(ctx->stream));
+    const u128* h = (const u128*)ctx->scratch.stage;
+    for (int i = 0; i < k; i++) out[i].assign(h + (size_t)i * n, h + (size_t)(i + 1) * n);
+    return BITZ_OK;
+}
+
+// the host's fold of Montgomery tables by r (Montgomery)
+void host_fold(const Fq& F, std::vector<u128>& v, u128 rm) {
+    const size_t n = v.size() / 2;
+    for (size_t i = 0; i < n; i++) {
+        const u128 v0 = v[2 * i], v1 = v[2 * i + 1];
+        v[i] = addmod(v0, F.m.mont(rm, addmod(v1, F.q - v0, F.q)), F.q);
+    }
+    v.resize(n);
 }
 
 u128 horner(const Fq& F, const u128* c, int n, u128 x) {
@@ -809,16 +841,6 @@ void cubic_coeffs(const Fq& F, const u128 e[4], u128 c[4]) {
     c[1] = F.mul(F.sub(F.add(F.sub(F.mul(2, e[3]), F.mul(9, e[2])), F.mul(18, e[1])), F.mul(11, e[0])), F.inv6);
 }
 
-// v0 + r (v1 - v0) of a device pair (Montgomery) -> canonical
-int fold_last_pair(bitz_ctx* ctx, const Fq& F, const void* pair, u128 r, u128* out) {
-    u128 pv[2];
-    BITZ_HIP(hipMemcpyAsync(pv, pair, 32, hipMemcpyDeviceToHost, ctx->stream));
-    BITZ_HIP(hipStreamSynchronize(ctx->stream));
-    const u128 v0 = F.from_mont(pv[0]), v1 = F.from_mont(pv[1]);
-    *out = F.add(v0, F.mul(r, F.sub(v1, v0)));
-    return BITZ_OK;
-}
-
 // The three selector rounds of the inner sumcheck on the eight blocks:
 // W_b = coef_b E(r_gate) (zero for blocks 0, 5, 6, 7), Z_b(r_gate).
 template <class Msg>
@@ -900,10 +922,12 @@ int prove_mul_u64(bitz_ctx* ctx, const uint64_t* x, const uint64_t* y, const uin
     std::vector<uint8_t> payload;
     std::vector<u128> rx;
     u128 claim = 0, rprev = 0;
-    u128 *src[4] = {A, B, C, E}, *dst[4], *last[4] = {A, B, C, E};
-    for (int j = 0; j < g; j++) {
+    u128 *src[4] = {A, B, C, E}, *dst[4];
+    int j = 0;
+    for (; j < g; j++) {
         const uint64_t pairs = len >> (j + 1);
-        const unsigned nb = blocks_for(pairs);
+        if (pairs <= kHostPairs) break;
+        const unsigned nb = round_blocks(pairs);
         const u128 rm = F.to_mont(rprev);
         set_ptrs((j & 1) ? T1 : T2, dst, 4);
         hipLaunchKernelGGL(k_sp_outer, dim3(nb), dim3(kT), 0, ctx->stream, F.args, src[0], src[1], src[2], src[3],
@@ -911,7 +935,7 @@ int prove_mul_u64(bitz_ctx* ctx, const uint64_t* x, const uint64_t* y, const uin
                            (u128*)part.ptr);
         BITZ_LAUNCH_CHECK();
         if (j > 0)
-            for (int k = 0; k < 4; k++) src[k] = last[k] = dst[k];
+            for (int k = 0; k < 4; k++) src[k] = dst[k];
         u128 e[4], c[4];
         if ((rc = partial_sums(ctx, F, part, nb, 4, e))) return rc;
         cubic_coeffs(F, e, c);
@@ -923,9 +947,40 @@ int prove_mul_u64(bitz_ctx* ctx, const uint64_t* x, const uint64_t* y, const uin
         rx.push_back(r);
         rprev = r;
     }
+    // the last rounds on the host: the tables (not yet folded by r_(j-1))
+    // come down once
+    std::vector<u128> ht[4];
+    if ((rc = download_tables(ctx, src, 4, len >> (j > 0 ? j - 1 : 0), ht))) return rc;
+    if (j > 0)
+        for (auto& v : ht) host_fold(F, v, F.to_mont(rprev));
+    for (; j < g; j++) {
+        u128 e[4] = {0, 0, 0, 0}, c[4];
+        for (size_t i = 0; i + 1 < ht[0].size(); i += 2) {
+            u128 d[4], v[4];
+            for (int k = 0; k < 4; k++) {
+                v[k] = ht[k][i];
+                d[k] = addmod(ht[k][i + 1], F.q - ht[k][i], F.q);
+            }
+            for (int x = 0; x < 4; x++) {
+                if (x)
+                    for (int k = 0; k < 4; k++) v[k] = addmod(v[k], d[k], F.q);
+                const u128 ab = F.m.mont(v[0], v[1]);
+                e[x] = addmod(e[x], F.m.mont(v[3], addmod(ab, F.q - v[2], F.q)), F.q);
+            }
+        }
+        for (auto& x : e) x = F.from_mont(x);
+        cubic_coeffs(F, e, c);
+        absorb_elements(tr, c, 4);
+        put_elems(&payload, c, 4);
+        u128 r;
+        if (!squeeze_or_fail(tr, P.q, &r)) return BITZ_ERR_DEVICE;
+        claim = horner(F, c, 4, r);
+        rx.push_back(r);
+        rprev = r;
+        for (auto& v : ht) host_fold(F, v, F.to_mont(r));
+    }
     u128 fin[4];
-    for (int k = 0; k < 4; k++)
-        if ((rc = fold_last_pair(ctx, F, last[k], rprev, &fin[k]))) return rc;
+    for (int k = 0; k < 4; k++) fin[k] = F.from_mont(ht[k][0]);
     // an honest witness satisfies the terminal check; a dishonest one (caller
     // limbs) is proved anyway, for the verifier to reject
     if (lo == nullptr && claim != F.mul(fin[3], F.sub(F.mul(fin[0], fin[1]), fin[2]))) {
@@ -953,17 +1008,19 @@ int prove_mul_u64(bitz_ctx* ctx, const uint64_t* x, const uint64_t* y, const uin
     BITZ_LAUNCH_CHECK();
     u128 iclaim = F.add(F.add(fin[0], F.mul(rho, fin[1])), F.mul(rho2, fin[2]));
     std::vector<u128> ry;
-    u128 *isrc[2] = {E, U}, *idst[2], *ilast[2] = {E, U};
-    for (int j = 0; j < g; j++) {
+    u128 *isrc[2] = {E, U}, *idst[2];
+    j = 0;
+    for (; j < g; j++) {
         const uint64_t pairs = len >> (j + 1);
-        const unsigned nb = blocks_for(pairs);
+        if (pairs <= kHostPairs) break;
+        const unsigned nb = round_blocks(pairs);
         const u128 rm = F.to_mont(rprev);
         set_ptrs((j & 1) ? T1 : T2, idst, 2);
         hipLaunchKernelGGL(k_sp_inner, dim3(nb), dim3(kT), 0, ctx->stream, F.args, isrc[0], isrc[1], pairs,
                            j > 0 ? 1 : 0, (uint64_t)rm, (uint64_t)(rm >> 64), idst[0], idst[1], (u128*)part.ptr);
         BITZ_LAUNCH_CHECK();
         if (j > 0)
-            for (int k = 0; k < 2; k++) isrc[k] = ilast[k] = idst[k];
+            for (int k = 0; k < 2; k++) isrc[k] = idst[k];
         u128 cs[2];
         if ((rc = partial_sums(ctx, F, part, nb, 2, cs))) return rc;
         u128 c[3] = {cs[0], F.sub(F.sub(iclaim, F.add(cs[0], cs[0])), cs[1]), cs[1]};
@@ -975,8 +1032,30 @@ int prove_mul_u64(bitz_ctx* ctx, const uint64_t* x, const uint64_t* y, const uin
         ry.push_back(r);
         rprev = r;
     }
-    u128 e_rg;
-    if ((rc = fold_last_pair(ctx, F, ilast[0], rprev, &e_rg))) return rc;
+    std::vector<u128> hi2[2];
+    if ((rc = download_tables(ctx, isrc, 2, len >> (j > 0 ? j - 1 : 0), hi2))) return rc;
+    if (j > 0)
+        for (auto& v : hi2) host_fold(F, v, F.to_mont(rprev));
+    for (; j < g; j++) {
+        u128 c0 = 0, c2 = 0;
+        for (size_t i = 0; i + 1 < hi2[0].size(); i += 2) {
+            const u128 e0 = hi2[0][i], u0 = hi2[1][i];
+            c0 = addmod(c0, F.m.mont(e0, u0), F.q);
+            c2 = addmod(c2, F.m.mont(addmod(hi2[0][i + 1], F.q - e0, F.q), addmod(hi2[1][i + 1], F.q - u0, F.q)), F.q);
+        }
+        c0 = F.from_mont(c0);
+        c2 = F.from_mont(c2);
+        u128 c[3] = {c0, F.sub(F.sub(iclaim, F.add(c0, c0)), c2), c2};
+        absorb_elements(tr, c, 3);
+        put_elems(&payload, c, 3);
+        u128 r;
+        if (!squeeze_or_fail(tr, P.q, &r)) return BITZ_ERR_DEVICE;
+        iclaim = horner(F, c, 3, r);
+        ry.push_back(r);
+        rprev = r;
+        for (auto& v : hi2) host_fold(F, v, F.to_mont(r));
+    }
+    const u128 e_rg = F.from_mont(hi2[0][0]);
     // Z_b(r_gate) for the value blocks; the constant block is 1 at gate 0
     u128 z[8] = {1, 0, 0, 0, 0, 0, 0, 0}, w[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     for (u128 c : ry) z[0] = F.mul(z[0], F.sub(1, c));

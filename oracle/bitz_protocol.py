@@ -617,9 +617,12 @@ class OracleProver:
         return sum(c * (f % q) for c, f in zip(col_w, folds)) % q
 
     # ---- GKR (dense, gpgkr_prove) ----
-    def _gkr(self, it: Inner, zeta: list[int], row_images: list[int]):
-        t, s = self.t, self.s
-        leaves = O.forest_leaves(self.rows, t, s, gfa(row_images))
+    def _gkr(self, it: Inner, zeta: list[int], row_images: list[int], rows=None, t=None, s=None):
+        """over the committed grid, or over (rows, t, s) for a virtual
+        opening's derived grid"""
+        if rows is None:
+            rows, t, s = self.rows, self.t, self.s
+        leaves = O.forest_leaves(rows, t, s, gfa(row_images))
         layers = [leaves]
         while layers[-1].shape[0] > (1 << s) * 2:
             layers.append(O.product_level(layers[-1]))
@@ -676,31 +679,45 @@ class OracleProver:
         transcript `it`, with Round 0's point and value (opt, ood_y) already
         bound on the outer one; returns the proof bytes."""
         t, s = self.t, self.s
-        g = generator()
         it.public(self.root)
-        it.public(u64(t) + u64(s) + g16(q) + g16(g))
+        it.public(u64(t) + u64(s) + g16(q) + g16(generator()))
         it.public(claim_encoding(row_w, col_w, claimed))
+        u1, u2, target = self.reduce_claim(it, row_w, folds, self.rows, t, s)
+        return self.prove_lin(it, u1, u2, target, self.rows, t, s, True, opt, ood_y, r0_nonce)
+
+    def reduce_claim(self, it: Inner, row_w, folds, rows, t: int, s: int):
+        """send_fold and the GKR reduction (src/bitz/fold.rs:237-263,
+        reduce.rs:28-114) over a grid of bits (t, s): returns the factored
+        inner-product claim (u1, u2, target)"""
         for f in folds:
             it.prover(g16(f))
         zeta = [it.gf() for _ in range(s)]
-        row_images = O.gf_to_ints(O.pow_many(g, gfa(row_w)))
-        point, leaf_claim = self._gkr(it, zeta, row_images)
+        row_images = O.gf_to_ints(O.pow_many(generator(), gfa(row_w)))
+        point, leaf_claim = self._gkr(it, zeta, row_images, rows, t, s)
         alpha_c, alpha_b = point[:s], point[s:]
         eqb = eq_ints(alpha_b)
         u1 = [O.mul(a ^ 1, e) for a, e in zip(row_images, eqb)]
         u2 = eq_ints(alpha_c)
-        target = leaf_claim ^ 1
+        return u1, u2, leaf_claim ^ 1
+
+    def prove_lin(self, it: Inner, u1, u2, target: int, rows, t: int, s: int, bind: bool, opt, ood_y: int,
+                  r0_nonce) -> bytes:
+        """Pcs::prove_lin_rows (src/bitz/pcs.rs:337-377) of the inner-product
+        claim sum u1[b] u2[c] bit(c, b) = target over the committed bits
+        presented as the grid (rows, t, s): the statement (when `bind`), the
+        inner sumcheck, the ring switch and Ligerito; returns the proof bytes"""
         pcs40 = u64(self.m) + u64(self.r0) + u64(self.k0) + u64(0) + u64(1)
-        it.public(b"bitz/pcs/bit-inner-product/v2")
-        it.public(self.root)
-        it.public(pcs40)
-        it.public(claim_encoding(u1, u2, target))
+        if bind:
+            it.public(b"bitz/pcs/bit-inner-product/v2")
+            it.public(self.root)
+            it.public(pcs40)
+            it.public(claim_encoding(u1, u2, target))
         it.public(b"bitz/pcs/inner-product-sumcheck/v1")
-        comb = O.xi_combined_rows(self.rows, t, s, gfa(u2))
+        comb = O.xi_combined_rows(rows, t, s, gfa(u2))
         sc_point = []
         _, wts, target = self._sumcheck(it, comb, gfa(u1), target, sc_point)
         row_scalar = O.gf_to_ints(wts)[0]
-        folded = O.fold_rows_eq(self.rows, t, s, O.eq_table(gfa(sc_point)))
+        folded = O.fold_rows_eq(rows, t, s, O.eq_table(gfa(sc_point)))
         cols = gfa([O.mul(row_scalar, c) for c in u2])
         fin, cw_, target = self._sumcheck(it, folded, cols, target, sc_point)
         evaluation = O.gf_to_ints(fin)[0]
@@ -987,12 +1004,17 @@ def verify_claim(it: Inner, root: bytes, t: int, s: int, cfg: dict, q: int, row_
     """BitZVerifier::verify of the claim sum_{c,b} row_w[b] col_w[c]
     bit(c, b) = claimed (mod q) on the inner transcript `it` (reading the
     narg string), Round 0's point and value bound on the outer one."""
-    m, mp = t + s, t + s - LOG_PACKING
-    g = generator()
     it.public(root)
-    it.public(u64(t) + u64(s) + g16(q) + g16(g))
+    it.public(u64(t) + u64(s) + g16(q) + g16(generator()))
     it.public(claim_encoding(row_w, col_w, claimed))
-    # fold round (fold.rs receive_fold): range bound, reconstruction, zeta
+    u1, u2, target = verify_reduce(it, t, s, q, row_w, col_w, claimed)
+    verify_lin(it, root, t, s, cfg, u1, u2, target, True, opt, ood_y, narg, hints)
+
+
+def verify_reduce(it: Inner, t: int, s: int, q: int, row_w, col_w, claimed: int):
+    """receive_fold and gkr_reduce_verify (fold.rs, gkr.rs:337-383,
+    reduce.rs:28-58) over a grid (t, s): the factored claim (u1, u2, target)"""
+    g = generator()
     folds = [gint(it.read(16)) for _ in range(1 << s)]
     bound = (1 << t) * (q - 1)
     if any(f > bound for f in folds):
@@ -1030,12 +1052,20 @@ def verify_claim(it: Inner, root: bytes, t: int, s: int, cfg: dict, q: int, row_
     row_images = O.gf_to_ints(O.pow_many(g, gfa(row_w)))
     u1 = [O.mul(a ^ 1, e) for a, e in zip(row_images, eq_ints(alpha_b))]
     u2 = eq_ints(alpha_c)
-    target = claim ^ 1
+    return u1, u2, claim ^ 1
+
+
+def verify_lin(it: Inner, root: bytes, t: int, s: int, cfg: dict, u1, u2, target: int, bind: bool, opt,
+               ood_y: int, narg: bytes, hints: bytes) -> None:
+    """Pcs::verify_lin (src/bitz/pcs.rs:379-424) of sum u1[b] u2[c] bit(c, b)
+    = target over the committed bits seen as the grid (t, s)"""
+    m, mp = cfg["m"], cfg["m"] - LOG_PACKING
     pcs40 = u64(m) + u64(cfg["levels"][0]["rate"]) + u64(cfg["initial_k"]) + u64(0) + u64(1)
-    it.public(b"bitz/pcs/bit-inner-product/v2")
-    it.public(root)
-    it.public(pcs40)
-    it.public(claim_encoding(u1, u2, target))
+    if bind:
+        it.public(b"bitz/pcs/bit-inner-product/v2")
+        it.public(root)
+        it.public(pcs40)
+        it.public(claim_encoding(u1, u2, target))
     it.public(b"bitz/pcs/inner-product-sumcheck/v1")
     rows_w, cols_w = gfa(u1), gfa(u2)
     sc_point = []

@@ -490,7 +490,9 @@ class ChainWitness:
     """generate_sha256_chain_witnesses (chain.rs:635-702): the chain, every
     compression's hints, the packed source and the product tensor."""
 
-    def __init__(self, batch: ChainBatch, blocks):
+    def __init__(self, batch: ChainBatch, blocks, flip_hint=None):
+        """flip_hint: (instance, hint bit) to flip, a dishonest witness whose
+        derived grid still follows the map but breaks a constraint"""
         assert len(blocks) == batch.n
         self.blocks = [[int(w) for w in b] for b in blocks]
         states = [list(batch.rel.initial_state)]
@@ -500,6 +502,8 @@ class ChainWitness:
             hints, out = compression_hints(states[-1], b)
             states.append(out)
             cells = instance_source_bits(b, hints)
+            if flip_hint is not None and flip_hint[0] == len(states) - 2:
+                cells[512 + flip_hint[1]] ^= 1
             src.append(cells)
             st = [(states[-2][i // 32] >> (i % 32)) & 1 for i in range(256)]
             f_bars.append(np.concatenate([np.ones(1, np.uint8), cells[:512], np.array(st, np.uint8), cells[512:]]))
@@ -550,103 +554,209 @@ def public_words(blocks, digest, i: int) -> list[int]:
     return list(blocks[i]) + (list(digest) if i + 1 == len(blocks) else [0] * 8)
 
 
+def _outer(b: ChainBatch, root: bytes, blocks, digest, mp: int, round0):
+    """prove_linear / verify_linear's outer transcript (linear.rs:398-558,
+    586-705) up to the fork tag. `round0(tr)` runs Round 0's grinding,
+    draw and value (prover: computes y; verifier: checks the proof's) and
+    returns (point, y, nonce)."""
+    n, k = b.n, b.k
+    tr = P.Transcript()
+    stmt = b.statement_binding(blocks, digest)
+    binding = b.assignment_binding(root, stmt)
+    # statement frames (chain.rs:962-990)
+    frames = [(b"protocol", PROTOCOL_DOMAIN), (b"integer-relation", b.rel.digest),
+              (b"chained-boolean-map", b.map_digest)]
+    for tag, v in ((b"instance-vars", k), (b"instance-count", n), (b"assignment-row-vars", b.h[0]),
+                   (b"assignment-column-vars", b.h[1]), (b"source-row-vars", b.f[0]),
+                   (b"source-column-vars", b.f[1])):
+        frames.append((tag, u64(v)))
+    frames += [(b"public-sha256-chain", stmt), (b"assignment-oracle", binding)]
+    for tag, payload in frames:
+        S.spartan_message(tr, tag, payload)
+    # the opener policy (configuration.rs:224-227), then Round 0 bound
+    # before the prime (ood.rs:75-86, ligerito_flock.rs:1520-1549)
+    tr.absorb_slice(S.POLICY_LABEL)
+    tr.absorb_slice(P.ladder_digest(b.cfg))
+    P.ood_round_header(tr, mp, b.ood_bits)
+    opt, ood_y, r0_nonce = round0(tr)
+    tr.absorb_slice(b"\x50" + g16(ood_y))
+    # the runtime prime (prime.rs:233-262); the initial boundary is at
+    # difficulty 0 and absorbs nothing
+    S.spartan_message(tr, b"prime-domain", PRIME_DOMAIN)
+    S.spartan_message(tr, b"log-instance-capacity", u64(k))
+    S.spartan_message(tr, b"prime-min", S.u128b(b.pmin))
+    S.spartan_message(tr, b"prime-max", S.u128b(b.pmax))
+    q = P.sample_prime_context(tr, b.pmin, b.pmax, 128)
+    S.spartan_message(tr, b"prime-q", S.u128b(q))
+    S.spartan_message(tr, b"runtime-field-modulus", S.u128b(q))
+    S.spartan_message(tr, b"projected-linear-relation", b.rel.digest)
+    xi = challenge_point(tr, LOCAL_POINT_DOMAIN, LOCAL_VARS, q)
+    eta = challenge_point(tr, INSTANCE_POINT_DOMAIN, k, q)
+    alpha0 = challenge_point(tr, CONSTANT_ONE_DOMAIN, 1, q)[0]
+    lam = challenge_point(tr, PUBLIC_IO_DOMAIN, PUBLIC_BITS + 1, q)
+    alpha_pub = lam.pop()
+    # the rank-one batching (chain.rs:1080-1161)
+    coef = collapse(b.rel, S.eq_le(xi, q), q)
+    coef[0] = (coef[0] + alpha0) % q
+    for slot in range(PUBLIC_BITS):
+        c = chain_public_h_column(slot)
+        coef[c] = (coef[c] + alpha_pub * lam[slot]) % q
+    u = S.eq_le(eta, q)[:n]
+    mu = alpha0 * (sum(u) % q) % q
+    for i in range(n):
+        words = public_words(blocks, digest, i)
+        val = 0
+        for ws, word in enumerate(words):
+            for bit in range(32):
+                if (word >> bit) & 1:
+                    val += lam[32 * ws + bit]
+        mu = (mu + u[i] * alpha_pub % q * (val % q)) % q
+    # chain_product_opening_claim (chain.rs:1201-1251)
+    th, sh = b.h
+    row_w = S.eq_le(eta[:th], q)
+    high = S.eq_le(eta[th:], q)
+    hi_n = len(high)
+    col_w = [0] * (1 << sh)
+    for c in range(min(1 << sh, CHAIN_H_BAR * hi_n)):
+        col_w[c] = coef[c // hi_n] * high[c % hi_n] % q
+    # the claim frame (linear.rs:324-368)
+    hd = O.Blake3()
+    hd.update(CLAIM_DOMAIN + binding)
+    for pt in (xi, eta):
+        hd.update(u64(len(pt)) + b"".join(S.u128b(x) for x in pt))
+    hd.update(S.u128b(alpha0) + u64(len(lam)) + b"".join(S.u128b(x) for x in lam) + S.u128b(alpha_pub))
+    hd.update(u64(len(row_w)) + b"".join(S.u128b(x) for x in row_w))
+    hd.update(u64(len(col_w)) + b"".join(S.u128b(x) for x in col_w))
+    hd.update(S.u128b(mu))
+    claim_digest = hd.digest(32)
+    S.spartan_message(tr, CLAIM_TAG, claim_digest)
+    tag = tr.challenge(16) + tr.challenge(16)
+    return tr, dict(q=q, xi=xi, eta=eta, alpha0=alpha0, lam=lam, alpha_pub=alpha_pub, coef=coef, mu=mu,
+                    row_w=row_w, col_w=col_w, binding=binding, stmt=stmt, claim_digest=claim_digest, tag=tag,
+                    ood_point=opt, ood_y=ood_y, r0_nonce=r0_nonce)
+
+
+def transpose_weights(b: ChainBatch, u1, u2) -> np.ndarray:
+    """VirtualStatement::transpose_query (src/bitz/virt.rs:153-199) over the
+    chained map's column_rows (binary.rs:831-876): weight[src] = sum over
+    the derived cells d of src of u1[d mod 2^t] u2[d div 2^t], as a Gf128
+    array over the 2^m committed cells"""
+    n, th = b.n, b.h[0]
+    U1, U2 = P.gfa(u1), P.gfa(u2)
+    out = np.zeros((1 << sum(b.f), 2), dtype=np.uint64)
+    inst = np.arange(n, dtype=np.int64)
+    for name, src_i, dst_i in (("local", inst, inst), ("prev", inst[:-1], inst[1:]),
+                               ("first", inst[:1], inst[:1]), ("last", inst[-1:], inst[-1:])):
+        mp = b.rel.maps[name]
+        off = np.asarray(mp["offsets"], dtype=np.int64)
+        fcol = np.repeat(np.arange(CHAIN_F_BAR, dtype=np.int64), np.diff(off))
+        hrow = np.asarray(mp["indices"], dtype=np.int64)
+        if len(hrow) == 0:
+            continue
+        src = np.where(fcol[:, None] == 0, 0, 1 + src_i[None, :] * CHAIN_F_INSTANCE + fcol[:, None] - 1)
+        d = hrow[:, None] * n + dst_i[None, :]
+        src, d = src.reshape(-1), d.reshape(-1)
+        prod = O.mul_arrays(U1[d & ((1 << th) - 1)], U2[d >> th])
+        np.bitwise_xor.at(out[:, 0], src, prod[:, 0])
+        np.bitwise_xor.at(out[:, 1], src, prod[:, 1])
+    return out
+
+
+def virtual_frame(b: ChainBatch, q: int) -> bytes:
+    """VirtualStatement::frame (virt.rs:132-139): the claim parameters on
+    the derived grid, the committed shape, the map digest"""
+    th, sh = b.h
+    return u64(th) + u64(sh) + g16(q) + g16(P.generator()) + u64(b.f[0]) + u64(b.f[1]) + b.map_digest
+
+
+def _pcs40(b: ChainBatch) -> bytes:
+    return u64(sum(b.f)) + u64(b.cfg["levels"][0]["rate"]) + u64(b.cfg["initial_k"]) + u64(0) + u64(1)
+
+
 class ChainProver:
     """prove_sha256_chain -> prove_linear (linear.rs:371-582) on the
     Blake3Transcript, the opening on this build's inner transcript."""
 
-    def __init__(self, k: int, blocks):
+    def __init__(self, k: int, blocks, flip_hint=None):
         self.batch = ChainBatch(k)
-        self.wit = ChainWitness(self.batch, blocks)
+        self.wit = ChainWitness(self.batch, blocks, flip_hint)
         tf, sf = self.batch.f
         self.pcs = P.OracleProver(self.wit.source_rows, tf, sf, 1, 4)
         assert self.pcs.cfg == self.batch.cfg
 
-    def prove(self, with_opening: bool = True):
-        b, w = self.batch, self.wit
-        n, k = b.n, b.k
-        tr = P.Transcript()
-        stmt = b.statement_binding(w.blocks, w.digest)
-        binding = b.assignment_binding(self.pcs.root, stmt)
-        # statement frames (chain.rs:962-990)
-        frames = [(b"protocol", PROTOCOL_DOMAIN), (b"integer-relation", b.rel.digest),
-                  (b"chained-boolean-map", b.map_digest)]
-        for tag, v in ((b"instance-vars", k), (b"instance-count", n), (b"assignment-row-vars", b.h[0]),
-                       (b"assignment-column-vars", b.h[1]), (b"source-row-vars", b.f[0]),
-                       (b"source-column-vars", b.f[1])):
-            frames.append((tag, u64(v)))
-        frames += [(b"public-sha256-chain", stmt), (b"assignment-oracle", binding)]
-        for tag, payload in frames:
-            S.spartan_message(tr, tag, payload)
-        # the opener policy (configuration.rs:224-227), Round 0 bound early
-        # (ood.rs:75-86, ligerito_flock.rs:1520-1549)
-        tr.absorb_slice(S.POLICY_LABEL)
-        tr.absorb_slice(P.ladder_digest(b.cfg))
-        mp = self.pcs.mp
-        bits = b.ood_bits
-        P.ood_round_header(tr, mp, bits)
-        r0_nonce = None
+    def _round0(self, tr):
+        bits = self.batch.ood_bits
+        nonce = None
         if bits:
             seed = P.grinding_seed(tr, P.OOD_GRIND_DOMAIN, 0, bits)
-            r0_nonce = O.pow_find(seed[:16], seed[16:], bits)
-            P.absorb_grinding_nonce(tr, r0_nonce)
-        zeta = P.gint(tr.challenge(16))
-        opt = P.ood_point(zeta, mp)
-        ood_y = O.mle_eval(self.pcs.packed, P.gfa(opt))
-        tr.absorb_slice(b"\x50" + g16(ood_y))
-        # the runtime prime (prime.rs:233-262); the initial boundary is at
-        # difficulty 0 and absorbs nothing
-        S.spartan_message(tr, b"prime-domain", PRIME_DOMAIN)
-        S.spartan_message(tr, b"log-instance-capacity", u64(k))
-        S.spartan_message(tr, b"prime-min", S.u128b(b.pmin))
-        S.spartan_message(tr, b"prime-max", S.u128b(b.pmax))
-        q = P.sample_prime_context(tr, b.pmin, b.pmax, 128)
-        S.spartan_message(tr, b"prime-q", S.u128b(q))
-        S.spartan_message(tr, b"runtime-field-modulus", S.u128b(q))
-        S.spartan_message(tr, b"projected-linear-relation", b.rel.digest)
-        xi = challenge_point(tr, LOCAL_POINT_DOMAIN, LOCAL_VARS, q)
-        eta = challenge_point(tr, INSTANCE_POINT_DOMAIN, k, q)
-        alpha0 = challenge_point(tr, CONSTANT_ONE_DOMAIN, 1, q)[0]
-        lam = challenge_point(tr, PUBLIC_IO_DOMAIN, PUBLIC_BITS + 1, q)
-        alpha_pub = lam.pop()
-        # the rank-one batching (chain.rs:1080-1161)
-        coef = collapse(b.rel, S.eq_le(xi, q), q)
-        coef[0] = (coef[0] + alpha0) % q
-        for slot in range(PUBLIC_BITS):
-            c = chain_public_h_column(slot)
-            coef[c] = (coef[c] + alpha_pub * lam[slot]) % q
-        u = S.eq_le(eta, q)[:n]
-        mu = alpha0 * (sum(u) % q) % q
-        for i in range(n):
-            words = public_words(w.blocks, w.digest, i)
-            val = 0
-            for ws, word in enumerate(words):
-                for bit in range(32):
-                    if (word >> bit) & 1:
-                        val += lam[32 * ws + bit]
-            mu = (mu + u[i] * alpha_pub % q * (val % q)) % q
-        # chain_product_opening_claim (chain.rs:1201-1251)
-        th, sh = b.h
-        row_w = S.eq_le(eta[:th], q)
-        high = S.eq_le(eta[th:], q)
-        hi_n = len(high)
-        col_w = [0] * (1 << sh)
-        for c in range(1 << sh):
-            loc = c // hi_n
-            if loc < CHAIN_H_BAR:
-                col_w[c] = coef[loc] * high[c % hi_n] % q
-        # the claim frame (linear.rs:324-368)
-        hd = O.Blake3()
-        hd.update(CLAIM_DOMAIN + binding)
-        for pt in (xi, eta):
-            hd.update(u64(len(pt)) + b"".join(S.u128b(x) for x in pt))
-        hd.update(S.u128b(alpha0) + u64(len(lam)) + b"".join(S.u128b(x) for x in lam) + S.u128b(alpha_pub))
-        hd.update(u64(len(row_w)) + b"".join(S.u128b(x) for x in row_w))
-        hd.update(u64(len(col_w)) + b"".join(S.u128b(x) for x in col_w))
-        hd.update(S.u128b(mu))
-        claim_digest = hd.digest(32)
-        S.spartan_message(tr, CLAIM_TAG, claim_digest)
-        tag = tr.challenge(16) + tr.challenge(16)
-        out = dict(q=q, xi=xi, eta=eta, alpha0=alpha0, lam=lam, alpha_pub=alpha_pub, coef=coef, mu=mu,
-                   row_w=row_w, col_w=col_w, binding=binding, stmt=stmt, claim_digest=claim_digest, tag=tag,
-                   ood_point=opt, ood_y=ood_y, r0_nonce=r0_nonce)
+            nonce = O.pow_find(seed[:16], seed[16:], bits)
+            P.absorb_grinding_nonce(tr, nonce)
+        opt = P.ood_point(P.gint(tr.challenge(16)), self.pcs.mp)
+        return opt, O.mle_eval(self.pcs.packed, P.gfa(opt)), nonce
+
+    def prove(self, with_opening: bool = True):
+        """returns (outer transcript, dict of the PIOP values; "proof": the
+        opening's bytes when with_opening)"""
+        b, w = self.batch, self.wit
+        tr, out = _outer(b, self.pcs.root, w.blocks, w.digest, self.pcs.mp, self._round0)
+        if with_opening:
+            out["proof"] = self.open(out)
         return tr, out
+
+    def open(self, out) -> bytes:
+        """prove_virtual_opening (bitz_opener.rs:1597-1624) ->
+        BitZProver::prove_virtual (virt.rs:214-301)"""
+        b = self.batch
+        th, sh = b.h
+        it = P.Inner(VIRTUAL_SESSION, out["tag"])
+        it.public(VIRTUAL_LABEL)
+        it.public(self.pcs.root)
+        it.public(virtual_frame(b, out["q"]))
+        it.public(_pcs40(b))
+        it.public(P.claim_encoding(out["row_w"], out["col_w"], out["mu"]))
+        folds = O.gf_to_ints(O.fold_columns(self.wit.product_rows, th, sh, P.gfa(out["row_w"])))
+        u1, u2, target = self.pcs.reduce_claim(it, out["row_w"], folds, self.wit.product_rows, th, sh)
+        wf = O.gf_to_ints(transpose_weights(b, u1, u2))
+        return self.pcs.prove_lin(it, wf, [1], target, self.pcs.rows, self.pcs.m, 0, False, out["ood_point"],
+                                  out["ood_y"], out["r0_nonce"])
+
+
+def verify_chain(k: int, blocks, digest, root: bytes, proof: bytes) -> P.Transcript:
+    """verify_sha256_chain (chain.rs:850-872) -> verify_linear: replays the
+    outer transcript from the statement, the root and the proof's Round-0
+    value, then checks the virtual opening; raises P.VerifyError, returns the
+    outer transcript"""
+    b = ChainBatch(k)
+    if len(blocks) != b.n:
+        raise P.VerifyError("block count")
+    narg, hints, ood_y, r0_nonce = P.proof_tail_parse(proof)
+    if ood_y is None:
+        raise P.VerifyError("missing Round-0 claim")
+    mp = sum(b.f) - P.LOG_PACKING
+
+    def round0(tr):
+        bits = b.ood_bits
+        if bits:
+            if r0_nonce is None:
+                raise P.VerifyError("missing Round-0 nonce")
+            seed = P.grinding_seed(tr, P.OOD_GRIND_DOMAIN, 0, bits)
+            ok = O.pow_valid(seed[:16], seed[16:], r0_nonce, bits)
+            P.absorb_grinding_nonce(tr, r0_nonce)
+            if not ok:
+                raise P.VerifyError("Round-0 proof of work")
+        elif r0_nonce is not None:
+            raise P.VerifyError("unexpected Round-0 nonce")
+        return P.ood_point(P.gint(tr.challenge(16)), mp), ood_y, r0_nonce
+
+    tr, out = _outer(b, root, [[int(x) for x in blk] for blk in blocks], [int(x) for x in digest], mp, round0)
+    th, sh = b.h
+    it = P.Inner(VIRTUAL_SESSION, out["tag"], narg)
+    it.public(VIRTUAL_LABEL)
+    it.public(root)
+    it.public(virtual_frame(b, out["q"]))
+    it.public(_pcs40(b))
+    it.public(P.claim_encoding(out["row_w"], out["col_w"], out["mu"]))
+    u1, u2, target = P.verify_reduce(it, th, sh, out["q"], out["row_w"], out["col_w"], out["mu"])
+    wf = O.gf_to_ints(transpose_weights(b, u1, u2))
+    P.verify_lin(it, root, sum(b.f), 0, b.cfg, wf, [1], target, False, out["ood_point"], ood_y, narg, hints)
+    return tr

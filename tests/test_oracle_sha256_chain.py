@@ -125,7 +125,7 @@ def test_product_claim_by_brute_force(pinned):
         if cw:
             acc += cw * (sum(rw[b] for b in np.nonzero(bits[c])[0]) % q)
     assert acc % q == out["mu"]
-    # a different digest (a wrong statement) changes mu
+    # the runtime prime lies in [2^112, 2^113) (derive_params at 2^7 instances)
     assert out["q"].bit_length() == 113
 
 
@@ -137,3 +137,58 @@ def test_chain_states_and_digest(pinned):
         assert w.states[i + 1] == C.compress(w.states[i], w.blocks[i])
     # source cells: the constant, then block and hint bits per instance
     assert w.source_cells[0] == 1 and len(w.source_cells) == 1 + 128 * C.CHAIN_F_INSTANCE
+
+
+@pytest.fixture(scope="module")
+def full_proof():
+    pr = C.ChainProver(7, C.pin_blocks(128))
+    tr, out = pr.prove(with_opening=True)
+    return pr, tr, out
+
+
+def test_full_chain_proof_verifies(full_proof):
+    """the CPU verifier replays the pinned outer transcript and accepts the
+    virtual opening"""
+    pr, tr, out = full_proof
+    assert O.blake3(bytes(tr.buf), 32).hex() == PIN_LAMBDA100
+    vt = C.verify_chain(7, pr.wit.blocks, pr.wit.digest, pr.pcs.root, out["proof"])
+    assert bytes(vt.buf) == bytes(tr.buf)
+
+
+def test_wrong_statement_and_tampering_rejected(full_proof):
+    pr, _, out = full_proof
+    proof = out["proof"]
+    bad_digest = list(pr.wit.digest)
+    bad_digest[3] ^= 1 << 7
+    with pytest.raises(C.P.VerifyError):
+        C.verify_chain(7, pr.wit.blocks, bad_digest, pr.pcs.root, proof)
+    blocks = [list(b) for b in pr.wit.blocks]
+    blocks[40][2] ^= 1
+    with pytest.raises(C.P.VerifyError):
+        C.verify_chain(7, blocks, pr.wit.digest, pr.pcs.root, proof)
+    for pos in (20, 16 * 200 + 3, len(proof) // 2, len(proof) - 40):
+        bad = bytearray(proof)
+        bad[pos] ^= 0x04
+        with pytest.raises(C.P.VerifyError):
+            C.verify_chain(7, pr.wit.blocks, pr.wit.digest, pr.pcs.root, bytes(bad))
+
+
+def test_unsatisfying_witness_rejected():
+    """one flipped hint bit: the derived grid still follows the map, the
+    relation's collapsed rows no longer vanish, and the folds miss the claim"""
+    pr = C.ChainProver(7, C.pin_blocks(128), flip_hint=(77, 300))
+    _, out = pr.prove(with_opening=True)
+    with pytest.raises(C.P.VerifyError):
+        C.verify_chain(7, pr.wit.blocks, pr.wit.digest, pr.pcs.root, out["proof"])
+
+
+def test_derived_grid_inconsistent_with_commitment_cannot_be_proved():
+    """virt.rs:605-620: a product tensor that is not M f fails the opening's
+    inner-product check in the prover"""
+    pr = C.ChainProver(7, C.pin_blocks(128))
+    _, out = pr.prove(with_opening=False)
+    rows = pr.wit.product_rows.copy()
+    rows[12345] ^= np.uint64(1 << 9)
+    pr.wit.product_rows = rows
+    with pytest.raises(AssertionError):
+        pr.open(out)

@@ -1,0 +1,640 @@
+// The SHA-256 chain proof: H_{i+1} = Compress(H_i, M_i) for N = 2^k blocks,
+// proved as one linear relation over a derived bit grid and opened through
+// the commitment to its source bits.
+//
+// prove_sha256_chain (src/piop/spartan/sha256/chain.rs:817-845) ->
+// prove_linear (protocol/linear.rs:371-582):
+//   the witness (chain.rs:635-794): the chain of states on the host, then
+//     per instance on the device the compression's hint bits (the schedule
+//     sums, the doubled round sums, the output sums), the committed source
+//     cells [1 | block_i | hints_i ...] and the product tensor
+//     D[local, instance] = (M f_bar_i)[local] with the terminal rows;
+//   the commitment of the source rows (custom:1:4);
+//   the statement frames (chain.rs:962-990: protocol, relation digest, map
+//     digest, geometry, the public statement binding, the assignment
+//     binding), the opener policy, Round 0, the runtime prime
+//     (sha256/prime.rs:233-262), the relation frames, the local and instance
+//     points, the constant-one and public-IO batching challenges
+//     (linear.rs:306-340), the collapsed local rows beta = C^T eq(xi)
+//     (sumcheck/bridge/repeated.rs:4-22), the rank-one product claim
+//     (chain.rs:1080-1251) and its frame (linear.rs:324-368), the fork tag;
+//   the virtual opening (bitz_opener.rs:1597-1624, src/bitz/virt.rs:
+//     132-301): its statement, the fold and GKR over the product tensor, the
+//     transposed weights through the chained map (binary.rs:831-876), and the
+//     single-column inner product on the committed bits.
+// Every grinding boundary of the lambda = 100 profile is at difficulty 0 for
+// these shapes and absorbs nothing.
+#include <string.h>
+
+#include <algorithm>
+#include <array>
+#include <memory>
+
+#include "chain_impl.h"
+#include "common.h"
+#include "gf128_dev.h"
+#include "host_gf.h"
+#include "host_proto.h"
+#include "modq.h"
+#include "open_impl.h"
+#include "spartan_impl.h"
+
+namespace bitz {
+namespace {
+
+using namespace chain;
+
+constexpr char kProtocolDomain[] = "bitz/spartan-sha256-chain/product-linear/v1";
+constexpr char kAssignmentDomain[] = "bitz/spartan-sha256-chain-assignment/runtime-prime/v1";
+constexpr char kStatementDomain[] = "bitz/spartan-sha256-chain/public-statement/v1";
+constexpr char kPublicIoDomain[] = "bitz/spartan-sha256-chain/public-io-batch/v1";
+constexpr char kConstantOneDomain[] = "bitz/spartan-sha256-chain/constant-one-batch/v1";
+constexpr char kLocalPointDomain[] = "bitz/spartan-sha256-chain/local-row-point/v1";
+constexpr char kInstancePointDomain[] = "bitz/spartan-sha256-chain/instance-point/v1";
+constexpr char kClaimDomain[] = "bitz/spartan-sha256-chain-opening/v1";
+constexpr char kClaimTag[] = "sha256-chain-opening-claim";
+constexpr char kPrimeDomain[] = "bitz/spartan-sha256/runtime-prime/v1";
+constexpr char kPolicyLabel[] = "bitz/ligerito-policy/early-ood/v1";
+constexpr char kVirtualSession[] = "bitz/bitz-opener/virtual/v1";
+constexpr char kVirtualLabel[] = "bitz/virtual-statement/v2";
+constexpr char kProfileName[] = "lambda100";
+constexpr int kRate = 1, kBatch = 4;  // LigeritoSelection::JOHNSON (configuration.rs:24-27)
+
+// FIPS 180-4 4.2.2 (device copy)
+__constant__ uint32_t c_K[64] = {
+    0x428a2f98, 0x71374491, 0xb5c0fbcf, 0xe9b5dba5, 0x3956c25b, 0x59f111f1, 0x923f82a4, 0xab1c5ed5,
+    0xd807aa98, 0x12835b01, 0x243185be, 0x550c7dc3, 0x72be5d74, 0x80deb1fe, 0x9bdc06a7, 0xc19bf174,
+    0xe49b69c1, 0xefbe4786, 0x0fc19dc6, 0x240ca1cc, 0x2de92c6f, 0x4a7484aa, 0x5cb0a9dc, 0x76f988da,
+    0x983e5152, 0xa831c66d, 0xb00327c8, 0xbf597fc7, 0xc6e00bf3, 0xd5a79147, 0x06ca6351, 0x14292967,
+    0x27b70a85, 0x2e1b2138, 0x4d2c6dfc, 0x53380d13, 0x650a7354, 0x766a0abb, 0x81c2c92e, 0x92722c85,
+    0xa2bfe8a1, 0xa81a664b, 0xc24b8b70, 0xc76c51a3, 0xd192e819, 0xd6990624, 0xf40e3585, 0x106aa070,
+    0x19a4c116, 0x1e376c08, 0x2748774c, 0x34b0bcb5, 0x391c0cb3, 0x4ed8aa4a, 0x5b9cca4f, 0x682e6ff3,
+    0x748f82ee, 0x78a5636f, 0x84c87814, 0x8cc70208, 0x90befffa, 0xa4506ceb, 0xbef9a3f7, 0xc67178f2};
+
+__device__ __forceinline__ uint32_t rotr32(uint32_t x, int n) { return (x >> n) | (x << (32 - n)); }
+
+// the wave's 64 instances' bit `b` of v, one LDS word per f_bar column
+__device__ __forceinline__ void emit_bits(uint64_t* col, int base, uint64_t v, int nbits, uint32_t lane) {
+#pragma unroll 1
+    for (int b = 0; b < nbits; b++) {
+        const uint64_t m = __ballot((int)((v >> b) & 1ull));
+        if (lane == 0) col[base + b] = m;
+    }
+}
+
+// One wavefront per 64 consecutive instances (n is a multiple of 64).
+// Lane j replays instance 64 g + j's compression and emits its f_bar bits in
+// the circuit's column order ([1 | block | state | hints], hints in
+// allocation order: 48 schedule sums of 34 bits, per round the e then the a
+// half-sums of 35 bits, 8 output sums of 33 bits); a ballot per bit makes
+// col[c] the 64 instances' bits of column c. Then:
+//  * the product tensor: derived row r of the group is the XOR of the col
+//    words of M's row r (m_off, m_cols over f_bar), stored as one 64-bit
+//    row word of D (grid th x sh: cell = (r H + hi) 2^th + (instance mod
+//    2^th), H = n >> th); the terminal rows hold the last instance's output
+//    bits; rows past the chain's 20713 stay zero;
+//  * the committed source stream: lane j's 6888 cells (block bits, then hint
+//    bits) at bit 1 + 6888 i of src_bits, OR-ed word by word (neighbouring
+//    instances share the end words); cell 0 (the constant) is set once.
+__global__ __launch_bounds__(64) void k_chain_witness(const uint32_t* __restrict__ blocks,
+                                                      const uint32_t* __restrict__ states, uint64_t n,
+                                                      const uint32_t* __restrict__ m_off,
+                                                      const uint32_t* __restrict__ m_cols, int th,
+                                                      uint64_t* __restrict__ D, uint64_t* __restrict__ src_bits) {
+    __shared__ uint64_t col[kFBar];
+    const uint32_t lane = threadIdx.x;
+    const uint64_t inst = blockIdx.x * 64ull + lane;
+    uint32_t W[64];
+    uint32_t st[8];
+#pragma unroll
+    for (int i = 0; i < 16; i++) W[i] = blocks[inst * 16 + i];
+#pragma unroll
+    for (int i = 0; i < 8; i++) st[i] = states[inst * 8 + i];
+    if (lane == 0) col[0] = ~0ull;
+#pragma unroll
+    for (int i = 0; i < 16; i++) emit_bits(col, 1 + 32 * i, W[i], 32, lane);
+#pragma unroll
+    for (int i = 0; i < 8; i++) emit_bits(col, 513 + 32 * i, st[i], 32, lane);
+    int pos = 769;
+#pragma unroll
+    for (int i = 16; i < 64; i++) {
+        const uint32_t s0 = rotr32(W[i - 15], 7) ^ rotr32(W[i - 15], 18) ^ (W[i - 15] >> 3);
+        const uint32_t s1 = rotr32(W[i - 2], 17) ^ rotr32(W[i - 2], 19) ^ (W[i - 2] >> 10);
+        const uint64_t total = (uint64_t)W[i - 16] + s0 + W[i - 7] + s1;
+        emit_bits(col, pos, total, 34, lane);
+        pos += 34;
+        W[i] = (uint32_t)total;
+    }
+    uint32_t a = st[0], b = st[1], c = st[2], d = st[3], e = st[4], f = st[5], g = st[6], h = st[7];
+#pragma unroll
+    for (int i = 0; i < 64; i++) {
+        const uint32_t big1 = rotr32(e, 6) ^ rotr32(e, 11) ^ rotr32(e, 25);
+        const uint32_t ch = (e & f) ^ (~e & g);
+        const uint32_t big0 = rotr32(a, 2) ^ rotr32(a, 13) ^ rotr32(a, 22);
+        const uint32_t maj = (a & b) ^ (a & c) ^ (b & c);
+        const uint64_t common = (uint64_t)h + big1 + c_K[i] + W[i] + ch;
+        const uint64_t e_half = common + d;
+        const uint64_t a_half = common + big0 + maj;
+        emit_bits(col, pos, e_half, 35, lane);
+        emit_bits(col, pos + 35, a_half, 35, lane);
+        pos += 70;
+        h = g;
+        g = f;
+        f = e;
+        e = (uint32_t)e_half;
+        d = c;
+        c = b;
+        b = a;
+        a = (uint32_t)a_half;
+    }
+    const uint32_t wk[8] = {a, b, c, d, e, f, g, h};
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+        emit_bits(col, pos, (uint64_t)st[i] + wk[i], 33, lane);
+        pos += 33;
+    }
+    __syncthreads();
+    // the product tensor's row words of this group
+    const uint64_t first = blockIdx.x * 64ull;
+    const uint64_t H = n >> th;
+    const uint64_t hi = first >> th;
+    const uint64_t wsel = (first & ((1ull << th) - 1)) >> 6;
+    const uint64_t wpc = 1ull << (th - 6);
+    const bool last_group = first + 64 == n;
+    for (uint32_t r = lane; r < (uint32_t)kChainHBar; r += 64) {
+        uint64_t v = 0;
+        if (r < (uint32_t)kHBar) {
+            for (uint32_t k = m_off[r]; k < m_off[r + 1]; k++) v ^= col[m_cols[k]];
+        } else if (last_group) {
+            const int t = (int)r - kHBar;
+            v = col[kOutputFBase + 33 * (t / 32) + t % 32] & (1ull << 63);
+        }
+        D[((uint64_t)r * H + hi) * wpc + wsel] = v;
+    }
+    // the committed source stream, instance-aligned words shifted into place
+    const uint64_t start = 1 + inst * (uint64_t)kFInstance;
+    const int off = (int)(start & 63);
+    uint64_t wo = start >> 6;
+    uint64_t prev = 0;
+    for (int q = 0; q * 64 < kFInstance; q++) {
+        uint64_t cur = 0;
+        for (int bb = 0; bb < 64; bb++) {
+            const int j = q * 64 + bb;
+            if (j >= kFInstance) break;
+            const int cidx = j < 512 ? 1 + j : 769 + (j - 512);
+            cur |= ((col[cidx] >> lane) & 1ull) << bb;
+        }
+        const uint64_t word = off ? (cur << off) | (prev >> (64 - off)) : cur;
+        if (word) atomicOr((unsigned long long*)&src_bits[wo], (unsigned long long)word);
+        wo++;
+        prev = cur;
+    }
+    if (off && (prev >> (64 - off))) atomicOr((unsigned long long*)&src_bits[wo], (unsigned long long)(prev >> (64 - off)));
+    if (inst == 0) atomicOr((unsigned long long*)&src_bits[0], 1ull);
+}
+
+// Transposed weights (VirtualStatement::transpose_query over the chained
+// map): committed cell src = 1 + 6888 i + (lc - 1) (lc: chained f_bar
+// column 1..6888) reads the derived rows of column lc of each map part at
+// one instance: local at i, prev at i + 1, first at 0, last at n - 1. A
+// derived cell d = r n + i' lies at row i' mod 2^th, column r H + (i' >> th),
+// so each part contributes u1[i' mod 2^th] times the XOR of its rows' u2
+// entries. Cell 0 (the constant) is set by the host; cells past the live
+// ones are zero.
+struct MapDev {
+    const uint32_t* off;
+    const uint32_t* rows;
+};
+__device__ __forceinline__ gf128 part_sum(MapDev m, uint32_t lc, uint64_t inst, const uint4* __restrict__ u2,
+                                          uint64_t H, int th) {
+    gf128 acc = gf_zero();
+    const uint64_t colhi = inst >> th;
+    for (uint32_t k = m.off[lc]; k < m.off[lc + 1]; k++) acc = gf_add(acc, gf_load(&u2[m.rows[k] * H + colhi]));
+    return acc;
+}
+__global__ __launch_bounds__(256) void k_chain_transpose(const uint4* __restrict__ u1, const uint4* __restrict__ u2,
+                                                         uint64_t n, int th, uint64_t cells, MapDev loc, MapDev prv,
+                                                         MapDev fst, MapDev lst, uint4* __restrict__ wf) {
+    const uint64_t H = n >> th, mask = (1ull << th) - 1, live = 1 + n * (uint64_t)kFInstance;
+    for (uint64_t src = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x + 1; src < cells;
+         src += (uint64_t)gridDim.x * blockDim.x) {
+        gf128 acc = gf_zero();
+        if (src < live) {
+            const uint64_t i = (src - 1) / kFInstance;
+            const uint32_t lc = 1 + (uint32_t)((src - 1) % kFInstance);
+            acc = gf_mul(gf_load(&u1[i & mask]), part_sum(loc, lc, i, u2, H, th));
+            if (i + 1 < n) {
+                const gf128 s = part_sum(prv, lc, i + 1, u2, H, th);
+                if (!gf_is_zero(s)) acc = gf_add(acc, gf_mul(gf_load(&u1[(i + 1) & mask]), s));
+            }
+            if (i == 0) acc = gf_add(acc, gf_mul(gf_load(&u1[0]), part_sum(fst, lc, 0, u2, H, th)));
+            if (i + 1 == n) acc = gf_add(acc, gf_mul(gf_load(&u1[i & mask]), part_sum(lst, lc, i, u2, H, th)));
+        }
+        gf_store(&wf[src], acc);
+    }
+}
+
+// ---- host: the batch's device data ----
+struct ChainDev {
+    DevBuf m_off, m_cols, maps[4][2];
+    MapDev md[4];
+};
+int upload_relation(bitz_ctx* ctx, const Relation& r, ChainDev* cd) {
+    auto up = [&](const std::vector<uint32_t>& v, DevBuf& b) -> int {
+        if (!b.alloc(std::max<size_t>(v.size(), 1) * 4)) return BITZ_ERR_ALLOC;
+        BITZ_HIP(hipMemcpyAsync(b.ptr, v.data(), v.size() * 4, hipMemcpyHostToDevice, ctx->stream));
+        return BITZ_OK;
+    };
+    int rc;
+    if ((rc = up(r.m_off, cd->m_off)) || (rc = up(r.m_cols, cd->m_cols))) return rc;
+    const Csc* parts[4] = {&r.local, &r.prev, &r.first, &r.last};
+    for (int p = 0; p < 4; p++) {
+        if ((rc = up(parts[p]->offsets, cd->maps[p][0])) || (rc = up(parts[p]->rows, cd->maps[p][1]))) return rc;
+        cd->md[p] = MapDev{(const uint32_t*)cd->maps[p][0].ptr, (const uint32_t*)cd->maps[p][1].ptr};
+    }
+    // the host vectors must outlive the copies
+    BITZ_HIP(hipStreamSynchronize(ctx->stream));
+    return BITZ_OK;
+}
+
+void put64(std::vector<uint8_t>* o, uint64_t v) {
+    for (int i = 0; i < 8; i++) o->push_back((uint8_t)(v >> (8 * i)));
+}
+void put128(std::vector<uint8_t>* o, u128 v) {
+    put64(o, (uint64_t)v);
+    put64(o, (uint64_t)(v >> 64));
+}
+void put32(std::vector<uint8_t>* o, uint32_t v) {
+    for (int i = 0; i < 4; i++) o->push_back((uint8_t)(v >> (8 * i)));
+}
+void put_str(std::vector<uint8_t>* o, const char* s) { o->insert(o->end(), s, s + strlen(s)); }
+
+// hash_security_params (sha256/proof.rs:2000-2034) at lambda 100
+void security_bytes(const Batch& b, int ood_bits, std::vector<uint8_t>* o) {
+    put64(o, strlen(kProfileName));
+    put_str(o, kProfileName);
+    put32(o, 100);
+    put128(o, b.pmin);
+    put128(o, b.pmax);
+    o->push_back(0);  // not the full-width fingerprint
+    put32(o, 0);      // initial, PIOP round and terminal grinding
+    put32(o, 0);
+    put32(o, 0);
+    o->push_back(0);  // no reduction draw
+    put32(o, 0);      // forest and ring-switch grinding
+    put32(o, 0);
+    o->push_back(1);  // Round 0 runs
+    put32(o, (uint32_t)ood_bits);
+    put64(o, 100);
+}
+
+// hash_ligerito_config (proof.rs:2036-2067) of the ladder's prover config
+// (pcs/ligerito.rs:1441-1475)
+void ligerito_bytes(const LigConfig& cfg, std::vector<uint8_t>* o) {
+    const auto& lv = cfg.levels;
+    put64(o, lv.size() - 1);
+    put64(o, lv[0].log_msg_cols);
+    put64(o, cfg.initial_k);
+    put64(o, cfg.initial_k);
+    auto vec = [&](auto field, size_t from) {
+        put64(o, lv.size() - from);
+        for (size_t i = from; i < lv.size(); i++) put64(o, (uint64_t)field(lv[i]));
+    };
+    vec([](const LigLevel& l) { return l.log_inv_rate; }, 0);
+    vec([](const LigLevel& l) { return l.log_msg_cols; }, 1);
+    vec([](const LigLevel& l) { return l.k_recursive; }, 1);
+    vec([](const LigLevel& l) { return l.queries; }, 0);
+    vec([](const LigLevel& l) { return l.grinding_bits; }, 0);
+    vec([](const LigLevel& l) { return l.fold_grinding_bits; }, 0);
+    vec([](const LigLevel& l) { return l.ood_samples; }, 0);
+    o->push_back(1);  // BLAKE3
+}
+
+u128 modq(const ModN& F, int64_t v) {
+    const u128 q = F.n;
+    return v >= 0 ? (u128)v % q : (q - (u128)(-(v + 1)) % q - 1) % q;
+}
+std::vector<u128> eq_le(const ModN& F, const std::vector<u128>& pt) {
+    std::vector<u128> t(1, 1);
+    for (u128 c : pt) {
+        const size_t half = t.size();
+        t.resize(2 * half);
+        for (size_t j = 0; j < half; j++) {
+            const u128 one = F.mul(t[j], c);
+            t[j + half] = one;
+            t[j] = addmod(t[j], F.n - one, F.n);
+        }
+    }
+    return t;
+}
+
+struct Transposer {
+    bitz_ctx* ctx;
+    const Batch* b;
+    ChainDev* cd;
+};
+int transpose_cb(void* user, const void* u1, const void* u2, void* wf) {
+    Transposer* T = (Transposer*)user;
+    bitz_ctx* ctx = T->ctx;
+    const Batch& b = *T->b;
+    const Relation& r = *b.rel;
+    const uint64_t cells = 1ull << (b.tf + b.sf);
+    const uint64_t H = b.n >> b.th;
+    BITZ_REQUIRE(cells < (1ull << 40), BITZ_ERR_SHAPE, "chain transpose: too many cells");
+    hipLaunchKernelGGL(k_chain_transpose, dim3((unsigned)std::min<uint64_t>((cells + 255) / 256, 65536)), dim3(256),
+                       0, ctx->stream, (const uint4*)u1, (const uint4*)u2, b.n, b.th, cells, T->cd->md[0],
+                       T->cd->md[1], T->cd->md[2], T->cd->md[3], (uint4*)wf);
+    BITZ_LAUNCH_CHECK();
+    // cell 0: every instance's local column-0 rows (sum_lo u1 times the
+    // rows' u2 sums over the high instances), instance 0's first rows and
+    // instance n - 1's last rows
+    std::vector<hgf> hu1(1ull << b.th), hu2(1ull << b.sh);
+    BITZ_HIP(hipMemcpyAsync(hu1.data(), u1, hu1.size() * 16, hipMemcpyDeviceToHost, ctx->stream));
+    BITZ_HIP(hipMemcpyAsync(hu2.data(), u2, hu2.size() * 16, hipMemcpyDeviceToHost, ctx->stream));
+    BITZ_HIP(hipStreamSynchronize(ctx->stream));
+    hgf s1{0, 0};
+    for (auto& v : hu1) s1 = hgf_add(s1, v);
+    hgf w0{0, 0};
+    for (uint32_t k = r.local.offsets[0]; k < r.local.offsets[1]; k++) {
+        hgf acc{0, 0};
+        for (uint64_t hi = 0; hi < H; hi++) acc = hgf_add(acc, hu2[r.local.rows[k] * H + hi]);
+        w0 = hgf_add(w0, hgf_mul(s1, acc));
+    }
+    auto part0 = [&](const Csc& m, uint64_t inst) {
+        hgf acc{0, 0};
+        for (uint32_t k = m.offsets[0]; k < m.offsets[1]; k++) acc = hgf_add(acc, hu2[m.rows[k] * H + (inst >> b.th)]);
+        return hgf_mul(hu1[inst & ((1ull << b.th) - 1)], acc);
+    };
+    w0 = hgf_add(w0, part0(r.first, 0));
+    w0 = hgf_add(w0, part0(r.last, b.n - 1));
+    BITZ_HIP(hipMemcpyAsync(wf, &w0, 16, hipMemcpyHostToDevice, ctx->stream));
+    BITZ_HIP(hipStreamSynchronize(ctx->stream));
+    return BITZ_OK;
+}
+
+int prove_chain(bitz_ctx* ctx, const uint32_t* blocks, int k, const uint32_t* initial_state, uint8_t root[32],
+                uint32_t digest_out[8], uint8_t state_out[32], std::vector<uint8_t>* proof) {
+    std::array<uint32_t, 8> iv;
+    for (int i = 0; i < 8; i++) iv[i] = initial_state ? initial_state[i] : kIV[i];
+    Batch b;
+    BITZ_REQUIRE(make_batch(k, iv, &b), BITZ_ERR_SHAPE, "sha256 chain: needs 2^7..2^16 compressions");
+    const Relation& rel = *b.rel;
+    const uint64_t n = b.n;
+    PhaseClock phase(ctx);
+    phase.mark("chain_states");
+    // the native chain (the statement's digest)
+    std::vector<uint32_t> states((n + 1) * 8);
+    memcpy(states.data(), iv.data(), 32);
+    for (uint64_t i = 0; i < n; i++) compress(&states[i * 8], &blocks[i * 16], &states[(i + 1) * 8]);
+    const uint32_t* digest = &states[n * 8];
+    memcpy(digest_out, digest, 32);
+    phase.mark("chain_witness");
+    int rc;
+    ChainDev cd;
+    if ((rc = upload_relation(ctx, rel, &cd))) return rc;
+    DevBuf dblocks, dstates, src, D;
+    const uint64_t src_words = 1ull << (b.tf + b.sf - 6), d_words = 1ull << (b.th + b.sh - 6);
+    if (!dblocks.alloc(n * 64) || !dstates.alloc((n + 1) * 32) || !src.alloc(src_words * 8) || !D.alloc(d_words * 8))
+        return BITZ_ERR_ALLOC;
+    BITZ_HIP(hipMemcpyAsync(dblocks.ptr, blocks, n * 64, hipMemcpyHostToDevice, ctx->stream));
+    BITZ_HIP(hipMemcpyAsync(dstates.ptr, states.data(), (n + 1) * 32, hipMemcpyHostToDevice, ctx->stream));
+    BITZ_HIP(hipMemsetAsync(src.ptr, 0, src_words * 8, ctx->stream));
+    BITZ_HIP(hipMemsetAsync(D.ptr, 0, d_words * 8, ctx->stream));
+    hipLaunchKernelGGL(k_chain_witness, dim3((unsigned)(n / 64)), dim3(64), 0, ctx->stream,
+                       (const uint32_t*)dblocks.ptr, (const uint32_t*)dstates.ptr, n, (const uint32_t*)cd.m_off.ptr,
+                       (const uint32_t*)cd.m_cols.ptr, b.th, (uint64_t*)D.ptr, (uint64_t*)src.ptr);
+    BITZ_LAUNCH_CHECK();
+    BITZ_HIP(hipStreamSynchronize(ctx->stream));  // blocks and states are the caller's / local host memory
+    phase.mark("chain_commit");
+    std::unique_ptr<bitz_commit_handle, void (*)(bitz_commit_handle*)> h(nullptr, bitz_commit_free);
+    {
+        bitz_commit_handle* hp = nullptr;
+        if ((rc = bitz_commit_device_rows(ctx, src.ptr, b.tf, b.sf, kBatch, kRate, root, &hp))) return rc;
+        h.reset(hp);
+    }
+    src = DevBuf();
+    LigConfig cfg;
+    {
+        std::string err = custom_johnson_config(b.tf + b.sf, kRate, kBatch, 100, &cfg);
+        BITZ_REQUIRE(err.empty(), BITZ_ERR_SHAPE, "sha256 chain: Ligerito configuration: " + err);
+    }
+    phase.mark("chain_statement");
+    // chain_statement_binding (chain.rs:1364-1383)
+    uint8_t stmt[32], binding[32];
+    {
+        std::vector<uint8_t> o;
+        put_str(&o, kStatementDomain);
+        for (uint32_t w : iv) put32(&o, w);
+        put64(&o, n);
+        for (uint64_t i = 0; i < n * 16; i++) put32(&o, blocks[i]);
+        for (int i = 0; i < 8; i++) put32(&o, digest[i]);
+        blake3_hash_host(o.data(), o.size(), stmt);
+    }
+    // chain_assignment_binding (chain.rs:1385-1418)
+    {
+        std::vector<uint8_t> o;
+        put_str(&o, kAssignmentDomain);
+        o.insert(o.end(), root, root + 32);
+        o.insert(o.end(), rel.digest, rel.digest + 32);
+        o.insert(o.end(), b.map_digest, b.map_digest + 32);
+        o.insert(o.end(), stmt, stmt + 32);
+        const uint64_t vals[11] = {(uint64_t)(b.tf + b.sf), kRate, kBatch, n, (uint64_t)k, (uint64_t)b.th,
+                                   (uint64_t)b.sh, 1, (uint64_t)b.tf, (uint64_t)b.sf, 1};
+        for (uint64_t v : vals) put64(&o, v);
+        o.push_back(0);  // profile code of the commitment
+        o.push_back(1);  // BLAKE3
+        security_bytes(b, cfg.round0_grinding_bits, &o);
+        ligerito_bytes(cfg, &o);
+        blake3_hash_host(o.data(), o.size(), binding);
+    }
+    Transcript tr;
+    spartan_frame(tr, "protocol", kProtocolDomain, strlen(kProtocolDomain));
+    spartan_frame(tr, "integer-relation", rel.digest, 32);
+    spartan_frame(tr, "chained-boolean-map", b.map_digest, 32);
+    {
+        const char* tags[6] = {"instance-vars", "instance-count", "assignment-row-vars", "assignment-column-vars",
+                               "source-row-vars", "source-column-vars"};
+        const uint64_t vals[6] = {(uint64_t)k, n, (uint64_t)b.th, (uint64_t)b.sh, (uint64_t)b.tf, (uint64_t)b.sf};
+        for (int i = 0; i < 6; i++) spartan_frame(tr, tags[i], &vals[i], 8);
+    }
+    spartan_frame(tr, "public-sha256-chain", stmt, 32);
+    spartan_frame(tr, "assignment-oracle", binding, 32);
+    tr.absorb_slice(kPolicyLabel);
+    {
+        uint8_t dg[32];
+        ligerito_digest(cfg, dg);
+        tr.absorb_slice(dg, 32);
+    }
+    Round0Claim r0;
+    if ((rc = round0_prove(ctx, h.get(), tr, &r0))) return rc;
+    // the runtime prime and the relation frames
+    {
+        uint8_t v[16];
+        spartan_frame(tr, "prime-domain", kPrimeDomain, strlen(kPrimeDomain));
+        const uint64_t kk = (uint64_t)k;
+        spartan_frame(tr, "log-instance-capacity", &kk, 8);
+        u128_put(b.pmin, v);
+        spartan_frame(tr, "prime-min", v, 16);
+        u128_put(b.pmax, v);
+        spartan_frame(tr, "prime-max", v, 16);
+    }
+    u128 q;
+    {
+        std::string err;
+        if (!sample_prime_context(tr, b.pmin, b.pmax, 128, &q, &err)) {
+            set_error("sha256 chain: " + err);
+            return BITZ_ERR_DEVICE;
+        }
+        uint8_t v[16];
+        u128_put(q, v);
+        spartan_frame(tr, "prime-q", v, 16);
+        spartan_frame(tr, "runtime-field-modulus", v, 16);
+        spartan_frame(tr, "projected-linear-relation", rel.digest, 32);
+    }
+    ModN F;
+    F.set(q);
+    auto point = [&](const char* domain, int vars, std::vector<u128>* out) -> bool {
+        spartan_frame(tr, "challenge-domain", domain, strlen(domain));
+        out->resize(vars);
+        for (auto& v : *out)
+            if (!spartan_squeeze(tr, q, &v)) return false;
+        return true;
+    };
+    std::vector<u128> xi, eta, a0, lam;
+    if (!point(kLocalPointDomain, kLocalVars, &xi) || !point(kInstancePointDomain, k, &eta) ||
+        !point(kConstantOneDomain, 1, &a0) || !point(kPublicIoDomain, kPublicBits + 1, &lam)) {
+        set_error("sha256 chain: sampling exhausted");
+        return BITZ_ERR_DEVICE;
+    }
+    const u128 alpha0 = a0[0], alpha_pub = lam.back();
+    lam.pop_back();
+    phase.mark("chain_batching");
+    // beta = C^T eq(xi) and the rank-one coefficients (chain.rs:1080-1130)
+    std::vector<u128> coef(kChainHBar, 0);
+    {
+        const std::vector<u128> ex = eq_le(F, xi);
+        for (int c = 0; c < kChainHBar; c++) {
+            u128 acc = 0;
+            for (uint32_t e = rel.c_off[c]; e < rel.c_off[c + 1]; e++)
+                acc = addmod(acc, F.mul(ex[rel.c_rows[e]], modq(F, rel.c_vals[e])), q);
+            coef[c] = acc;
+        }
+        coef[0] = addmod(coef[0], alpha0, q);
+        for (int p = 0; p < kPublicBits; p++) {
+            const int c = public_h_column(p);
+            coef[c] = addmod(coef[c], F.mul(alpha_pub, lam[p]), q);
+        }
+    }
+    // mu = alpha0 sum_i u_i + alpha_pub sum_i u_i sum_p lambda_p bit_p (chain.rs:1131-1160)
+    u128 mu;
+    {
+        const std::vector<u128> u = eq_le(F, eta);
+        std::vector<u128> tab((size_t)96 * 256);
+        for (int t8 = 0; t8 < 96; t8++) {
+            tab[t8 * 256] = 0;
+            for (int v = 1; v < 256; v++) {
+                const int bit = __builtin_ctz(v);
+                tab[t8 * 256 + v] = addmod(tab[t8 * 256 + (v & (v - 1))], lam[8 * t8 + bit], q);
+            }
+        }
+        u128 usum = 0, acc = 0;
+        for (uint64_t i = 0; i < n; i++) {
+            usum = addmod(usum, u[i], q);
+            u128 val = 0;
+            for (int ws = 0; ws < 24; ws++) {
+                const uint32_t word = ws < 16 ? blocks[i * 16 + ws] : (i + 1 == n ? digest[ws - 16] : 0u);
+                for (int by = 0; by < 4; by++) val = addmod(val, tab[(4 * ws + by) * 256 + ((word >> (8 * by)) & 255)], q);
+            }
+            acc = addmod(acc, F.mul(u[i], val), q);
+        }
+        mu = addmod(F.mul(alpha0, usum), F.mul(alpha_pub, acc), q);
+    }
+    // chain_product_opening_claim (chain.rs:1201-1251)
+    const std::vector<u128> roww(eq_le(F, std::vector<u128>(eta.begin(), eta.begin() + b.th)));
+    const std::vector<u128> high(eq_le(F, std::vector<u128>(eta.begin() + b.th, eta.end())));
+    const uint64_t H = high.size(), ncols = 1ull << b.sh;
+    std::vector<u128> colw(ncols, 0);
+    for (uint64_t c = 0; c < std::min<uint64_t>(ncols, (uint64_t)kChainHBar * H); c++)
+        colw[c] = F.mul(coef[c / H], high[c % H]);
+    // the claim frame (linear.rs:324-368)
+    {
+        Blake3 hd;
+        std::vector<uint8_t> o;
+        put_str(&o, kClaimDomain);
+        o.insert(o.end(), binding, binding + 32);
+        put64(&o, xi.size());
+        for (u128 v : xi) put128(&o, v);
+        put64(&o, eta.size());
+        for (u128 v : eta) put128(&o, v);
+        put128(&o, alpha0);
+        put64(&o, lam.size());
+        for (u128 v : lam) put128(&o, v);
+        put128(&o, alpha_pub);
+        put64(&o, roww.size());
+        for (u128 v : roww) put128(&o, v);
+        hd.update(o.data(), o.size());
+        o.clear();
+        put64(&o, colw.size());
+        hd.update(o.data(), o.size());
+        hd.update(colw.data(), colw.size() * 16);
+        o.clear();
+        put128(&o, mu);
+        hd.update(o.data(), o.size());
+        uint8_t dg[32];
+        hd.finalize_xof(dg, 32);
+        spartan_frame(tr, kClaimTag, dg, 32);
+    }
+    uint8_t tag[32];
+    tr.challenge(tag, 16);
+    tr.challenge(tag + 16, 16);
+    if (state_out) tr.digest(state_out);
+    phase.mark("chain_open");
+    // the virtual opening: its statement (virt.rs:132-150), then the claim
+    std::vector<std::vector<uint8_t>> lead(4);
+    put_str(&lead[0], kVirtualLabel);
+    lead[1].assign(root, root + 32);
+    {
+        std::vector<uint8_t>& o = lead[2];
+        put64(&o, b.th);
+        put64(&o, b.sh);
+        put128(&o, q);
+        uint8_t g[16];
+        const hgf gen = bitz_generator();
+        memcpy(g, &gen, 16);
+        o.insert(o.end(), g, g + 16);
+        put64(&o, b.tf);
+        put64(&o, b.sf);
+        o.insert(o.end(), b.map_digest, b.map_digest + 32);
+        const uint64_t pcs40[5] = {(uint64_t)(b.tf + b.sf), kRate, kBatch, 0, 1};
+        for (uint64_t v : pcs40) put64(&lead[3], v);
+    }
+    DevBuf droww, dcolw;
+    if (!droww.alloc(roww.size() * 16) || !dcolw.alloc(colw.size() * 16)) return BITZ_ERR_ALLOC;
+    BITZ_HIP(hipMemcpyAsync(droww.ptr, roww.data(), roww.size() * 16, hipMemcpyHostToDevice, ctx->stream));
+    BITZ_HIP(hipMemcpyAsync(dcolw.ptr, colw.data(), colw.size() * 16, hipMemcpyHostToDevice, ctx->stream));
+    Transposer T{ctx, &b, &cd};
+    rc = open_virtual_claim(ctx, h.get(), kVirtualSession, tag, lead, D.ptr, b.th, b.sh, q, droww.ptr, dcolw.ptr, mu,
+                            transpose_cb, &T, r0, proof);
+    BITZ_HIP(hipStreamSynchronize(ctx->stream));  // roww, colw (host vectors) copied before they go
+    return rc;
+}
+
+}  // namespace
+}  // namespace bitz
+
+extern "C" int bitz_prove_sha256_chain(bitz_ctx* ctx, const uint32_t* blocks, uint32_t log_n,
+                                       const uint32_t* initial_state, uint8_t root[32], uint32_t digest[8],
+                                       uint8_t state[32], uint8_t* proof, uint64_t capacity, uint64_t* proof_len) {
+    BITZ_REQUIRE(ctx && blocks && root && digest && proof_len, BITZ_ERR_ARG, "bitz_prove_sha256_chain: null argument");
+    BITZ_HIP(hipSetDevice(ctx->device));
+    bitz::PoolScope scope(&ctx->pool);
+    std::vector<uint8_t> bytes;
+    uint8_t st[32];
+    int rc = bitz::prove_chain(ctx, blocks, (int)log_n, initial_state, root, digest, st, &bytes);
+    if (rc) return rc;
+    if (state) memcpy(state, st, 32);
+    *proof_len = bytes.size();
+    BITZ_REQUIRE(proof && capacity >= bytes.size(), BITZ_ERR_ARG, "bitz_prove_sha256_chain: proof buffer too small");
+    memcpy(proof, bytes.data(), bytes.size());
+    return BITZ_OK;
+}

@@ -1204,6 +1204,12 @@ __global__ void k_mul_elem(const uint4* __restrict__ a, const uint4* __restrict_
 }
 
 // v[x] *= b[x] (each thread reads and writes only its own index)
+// out[x] = bit x of the flat bit stream (the committed bits as one column)
+__global__ void k_bits_to_gf(const uint64_t* __restrict__ bits, uint64_t n, uint4* __restrict__ out) {
+    for (uint64_t x = grid_tid(); x < n; x += grid_threads())
+        gf_store_nt(&out[x], gf128{{(uint32_t)((bits[x >> 6] >> (x & 63)) & 1ull), 0u, 0u, 0u}});
+}
+
 __global__ void k_mul_inplace(uint4* v, const uint4* __restrict__ b, uint64_t n) {
     for (uint64_t x = grid_tid(); x < n; x += grid_threads()) gf_store(&v[x], gf_mul(gf_load(&v[x]), gf_load(&b[x])));
 }
@@ -2357,6 +2363,12 @@ int mul_elem(bitz_ctx* ctx, const void* a, const void* b, uint64_t n, void* out,
     else
         hipLaunchKernelGGL(k_mul_elem, dim3(blocks_for(n, 4)), dim3(kT), 0, ctx->stream, (const uint4*)a,
                            (const uint4*)b, n, (uint4*)out);
+    BITZ_LAUNCH_CHECK();
+    return BITZ_OK;
+}
+
+int bits_to_gf(bitz_ctx* ctx, const uint64_t* bits, uint64_t n, void* out) {
+    hipLaunchKernelGGL(k_bits_to_gf, dim3(blocks_for(n, 4)), dim3(kT), 0, ctx->stream, bits, n, (uint4*)out);
     BITZ_LAUNCH_CHECK();
     return BITZ_OK;
 }

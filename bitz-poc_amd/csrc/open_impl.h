@@ -132,6 +132,8 @@ int dot(bitz_ctx* ctx, const void* a, const void* b, uint64_t n, void* out_dev);
 int mul_elem(bitz_ctx* ctx, const void* a, const void* b, uint64_t n, void* out, int plus_one);
 int mul_inplace(bitz_ctx* ctx, void* v, const void* b, uint64_t n);
 int scale(bitz_ctx* ctx, const void* a, uint64_t n, const uint8_t c[16], void* out);
+// out[x] = bit x of a flat bit stream (2^k bits as Gf128 0 or 1)
+int bits_to_gf(bitz_ctx* ctx, const uint64_t* bits, uint64_t n, void* out);
 
 // commit pipeline (commit.hip)
 // One GKR round's Fiat-Shamir step on the device, after a sums launch whose

@@ -903,6 +903,17 @@ struct Prover {
     int open_claim(InnerTranscript& it, PhaseClock& phase, u128 q, DevBuf& colw, u128 claimed,
                    const std::vector<uint8_t>& fold_bytes, const std::vector<G>& ood_point, G ood_y,
                    bool has_r0_nonce, uint64_t r0_nonce, std::vector<uint8_t>* proof_out);
+    // the fold messages and the GKR over the bit grid `bits` (column-major
+    // rows, this->t x this->s, row weights in roww): the factored
+    // inner-product claim u1 (2^t), u2 (2^s), target
+    int reduce_claim(InnerTranscript& it, PhaseClock& phase, const void* bits, const std::vector<uint8_t>& fold_bytes,
+                     DevBuf& u1, DevBuf& u2, G* ip_target);
+    // Pcs::prove_lin_rows of sum u1[b] u2[c] bit(c, b) = target over the
+    // committed bits seen as the grid (lt, ls): (t, s) as committed, or
+    // (m, 0), one column of every committed bit
+    int prove_lin(InnerTranscript& it, PhaseClock& phase, DevBuf& u1, DevBuf& u2, int lt, int ls, G ip_target,
+                  bool bind, const std::vector<G>& ood_point, G ood_y, bool has_r0_nonce, uint64_t r0_nonce,
+                  std::vector<uint8_t>* proof_out);
 };
 
 // Ligerito challenger over the inner transcript (src/bitz/pcs.rs:1062-1175)
@@ -1081,10 +1092,6 @@ int Prover::open_claim(InnerTranscript& it, PhaseClock& phase, u128 q, DevBuf& c
                        bool has_r0_nonce, uint64_t r0_nonce, std::vector<uint8_t>* proof_out) {
     int rc;
     const uint64_t nrows = 1ull << t, ncols = 1ull << s;
-    const uint32_t r0bits = has_r0_nonce ? 1u : 0u;
-    uint8_t gb[16], yb[16];
-    gput(g, gb);
-    gput(ood_y, yb);
     phase.mark("fold_round");
     // ---------------- BitZ opening on the inner transcript ----------------
     it.public_message(h->root, 32);
@@ -1107,6 +1114,18 @@ int Prover::open_claim(InnerTranscript& it, PhaseClock& phase, u128 q, DevBuf& c
         }
         if ((rc = public_weights(it, roww.ptr, nrows, colw.ptr, ncols, claim_g))) return rc;
     }
+    DevBuf u1, u2;
+    G ip_target;
+    if ((rc = reduce_claim(it, phase, h->packed.ptr, fold_bytes, u1, u2, &ip_target))) return rc;
+    return prove_lin(it, phase, u1, u2, t, s, ip_target, true, ood_point, ood_y, has_r0_nonce, r0_nonce, proof_out);
+}
+
+int Prover::reduce_claim(InnerTranscript& it, PhaseClock& phase, const void* bits,
+                         const std::vector<uint8_t>& fold_bytes, DevBuf& u1, DevBuf& u2, G* ip_target) {
+    int rc;
+    const uint64_t nrows = 1ull << t, ncols = 1ull << s;
+    uint8_t gb[16];
+    gput(g, gb);
     // fold round (src/bitz/fold.rs:237-263): one prover message per column
     it.prover_messages(fold_bytes.data(), ncols, 16);
     std::vector<G> zeta(s);
@@ -1120,7 +1139,7 @@ int Prover::open_claim(InnerTranscript& it, PhaseClock& phase, u128 q, DevBuf& c
     // packed columns for the leaf layer and the combined rows
     uint64_t groups = (ncols + 63) / 64;
     if (!packed.alloc((groups << t) * 8)) return BITZ_ERR_ALLOC;
-    if ((rc = pack_columns(ctx, (const uint64_t*)h->packed.ptr, t, s, (uint64_t*)packed.ptr))) return rc;
+    if ((rc = pack_columns(ctx, (const uint64_t*)bits, t, s, (uint64_t*)packed.ptr))) return rc;
     // GKR (reduce.rs:100-114)
     std::vector<G> leaf_point;
     G leaf_claim;
@@ -1131,14 +1150,24 @@ int Prover::open_claim(InnerTranscript& it, PhaseClock& phase, u128 q, DevBuf& c
     // exit_from_terminal (reduce.rs:28-58)
     std::vector<G> alpha_c(leaf_point.begin(), leaf_point.begin() + s);
     std::vector<G> alpha_b(leaf_point.begin() + s, leaf_point.end());
-    G ip_target = hgf_add(leaf_claim, gone());
-    DevBuf u1, u2, a2, w2;
+    *ip_target = hgf_add(leaf_claim, gone());
     if ((rc = eq_dev(alpha_b, tmp0))) return rc;
     if (!u1.alloc(nrows * 16)) return BITZ_ERR_ALLOC;
     if ((rc = mul_elem(ctx, rowimg.ptr, tmp0.ptr, nrows, u1.ptr, 1))) return rc;
-    if ((rc = eq_dev(alpha_c, u2))) return rc;
-    // prove_lin / InnerProduct statement (src/bitz/pcs.rs:362-372, 878-888)
-    {
+    return eq_dev(alpha_c, u2);
+}
+
+int Prover::prove_lin(InnerTranscript& it, PhaseClock& phase, DevBuf& u1, DevBuf& u2, int lt, int ls, G ip_target,
+                      bool bind, const std::vector<G>& ood_point, G ood_y, bool has_r0_nonce, uint64_t r0_nonce,
+                      std::vector<uint8_t>* proof_out) {
+    int rc;
+    const uint64_t nrows = 1ull << lt, ncols = 1ull << ls;
+    const uint32_t r0bits = has_r0_nonce ? 1u : 0u;
+    uint8_t yb[16];
+    gput(ood_y, yb);
+    // prove_lin / InnerProduct statement (src/bitz/pcs.rs:362-372, 878-888),
+    // bound here unless the caller's statement already covers the claim
+    if (bind) {
         it.public_message(std::string("bitz/pcs/bit-inner-product/v2"));
         it.public_message(h->root, 32);
         uint64_t pcs40[5] = {(uint64_t)m, (uint64_t)h->log_inv_rate, (uint64_t)h->log_batch, 0, 1};
@@ -1148,24 +1177,30 @@ int Prover::open_claim(InnerTranscript& it, PhaseClock& phase, u128 q, DevBuf& c
         if ((rc = public_weights(it, u1.ptr, nrows, u2.ptr, ncols, ip_target))) return rc;
     }
     it.public_message(std::string("bitz/pcs/inner-product-sumcheck/v1"));
-    // row rounds on the column-combined table
-    DevBuf comb;
+    // row rounds on the column-combined table (a single column: the bits)
+    DevBuf comb, a2, w2;
     if (!comb.alloc(nrows * 16) || !a2.alloc(nrows * 16) || !w2.alloc(nrows * 16)) return BITZ_ERR_ALLOC;
-    if ((rc = xi_combined(ctx, (const uint64_t*)packed.ptr, t, s, u2.ptr, comb.ptr))) return rc;
+    if (ls > 0) {
+        if ((rc = xi_combined(ctx, (const uint64_t*)packed.ptr, lt, ls, u2.ptr, comb.ptr))) return rc;
+    } else {
+        if ((rc = bits_to_gf(ctx, (const uint64_t*)h->packed.ptr, nrows, comb.ptr))) return rc;
+    }
     std::vector<G> sc_point;
     G target = ip_target;
     if ((rc = sumcheck_rounds(it, comb, u1, a2, w2, nrows, &target, sc_point))) return rc;
-    uint8_t rs_bytes[16];
-    gput(final_w, rs_bytes);
-    // column rounds on the row-folded table
-    if ((rc = eq_dev(sc_point, tmp0))) return rc;
-    DevBuf folded, cols, f2, c2;
-    if (!folded.alloc(ncols * 16) || !cols.alloc(ncols * 16) || !f2.alloc(ncols * 16) || !c2.alloc(ncols * 16))
-        return BITZ_ERR_ALLOC;
-    if ((rc = fold_rows_eq(ctx, (const uint64_t*)h->packed.ptr, t, s, tmp0.ptr, folded.ptr))) return rc;
-    if ((rc = scale(ctx, u2.ptr, ncols, rs_bytes, cols.ptr))) return rc;
-    if ((rc = sumcheck_rounds(it, folded, cols, f2, c2, ncols, &target, sc_point))) return rc;
     uint8_t evb[32];
+    if (ls > 0) {
+        uint8_t rs_bytes[16];
+        gput(final_w, rs_bytes);
+        // column rounds on the row-folded table
+        if ((rc = eq_dev(sc_point, tmp0))) return rc;
+        DevBuf folded, cols, f2, c2;
+        if (!folded.alloc(ncols * 16) || !cols.alloc(ncols * 16) || !f2.alloc(ncols * 16) || !c2.alloc(ncols * 16))
+            return BITZ_ERR_ALLOC;
+        if ((rc = fold_rows_eq(ctx, (const uint64_t*)h->packed.ptr, lt, ls, tmp0.ptr, folded.ptr))) return rc;
+        if ((rc = scale(ctx, u2.ptr, ncols, rs_bytes, cols.ptr))) return rc;
+        if ((rc = sumcheck_rounds(it, folded, cols, f2, c2, ncols, &target, sc_point))) return rc;
+    }
     gput(final_a, evb);
     gput(final_w, evb + 16);
     G evaluation = gfrom(evb);
@@ -2352,6 +2387,11 @@ int make_prover(bitz_ctx* ctx, const bitz_commit_handle* h, int target_bits, Pro
 }  // namespace
 
 // ---- the BitZ opening of a linear claim, for composed proofs ----
+hgf bitz_generator() {
+    static const G gen = smallest_generator();
+    return gen;
+}
+
 int round0_prove(bitz_ctx* ctx, const bitz_commit_handle* h, Transcript& outer, Round0Claim* r0) {
     Prover p;
     int rc = make_prover(ctx, h, 100, &p);
@@ -2401,6 +2441,46 @@ int open_linear_claim(bitz_ctx* ctx, const bitz_commit_handle* h, const char* se
     if (lead_len) it.public_message(lead, lead_len);
     PhaseClock phase(ctx);
     return p.open_claim(it, phase, q, colw, claimed, fold_bytes, r0.point, r0.y, r0.has_nonce, r0.nonce, proof);
+}
+
+int open_virtual_claim(bitz_ctx* ctx, const bitz_commit_handle* h, const char* session, const uint8_t tag[32],
+                       const std::vector<std::vector<uint8_t>>& lead, const void* grid, int th, int sh, u128 q,
+                       const void* roww, const void* colw_in, u128 claimed, TransposeFn transpose, void* user,
+                       const Round0Claim& r0, std::vector<uint8_t>* proof) {
+    Prover p;
+    int rc = make_prover(ctx, h, 100, &p);
+    if (rc) return rc;
+    BITZ_REQUIRE(claimed < q, BITZ_ERR_ARG, "BitZ claim: value not reduced");
+    BITZ_REQUIRE(th >= 7 && sh >= 1 && th + sh <= 35, BITZ_ERR_SHAPE, "virtual opening: derived grid shape");
+    // the fold and the GKR run over the derived grid
+    p.t = th;
+    p.s = sh;
+    const uint64_t nrows = 1ull << th, ncols = 1ull << sh;
+    DevBuf colw;
+    if (!p.roww.alloc(nrows * 16) || !colw.alloc(ncols * 16) || !p.folds.alloc(ncols * 16)) return BITZ_ERR_ALLOC;
+    BITZ_HIP(hipMemcpyAsync(p.roww.ptr, roww, nrows * 16, hipMemcpyDeviceToDevice, ctx->stream));
+    BITZ_HIP(hipMemcpyAsync(colw.ptr, colw_in, ncols * 16, hipMemcpyDeviceToDevice, ctx->stream));
+    std::vector<uint8_t> fold_bytes(ncols * 16);
+    if ((rc = fold_columns(ctx, (const uint64_t*)grid, th, sh, p.roww.ptr, p.folds.ptr))) return rc;
+    if ((rc = p.download(p.folds.ptr, fold_bytes.data(), fold_bytes.size()))) return rc;
+    InnerTranscript it(session, tag, 32);
+    for (auto& msg : lead) it.public_message(msg.data(), msg.size());
+    {
+        uint8_t b[16];
+        u128_put(claimed, b);
+        if ((rc = p.public_weights(it, p.roww.ptr, nrows, colw.ptr, ncols, gfrom(b)))) return rc;
+    }
+    PhaseClock phase(ctx);
+    phase.mark("fold_round");
+    DevBuf u1, u2, wf, one;
+    G target;
+    if ((rc = p.reduce_claim(it, phase, grid, fold_bytes, u1, u2, &target))) return rc;
+    phase.mark("transpose");
+    if (!wf.alloc((16ull << p.m))) return BITZ_ERR_ALLOC;
+    if ((rc = transpose(user, u1.ptr, u2.ptr, wf.ptr))) return rc;
+    if ((rc = p.upload_gf(std::vector<G>{gone()}, one))) return rc;
+    phase.mark("inner_sumcheck");
+    return p.prove_lin(it, phase, wf, one, p.m, 0, target, false, r0.point, r0.y, r0.has_nonce, r0.nonce, proof);
 }
 
 int verify_linear_claim(bitz_ctx* ctx, const uint8_t root[32], int t, int s, int r0, int k0, const char* session,

@@ -1240,6 +1240,12 @@ int verify_mul_u64(bitz_ctx* ctx, const uint8_t root[32], uint64_t n, int split,
 }
 
 }  // namespace
+
+void spartan_frame(Transcript& tr, const char* tag, const void* payload, size_t len) {
+    spartan_message(tr, tag, payload, len);
+}
+bool spartan_squeeze(Transcript& tr, u128 q, u128* out) { return squeeze(tr, q, out); }
+
 }  // namespace bitz
 
 extern "C" int bitz_prove_mul_u64(bitz_ctx* ctx, const void* x, const void* y, const void* lo, const void* hi,

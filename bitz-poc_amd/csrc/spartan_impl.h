@@ -34,5 +34,28 @@ int verify_linear_claim(bitz_ctx* ctx, const uint8_t root[32], int t, int s, int
                         const void* colw, u128 claimed, const std::vector<hgf>& opt, hgf ood_y,
                         const uint8_t* narg, uint64_t narg_len, const uint8_t* hints, uint64_t hints_len);
 bool round0_pow_valid(const uint8_t seed[32], uint64_t nonce, uint32_t bits);
+// the BitZ generator g (smallest_generator, src/pcs.rs:85-102)
+hgf bitz_generator();
+
+// absorb_spartan_message (src/piop/spartan/mod.rs:349-358): the frame
+// domain, the tag and the payload, each behind its u64 length
+void spartan_frame(Transcript& tr, const char* tag, const void* payload, size_t len);
+// squeeze_field (mod.rs:371-384): sample_below(q) over 256 draws, then
+// absorbed as a field element; false when sampling is exhausted
+bool spartan_squeeze(Transcript& tr, u128 q, u128* out);
+
+// BitZProver::prove_virtual (src/bitz/virt.rs:214-301) after its statement:
+// on an inner transcript that starts with the session, the fork tag and the
+// `lead` public messages, the claim sum roww[b] colw[c] d(c, b) = claimed
+// (mod q) over the derived grid `grid` (th x sh, column-major rows), whose
+// folds and GKR run there; `transpose` turns the reduced factored claim
+// (u1: 2^th, u2: 2^sh Gf128) into one weight per committed cell (wf:
+// 2^(t+s) Gf128), opened as a single-column inner product on the committed
+// bits of h. proof = BitZOpeningProof::to_bytes.
+using TransposeFn = int (*)(void* user, const void* u1, const void* u2, void* wf);
+int open_virtual_claim(bitz_ctx* ctx, const bitz_commit_handle* h, const char* session, const uint8_t tag[32],
+                       const std::vector<std::vector<uint8_t>>& lead, const void* grid, int th, int sh, u128 q,
+                       const void* roww, const void* colw, u128 claimed, TransposeFn transpose, void* user,
+                       const Round0Claim& r0, std::vector<uint8_t>* proof);
 
 }  // namespace bitz

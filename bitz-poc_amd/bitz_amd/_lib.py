@@ -58,6 +58,7 @@ SIGNATURES = {
     "bitz_prove_mul_u64": (_I, [_P, _P, _P, _P, _P, _U64, C.c_int32, _P, _P, _P, _U64, _P]),
     "bitz_verify_mul_u64": (_I, [_P, _P, _U64, C.c_int32, _P, _U64, _P]),
     "bitz_prove_sha256_chain": (_I, [_P, _P, _U32, _P, _P, _P, _P, _P, _U64, _P]),
+    "bitz_verify_sha256_chain": (_I, [_P, _P, _U32, _P, _P, _P, _P, _U64, _P]),
     "bitz_gf_mul_dev": (_I, [_P, _P, _P, _P, _U64]),
     "bitz_gf_mul_const_dev": (_I, [_P, _P, _P, _P, _U64]),
     "bitz_blake3_leaves_dev": (_I, [_P, _P, _U64, _U64, _P]),

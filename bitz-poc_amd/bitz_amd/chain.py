@@ -45,3 +45,22 @@ def prove_chain(ctx, blocks, initial_state=None):
             continue
         _lib.check(rc, "bitz_prove_sha256_chain")
         return bytes(root), [int(x) for x in digest], bytes(state), bytes(buf)[:plen.value]
+
+
+def verify_chain(ctx, blocks, digest, root: bytes, proof: bytes, initial_state=None) -> bytes:
+    """verify_sha256_chain (chain.rs:850-872): the statement (blocks, digest,
+    initial state), the root and the proof; raises BitzError when rejected,
+    returns the verifier's outer transcript state digest."""
+    b = np.ascontiguousarray(np.asarray(blocks, dtype=np.uint32))
+    n = b.shape[0]
+    k = n.bit_length() - 1
+    if b.ndim != 2 or b.shape[1] != 16 or n != 1 << k:
+        raise ValueError("blocks must be (2^k, 16) u32 words")
+    d = np.ascontiguousarray(np.asarray(digest, dtype=np.uint32))
+    iv = None if initial_state is None else np.ascontiguousarray(np.asarray(initial_state, dtype=np.uint32))
+    state = (C.c_uint8 * 32)()
+    pb = (C.c_uint8 * max(len(proof), 1)).from_buffer_copy(proof or b"\0")
+    _lib.call("bitz_verify_sha256_chain", ctx.ptr, C.c_void_p(b.ctypes.data), k,
+              C.c_void_p(iv.ctypes.data) if iv is not None else None, C.c_void_p(d.ctypes.data), bytes(root), pb,
+              len(proof), state)
+    return bytes(state)

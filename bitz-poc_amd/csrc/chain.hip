@@ -372,6 +372,228 @@ int transpose_cb(void* user, const void* u1, const void* u2, void* wf) {
     return BITZ_OK;
 }
 
+// Round 0 on the outer transcript (prover: prove it; verifier: check the
+// proof's nonce and value), returning BITZ_OK or an error
+using Round0Fn = int (*)(void* user, Transcript& tr);
+
+struct Outer {
+    u128 q = 0, mu = 0;
+    std::vector<u128> roww, colw;  // the rank-one claim's weights (canonical)
+    uint8_t tag[32];
+};
+
+// prove_linear / verify_linear's outer transcript for the chain
+// (linear.rs:398-558, 586-705) from the statement frames to the fork tag
+int chain_outer(const Batch& b, const uint32_t* blocks, const uint32_t* digest, const uint8_t root[32],
+                const LigConfig& cfg, Transcript& tr, Round0Fn round0, void* user, Outer* o) {
+    int rc;
+    const Relation& rel = *b.rel;
+    const uint64_t n = b.n;
+    const int k = b.k;
+    const std::array<uint32_t, 8>& iv = rel.initial_state;
+    // chain_statement_binding (chain.rs:1364-1383)
+    uint8_t stmt[32], binding[32];
+    {
+        std::vector<uint8_t> o;
+        put_str(&o, kStatementDomain);
+        for (uint32_t w : iv) put32(&o, w);
+        put64(&o, n);
+        for (uint64_t i = 0; i < n * 16; i++) put32(&o, blocks[i]);
+        for (int i = 0; i < 8; i++) put32(&o, digest[i]);
+        blake3_hash_host(o.data(), o.size(), stmt);
+    }
+    // chain_assignment_binding (chain.rs:1385-1418)
+    {
+        std::vector<uint8_t> o;
+        put_str(&o, kAssignmentDomain);
+        o.insert(o.end(), root, root + 32);
+        o.insert(o.end(), rel.digest, rel.digest + 32);
+        o.insert(o.end(), b.map_digest, b.map_digest + 32);
+        o.insert(o.end(), stmt, stmt + 32);
+        const uint64_t vals[11] = {(uint64_t)(b.tf + b.sf), kRate, kBatch, n, (uint64_t)k, (uint64_t)b.th,
+                                   (uint64_t)b.sh, 1, (uint64_t)b.tf, (uint64_t)b.sf, 1};
+        for (uint64_t v : vals) put64(&o, v);
+        o.push_back(0);  // profile code of the commitment
+        o.push_back(1);  // BLAKE3
+        security_bytes(b, cfg.round0_grinding_bits, &o);
+        ligerito_bytes(cfg, &o);
+        blake3_hash_host(o.data(), o.size(), binding);
+    }
+    spartan_frame(tr, "protocol", kProtocolDomain, strlen(kProtocolDomain));
+    spartan_frame(tr, "integer-relation", rel.digest, 32);
+    spartan_frame(tr, "chained-boolean-map", b.map_digest, 32);
+    {
+        const char* tags[6] = {"instance-vars", "instance-count", "assignment-row-vars", "assignment-column-vars",
+                               "source-row-vars", "source-column-vars"};
+        const uint64_t vals[6] = {(uint64_t)k, n, (uint64_t)b.th, (uint64_t)b.sh, (uint64_t)b.tf, (uint64_t)b.sf};
+        for (int i = 0; i < 6; i++) spartan_frame(tr, tags[i], &vals[i], 8);
+    }
+    spartan_frame(tr, "public-sha256-chain", stmt, 32);
+    spartan_frame(tr, "assignment-oracle", binding, 32);
+    tr.absorb_slice(kPolicyLabel);
+    {
+        uint8_t dg[32];
+        ligerito_digest(cfg, dg);
+        tr.absorb_slice(dg, 32);
+    }
+    if ((rc = round0(user, tr))) return rc;
+    // the runtime prime and the relation frames
+    {
+        uint8_t v[16];
+        spartan_frame(tr, "prime-domain", kPrimeDomain, strlen(kPrimeDomain));
+        const uint64_t kk = (uint64_t)k;
+        spartan_frame(tr, "log-instance-capacity", &kk, 8);
+        u128_put(b.pmin, v);
+        spartan_frame(tr, "prime-min", v, 16);
+        u128_put(b.pmax, v);
+        spartan_frame(tr, "prime-max", v, 16);
+    }
+    u128 q;
+    {
+        std::string err;
+        if (!sample_prime_context(tr, b.pmin, b.pmax, 128, &q, &err)) {
+            set_error("sha256 chain: " + err);
+            return BITZ_ERR_VERIFY;
+        }
+        uint8_t v[16];
+        u128_put(q, v);
+        spartan_frame(tr, "prime-q", v, 16);
+        spartan_frame(tr, "runtime-field-modulus", v, 16);
+        spartan_frame(tr, "projected-linear-relation", rel.digest, 32);
+    }
+    ModN F;
+    F.set(q);
+    auto point = [&](const char* domain, int vars, std::vector<u128>* out) -> bool {
+        spartan_frame(tr, "challenge-domain", domain, strlen(domain));
+        out->resize(vars);
+        for (auto& v : *out)
+            if (!spartan_squeeze(tr, q, &v)) return false;
+        return true;
+    };
+    std::vector<u128> xi, eta, a0, lam;
+    if (!point(kLocalPointDomain, kLocalVars, &xi) || !point(kInstancePointDomain, k, &eta) ||
+        !point(kConstantOneDomain, 1, &a0) || !point(kPublicIoDomain, kPublicBits + 1, &lam)) {
+        set_error("sha256 chain: sampling exhausted");
+        return BITZ_ERR_VERIFY;
+    }
+    const u128 alpha0 = a0[0], alpha_pub = lam.back();
+    lam.pop_back();
+    // beta = C^T eq(xi) and the rank-one coefficients (chain.rs:1080-1130)
+    std::vector<u128> coef(kChainHBar, 0);
+    {
+        const std::vector<u128> ex = eq_le(F, xi);
+        for (int c = 0; c < kChainHBar; c++) {
+            u128 acc = 0;
+            for (uint32_t e = rel.c_off[c]; e < rel.c_off[c + 1]; e++)
+                acc = addmod(acc, F.mul(ex[rel.c_rows[e]], modq(F, rel.c_vals[e])), q);
+            coef[c] = acc;
+        }
+        coef[0] = addmod(coef[0], alpha0, q);
+        for (int p = 0; p < kPublicBits; p++) {
+            const int c = public_h_column(p);
+            coef[c] = addmod(coef[c], F.mul(alpha_pub, lam[p]), q);
+        }
+    }
+    // mu = alpha0 sum_i u_i + alpha_pub sum_i u_i sum_p lambda_p bit_p (chain.rs:1131-1160)
+    u128 mu;
+    {
+        const std::vector<u128> u = eq_le(F, eta);
+        std::vector<u128> tab((size_t)96 * 256);
+        for (int t8 = 0; t8 < 96; t8++) {
+            tab[t8 * 256] = 0;
+            for (int v = 1; v < 256; v++) {
+                const int bit = __builtin_ctz(v);
+                tab[t8 * 256 + v] = addmod(tab[t8 * 256 + (v & (v - 1))], lam[8 * t8 + bit], q);
+            }
+        }
+        u128 usum = 0, acc = 0;
+        for (uint64_t i = 0; i < n; i++) {
+            usum = addmod(usum, u[i], q);
+            u128 val = 0;
+            for (int ws = 0; ws < 24; ws++) {
+                const uint32_t word = ws < 16 ? blocks[i * 16 + ws] : (i + 1 == n ? digest[ws - 16] : 0u);
+                for (int by = 0; by < 4; by++) val = addmod(val, tab[(4 * ws + by) * 256 + ((word >> (8 * by)) & 255)], q);
+            }
+            acc = addmod(acc, F.mul(u[i], val), q);
+        }
+        mu = addmod(F.mul(alpha0, usum), F.mul(alpha_pub, acc), q);
+    }
+    // chain_product_opening_claim (chain.rs:1201-1251)
+    o->roww = eq_le(F, std::vector<u128>(eta.begin(), eta.begin() + b.th));
+    const std::vector<u128>& roww = o->roww;
+    const std::vector<u128> high(eq_le(F, std::vector<u128>(eta.begin() + b.th, eta.end())));
+    const uint64_t H = high.size(), ncols = 1ull << b.sh;
+    o->colw.assign(ncols, 0);
+    std::vector<u128>& colw = o->colw;
+    for (uint64_t c = 0; c < std::min<uint64_t>(ncols, (uint64_t)kChainHBar * H); c++)
+        colw[c] = F.mul(coef[c / H], high[c % H]);
+    // the claim frame (linear.rs:324-368)
+    {
+        Blake3 hd;
+        std::vector<uint8_t> o;
+        put_str(&o, kClaimDomain);
+        o.insert(o.end(), binding, binding + 32);
+        put64(&o, xi.size());
+        for (u128 v : xi) put128(&o, v);
+        put64(&o, eta.size());
+        for (u128 v : eta) put128(&o, v);
+        put128(&o, alpha0);
+        put64(&o, lam.size());
+        for (u128 v : lam) put128(&o, v);
+        put128(&o, alpha_pub);
+        put64(&o, roww.size());
+        for (u128 v : roww) put128(&o, v);
+        hd.update(o.data(), o.size());
+        o.clear();
+        put64(&o, colw.size());
+        hd.update(o.data(), o.size());
+        hd.update(colw.data(), colw.size() * 16);
+        o.clear();
+        put128(&o, mu);
+        hd.update(o.data(), o.size());
+        uint8_t dg[32];
+        hd.finalize_xof(dg, 32);
+        spartan_frame(tr, kClaimTag, dg, 32);
+    }
+    o->q = q;
+    o->mu = mu;
+    tr.challenge(o->tag, 16);
+    tr.challenge(o->tag + 16, 16);
+    return BITZ_OK;
+}
+
+// the virtual statement's public messages (virt.rs:132-150): label, root,
+// claim parameters on the derived grid + committed shape + map digest, the
+// 40-byte Pcs frame; the claim follows
+std::vector<std::vector<uint8_t>> virtual_lead(const Batch& b, const uint8_t root[32], u128 q) {
+    std::vector<std::vector<uint8_t>> lead(4);
+    put_str(&lead[0], kVirtualLabel);
+    lead[1].assign(root, root + 32);
+    std::vector<uint8_t>& o = lead[2];
+    put64(&o, b.th);
+    put64(&o, b.sh);
+    put128(&o, q);
+    const hgf gen = bitz_generator();
+    put64(&o, gen.lo);
+    put64(&o, gen.hi);
+    put64(&o, b.tf);
+    put64(&o, b.sf);
+    o.insert(o.end(), b.map_digest, b.map_digest + 32);
+    const uint64_t pcs40[5] = {(uint64_t)(b.tf + b.sf), kRate, kBatch, 0, 1};
+    for (uint64_t v : pcs40) put64(&lead[3], v);
+    return lead;
+}
+
+struct ProveR0 {
+    bitz_ctx* ctx;
+    const bitz_commit_handle* h;
+    Round0Claim r0;
+};
+int prove_round0(void* user, Transcript& tr) {
+    ProveR0* P = (ProveR0*)user;
+    return round0_prove(P->ctx, P->h, tr, &P->r0);
+}
+
 int prove_chain(bitz_ctx* ctx, const uint32_t* blocks, int k, const uint32_t* initial_state, uint8_t root[32],
                 uint32_t digest_out[8], uint8_t state_out[32], std::vector<uint8_t>* proof) {
     std::array<uint32_t, 8> iv;
@@ -419,203 +641,135 @@ int prove_chain(bitz_ctx* ctx, const uint32_t* blocks, int k, const uint32_t* in
         BITZ_REQUIRE(err.empty(), BITZ_ERR_SHAPE, "sha256 chain: Ligerito configuration: " + err);
     }
     phase.mark("chain_statement");
-    // chain_statement_binding (chain.rs:1364-1383)
-    uint8_t stmt[32], binding[32];
-    {
-        std::vector<uint8_t> o;
-        put_str(&o, kStatementDomain);
-        for (uint32_t w : iv) put32(&o, w);
-        put64(&o, n);
-        for (uint64_t i = 0; i < n * 16; i++) put32(&o, blocks[i]);
-        for (int i = 0; i < 8; i++) put32(&o, digest[i]);
-        blake3_hash_host(o.data(), o.size(), stmt);
-    }
-    // chain_assignment_binding (chain.rs:1385-1418)
-    {
-        std::vector<uint8_t> o;
-        put_str(&o, kAssignmentDomain);
-        o.insert(o.end(), root, root + 32);
-        o.insert(o.end(), rel.digest, rel.digest + 32);
-        o.insert(o.end(), b.map_digest, b.map_digest + 32);
-        o.insert(o.end(), stmt, stmt + 32);
-        const uint64_t vals[11] = {(uint64_t)(b.tf + b.sf), kRate, kBatch, n, (uint64_t)k, (uint64_t)b.th,
-                                   (uint64_t)b.sh, 1, (uint64_t)b.tf, (uint64_t)b.sf, 1};
-        for (uint64_t v : vals) put64(&o, v);
-        o.push_back(0);  // profile code of the commitment
-        o.push_back(1);  // BLAKE3
-        security_bytes(b, cfg.round0_grinding_bits, &o);
-        ligerito_bytes(cfg, &o);
-        blake3_hash_host(o.data(), o.size(), binding);
-    }
     Transcript tr;
-    spartan_frame(tr, "protocol", kProtocolDomain, strlen(kProtocolDomain));
-    spartan_frame(tr, "integer-relation", rel.digest, 32);
-    spartan_frame(tr, "chained-boolean-map", b.map_digest, 32);
-    {
-        const char* tags[6] = {"instance-vars", "instance-count", "assignment-row-vars", "assignment-column-vars",
-                               "source-row-vars", "source-column-vars"};
-        const uint64_t vals[6] = {(uint64_t)k, n, (uint64_t)b.th, (uint64_t)b.sh, (uint64_t)b.tf, (uint64_t)b.sf};
-        for (int i = 0; i < 6; i++) spartan_frame(tr, tags[i], &vals[i], 8);
-    }
-    spartan_frame(tr, "public-sha256-chain", stmt, 32);
-    spartan_frame(tr, "assignment-oracle", binding, 32);
-    tr.absorb_slice(kPolicyLabel);
-    {
-        uint8_t dg[32];
-        ligerito_digest(cfg, dg);
-        tr.absorb_slice(dg, 32);
-    }
-    Round0Claim r0;
-    if ((rc = round0_prove(ctx, h.get(), tr, &r0))) return rc;
-    // the runtime prime and the relation frames
-    {
-        uint8_t v[16];
-        spartan_frame(tr, "prime-domain", kPrimeDomain, strlen(kPrimeDomain));
-        const uint64_t kk = (uint64_t)k;
-        spartan_frame(tr, "log-instance-capacity", &kk, 8);
-        u128_put(b.pmin, v);
-        spartan_frame(tr, "prime-min", v, 16);
-        u128_put(b.pmax, v);
-        spartan_frame(tr, "prime-max", v, 16);
-    }
-    u128 q;
-    {
-        std::string err;
-        if (!sample_prime_context(tr, b.pmin, b.pmax, 128, &q, &err)) {
-            set_error("sha256 chain: " + err);
-            return BITZ_ERR_DEVICE;
-        }
-        uint8_t v[16];
-        u128_put(q, v);
-        spartan_frame(tr, "prime-q", v, 16);
-        spartan_frame(tr, "runtime-field-modulus", v, 16);
-        spartan_frame(tr, "projected-linear-relation", rel.digest, 32);
-    }
-    ModN F;
-    F.set(q);
-    auto point = [&](const char* domain, int vars, std::vector<u128>* out) -> bool {
-        spartan_frame(tr, "challenge-domain", domain, strlen(domain));
-        out->resize(vars);
-        for (auto& v : *out)
-            if (!spartan_squeeze(tr, q, &v)) return false;
-        return true;
-    };
-    std::vector<u128> xi, eta, a0, lam;
-    if (!point(kLocalPointDomain, kLocalVars, &xi) || !point(kInstancePointDomain, k, &eta) ||
-        !point(kConstantOneDomain, 1, &a0) || !point(kPublicIoDomain, kPublicBits + 1, &lam)) {
-        set_error("sha256 chain: sampling exhausted");
-        return BITZ_ERR_DEVICE;
-    }
-    const u128 alpha0 = a0[0], alpha_pub = lam.back();
-    lam.pop_back();
-    phase.mark("chain_batching");
-    // beta = C^T eq(xi) and the rank-one coefficients (chain.rs:1080-1130)
-    std::vector<u128> coef(kChainHBar, 0);
-    {
-        const std::vector<u128> ex = eq_le(F, xi);
-        for (int c = 0; c < kChainHBar; c++) {
-            u128 acc = 0;
-            for (uint32_t e = rel.c_off[c]; e < rel.c_off[c + 1]; e++)
-                acc = addmod(acc, F.mul(ex[rel.c_rows[e]], modq(F, rel.c_vals[e])), q);
-            coef[c] = acc;
-        }
-        coef[0] = addmod(coef[0], alpha0, q);
-        for (int p = 0; p < kPublicBits; p++) {
-            const int c = public_h_column(p);
-            coef[c] = addmod(coef[c], F.mul(alpha_pub, lam[p]), q);
-        }
-    }
-    // mu = alpha0 sum_i u_i + alpha_pub sum_i u_i sum_p lambda_p bit_p (chain.rs:1131-1160)
-    u128 mu;
-    {
-        const std::vector<u128> u = eq_le(F, eta);
-        std::vector<u128> tab((size_t)96 * 256);
-        for (int t8 = 0; t8 < 96; t8++) {
-            tab[t8 * 256] = 0;
-            for (int v = 1; v < 256; v++) {
-                const int bit = __builtin_ctz(v);
-                tab[t8 * 256 + v] = addmod(tab[t8 * 256 + (v & (v - 1))], lam[8 * t8 + bit], q);
-            }
-        }
-        u128 usum = 0, acc = 0;
-        for (uint64_t i = 0; i < n; i++) {
-            usum = addmod(usum, u[i], q);
-            u128 val = 0;
-            for (int ws = 0; ws < 24; ws++) {
-                const uint32_t word = ws < 16 ? blocks[i * 16 + ws] : (i + 1 == n ? digest[ws - 16] : 0u);
-                for (int by = 0; by < 4; by++) val = addmod(val, tab[(4 * ws + by) * 256 + ((word >> (8 * by)) & 255)], q);
-            }
-            acc = addmod(acc, F.mul(u[i], val), q);
-        }
-        mu = addmod(F.mul(alpha0, usum), F.mul(alpha_pub, acc), q);
-    }
-    // chain_product_opening_claim (chain.rs:1201-1251)
-    const std::vector<u128> roww(eq_le(F, std::vector<u128>(eta.begin(), eta.begin() + b.th)));
-    const std::vector<u128> high(eq_le(F, std::vector<u128>(eta.begin() + b.th, eta.end())));
-    const uint64_t H = high.size(), ncols = 1ull << b.sh;
-    std::vector<u128> colw(ncols, 0);
-    for (uint64_t c = 0; c < std::min<uint64_t>(ncols, (uint64_t)kChainHBar * H); c++)
-        colw[c] = F.mul(coef[c / H], high[c % H]);
-    // the claim frame (linear.rs:324-368)
-    {
-        Blake3 hd;
-        std::vector<uint8_t> o;
-        put_str(&o, kClaimDomain);
-        o.insert(o.end(), binding, binding + 32);
-        put64(&o, xi.size());
-        for (u128 v : xi) put128(&o, v);
-        put64(&o, eta.size());
-        for (u128 v : eta) put128(&o, v);
-        put128(&o, alpha0);
-        put64(&o, lam.size());
-        for (u128 v : lam) put128(&o, v);
-        put128(&o, alpha_pub);
-        put64(&o, roww.size());
-        for (u128 v : roww) put128(&o, v);
-        hd.update(o.data(), o.size());
-        o.clear();
-        put64(&o, colw.size());
-        hd.update(o.data(), o.size());
-        hd.update(colw.data(), colw.size() * 16);
-        o.clear();
-        put128(&o, mu);
-        hd.update(o.data(), o.size());
-        uint8_t dg[32];
-        hd.finalize_xof(dg, 32);
-        spartan_frame(tr, kClaimTag, dg, 32);
-    }
-    uint8_t tag[32];
-    tr.challenge(tag, 16);
-    tr.challenge(tag + 16, 16);
+    ProveR0 pr0{ctx, h.get(), {}};
+    Outer O;
+    if ((rc = chain_outer(b, blocks, digest, root, cfg, tr, prove_round0, &pr0, &O))) return rc;
     if (state_out) tr.digest(state_out);
     phase.mark("chain_open");
-    // the virtual opening: its statement (virt.rs:132-150), then the claim
-    std::vector<std::vector<uint8_t>> lead(4);
-    put_str(&lead[0], kVirtualLabel);
-    lead[1].assign(root, root + 32);
-    {
-        std::vector<uint8_t>& o = lead[2];
-        put64(&o, b.th);
-        put64(&o, b.sh);
-        put128(&o, q);
-        uint8_t g[16];
-        const hgf gen = bitz_generator();
-        memcpy(g, &gen, 16);
-        o.insert(o.end(), g, g + 16);
-        put64(&o, b.tf);
-        put64(&o, b.sf);
-        o.insert(o.end(), b.map_digest, b.map_digest + 32);
-        const uint64_t pcs40[5] = {(uint64_t)(b.tf + b.sf), kRate, kBatch, 0, 1};
-        for (uint64_t v : pcs40) put64(&lead[3], v);
-    }
+    const std::vector<std::vector<uint8_t>> lead = virtual_lead(b, root, O.q);
     DevBuf droww, dcolw;
-    if (!droww.alloc(roww.size() * 16) || !dcolw.alloc(colw.size() * 16)) return BITZ_ERR_ALLOC;
-    BITZ_HIP(hipMemcpyAsync(droww.ptr, roww.data(), roww.size() * 16, hipMemcpyHostToDevice, ctx->stream));
-    BITZ_HIP(hipMemcpyAsync(dcolw.ptr, colw.data(), colw.size() * 16, hipMemcpyHostToDevice, ctx->stream));
+    if (!droww.alloc(O.roww.size() * 16) || !dcolw.alloc(O.colw.size() * 16)) return BITZ_ERR_ALLOC;
+    BITZ_HIP(hipMemcpyAsync(droww.ptr, O.roww.data(), O.roww.size() * 16, hipMemcpyHostToDevice, ctx->stream));
+    BITZ_HIP(hipMemcpyAsync(dcolw.ptr, O.colw.data(), O.colw.size() * 16, hipMemcpyHostToDevice, ctx->stream));
     Transposer T{ctx, &b, &cd};
-    rc = open_virtual_claim(ctx, h.get(), kVirtualSession, tag, lead, D.ptr, b.th, b.sh, q, droww.ptr, dcolw.ptr, mu,
-                            transpose_cb, &T, r0, proof);
-    BITZ_HIP(hipStreamSynchronize(ctx->stream));  // roww, colw (host vectors) copied before they go
+    rc = open_virtual_claim(ctx, h.get(), kVirtualSession, O.tag, lead, D.ptr, b.th, b.sh, O.q, droww.ptr, dcolw.ptr,
+                            O.mu, transpose_cb, &T, pr0.r0, proof);
+    BITZ_HIP(hipStreamSynchronize(ctx->stream));  // the host weights are copied before they go
+    return rc;
+}
+
+// the opening's bytes (BitZOpeningProof::from_bytes, bitz_opener.rs:325-372)
+struct OpeningBytes {
+    const uint8_t *narg = nullptr, *hints = nullptr, *y = nullptr;
+    uint64_t narg_len = 0, hints_len = 0, nonce = 0;
+    bool has_nonce = false;
+};
+bool parse_opening(const uint8_t* p, size_t len, OpeningBytes* o) {
+    size_t pos = 0;
+    auto rd64 = [&](uint64_t* v) {
+        if (pos + 8 > len) return false;
+        memcpy(v, p + pos, 8);
+        pos += 8;
+        return true;
+    };
+    if (!rd64(&o->narg_len) || o->narg_len > len - pos) return false;
+    o->narg = p + pos;
+    pos += o->narg_len;
+    if (!rd64(&o->hints_len) || o->hints_len > len - pos) return false;
+    o->hints = p + pos;
+    pos += o->hints_len;
+    if (pos + 17 > len || p[pos] != 1) return false;
+    o->y = p + pos + 1;
+    pos += 17;
+    if (pos >= len || p[pos] > 1) return false;
+    o->has_nonce = p[pos++] == 1;
+    if (o->has_nonce && !rd64(&o->nonce)) return false;
+    return pos == len;
+}
+
+struct VerifyR0 {
+    const OpeningBytes* ob;
+    int mp;
+    uint32_t bits;
+    std::vector<hgf> point;
+    hgf y;
+};
+// verify_ood_round (ligerito_flock.rs:1551-1600): the header, the grinding
+// boundary with the proof's nonce, zeta, the proof's y absorbed
+int verify_round0(void* user, Transcript& tr) {
+    VerifyR0* V = (VerifyR0*)user;
+    ood_round_header(tr, V->mp, V->bits);
+    if (V->bits) {
+        if (!V->ob->has_nonce) {
+            set_error("verify: missing Round-0 nonce");
+            return BITZ_ERR_VERIFY;
+        }
+        uint8_t seed[32];
+        grinding_seed(tr, "bitz/core/ood-round-grinding/v1", 0, V->bits, seed);
+        const bool ok = round0_pow_valid(seed, V->ob->nonce, V->bits);
+        absorb_grinding_nonce(tr, V->ob->nonce);
+        if (!ok) {
+            set_error("verify: Round-0 proof of work");
+            return BITZ_ERR_VERIFY;
+        }
+    } else if (V->ob->has_nonce) {
+        set_error("verify: unexpected Round-0 nonce");
+        return BITZ_ERR_VERIFY;
+    }
+    uint8_t zb[16];
+    tr.challenge(zb, 16);
+    hgf cur;
+    memcpy(&cur.lo, zb, 8);
+    memcpy(&cur.hi, zb + 8, 8);
+    V->point.resize(V->mp);
+    for (int i = 0; i < V->mp; i++) {
+        V->point[i] = cur;
+        cur = hgf_mul(cur, cur);
+    }
+    uint8_t tagged[17];
+    tagged[0] = 0x50;
+    memcpy(tagged + 1, V->ob->y, 16);
+    tr.absorb_slice(tagged, 17);
+    memcpy(&V->y.lo, V->ob->y, 8);
+    memcpy(&V->y.hi, V->ob->y + 8, 8);
+    return BITZ_OK;
+}
+
+int verify_chain(bitz_ctx* ctx, const uint32_t* blocks, int k, const uint32_t* initial_state, const uint32_t digest[8],
+                 const uint8_t root[32], const uint8_t* proof, size_t plen, uint8_t state_out[32]) {
+    std::array<uint32_t, 8> iv;
+    for (int i = 0; i < 8; i++) iv[i] = initial_state ? initial_state[i] : kIV[i];
+    Batch b;
+    BITZ_REQUIRE(make_batch(k, iv, &b), BITZ_ERR_SHAPE, "sha256 chain: needs 2^7..2^16 compressions");
+    OpeningBytes ob;
+    if (!parse_opening(proof, plen, &ob)) {
+        set_error("verify: malformed opening proof");
+        return BITZ_ERR_VERIFY;
+    }
+    LigConfig cfg;
+    {
+        std::string err = custom_johnson_config(b.tf + b.sf, kRate, kBatch, 100, &cfg);
+        BITZ_REQUIRE(err.empty(), BITZ_ERR_SHAPE, "sha256 chain: Ligerito configuration: " + err);
+    }
+    int rc;
+    Transcript tr;
+    VerifyR0 vr0{&ob, b.tf + b.sf - 7, (uint32_t)cfg.round0_grinding_bits, {}, {0, 0}};
+    Outer O;
+    if ((rc = chain_outer(b, blocks, digest, root, cfg, tr, verify_round0, &vr0, &O))) return rc;
+    if (state_out) tr.digest(state_out);
+    ChainDev cd;
+    if ((rc = upload_relation(ctx, *b.rel, &cd))) return rc;
+    const std::vector<std::vector<uint8_t>> lead = virtual_lead(b, root, O.q);
+    DevBuf droww, dcolw;
+    if (!droww.alloc(O.roww.size() * 16) || !dcolw.alloc(O.colw.size() * 16)) return BITZ_ERR_ALLOC;
+    BITZ_HIP(hipMemcpyAsync(droww.ptr, O.roww.data(), O.roww.size() * 16, hipMemcpyHostToDevice, ctx->stream));
+    BITZ_HIP(hipMemcpyAsync(dcolw.ptr, O.colw.data(), O.colw.size() * 16, hipMemcpyHostToDevice, ctx->stream));
+    Transposer T{ctx, &b, &cd};
+    rc = verify_virtual_claim(ctx, root, b.tf, b.sf, kVirtualSession, O.tag, lead, b.th, b.sh, O.q, droww.ptr,
+                              dcolw.ptr, O.mu, transpose_cb, &T, vr0.point, vr0.y, ob.narg, ob.narg_len, ob.hints,
+                              ob.hints_len);
+    BITZ_HIP(hipStreamSynchronize(ctx->stream));
     return rc;
 }
 
@@ -637,4 +791,15 @@ extern "C" int bitz_prove_sha256_chain(bitz_ctx* ctx, const uint32_t* blocks, ui
     BITZ_REQUIRE(proof && capacity >= bytes.size(), BITZ_ERR_ARG, "bitz_prove_sha256_chain: proof buffer too small");
     memcpy(proof, bytes.data(), bytes.size());
     return BITZ_OK;
+}
+
+extern "C" int bitz_verify_sha256_chain(bitz_ctx* ctx, const uint32_t* blocks, uint32_t log_n,
+                                        const uint32_t* initial_state, const uint32_t digest[8],
+                                        const uint8_t root[32], const uint8_t* proof, uint64_t proof_len,
+                                        uint8_t state[32]) {
+    BITZ_REQUIRE(ctx && blocks && digest && root && (proof || proof_len == 0), BITZ_ERR_ARG,
+                 "bitz_verify_sha256_chain: null argument");
+    BITZ_HIP(hipSetDevice(ctx->device));
+    bitz::PoolScope scope(&ctx->pool);
+    return bitz::verify_chain(ctx, blocks, (int)log_n, initial_state, digest, root, proof, (size_t)proof_len, state);
 }

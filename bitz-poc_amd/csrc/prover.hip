@@ -1806,6 +1806,15 @@ struct Verifier {
 int verify_claim_impl(bitz_ctx* ctx, InnerReader& it, const uint8_t root[32], int t, int s, const LigConfig& cfg,
                       u128 q, DevBuf& roww, DevBuf& colw, u128 claimed, const std::vector<G>& opt, G ood_y,
                       uint64_t narg_len, const uint8_t* hints, uint64_t hints_len);
+// receive_fold and the GKR verification over a grid (t, s): the factored
+// claim u1 (2^t), u2 (2^s) and its target
+int verify_reduce_impl(bitz_ctx* ctx, InnerReader& it, int t, int s, u128 q, DevBuf& roww, DevBuf& colw,
+                       u128 claimed, DevBuf& u1, DevBuf& u2, G* target_out);
+// Pcs::verify_lin of sum u1[b] u2[c] bit(c, b) = target over the committed
+// bits seen as the grid (t, s); the statement bound only when `bind`
+int verify_lin_impl(bitz_ctx* ctx, InnerReader& it, const uint8_t root[32], int t, int s, const LigConfig& cfg,
+                    DevBuf& u1, DevBuf& u2, G target, bool bind, const std::vector<G>& opt, G ood_y,
+                    uint64_t narg_len, const uint8_t* hints, uint64_t hints_len);
 
 int verify_standalone_impl(bitz_ctx* ctx, const uint8_t root[32], int t, int s, int r0, int k0, u128 claimed,
                            const uint8_t* proof, size_t plen) {
@@ -1905,14 +1914,7 @@ int verify_claim_impl(bitz_ctx* ctx, InnerReader& it, const uint8_t root[32], in
                       u128 q, DevBuf& roww, DevBuf& colw, u128 claimed, const std::vector<G>& opt, G ood_y,
                       uint64_t narg_len, const uint8_t* hints, uint64_t hints_len) {
     int rc;
-    const int m = t + s, mp = m - 7;
-    const int r0 = cfg.levels[0].log_inv_rate, k0 = cfg.initial_k;
     static G gen = smallest_generator();
-    const G g = gen;
-    Verifier V{ctx};
-    uint8_t gb[16], yb[16];
-    gput(g, gb);
-    gput(ood_y, yb);
     // ---- inner transcript: instance, fold round ----
     it.public_message(root, 32);
     {
@@ -1921,19 +1923,32 @@ int verify_claim_impl(bitz_ctx* ctx, InnerReader& it, const uint8_t root[32], in
         memcpy(frame, &tt, 8);
         memcpy(frame + 8, &ss, 8);
         u128_put(q, frame + 16);
-        gput(g, frame + 32);
+        gput(gen, frame + 32);
         it.public_message(frame, 48);
     }
-    const uint64_t nrows = 1ull << t, ncols = 1ull << s;
-    DevBuf folds, img, tmp, u1, u2;
-    if (!folds.alloc(ncols * 16) || !img.alloc(nrows * 16)) return BITZ_ERR_ALLOC;
-    G claimed_g;
     {
         uint8_t b[16];
         u128_put(claimed, b);
-        claimed_g = gfrom(b);
+        if ((rc = public_weights_dev(ctx, it, roww.ptr, 1ull << t, colw.ptr, 1ull << s, gfrom(b)))) return rc;
     }
-    if ((rc = public_weights_dev(ctx, it, roww.ptr, nrows, colw.ptr, ncols, claimed_g))) return rc;
+    DevBuf u1, u2;
+    G target;
+    if ((rc = verify_reduce_impl(ctx, it, t, s, q, roww, colw, claimed, u1, u2, &target))) return rc;
+    return verify_lin_impl(ctx, it, root, t, s, cfg, u1, u2, target, true, opt, ood_y, narg_len, hints, hints_len);
+}
+
+int verify_reduce_impl(bitz_ctx* ctx, InnerReader& it, int t, int s, u128 q, DevBuf& roww, DevBuf& colw,
+                       u128 claimed, DevBuf& u1, DevBuf& u2, G* target_out) {
+    int rc;
+    static G gen = smallest_generator();
+    const G g = gen;
+    Verifier V{ctx};
+    uint8_t gb[16];
+    gput(g, gb);
+    const uint64_t nrows = 1ull << t, ncols = 1ull << s;
+    DevBuf folds, img, tmp;
+    // img holds the column images (2^s) and later the row images (2^t)
+    if (!folds.alloc(ncols * 16) || !img.alloc(std::max(nrows, ncols) * 16)) return BITZ_ERR_ALLOC;
     // fold round (fold.rs receive_fold): range bound, reconstruction
     const uint8_t* fb = it.read_many(ncols, 16);
     V_CHECK(fb != nullptr, "narg string exhausted");
@@ -2000,12 +2015,27 @@ int verify_claim_impl(bitz_ctx* ctx, InnerReader& it, const uint8_t root[32], in
     if (!u1.alloc(nrows * 16)) return BITZ_ERR_ALLOC;
     if ((rc = mul_elem(ctx, img.ptr, tmp.ptr, nrows, u1.ptr, 1))) return rc;
     if ((rc = V.eq_into(alpha_c, u2))) return rc;
-    G target = hgf_add(claim, gone());
+    *target_out = hgf_add(claim, gone());
+    return BITZ_OK;
+}
+
+int verify_lin_impl(bitz_ctx* ctx, InnerReader& it, const uint8_t root[32], int t, int s, const LigConfig& cfg,
+                    DevBuf& u1, DevBuf& u2, G target, bool bind, const std::vector<G>& opt, G ood_y,
+                    uint64_t narg_len, const uint8_t* hints, uint64_t hints_len) {
+    int rc;
+    const int m = cfg.m, mp = m - 7;
+    const int r0 = cfg.levels[0].log_inv_rate, k0 = cfg.initial_k;
+    Verifier V{ctx};
+    uint8_t yb[16];
+    gput(ood_y, yb);
+    const uint64_t nrows = 1ull << t, ncols = 1ull << s;
     const uint64_t pcs40[5] = {(uint64_t)m, (uint64_t)r0, (uint64_t)k0, 0, 1};
-    it.public_message(std::string("bitz/pcs/bit-inner-product/v2"));
-    it.public_message(root, 32);
-    it.public_message(pcs40, 40);
-    if ((rc = public_weights_dev(ctx, it, u1.ptr, nrows, u2.ptr, ncols, target))) return rc;
+    if (bind) {
+        it.public_message(std::string("bitz/pcs/bit-inner-product/v2"));
+        it.public_message(root, 32);
+        it.public_message(pcs40, 40);
+        if ((rc = public_weights_dev(ctx, it, u1.ptr, nrows, u2.ptr, ncols, target))) return rc;
+    }
     it.public_message(std::string("bitz/pcs/inner-product-sumcheck/v1"));
     std::vector<G> sc_point;
     for (int round = 0; round < t + s; round++) {
@@ -2504,6 +2534,46 @@ int verify_linear_claim(bitz_ctx* ctx, const uint8_t root[32], int t, int s, int
     InnerReader it(session, tag, 32, narg, narg_len);
     if (lead_len) it.public_message(lead, lead_len);
     return verify_claim_impl(ctx, it, root, t, s, cfg, q, rw, cw, claimed, opt, ood_y, narg_len, hints, hints_len);
+}
+
+int verify_virtual_claim(bitz_ctx* ctx, const uint8_t root[32], int tf, int sf, const char* session,
+                         const uint8_t tag[32], const std::vector<std::vector<uint8_t>>& lead, int th, int sh, u128 q,
+                         const void* roww, const void* colw, u128 claimed, TransposeFn transpose, void* user,
+                         const std::vector<G>& opt, G ood_y, const uint8_t* narg, uint64_t narg_len,
+                         const uint8_t* hints, uint64_t hints_len) {
+    LigConfig cfg;
+    std::string err = custom_johnson_config(tf + sf, 1, 4, 100, &cfg);
+    BITZ_REQUIRE(err.empty(), BITZ_ERR_SHAPE, "Ligerito configuration: " + err);
+    BITZ_REQUIRE(th >= 7 && sh >= 1 && th + sh <= 35 && cfg.levels.size() >= 2 && tf + sf - 7 <= kMaxTensorN,
+                 BITZ_ERR_SHAPE, "verify: unsupported shape");
+    if (claimed >= q) {
+        set_error("verify: claim not reduced");
+        return BITZ_ERR_VERIFY;
+    }
+    const uint64_t nrows = 1ull << th, ncols = 1ull << sh;
+    DevBuf rw, cw, u1, u2, wf, one;
+    if (!rw.alloc(nrows * 16) || !cw.alloc(ncols * 16)) return BITZ_ERR_ALLOC;
+    BITZ_HIP(hipMemcpyAsync(rw.ptr, roww, nrows * 16, hipMemcpyDeviceToDevice, ctx->stream));
+    BITZ_HIP(hipMemcpyAsync(cw.ptr, colw, ncols * 16, hipMemcpyDeviceToDevice, ctx->stream));
+    InnerReader it(session, tag, 32, narg, narg_len);
+    for (auto& msg : lead) it.public_message(msg.data(), msg.size());
+    int rc;
+    {
+        uint8_t b[16];
+        u128_put(claimed, b);
+        if ((rc = public_weights_dev(ctx, it, rw.ptr, nrows, cw.ptr, ncols, gfrom(b)))) return rc;
+    }
+    G target;
+    if ((rc = verify_reduce_impl(ctx, it, th, sh, q, rw, cw, claimed, u1, u2, &target))) return rc;
+    if (!wf.alloc(16ull << cfg.m) || !one.alloc(16)) return BITZ_ERR_ALLOC;
+    if ((rc = transpose(user, u1.ptr, u2.ptr, wf.ptr))) return rc;
+    {
+        const G o = gone();
+        BITZ_HIP(hipMemcpyAsync(one.ptr, &o, 16, hipMemcpyHostToDevice, ctx->stream));
+        BITZ_HIP(hipStreamSynchronize(ctx->stream));
+    }
+    return verify_lin_impl(ctx, it, root, cfg.m, 0, cfg, wf, one, target, false, opt, ood_y, narg_len, hints,
+                           hints_len);
 }
 
 bool round0_pow_valid(const uint8_t seed[32], uint64_t nonce, uint32_t bits) {

@@ -14,6 +14,10 @@
 
 namespace bitz {
 
+// turns a reduced factored claim (u1, u2 on a derived grid) into one weight
+// per committed cell (wf)
+using TransposeFn = int (*)(void* user, const void* u1, const void* u2, void* wf);
+
 struct Round0Claim {
     std::vector<hgf> point;  // the out-of-domain point (zeta, zeta^2, ...)
     hgf y;                   // the packed message's MLE there
@@ -37,6 +41,15 @@ bool round0_pow_valid(const uint8_t seed[32], uint64_t nonce, uint32_t bits);
 // the BitZ generator g (smallest_generator, src/pcs.rs:85-102)
 hgf bitz_generator();
 
+// BitZVerifier::verify_virtual (virt.rs:304-360) after its statement: the
+// mirror of open_virtual_claim, reading the opening (narg, hints) of a
+// commitment to 2^(tf+sf) cells (custom:1:4)
+int verify_virtual_claim(bitz_ctx* ctx, const uint8_t root[32], int tf, int sf, const char* session,
+                         const uint8_t tag[32], const std::vector<std::vector<uint8_t>>& lead, int th, int sh, u128 q,
+                         const void* roww, const void* colw, u128 claimed, TransposeFn transpose, void* user,
+                         const std::vector<hgf>& opt, hgf ood_y, const uint8_t* narg, uint64_t narg_len,
+                         const uint8_t* hints, uint64_t hints_len);
+
 // absorb_spartan_message (src/piop/spartan/mod.rs:349-358): the frame
 // domain, the tag and the payload, each behind its u64 length
 void spartan_frame(Transcript& tr, const char* tag, const void* payload, size_t len);
@@ -52,7 +65,6 @@ bool spartan_squeeze(Transcript& tr, u128 q, u128* out);
 // (u1: 2^th, u2: 2^sh Gf128) into one weight per committed cell (wf:
 // 2^(t+s) Gf128), opened as a single-column inner product on the committed
 // bits of h. proof = BitZOpeningProof::to_bytes.
-using TransposeFn = int (*)(void* user, const void* u1, const void* u2, void* wf);
 int open_virtual_claim(bitz_ctx* ctx, const bitz_commit_handle* h, const char* session, const uint8_t tag[32],
                        const std::vector<std::vector<uint8_t>>& lead, const void* grid, int th, int sh, u128 q,
                        const void* roww, const void* colw, u128 claimed, TransposeFn transpose, void* user,

@@ -435,6 +435,32 @@ int bitz_prove_mul_u64(bitz_ctx* ctx, const void* x, const void* y, const void* 
 int bitz_verify_mul_u64(bitz_ctx* ctx, const uint8_t root[32], uint64_t n, int32_t split_shift,
                         const uint8_t* proof, uint64_t proof_len, uint8_t state[32]);
 
+/* ---- the SHA-256 chain proof (linear PIOP over F_q, virtual BitZ opening) ---- */
+/* generate_sha256_chain_witnesses + commit_sha256_chain_witness +
+ * prove_sha256_chain at the lambda = 100 profile (custom:1:4)
+ * (src/piop/spartan/sha256/chain.rs:635-845 -> protocol/linear.rs:371-582 ->
+ * bitz_opener.rs:1597-1624, src/bitz/virt.rs:214-301): blocks are 2^log_n
+ * host blocks of 16 u32 words (log_n in 7..16), chained from initial_state
+ * (8 u32 words; NULL = FIPS 180-4's H_0): H_{i+1} = Compress(H_i, M_i). The
+ * compression circuit is compiled on the host (once per initial state); the
+ * witness, the commitment and the opening run on the device. Writes the
+ * commitment root of the source bits, the final chaining value H_N, the
+ * outer Blake3Transcript's state digest after the proof (the value
+ * tests/transcript_state_pins.rs:471-497 pins for 2^7 blocks) and the proof
+ * (BitZOpeningProof::to_bytes; the boundary nonces are zero at this
+ * profile). *proof_len is set even when capacity is too small (BITZ_ERR_ARG). */
+int bitz_prove_sha256_chain(bitz_ctx* ctx, const uint32_t* blocks, uint32_t log_n, const uint32_t* initial_state,
+                            uint8_t root[32], uint32_t digest[8], uint8_t state[32], uint8_t* proof,
+                            uint64_t capacity, uint64_t* proof_len);
+/* verify_sha256_chain (chain.rs:850-872 -> linear.rs:586-705,
+ * virt.rs:304-360): replays the outer transcript from the statement (blocks,
+ * initial_state, digest), the root and the proof's Round-0 value, and checks
+ * the virtual opening. BITZ_OK when accepted, BITZ_ERR_VERIFY otherwise;
+ * state (nullable) receives the outer transcript digest. */
+int bitz_verify_sha256_chain(bitz_ctx* ctx, const uint32_t* blocks, uint32_t log_n, const uint32_t* initial_state,
+                             const uint32_t digest[8], const uint8_t root[32], const uint8_t* proof,
+                             uint64_t proof_len, uint8_t state[32]);
+
 /* The device transcript (Blake3Transcript, src/transcript/mod.rs:56-96) checked
  * against the host's: a host transcript absorbs `prefix` (absorb_slice), its
  * state goes to the device, which then runs n x (absorb_slice(msg_i of

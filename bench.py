@@ -134,6 +134,8 @@ def parse():
     ap.add_argument("--batch-jobs", type=int, default=64)
     ap.add_argument("--mul-log", type=int, default=20, help="log2 multiplications of the u64 proof workload")
     ap.add_argument("--no-mul", action="store_true", help="skip the u64 multiplication proof workload")
+    ap.add_argument("--chain-log", type=int, default=12, help="log2 compressions of the SHA-256 chain workload")
+    ap.add_argument("--no-chain", action="store_true", help="skip the SHA-256 chain workload")
     ap.add_argument("--batch-workers", type=int, default=8)
     ap.add_argument("--serve-jobs", type=int, default=96)
     ap.add_argument("--summarize-profiles", nargs=4, metavar=("STATS", "SQ", "FETCH", "WRITE"),
@@ -360,6 +362,86 @@ def commit_batch_steps(args, ctx):
     }
 
 
+def sha256_chain_steps(args, ctx):
+    """The SHA-256 chain proof (config 4): 2^chain_log chained compressions
+    over a seeded random message (its 64-byte blocks as big-endian u32
+    words), timed over `steps` proofs, verified by the library; then 16
+    chains of 2^8..2^12 compressions (log-uniform, seeded) on one server
+    (8 workers, jobs overlapping on the GPU) against the same chains one at
+    a time, proof bytes compared chain by chain, every proof verified."""
+    import numpy as np
+    import torch
+
+    from bitz_amd.chain import prove_chain, verify_chain
+    from bitz_amd.device import kernel_stats, set_timing
+    from bitz_amd.serve import ChainJob, Server
+
+    k = args.chain_log
+    rng = np.random.default_rng(4)
+    msg = rng.integers(0, 256, size=64 << k, dtype=np.uint8)
+    blocks = msg.reshape(1 << k, 16, 4)
+    blocks = ((blocks[..., 0].astype(np.uint32) << 24) | (blocks[..., 1].astype(np.uint32) << 16)
+              | (blocks[..., 2].astype(np.uint32) << 8) | blocks[..., 3].astype(np.uint32))
+    prove_chain(ctx, blocks)
+    set_timing(ctx, True)
+    prove_chain(ctx, blocks)
+    prof = kernel_stats(ctx)
+    set_timing(ctx, False)
+    phases = {kk[6:]: round(v[1], 3) for kk, v in prof.items() if kk.startswith("phase:")}
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        root, digest, state, proof = prove_chain(ctx, blocks)
+    dt = (time.perf_counter() - t0) / args.steps
+    tv = time.perf_counter()
+    verify_chain(ctx, blocks, digest, root, proof)
+    tv = time.perf_counter() - tv
+    # 16 chains of 2^8..2^12 compressions, one at a time and on one server
+    brng = np.random.default_rng(16)
+    lens = [int(brng.integers(8, 13)) for _ in range(16)]
+    chains = [brng.integers(0, 2**32, size=(1 << ln, 16), dtype=np.uint64).astype(np.uint32) for ln in lens]
+
+    def run(workers):
+        with Server(workers=workers, hbm_budget=200 << 30) as srv:
+            for _ in range(2):  # the first pass fills the contexts' pools and relation caches
+                jobs = [ChainJob(b, verify=True) for b in chains]
+                t1 = time.perf_counter()
+                for j in sorted(jobs, key=lambda j: -j.footprint):
+                    srv.submit(j)
+                srv.wait_all()
+                d = time.perf_counter() - t1
+        for j in jobs:
+            j.check()
+        return jobs, d
+
+    seq, dt_seq = run(1)
+    bat, dt_bat = run(args.batch_workers)
+    return {
+        "workload": f"SHA-256 chain proof: 2^{k} chained compressions from FIPS 180-4's H_0 over a seeded random "
+                    f"{64 << k >> 10} KiB message (u32 words), lambda 100, custom:1:4: witness, commit of the "
+                    "source bits, linear PIOP over a runtime prime, virtual BitZ opening of the product tensor",
+        "ms_per_proof": round(dt * 1e3, 3),
+        "compressions_per_s": round((1 << k) / dt, 1),
+        "proofs_per_s": round(1.0 / dt, 2),
+        "steps": args.steps,
+        "proof_bytes": len(proof),
+        "verify_ms": round(tv * 1e3, 3),
+        "proof_verified": True,
+        "host_phases_ms": phases,
+        "batch16": {
+            "workload": "16 SHA-256 chains of 2^8..2^12 compressions (log-uniform, seeded), one server",
+            "chain_logs": lens,
+            "workers": args.batch_workers,
+            "proofs_per_s": round(16 / dt_bat, 2),
+            "sequential_proofs_per_s": round(16 / dt_seq, 2),
+            "speedup": round(dt_seq / dt_bat, 3),
+            "bit_exact_vs_sequential": all(a.proof == b.proof and a.root == b.root and a.digest == b.digest
+                                           for a, b in zip(seq, bat)),
+            "all_verified": all(j.verified for j in seq + bat),
+        },
+    }
+
+
 def batched_and_serving(args, ctx):
     """Batched commit+open of independent polynomials of mixed sizes and
     rates on one server (bitz_server: per-worker contexts, jobs overlapping on
@@ -405,23 +487,25 @@ def batched_and_serving(args, ctx):
     ok, vms = verify_all(ctx, bat, specs)
     batched["all_verified"] = ok
     batched["verify_ms_per_proof"] = round(vms, 3)
-    # serving: Poisson arrivals at 70% of the batched rate, half of them
-    # commit+open jobs over host rows, half full u64-multiplication proofs
-    # (2^12..2^22 multiplications: commit, Spartan PIOP, BitZ opening); every
+    # serving: Poisson arrivals at 70% of the mix's capacity, half of them
+    # full u64-multiplication proofs (2^12..2^22 multiplications: commit,
+    # Spartan PIOP, BitZ opening), half SHA-256 chain proofs (2^7..2^12
+    # compressions: witness, commit, linear PIOP, virtual opening); every
     # proof is verified by the worker that made it, inside the measured latency
-    from bitz_amd.serve import MulProofJob
-    srng = np.random.default_rng(65)
-    sspecs = _job_specs(srng, args.serve_jobs // 2, 14, 22)
+    from bitz_amd.serve import ChainJob, MulProofJob
+    npairs = args.serve_jobs // 2
 
     def mixed_jobs(limit=None):
         mrng = np.random.default_rng(67)
         out = []
-        for t, s, r, rows in sspecs[:limit]:
-            out.append(Job(rows, t, s, 4, r, verify=True))
+        for _ in range(npairs if limit is None else limit):
             ln = int(mrng.integers(12, 23))
             x = mrng.integers(0, 2**64, size=1 << ln, dtype=np.uint64)
             y = mrng.integers(0, 2**64, size=1 << ln, dtype=np.uint64)
             out.append(MulProofJob(x, y, verify=True))
+            cl = int(mrng.integers(7, 13))
+            out.append(ChainJob(mrng.integers(0, 2**32, size=(1 << cl, 16), dtype=np.uint64).astype(np.uint32),
+                                verify=True))
         return out
 
     jobs = mixed_jobs()
@@ -452,11 +536,12 @@ def batched_and_serving(args, ctx):
     for j in jobs:
         j.check()
     lat = np.array([j.latency_ms for j in jobs])
-    lat_mul = np.array([j.latency_ms for j in jobs[1::2]])
+    lat_mul = np.array([j.latency_ms for j in jobs[0::2]])
+    lat_chain = np.array([j.latency_ms for j in jobs[1::2]])
     serving = {
-        "workload": f"{len(jobs)} Poisson arrivals: commit+open of host rows (2^14..2^22 coefficients, mixed "
-                    "rates) and full u64-multiplication proofs (2^12..2^22 multiplications: commit, Spartan PIOP, "
-                    "BitZ opening), 50/50, "
+        "workload": f"{len(jobs)} Poisson arrivals: full u64-multiplication proofs (2^12..2^22 multiplications: "
+                    "commit, Spartan PIOP, BitZ opening) and SHA-256 chain proofs (2^7..2^12 compressions: "
+                    "witness, commit, linear PIOP, virtual opening), 50/50, "
                     "admitted by device-memory footprint (64 GiB budget), each proof verified by its worker",
         "capacity_proofs_per_s": round(capacity, 2),
         "arrival_rate_per_s": round(rate, 2),
@@ -467,6 +552,8 @@ def batched_and_serving(args, ctx):
         "max_ms": round(float(lat.max()), 3),
         "mul_jobs_p50_ms": round(float(np.percentile(lat_mul, 50)), 3),
         "mul_jobs_p99_ms": round(float(np.percentile(lat_mul, 99)), 3),
+        "chain_jobs_p50_ms": round(float(np.percentile(lat_chain, 50)), 3),
+        "chain_jobs_p99_ms": round(float(np.percentile(lat_chain, 99)), 3),
         "all_verified": bool(all(j.verified for j in jobs)),
     }
     return batched, serving
@@ -617,6 +704,7 @@ def main():
     verify_ms = round((time.perf_counter() - tv) * 1e3, 3)
     host_rows_rate = None if args.no_host_rows else host_rows_steps(args, ctx, coeffs, t, s, claimed, proof)
     mul_u64 = None if args.no_mul else mul_u64_steps(args, ctx)
+    sha256_chain = None if args.no_chain else sha256_chain_steps(args, ctx)
     commit_batch = None if args.no_batched else commit_batch_steps(args, ctx)
     batched = serving = None
     if not args.no_batched:
@@ -646,6 +734,7 @@ def main():
         "batched": batched,
         "serving": serving,
         "mul_u64": mul_u64,
+        "sha256_chain": sha256_chain,
         "commit_batch": commit_batch,
         "kernels": {k: {"launches": int(v[0]), "ms": round(v[1], 3)} for k, v in
                     sorted(profile.items(), key=lambda kv: -kv[1][1])},

@@ -27,9 +27,11 @@ class _JobStruct(C.Structure):
         ("kind", C.c_uint32), ("verify", C.c_uint32),
         ("mul_x", C.c_void_p), ("mul_y", C.c_void_p), ("mul_n", C.c_uint64),
         ("verified", C.c_int32), ("reserved", C.c_int32),
+        ("chain_blocks", C.c_void_p), ("chain_log_n", C.c_uint32), ("chain_digest", C.c_uint32 * 8),
+        ("reserved2", C.c_uint32),
     ]
 
-JOB_ROWS, JOB_MUL_U64, JOB_MUL_U64_PROOF = 0, 1, 2
+JOB_ROWS, JOB_MUL_U64, JOB_MUL_U64_PROOF, JOB_SHA256_CHAIN = 0, 1, 2, 3
 
 
 class Job:
@@ -134,6 +136,39 @@ class MulProofJob(MulJob):
     def __init__(self, x: np.ndarray, y: np.ndarray, proof_capacity: int = 1 << 21, verify: bool = False):
         super().__init__(x, y, 4, 1, None, proof_capacity, verify)
         self.s_.kind = JOB_MUL_U64_PROOF
+
+
+class ChainJob(Job):
+    """The SHA-256 chain proof of 2^k blocks (16 u32 words each) from FIPS
+    180-4's initial state (bitz_prove_sha256_chain; prove_sha256_chain,
+    src/piop/spartan/sha256/chain.rs:817-845), verified by its worker
+    (bitz_verify_sha256_chain) when verify is set. `claimed` is the first
+    half of the outer transcript state digest, `digest` the final chaining
+    value."""
+
+    def __init__(self, blocks: np.ndarray, proof_capacity: int = 1 << 21, verify: bool = False):
+        b = np.ascontiguousarray(np.asarray(blocks, dtype=np.uint32))
+        n = b.shape[0]
+        k = n.bit_length() - 1
+        if b.ndim != 2 or b.shape[1] != 16 or n != 1 << k or not 7 <= k <= 16:
+            raise ValueError("blocks must be (2^k, 16) u32 words with 7 <= k <= 16")
+        self._blocks = b
+        self._proof = (C.c_uint8 * proof_capacity)()
+        j = _JobStruct()
+        j.kind = JOB_SHA256_CHAIN
+        j.verify = 1 if verify else 0
+        j.chain_blocks = b.ctypes.data
+        j.chain_log_n = k
+        j.t = min(k, 13)
+        j.s = 15 + k - j.t
+        j.log_batch, j.log_inv_rate = 4, 1
+        j.proof = C.cast(self._proof, C.c_void_p)
+        j.proof_capacity = proof_capacity
+        self.s_ = j
+
+    @property
+    def digest(self) -> list[int]:
+        return [int(x) for x in self.s_.chain_digest]
 
 
 class Server:

@@ -109,9 +109,25 @@ int run_mul_proof_job(bitz_ctx* ctx, bitz_job* j) {
     return rc;
 }
 
+// the SHA-256 chain proof of one job (bitz_prove_sha256_chain), its outer
+// transcript state in claimed[0..16)
+int run_chain_job(bitz_ctx* ctx, bitz_job* j) {
+    uint8_t state[32];
+    int rc = bitz_prove_sha256_chain(ctx, j->chain_blocks, j->chain_log_n, nullptr, j->root, j->chain_digest, state,
+                                     j->proof, j->proof_capacity, &j->proof_len);
+    if (rc) return rc;
+    memcpy(j->claimed, state, 16);
+    if (!j->verify) return rc;
+    rc = bitz_verify_sha256_chain(ctx, j->chain_blocks, j->chain_log_n, nullptr, j->chain_digest, j->root, j->proof,
+                                  j->proof_len, nullptr);
+    j->verified = rc == BITZ_OK ? 1 : 0;
+    return rc;
+}
+
 // commit + (evaluation) + proof (+ verification) of one job on one context
 int run_job(bitz_ctx* ctx, bitz_job* j) {
     if (j->kind == BITZ_JOB_MUL_U64_PROOF) return run_mul_proof_job(ctx, j);
+    if (j->kind == BITZ_JOB_SHA256_CHAIN) return run_chain_job(ctx, j);
     bitz_commit_handle* h = nullptr;
     int rc;
     if (j->kind == BITZ_JOB_MUL_U64) {
@@ -224,6 +240,16 @@ extern "C" int bitz_server_submit(bitz_server* srv, bitz_job* job) {
         if (int rc = bitz_mul_layout(64, job->mul_n, 0, lay)) return rc;
         job->t = lay[2];
         job->s = lay[1];
+    } else if (job->kind == BITZ_JOB_SHA256_CHAIN) {
+        if (!job->chain_blocks || job->chain_log_n < 7 || job->chain_log_n > 16) {
+            bitz::set_error("bitz_server_submit: chain jobs need 2^7..2^16 host blocks");
+            return BITZ_ERR_ARG;
+        }
+        // the product tensor's grid (the largest buffers the job holds)
+        job->t = std::min(job->chain_log_n, 13u);
+        job->s = 15 + job->chain_log_n - job->t;
+        job->log_batch = 4;
+        job->log_inv_rate = 1;
     } else if (job->kind != BITZ_JOB_ROWS || !job->rows) {
         bitz::set_error("bitz_server_submit: null rows or unknown job kind");
         return BITZ_ERR_ARG;

@@ -362,18 +362,28 @@ typedef struct bitz_job {
      * proof of mul_x, mul_y (bitz_prove_mul_u64 with the products computed
      * on the device, custom:1:4; log_batch and log_inv_rate must be 4 and 1),
      * verified by bitz_verify_mul_u64 when verify is set; claimed receives
-     * the first 16 bytes of the outer transcript state digest. */
-    uint32_t kind;           /* BITZ_JOB_ROWS (0), BITZ_JOB_MUL_U64 (1) or BITZ_JOB_MUL_U64_PROOF (2) */
+     * the first 16 bytes of the outer transcript state digest. Kind
+     * BITZ_JOB_SHA256_CHAIN: the SHA-256 chain proof of 2^chain_log_n host
+     * blocks chain_blocks from FIPS 180-4's H_0 (bitz_prove_sha256_chain),
+     * verified by bitz_verify_sha256_chain when verify is set; chain_digest
+     * receives H_N, claimed the first 16 bytes of the state digest, t and s
+     * the product tensor's grid (the footprint's shape). */
+    uint32_t kind;           /* BITZ_JOB_ROWS (0), BITZ_JOB_MUL_U64 (1), _MUL_U64_PROOF (2), _SHA256_CHAIN (3) */
     uint32_t verify;         /* nonzero: the worker verifies the proof (bitz_verify_standalone) */
     const uint64_t* mul_x;
     const uint64_t* mul_y;
     uint64_t mul_n;
     int32_t verified;        /* out: 1 when the proof was verified and accepted */
     int32_t reserved;
+    const uint32_t* chain_blocks; /* 2^chain_log_n x 16 u32 words (host) */
+    uint32_t chain_log_n;
+    uint32_t chain_digest[8];     /* out */
+    uint32_t reserved2;
 } bitz_job;
 #define BITZ_JOB_ROWS 0
 #define BITZ_JOB_MUL_U64 1
 #define BITZ_JOB_MUL_U64_PROOF 2
+#define BITZ_JOB_SHA256_CHAIN 3
 typedef struct bitz_server bitz_server;
 /* device bytes one job holds while it runs (0 for an invalid shape) */
 uint64_t bitz_job_footprint(uint32_t t, uint32_t s, uint32_t log_batch, uint32_t log_inv_rate);

@@ -221,3 +221,45 @@ def test_poisson_arrivals_continuous_batching():
         assert j.s_.start_ms >= j.s_.submit_ms and j.s_.end_ms >= j.s_.start_ms
     spans = sorted((j.s_.start_ms, j.s_.end_ms) for j in live)
     assert any(spans[i + 1][0] < spans[i][1] for i in range(len(spans) - 1))
+
+
+def test_config5_mix_of_multiplication_and_chain_proofs():
+    """the serving mix of config 5 under Poisson arrivals: u64 multiplication
+    proofs and SHA-256 chain proofs alternating, submitted at exponential
+    inter-arrival times on a 4-worker server; each is verified by its worker
+    and equals the same proof made alone (root, transcript state, proof
+    bytes, and the chain's digest)"""
+    import time
+
+    from bitz_amd.chain import prove_chain
+    from bitz_amd.serve import ChainJob, MulProofJob, Server
+    from gpu_util import ctx
+    rng = np.random.default_rng(55)
+    specs = []
+    for i in range(12):
+        if i % 2 == 0:
+            n = int(rng.choice([1 << 12, 3000, 1 << 13]))
+            specs.append(("mul", rng.integers(0, 2**64, size=n, dtype=np.uint64),
+                          rng.integers(0, 2**64, size=n, dtype=np.uint64)))
+        else:
+            k = int(rng.integers(7, 11))
+            specs.append(("chain", rng.integers(0, 2**32, size=(1 << k, 16), dtype=np.uint64).astype(np.uint32)))
+    log("serve: chain proofs alone")
+    alone = {}
+    for i, sp in enumerate(specs):
+        if sp[0] == "chain":
+            root, digest, state, proof = prove_chain(ctx(), sp[1])
+            alone[i] = (root, digest, int.from_bytes(state[:16], "little"), proof)
+    log("serve: 12 Poisson arrivals (u64 and chain proofs) on a 4-worker server")
+    live = [MulProofJob(sp[1], sp[2], verify=True) if sp[0] == "mul" else ChainJob(sp[1], verify=True)
+            for sp in specs]
+    with Server(workers=4) as srv:
+        for j, gap in zip(live, rng.exponential(0.003, size=len(specs))):
+            time.sleep(float(gap))
+            srv.submit(j)
+        srv.wait_all()
+    for i, j in enumerate(live):
+        j.check()
+        assert j.verified
+        if i in alone:
+            assert (j.root, j.digest, j.claimed, j.proof) == alone[i]

@@ -136,6 +136,8 @@ def parse():
     ap.add_argument("--no-mul", action="store_true", help="skip the u64 multiplication proof workload")
     ap.add_argument("--chain-log", type=int, default=12, help="log2 compressions of the SHA-256 chain workload")
     ap.add_argument("--no-chain", action="store_true", help="skip the SHA-256 chain workload")
+    ap.add_argument("--chain-only", action="store_true",
+                    help="run only the SHA-256 chain workload and print its object (profiling aid)")
     ap.add_argument("--batch-workers", type=int, default=8)
     ap.add_argument("--serve-jobs", type=int, default=96)
     ap.add_argument("--summarize-profiles", nargs=4, metavar=("STATS", "SQ", "FETCH", "WRITE"),
@@ -396,6 +398,21 @@ def sha256_chain_steps(args, ctx):
     tv = time.perf_counter()
     verify_chain(ctx, blocks, digest, root, proof)
     tv = time.perf_counter() - tv
+    out = {
+        "workload": f"SHA-256 chain proof: 2^{k} chained compressions from FIPS 180-4's H_0 over a seeded random "
+                    f"{64 << k >> 10} KiB message (u32 words), lambda 100, custom:1:4: witness, commit of the "
+                    "source bits, linear PIOP over a runtime prime, virtual BitZ opening of the product tensor",
+        "ms_per_proof": round(dt * 1e3, 3),
+        "compressions_per_s": round((1 << k) / dt, 1),
+        "proofs_per_s": round(1.0 / dt, 2),
+        "steps": args.steps,
+        "proof_bytes": len(proof),
+        "verify_ms": round(tv * 1e3, 3),
+        "proof_verified": True,
+        "host_phases_ms": phases,
+    }
+    if args.no_batched:
+        return out
     # 16 chains of 2^8..2^12 compressions, one at a time and on one server
     brng = np.random.default_rng(16)
     lens = [int(brng.integers(8, 13)) for _ in range(16)]
@@ -416,18 +433,7 @@ def sha256_chain_steps(args, ctx):
 
     seq, dt_seq = run(1)
     bat, dt_bat = run(args.batch_workers)
-    return {
-        "workload": f"SHA-256 chain proof: 2^{k} chained compressions from FIPS 180-4's H_0 over a seeded random "
-                    f"{64 << k >> 10} KiB message (u32 words), lambda 100, custom:1:4: witness, commit of the "
-                    "source bits, linear PIOP over a runtime prime, virtual BitZ opening of the product tensor",
-        "ms_per_proof": round(dt * 1e3, 3),
-        "compressions_per_s": round((1 << k) / dt, 1),
-        "proofs_per_s": round(1.0 / dt, 2),
-        "steps": args.steps,
-        "proof_bytes": len(proof),
-        "verify_ms": round(tv * 1e3, 3),
-        "proof_verified": True,
-        "host_phases_ms": phases,
+    out.update({
         "batch16": {
             "workload": "16 SHA-256 chains of 2^8..2^12 compressions (log-uniform, seeded), one server",
             "chain_logs": lens,
@@ -439,7 +445,8 @@ def sha256_chain_steps(args, ctx):
                                            for a, b in zip(seq, bat)),
             "all_verified": all(j.verified for j in seq + bat),
         },
-    }
+    })
+    return out
 
 
 def batched_and_serving(args, ctx):
@@ -601,6 +608,11 @@ def main():
 
     n, t, s = shape_for(args.log_coeffs)
     ctx = Context(0)
+    if args.chain_only:
+        # the SHA-256 chain workload alone (for its kernel profiles); prints
+        # its own JSON object, not the bench line
+        print(json.dumps({"sha256_chain": sha256_chain_steps(args, ctx)}), flush=True)
+        return
     if args.device_transcript:
         from bitz_amd import _lib
         _lib.call("bitz_ctx_set_device_transcript", ctx.ptr, 1)

@@ -73,8 +73,10 @@ __constant__ uint32_t c_K[64] = {
 
 __device__ __forceinline__ uint32_t rotr32(uint32_t x, int n) { return (x >> n) | (x << (32 - n)); }
 
-// the wave's 64 instances' bit `b` of v, one LDS word per f_bar column
-__device__ __forceinline__ void emit_bits(uint64_t* col, int base, uint64_t v, int nbits, uint32_t lane) {
+// the 64 instances' bit `b` of v, one LDS word per f_bar column (each wave
+// holds the same 64 instances; only the wave that owns the call emits)
+__device__ __forceinline__ void emit_bits(uint64_t* col, int base, uint64_t v, int nbits, uint32_t lane, bool mine) {
+    if (!mine) return;
 #pragma unroll 1
     for (int b = 0; b < nbits; b++) {
         const uint64_t m = __ballot((int)((v >> b) & 1ull));
@@ -82,27 +84,30 @@ __device__ __forceinline__ void emit_bits(uint64_t* col, int base, uint64_t v, i
     }
 }
 
-// One wavefront per 64 consecutive instances (n is a multiple of 64).
-// Lane j replays instance 64 g + j's compression and emits its f_bar bits in
-// the circuit's column order ([1 | block | state | hints], hints in
-// allocation order: 48 schedule sums of 34 bits, per round the e then the a
-// half-sums of 35 bits, 8 output sums of 33 bits); a ballot per bit makes
-// col[c] the 64 instances' bits of column c. Then:
+// One workgroup of four wavefronts per 64 consecutive instances (n is a
+// multiple of 64). Every wave replays the 64 compressions (lane j: instance
+// 64 g + j) and emits the f_bar bits of every fourth call, in the circuit's
+// column order ([1 | block | state | hints], hints in allocation order: 48
+// schedule sums of 34 bits, per round the e then the a half-sums of 35 bits,
+// 8 output sums of 33 bits); a ballot per bit makes col[c] the 64
+// instances' bits of column c. Then:
 //  * the product tensor: derived row r of the group is the XOR of the col
 //    words of M's row r (m_off, m_cols over f_bar), stored as one 64-bit
 //    row word of D (grid th x sh: cell = (r H + hi) 2^th + (instance mod
 //    2^th), H = n >> th); the terminal rows hold the last instance's output
 //    bits; rows past the chain's 20713 stay zero;
-//  * the committed source stream: lane j's 6888 cells (block bits, then hint
-//    bits) at bit 1 + 6888 i of src_bits, OR-ed word by word (neighbouring
-//    instances share the end words); cell 0 (the constant) is set once.
-__global__ __launch_bounds__(64) void k_chain_witness(const uint32_t* __restrict__ blocks,
-                                                      const uint32_t* __restrict__ states, uint64_t n,
-                                                      const uint32_t* __restrict__ m_off,
-                                                      const uint32_t* __restrict__ m_cols, int th,
-                                                      uint64_t* __restrict__ D, uint64_t* __restrict__ src_bits) {
+//  * the committed source stream: instance i's 6888 cells (block bits, then
+//    hint bits) at bit 1 + 6888 i of src_bits, as 64-bit words shifted into
+//    place, the wave w taking a quarter of them; words are OR-ed
+//    (neighbouring instances share the end words); cell 0 (the constant) is
+//    set once.
+__global__ __launch_bounds__(256) void k_chain_witness(const uint32_t* __restrict__ blocks,
+                                                       const uint32_t* __restrict__ states, uint64_t n,
+                                                       const uint32_t* __restrict__ m_off,
+                                                       const uint32_t* __restrict__ m_cols, int th,
+                                                       uint64_t* __restrict__ D, uint64_t* __restrict__ src_bits) {
     __shared__ uint64_t col[kFBar];
-    const uint32_t lane = threadIdx.x;
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const uint64_t inst = blockIdx.x * 64ull + lane;
     uint32_t W[64];
     uint32_t st[8];
@@ -110,18 +115,19 @@ __global__ __launch_bounds__(64) void k_chain_witness(const uint32_t* __restrict
     for (int i = 0; i < 16; i++) W[i] = blocks[inst * 16 + i];
 #pragma unroll
     for (int i = 0; i < 8; i++) st[i] = states[inst * 8 + i];
-    if (lane == 0) col[0] = ~0ull;
+    if (threadIdx.x == 0) col[0] = ~0ull;
+    uint32_t call = 0;  // emission calls, dealt round-robin to the waves
 #pragma unroll
-    for (int i = 0; i < 16; i++) emit_bits(col, 1 + 32 * i, W[i], 32, lane);
+    for (int i = 0; i < 16; i++) emit_bits(col, 1 + 32 * i, W[i], 32, lane, (call++ & 3) == wave);
 #pragma unroll
-    for (int i = 0; i < 8; i++) emit_bits(col, 513 + 32 * i, st[i], 32, lane);
+    for (int i = 0; i < 8; i++) emit_bits(col, 513 + 32 * i, st[i], 32, lane, (call++ & 3) == wave);
     int pos = 769;
 #pragma unroll
     for (int i = 16; i < 64; i++) {
         const uint32_t s0 = rotr32(W[i - 15], 7) ^ rotr32(W[i - 15], 18) ^ (W[i - 15] >> 3);
         const uint32_t s1 = rotr32(W[i - 2], 17) ^ rotr32(W[i - 2], 19) ^ (W[i - 2] >> 10);
         const uint64_t total = (uint64_t)W[i - 16] + s0 + W[i - 7] + s1;
-        emit_bits(col, pos, total, 34, lane);
+        emit_bits(col, pos, total, 34, lane, (call++ & 3) == wave);
         pos += 34;
         W[i] = (uint32_t)total;
     }
@@ -135,8 +141,8 @@ __global__ __launch_bounds__(64) void k_chain_witness(const uint32_t* __restrict
         const uint64_t common = (uint64_t)h + big1 + c_K[i] + W[i] + ch;
         const uint64_t e_half = common + d;
         const uint64_t a_half = common + big0 + maj;
-        emit_bits(col, pos, e_half, 35, lane);
-        emit_bits(col, pos + 35, a_half, 35, lane);
+        emit_bits(col, pos, e_half, 35, lane, (call++ & 3) == wave);
+        emit_bits(col, pos + 35, a_half, 35, lane, (call++ & 3) == wave);
         pos += 70;
         h = g;
         g = f;
@@ -150,7 +156,7 @@ __global__ __launch_bounds__(64) void k_chain_witness(const uint32_t* __restrict
     const uint32_t wk[8] = {a, b, c, d, e, f, g, h};
 #pragma unroll
     for (int i = 0; i < 8; i++) {
-        emit_bits(col, pos, (uint64_t)st[i] + wk[i], 33, lane);
+        emit_bits(col, pos, (uint64_t)st[i] + wk[i], 33, lane, (call++ & 3) == wave);
         pos += 33;
     }
     __syncthreads();
@@ -161,7 +167,7 @@ __global__ __launch_bounds__(64) void k_chain_witness(const uint32_t* __restrict
     const uint64_t wsel = (first & ((1ull << th) - 1)) >> 6;
     const uint64_t wpc = 1ull << (th - 6);
     const bool last_group = first + 64 == n;
-    for (uint32_t r = lane; r < (uint32_t)kChainHBar; r += 64) {
+    for (uint32_t r = threadIdx.x; r < (uint32_t)kChainHBar; r += blockDim.x) {
         uint64_t v = 0;
         if (r < (uint32_t)kHBar) {
             for (uint32_t k = m_off[r]; k < m_off[r + 1]; k++) v ^= col[m_cols[k]];
@@ -171,12 +177,9 @@ __global__ __launch_bounds__(64) void k_chain_witness(const uint32_t* __restrict
         }
         D[((uint64_t)r * H + hi) * wpc + wsel] = v;
     }
-    // the committed source stream, instance-aligned words shifted into place
-    const uint64_t start = 1 + inst * (uint64_t)kFInstance;
-    const int off = (int)(start & 63);
-    uint64_t wo = start >> 6;
-    uint64_t prev = 0;
-    for (int q = 0; q * 64 < kFInstance; q++) {
+    // the committed source stream: instance-aligned local words q (cells
+    // 64 q .. 64 q + 63 of the instance), shifted into place
+    auto local_word = [&](int q) -> uint64_t {
         uint64_t cur = 0;
         for (int bb = 0; bb < 64; bb++) {
             const int j = q * 64 + bb;
@@ -184,13 +187,22 @@ __global__ __launch_bounds__(64) void k_chain_witness(const uint32_t* __restrict
             const int cidx = j < 512 ? 1 + j : 769 + (j - 512);
             cur |= ((col[cidx] >> lane) & 1ull) << bb;
         }
+        return cur;
+    };
+    constexpr int kWords = (kFInstance + 63) / 64, kPer = (kWords + 3) / 4;
+    const uint64_t start = 1 + inst * (uint64_t)kFInstance, w0 = start >> 6;
+    const int off = (int)(start & 63);
+    const int q0 = (int)wave * kPer, q1 = min(kWords, q0 + kPer);
+    uint64_t prev = q0 > 0 ? local_word(q0 - 1) : 0;
+    for (int q = q0; q < q1; q++) {
+        const uint64_t cur = local_word(q);
         const uint64_t word = off ? (cur << off) | (prev >> (64 - off)) : cur;
-        if (word) atomicOr((unsigned long long*)&src_bits[wo], (unsigned long long)word);
-        wo++;
+        if (word) atomicOr((unsigned long long*)&src_bits[w0 + q], (unsigned long long)word);
         prev = cur;
     }
-    if (off && (prev >> (64 - off))) atomicOr((unsigned long long*)&src_bits[wo], (unsigned long long)(prev >> (64 - off)));
-    if (inst == 0) atomicOr((unsigned long long*)&src_bits[0], 1ull);
+    if (q1 == kWords && off && (prev >> (64 - off)))
+        atomicOr((unsigned long long*)&src_bits[w0 + kWords], (unsigned long long)(prev >> (64 - off)));
+    if (inst == 0 && wave == 0) atomicOr((unsigned long long*)&src_bits[0], 1ull);
 }
 
 // Transposed weights (VirtualStatement::transpose_query over the chained
@@ -310,10 +322,6 @@ void ligerito_bytes(const LigConfig& cfg, std::vector<uint8_t>* o) {
     o->push_back(1);  // BLAKE3
 }
 
-u128 modq(const ModN& F, int64_t v) {
-    const u128 q = F.n;
-    return v >= 0 ? (u128)v % q : (q - (u128)(-(v + 1)) % q - 1) % q;
-}
 std::vector<u128> eq_le(const ModN& F, const std::vector<u128>& pt) {
     std::vector<u128> t(1, 1);
     for (u128 c : pt) {
@@ -385,7 +393,10 @@ struct Outer {
 // prove_linear / verify_linear's outer transcript for the chain
 // (linear.rs:398-558, 586-705) from the statement frames to the fork tag
 int chain_outer(const Batch& b, const uint32_t* blocks, const uint32_t* digest, const uint8_t root[32],
-                const LigConfig& cfg, Transcript& tr, Round0Fn round0, void* user, Outer* o) {
+                const LigConfig& cfg, Transcript& tr, Round0Fn round0, void* user, Outer* o, PhaseClock* ph) {
+    auto mark = [&](const char* name) {
+        if (ph) ph->mark(name);
+    };
     int rc;
     const Relation& rel = *b.rel;
     const uint64_t n = b.n;
@@ -400,7 +411,9 @@ int chain_outer(const Batch& b, const uint32_t* blocks, const uint32_t* digest, 
         put64(&o, n);
         for (uint64_t i = 0; i < n * 16; i++) put32(&o, blocks[i]);
         for (int i = 0; i < 8; i++) put32(&o, digest[i]);
-        blake3_hash_host(o.data(), o.size(), stmt);
+        Blake3 hs;
+        hs.update(o.data(), o.size());
+        hs.finalize_xof(stmt, 32);
     }
     // chain_assignment_binding (chain.rs:1385-1418)
     {
@@ -436,7 +449,9 @@ int chain_outer(const Batch& b, const uint32_t* blocks, const uint32_t* digest, 
         ligerito_digest(cfg, dg);
         tr.absorb_slice(dg, 32);
     }
+    mark("chain_round0");
     if ((rc = round0(user, tr))) return rc;
+    mark("chain_prime");
     // the runtime prime and the relation frames
     {
         uint8_t v[16];
@@ -478,15 +493,28 @@ int chain_outer(const Batch& b, const uint32_t* blocks, const uint32_t* digest, 
     }
     const u128 alpha0 = a0[0], alpha_pub = lam.back();
     lam.pop_back();
+    mark("chain_batching");
     // beta = C^T eq(xi) and the rank-one coefficients (chain.rs:1080-1130)
     std::vector<u128> coef(kChainHBar, 0);
     {
-        const std::vector<u128> ex = eq_le(F, xi);
+        // eq(xi) in Montgomery form; a column's signed small coefficients
+        // (|v| < 2^37) are summed as exact 256-bit products, positive and
+        // negative apart, then reduced once each: redc(sum a R v) = sum a v
+        std::vector<u128> ex = eq_le(F, xi);
+        for (auto& v : ex) v = F.to_mont(v);
         for (int c = 0; c < kChainHBar; c++) {
-            u128 acc = 0;
-            for (uint32_t e = rel.c_off[c]; e < rel.c_off[c + 1]; e++)
-                acc = addmod(acc, F.mul(ex[rel.c_rows[e]], modq(F, rel.c_vals[e])), q);
-            coef[c] = acc;
+            u128 ph = 0, pl = 0, nh = 0, nl = 0;
+            for (uint32_t e = rel.c_off[c]; e < rel.c_off[c + 1]; e++) {
+                const int64_t v = rel.c_vals[e];
+                u128 hi, lo;
+                ModN::wide(ex[rel.c_rows[e]], (u128)(v >= 0 ? (uint64_t)v : (uint64_t)0 - (uint64_t)v), &hi, &lo);
+                u128& H = v >= 0 ? ph : nh;
+                u128& L = v >= 0 ? pl : nl;
+                L += lo;
+                H += hi + (L < lo ? 1 : 0);
+            }
+            const u128 pos = F.redc(ph, pl), neg = F.redc(nh, nl);
+            coef[c] = addmod(pos, neg ? q - neg : 0, q);
         }
         coef[0] = addmod(coef[0], alpha0, q);
         for (int p = 0; p < kPublicBits; p++) {
@@ -495,16 +523,18 @@ int chain_outer(const Batch& b, const uint32_t* blocks, const uint32_t* digest, 
         }
     }
     // mu = alpha0 sum_i u_i + alpha_pub sum_i u_i sum_p lambda_p bit_p (chain.rs:1131-1160)
+    mark("chain_mu");
     u128 mu;
     {
         const std::vector<u128> u = eq_le(F, eta);
+        // byte tables of the slot weights, unreduced: an entry sums at most 8
+        // weights (< 2^116) and an instance's value 96 entries (< 2^123), so
+        // u_i * value < 2^236 < R q and one Montgomery product per instance
+        // gives u_i value R^-1; the sum is brought back by R at the end
         std::vector<u128> tab((size_t)96 * 256);
         for (int t8 = 0; t8 < 96; t8++) {
             tab[t8 * 256] = 0;
-            for (int v = 1; v < 256; v++) {
-                const int bit = __builtin_ctz(v);
-                tab[t8 * 256 + v] = addmod(tab[t8 * 256 + (v & (v - 1))], lam[8 * t8 + bit], q);
-            }
+            for (int v = 1; v < 256; v++) tab[t8 * 256 + v] = tab[t8 * 256 + (v & (v - 1))] + lam[8 * t8 + __builtin_ctz(v)];
         }
         u128 usum = 0, acc = 0;
         for (uint64_t i = 0; i < n; i++) {
@@ -512,10 +542,11 @@ int chain_outer(const Batch& b, const uint32_t* blocks, const uint32_t* digest, 
             u128 val = 0;
             for (int ws = 0; ws < 24; ws++) {
                 const uint32_t word = ws < 16 ? blocks[i * 16 + ws] : (i + 1 == n ? digest[ws - 16] : 0u);
-                for (int by = 0; by < 4; by++) val = addmod(val, tab[(4 * ws + by) * 256 + ((word >> (8 * by)) & 255)], q);
+                for (int by = 0; by < 4; by++) val += tab[(4 * ws + by) * 256 + ((word >> (8 * by)) & 255)];
             }
-            acc = addmod(acc, F.mul(u[i], val), q);
+            acc = addmod(acc, F.mont(u[i], val), q);
         }
+        acc = F.to_mont(acc);
         mu = addmod(F.mul(alpha0, usum), F.mul(alpha_pub, acc), q);
     }
     // chain_product_opening_claim (chain.rs:1201-1251)
@@ -525,9 +556,13 @@ int chain_outer(const Batch& b, const uint32_t* blocks, const uint32_t* digest, 
     const uint64_t H = high.size(), ncols = 1ull << b.sh;
     o->colw.assign(ncols, 0);
     std::vector<u128>& colw = o->colw;
-    for (uint64_t c = 0; c < std::min<uint64_t>(ncols, (uint64_t)kChainHBar * H); c++)
-        colw[c] = F.mul(coef[c / H], high[c % H]);
+    if (H == 1)  // eq over no coordinates: the coefficients themselves
+        std::copy(coef.begin(), coef.end(), colw.begin());
+    else
+        for (uint64_t c = 0; c < std::min<uint64_t>(ncols, (uint64_t)kChainHBar * H); c++)
+            colw[c] = F.mul(coef[c / H], high[c % H]);
     // the claim frame (linear.rs:324-368)
+    mark("chain_claim_frame");
     {
         Blake3 hd;
         std::vector<uint8_t> o;
@@ -622,7 +657,7 @@ int prove_chain(bitz_ctx* ctx, const uint32_t* blocks, int k, const uint32_t* in
     BITZ_HIP(hipMemcpyAsync(dstates.ptr, states.data(), (n + 1) * 32, hipMemcpyHostToDevice, ctx->stream));
     BITZ_HIP(hipMemsetAsync(src.ptr, 0, src_words * 8, ctx->stream));
     BITZ_HIP(hipMemsetAsync(D.ptr, 0, d_words * 8, ctx->stream));
-    hipLaunchKernelGGL(k_chain_witness, dim3((unsigned)(n / 64)), dim3(64), 0, ctx->stream,
+    hipLaunchKernelGGL(k_chain_witness, dim3((unsigned)(n / 64)), dim3(256), 0, ctx->stream,
                        (const uint32_t*)dblocks.ptr, (const uint32_t*)dstates.ptr, n, (const uint32_t*)cd.m_off.ptr,
                        (const uint32_t*)cd.m_cols.ptr, b.th, (uint64_t*)D.ptr, (uint64_t*)src.ptr);
     BITZ_LAUNCH_CHECK();
@@ -644,7 +679,7 @@ int prove_chain(bitz_ctx* ctx, const uint32_t* blocks, int k, const uint32_t* in
     Transcript tr;
     ProveR0 pr0{ctx, h.get(), {}};
     Outer O;
-    if ((rc = chain_outer(b, blocks, digest, root, cfg, tr, prove_round0, &pr0, &O))) return rc;
+    if ((rc = chain_outer(b, blocks, digest, root, cfg, tr, prove_round0, &pr0, &O, &phase))) return rc;
     if (state_out) tr.digest(state_out);
     phase.mark("chain_open");
     const std::vector<std::vector<uint8_t>> lead = virtual_lead(b, root, O.q);
@@ -756,7 +791,7 @@ int verify_chain(bitz_ctx* ctx, const uint32_t* blocks, int k, const uint32_t* i
     Transcript tr;
     VerifyR0 vr0{&ob, b.tf + b.sf - 7, (uint32_t)cfg.round0_grinding_bits, {}, {0, 0}};
     Outer O;
-    if ((rc = chain_outer(b, blocks, digest, root, cfg, tr, verify_round0, &vr0, &O))) return rc;
+    if ((rc = chain_outer(b, blocks, digest, root, cfg, tr, verify_round0, &vr0, &O, nullptr))) return rc;
     if (state_out) tr.digest(state_out);
     ChainDev cd;
     if ((rc = upload_relation(ctx, *b.rel, &cd))) return rc;

@@ -19,6 +19,7 @@
 #include "host_proto.h"
 #include "modq.h"
 #include "open_impl.h"
+#include "residual.h"
 #include "spartan_impl.h"
 
 namespace bitz {
@@ -1517,11 +1518,12 @@ int Prover::prove_lin(InnerTranscript& it, PhaseClock& phase, DevBuf& u1, DevBuf
 
 // ------------------------------------------------------------- the verifier
 // verify_standalone (src/piop/spartan/protocol/bitz_opener.rs:709-747) ->
-// BitZVerifier::verify (src/bitz/mod.rs:218-255) -> the recursive Ligerito
-// verifier (pcs/ligerito.rs:4585-4760) with the residual evaluated densely.
-// Transcript checks run on the host; the dense tables (eq tables, images,
-// the ring-switch basis, the induced bases and their folds) are built by the
-// prover's kernels on the context's device.
+// BitZVerifier::verify (src/bitz/mod.rs:218-255) -> the succinct recursive
+// Ligerito verifier (pcs/ligerito.rs:3967-4440, src/bitz/pcs.rs:508-618).
+// Transcript checks, Merkle multi-proofs and the residual (every basis
+// component in closed form, residual.h) run on the host; the tables the size
+// of the claim's grid (fold images, eq tables over its rows and columns and
+// their folds) are built by the prover's kernels on the context's device.
 #define V_CHECK(cond, msg)                                  \
     do {                                                    \
         if (!(cond)) {                                      \
@@ -2111,21 +2113,11 @@ int verify_lin_impl(bitz_ctx* ctx, InnerReader& it, const uint8_t root[32], int 
     }
     const G eta = it.gf();
     packed_target = hgf_add(packed_target, hgf_mul(eta, ood_y));
-    // the dense initial basis b(x) = sum_i bit_i(suffix[x]) bw[i] + eta eq(opt)[x]
-    const uint64_t np = 1ull << mp;
-    DevBuf suffix, basis, bwd, eqo;
-    if ((rc = V.eq_into(std::vector<G>(sc_point.begin() + 7, sc_point.end()), suffix))) return rc;
-    if (!bwd.alloc(128 * 16) || !basis.alloc(np * 16)) return BITZ_ERR_ALLOC;
-    if ((rc = upload_async(ctx, bw.data(), 128 * 16, bwd.ptr))) return rc;
-    if ((rc = fold_b128(ctx, suffix.ptr, np, bwd.ptr, basis.ptr))) return rc;
-    if ((rc = V.eq_into(opt, eqo))) return rc;
-    {
-        uint8_t eb[16];
-        gput(eta, eb);
-        if ((rc = axpy(ctx, basis.ptr, eqo.ptr, np, eb))) return rc;
-    }
-    suffix = DevBuf();
-    eqo = DevBuf();
+    // the initial basis b(x) = sum_i bit_i(eq(suffix, x)) bw[i] + eta eq(opt, x)
+    // over mp coordinates is never built: the final check evaluates it at the
+    // residual in closed form (residual.h)
+    const std::vector<G> suffix_pt(sc_point.begin() + 7, sc_point.end());
+    V_CHECK((int)suffix_pt.size() == mp && (int)opt.size() == mp, "Round-0 point length");
     // ---- the Ligerito proof of the hint stream ----
     V_CHECK(hints_len >= 4, "missing Ligerito hint");
     uint32_t ln;
@@ -2238,36 +2230,29 @@ int verify_lin_impl(bitz_ctx* ctx, InnerReader& it, const uint8_t root[32], int 
         }
         return tot;
     };
-    // combined basis components: (device table, entries, first challenge index, weight)
-    struct Comp {
-        std::unique_ptr<DevBuf> tab;
-        uint64_t n;
+    // basis components introduced by the recursion, each folded by the
+    // challenges after its introduction (index `start` into ris) and weighted
+    // by its batching challenge: the induced bases of the query sets and the
+    // eq bases of the OOD samples (pcs/ligerito.rs:4362-4412)
+    struct InducedComp {
+        int log_n;
+        std::vector<uint64_t> queries;
+        std::vector<G> aw;
         size_t start;
         G w;
     };
-    std::vector<Comp> comps;
-    comps.push_back(Comp{std::make_unique<DevBuf>(std::move(basis)), np, 0, gone()});
-    DevBuf qdev, sksd, awd;
+    struct EqComp {
+        std::vector<G> z;
+        size_t start;
+        G w;
+    };
+    std::vector<InducedComp> induced_comps;
+    std::vector<EqComp> eq_comps;
     auto induce_comp = [&](int log_n, const std::vector<uint64_t>& queries, const std::vector<G>& alpha, size_t start,
                            G w) -> int {
         std::vector<G> aw = host_eq(alpha);
         aw.resize(queries.size());
-        std::vector<G> sks, inv;
-        sk_at_vks(log_n, &sks, &inv);
-        sks.insert(sks.end(), inv.begin(), inv.end());
-        if (!qdev.alloc(queries.size() * 8 + 16) || !sksd.alloc(sks.size() * 16) ||
-            !awd.alloc(std::max<size_t>(aw.size(), 1) * 16))
-            return BITZ_ERR_ALLOC;
-        int r_;
-        if ((r_ = upload_async(ctx, queries.data(), queries.size() * 8, qdev.ptr))) return r_;
-        if ((r_ = upload_async(ctx, sks.data(), sks.size() * 16, sksd.ptr))) return r_;
-        if ((r_ = upload_async(ctx, aw.data(), aw.size() * 16, awd.ptr))) return r_;
-        auto tab = std::make_unique<DevBuf>();
-        if (!tab->alloc(16ull << log_n)) return BITZ_ERR_ALLOC;
-        if ((r_ = induce_basis(ctx, (const uint64_t*)qdev.ptr, (uint32_t)queries.size(), log_n, sksd.ptr,
-                               (const uint4*)sksd.ptr + (log_n + 1), awd.ptr, tab->ptr)))
-            return r_;
-        comps.push_back(Comp{std::move(tab), 1ull << log_n, start, w});
+        induced_comps.push_back(InducedComp{log_n, queries, std::move(aw), start, w});
         return BITZ_OK;
     };
     it.public_message(std::string("flock-ligerito-basis-v0"));
@@ -2301,10 +2286,7 @@ int verify_lin_impl(bitz_ctx* ctx, InnerReader& it, const uint8_t root[32], int 
             const G beta = it.gf();
             rq = qfold(rq, iq, beta);
             t_r = hgf_add(t_r, hgf_mul(beta, y));
-            auto tab = std::make_unique<DevBuf>();
-            int r_ = V.eq_into(z, *tab);
-            if (r_) return r_;
-            comps.push_back(Comp{std::move(tab), 1ull << n_vars, ris.size(), beta});
+            eq_comps.push_back(EqComp{z, ris.size(), beta});
         }
         return BITZ_OK;
     };
@@ -2356,13 +2338,31 @@ int verify_lin_impl(bitz_ctx* ctx, InnerReader& it, const uint8_t root[32], int 
             const G beta_l = it.gf();
             t_r = hgf_add(t_r, hgf_mul(beta_l, esl));
             if ((rc = induce_comp(n_cur, ql, al, ris.size(), beta_l))) return rc;
-            // dense residual: each component folded by the challenges after it
-            std::vector<G> combined(lp.yr.size(), gzero());
-            for (const Comp& c : comps) {
-                std::vector<G> vals;
-                if ((rc = V.fold_chain(c.tab->ptr, c.n, ris, c.start, &vals))) return rc;
-                V_CHECK(vals.size() == lp.yr.size(), "residual length");
-                for (size_t y = 0; y < vals.size(); y++) combined[y] = hgf_add(combined[y], hgf_mul(c.w, vals[y]));
+            // succinct residual: every component at (its folding challenges, y)
+            // for the binary points y of the last n_cur coordinates
+            using residual::E;
+            const uint64_t ny = lp.yr.size();
+            V_CHECK((int)ris.size() + n_cur == mp, "residual dimension");
+            std::vector<G> combined(ny), part(ny);
+            residual::ring_switch((const E*)suffix_pt.data(), mp, (const E*)ris.data(), (int)ris.size(),
+                                  (const E*)bw.data(), (E*)combined.data());
+            residual::eq_point((const E*)opt.data(), mp, (const E*)ris.data(), n_cur, E{eta.lo, eta.hi},
+                               (E*)part.data());
+            for (uint64_t y = 0; y < ny; y++) combined[y] = hgf_add(combined[y], part[y]);
+            for (const EqComp& c : eq_comps) {
+                V_CHECK(c.start + c.z.size() == ris.size() + n_cur, "residual dimension");
+                residual::eq_point((const E*)c.z.data(), (int)c.z.size(), (const E*)ris.data() + c.start, n_cur,
+                                   E{c.w.lo, c.w.hi}, (E*)part.data());
+                for (uint64_t y = 0; y < ny; y++) combined[y] = hgf_add(combined[y], part[y]);
+            }
+            for (const InducedComp& c : induced_comps) {
+                V_CHECK(c.start + c.log_n == ris.size() + n_cur, "residual dimension");
+                std::vector<G> sks, inv;
+                sk_at_vks(c.log_n, &sks, &inv);
+                residual::induced(c.log_n, c.queries.data(), (int)c.queries.size(), (const E*)c.aw.data(),
+                                  (const E*)sks.data(), (const E*)inv.data(), (const E*)ris.data() + c.start, n_cur,
+                                  (E*)part.data());
+                for (uint64_t y = 0; y < ny; y++) combined[y] = hgf_add(combined[y], hgf_mul(c.w, part[y]));
             }
             G inner = gzero();
             for (size_t y = 0; y < combined.size(); y++) inner = hgf_add(inner, hgf_mul(lp.yr[y], combined[y]));

@@ -476,6 +476,7 @@ int chain_outer(const Batch& b, const uint32_t* blocks, const uint32_t* digest, 
         spartan_frame(tr, "runtime-field-modulus", v, 16);
         spartan_frame(tr, "projected-linear-relation", rel.digest, 32);
     }
+    mark("chain_points");
     ModN F;
     F.set(q);
     auto point = [&](const char* domain, int vars, std::vector<u128>* out) -> bool {

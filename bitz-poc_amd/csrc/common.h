@@ -247,6 +247,10 @@ constexpr double kGfMulOps = 430.0;
 constexpr double kGfTabMulOps = 125.0;
 constexpr double kBlake3Ops = 800.0;
 
+// bitz_ctx_set_timing_only with this name records the host phases and times
+// no launch (no kernel has this name)
+constexpr const char* kPhasesOnly = "phases";
+
 // RAII bracket: records start/stop events on ctx->stream when timing is on.
 struct TimedLaunch {
     bitz_ctx* ctx;

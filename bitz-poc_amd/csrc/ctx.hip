@@ -45,7 +45,7 @@ TimedLaunch::~TimedLaunch() {
 }
 
 void PhaseClock::close() {
-    if (!cur.empty() && ctx->timing && ctx->timing_only.empty()) {
+    if (!cur.empty() && ctx->timing && (ctx->timing_only.empty() || ctx->timing_only == kPhasesOnly)) {
         double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
         auto& st = ctx->stats["phase:" + cur];
         st.launches++;

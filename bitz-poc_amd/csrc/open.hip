@@ -1210,6 +1210,92 @@ __global__ void k_bits_to_gf(const uint64_t* __restrict__ bits, uint64_t n, uint
         gf_store_nt(&out[x], gf128{{(uint32_t)((bits[x >> 6] >> (x & 63)) & 1ull), 0u, 0u, 0u}});
 }
 
+// Inner-product sumcheck rounds J = 0, 1, 2 over a single column of bits
+// (src/bitz/sumcheck.rs:146-198, the table m being the committed bits). After
+// J folds the table is A_J[k] = sum_c e_c bit(2^J k + c), e the eq table of
+// the J challenges, so each round sum splits by c into XORs of the weights
+// at the set bits: acc[s 2^J + c] for s = (m0 w0, m1 w1, (m0 + m1)(w0 + w1));
+// the host weighs them by e_c. For J > 0 the kernel first folds the weights
+// by the last challenge: W_J[k] = W[2k] + r (W[2k] + W[2k + 1]) (written to
+// w_out, 2 per pair), so A is never materialised and the round needs no
+// variable product. Thread pair i covers bits 2^(J+1) i .. 2^(J+1) (i + 1).
+template <int J>
+__global__ __launch_bounds__(kT) void k_bits_round(const uint64_t* __restrict__ bits, const uint4* __restrict__ w,
+                                                   uint64_t pairs, uint64_t r_lo, uint64_t r_hi,
+                                                   uint4* __restrict__ w_out, uint4* __restrict__ partials) {
+    constexpr int C = 1 << J, K = 3 * C, NB = 2 * C;
+    __shared__ uint4 tab[512];
+    if (J > 0) {
+        gf_table_build(tab, gf_from_u64(r_lo, r_hi), threadIdx.x, blockDim.x);
+        __syncthreads();
+    }
+    gf128 acc[K];
+#pragma unroll
+    for (int k = 0; k < K; k++) acc[k] = gf_zero();
+    for (uint64_t i = grid_tid(); i < pairs; i += grid_threads()) {
+        gf128 w0, w1;
+        if (J == 0) {
+            w0 = gf_load(&w[2 * i]);
+            w1 = gf_load(&w[2 * i + 1]);
+        } else {
+            const gf128 a0 = gf_load(&w[4 * i]), a1 = gf_load(&w[4 * i + 1]);
+            const gf128 b0 = gf_load(&w[4 * i + 2]), b1 = gf_load(&w[4 * i + 3]);
+            w0 = gf_add(a0, gf_mul_tab(tab, gf_add(a0, a1)));
+            w1 = gf_add(b0, gf_mul_tab(tab, gf_add(b0, b1)));
+            gf_store(&w_out[2 * i], w0);
+            gf_store(&w_out[2 * i + 1], w1);
+        }
+        const gf128 d = gf_add(w0, w1);
+        const uint64_t bit0 = (uint64_t)NB * i;
+        const uint32_t pat = (uint32_t)(bits[bit0 >> 6] >> (bit0 & 63)) & ((1u << NB) - 1);
+#pragma unroll
+        for (int c = 0; c < C; c++) {
+            const uint32_t e = (pat >> c) & 1u, o = (pat >> (C + c)) & 1u;
+            const uint32_t me = 0u - e, mo = 0u - o, mx = 0u - (e ^ o);
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                acc[c].w[q] ^= w0.w[q] & me;
+                acc[C + c].w[q] ^= w1.w[q] & mo;
+                acc[2 * C + c].w[q] ^= d.w[q] & mx;
+            }
+        }
+    }
+    block_xor_store<K>(acc, partials + (size_t)blockIdx.x * K);
+}
+
+// Round 3's start for the single bit column: W_3 = fold of W_2 by r (as in
+// k_bits_round) and A_3[k] = T[byte k of the bits], T[v] = sum_{c in v} e_c
+// over the eq table e of the first three challenges (built per block).
+struct E8 {
+    uint4 e[8];
+};
+__global__ __launch_bounds__(kT) void k_bits_materialize(const uint64_t* __restrict__ bits,
+                                                         const uint4* __restrict__ w, uint64_t n_out, uint64_t r_lo,
+                                                         uint64_t r_hi, E8 e8, uint4* __restrict__ a_out,
+                                                         uint4* __restrict__ w_out) {
+    __shared__ uint4 tab[512];
+    __shared__ uint4 pt[256];
+    gf_table_build(tab, gf_from_u64(r_lo, r_hi), threadIdx.x, blockDim.x);
+    for (int v = threadIdx.x; v < 256; v += blockDim.x) {
+        uint4 s = make_uint4(0, 0, 0, 0);
+        for (int c = 0; c < 8; c++)
+            if ((v >> c) & 1) {
+                s.x ^= e8.e[c].x;
+                s.y ^= e8.e[c].y;
+                s.z ^= e8.e[c].z;
+                s.w ^= e8.e[c].w;
+            }
+        pt[v] = s;
+    }
+    __syncthreads();
+    for (uint64_t k = grid_tid(); k < n_out; k += grid_threads()) {
+        const gf128 a0 = gf_load(&w[2 * k]), a1 = gf_load(&w[2 * k + 1]);
+        gf_store(&w_out[k], gf_add(a0, gf_mul_tab(tab, gf_add(a0, a1))));
+        const uint32_t byte = (uint32_t)(bits[k >> 3] >> ((k & 7) * 8)) & 255u;
+        a_out[k] = pt[byte];
+    }
+}
+
 __global__ void k_mul_inplace(uint4* v, const uint4* __restrict__ b, uint64_t n) {
     for (uint64_t x = grid_tid(); x < n; x += grid_threads()) gf_store(&v[x], gf_mul(gf_load(&v[x]), gf_load(&b[x])));
 }
@@ -1492,6 +1578,8 @@ int sums_finish(bitz_ctx* ctx, unsigned nblocks, int K, void* out_dev) {
             case 1: hipLaunchKernelGGL(k_xor_partials_mbox<1>, dim3(1), dim3(kT), 0, ctx->stream, parts, nblocks, dst, vals, flag, seq); break;
             case 2: hipLaunchKernelGGL(k_xor_partials_mbox<2>, dim3(1), dim3(kT), 0, ctx->stream, parts, nblocks, dst, vals, flag, seq); break;
             case 3: hipLaunchKernelGGL(k_xor_partials_mbox<3>, dim3(1), dim3(kT), 0, ctx->stream, parts, nblocks, dst, vals, flag, seq); break;
+            case 6: hipLaunchKernelGGL(k_xor_partials_mbox<6>, dim3(1), dim3(kT), 0, ctx->stream, parts, nblocks, dst, vals, flag, seq); break;
+            case 12: hipLaunchKernelGGL(k_xor_partials_mbox<12>, dim3(1), dim3(kT), 0, ctx->stream, parts, nblocks, dst, vals, flag, seq); break;
             default: return BITZ_ERR_ARG;
         }
         BITZ_LAUNCH_CHECK();
@@ -1503,6 +1591,8 @@ int sums_finish(bitz_ctx* ctx, unsigned nblocks, int K, void* out_dev) {
         case 1: hipLaunchKernelGGL(k_xor_partials<1>, dim3(1), dim3(kT), 0, ctx->stream, parts, nblocks, dst); break;
         case 2: hipLaunchKernelGGL(k_xor_partials<2>, dim3(1), dim3(kT), 0, ctx->stream, parts, nblocks, dst); break;
         case 3: hipLaunchKernelGGL(k_xor_partials<3>, dim3(1), dim3(kT), 0, ctx->stream, parts, nblocks, dst); break;
+        case 6: hipLaunchKernelGGL(k_xor_partials<6>, dim3(1), dim3(kT), 0, ctx->stream, parts, nblocks, dst); break;
+        case 12: hipLaunchKernelGGL(k_xor_partials<12>, dim3(1), dim3(kT), 0, ctx->stream, parts, nblocks, dst); break;
         default: return BITZ_ERR_ARG;
     }
     BITZ_LAUNCH_CHECK();
@@ -2363,6 +2453,42 @@ int mul_elem(bitz_ctx* ctx, const void* a, const void* b, uint64_t n, void* out,
     else
         hipLaunchKernelGGL(k_mul_elem, dim3(blocks_for(n, 4)), dim3(kT), 0, ctx->stream, (const uint4*)a,
                            (const uint4*)b, n, (uint4*)out);
+    BITZ_LAUNCH_CHECK();
+    return BITZ_OK;
+}
+
+int bits_round(bitz_ctx* ctx, int J, const uint64_t* bits, const void* w, uint64_t n, const uint8_t r[16],
+               void* w_out) {
+    BITZ_REQUIRE(J >= 0 && J <= 2 && n >= 2 && n % 2 == 0, BITZ_ERR_ARG, "bits_round: round or size out of range");
+    BITZ_REQUIRE(J == 0 || (r && w_out && w_out != w), BITZ_ERR_ARG, "bits_round: fold needs a challenge and an output");
+    uint64_t lo = 0, hi = 0;
+    if (J > 0) split16(r, &lo, &hi);
+    const uint64_t pairs = n / 2;  // pairs of the round's table (n entries after J folds)
+    const int K = 3 << J;
+    const unsigned nb = blocks_for(pairs, 4, 2048);
+    int rc = ctx->scratch.ensure((size_t)nb * K * 16);
+    if (rc) return rc;
+    uint4* parts = (uint4*)ctx->scratch.partials.ptr;
+    TimedLaunch tl(ctx, "bits_round", (double)pairs * (J ? 96.0 : 32.0) + (double)n / 8 * (1 << J), 0.0);
+    switch (J) {
+        case 0: hipLaunchKernelGGL(k_bits_round<0>, dim3(nb), dim3(kT), 0, ctx->stream, bits, (const uint4*)w, pairs, lo, hi, (uint4*)w_out, parts); break;
+        case 1: hipLaunchKernelGGL(k_bits_round<1>, dim3(nb), dim3(kT), 0, ctx->stream, bits, (const uint4*)w, pairs, lo, hi, (uint4*)w_out, parts); break;
+        default: hipLaunchKernelGGL(k_bits_round<2>, dim3(nb), dim3(kT), 0, ctx->stream, bits, (const uint4*)w, pairs, lo, hi, (uint4*)w_out, parts); break;
+    }
+    BITZ_LAUNCH_CHECK();
+    return sums_finish(ctx, nb, K, nullptr);
+}
+
+int bits_materialize(bitz_ctx* ctx, const uint64_t* bits, const void* w, uint64_t n_out, const uint8_t r[16],
+                     const uint8_t e8[128], void* a_out, void* w_out) {
+    BITZ_REQUIRE(r && e8 && a_out && w_out && w_out != w, BITZ_ERR_ARG, "bits_materialize: bad arguments");
+    uint64_t lo, hi;
+    split16(r, &lo, &hi);
+    E8 e;
+    memcpy(&e, e8, sizeof(E8));
+    TimedLaunch tl(ctx, "bits_materialize", 64.0 * (double)n_out + (double)n_out / 8, 0.0);
+    hipLaunchKernelGGL(k_bits_materialize, dim3(blocks_for(n_out, 4)), dim3(kT), 0, ctx->stream, bits, (const uint4*)w,
+                       n_out, lo, hi, e, (uint4*)a_out, (uint4*)w_out);
     BITZ_LAUNCH_CHECK();
     return BITZ_OK;
 }

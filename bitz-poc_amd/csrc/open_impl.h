@@ -134,6 +134,19 @@ int mul_inplace(bitz_ctx* ctx, void* v, const void* b, uint64_t n);
 int scale(bitz_ctx* ctx, const void* a, uint64_t n, const uint8_t c[16], void* out);
 // out[x] = bit x of a flat bit stream (2^k bits as Gf128 0 or 1)
 int bits_to_gf(bitz_ctx* ctx, const uint64_t* bits, uint64_t n, void* out);
+// Inner-product sumcheck round J (0..2) over a single column of bits: the
+// round's table has n entries (the column after J folds); for J > 0 the
+// weights w (2n entries) are first folded by r into w_out (n entries). The
+// 3 * 2^J XOR accumulators acc[s 2^J + c] go to the mailbox (read_result):
+// round sum s is sum_c e_c acc[s 2^J + c], e the eq table of the J earlier
+// challenges (s: m0 w0, m1 w1, (m0 + m1)(w0 + w1)).
+int bits_round(bitz_ctx* ctx, int J, const uint64_t* bits, const void* w, uint64_t n, const uint8_t r[16],
+               void* w_out);
+// The table after three folds, A_3[k] = sum_{c < 8} e8[c] bit(8k + c) (e8:
+// the eq table of the three challenges, 8 elements), and the weights folded
+// by r from 2 n_out to n_out entries.
+int bits_materialize(bitz_ctx* ctx, const uint64_t* bits, const void* w, uint64_t n_out, const uint8_t r[16],
+                     const uint8_t e8[128], void* a_out, void* w_out);
 
 // commit pipeline (commit.hip)
 // One GKR round's Fiat-Shamir step on the device, after a sums launch whose

@@ -841,6 +841,57 @@ struct Prover {
     }
     G final_a{0, 0}, final_w{0, 0};  // the tables' last entries after the rounds
 
+    // The inner-product sumcheck over a single column of committed bits
+    // (n = 2^m rows, weights w): rounds 0..2 from the bits themselves
+    // (bits_round: XOR accumulators weighed here by the eq table of the
+    // earlier challenges, the weights folded on the way), then the table
+    // after three folds from byte patterns (bits_materialize) and the
+    // remaining rounds as sumcheck_rounds. Same messages as bits_to_gf
+    // followed by sumcheck_rounds.
+    int bit_sumcheck_rounds(InnerTranscript& it, DevBuf& w, uint64_t n, G* target, std::vector<G>& point) {
+        int rc;
+        const uint64_t* bits = (const uint64_t*)h->packed.ptr;
+        DevBuf wa, wb, a3, a2, w2;
+        if (!wa.alloc((n / 2) * 16) || !wb.alloc((n / 4) * 16) || !a3.alloc((n / 8) * 16) ||
+            !a2.alloc((n / 16) * 16) || !w2.alloc((n / 16) * 16))
+            return BITZ_ERR_ALLOC;
+        std::vector<G> rs;
+        uint8_t rb[16];
+        for (int J = 0; J < 3; J++) {
+            const void* win = J == 2 ? wa.ptr : w.ptr;
+            void* wout = J == 0 ? nullptr : (J == 1 ? wa.ptr : wb.ptr);
+            if ((rc = bits_round(ctx, J, bits, win, n >> J, J ? rb : nullptr, wout))) return rc;
+            const int C = 1 << J;
+            uint8_t acc[12 * 16];
+            if ((rc = read_result(ctx, 3 * C, acc))) return rc;
+            const std::vector<G> e = host_eq(rs);
+            G v[3] = {gzero(), gzero(), gzero()};
+            for (int sidx = 0; sidx < 3; sidx++)
+                for (int c = 0; c < C; c++) v[sidx] = hgf_add(v[sidx], hgf_mul(e[c], gfrom(acc + 16 * (sidx * C + c))));
+            G c0 = v[0], c2 = v[2], c1 = hgf_add(hgf_add(v[0], v[1]), v[2]);
+            if (!geq(hgf_add(c1, c2), *target)) {
+                set_error("inner sumcheck: claim does not match the committed bits");
+                return BITZ_ERR_ARG;
+            }
+            uint8_t msg[48];
+            gput(c0, msg);
+            gput(c1, msg + 16);
+            gput(c2, msg + 32);
+            it.prover_message(msg, 48);
+            it.verifier_gf(rb);
+            const G r = gfrom(rb);
+            *target = hgf_add(c0, hgf_mul(r, hgf_add(c1, hgf_mul(r, c2))));
+            point.push_back(r);
+            rs.push_back(r);
+        }
+        // A_3 from byte patterns over e(r0, r1, r2), W_3 = fold of W_2 by r2
+        const std::vector<G> e8 = host_eq(rs);
+        uint8_t eb[128];
+        for (int c = 0; c < 8; c++) gput(e8[c], eb + 16 * c);
+        if ((rc = bits_materialize(ctx, bits, wb.ptr, n / 8, rb, eb, a3.ptr, wa.ptr))) return rc;
+        return sumcheck_rounds(it, a3, wa, a2, w2, n / 8, target, point);
+    }
+
     int commit_level(const void* f, int log_msg, int log_lanes, int rate, DevBuf& cw, DevBuf& tree, uint8_t root[32]) {
         int k_code = log_msg - log_lanes + rate;
         if (!cw.alloc((16ull << (log_msg + rate))) || !tree.alloc((2ull << k_code) * 32)) return BITZ_ERR_ALLOC;
@@ -982,12 +1033,13 @@ int Prover::run(const uint8_t* claimed_in, uint8_t* claimed_out, std::vector<uin
     int rc;
     const uint64_t nrows = 1ull << t, ncols = 1ull << s;
     PhaseClock phase(ctx);
-    phase.mark(proof_out ? "statement_round0" : "eval_statement_round0");
+    phase.mark(proof_out ? "statement" : "eval_statement");
     // ---------------- outer transcript: statement and Round 0 ----------------
     Transcript outer;
     uint8_t gb[16];
     gput(g, gb);
     bind_standalone_statement(outer, h->root, cfg, t, s, gb);
+    phase.mark(proof_out ? "round0" : "eval_round0");
     const uint32_t r0bits = (uint32_t)cfg.round0_grinding_bits;
     ood_round_header(outer, mp, r0bits);
     uint64_t r0_nonce = 0;
@@ -1179,16 +1231,21 @@ int Prover::prove_lin(InnerTranscript& it, PhaseClock& phase, DevBuf& u1, DevBuf
     }
     it.public_message(std::string("bitz/pcs/inner-product-sumcheck/v1"));
     // row rounds on the column-combined table (a single column: the bits)
-    DevBuf comb, a2, w2;
-    if (!comb.alloc(nrows * 16) || !a2.alloc(nrows * 16) || !w2.alloc(nrows * 16)) return BITZ_ERR_ALLOC;
-    if (ls > 0) {
-        if ((rc = xi_combined(ctx, (const uint64_t*)packed.ptr, lt, ls, u2.ptr, comb.ptr))) return rc;
-    } else {
-        if ((rc = bits_to_gf(ctx, (const uint64_t*)h->packed.ptr, nrows, comb.ptr))) return rc;
-    }
     std::vector<G> sc_point;
     G target = ip_target;
-    if ((rc = sumcheck_rounds(it, comb, u1, a2, w2, nrows, &target, sc_point))) return rc;
+    if (ls == 0 && nrows >= 64) {
+        // a single column: the committed bits themselves
+        if ((rc = bit_sumcheck_rounds(it, u1, nrows, &target, sc_point))) return rc;
+    } else {
+        DevBuf comb, a2, w2;
+        if (!comb.alloc(nrows * 16) || !a2.alloc(nrows * 16) || !w2.alloc(nrows * 16)) return BITZ_ERR_ALLOC;
+        if (ls > 0) {
+            if ((rc = xi_combined(ctx, (const uint64_t*)packed.ptr, lt, ls, u2.ptr, comb.ptr))) return rc;
+        } else {
+            if ((rc = bits_to_gf(ctx, (const uint64_t*)h->packed.ptr, nrows, comb.ptr))) return rc;
+        }
+        if ((rc = sumcheck_rounds(it, comb, u1, a2, w2, nrows, &target, sc_point))) return rc;
+    }
     uint8_t evb[32];
     if (ls > 0) {
         uint8_t rs_bytes[16];
@@ -1477,7 +1534,7 @@ int Prover::prove_lin(InnerTranscript& it, PhaseClock& phase, DevBuf& u1, DevBuf
         std::swap(tree_prev, tree_next);
         prev_level = i + 2;
     }
-    phase.mark("encode");
+    phase.mark("proof_bytes");
     // ---------------- LigeritoProof as a hint (write_opening_proof) ----------------
     Writer w;
     w.bytes(h->root, 32);

@@ -80,8 +80,10 @@ const char* bitz_last_error(void);
  * returns the number of kernels. */
 int bitz_ctx_set_timing(bitz_ctx* ctx, int enable);
 /* Timing restricted to the launches of one kernel name (as reported by
- * bitz_ctx_kernel_stats); host phases are not recorded. Stats reset. A
- * measurement aid of this library, with no counterpart in the reference. */
+ * bitz_ctx_kernel_stats); host phases are not recorded. Stats reset. The
+ * name "phases" times no launch and records the host phases only (wall
+ * clock between phase marks, without the events' cost). A measurement aid
+ * of this library, with no counterpart in the reference. */
 int bitz_ctx_set_timing_only(bitz_ctx* ctx, const char* kernel);
 int bitz_ctx_kernel_stats(bitz_ctx* ctx, char* names, uint64_t names_cap, double* values, uint64_t values_cap);
 /* Launch-shape choices of the context (none changes a result; the reference

@@ -217,33 +217,60 @@ struct MapDev {
     const uint32_t* off;
     const uint32_t* rows;
 };
-__device__ __forceinline__ gf128 part_sum(MapDev m, uint32_t lc, uint64_t inst, const uint4* __restrict__ u2,
-                                          uint64_t H, int th) {
-    gf128 acc = gf_zero();
-    const uint64_t colhi = inst >> th;
-    for (uint32_t k = m.off[lc]; k < m.off[lc + 1]; k++) acc = gf_add(acc, gf_load(&u2[m.rows[k] * H + colhi]));
-    return acc;
-}
-__global__ __launch_bounds__(256) void k_chain_transpose(const uint4* __restrict__ u1, const uint4* __restrict__ u2,
-                                                         uint64_t n, int th, uint64_t cells, MapDev loc, MapDev prv,
-                                                         MapDev fst, MapDev lst, uint4* __restrict__ wf) {
-    const uint64_t H = n >> th, mask = (1ull << th) - 1, live = 1 + n * (uint64_t)kFInstance;
-    for (uint64_t src = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x + 1; src < cells;
-         src += (uint64_t)gridDim.x * blockDim.x) {
+// P[p][h][lc] (map part p = local, prev, first, last; h < H = n >> th; lc <
+// kChainFBar) = the XOR of u2[row H + h] over the derived rows of column lc
+// of part p: what every instance with i >> th = h reads for that column.
+__global__ __launch_bounds__(256) void k_chain_part_sums(const uint4* __restrict__ u2, uint64_t H, MapDev loc,
+                                                         MapDev prv, MapDev fst, MapDev lst, uint4* __restrict__ P) {
+    const uint64_t per = H * (uint64_t)kChainFBar, total = 4 * per;
+    for (uint64_t idx = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; idx < total;
+         idx += (uint64_t)gridDim.x * blockDim.x) {
+        const int p = (int)(idx / per);
+        const uint64_t rem = idx - p * per, h = rem / kChainFBar;
+        const uint32_t lc = (uint32_t)(rem - h * kChainFBar);
+        const MapDev m = p == 0 ? loc : p == 1 ? prv : p == 2 ? fst : lst;
         gf128 acc = gf_zero();
-        if (src < live) {
-            const uint64_t i = (src - 1) / kFInstance;
-            const uint32_t lc = 1 + (uint32_t)((src - 1) % kFInstance);
-            acc = gf_mul(gf_load(&u1[i & mask]), part_sum(loc, lc, i, u2, H, th));
-            if (i + 1 < n) {
-                const gf128 s = part_sum(prv, lc, i + 1, u2, H, th);
-                if (!gf_is_zero(s)) acc = gf_add(acc, gf_mul(gf_load(&u1[(i + 1) & mask]), s));
-            }
-            if (i == 0) acc = gf_add(acc, gf_mul(gf_load(&u1[0]), part_sum(fst, lc, 0, u2, H, th)));
-            if (i + 1 == n) acc = gf_add(acc, gf_mul(gf_load(&u1[i & mask]), part_sum(lst, lc, i, u2, H, th)));
-        }
-        gf_store(&wf[src], acc);
+        for (uint32_t k = m.off[lc]; k < m.off[lc + 1]; k++) acc = gf_add(acc, gf_load(&u2[m.rows[k] * H + h]));
+        gf_store(&P[idx], acc);
     }
+}
+
+// One block per instance i: its committed cells 1 + 6888 i + (lc - 1) take
+// u1[i mod 2^th] times the local part's sum (and, at i = 0 and i = n - 1,
+// the first and last parts') plus u1[(i + 1) mod 2^th] times the prev
+// part's sum at instance i + 1. Both multipliers are constant over the
+// block: window tables in LDS.
+__global__ __launch_bounds__(256) void k_chain_transpose(const uint4* __restrict__ u1, const uint4* __restrict__ P,
+                                                         uint64_t n, int th, uint4* __restrict__ wf) {
+    __shared__ uint4 t0[512], t1[512];
+    const uint64_t i = blockIdx.x, H = n >> th, mask = (1ull << th) - 1;
+    const bool has_next = i + 1 < n;
+    gf_table_build(t0, gf_load(&u1[i & mask]), threadIdx.x, blockDim.x);
+    if (has_next) gf_table_build(t1, gf_load(&u1[(i + 1) & mask]), threadIdx.x, blockDim.x);
+    __syncthreads();
+    const uint64_t h0 = i >> th, h1 = (i + 1) >> th;
+    const uint4* Ploc = P + h0 * kChainFBar;
+    const uint4* Pprv = P + (H + (has_next ? h1 : 0)) * kChainFBar;
+    const uint4* Pfst = P + 2 * H * kChainFBar;
+    const uint4* Plst = P + (3 * H + h0) * kChainFBar;
+    uint4* out = wf + 1 + (uint64_t)kFInstance * i - 1;
+    for (uint32_t lc = 1 + threadIdx.x; lc < (uint32_t)kChainFBar; lc += blockDim.x) {
+        gf128 acc = gf_mul_tab(t0, gf_load(&Ploc[lc]));
+        if (has_next) {
+            const gf128 sp = gf_load(&Pprv[lc]);
+            if (!gf_is_zero(sp)) acc = gf_add(acc, gf_mul_tab(t1, sp));
+        }
+        if (i == 0) acc = gf_add(acc, gf_mul_tab(t0, gf_load(&Pfst[lc])));
+        if (!has_next) acc = gf_add(acc, gf_mul_tab(t0, gf_load(&Plst[lc])));
+        gf_store(&out[lc], acc);
+    }
+}
+
+// wf[x] = 0 for x in [from, to): the cells past the live ones
+__global__ __launch_bounds__(256) void k_chain_zero(uint4* __restrict__ wf, uint64_t from, uint64_t to) {
+    for (uint64_t x = from + blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; x < to;
+         x += (uint64_t)gridDim.x * blockDim.x)
+        wf[x] = make_uint4(0, 0, 0, 0);
 }
 
 // ---- host: the batch's device data ----
@@ -348,11 +375,24 @@ int transpose_cb(void* user, const void* u1, const void* u2, void* wf) {
     const Relation& r = *b.rel;
     const uint64_t cells = 1ull << (b.tf + b.sf);
     const uint64_t H = b.n >> b.th;
-    BITZ_REQUIRE(cells < (1ull << 40), BITZ_ERR_SHAPE, "chain transpose: too many cells");
-    hipLaunchKernelGGL(k_chain_transpose, dim3((unsigned)std::min<uint64_t>((cells + 255) / 256, 65536)), dim3(256),
-                       0, ctx->stream, (const uint4*)u1, (const uint4*)u2, b.n, b.th, cells, T->cd->md[0],
-                       T->cd->md[1], T->cd->md[2], T->cd->md[3], (uint4*)wf);
+    const uint64_t live = 1 + b.n * (uint64_t)kFInstance;
+    BITZ_REQUIRE(cells < (1ull << 40) && live <= cells && b.n <= (1ull << 20), BITZ_ERR_SHAPE,
+                 "chain transpose: grid out of range");
+    DevBuf P;
+    if (!P.alloc(4 * H * (uint64_t)kChainFBar * 16)) return BITZ_ERR_ALLOC;
+    const uint64_t psums = 4 * H * (uint64_t)kChainFBar;
+    hipLaunchKernelGGL(k_chain_part_sums, dim3((unsigned)std::min<uint64_t>((psums + 255) / 256, 65536)), dim3(256), 0,
+                       ctx->stream, (const uint4*)u2, H, T->cd->md[0], T->cd->md[1], T->cd->md[2], T->cd->md[3],
+                       (uint4*)P.ptr);
     BITZ_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_chain_transpose, dim3((unsigned)b.n), dim3(256), 0, ctx->stream, (const uint4*)u1,
+                       (const uint4*)P.ptr, b.n, b.th, (uint4*)wf);
+    BITZ_LAUNCH_CHECK();
+    if (live < cells) {
+        hipLaunchKernelGGL(k_chain_zero, dim3((unsigned)std::min<uint64_t>((cells - live + 255) / 256, 65536)),
+                           dim3(256), 0, ctx->stream, (uint4*)wf, live, cells);
+        BITZ_LAUNCH_CHECK();
+    }
     // cell 0: every instance's local column-0 rows (sum_lo u1 times the
     // rows' u2 sums over the high instances), instance 0's first rows and
     // instance n - 1's last rows

@@ -10,6 +10,7 @@
 #include <vector>
 
 #include "modq.h"
+#include "sha256_host.h"
 
 namespace bitz {
 namespace chain {
@@ -48,8 +49,7 @@ struct Relation {
 // the relation for a chain from `initial_state` (built once per state, kept)
 std::shared_ptr<const Relation> relation(const std::array<uint32_t, 8>& initial_state);
 
-// the native compression (FIPS 180-4 6.2.2)
-void compress(const uint32_t state[8], const uint32_t block[16], uint32_t out[8]);
+// FIPS 180-4 5.3.3: H_0 (compress: sha256_host.h)
 extern const uint32_t kIV[8];
 
 // prepare_chain_instances at lambda 100: N = 2^k instances

@@ -1,7 +1,10 @@
 // Host-side protocol pieces (see host_proto.h).
 #include "host_proto.h"
 
+#include <cpuid.h>
+#include <immintrin.h>
 #include <math.h>
+#include <string.h>
 
 #include <algorithm>
 #include <thread>
@@ -31,8 +34,8 @@ static constexpr Sched make_sched() {
 }
 static constexpr Sched kSched = make_sched();
 
-void blake3_compress_host(const uint32_t cv[8], const uint8_t block[64], uint64_t counter, uint32_t block_len,
-                          uint32_t flags, uint32_t out[16]) {
+void blake3_compress_portable(const uint32_t cv[8], const uint8_t block[64], uint64_t counter, uint32_t block_len,
+                              uint32_t flags, uint32_t out[16]) {
     uint32_t m[16], s[16];
     for (int i = 0; i < 16; i++)
         m[i] = (uint32_t)block[4 * i] | ((uint32_t)block[4 * i + 1] << 8) | ((uint32_t)block[4 * i + 2] << 16) |
@@ -72,6 +75,71 @@ void blake3_compress_host(const uint32_t cv[8], const uint8_t block[64], uint64_
         out[i] = s[i] ^ s[i + 8];
         out[i + 8] = s[i + 8] ^ cv[i];
     }
+}
+
+// The same compression with the state's four rows in 128-bit registers: the
+// column step runs the four G functions lane-parallel, the rows are then
+// rotated by 1, 2, 3 lanes so the diagonals become columns, and back
+// (the 16- and 8-bit rotations are byte shuffles).
+__attribute__((target("sse4.1,ssse3"), always_inline)) static inline void b3_g4(__m128i& r0, __m128i& r1, __m128i& r2,
+                                                                              __m128i& r3, __m128i mx, __m128i my) {
+    const __m128i rot16 = _mm_setr_epi8(2, 3, 0, 1, 6, 7, 4, 5, 10, 11, 8, 9, 14, 15, 12, 13);
+    const __m128i rot8 = _mm_setr_epi8(1, 2, 3, 0, 5, 6, 7, 4, 9, 10, 11, 8, 13, 14, 15, 12);
+    r0 = _mm_add_epi32(_mm_add_epi32(r0, r1), mx);
+    r3 = _mm_shuffle_epi8(_mm_xor_si128(r3, r0), rot16);
+    r2 = _mm_add_epi32(r2, r3);
+    __m128i x = _mm_xor_si128(r1, r2);
+    r1 = _mm_or_si128(_mm_srli_epi32(x, 12), _mm_slli_epi32(x, 20));
+    r0 = _mm_add_epi32(_mm_add_epi32(r0, r1), my);
+    r3 = _mm_shuffle_epi8(_mm_xor_si128(r3, r0), rot8);
+    r2 = _mm_add_epi32(r2, r3);
+    x = _mm_xor_si128(r1, r2);
+    r1 = _mm_or_si128(_mm_srli_epi32(x, 7), _mm_slli_epi32(x, 25));
+}
+
+__attribute__((target("sse4.1,ssse3"))) void blake3_compress_sse41(const uint32_t cv[8], const uint8_t block[64],
+                                                                   uint64_t counter, uint32_t block_len,
+                                                                   uint32_t flags, uint32_t out[16]) {
+    uint32_t m[16];
+    memcpy(m, block, 64);  // little-endian words
+    const __m128i c0 = _mm_loadu_si128((const __m128i*)cv), c1 = _mm_loadu_si128((const __m128i*)(cv + 4));
+    __m128i r0 = c0, r1 = c1;
+    __m128i r2 = _mm_setr_epi32((int)kIV[0], (int)kIV[1], (int)kIV[2], (int)kIV[3]);
+    __m128i r3 = _mm_setr_epi32((int)(uint32_t)counter, (int)(uint32_t)(counter >> 32), (int)block_len, (int)flags);
+    for (int r = 0; r < 7; r++) {
+        const uint8_t* q = kSched.idx[r];
+        b3_g4(r0, r1, r2, r3, _mm_setr_epi32((int)m[q[0]], (int)m[q[2]], (int)m[q[4]], (int)m[q[6]]),
+              _mm_setr_epi32((int)m[q[1]], (int)m[q[3]], (int)m[q[5]], (int)m[q[7]]));
+        r1 = _mm_shuffle_epi32(r1, _MM_SHUFFLE(0, 3, 2, 1));
+        r2 = _mm_shuffle_epi32(r2, _MM_SHUFFLE(1, 0, 3, 2));
+        r3 = _mm_shuffle_epi32(r3, _MM_SHUFFLE(2, 1, 0, 3));
+        b3_g4(r0, r1, r2, r3, _mm_setr_epi32((int)m[q[8]], (int)m[q[10]], (int)m[q[12]], (int)m[q[14]]),
+              _mm_setr_epi32((int)m[q[9]], (int)m[q[11]], (int)m[q[13]], (int)m[q[15]]));
+        r1 = _mm_shuffle_epi32(r1, _MM_SHUFFLE(2, 1, 0, 3));
+        r2 = _mm_shuffle_epi32(r2, _MM_SHUFFLE(1, 0, 3, 2));
+        r3 = _mm_shuffle_epi32(r3, _MM_SHUFFLE(0, 3, 2, 1));
+    }
+    _mm_storeu_si128((__m128i*)out, _mm_xor_si128(r0, r2));
+    _mm_storeu_si128((__m128i*)(out + 4), _mm_xor_si128(r1, r3));
+    _mm_storeu_si128((__m128i*)(out + 8), _mm_xor_si128(r2, c0));
+    _mm_storeu_si128((__m128i*)(out + 12), _mm_xor_si128(r3, c1));
+}
+
+bool blake3_using_sse41() {
+    static const bool v = [] {
+        unsigned a, b, c, d;
+        // leaf 1 ECX: bit 9 SSSE3, bit 19 SSE4.1
+        return __get_cpuid(1, &a, &b, &c, &d) && ((c >> 9) & 1) && ((c >> 19) & 1);
+    }();
+    return v;
+}
+
+void blake3_compress_host(const uint32_t cv[8], const uint8_t block[64], uint64_t counter, uint32_t block_len,
+                          uint32_t flags, uint32_t out[16]) {
+    if (blake3_using_sse41())
+        blake3_compress_sse41(cv, block, counter, block_len, flags, out);
+    else
+        blake3_compress_portable(cv, block, counter, block_len, flags, out);
 }
 
 static void words_le(const uint32_t* w, int n, uint8_t* out) {

@@ -57,8 +57,15 @@ class Blake3 {
     int stack_len_ = 0;
 };
 
+// The compression function: the SSE4.1 row form where the CPU has it, the
+// portable one otherwise (identical outputs; both kept for the tests).
 void blake3_compress_host(const uint32_t cv[8], const uint8_t block[64], uint64_t counter, uint32_t block_len,
                           uint32_t flags, uint32_t out[16]);
+void blake3_compress_portable(const uint32_t cv[8], const uint8_t block[64], uint64_t counter, uint32_t block_len,
+                              uint32_t flags, uint32_t out[16]);
+void blake3_compress_sse41(const uint32_t cv[8], const uint8_t block[64], uint64_t counter, uint32_t block_len,
+                           uint32_t flags, uint32_t out[16]);
+bool blake3_using_sse41();
 // Chaining values: a whole non-final chunk, a non-root parent node.
 void blake3_chunk_cv(const uint8_t chunk[1024], uint64_t counter, uint32_t cv[8]);
 void blake3_parent_cv(const uint32_t left[8], const uint32_t right[8], uint32_t cv[8]);

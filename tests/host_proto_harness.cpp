@@ -42,3 +42,17 @@ int hp_runs(uint64_t c0, uint64_t m, int* ks, int cap) {
     return (int)v.size();
 }
 }
+
+extern "C" {
+// the BLAKE3 compression: 0 portable, 1 SSE4.1 rows, 2 the dispatched entry
+int hp_has_sse41(void) { return bitz::blake3_using_sse41() ? 1 : 0; }
+void hp_compress(int path, const uint32_t* cv, const uint8_t* block, uint64_t counter, uint32_t block_len,
+                 uint32_t flags, uint32_t* out) {
+    if (path == 1)
+        bitz::blake3_compress_sse41(cv, block, counter, block_len, flags, out);
+    else if (path == 2)
+        bitz::blake3_compress_host(cv, block, counter, block_len, flags, out);
+    else
+        bitz::blake3_compress_portable(cv, block, counter, block_len, flags, out);
+}
+}

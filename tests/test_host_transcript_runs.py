@@ -54,3 +54,27 @@ def test_runs_tile_aligned(hp, c0, m):
         c += 1 << k
     assert c == c0 + m
     assert n <= 2 * max(m, 1).bit_length()
+
+
+def test_compress_paths_agree(hp):
+    """the SSE4.1 row form of the BLAKE3 compression (used where the CPU has
+    it) equals the portable one on random inputs and every flag combination;
+    the portable form's digests are pinned against the oracle BLAKE3
+    (test_host_protocol.py)"""
+    hp.hp_has_sse41.restype = C.c_int
+    hp.hp_compress.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.c_uint64, C.c_uint32, C.c_uint32, C.c_void_p]
+    paths = [0, 1, 2] if hp.hp_has_sse41() else [0, 2]
+    rng = np.random.default_rng(3)
+    for i in range(300):
+        cv = rng.integers(0, 2**32, size=8, dtype=np.uint64).astype(np.uint32)
+        block = rng.integers(0, 256, size=64, dtype=np.uint8)
+        counter = int(rng.integers(0, 2**63))
+        block_len = int(rng.integers(0, 65))
+        flags = i % 16
+        outs = []
+        for p in paths:
+            out = np.zeros(16, np.uint32)
+            hp.hp_compress(p, cv.ctypes.data, block.ctypes.data, counter, block_len, flags, out.ctypes.data)
+            outs.append(out)
+        for o in outs[1:]:
+            assert (o == outs[0]).all()

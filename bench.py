@@ -140,6 +140,8 @@ def parse():
     ap.add_argument("--no-chain", action="store_true", help="skip the SHA-256 chain workload")
     ap.add_argument("--chain-only", action="store_true",
                     help="run only the SHA-256 chain workload and print its object (profiling aid)")
+    ap.add_argument("--mul-only", action="store_true",
+                    help="run only the u64 multiplication proof workload and print its object (profiling aid)")
     ap.add_argument("--batch-workers", type=int, default=8)
     ap.add_argument("--serve-jobs", type=int, default=96)
     ap.add_argument("--summarize-profiles", nargs=4, metavar=("STATS", "SQ", "FETCH", "WRITE"),
@@ -616,6 +618,9 @@ def main():
         # the SHA-256 chain workload alone (for its kernel profiles); prints
         # its own JSON object, not the bench line
         print(json.dumps({"sha256_chain": sha256_chain_steps(args, ctx)}), flush=True)
+        return
+    if args.mul_only:
+        print(json.dumps({"mul_u64": mul_u64_steps(args, ctx)}), flush=True)
         return
     if args.device_transcript:
         from bitz_amd import _lib

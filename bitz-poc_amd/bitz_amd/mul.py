@@ -123,14 +123,18 @@ def prove_u64(ctx, x, y, split_shift: int = 0, lo=None, hi=None):
     state = (C.c_uint8 * 32)()
     plen = C.c_uint64(0)
     lib = _lib.load()
-    rc = lib.bitz_prove_mul_u64(ctx.ptr, p(x), p(y), p(lo), p(hi), n, split_shift, root, state, None, 0,
-                                C.byref(plen))
-    if rc != _lib.BITZ_ERR_ARG or plen.value == 0:
+    # one proof into a buffer larger than any supported shape's proof; a
+    # second run only if the library reports a longer one
+    cap = 1 << 22
+    while True:
+        buf = (C.c_uint8 * cap)()
+        rc = lib.bitz_prove_mul_u64(ctx.ptr, p(x), p(y), p(lo), p(hi), n, split_shift, root, state, buf, cap,
+                                    C.byref(plen))
+        if rc == _lib.BITZ_ERR_ARG and plen.value > cap:
+            cap = plen.value
+            continue
         _lib.check(rc, "bitz_prove_mul_u64")
-    buf = (C.c_uint8 * plen.value)()
-    _lib.call("bitz_prove_mul_u64", ctx.ptr, p(x), p(y), p(lo), p(hi), n, split_shift, root, state, buf, plen.value,
-              C.byref(plen))
-    return bytes(root), bytes(state), bytes(buf)[:plen.value]
+        return bytes(root), bytes(state), bytes(buf)[:plen.value]
 
 
 def verify_u64(ctx, root: bytes, n: int, proof: bytes, split_shift: int = 0) -> bytes:

@@ -93,6 +93,9 @@ SIGNATURES = {
     "bitz_gkr_nibble_rows_dev": (_I, [_P, _P, _I, _I, _P]),
     "bitz_gkr_nibble_round_sums_dev": (_I, [_P, _I, _P, _P, _I, _I, _P, _P, _P, _I, _P]),
     "bitz_gkr_nibble_fold_dev": (_I, [_P, _I, _P, _P, _I, _I, _P, _P, _P, _P]),
+    "bitz_gkr_jit_round_sums_dev": (_I, [_P, _I, _P, _P, _I, _I, _P, _P, _P, _I, _P]),
+    "bitz_gkr_jit_fold_dev": (_I, [_P, _I, _P, _P, _I, _I, _P, _P, _P, _P]),
+    "bitz_gkr_level4_dev": (_I, [_P, _P, _P, _I, _I, _P]),
     "bitz_fold_1b_rows_noalloc": (_I, [_P, _P, _P, _U64, _P]),
     "bitz_fold_b128_dev": (_I, [_P, _P, _U64, _P, _P]),
     "bitz_axpy_dev": (_I, [_P, _P, _P, _U64, _P]),

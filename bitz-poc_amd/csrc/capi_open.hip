@@ -224,6 +224,41 @@ extern "C" int bitz_gkr_nibble_fold_dev(bitz_ctx* ctx, int level, const void* nr
     return bitz::gkr_nibble_fold(ctx, level, nr, A, t, s, ch, nullptr, eq_hi, l_out, r_out);
 }
 
+extern "C" int bitz_gkr_jit_round_sums_dev(bitz_ctx* ctx, int level, const void* nr, const void* A, int t, int s,
+                                           const uint8_t* chal, const void* eq_c, const void* eq_y, int send_one,
+                                           uint8_t out_host[32]) {
+    PRELUDE();
+    BITZ_REQUIRE(level >= 0 && level <= 3, BITZ_ERR_ARG, "bitz_gkr_jit_round_sums_dev: level must be 0..3");
+    BITZ_REQUIRE(nr && A && eq_c && eq_y && out_host && (level == 3 || chal), BITZ_ERR_ARG,
+                 "bitz_gkr_jit_round_sums_dev: null argument");
+    BITZ_REQUIRE(t >= 5 && s >= 0 && t + s <= 40, BITZ_ERR_SHAPE, "bitz_gkr_jit_round_sums_dev: bad shape");
+    bitz::hgf ch[4] = {};
+    if (level < 3) chal_from_bytes(chal, 3 - level, ch);
+    CALL(bitz::gkr_jit_tables(ctx, level, A, t, ch, nullptr));
+    CALL(bitz::gkr_jit_round_sums(ctx, level, nr, t, s, eq_c, eq_y, send_one, nullptr));
+    return bitz::read_result(ctx, 2, out_host);
+}
+
+extern "C" int bitz_gkr_jit_fold_dev(bitz_ctx* ctx, int level, const void* nr, const void* A, int t, int s,
+                                     const uint8_t* chal, const void* eq_hi, void* l_out, void* r_out) {
+    PRELUDE();
+    BITZ_REQUIRE(level >= 0 && level <= 3, BITZ_ERR_ARG, "bitz_gkr_jit_fold_dev: level must be 0..3");
+    BITZ_REQUIRE(nr && A && chal && l_out && r_out, BITZ_ERR_ARG, "bitz_gkr_jit_fold_dev: null argument");
+    BITZ_REQUIRE(t >= 5 && s >= 0 && t + s <= 40 && (!eq_hi || s > 8), BITZ_ERR_SHAPE,
+                 "bitz_gkr_jit_fold_dev: bad shape (eq_hi needs s > 8)");
+    bitz::hgf ch[4] = {};
+    chal_from_bytes(chal, 4 - level, ch);
+    CALL(bitz::gkr_jit_tables(ctx, level, A, t, ch, nullptr));
+    return bitz::gkr_jit_fold(ctx, level, nr, t, s, ch, nullptr, eq_hi, l_out, r_out);
+}
+
+extern "C" int bitz_gkr_level4_dev(bitz_ctx* ctx, const void* nr, const void* A, int t, int s, void* out) {
+    PRELUDE();
+    BITZ_REQUIRE(nr && A && out, BITZ_ERR_ARG, "bitz_gkr_level4_dev: null argument");
+    BITZ_REQUIRE(t >= 5 && s >= 0 && t + s <= 40, BITZ_ERR_SHAPE, "bitz_gkr_level4_dev: bad shape");
+    return bitz::gkr_level4(ctx, nr, A, t, s, out);
+}
+
 extern "C" int bitz_gkr_level1_dev(bitz_ctx* ctx, const void* packed, const void* A, int t, int s, void* out) {
     PRELUDE();
     BITZ_REQUIRE(packed && A && out, BITZ_ERR_ARG, "bitz_gkr_level1_dev: null argument");

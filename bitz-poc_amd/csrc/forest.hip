@@ -65,7 +65,7 @@ struct Nib {
 // Challenges of the layer's rounds: host values as scalars, or (with the
 // transcript on the device) a device array.
 struct Chal {
-    uint64_t lo[3], hi[3];
+    uint64_t lo[4], hi[4];
     const uint4* dev;
 };
 __device__ __forceinline__ gf128 chal(const Chal& c, int i) {
@@ -262,6 +262,189 @@ __global__ __launch_bounds__(kT) void k_nibble_fold_hi(const uint16_t* __restric
     }
 }
 
+// ---- the table round and fold (Forest::jit_round_sums, jit_fold_round_with,
+// src/bitz/forest.rs:990-1176) ----
+// After its bit and row-word rounds (level 3: none), a low layer's halves at
+// (y, c), y < Y4, are F_p[y][sel_p(w)], w the slot's row word, F the per-row
+// 256-entry tables F[(y*2 + p)*256 + i]: for levels 0-2 the fold tables of
+// the row-word round (k_nibble_ftab, indices ie / io), for level 3 the
+// products of the eight leaves of the word's low (E) or high (O) byte. The
+// next round binds y's top bit, so E_h, O_h (h = 0, 1) at (y', c),
+// y' < Y5 = 2^(t-5), read the word of row y' + h Y5. Its sums bucket
+// eq_c[c] O_b(c) by E's index per row (one LDS XOR per bucket, then 768
+// products per row); its fold is two lookups per half in the tables
+// pre-scaled by 1 + r (h = 0) and r (h = 1), into halves of 2^(t+s-5)
+// entries where the layer's dense rounds start.
+template <int L>
+struct Sel {
+    __device__ __forceinline__ static uint32_t e(uint32_t w) {
+        if constexpr (L == 3) {
+            return w & 255u;
+        } else {
+            uint32_t ie, io;
+            nibble_fold_idx<L>(w, &ie, &io);
+            return ie;
+        }
+    }
+    __device__ __forceinline__ static uint32_t o(uint32_t w) {
+        if constexpr (L == 3) {
+            return w >> 8;
+        } else {
+            uint32_t ie, io;
+            nibble_fold_idx<L>(w, &ie, &io);
+            return io;
+        }
+    }
+};
+
+// Level 3's tables: F[(y*2 + p)*256 + i] = prod over the set bits u of i of
+// A[y + Y4 (p + 2u)] (the rows of word bit w = p + 2u), as the product of
+// the tables of its low and high nibble.
+__global__ __launch_bounds__(kT) void k_level3_ftab(const uint4* __restrict__ A, int t, uint4* __restrict__ F) {
+    __shared__ uint4 part[2][16];
+    const uint64_t y4 = 1ull << (t - 4), y = blockIdx.x >> 1;
+    const uint32_t p = blockIdx.x & 1u, i = threadIdx.x;
+    if (i < 32) {
+        const uint32_t half = i >> 4, j = i & 15u;
+        gf128 acc = gf_one();
+        bool first = true;
+#pragma unroll
+        for (uint32_t u4 = 0; u4 < 4; u4++) {
+            if ((j >> u4) & 1u) {
+                const uint32_t u = u4 + 4 * half;
+                const gf128 a = gf_load(&A[y + y4 * (p + 2 * u)]);
+                acc = first ? a : gf_mul(acc, a);
+                first = false;
+            }
+        }
+        gf_store(&part[half][j], acc);
+    }
+    __syncthreads();
+    gf_store(&F[(uint64_t)blockIdx.x * 256 + i], gf_mul(gf_load(&part[0][i & 15u]), gf_load(&part[1][i >> 4])));
+}
+
+// Level 4 from level 3's tables: out[(y << s) | c] = F_0[y][lo(w)] F_1[y][hi(w)].
+__global__ __launch_bounds__(kT) void k_level4_from_tab(const uint16_t* __restrict__ nr, const uint4* __restrict__ F,
+                                                        int s, uint4* __restrict__ out) {
+    __shared__ uint4 tab[512];
+    const uint64_t y = blockIdx.x, cols = 1ull << s;
+    for (uint32_t i = threadIdx.x; i < 512; i += kT) tab[i] = F[y * 512 + i];
+    __syncthreads();
+    for (uint64_t c = threadIdx.x; c < cols; c += kT) {
+        const uint32_t w = nr[(y << s) | c];
+        gf_store(&out[(y << s) | c], gf_mul(gf_load(&tab[w & 255u]), gf_load(&tab[256 + (w >> 8)])));
+    }
+}
+
+// The table round's sums, block y' < Y5: buckets over E's index of
+//   END[i]  = sum of eq_c[c] O_ss(c) over the columns with E_ss index i,
+//   INFh[i] = sum of eq_c[c] (O_0(c) + O_1(c)) over those with E_h index i,
+// then end = sum_i E_ss[i] END[i], inf = sum_h sum_i E_h[i] INFh[i]
+// (= sum eq (E_0 + E_1)(O_0 + O_1)), the E tables pre-scaled by eq_y[y'].
+template <int L>
+__global__ __launch_bounds__(kT) void k_jit_round_sums(const uint16_t* __restrict__ nr, const uint4* __restrict__ F,
+                                                       int t, int s, const uint4* __restrict__ eq_c,
+                                                       const uint4* __restrict__ eq_y, int send_one,
+                                                       uint4* __restrict__ partials) {
+    __shared__ uint4 TE[2][256], TO[2][256];
+    __shared__ uint32_t B[4][3][256];  // [word][END, INF0, INF1][index]
+    const uint64_t y = blockIdx.x, y5 = 1ull << (t - 5), cols = 1ull << s;
+    for (uint32_t i = threadIdx.x; i < 4 * 3 * 256; i += kT) (&B[0][0][0])[i] = 0u;
+    {
+        const gf128 wy = gf_load(&eq_y[y]);
+        const uint32_t i = threadIdx.x;
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+            const uint64_t yh = y + (uint64_t)h * y5;
+            gf_store(&TE[h][i], gf_mul(gf_load(&F[(yh * 2 + 0) * 256 + i]), wy));
+            TO[h][i] = F[(yh * 2 + 1) * 256 + i];
+        }
+    }
+    __syncthreads();
+    const uint16_t* w0p = nr + (y << s);
+    const uint16_t* w1p = nr + ((y + y5) << s);
+    for (uint64_t c = threadIdx.x; c < cols; c += kT) {
+        const uint32_t w0 = w0p[c], w1 = w1p[c];
+        const gf128 e = gf_load(&eq_c[c]);
+        const gf128 u0 = gf_mul(e, gf_load(&TO[0][Sel<L>::o(w0)]));
+        const gf128 u1 = gf_mul(e, gf_load(&TO[1][Sel<L>::o(w1)]));
+        const uint32_t a0 = Sel<L>::e(w0), a1 = Sel<L>::e(w1);
+        const uint32_t as = send_one ? a1 : a0;
+        const gf128 us = send_one ? u1 : u0, uu = gf_add(u0, u1);
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            atomicXor(&B[k][0][as], us.w[k]);
+            atomicXor(&B[k][1][a0], uu.w[k]);
+            atomicXor(&B[k][2][a1], uu.w[k]);
+        }
+    }
+    __syncthreads();
+    const uint32_t i = threadIdx.x;
+    const gf128 bend = gf128{{B[0][0][i], B[1][0][i], B[2][0][i], B[3][0][i]}};
+    const gf128 b0 = gf128{{B[0][1][i], B[1][1][i], B[2][1][i], B[3][1][i]}};
+    const gf128 b1 = gf128{{B[0][2][i], B[1][2][i], B[2][2][i], B[3][2][i]}};
+    gf256 inf = gf_mul_wide(gf_load(&TE[0][i]), b0);
+    gf256_acc(&inf, gf_mul_wide(gf_load(&TE[1][i]), b1));
+    gf128 sums[2] = {gf_mul(gf_load(&TE[send_one ? 1 : 0][i]), bend), gf_reduce(inf)};
+    block_xor_store<2>(sums, partials + (size_t)blockIdx.x * 2);
+}
+
+// F rows y < Y5 times (1 + r), rows Y5 <= y < Y4 times r (r = challenge idx)
+__global__ __launch_bounds__(kT) void k_jit_prescale(uint4* __restrict__ F, int t, Chal ch, int idx) {
+    __shared__ uint4 tab0[512], tab1[512];
+    const gf128 r = chal(ch, idx);
+    gf_table_build(tab0, gf_add(r, gf_one()), threadIdx.x, blockDim.x);
+    gf_table_build(tab1, r, threadIdx.x, blockDim.x);
+    __syncthreads();
+    const uint64_t y5 = 1ull << (t - 5), n = (1ull << (t - 4)) * 512;
+    for (uint64_t e = grid_tid(); e < n; e += grid_threads())
+        gf_store(&F[e], gf_mul_tab((e >> 9) < y5 ? tab0 : tab1, gf_load(&F[e])));
+}
+
+// The folded halves from the pre-scaled tables, x = (y' << s) | c:
+//   l[x] = F_0[y'][e(w0)] + F_0[y' + Y5][e(w1)], r[x] likewise with F_1, o().
+template <int L>
+__device__ __forceinline__ void jit_fold_pair(const uint16_t* __restrict__ nr, const uint4* __restrict__ F, int s,
+                                              uint64_t y5, uint64_t y, uint64_t c, gf128* lv, gf128* rv) {
+    const uint32_t w0 = nr[(y << s) | c], w1 = nr[((y + y5) << s) | c];
+    const uint64_t f0 = y * 512, f1 = (y + y5) * 512;
+    *lv = gf_add(gf_load(&F[f0 + Sel<L>::e(w0)]), gf_load(&F[f1 + Sel<L>::e(w1)]));
+    *rv = gf_add(gf_load(&F[f0 + 256 + Sel<L>::o(w0)]), gf_load(&F[f1 + 256 + Sel<L>::o(w1)]));
+}
+
+template <int L>
+__global__ __launch_bounds__(kT) void k_jit_fold(const uint16_t* __restrict__ nr, const uint4* __restrict__ F, int t,
+                                                 int s, uint4* __restrict__ l_out, uint4* __restrict__ r_out) {
+    const uint64_t y5 = 1ull << (t - 5), n = y5 << s;
+    for (uint64_t x = grid_tid(); x < n; x += grid_threads()) {
+        gf128 lv, rv;
+        jit_fold_pair<L>(nr, F, s, y5, x >> s, x & ((1ull << s) - 1), &lv, &rv);
+        gf_store_nt(&l_out[x], lv);
+        gf_store_nt(&r_out[x], rv);
+    }
+}
+
+// The same with the right half times eq_hi[c >> 8] (s > 8), blocks mapped
+// to rows as in k_nibble_fold_hi (XCD-aware: block b takes rows y' = b mod 8).
+template <int L>
+__global__ __launch_bounds__(kT) void k_jit_fold_hi(const uint16_t* __restrict__ nr, const uint4* __restrict__ F,
+                                                    int t, int s, const uint4* __restrict__ eq_hi, uint32_t lanes8,
+                                                    uint4* __restrict__ l_out, uint4* __restrict__ r_out) {
+    __shared__ uint4 tab[512];
+    const uint32_t xcd = blockIdx.x & 7u, li = blockIdx.x >> 3;
+    const uint32_t g = li & ((1u << (s - 8)) - 1), l8 = li >> (s - 8);
+    gf_table_build(tab, gf_load(&eq_hi[g]), threadIdx.x, blockDim.x);
+    __syncthreads();
+    const uint64_t y5 = 1ull << (t - 5), c = ((uint64_t)g << 8) + threadIdx.x;
+    for (uint64_t y = xcd + 8ull * l8; y < y5; y += 8ull * lanes8) {
+        gf128 lv, rv;
+        jit_fold_pair<L>(nr, F, s, y5, y, c, &lv, &rv);
+        const uint64_t x = (y << s) + c;
+        gf_store_nt(&l_out[x], lv);
+        gf_store_nt(&r_out[x], gf_mul_tab(tab, rv));
+    }
+}
+
 Chal chal_args(const hgf* host, int n, const void* dev) {
     Chal c{};
     for (int i = 0; i < n && host; i++) {
@@ -283,6 +466,33 @@ int nibble_ttab(bitz_ctx* ctx, int level, const void* A, int t, const Chal& ch, 
         hipLaunchKernelGGL(k_nibble_ttab<1>, grid, block, 0, ctx->stream, (const uint4*)A, t, ch, *T);
     else
         hipLaunchKernelGGL(k_nibble_ttab<2>, grid, block, 0, ctx->stream, (const uint4*)A, t, ch, *T);
+    BITZ_LAUNCH_CHECK();
+    return BITZ_OK;
+}
+
+// F[(y*2 + p)*256 + i] (ctx->scratch.nib_tab): the row-word round's fold tables
+// of level 0-2 by challenge 2 - level (k_nibble_ftab), or level 3's leaf
+// product tables (k_level3_ftab).
+int nibble_ftab(bitz_ctx* ctx, int level, const void* A, int t, const Chal& ch, uint4** F) {
+    const uint64_t y4 = 1ull << (t - 4);
+    if (!ctx->scratch.nib_tab.alloc(y4 * 512 * 16)) return BITZ_ERR_ALLOC;
+    *F = (uint4*)ctx->scratch.nib_tab.ptr;
+    if (level == 3) {
+        BITZ_REQUIRE(2 * y4 <= 0x7fffffffull, BITZ_ERR_SHAPE, "level-3 tables: too many rows");
+        hipLaunchKernelGGL(k_level3_ftab, dim3((unsigned)(2 * y4)), dim3(kT), 0, ctx->stream, (const uint4*)A, t, *F);
+        BITZ_LAUNCH_CHECK();
+        return BITZ_OK;
+    }
+    uint4* T;
+    int rc = nibble_ttab(ctx, level, A, t, ch, &T);
+    if (rc) return rc;
+    const dim3 grid(blocks_for(y4 * 512)), block(kT);
+    if (level == 0)
+        hipLaunchKernelGGL(k_nibble_ftab<0>, grid, block, 0, ctx->stream, T, t, ch, *F);
+    else if (level == 1)
+        hipLaunchKernelGGL(k_nibble_ftab<1>, grid, block, 0, ctx->stream, T, t, ch, *F);
+    else
+        hipLaunchKernelGGL(k_nibble_ftab<2>, grid, block, 0, ctx->stream, T, t, ch, *F);
     BITZ_LAUNCH_CHECK();
     return BITZ_OK;
 }
@@ -332,22 +542,10 @@ int gkr_nibble_fold(bitz_ctx* ctx, int level, const void* nr, const void* A, int
     BITZ_REQUIRE(level >= 0 && level <= 2 && t >= 4, BITZ_ERR_SHAPE, "gkr_nibble_fold: bad level or shape");
     BITZ_REQUIRE(!eq_hi || s > 8, BITZ_ERR_SHAPE, "gkr_nibble_fold: eq_hi needs s > 8");
     const Chal ch = chal_args(chal_host, 3 - level, chal_dev);
-    uint4* T;
-    int rc = nibble_ttab(ctx, level, A, t, ch, &T);
+    uint4* F;
+    int rc = nibble_ftab(ctx, level, A, t, ch, &F);
     if (rc) return rc;
     const uint64_t y4 = 1ull << (t - 4), n = y4 << s;
-    if (!ctx->scratch.nib_tab.alloc(y4 * 512 * 16)) return BITZ_ERR_ALLOC;
-    uint4* F = (uint4*)ctx->scratch.nib_tab.ptr;
-    {
-        const dim3 grid(blocks_for(y4 * 512)), block(kT);
-        if (level == 0)
-            hipLaunchKernelGGL(k_nibble_ftab<0>, grid, block, 0, ctx->stream, T, t, ch, F);
-        else if (level == 1)
-            hipLaunchKernelGGL(k_nibble_ftab<1>, grid, block, 0, ctx->stream, T, t, ch, F);
-        else
-            hipLaunchKernelGGL(k_nibble_ftab<2>, grid, block, 0, ctx->stream, T, t, ch, F);
-        BITZ_LAUNCH_CHECK();
-    }
     TimedLaunch tl(ctx, "nibble_fold", 2.0 * (double)n + 32.0 * (double)n, 0.0);
     const uint16_t* w = (const uint16_t*)nr;
     if (eq_hi) {
@@ -373,6 +571,107 @@ int gkr_nibble_fold(bitz_ctx* ctx, int level, const void* nr, const void* A, int
         else
             hipLaunchKernelGGL(k_nibble_fold<2>, grid, block, 0, ctx->stream, w, F, t, s, (uint4*)l_out, (uint4*)r_out);
     }
+    BITZ_LAUNCH_CHECK();
+    return BITZ_OK;
+}
+
+int gkr_jit_tables(bitz_ctx* ctx, int level, const void* A, int t, const hgf* chal_host, const void* chal_dev) {
+    BITZ_REQUIRE(level >= 0 && level <= 3 && t >= 5, BITZ_ERR_SHAPE, "gkr_jit_tables: bad level or shape");
+    const Chal ch = chal_args(chal_host, 3 - level, chal_dev);
+    uint4* F;
+    return nibble_ftab(ctx, level, A, t, ch, &F);
+}
+
+int gkr_jit_round_sums(bitz_ctx* ctx, int level, const void* nr, int t, int s, const void* eq_c, const void* eq_y,
+                       int send_one, void* out_dev) {
+    BITZ_REQUIRE(level >= 0 && level <= 3 && t >= 5, BITZ_ERR_SHAPE, "gkr_jit_round_sums: bad level or shape");
+    const uint64_t y4 = 1ull << (t - 4), y5 = y4 >> 1, cols = 1ull << s;
+    BITZ_REQUIRE(ctx->scratch.nib_tab.ptr && ctx->scratch.nib_tab.bytes >= y4 * 512 * 16, BITZ_ERR_ARG,
+                 "gkr_jit_round_sums: tables not built");
+    BITZ_REQUIRE(y5 <= 0x7fffffffull, BITZ_ERR_SHAPE, "gkr_jit_round_sums: too many rows");
+    const unsigned nb = (unsigned)y5;  // one block per row
+    int rc = ctx->scratch.ensure((size_t)nb * 2 * 16);
+    if (rc) return rc;
+    {
+        TimedLaunch tl(ctx, "jit_round_sums", (double)(y5 * cols) * (4.0 + 16.0) + 16.0 * 1024.0 * (double)y5,
+                       (double)(y5 * cols) * 2.0 * kGfMulOps);
+        const dim3 grid(nb), block(kT);
+        const uint16_t* w = (const uint16_t*)nr;
+        const uint4* F = (const uint4*)ctx->scratch.nib_tab.ptr;
+        uint4* parts = (uint4*)ctx->scratch.partials.ptr;
+        const uint4* ec = (const uint4*)eq_c;
+        const uint4* ey = (const uint4*)eq_y;
+        if (level == 0)
+            hipLaunchKernelGGL(k_jit_round_sums<0>, grid, block, 0, ctx->stream, w, F, t, s, ec, ey, send_one, parts);
+        else if (level == 1)
+            hipLaunchKernelGGL(k_jit_round_sums<1>, grid, block, 0, ctx->stream, w, F, t, s, ec, ey, send_one, parts);
+        else if (level == 2)
+            hipLaunchKernelGGL(k_jit_round_sums<2>, grid, block, 0, ctx->stream, w, F, t, s, ec, ey, send_one, parts);
+        else
+            hipLaunchKernelGGL(k_jit_round_sums<3>, grid, block, 0, ctx->stream, w, F, t, s, ec, ey, send_one, parts);
+        BITZ_LAUNCH_CHECK();
+    }
+    return sums_finish(ctx, nb, 2, out_dev);
+}
+
+int gkr_jit_fold(bitz_ctx* ctx, int level, const void* nr, int t, int s, const hgf* chal_host, const void* chal_dev,
+                 const void* eq_hi, void* l_out, void* r_out) {
+    BITZ_REQUIRE(level >= 0 && level <= 3 && t >= 5, BITZ_ERR_SHAPE, "gkr_jit_fold: bad level or shape");
+    BITZ_REQUIRE(!eq_hi || s > 8, BITZ_ERR_SHAPE, "gkr_jit_fold: eq_hi needs s > 8");
+    const uint64_t y4 = 1ull << (t - 4), y5 = y4 >> 1, n = y5 << s;
+    BITZ_REQUIRE(ctx->scratch.nib_tab.ptr && ctx->scratch.nib_tab.bytes >= y4 * 512 * 16, BITZ_ERR_ARG,
+                 "gkr_jit_fold: tables not built");
+    const Chal ch = chal_args(chal_host, 4 - level, chal_dev);
+    uint4* F = (uint4*)ctx->scratch.nib_tab.ptr;
+    hipLaunchKernelGGL(k_jit_prescale, dim3(blocks_for(y4 * 512, 4, 1024)), dim3(kT), 0, ctx->stream, F, t, ch,
+                       3 - level);
+    BITZ_LAUNCH_CHECK();
+    TimedLaunch tl(ctx, "jit_fold", 4.0 * (double)n + 32.0 * (double)n, eq_hi ? (double)n * kGfTabMulOps : 0.0);
+    const uint16_t* w = (const uint16_t*)nr;
+    if (eq_hi) {
+        const uint32_t runs = 1u << (s - 8);
+        const uint32_t lanes8 = (uint32_t)std::max<uint64_t>(
+            1, std::min<uint64_t>(std::max<uint64_t>(y5 / 8, 1), std::max(1u, 1024u / runs)));
+        const dim3 grid(8u * runs * lanes8), block(kT);
+        const uint4* eh = (const uint4*)eq_hi;
+        if (level == 0)
+            hipLaunchKernelGGL(k_jit_fold_hi<0>, grid, block, 0, ctx->stream, w, F, t, s, eh, lanes8, (uint4*)l_out,
+                               (uint4*)r_out);
+        else if (level == 1)
+            hipLaunchKernelGGL(k_jit_fold_hi<1>, grid, block, 0, ctx->stream, w, F, t, s, eh, lanes8, (uint4*)l_out,
+                               (uint4*)r_out);
+        else if (level == 2)
+            hipLaunchKernelGGL(k_jit_fold_hi<2>, grid, block, 0, ctx->stream, w, F, t, s, eh, lanes8, (uint4*)l_out,
+                               (uint4*)r_out);
+        else
+            hipLaunchKernelGGL(k_jit_fold_hi<3>, grid, block, 0, ctx->stream, w, F, t, s, eh, lanes8, (uint4*)l_out,
+                               (uint4*)r_out);
+    } else {
+        const dim3 grid(blocks_for(n, 1, 8192)), block(kT);
+        if (level == 0)
+            hipLaunchKernelGGL(k_jit_fold<0>, grid, block, 0, ctx->stream, w, F, t, s, (uint4*)l_out, (uint4*)r_out);
+        else if (level == 1)
+            hipLaunchKernelGGL(k_jit_fold<1>, grid, block, 0, ctx->stream, w, F, t, s, (uint4*)l_out, (uint4*)r_out);
+        else if (level == 2)
+            hipLaunchKernelGGL(k_jit_fold<2>, grid, block, 0, ctx->stream, w, F, t, s, (uint4*)l_out, (uint4*)r_out);
+        else
+            hipLaunchKernelGGL(k_jit_fold<3>, grid, block, 0, ctx->stream, w, F, t, s, (uint4*)l_out, (uint4*)r_out);
+    }
+    BITZ_LAUNCH_CHECK();
+    return BITZ_OK;
+}
+
+int gkr_level4(bitz_ctx* ctx, const void* nr, const void* A, int t, int s, void* out) {
+    BITZ_REQUIRE(t >= 5, BITZ_ERR_SHAPE, "gkr_level4: t < 5");
+    const uint64_t y4 = 1ull << (t - 4), n = y4 << s;
+    BITZ_REQUIRE(y4 <= 0x7fffffffull, BITZ_ERR_SHAPE, "gkr_level4: too many rows");
+    const Chal ch{};
+    uint4* F;
+    int rc = nibble_ftab(ctx, 3, A, t, ch, &F);
+    if (rc) return rc;
+    TimedLaunch tl(ctx, "level4", 2.0 * (double)n + 16.0 * (double)n, (double)n * kGfMulOps);
+    hipLaunchKernelGGL(k_level4_from_tab, dim3((unsigned)y4), dim3(kT), 0, ctx->stream, (const uint16_t*)nr, F, s,
+                       (uint4*)out);
     BITZ_LAUNCH_CHECK();
     return BITZ_OK;
 }

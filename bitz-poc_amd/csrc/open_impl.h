@@ -99,6 +99,22 @@ int gkr_nibble_round_sums(bitz_ctx* ctx, int level, const void* nr, const void* 
                           const void* chal_dev, const void* eq_c, const void* eq_y, int send_one, void* out_dev);
 int gkr_nibble_fold(bitz_ctx* ctx, int level, const void* nr, const void* A, int t, int s, const hgf* chal_host,
                     const void* chal_dev, const void* eq_hi, void* l_out, void* r_out);
+// The table round after a low layer's row-word round (forest.hip;
+// Forest::jit_round_sums / jit_fold_round_with, src/bitz/forest.rs:990-1176):
+// gkr_jit_tables builds the per-row tables F of level 0-2 from the layer's
+// challenges chal[0..3-level) (the row-word round's fold tables), or level
+// 3's leaf product tables (no challenges), in ctx->scratch.nib_tab;
+// gkr_jit_round_sums takes round 3 - level's sums (weights eq_y[y'],
+// y' < 2^(t-5), times eq_c[c]); gkr_jit_fold pre-scales the tables by
+// chal[3 - level] and folds into halves of 2^(t+s-5) entries (right half
+// times eq_hi[c >> 8] when given, s > 8). gkr_level4 writes level 4
+// (2^(t+s-4) entries) from level 3's tables.
+int gkr_jit_tables(bitz_ctx* ctx, int level, const void* A, int t, const hgf* chal_host, const void* chal_dev);
+int gkr_jit_round_sums(bitz_ctx* ctx, int level, const void* nr, int t, int s, const void* eq_c, const void* eq_y,
+                       int send_one, void* out_dev);
+int gkr_jit_fold(bitz_ctx* ctx, int level, const void* nr, int t, int s, const hgf* chal_host, const void* chal_dev,
+                 const void* eq_hi, void* l_out, void* r_out);
+int gkr_level4(bitz_ctx* ctx, const void* nr, const void* A, int t, int s, void* out);
 // Reduces nblocks x K block partials (ctx->scratch.partials) to out_dev, or
 // to the mailbox for the host when out_dev is null (deferred when the
 // context defers sums to the device transcript).

@@ -358,14 +358,19 @@ struct Prover {
 
     // ---- batched grand-product GKR (Forest::prove == gpgkr_prove) ----
     // Layer kinds: halves materialised, or an implicit low level whose first
-    // rounds run off the bits and whose fold writes halves of 2^(t+s-4)
-    // entries: the leaves (rounds 0-1 from bit patterns, round 2 from row
-    // words), level 1 (round 0 from bit patterns, round 1 from row words),
-    // level 2 (round 0 from row words).
-    enum LayerKind { kDense = 0, kLevel1 = 1, kLeaf = 2, kLevel2 = 3 };
-    // the level a low layer's row-word round serves, and that round's index
-    static int nibble_level(LayerKind kind) { return kind == kLeaf ? 0 : kind == kLevel1 ? 1 : 2; }
+    // rounds run off the bits and the row words, and whose table round's fold
+    // writes halves of 2^(t+s-5) entries: the leaves (rounds 0-1 from bit
+    // patterns, round 2 from row words, round 3 from the row-word fold
+    // tables), level 1 (round 0 from bit patterns, round 1 from row words,
+    // round 2 from tables), level 2 (round 0 from row words, round 1 from
+    // tables), level 3 (round 0 from its leaf product tables).
+    enum LayerKind { kDense = 0, kLevel1 = 1, kLeaf = 2, kLevel2 = 3, kLevel3 = 4 };
+    // the level a low layer serves, its row-word round (none for level 3) and its table round
+    static int nibble_level(LayerKind kind) {
+        return kind == kLeaf ? 0 : kind == kLevel1 ? 1 : kind == kLevel2 ? 2 : 3;
+    }
     static int nibble_round(LayerKind kind) { return 2 - nibble_level(kind); }
+    static int table_round(LayerKind kind) { return 3 - nibble_level(kind); }
     int gkr_layer(InnerTranscript& it, uint4* l, uint4* r, LayerKind kind, std::vector<G>& internal, G* claim) {
         const bool leaf = kind == kLeaf;
         const int L = (int)internal.size();
@@ -453,14 +458,15 @@ struct Prover {
         bool tail_done = false;
         BITZ_REQUIRE(kind == kDense || (L >= 4 && t >= 7 && (!weigh || wsplit)), BITZ_ERR_SHAPE,
                      "GKR: layer too small for bit patterns");
-        std::vector<hgf> chal;  // the layer's challenges so far (for its row-word round)
+        std::vector<hgf> chal;  // the layer's challenges so far (for its row-word and table rounds)
         const int nround = kind == kDense ? -1 : nibble_round(kind);
+        const int jround = kind == kDense ? -1 : table_round(kind);
         for (int round = 0; round < L; round++) {
             G z = internal[round];
             int rem = L - 1 - round;
             uint64_t hh = 1ull << rem;
             int s_c = std::min(s, rem);
-            const bool bit_round = kind != kDense && round <= nround;
+            const bool bit_round = kind != kDense && round <= jround;
             if (!bit_round && rem <= ctx->tune.host_tail_log && (!weighted || rem < s)) {
                 // the remaining rounds on the host, from dense halves
                 if (pending_fold && (rc = fold_pair(ctx, l, r, 2 * hh, pending_rho))) return rc;
@@ -477,10 +483,15 @@ struct Prover {
                 if ((rc = eq_dev(yv, eqy))) return rc;
             }
             int send_one = geq(z, gzero()) ? 1 : 0;
-            if (round == nround)
+            if (round == nround) {
                 rc = gkr_nibble_round_sums(ctx, nibble_level(kind), nibbles.ptr, rowimg.ptr, t, s, chal.data(),
                                            nullptr, eqc.ptr, eqy.ptr, send_one, nullptr);
-            else if (leaf && round == 0)
+            } else if (round == jround) {
+                rc = gkr_jit_tables(ctx, nibble_level(kind), rowimg.ptr, t, chal.data(), nullptr);
+                if (!rc)
+                    rc = gkr_jit_round_sums(ctx, nibble_level(kind), nibbles.ptr, t, s, eqc.ptr, eqy.ptr, send_one,
+                                            nullptr);
+            } else if (leaf && round == 0)
                 rc = gkr_leaf_round_sums(ctx, (const uint64_t*)packed.ptr, rowimg.ptr, t, s, eqc.ptr, s_c, eqy.ptr,
                                          send_one, nullptr);
             else if (leaf && round == 1)
@@ -522,10 +533,10 @@ struct Prover {
             G rr = gfrom(rb);
             next.push_back(rr);
             if (bit_round) chal.push_back(hgf{rr.lo, rr.hi});
-            if (round == nround) {
-                // the level stays implicit up to here: its fold writes the halves
-                if ((rc = gkr_nibble_fold(ctx, nibble_level(kind), nibbles.ptr, rowimg.ptr, t, s, chal.data(), nullptr,
-                                          hi_weights(weigh), l, r)))
+            if (round == jround) {
+                // the level stays implicit up to here: the table fold writes the halves
+                if ((rc = gkr_jit_fold(ctx, nibble_level(kind), nibbles.ptr, t, s, chal.data(), nullptr,
+                                       hi_weights(weigh), l, r)))
                     return rc;
                 weighted = weigh;
                 pending_fold = false;
@@ -606,36 +617,36 @@ struct Prover {
 
     // returns the leaf point (LE, t+s coordinates) and claim
     int gkr(InnerTranscript& it, const std::vector<G>& zeta, std::vector<G>* point, G* claim) {
-        // Storage: the halves the leaf, level-1 and level-2 layers fold into
-        // (2 x 2^(t+s-4) entries, shared: the layers run one after another),
-        // then levels 3..t-1 back to back (level j: 2^(t+s-j) entries). Levels
-        // 0-2 are never materialised; level 3 comes straight from the bits,
-        // the upper levels from products.
-        const uint64_t h4 = 1ull << (t + s - 4);
-        uint64_t total = 2 * h4;
-        for (int j = 3; j < t; j++) total += 1ull << (t + s - j);
+        // Storage: the halves the leaf and level-1..3 layers fold into
+        // (2 x 2^(t+s-5) entries, shared: the layers run one after another),
+        // then levels 4..t-1 back to back (level j: 2^(t+s-j) entries). Levels
+        // 0-3 are never materialised; level 4 comes from level 3's tables of
+        // the row words, the upper levels from products.
+        const uint64_t h4 = 1ull << (t + s - 4), h5 = h4 >> 1;
+        uint64_t total = 2 * h5;
+        for (int j = 4; j < t; j++) total += 1ull << (t + s - j);
         if (!levels.alloc(total * 16)) return BITZ_ERR_ALLOC;
         std::vector<uint4*> lv(t + 1, nullptr);
         uint4* low = (uint4*)levels.ptr;
-        uint64_t off = 2 * h4;
-        for (int j = 3; j < t; j++) {
+        uint64_t off = 2 * h5;
+        for (int j = 4; j < t; j++) {
             lv[j] = low + off;
             off += 1ull << (t + s - j);
         }
-        if (!nibbles.alloc(2 * h4 * 2)) return BITZ_ERR_ALLOC;  // 2^(t+s-4) u16
+        if (!nibbles.alloc(h4 * 2)) return BITZ_ERR_ALLOC;  // 2^(t+s-4) u16
         int rc = gkr_nibble_rows(ctx, (const uint64_t*)packed.ptr, t, s, nibbles.ptr);
         if (rc) return rc;
-        if ((rc = gkr_level3(ctx, (const uint64_t*)packed.ptr, rowimg.ptr, t, s, lv[3]))) return rc;
-        for (int j = 3; j + 1 < t; j++)
+        if ((rc = gkr_level4(ctx, nibbles.ptr, rowimg.ptr, t, s, lv[4]))) return rc;
+        for (int j = 4; j + 1 < t; j++)
             if ((rc = product_level(ctx, lv[j], 1ull << (t + s - j), lv[j + 1]))) return rc;
         std::vector<G> internal(zeta.rbegin(), zeta.rend());
         G cl = gzero();
-        for (int j = t - 1; j >= 3; j--) {
+        for (int j = t - 1; j >= 4; j--) {
             uint64_t half = 1ull << (t + s - j - 1);
             if ((rc = gkr_layer(it, lv[j], lv[j] + half, kDense, internal, &cl))) return rc;
         }
-        for (LayerKind kind : {kLevel2, kLevel1, kLeaf})
-            if ((rc = gkr_layer(it, low, low + h4, kind, internal, &cl))) return rc;
+        for (LayerKind kind : {kLevel3, kLevel2, kLevel1, kLeaf})
+            if ((rc = gkr_layer(it, low, low + h5, kind, internal, &cl))) return rc;
         point->assign(internal.rbegin(), internal.rend());
         *claim = cl;
         return BITZ_OK;
@@ -685,16 +696,22 @@ struct Prover {
         BITZ_REQUIRE(kind == kDense || (L >= 4 && t >= 7 && (!weigh || wsplit)), BITZ_ERR_SHAPE,
                      "GKR: layer too small for bit patterns");
         const int nround = kind == kDense ? -1 : nibble_round(kind);
+        const int jround = kind == kDense ? -1 : table_round(kind);
         for (int round = 0; round < L; round++) {
             int rem = L - 1 - round;
             uint64_t hh = 1ull << rem;
             int s_c = std::min(s, rem);
             int send_one = geq(internal[round], gzero()) ? 1 : 0;
-            // the layer's challenges r_0.. are rho[0..]: the row-word kernels read them there
-            if (round == nround)
+            // the layer's challenges r_0.. are rho[0..]: the row-word and table kernels read them there
+            if (round == nround) {
                 rc = gkr_nibble_round_sums(ctx, nibble_level(kind), nibbles.ptr, rowimg.ptr, t, s, nullptr, rho,
                                            eqc.ptr, eqy.ptr, send_one, nullptr);
-            else if (leaf && round == 0)
+            } else if (round == jround) {
+                rc = gkr_jit_tables(ctx, nibble_level(kind), rowimg.ptr, t, nullptr, rho);
+                if (!rc)
+                    rc = gkr_jit_round_sums(ctx, nibble_level(kind), nibbles.ptr, t, s, eqc.ptr, eqy.ptr, send_one,
+                                            nullptr);
+            } else if (leaf && round == 0)
                 rc = gkr_leaf_round_sums(ctx, (const uint64_t*)packed.ptr, rowimg.ptr, t, s, eqc.ptr, s_c, eqy.ptr,
                                          send_one, nullptr);
             else if (leaf && round == 1)
@@ -727,12 +744,12 @@ struct Prover {
                                    (uint32_t)round)))
                 return rc;
             pending_fold = false;
-            if (round == nround) {
-                if ((rc = gkr_nibble_fold(ctx, nibble_level(kind), nibbles.ptr, rowimg.ptr, t, s, nullptr, rho,
-                                          hi_weights(weigh), l, r)))
+            if (round == jround) {
+                if ((rc = gkr_jit_fold(ctx, nibble_level(kind), nibbles.ptr, t, s, nullptr, rho, hi_weights(weigh),
+                                       l, r)))
                     return rc;
                 weighted = weigh;
-            } else if (kind != kDense && round < nround) {
+            } else if (kind != kDense && round < jround) {
                 // the level stays implicit: its next round reads the bits again
             } else {
                 pending_fold = true;

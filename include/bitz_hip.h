@@ -226,6 +226,22 @@ int bitz_gkr_nibble_round_sums_dev(bitz_ctx* ctx, int level, const void* nr, con
                                    uint8_t out_host[32]);
 int bitz_gkr_nibble_fold_dev(bitz_ctx* ctx, int level, const void* nr, const void* A, int t, int s,
                              const uint8_t* chal, const void* eq_hi, void* l_out, void* r_out);
+/* The table round after the row-word round (Forest::prove_jit_level,
+ * jit_round_sums, jit_fold_round_with, src/bitz/forest.rs:583-685, 990-1176).
+ * The level-l layer (l = 0..2) after rounds 0..2-l (challenges chal[0..3-l)),
+ * or the level-3 layer from its start (chal unused), has halves that are
+ * per-row 256-entry tables of its slots' row words. bitz_gkr_jit_round_sums_dev
+ * takes round 3 - l's sums (weights eq_y[y'] (2^(t-5) entries) * eq_c[c]);
+ * bitz_gkr_jit_fold_dev folds by chal[3 - l] into halves of 2^(t+s-5) entries
+ * (right half times eq_hi[c >> 8] when given, s > 8). bitz_gkr_level4_dev
+ * writes level 4 of the product tree (2^(t+s-4) entries, level_up of level 3,
+ * src/bitz/forest.rs:864-898) from the row words. */
+int bitz_gkr_jit_round_sums_dev(bitz_ctx* ctx, int level, const void* nr, const void* A, int t, int s,
+                                const uint8_t* chal, const void* eq_c, const void* eq_y, int send_one,
+                                uint8_t out_host[32]);
+int bitz_gkr_jit_fold_dev(bitz_ctx* ctx, int level, const void* nr, const void* A, int t, int s,
+                          const uint8_t* chal, const void* eq_hi, void* l_out, void* r_out);
+int bitz_gkr_level4_dev(bitz_ctx* ctx, const void* nr, const void* A, int t, int s, void* out);
 /* one product level: out[i] = in[i] * in[i + n/2] (gkr.rs:93-108) */
 int bitz_product_level_dev(bitz_ctx* ctx, const void* in, uint64_t n, void* out);
 /* xi_combined_rows_packed (src/ligerito.rs:563-630): out[b] = sum_c w2[c] bit(c,b) */

@@ -146,3 +146,131 @@ def test_nibble_layers_at_2p20_coefficients(level):
     ops.fold_halves_(l, hh, chal[k])
     ops.fold_halves_(r, hh, chal[k])
     assert torch.equal(gl, l[:hh]) and torch.equal(gr, r[:hh])
+
+
+def _jit_round_sums(level, nr, dA, t, s, chal, eq_c, eq_y, send_one):
+    from bitz_amd import _lib
+    out = (C.c_uint8 * 32)()
+    _lib.call("bitz_gkr_jit_round_sums_dev", ctx().ptr, level, _vp(nr), _vp(dA), t, s, _b16(chal), _vp(eq_c),
+              _vp(eq_y), 1 if send_one else 0, C.cast(out, C.c_void_p))
+    b = bytes(out)
+    return int.from_bytes(b[:16], "little"), int.from_bytes(b[16:], "little")
+
+
+def _jit_fold(level, nr, dA, t, s, chal, eq_hi=None):
+    from bitz_amd import _lib
+    n = 1 << (t + s - 5)
+    l, r = ctx().empty_gf(n), ctx().empty_gf(n)
+    _lib.call("bitz_gkr_jit_fold_dev", ctx().ptr, level, _vp(nr), _vp(dA), t, s, _b16(chal),
+              _vp(eq_hi) if eq_hi is not None else None, _vp(l), _vp(r))
+    return l, r
+
+
+@pytest.mark.parametrize("level", [0, 1, 2, 3])
+@pytest.mark.parametrize("t,s", [(7, 1), (8, 3), (9, 6), (10, 8), (11, 9), (12, 5)])
+def test_table_round_against_oracle(level, t, s):
+    """Round 3 - l of the level-l layer from its per-row tables (the sums and
+    the fold into halves of 2^(t+s-5) entries) against the oracle's dense
+    layer folded by the same challenges"""
+    rows, A, ops, packed, dA, nr = _setup(t, s, 100 * t + s + 7)
+    log(f"table round level={level} t={t} s={s}")
+    lvl = O.forest_leaves(rows, t, s, A)
+    for _ in range(level):
+        lvl = O.product_level(lvl)
+    half = lvl.shape[0] // 2
+    l, r = lvl[:half].copy(), lvl[half:].copy()
+    j = 3 - level
+    chal = _rand_ints(j + 1, 13 * t + level)
+    for i in range(j):
+        l, r = O.fold_halves(l, chal[i]), O.fold_halves(r, chal[i])
+    assert l.shape[0] == 1 << (t + s - 4)
+    point = _rand_ints(t + s - 5, 17 * t + s + level)
+    pt = O.ints_to_gf(point)
+    eq_c, eq_y = O.eq_table(pt[:s]), O.eq_table(pt[s:])
+    full_eq = O.eq_table(pt)
+    for so in (False, True):
+        got = _jit_round_sums(level, nr, dA, t, s, chal[:j], to_dev(eq_c), to_dev(eq_y), so)
+        assert got == O.gkr_round_sums(l, r, full_eq, so)
+    gl, gr = _jit_fold(level, nr, dA, t, s, chal)
+    fl, fr = O.fold_halves(l, chal[j]), O.fold_halves(r, chal[j])
+    assert np.array_equal(to_host(gl), fl)
+    assert np.array_equal(to_host(gr), fr)
+    if s > 8:
+        eq_hi = O.eq_table(O.ints_to_gf(_rand_ints(s - 8, 5 * t + level)))
+        wl, wr = _jit_fold(level, nr, dA, t, s, chal, to_dev(eq_hi))
+        assert np.array_equal(to_host(wl), fl)
+        cols_hi = np.repeat(eq_hi, 256, axis=0)
+        assert np.array_equal(to_host(wr), O.mul_arrays(fr, np.tile(cols_hi, (fr.shape[0] >> s, 1))))
+
+
+@pytest.mark.parametrize("t,s", [(7, 1), (9, 6), (11, 9)])
+def test_level4_from_row_words_against_oracle(t, s):
+    from bitz_amd import _lib
+    rows, A, ops, packed, dA, nr = _setup(t, s, 31 * t + s)
+    log(f"level 4 from row words t={t} s={s}")
+    lvl = O.forest_leaves(rows, t, s, A)
+    for _ in range(4):
+        lvl = O.product_level(lvl)
+    out = ctx().empty_gf(1 << (t + s - 4))
+    _lib.call("bitz_gkr_level4_dev", ctx().ptr, _vp(nr), _vp(dA), t, s, _vp(out))
+    assert np.array_equal(to_host(out), lvl)
+
+
+def _dense_level(ops, rows, dA, t, s, level):
+    import torch
+    rows_d = torch.from_numpy(rows.view(np.int64)).to("cuda")
+    shifts = torch.arange(64, device="cuda", dtype=torch.int64)
+    bits = ((rows_d.view(1 << s, 1 << (t - 6), 1) >> shifts) & 1).to(torch.bool).view(1 << s, 1 << t).t()
+    one = torch.tensor([1, 0], dtype=torch.int64, device="cuda")
+    lvl = torch.where(bits.unsqueeze(-1), dA.view(1 << t, 1, 2), one.view(1, 1, 2)).reshape(-1, 2).contiguous()
+    del bits
+    for _ in range(level):
+        lvl = ops.product_level(lvl)
+    return lvl
+
+
+@pytest.mark.parametrize("level", [0, 1, 2, 3])
+def test_table_round_at_2p20_coefficients(level):
+    """t = 15, s = 11: the table round's sums and folds (plain and with the
+    eq_hi-weighted right half) against the device's dense kernels, and level
+    4 from the row words against three product levels over level 1"""
+    import torch
+    t, s = 15, 11
+    rows, A, ops, packed, dA, nr = _setup(t, s, 5151 + level)
+    log(f"table round level={level} at t={t} s={s}")
+    lvl = _dense_level(ops, rows, dA, t, s, level)
+    half = lvl.shape[0] // 2
+    l, r = lvl[:half].clone(), lvl[half:].clone()
+    del lvl
+    j = 3 - level
+    chal = _rand_ints(j + 1, 77 + level)
+    h = half
+    for i in range(j):
+        h //= 2
+        ops.fold_halves_(l, h, chal[i])
+        ops.fold_halves_(r, h, chal[i])
+    assert h == 1 << (t + s - 4)
+    point = _rand_ints(t + s - 5, 9 + level)
+    pt = O.ints_to_gf(point)
+    eq_c, eq_y = O.eq_table(pt[:s]), O.eq_table(pt[s:])
+    hh = h // 2
+    for so in (False, True):
+        want = ops.gkr_round_sums(l[:h].contiguous(), r[:h].contiguous(), hh, to_dev(eq_c), s, to_dev(eq_y), so)
+        got = _jit_round_sums(level, nr, dA, t, s, chal[:j], to_dev(eq_c), to_dev(eq_y), so)
+        assert got == tuple(want)
+    gl, gr = _jit_fold(level, nr, dA, t, s, chal)
+    eq_hi = O.eq_table(O.ints_to_gf(_rand_ints(s - 8, 3 + level)))
+    wl, wr = _jit_fold(level, nr, dA, t, s, chal, to_dev(eq_hi))
+    ops.fold_halves_(l, hh, chal[j])
+    ops.fold_halves_(r, hh, chal[j])
+    assert torch.equal(gl, l[:hh]) and torch.equal(gr, r[:hh])
+    assert torch.equal(wl, l[:hh])
+    cols_hi = torch.from_numpy(np.repeat(eq_hi, 256, axis=0).view(np.int64)).to("cuda")
+    want_r = ctx().gf_mul(r[:hh].contiguous(), cols_hi.repeat(hh >> s, 1).contiguous())
+    assert torch.equal(wr, want_r)
+    if level == 0:
+        from bitz_amd import _lib
+        lv4 = _dense_level(ops, rows, dA, t, s, 4)
+        out = ctx().empty_gf(1 << (t + s - 4))
+        _lib.call("bitz_gkr_level4_dev", ctx().ptr, _vp(nr), _vp(dA), t, s, _vp(out))
+        assert torch.equal(out, lv4)

@@ -137,6 +137,8 @@ struct Scratch {
     DevBuf partials, result, eq_lo, fold_partial, comb, what, misc, leaf_tab;
     // the row-word rounds' per-row value tables and product / fold tables
     DevBuf nib_t, nib_tab;
+    // row words built for a pattern-sums call that was not given them
+    DevBuf nib_rows;
     // the proof-of-work search's best nonce: all ones between searches (each
     // search's final report resets it)
     DevBuf pow_best;

@@ -511,56 +511,70 @@ __global__ __launch_bounds__(kT) void k_leaf_pattern_sums(const uint64_t* __rest
     block_xor_store<2>(acc, partials + (size_t)blockIdx.x * 2);
 }
 
-// The same sums with the rows' tables staged in LDS, for at least 1024
-// columns: block (1024 columns, rpb rows) takes G = 2048 / 2^K rows at a time,
-// copies their G x 2^K table entries (64 KiB) into LDS with coalesced loads,
-// and each thread walks its 4 columns over those rows, reading each entry
-// from LDS. A row's table crosses the memory hierarchy once per column block
-// (8 times at 2^13 columns) instead of once per wavefront gather.
+// The same sums from the 16-bit row words (nr[(y' << s) | c], y' < Y4 =
+// 2^(t-4): column c's bits at rows y' + w Y4, even w in the low byte, odd w
+// in the high byte). Block y' scatters eq_c[c] into 256 buckets per byte of
+// its words (LDS XOR), then contracts the buckets with its rows' tables, so
+// each table entry is read once:
+//   K = 8, ny = 2 Y4: row y = y' + h Y4 reads byte h (pattern bit k <-> row
+//     y + k ny = y' + Y4 (h + 2k), word bit h + 2k);
+//   K = 4, ny = 4 Y4: row y = y' + v Y4 reads bits (v >> 1) + 2k of byte
+//     v & 1 (row y' + Y4 (v + 4k)); the 256 buckets of that byte are first
+//     summed down to the 16 patterns.
 template <int K>
-__global__ __launch_bounds__(kT) void k_leaf_pattern_sums_lds(const uint64_t* __restrict__ packed, int t, int s,
-                                                              uint64_t stride, const uint4* __restrict__ qtab,
-                                                              const uint4* __restrict__ eq_c, uint64_t rpb,
-                                                              uint4* __restrict__ partials) {
-    constexpr int G = 2048 >> K;
-    __shared__ uint4 tab[(G << K) * 2];
-    const uint64_t nrows = 1ull << t, colblocks = (1ull << s) >> 10;
-    const uint64_t cb = blockIdx.x % colblocks, rb = blockIdx.x / colblocks;
-    const uint64_t y0 = rb * rpb, cbase = cb << 10;
-    uint32_t e[4][4], q[4][4];
+__global__ __launch_bounds__(kT) void k_byte_bucket_sums(const uint16_t* __restrict__ nr, int t, int s,
+                                                         const uint4* __restrict__ qtab,
+                                                         const uint4* __restrict__ eq_c,
+                                                         uint4* __restrict__ partials) {
+    static_assert(K == 4 || K == 8, "K is 4 or 8");
+    __shared__ uint32_t B[4][2][256];  // [word][byte][index]
+    const uint64_t yp = blockIdx.x, y4 = 1ull << (t - 4), cols = 1ull << s;
+    for (uint32_t i = threadIdx.x; i < 4 * 2 * 256; i += kT) (&B[0][0][0])[i] = 0u;
+    __syncthreads();
+    const uint16_t* wp = nr + (yp << s);
+    for (uint64_t c = threadIdx.x; c < cols; c += kT) {
+        const uint32_t w = wp[c], lo = w & 255u, hi = w >> 8;
+        const uint4 e = eq_c[c];
+        atomicXor(&B[0][0][lo], e.x);
+        atomicXor(&B[1][0][lo], e.y);
+        atomicXor(&B[2][0][lo], e.z);
+        atomicXor(&B[3][0][lo], e.w);
+        atomicXor(&B[0][1][hi], e.x);
+        atomicXor(&B[1][1][hi], e.y);
+        atomicXor(&B[2][1][hi], e.z);
+        atomicXor(&B[3][1][hi], e.w);
+    }
+    __syncthreads();
+    gf256 end = gf256_zero(), inf = gf256_zero();
+    if constexpr (K == 8) {
+        const uint32_t i = threadIdx.x;
 #pragma unroll
-    for (int j = 0; j < 4; j++)
-#pragma unroll
-        for (int w = 0; w < 4; w++) e[j][w] = q[j][w] = 0;
-    for (uint64_t g0 = y0; g0 < y0 + rpb; g0 += G) {
-        __syncthreads();  // the previous rows' readers are done
-        const uint4* src = qtab + (g0 << K) * 2;
-        for (uint32_t i = threadIdx.x; i < (uint32_t)((G << K) * 2); i += kT) tab[i] = src[i];
-        __syncthreads();
-#pragma unroll
-        for (int j = 0; j < 4; j++) {
-            const uint64_t c = cbase + threadIdx.x + 256 * j;
-            const uint64_t* col = packed + (c >> 6) * nrows;
-            const int bit = (int)(c & 63);
-            for (int gy = 0; gy < G; gy++) {
-                const uint64_t y = g0 + gy;
-                uint32_t pat = 0;
-#pragma unroll
-                for (int k = 0; k < K; k++) pat |= (uint32_t)((col[y + k * stride] >> bit) & 1ull) << k;
-                const uint4 a = tab[((gy << K) + pat) * 2], b = tab[((gy << K) + pat) * 2 + 1];
-                e[j][0] ^= a.x, e[j][1] ^= a.y, e[j][2] ^= a.z, e[j][3] ^= a.w;
-                q[j][0] ^= b.x, q[j][1] ^= b.y, q[j][2] ^= b.z, q[j][3] ^= b.w;
-            }
+        for (uint32_t h = 0; h < 2; h++) {
+            const gf128 bv = gf128{{B[0][h][i], B[1][h][i], B[2][h][i], B[3][h][i]}};
+            const uint4* q = qtab + ((((yp + h * y4) << 8) + i) << 1);
+            gf256_acc(&end, gf_mul_wide(gf_load(&q[0]), bv));
+            gf256_acc(&inf, gf_mul_wide(gf_load(&q[1]), bv));
         }
-    }
-    gf128 acc[2] = {gf_zero(), gf_zero()};
+    } else if (threadIdx.x < 64) {
+        const uint32_t v = threadIdx.x >> 4, p = threadIdx.x & 15u, byte = v & 1u, sh = v >> 1;
+        uint32_t fixed = 0;
 #pragma unroll
-    for (int j = 0; j < 4; j++) {
-        const gf128 wc = gf_load(&eq_c[cbase + threadIdx.x + 256 * j]);
-        acc[0] = gf_add(acc[0], gf_mul(gf128{{e[j][0], e[j][1], e[j][2], e[j][3]}}, wc));
-        acc[1] = gf_add(acc[1], gf_mul(gf128{{q[j][0], q[j][1], q[j][2], q[j][3]}}, wc));
+        for (uint32_t k = 0; k < 4; k++) fixed |= ((p >> k) & 1u) << (sh + 2 * k);
+        uint32_t m[4] = {0, 0, 0, 0};
+        for (uint32_t f = 0; f < 16; f++) {
+            uint32_t b = fixed;
+#pragma unroll
+            for (uint32_t k = 0; k < 4; k++) b |= ((f >> k) & 1u) << ((1 - sh) + 2 * k);
+#pragma unroll
+            for (int wd = 0; wd < 4; wd++) m[wd] ^= B[wd][byte][b];
+        }
+        const gf128 bv = gf128{{m[0], m[1], m[2], m[3]}};
+        const uint4* q = qtab + ((((yp + v * y4) << 4) + p) << 1);
+        gf256_acc(&end, gf_mul_wide(gf_load(&q[0]), bv));
+        gf256_acc(&inf, gf_mul_wide(gf_load(&q[1]), bv));
     }
-    block_xor_store<2>(acc, partials + (size_t)blockIdx.x * 2);
+    gf128 sums[2] = {gf_reduce(end), gf_reduce(inf)};
+    block_xor_store<2>(sums, partials + (size_t)blockIdx.x * 2);
 }
 
 // Round 1 of the leaf layer. After round 0 folded by rho0, the halves are
@@ -1725,37 +1739,46 @@ int gkr_round_sums(bitz_ctx* ctx, const void* l, const void* r, uint64_t h, cons
 static size_t leaf_tab_bytes(int t) { return (size_t)16 * ((1ull << (t + 1)) + (1ull << (t + 6))); }
 static uint4* leaf_tab_hi(bitz_ctx* ctx, int t) { return (uint4*)ctx->scratch.leaf_tab.ptr + (1ull << (t + 1)); }
 
-// threads of a pattern-sums launch: (column, ylane) pairs, about 2^20
+// threads of a gather pattern-sums launch: (column, ylane) pairs, about 2^20
 static uint64_t pattern_ylanes(int s, uint64_t ny) {
     return s >= 20 ? 1 : std::min<uint64_t>(ny, (1ull << 20) >> s);
 }
 
-// Blocks of a K-bit pattern-sums launch over ny rows (row stride ny): the
-// LDS-table kernel (rpb rows per block) for at least 1024 columns, the
-// gather kernel otherwise (rpb = 0).
+// A K-bit pattern-sums launch over ny = 2^(K/4 - 1) Y4... rows (row stride
+// ny): off the row words nr (built into the scratch buffer when not given)
+// for t >= 4, one block per row-word row; the per-(column, row) gather
+// kernel for smaller t. Returns the block count.
 template <int K>
-static unsigned pattern_sums_blocks(int s, uint64_t ny, uint64_t* rpb) {
-    constexpr uint64_t G = 2048 >> K;
-    if (s >= 10 && ny >= G && ny % G == 0) {
-        const uint64_t colblocks = (1ull << s) >> 10;
-        uint64_t rowblocks = std::min<uint64_t>(std::max<uint64_t>(1, 512 / colblocks), ny / G);
-        while (rowblocks > 1 && ny % (rowblocks * G)) rowblocks >>= 1;
-        *rpb = ny / rowblocks;
-        return (unsigned)(colblocks * rowblocks);
+static int launch_pattern_sums(bitz_ctx* ctx, const uint64_t* packed, const void* nr, int t, int s, uint64_t ny,
+                               const uint4* tab, const void* eq_c, unsigned* nb_out) {
+    if (t >= 4) {
+        const uint64_t y4 = 1ull << (t - 4);
+        BITZ_REQUIRE(ny == (K == 8 ? 2 * y4 : 4 * y4) && y4 <= 0x7fffffffull, BITZ_ERR_SHAPE,
+                     "pattern sums: rows do not match the row words");
+        if (!nr) {
+            if (!ctx->scratch.nib_rows.alloc((y4 << s) * 2)) return BITZ_ERR_ALLOC;
+            int rc = gkr_nibble_rows(ctx, packed, t, s, ctx->scratch.nib_rows.ptr);
+            if (rc) return rc;
+            nr = ctx->scratch.nib_rows.ptr;
+        }
+        const unsigned nb = (unsigned)y4;
+        int rc = ctx->scratch.ensure((size_t)nb * 2 * 16);
+        if (rc) return rc;
+        hipLaunchKernelGGL(k_byte_bucket_sums<K>, dim3(nb), dim3(kT), 0, ctx->stream, (const uint16_t*)nr, t, s, tab,
+                           (const uint4*)eq_c, (uint4*)ctx->scratch.partials.ptr);
+        BITZ_LAUNCH_CHECK();
+        *nb_out = nb;
+        return BITZ_OK;
     }
-    *rpb = 0;
-    return (unsigned)(((1ull << s) * pattern_ylanes(s, ny) + kT - 1) / kT);
-}
-template <int K>
-static void launch_pattern_sums(bitz_ctx* ctx, unsigned nb, uint64_t rpb, const uint64_t* packed, int t, int s,
-                                uint64_t ny, const uint4* tab, const void* eq_c) {
-    uint4* parts = (uint4*)ctx->scratch.partials.ptr;
-    if (rpb)
-        hipLaunchKernelGGL(k_leaf_pattern_sums_lds<K>, dim3(nb), dim3(kT), 0, ctx->stream, packed, t, s, ny, tab,
-                           (const uint4*)eq_c, rpb, parts);
-    else
-        hipLaunchKernelGGL(k_leaf_pattern_sums<K>, dim3(nb), dim3(kT), 0, ctx->stream, packed, t, s, ny, ny, tab,
-                           (const uint4*)eq_c, pattern_ylanes(s, ny), parts);
+    const uint64_t ylanes = pattern_ylanes(s, ny);
+    const unsigned nb = (unsigned)(((1ull << s) * ylanes + kT - 1) / kT);
+    int rc = ctx->scratch.ensure((size_t)nb * 2 * 16);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_leaf_pattern_sums<K>, dim3(nb), dim3(kT), 0, ctx->stream, packed, t, s, ny, ny, tab,
+                       (const uint4*)eq_c, ylanes, (uint4*)ctx->scratch.partials.ptr);
+    BITZ_LAUNCH_CHECK();
+    *nb_out = nb;
+    return BITZ_OK;
 }
 
 int gkr_premul_round_sums(bitz_ctx* ctx, const void* l, void* r, uint64_t h, const void* eq_c, int s,
@@ -1859,24 +1882,20 @@ int gkr_fold_round_sums(bitz_ctx* ctx, void* l, void* r, uint64_t h, const uint8
 }
 
 int gkr_leaf_round_sums(bitz_ctx* ctx, const uint64_t* packed, const void* A, int t, int s, const void* eq_c,
-                        int s_c, const void* eq_y, int send_one, void* out_dev) {
+                        int s_c, const void* eq_y, int send_one, void* out_dev, const void* nr) {
     uint64_t h = 1ull << (t + s - 2);
     if (s_c == s && t >= 2) {
         // weights eq_y[y] * eq_c[c]: the bit-pattern path
         const uint64_t q = 1ull << (t - 2), cols = 1ull << s;
         if (!ctx->scratch.leaf_tab.alloc(leaf_tab_bytes(t))) return BITZ_ERR_ALLOC;
         uint4* qtab = (uint4*)ctx->scratch.leaf_tab.ptr;
-        uint64_t rpb;
-        const unsigned nb = pattern_sums_blocks<4>(s, q, &rpb);
-        const double threads = (double)cols;
-        int rc = ctx->scratch.ensure((size_t)nb * 2 * 16);
-        if (rc) return rc;
-        TimedLaunch tl(ctx, "gkr_leaf_round_sums", (double)(1ull << (t + s - 3)) + q * 512.0,
-                       q * 64.0 * kGfMulOps + 2.0 * threads * kGfMulOps);
+        TimedLaunch tl(ctx, "gkr_leaf_round_sums", 2.0 * (double)(q << s) / 4.0 + 16.0 * (double)cols + q * 512.0,
+                       q * 64.0 * kGfMulOps);
         hipLaunchKernelGGL(k_leaf_qtab, dim3((unsigned)((q * 16 + kT - 1) / kT)), dim3(kT), 0, ctx->stream,
                            (const uint4*)A, t, (const uint4*)eq_y, send_one, qtab);
-        launch_pattern_sums<4>(ctx, nb, rpb, packed, t, s, q, (const uint4*)qtab, eq_c);
-        BITZ_LAUNCH_CHECK();
+        unsigned nb;
+        int rc = launch_pattern_sums<4>(ctx, packed, nr, t, s, q, (const uint4*)qtab, eq_c, &nb);
+        if (rc) return rc;
         return sums_finish(ctx, nb, 2, out_dev);
     }
     uint32_t R = round_chunk_r(ctx->tune, h, s_c);
@@ -1928,7 +1947,8 @@ int gkr_leaf_fold(bitz_ctx* ctx, const uint64_t* packed, const void* A, int t, i
 }
 
 int gkr_leaf_round1_sums(bitz_ctx* ctx, const uint64_t* packed, const void* A, int t, int s, const uint8_t rho0[16],
-                         const void* eq_c, const void* eq_y, int send_one, void* out_dev, const void* rho0_dev) {
+                         const void* eq_c, const void* eq_y, int send_one, void* out_dev, const void* rho0_dev,
+                         const void* nr) {
     BITZ_REQUIRE(t >= 3, BITZ_ERR_SHAPE, "gkr_leaf_round1_sums: t < 3");
     uint64_t lo, hi;
     split16(rho0, &lo, &hi);
@@ -1936,19 +1956,15 @@ int gkr_leaf_round1_sums(bitz_ctx* ctx, const uint64_t* packed, const void* A, i
     if (!ctx->scratch.leaf_tab.alloc(leaf_tab_bytes(t))) return BITZ_ERR_ALLOC;
     uint4* ftab = (uint4*)ctx->scratch.leaf_tab.ptr;
     uint4* q1tab = leaf_tab_hi(ctx, t);
-    uint64_t rpb;
-    const unsigned nb = pattern_sums_blocks<8>(s, q8, &rpb);
-    const double threads = (double)cols;
-    int rc = ctx->scratch.ensure((size_t)nb * 2 * 16);
-    if (rc) return rc;
-    TimedLaunch tl(ctx, "gkr_leaf_round1_sums", (double)(1ull << (t + s - 3)) + q8 * 8192.0,
-                   (q * 8.0 + q8 * 1024.0 + 2.0 * threads) * kGfMulOps);
+    TimedLaunch tl(ctx, "gkr_leaf_round1_sums", (double)(q8 << s) + 16.0 * (double)cols + q8 * 8192.0,
+                   (q * 8.0 + q8 * 1024.0) * kGfMulOps);
     hipLaunchKernelGGL(k_leaf_ftab, dim3((unsigned)((q * 8 + kT - 1) / kT)), dim3(kT), 0, ctx->stream,
                        (const uint4*)A, t, lo, hi, (const uint4*)rho0_dev, ftab);
     hipLaunchKernelGGL(k_leaf_q1tab, dim3((unsigned)((q8 * 256 + kT - 1) / kT)), dim3(kT), 0, ctx->stream,
                        (const uint4*)ftab, t, (const uint4*)eq_y, send_one, q1tab);
-    launch_pattern_sums<8>(ctx, nb, rpb, packed, t, s, q8, (const uint4*)q1tab, eq_c);
-    BITZ_LAUNCH_CHECK();
+    unsigned nb;
+    int rc = launch_pattern_sums<8>(ctx, packed, nr, t, s, q8, (const uint4*)q1tab, eq_c, &nb);
+    if (rc) return rc;
     return sums_finish(ctx, nb, 2, out_dev);
 }
 
@@ -2005,25 +2021,21 @@ int gkr_leaf_fold2(bitz_ctx* ctx, const uint64_t* packed, const void* A, int t, 
 }
 
 int gkr_level1_round0_sums(bitz_ctx* ctx, const uint64_t* packed, const void* A, int t, int s, const void* eq_c,
-                           const void* eq_y, int send_one, void* out_dev) {
+                           const void* eq_y, int send_one, void* out_dev, const void* nr) {
     BITZ_REQUIRE(t >= 3, BITZ_ERR_SHAPE, "gkr_level1_round0_sums: t < 3");
     const uint64_t Y1 = 1ull << (t - 1), Y3 = 1ull << (t - 3), cols = 1ull << s;
     if (!ctx->scratch.leaf_tab.alloc(leaf_tab_bytes(t))) return BITZ_ERR_ALLOC;
     uint4* vtab = (uint4*)ctx->scratch.leaf_tab.ptr;
     uint4* qtab = leaf_tab_hi(ctx, t);
-    uint64_t rpb;
-    const unsigned nb = pattern_sums_blocks<8>(s, Y3, &rpb);
-    const double threads = (double)cols;
-    int rc = ctx->scratch.ensure((size_t)nb * 2 * 16);
-    if (rc) return rc;
-    TimedLaunch tl(ctx, "gkr_level1_round0_sums", (double)(1ull << (t + s - 3)) + Y3 * 8192.0,
-                   (Y1 + Y3 * 1024.0 + 2.0 * threads) * kGfMulOps);
+    TimedLaunch tl(ctx, "gkr_level1_round0_sums", (double)(Y3 << s) + 16.0 * (double)cols + Y3 * 8192.0,
+                   (Y1 + Y3 * 1024.0) * kGfMulOps);
     hipLaunchKernelGGL(k_level1_vtab, dim3((unsigned)((Y1 + kT - 1) / kT)), dim3(kT), 0, ctx->stream,
                        (const uint4*)A, t, vtab);
     hipLaunchKernelGGL(k_level1_qtab, dim3((unsigned)((Y3 * 256 + kT - 1) / kT)), dim3(kT), 0, ctx->stream,
                        (const uint4*)vtab, t, (const uint4*)eq_y, send_one, qtab);
-    launch_pattern_sums<8>(ctx, nb, rpb, packed, t, s, Y3, (const uint4*)qtab, eq_c);
-    BITZ_LAUNCH_CHECK();
+    unsigned nb;
+    int rc = launch_pattern_sums<8>(ctx, packed, nr, t, s, Y3, (const uint4*)qtab, eq_c, &nb);
+    if (rc) return rc;
     return sums_finish(ctx, nb, 2, out_dev);
 }
 

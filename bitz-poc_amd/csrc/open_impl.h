@@ -43,8 +43,11 @@ int gkr_round_sums(bitz_ctx* ctx, const void* l, const void* r, uint64_t h, cons
 int gkr_fold_round_sums(bitz_ctx* ctx, void* l, void* r, uint64_t h, const uint8_t rho[16], const void* eq_c,
                         int s_c, const void* eq_y, int send_one, void* out_dev, const void* rho_dev = nullptr,
                         const void* eq_lo = nullptr, int lo_bits = 0);
+// (with s_c == s from K-bit row patterns: off the row words nr (t >= 4; built
+// into a scratch buffer when nr is null), the gather kernel below t = 4; the
+// same for gkr_leaf_round1_sums and gkr_level1_round0_sums)
 int gkr_leaf_round_sums(bitz_ctx* ctx, const uint64_t* packed, const void* A, int t, int s, const void* eq_c,
-                        int s_c, const void* eq_y, int send_one, void* out_dev);
+                        int s_c, const void* eq_y, int send_one, void* out_dev, const void* nr = nullptr);
 int fold_halves(bitz_ctx* ctx, void* v, uint64_t h, const uint8_t rho[16]);
 // both halves l and r (2h entries each) folded by rho in one launch
 // (rho_dev: the challenge in device memory, from the device transcript; rho unused then)
@@ -55,7 +58,7 @@ int gkr_leaf_fold(bitz_ctx* ctx, const uint64_t* packed, const void* A, int t, i
 // patterns; weights eq_y[y] * eq_c[c] over all s column coordinates
 int gkr_leaf_round1_sums(bitz_ctx* ctx, const uint64_t* packed, const void* A, int t, int s, const uint8_t rho0[16],
                          const void* eq_c, const void* eq_y, int send_one, void* out_dev,
-                         const void* rho0_dev = nullptr);
+                         const void* rho0_dev = nullptr, const void* nr = nullptr);
 // the leaf layer folded by rho0 then rho1 into halves of 2^(t+s-3) entries;
 // with eq_c the right half is written multiplied by eq_c[c] (c = low s bits),
 // with eq_hi (s > 8, eq_c null) by eq_hi[c >> 8]
@@ -79,7 +82,7 @@ int tensor_table(bitz_ctx* ctx, const void* a, const void* b, int n, void* out);
 // round-0 sums (weights eq_y[y] (2^(t-3)) * eq_c[c]) and the round-0 fold
 // into halves of 2^(t+s-3) entries (right half times eq_c when given)
 int gkr_level1_round0_sums(bitz_ctx* ctx, const uint64_t* packed, const void* A, int t, int s, const void* eq_c,
-                           const void* eq_y, int send_one, void* out_dev);
+                           const void* eq_y, int send_one, void* out_dev, const void* nr = nullptr);
 int gkr_level1_fold(bitz_ctx* ctx, const uint64_t* packed, const void* A, int t, int s, const uint8_t rho[16],
                     const void* eq_c, void* l_out, void* r_out, const void* rho_dev = nullptr,
                     const void* eq_hi = nullptr);

@@ -493,13 +493,13 @@ struct Prover {
                                             nullptr);
             } else if (leaf && round == 0)
                 rc = gkr_leaf_round_sums(ctx, (const uint64_t*)packed.ptr, rowimg.ptr, t, s, eqc.ptr, s_c, eqy.ptr,
-                                         send_one, nullptr);
+                                         send_one, nullptr, nibbles.ptr);
             else if (leaf && round == 1)
                 rc = gkr_leaf_round1_sums(ctx, (const uint64_t*)packed.ptr, rowimg.ptr, t, s, rho0, eqc.ptr, eqy.ptr,
-                                          send_one, nullptr);
+                                          send_one, nullptr, nullptr, nibbles.ptr);
             else if (kind == kLevel1 && round == 0)
                 rc = gkr_level1_round0_sums(ctx, (const uint64_t*)packed.ptr, rowimg.ptr, t, s, eqc.ptr, eqy.ptr,
-                                            send_one, nullptr);
+                                            send_one, nullptr, nibbles.ptr);
             else if (pending_fold && weighted && rem >= s) {
                 rc = gkr_fold_round_sums(ctx, l, r, hh, pending_rho, nullptr, s, eqy.ptr, send_one, nullptr, nullptr,
                                          wsplit ? eqlo.ptr : nullptr, wlo);
@@ -713,13 +713,13 @@ struct Prover {
                                             nullptr);
             } else if (leaf && round == 0)
                 rc = gkr_leaf_round_sums(ctx, (const uint64_t*)packed.ptr, rowimg.ptr, t, s, eqc.ptr, s_c, eqy.ptr,
-                                         send_one, nullptr);
+                                         send_one, nullptr, nibbles.ptr);
             else if (leaf && round == 1)
                 rc = gkr_leaf_round1_sums(ctx, (const uint64_t*)packed.ptr, rowimg.ptr, t, s, nullptr, eqc.ptr,
-                                          eqy.ptr, send_one, nullptr, rho);
+                                          eqy.ptr, send_one, nullptr, rho, nibbles.ptr);
             else if (kind == kLevel1 && round == 0)
                 rc = gkr_level1_round0_sums(ctx, (const uint64_t*)packed.ptr, rowimg.ptr, t, s, eqc.ptr, eqy.ptr,
-                                            send_one, nullptr);
+                                            send_one, nullptr, nibbles.ptr);
             else if (pending_fold && weighted && rem >= s) {
                 rc = gkr_fold_round_sums(ctx, l, r, hh, nullptr, nullptr, s, eqy.ptr, send_one, nullptr, pending_rho,
                                          wsplit ? eqlo.ptr : nullptr, wlo);

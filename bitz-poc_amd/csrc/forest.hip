@@ -30,6 +30,7 @@
 #include "gf128_dev.h"
 #include "open_impl.h"
 #include "reduce_dev.h"
+#include "bits_dev.h"
 
 namespace bitz {
 namespace {
@@ -87,6 +88,36 @@ __global__ __launch_bounds__(kT) void k_nibble_rows(const uint64_t* __restrict__
             w |= (uint32_t)((col[(2 * i + 1) * y4] >> bit) & 1ull) << (8 + i);
         }
         nr[x] = (uint16_t)w;
+    }
+}
+
+// The same words by wavefront transposes (s >= 6, t >= 6): a wave takes
+// column group g (64 columns) and rows y0..y0+3; lane L = 4w + yo loads the
+// packed word of row y0 + yo + w Y4, and after the 64 x 64 bit transpose
+// lane j holds bit(c, y0 + yo + w Y4) at bit 4w + yo for its column c =
+// 64g + j. Each of its four words is then two 8-bit gathers (bits 8i and
+// 8i + 4 of m >> yo: the even and the odd w) by one multiply each.
+__device__ __forceinline__ uint32_t gather8(uint64_t x) {
+    return (uint32_t)(((x & 0x0101010101010101ull) * 0x0102040810204080ull) >> 56);
+}
+
+__global__ __launch_bounds__(kT) void k_nibble_rows_wave(const uint64_t* __restrict__ packed, int t, int s,
+                                                         uint16_t* __restrict__ nr) {
+    const uint64_t y4 = 1ull << (t - 4), nrows = 1ull << t;
+    const int ylog = t - 6;  // log2 of the row quads
+    const uint64_t steps = 1ull << (ylog + s - 6);
+    const int lane = threadIdx.x & 63;
+    const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+    const uint32_t w = (uint32_t)lane >> 2, yo = (uint32_t)lane & 3u;
+    for (uint64_t st = grid_tid() >> 6; st < steps; st += nwaves) {
+        const uint64_t y0 = (st & ((1ull << ylog) - 1)) << 2, g = st >> ylog;
+        const uint64_t m = wave_transpose64(packed[g * nrows + y0 + yo + w * y4], lane);
+        uint16_t* out = nr + (y0 << s) + (g << 6) + lane;
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const uint64_t x = m >> q;
+            out[(uint64_t)q << s] = (uint16_t)(gather8(x) | (gather8(x >> 4) << 8));
+        }
     }
 }
 
@@ -503,8 +534,12 @@ int gkr_nibble_rows(bitz_ctx* ctx, const uint64_t* packed, int t, int s, void* n
     BITZ_REQUIRE(t >= 4, BITZ_ERR_SHAPE, "gkr_nibble_rows: t < 4");
     const uint64_t n = (1ull << (t - 4)) << s;
     TimedLaunch tl(ctx, "nibble_rows", (double)(n * 2) + (double)((1ull << (t + s)) / 8), 0.0);
-    hipLaunchKernelGGL(k_nibble_rows, dim3(blocks_for(n, 1, 8192)), dim3(kT), 0, ctx->stream, packed, t, s,
-                       (uint16_t*)nr);
+    if (s >= 6 && t >= 6)
+        hipLaunchKernelGGL(k_nibble_rows_wave, dim3(blocks_for(n / 64, 1, 8192)), dim3(kT), 0, ctx->stream, packed,
+                           t, s, (uint16_t*)nr);
+    else
+        hipLaunchKernelGGL(k_nibble_rows, dim3(blocks_for(n, 1, 8192)), dim3(kT), 0, ctx->stream, packed, t, s,
+                           (uint16_t*)nr);
     BITZ_LAUNCH_CHECK();
     return BITZ_OK;
 }

@@ -149,6 +149,11 @@ struct Scratch {
     // reduction kernel itself, so the host polls instead of copying + syncing.
     void* mbox = nullptr;
     void* mbox_dev = nullptr;
+    // pinned, device-mapped, coherent host memory the host tails of the GKR
+    // layers arrive in (written by a kernel, announced through the mailbox flag)
+    void* tail = nullptr;
+    void* tail_dev = nullptr;
+    size_t tail_bytes = 0;
     uint32_t seq = 0;
     bool mbox_pending = false;
     // pinned host staging for the host-side last rounds of a sumcheck
@@ -279,6 +284,16 @@ struct PhaseClock {
     }
     void close();
     ~PhaseClock() { close(); }
+};
+// Accumulates the lifetime of the scope into "phase:<name>" (as PhaseClock),
+// for sub-phases that recur inside one phase (GKR layer set-up, host tails,
+// mailbox waits).
+struct SubPhase {
+    bitz_ctx* ctx;
+    const char* name;
+    std::chrono::steady_clock::time_point t0;
+    SubPhase(bitz_ctx* c, const char* n) : ctx(c), name(n), t0(std::chrono::steady_clock::now()) {}
+    ~SubPhase();
 };
 }  // namespace bitz
 

@@ -54,6 +54,14 @@ void PhaseClock::close() {
     cur.clear();
 }
 
+SubPhase::~SubPhase() {
+    if (ctx->timing && (ctx->timing_only.empty() || ctx->timing_only == kPhasesOnly)) {
+        auto& st = ctx->stats[std::string("phase:") + name];
+        st.launches++;
+        st.ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    }
+}
+
 static void resolve_timings(bitz_ctx* ctx) {
     for (auto& p : ctx->pending) {
         (void)hipEventSynchronize(p.stop);

@@ -1492,6 +1492,19 @@ __global__ __launch_bounds__(kFsThreads) void k_transcript_challenges(bitz::B3St
     }
 }
 
+// l[0..n) then r[0..n) into host-mapped memory, then (after every thread's
+// system-scope fence) the sequence flag the host polls on. One block.
+__global__ void k_pair_to_host(const uint4* __restrict__ l, const uint4* __restrict__ r, uint64_t n, uint4* dst,
+                               uint32_t* flag, uint32_t seq) {
+    for (uint64_t i = threadIdx.x; i < n; i += blockDim.x) {
+        dst[i] = l[i];
+        dst[n + i] = r[i];
+    }
+    __threadfence_system();
+    __syncthreads();
+    if (threadIdx.x == 0) *(volatile uint32_t*)flag = seq;
+}
+
 }  // namespace
 
 // ================================================================ host side
@@ -1518,6 +1531,7 @@ int Scratch::ensure(size_t partial_bytes) {
 }
 Scratch::~Scratch() {
     if (host) (void)hipHostFree(host);
+    if (tail) (void)hipHostFree(tail);
     if (mbox) (void)hipHostFree(mbox);
     if (stage) (void)hipHostFree(stage);
     if (up) (void)hipHostFree(up);
@@ -1613,36 +1627,72 @@ int sums_finish(bitz_ctx* ctx, unsigned nblocks, int K, void* out_dev) {
     return BITZ_OK;
 }
 
+// Spins on the mailbox flag until it reads the pending sequence number;
+// every 1024 polls ask whether the stream has drained, so a launch that
+// never delivers ends in an error, not a hang.
+static int wait_mbox(bitz_ctx* ctx, const char* what) {
+    Scratch& sc = ctx->scratch;
+    volatile uint32_t* flag = (volatile uint32_t*)sc.mbox;
+    for (uint64_t spin = 1;; spin++) {
+        if (*flag == sc.seq) break;
+        if (spin > (1u << 16)) std::this_thread::yield();  // a long kernel: share the core
+        if ((spin & 1023) == 0) {
+            hipError_t q = hipStreamQuery(ctx->stream);
+            if (q == hipSuccess) {
+                if (*flag == sc.seq) break;
+                set_error(std::string(what) + ": the stream finished without delivering its values");
+                return BITZ_ERR_DEVICE;
+            }
+            if (q != hipErrorNotReady) {
+                set_error(std::string(what) + ": " + hipGetErrorString(q));
+                return BITZ_ERR_DEVICE;
+            }
+        }
+    }
+    std::atomic_thread_fence(std::memory_order_acquire);
+    sc.mbox_pending = false;
+    return BITZ_OK;
+}
+
 int read_result(bitz_ctx* ctx, int K, uint8_t* host_out) {
     Scratch& sc = ctx->scratch;
     if (sc.mbox_pending) {
-        // spin on the flag; every 1024 polls ask whether the stream has
-        // drained, so a launch that never delivers ends in an error, not a hang
-        volatile uint32_t* flag = (volatile uint32_t*)sc.mbox;
-        for (uint64_t spin = 1;; spin++) {
-            if (*flag == sc.seq) break;
-            if (spin > (1u << 16)) std::this_thread::yield();  // a long kernel: share the core
-            if ((spin & 1023) == 0) {
-                hipError_t q = hipStreamQuery(ctx->stream);
-                if (q == hipSuccess) {
-                    if (*flag == sc.seq) break;
-                    set_error("read_result: the reduction finished without delivering its sums");
-                    return BITZ_ERR_DEVICE;
-                }
-                if (q != hipErrorNotReady) {
-                    set_error(std::string("read_result: ") + hipGetErrorString(q));
-                    return BITZ_ERR_DEVICE;
-                }
-            }
-        }
-        std::atomic_thread_fence(std::memory_order_acquire);
+        if (int rc = wait_mbox(ctx, "read_result")) return rc;
         memcpy(host_out, (const uint8_t*)sc.mbox + 64, K * 16);
-        sc.mbox_pending = false;
         return BITZ_OK;
     }
     BITZ_HIP(hipMemcpyAsync(sc.host, sc.result.ptr, K * 16, hipMemcpyDeviceToHost, ctx->stream));
     BITZ_HIP(hipStreamSynchronize(ctx->stream));
     memcpy(host_out, sc.host, K * 16);
+    return BITZ_OK;
+}
+
+int download_pair_mapped(bitz_ctx* ctx, const void* l, const void* r, uint64_t n, void** host_out) {
+    Scratch& sc = ctx->scratch;
+    if (int rc = sc.ensure(16)) return rc;
+    BITZ_REQUIRE(!sc.mbox_pending, BITZ_ERR_ARG, "download_pair_mapped: a mailbox result is still pending");
+    const size_t bytes = 2 * n * 16;
+    if (bytes > sc.tail_bytes) {
+        BITZ_HIP(hipStreamSynchronize(ctx->stream));  // no kernel still writes the old buffer
+        if (sc.tail) (void)hipHostFree(sc.tail);
+        sc.tail = sc.tail_dev = nullptr;
+        sc.tail_bytes = 0;
+        if (hipHostMalloc(&sc.tail, bytes, hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess ||
+            hipHostGetDevicePointer(&sc.tail_dev, sc.tail, 0) != hipSuccess) {
+            if (sc.tail) (void)hipHostFree(sc.tail);
+            sc.tail = sc.tail_dev = nullptr;
+            set_error("download_pair_mapped: hipHostMalloc failed");
+            return BITZ_ERR_ALLOC;
+        }
+        sc.tail_bytes = bytes;
+    }
+    const uint32_t seq = ++sc.seq;
+    hipLaunchKernelGGL(k_pair_to_host, dim3(1), dim3(1024), 0, ctx->stream, (const uint4*)l, (const uint4*)r, n,
+                       (uint4*)sc.tail_dev, (uint32_t*)sc.mbox_dev, seq);
+    BITZ_LAUNCH_CHECK();
+    sc.mbox_pending = true;
+    if (int rc = wait_mbox(ctx, "download_pair_mapped")) return rc;
+    *host_out = sc.tail;
     return BITZ_OK;
 }
 

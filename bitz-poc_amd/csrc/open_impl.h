@@ -122,6 +122,10 @@ int gkr_level4(bitz_ctx* ctx, const void* nr, const void* A, int t, int s, void*
 // to the mailbox for the host when out_dev is null (deferred when the
 // context defers sums to the device transcript).
 int sums_finish(bitz_ctx* ctx, unsigned nblocks, int K, void* out_dev);
+// l[0..n) and r[0..n) (device) into pinned host memory by one kernel that
+// announces them through the mailbox flag; *host_out points at l's copy,
+// r's follows (valid until the next call)
+int download_pair_mapped(bitz_ctx* ctx, const void* l, const void* r, uint64_t n, void** host_out);
 int product_level(bitz_ctx* ctx, const void* in, uint64_t n, void* out);
 int xi_combined(bitz_ctx* ctx, const uint64_t* packed, int t, int s, const void* w2, void* out);
 int fold_rows_eq(bitz_ctx* ctx, const uint64_t* rows, int t, int s, const void* eq, void* out);

@@ -15,6 +15,7 @@
 #include <set>
 #include <thread>
 
+#include <optional>
 #include "host_gf.h"
 #include "host_proto.h"
 #include "modq.h"
@@ -310,7 +311,11 @@ struct Prover {
     }
     int result(int K, G* out) {
         uint8_t buf[64];
-        int rc = read_result(ctx, K, buf);
+        int rc;
+        {
+            SubPhase sp(ctx, "sub_round_wait");
+            rc = read_result(ctx, K, buf);
+        }
         if (rc) return rc;
         for (int k = 0; k < K; k++) out[k] = gfrom(buf + 16 * k);
         return BITZ_OK;
@@ -372,6 +377,7 @@ struct Prover {
     static int nibble_round(LayerKind kind) { return 2 - nibble_level(kind); }
     static int table_round(LayerKind kind) { return 3 - nibble_level(kind); }
     int gkr_layer(InnerTranscript& it, uint4* l, uint4* r, LayerKind kind, std::vector<G>& internal, G* claim) {
+        std::optional<SubPhase> setup_phase(std::in_place, ctx, "sub_gkr_setup");
         const bool leaf = kind == kLeaf;
         const int L = (int)internal.size();
         std::vector<G> ext(internal.rbegin(), internal.rend());
@@ -458,6 +464,7 @@ struct Prover {
         bool tail_done = false;
         BITZ_REQUIRE(kind == kDense || (L >= 4 && t >= 7 && (!weigh || wsplit)), BITZ_ERR_SHAPE,
                      "GKR: layer too small for bit patterns");
+        setup_phase.reset();
         std::vector<hgf> chal;  // the layer's challenges so far (for its row-word and table rounds)
         const int nround = kind == kDense ? -1 : nibble_round(kind);
         const int jround = kind == kDense ? -1 : table_round(kind);
@@ -577,17 +584,18 @@ struct Prover {
     // src/bitz/kernels.rs:299-310). `last` receives the folded l[0], r[0].
     int host_tail(InnerTranscript& it, const uint4* l, const uint4* r, int round0, const std::vector<G>& ext,
                   const std::vector<G>& internal, G* factor, std::vector<G>* next, uint8_t last[32]) {
+        SubPhase sp(ctx, "sub_gkr_tail");
         using hostgf::E;
         const int L = (int)internal.size();
         const int rem0 = L - 1 - round0;
         const uint64_t n = 2ull << rem0;  // entries of each half at round0
-        int rc = ctx->scratch.ensure_stage(2 * n * sizeof(E));
-        if (rc) return rc;
-        E* hl = (E*)ctx->scratch.stage;
+        void* both;
+        {
+            SubPhase sw(ctx, "sub_gkr_tail_wait");
+            if (int rc = download_pair_mapped(ctx, l, r, n, &both)) return rc;
+        }
+        E* hl = (E*)both;
         E* hr = hl + n;
-        BITZ_HIP(hipMemcpyAsync(hl, l, n * sizeof(E), hipMemcpyDeviceToHost, ctx->stream));
-        BITZ_HIP(hipMemcpyAsync(hr, r, n * sizeof(E), hipMemcpyDeviceToHost, ctx->stream));
-        BITZ_HIP(hipStreamSynchronize(ctx->stream));
         std::vector<E> coords(rem0 > 0 ? rem0 : 1), w(1ull << rem0);
         for (int i = 0; i < rem0; i++) coords[i] = E{ext[i].lo, ext[i].hi};
         hostgf::eq_table(coords.data(), rem0, w.data());
@@ -803,12 +811,10 @@ struct Prover {
             G v[3];
             if (!ha && n <= (2ull << ctx->tune.host_tail_log)) {
                 // the remaining rounds on the host (as host_tail for GKR)
-                if ((rc = ctx->scratch.ensure_stage(2 * n * sizeof(hostgf::E)))) return rc;
-                ha = (hostgf::E*)ctx->scratch.stage;
+                void* both;
+                if ((rc = download_pair_mapped(ctx, A, W, n, &both))) return rc;
+                ha = (hostgf::E*)both;
                 hw = ha + n;
-                BITZ_HIP(hipMemcpyAsync(ha, A, n * 16, hipMemcpyDeviceToHost, ctx->stream));
-                BITZ_HIP(hipMemcpyAsync(hw, W, n * 16, hipMemcpyDeviceToHost, ctx->stream));
-                BITZ_HIP(hipStreamSynchronize(ctx->stream));
             }
             if (ha) {
                 hostgf::E e[3];

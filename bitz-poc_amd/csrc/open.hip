@@ -108,7 +108,7 @@ __device__ __forceinline__ gf128 tensor_prod4(const C& c, int base, int cnt, uin
 // coordinates' factors by a shuffle tree; then entry x = A[x & 15] B[x >> 4]
 // H. The longest chain is five products.
 template <class C>
-__global__ __launch_bounds__(256) void k_tensor(C c, int n, uint4* __restrict__ out) {
+__device__ __forceinline__ void tensor_block(const C& c, int n, uint32_t blk, uint4* __restrict__ out) {
     __shared__ uint4 ta[16], tb[16], th[1];
     const uint32_t t = threadIdx.x, wave = t >> 6, lane = t & 63;
     const int nlo = n < 8 ? n : 8, la = nlo < 4 ? nlo : 4, lb = nlo - la, nh = n - nlo;
@@ -119,7 +119,7 @@ __global__ __launch_bounds__(256) void k_tensor(C c, int n, uint4* __restrict__ 
     } else if (wave == 2) {
         gf128 v = gf_one();
         if ((int)lane < nh) {
-            const uint4 f = ((blockIdx.x >> lane) & 1u) ? c.b(nlo + lane) : c.a(nlo + lane);
+            const uint4 f = ((blk >> lane) & 1u) ? c.b(nlo + lane) : c.a(nlo + lane);
             v = gf_load(&f);
         }
         for (int off = 1; off < nh; off <<= 1) {  // lanes past nh hold one
@@ -136,8 +136,34 @@ __global__ __launch_bounds__(256) void k_tensor(C c, int n, uint4* __restrict__ 
         gf128 v = gf_load(&ta[t & ((1u << la) - 1)]);
         if (lb > 0) v = gf_mul(v, gf_load(&tb[t >> la]));
         if (nh > 0) v = gf_mul(v, gf_load(&th[0]));
-        gf_store(&out[((uint64_t)blockIdx.x << nlo) + t], v);
+        gf_store(&out[((uint64_t)blk << nlo) + t], v);
     }
+}
+
+template <class C>
+__global__ __launch_bounds__(256) void k_tensor(C c, int n, uint4* __restrict__ out) {
+    tensor_block(c, n, blockIdx.x, out);
+}
+
+// Several tables in one launch: job j owns blocks first[j]..first[j+1]-1.
+struct TensorJobs {
+    uint4 a[kTensorJobs][kTensorJobN];
+    uint4 b[kTensorJobs][kTensorJobN];
+    uint4* out[kTensorJobs];
+    int n[kTensorJobs];
+    uint32_t first[kTensorJobs + 1];
+    int njobs;
+};
+struct JobCoords {
+    const uint4* pa;
+    const uint4* pb;
+    __device__ __forceinline__ uint4 a(int i) const { return pa[i]; }
+    __device__ __forceinline__ uint4 b(int i) const { return pb[i]; }
+};
+__global__ __launch_bounds__(256) void k_tensor_jobs(TensorJobs J) {
+    int j = 0;
+    while (j + 1 < J.njobs && blockIdx.x >= J.first[j + 1]) j++;
+    tensor_block(JobCoords{J.a[j], J.b[j]}, J.n[j], blockIdx.x - J.first[j], J.out[j]);
 }
 
 // ------------------------------------------------------------- column folds
@@ -1892,6 +1918,31 @@ int eq_args(bitz_ctx* ctx, const hgf* r, int n, void* out) {
     const uint64_t total = 1ull << n;
     hipLaunchKernelGGL(k_eq_combine, dim3(blocks_for(total, 4)), dim3(kT), 0, ctx->stream, lo, nlo, hi, total,
                        (uint4*)out);
+    BITZ_LAUNCH_CHECK();
+    return BITZ_OK;
+}
+
+int tensor_args_multi(bitz_ctx* ctx, int njobs, const hgf* const* a, const hgf* const* b, const int* n,
+                      void* const* out) {
+    BITZ_REQUIRE(njobs >= 1 && njobs <= kTensorJobs, BITZ_ERR_ARG, "tensor_args_multi: 1..4 tables");
+    TensorJobs J{};
+    uint32_t blocks = 0;
+    for (int j = 0; j < njobs; j++) {
+        BITZ_REQUIRE(n[j] >= 0 && n[j] <= kTensorJobN && out[j], BITZ_ERR_SHAPE, "tensor_args_multi: bad table");
+        for (int i = 0; i < n[j]; i++) {
+            J.a[j][i] = make_uint4((uint32_t)a[j][i].lo, (uint32_t)(a[j][i].lo >> 32), (uint32_t)a[j][i].hi,
+                                   (uint32_t)(a[j][i].hi >> 32));
+            J.b[j][i] = make_uint4((uint32_t)b[j][i].lo, (uint32_t)(b[j][i].lo >> 32), (uint32_t)b[j][i].hi,
+                                   (uint32_t)(b[j][i].hi >> 32));
+        }
+        J.out[j] = (uint4*)out[j];
+        J.n[j] = n[j];
+        J.first[j] = blocks;
+        blocks += n[j] > 8 ? 1u << (n[j] - 8) : 1u;
+    }
+    J.first[njobs] = blocks;
+    J.njobs = njobs;
+    hipLaunchKernelGGL(k_tensor_jobs, dim3(blocks), dim3(256), 0, ctx->stream, J);
     BITZ_LAUNCH_CHECK();
     return BITZ_OK;
 }

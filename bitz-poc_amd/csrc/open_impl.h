@@ -23,6 +23,13 @@ struct TensorArgs {
     uint4 b[kMaxTensorN];
 };
 int tensor_args(bitz_ctx* ctx, const hgf* a, const hgf* b, int n, void* out);
+// Up to kTensorJobs such tables of at most kTensorJobN coordinates each in
+// one launch (their blocks side by side): table j over a[j], b[j] (n[j]
+// coordinates) into out[j].
+constexpr int kTensorJobs = 4;
+constexpr int kTensorJobN = 14;
+int tensor_args_multi(bitz_ctx* ctx, int njobs, const hgf* const* a, const hgf* const* b, const int* n,
+                      void* const* out);
 // BLAKE3 chaining values of m whole chunks of a transcript stream (chunk i at
 // data + 1024 i, counter c0 + i; data 4-byte aligned) merged into the
 // subtrees of the runs ks (blake3_subtree_runs(c0, m)): run r's root lands

@@ -334,6 +334,49 @@ struct Prover {
         return gkr_round_sums_rowonly(ctx, l, r, hh, s, eqy.ptr, send_one, nullptr, eqlo.ptr, wlo);
     }
 
+    // Product tables queued for one launch: those of at most kTensorJobN
+    // coordinates wait for flush() (tensor_args_multi, up to kTensorJobs per
+    // launch); larger ones are launched at once (eq_dev / tensor_args).
+    struct TableJobs {
+        std::vector<std::vector<G>> a, b;
+        std::vector<DevBuf*> bufs;
+        int add(Prover* p, std::vector<G> av, std::vector<G> bv, DevBuf& buf) {
+            const int n = (int)av.size();
+            if (!buf.alloc(16ull << n)) return BITZ_ERR_ALLOC;
+            if (n > kTensorJobN) return tensor_args(p->ctx, av.data(), bv.data(), n, buf.ptr);
+            a.push_back(std::move(av));
+            b.push_back(std::move(bv));
+            bufs.push_back(&buf);
+            return BITZ_OK;
+        }
+        int eq(Prover* p, const std::vector<G>& r, DevBuf& buf) {
+            if ((int)r.size() > kTensorJobN) return p->eq_dev(r, buf);
+            std::vector<G> av(r.size());
+            for (size_t i = 0; i < r.size(); i++) av[i] = hgf_add(r[i], gone());
+            return add(p, std::move(av), r, buf);
+        }
+        int flush(Prover* p) {
+            for (size_t j0 = 0; j0 < bufs.size(); j0 += kTensorJobs) {
+                const int nj = (int)std::min<size_t>(kTensorJobs, bufs.size() - j0);
+                const hgf* pa[kTensorJobs];
+                const hgf* pb[kTensorJobs];
+                int n[kTensorJobs];
+                void* out[kTensorJobs];
+                for (int j = 0; j < nj; j++) {
+                    pa[j] = a[j0 + j].data();
+                    pb[j] = b[j0 + j].data();
+                    n[j] = (int)a[j0 + j].size();
+                    out[j] = bufs[j0 + j]->ptr;
+                }
+                if (int rc = tensor_args_multi(p->ctx, nj, pa, pb, n, out)) return rc;
+            }
+            a.clear();
+            b.clear();
+            bufs.clear();
+            return BITZ_OK;
+        }
+    };
+
     int eq_dev(const std::vector<G>& r, DevBuf& buf) {
         if ((int)r.size() <= kMaxTensorN) {
             if (!buf.alloc((16ull << r.size()))) return BITZ_ERR_ALLOC;
@@ -406,12 +449,15 @@ struct Prover {
                 }
             }
         }
+        // the layer's tables of at most kTensorJobN coordinates go out in
+        // one launch (tensor_args_multi) after the set-up below
+        TableJobs tabs;
         if (per_layer) {
             int s_c0 = std::min(s, L - 1);
             std::vector<G> c(ext.begin(), ext.begin() + s_c0);
             std::vector<G> yv(ext.begin() + s_c0, ext.begin() + (L - 1));
-            if ((rc = eq_dev(c, eqc))) return rc;
-            if ((rc = eq_dev(yv, eqy))) return rc;
+            if ((rc = tabs.eq(this, c, eqc))) return rc;
+            if ((rc = tabs.eq(this, yv, eqy))) return rc;
         }
         // Column weights carried by the right half (weighable / batch_inverse,
         // src/bitz/gkr.rs:48-80; kernels.rs:114-131): eq_c depends only on the
@@ -427,8 +473,8 @@ struct Prover {
         wlo = std::min(s, 8);
         if (wsplit) {
             std::vector<G> clo(ext.begin(), ext.begin() + wlo), chi(ext.begin() + wlo, ext.begin() + s);
-            if ((rc = eq_dev(clo, eqlo))) return rc;
-            if (s > wlo && (rc = eq_dev(chi, eqhi))) return rc;
+            if ((rc = tabs.eq(this, clo, eqlo))) return rc;
+            if (s > wlo && (rc = tabs.eq(this, chi, eqhi))) return rc;
         }
         if (weigh) {
             std::vector<G> v(2 * s), inv(2 * s);
@@ -448,9 +494,11 @@ struct Prover {
             }
             // split weights: r never carries the low bits' factors
             for (int i = 0; wsplit && i < wlo; i++) inv[i] = inv[s + i] = gone();
-            if (!eqcinv.alloc(16ull << s)) return BITZ_ERR_ALLOC;
-            if ((rc = tensor_args(ctx, inv.data(), inv.data() + s, s, eqcinv.ptr))) return rc;
+            if ((rc = tabs.add(this, std::vector<G>(inv.begin(), inv.begin() + s),
+                               std::vector<G>(inv.begin() + s, inv.end()), eqcinv)))
+                return rc;
         }
+        if ((rc = tabs.flush(this))) return rc;
         if (per_layer && ctx->device_fs)
             return gkr_layer_device(it, l, r, kind, internal, corr, weigh, claim);
         bool weighted = false;  // r currently carries eq_c

@@ -154,6 +154,7 @@ struct Scratch {
     void* tail = nullptr;
     void* tail_dev = nullptr;
     size_t tail_bytes = 0;
+    DevBuf tail_done;  // the block count of the kernel that writes `tail`
     uint32_t seq = 0;
     bool mbox_pending = false;
     // pinned host staging for the host-side last rounds of a sumcheck

@@ -1518,17 +1518,26 @@ __global__ __launch_bounds__(kFsThreads) void k_transcript_challenges(bitz::B3St
     }
 }
 
-// l[0..n) then r[0..n) into host-mapped memory, then (after every thread's
-// system-scope fence) the sequence flag the host polls on. One block.
-__global__ void k_pair_to_host(const uint4* __restrict__ l, const uint4* __restrict__ r, uint64_t n, uint4* dst,
-                               uint32_t* flag, uint32_t seq) {
-    for (uint64_t i = threadIdx.x; i < n; i += blockDim.x) {
+// l[0..n) then r[0..n) into host-mapped memory, then the sequence flag the
+// host polls on: every block fences its stores at system scope and counts
+// itself in `done` (device memory, zero between launches); the last block to
+// arrive resets the count and writes the flag.
+__global__ __launch_bounds__(kT) void k_pair_to_host(const uint4* __restrict__ l, const uint4* __restrict__ r,
+                                                     uint64_t n, uint4* dst, uint32_t* done, uint32_t* flag,
+                                                     uint32_t seq) {
+    for (uint64_t i = grid_tid(); i < n; i += grid_threads()) {
         dst[i] = l[i];
         dst[n + i] = r[i];
     }
     __threadfence_system();
     __syncthreads();
-    if (threadIdx.x == 0) *(volatile uint32_t*)flag = seq;
+    if (threadIdx.x == 0) {
+        if (atomicAdd(done, 1u) == gridDim.x - 1) {
+            atomicExch(done, 0u);
+            __threadfence_system();
+            *(volatile uint32_t*)flag = seq;
+        }
+    }
 }
 
 }  // namespace
@@ -1712,9 +1721,14 @@ int download_pair_mapped(bitz_ctx* ctx, const void* l, const void* r, uint64_t n
         }
         sc.tail_bytes = bytes;
     }
+    if (!sc.tail_done.ptr) {
+        if (!sc.tail_done.alloc(256)) return BITZ_ERR_ALLOC;
+        BITZ_HIP(hipMemsetAsync(sc.tail_done.ptr, 0, 256, ctx->stream));
+    }
     const uint32_t seq = ++sc.seq;
-    hipLaunchKernelGGL(k_pair_to_host, dim3(1), dim3(1024), 0, ctx->stream, (const uint4*)l, (const uint4*)r, n,
-                       (uint4*)sc.tail_dev, (uint32_t*)sc.mbox_dev, seq);
+    hipLaunchKernelGGL(k_pair_to_host, dim3(blocks_for(n, 2, 64)), dim3(kT), 0, ctx->stream, (const uint4*)l,
+                       (const uint4*)r, n, (uint4*)sc.tail_dev, (uint32_t*)sc.tail_done.ptr, (uint32_t*)sc.mbox_dev,
+                       seq);
     BITZ_LAUNCH_CHECK();
     sc.mbox_pending = true;
     if (int rc = wait_mbox(ctx, "download_pair_mapped")) return rc;

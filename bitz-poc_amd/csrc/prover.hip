@@ -975,6 +975,7 @@ struct Prover {
     // (merkle.rs:698-745) from a device codeword/tree
     int open_level(const void* cw, const void* tree, uint64_t n_leaves, uint32_t lanes,
                    const std::vector<uint64_t>& queries, LevelOpen* out) {
+        SubPhase sp(ctx, "sub_lig_open");
         out->lanes = lanes;
         out->rows.resize(queries.size() * lanes);
         // sibling node indices (host only: they depend on the queries alone)
@@ -1070,6 +1071,7 @@ struct Challenger {
         return v;
     }
     int grind(uint32_t bits, uint64_t* nonce) {
+        SubPhase sp(ctx, "sub_lig_grind");
         it.public_message(std::string("pcs/flock/pow/v1"));
         it.public_message(&bits, 4);
         uint8_t seed[16];
@@ -1082,6 +1084,7 @@ struct Challenger {
     }
     // sample_distinct_queries (ligerito.rs:3191-3222)
     std::vector<uint64_t> queries(uint64_t block_len, int count) {
+        SubPhase sp(ctx, "sub_lig_queries");
         std::set<uint64_t> seen;
         std::vector<uint64_t> out;
         while ((int)out.size() < count) {
@@ -1514,6 +1517,7 @@ int Prover::prove_lin(InnerTranscript& it, PhaseClock& phase, DevBuf& u1, DevBuf
     // induce a level's basis from its opened rows (induce_sumcheck_poly)
     auto induce_intro = [&](int log_n, const LevelOpen& op, const std::vector<G>& v_ch,
                             const std::vector<uint64_t>& queries, const std::vector<G>& alpha) -> int {
+        std::optional<SubPhase> hp(std::in_place, ctx, "sub_lig_induce_host");
         std::vector<G> aw = host_eq(alpha);
         aw.resize(queries.size());
         std::vector<G> eqv = host_eq(v_ch);
@@ -1534,6 +1538,7 @@ int Prover::prove_lin(InnerTranscript& it, PhaseClock& phase, DevBuf& u1, DevBuf
         both.insert(both.end(), inv.begin(), inv.end());
         if ((r_ = upload_gf(both, tmp2))) return r_;
         if (!tmp0.alloc(16ull << log_n)) return BITZ_ERR_ALLOC;
+        hp.reset();
         if ((r_ = induce_basis(ctx, (const uint64_t*)qdev.ptr, (uint32_t)queries.size(), log_n, tmp2.ptr,
                                (const uint4*)tmp2.ptr + (log_n + 1), tmp1.ptr, tmp0.ptr)))
             return r_;

@@ -673,22 +673,23 @@ struct Prover {
 
     // returns the leaf point (LE, t+s coordinates) and claim
     int gkr(InnerTranscript& it, const std::vector<G>& zeta, std::vector<G>* point, G* claim) {
-        // Storage: the halves the leaf and level-1..3 layers fold into
-        // (2 x 2^(t+s-5) entries, shared: the layers run one after another),
-        // then levels 4..t-1 back to back (level j: 2^(t+s-j) entries). Levels
-        // 0-3 are never materialised; level 4 comes from level 3's tables of
-        // the row words, the upper levels from products.
+        // Storage: levels 4..t-1 back to back (level j: 2^(t+s-j) entries).
+        // Levels 0-3 are never materialised; level 4 comes from level 3's
+        // tables of the row words, the upper levels from products. Once layer
+        // 4 is proved, level 4's 2^(t+s-4) entries take the halves the level
+        // 3..0 layers fold into (2 x 2^(t+s-5) entries, shared: the layers run
+        // one after another).
         const uint64_t h4 = 1ull << (t + s - 4), h5 = h4 >> 1;
-        uint64_t total = 2 * h5;
+        uint64_t total = 0;
         for (int j = 4; j < t; j++) total += 1ull << (t + s - j);
         if (!levels.alloc(total * 16)) return BITZ_ERR_ALLOC;
         std::vector<uint4*> lv(t + 1, nullptr);
-        uint4* low = (uint4*)levels.ptr;
-        uint64_t off = 2 * h5;
+        uint64_t off = 0;
         for (int j = 4; j < t; j++) {
-            lv[j] = low + off;
+            lv[j] = (uint4*)levels.ptr + off;
             off += 1ull << (t + s - j);
         }
+        uint4* low = lv[4];
         if (!nibbles.alloc(h4 * 2)) return BITZ_ERR_ALLOC;  // 2^(t+s-4) u16
         int rc = gkr_nibble_rows(ctx, (const uint64_t*)packed.ptr, t, s, nibbles.ptr);
         if (rc) return rc;

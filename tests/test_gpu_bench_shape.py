@@ -4,8 +4,9 @@ replace, which test_gpu_open.py pins against the oracle. The dense leaves
 (1 + (A[y] - 1) bit(c, y), index (y << s) | c) are built on the device with
 torch from the same bits; the leaf layer's rounds 0 and 1 and its double
 fold, level 1's round 0 and fold (with and without carried column weights)
-and levels 1..3 are compared bit for bit on the device. Then GPU proofs at
-n = 28 and n = 30 are checked by the CPU verifier."""
+and levels 1..3 are compared bit for bit on the device, and so are the
+table rounds of levels 0-3 and level 4 from the row words. Then GPU proofs
+at n = 28 and n = 30 are checked by the CPU verifier."""
 import time
 
 import numpy as np
@@ -119,6 +120,59 @@ def test_levels_and_level1_round0(bench_shape):
     assert torch.equal(wl, l[:h])
     want_r = ctx().gf_mul(r[:h].contiguous(), to_dev(np.tile(eq_c, (h >> S, 1))))
     assert torch.equal(wr, want_r)
+
+
+@pytest.mark.parametrize("level", [3, 2, 1, 0])
+def test_table_round_at_bench_shape(bench_shape, level):
+    """the table round of the level-l layer (sums, plain and eq_hi-weighted
+    folds) and level 4 from the row words, against the dense kernels on the
+    dense level folded by the same challenges"""
+    import torch
+    from test_gpu_nibble import _jit_fold, _jit_round_sums, _nibble_rows
+    ops, rows, packed, dA, leaves = bench_shape
+    log(f"bench shape t={T} s={S}: table round of level {level} against the dense kernels")
+    nr = _nibble_rows(packed, T, S)
+    lvl = leaves
+    for _ in range(level):
+        lvl = ops.product_level(lvl)
+    half = lvl.shape[0] // 2
+    l, r = lvl[:half].clone(), lvl[half:].clone()
+    del lvl
+    j = 3 - level
+    chal = _rand_ints(j + 1, 40 + level)
+    h = half
+    for i in range(j):
+        h //= 2
+        ops.fold_halves_(l, h, chal[i])
+        ops.fold_halves_(r, h, chal[i])
+    assert h == 1 << (T + S - 4)
+    point = _rand_ints(T + S - 5, 50 + level)
+    eq_c, eq_y = _eq_parts(point, S)
+    hh = h // 2
+    for so in (False, True):
+        want = ops.gkr_round_sums(l[:h].contiguous(), r[:h].contiguous(), hh, to_dev(eq_c), S, to_dev(eq_y), so)
+        got = _jit_round_sums(level, nr, dA, T, S, chal[:j], to_dev(eq_c), to_dev(eq_y), so)
+        assert got == tuple(want)
+    import oracle as O
+    eq_hi = O.eq_table(O.ints_to_gf(_rand_ints(S - 8, 60 + level)))
+    wl, wr = _jit_fold(level, nr, dA, T, S, chal, to_dev(eq_hi))
+    ops.fold_halves_(l, hh, chal[j])
+    ops.fold_halves_(r, hh, chal[j])
+    assert torch.equal(wl, l[:hh])
+    del wl
+    cols_hi = torch.from_numpy(np.repeat(eq_hi, 256, axis=0).view(np.int64)).to("cuda")
+    assert torch.equal(wr, ctx().gf_mul(r[:hh].contiguous(), cols_hi.repeat(hh >> S, 1).contiguous()))
+    del wr, l, r
+    if level == 3:
+        from bitz_amd import _lib
+        from test_gpu_nibble import _vp
+        lv = leaves
+        for _ in range(4):
+            lv = ops.product_level(lv)
+        out = ctx().empty_gf(1 << (T + S - 4))
+        _lib.call("bitz_gkr_level4_dev", ctx().ptr, _vp(nr), _vp(dA), T, S, _vp(out))
+        assert torch.equal(out, lv)
+    torch.cuda.empty_cache()
 
 
 @pytest.mark.parametrize("t,s", [(16, 12), (17, 13)], ids=["n28", "n30"])

@@ -1706,8 +1706,8 @@ int download_pair_mapped(bitz_ctx* ctx, const void* l, const void* r, uint64_t n
     Scratch& sc = ctx->scratch;
     if (int rc = sc.ensure(16)) return rc;
     BITZ_REQUIRE(!sc.mbox_pending, BITZ_ERR_ARG, "download_pair_mapped: a mailbox result is still pending");
-    const size_t bytes = 2 * n * 16;
-    if (bytes > sc.tail_bytes) {
+    const size_t bytes = std::max<size_t>(2 * n * 16, 1u << 20);  // at least 1 MiB: grown rarely
+    if (2 * n * 16 > sc.tail_bytes) {
         BITZ_HIP(hipStreamSynchronize(ctx->stream));  // no kernel still writes the old buffer
         if (sc.tail) (void)hipHostFree(sc.tail);
         sc.tail = sc.tail_dev = nullptr;

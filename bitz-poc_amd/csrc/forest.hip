@@ -24,8 +24,12 @@
 // low byte (ab = 00), the high byte (11), or the mix of the two (01, 10),
 // and a row's sums are those tables against the eq_c buckets of its columns.
 // The fold by r_k is a lookup in two 256-entry tables per y,
-// F_p[i] = (1 + r_k) T_{0,p}[ext(i & Em)] + r_k T_{1,p}[ext((i & Om) >> sh)],
-// into halves of 2^(t+s-4) entries, where the layer's dense rounds start.
+// F_p[i] = (1 + r_k) T_{0,p}[ext(i & Em)] + r_k T_{1,p}[ext((i & Om) >> sh)]:
+// k_nibble_fold writes it out as halves of 2^(t+s-4) entries (the C ABI's
+// per-stage entry point); the prover keeps the tables F instead and runs one
+// more round off them (the table round below, which level 3 starts with),
+// whose fold writes halves of 2^(t+s-5) entries where the dense rounds
+// start.
 #include "common.h"
 #include "gf128_dev.h"
 #include "open_impl.h"

@@ -201,26 +201,54 @@ __global__ void k_fold_columns_partial(const uint64_t* __restrict__ rows, int t,
 // The same with the chunk's 4096 row weights staged in LDS (64 KiB) so the
 // per-set-bit gathers hit LDS: block = 256 consecutive columns of one chunk
 // of 64 words (needs 2^s a multiple of 256 and 64-word chunks).
+// The same partial sums through nibble tables (four Russians): block (chunk
+// of 16 words = 1024 rows, 256 columns) builds T[k][n][v] = sum of the
+// weights of rows 64k + 4n + j over the set bits j of v (16 words x 16
+// nibbles x 16 entries, 64 KiB of LDS, one 128-bit add per entry), then each
+// column adds one entry per nibble of its words: 16 lookups per word, no
+// branch on the bits.
 __global__ __launch_bounds__(256) void k_fold_columns_partial_lds(const uint64_t* __restrict__ rows, int t, int s,
                                                                   const uint4* __restrict__ w,
                                                                   uint4* __restrict__ partial) {
-    __shared__ uint4 ws[64 * 64];
+    __shared__ uint4 tab[16 * 16 * 16];
     const uint64_t words = 1ull << (t - 6), cols = 1ull << s;
     const uint64_t blocks_per_chunk = cols / 256;
     const uint64_t ch = blockIdx.x / blocks_per_chunk;
     const uint64_t c = (blockIdx.x % blocks_per_chunk) * 256 + threadIdx.x;
-    const uint4* wc = w + ch * 64 * 64;
-    for (uint32_t i = threadIdx.x; i < 64 * 64; i += 256) ws[i] = wc[i];
+    {
+        // thread (k, n) = threadIdx.x builds the 16 sums of rows 64k + 4n + j
+        const uint4* wn = w + ch * 1024 + 4 * threadIdx.x;
+        uint4* tn = tab + 16 * threadIdx.x;
+        uint64_t wl[4], wh[4];
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const uint4 e = wn[j];
+            wl[j] = (uint64_t)e.x | ((uint64_t)e.y << 32);
+            wh[j] = (uint64_t)e.z | ((uint64_t)e.w << 32);
+        }
+        uint64_t sl[16], sh[16];
+        sl[0] = sh[0] = 0;
+#pragma unroll
+        for (int v = 1; v < 16; v++) {
+            const int j = __builtin_ctz(v), u = v & (v - 1);
+            sl[v] = sl[u];
+            sh[v] = sh[u];
+            add128(sl[v], sh[v], wl[j], wh[j]);
+        }
+#pragma unroll
+        for (int v = 0; v < 16; v++)
+            tn[v] = make_uint4((uint32_t)sl[v], (uint32_t)(sl[v] >> 32), (uint32_t)sh[v], (uint32_t)(sh[v] >> 32));
+    }
     __syncthreads();
     uint64_t lo = 0, hi = 0;
-    const uint64_t* col = rows + c * words + ch * 64;
-    for (uint32_t k = 0; k < 64; k++) {
-        uint64_t x = col[k];
-        while (x) {
-            const int b = __builtin_ctzll(x);
-            const uint4 e = ws[k * 64 + b];
+    const uint64_t* col = rows + c * words + ch * 16;
+    for (uint32_t k = 0; k < 16; k++) {
+        const uint64_t x = col[k];
+        const uint4* tk = tab + 256 * k;
+#pragma unroll
+        for (int n = 0; n < 16; n++) {
+            const uint4 e = tk[16 * n + (uint32_t)((x >> (4 * n)) & 15u)];
             add128(lo, hi, (uint64_t)e.x | ((uint64_t)e.y << 32), (uint64_t)e.z | ((uint64_t)e.w << 32));
-            x &= x - 1;
         }
     }
     partial[ch * cols + c] = make_uint4((uint32_t)lo, (uint32_t)(lo >> 32), (uint32_t)hi, (uint32_t)(hi >> 32));
@@ -1762,12 +1790,13 @@ int eq_table(bitz_ctx* ctx, const void* r_dev, int n, void* out) {
 int fold_columns(bitz_ctx* ctx, const uint64_t* rows, int t, int s, const void* w, void* out) {
     BITZ_REQUIRE(t >= 6 && s >= 0, BITZ_ERR_SHAPE, "fold_columns: bad shape");
     uint64_t words = 1ull << (t - 6), cols = 1ull << s;
-    uint32_t chunk = words >= 64 ? 64 : (uint32_t)words;
+    const bool tables = words >= 16 && cols % 256 == 0;  // the nibble-table kernel: chunks of 16 words
+    uint32_t chunk = tables ? 16 : words >= 64 ? 64 : (uint32_t)words;
     uint64_t chunks = words / chunk;
     if (!ctx->scratch.fold_partial.alloc(chunks * cols * 16)) return BITZ_ERR_ALLOC;
     uint4* part = (uint4*)ctx->scratch.fold_partial.ptr;
     uint64_t items = cols * chunks;
-    if (chunk == 64 && cols % 256 == 0)
+    if (tables)
         hipLaunchKernelGGL(k_fold_columns_partial_lds, dim3((unsigned)(items / 256)), dim3(256), 0, ctx->stream, rows,
                            t, s, (const uint4*)w, part);
     else

@@ -60,6 +60,22 @@ __global__ void k_eq_small(const uint4* __restrict__ r, int n, uint4* __restrict
     }
 }
 
+// out[x] = lo[x & (2^nlo - 1)] * hi[x >> nlo] for runs of at least 2^10
+// entries per hi factor: block b takes entries b 2^pbl .. (b + 1) 2^pbl - 1
+// (pbl <= nlo, so one hi factor), whose product is a window-table product.
+__global__ __launch_bounds__(kT) void k_eq_combine_tab(const uint4* __restrict__ lo, int nlo,
+                                                       const uint4* __restrict__ hi, int pbl,
+                                                       uint4* __restrict__ out) {
+    __shared__ uint4 tab[512];
+    const uint64_t base = (uint64_t)blockIdx.x << pbl, lmask = (1ull << nlo) - 1;
+    gf_table_build(tab, gf_load(&hi[base >> nlo]), threadIdx.x, blockDim.x);
+    __syncthreads();
+    for (uint64_t i = threadIdx.x; i < (1ull << pbl); i += kT) {
+        const uint64_t x = base + i;
+        gf_store(&out[x], gf_mul_tab(tab, gf_load(&lo[x & lmask])));
+    }
+}
+
 // out[x] = lo[x & (2^nlo - 1)] * hi[x >> nlo]
 __global__ void k_eq_combine(const uint4* __restrict__ lo, int nlo, const uint4* __restrict__ hi,
                              uint64_t total, uint4* __restrict__ out) {
@@ -1764,6 +1780,21 @@ int download_pair_mapped(bitz_ctx* ctx, const void* l, const void* r, uint64_t n
     return BITZ_OK;
 }
 
+// out = lo (2^nlo) (x) hi (2^nhi): table products for long runs of one hi factor
+static int eq_combine(bitz_ctx* ctx, const uint4* lo, int nlo, const uint4* hi, int nhi, uint4* out) {
+    const uint64_t total = 1ull << (nlo + nhi);
+    if (nlo >= 10) {
+        const int pbl = std::min(nlo, 12);
+        hipLaunchKernelGGL(k_eq_combine_tab, dim3((unsigned)(total >> pbl)), dim3(kT), 0, ctx->stream, lo, nlo, hi,
+                           pbl, out);
+    } else {
+        hipLaunchKernelGGL(k_eq_combine, dim3(blocks_for(total, 4)), dim3(kT), 0, ctx->stream, lo, nlo, hi, total,
+                           out);
+    }
+    BITZ_LAUNCH_CHECK();
+    return BITZ_OK;
+}
+
 int eq_table(bitz_ctx* ctx, const void* r_dev, int n, void* out) {
     BITZ_REQUIRE(n >= 0 && n <= 40, BITZ_ERR_SHAPE, "eq_table: n out of range");
     if (n <= 12) {
@@ -1780,11 +1811,8 @@ int eq_table(bitz_ctx* ctx, const void* r_dev, int n, void* out) {
     BITZ_REQUIRE(nhi <= 14, BITZ_ERR_SHAPE, "eq_table: n too large");
     hipLaunchKernelGGL(k_eq_small, dim3(1), dim3(kT), 0, ctx->stream, (const uint4*)r_dev, nlo, lo);
     hipLaunchKernelGGL(k_eq_small, dim3(1), dim3(kT), 0, ctx->stream, (const uint4*)r_dev + nlo, nhi, hi);
-    uint64_t total = 1ull << n;
-    hipLaunchKernelGGL(k_eq_combine, dim3(blocks_for(total, 4)), dim3(kT), 0, ctx->stream, lo, nlo, hi, total,
-                       (uint4*)out);
     BITZ_LAUNCH_CHECK();
-    return BITZ_OK;
+    return eq_combine(ctx, lo, nlo, hi, nhi, (uint4*)out);
 }
 
 int fold_columns(bitz_ctx* ctx, const uint64_t* rows, int t, int s, const void* w, void* out) {
@@ -1958,11 +1986,7 @@ int eq_args(bitz_ctx* ctx, const hgf* r, int n, void* out) {
     int rc = tensor_args(ctx, a, r, nlo, lo);
     if (rc) return rc;
     if ((rc = tensor_args(ctx, a + nlo, r + nlo, nhi, hi))) return rc;
-    const uint64_t total = 1ull << n;
-    hipLaunchKernelGGL(k_eq_combine, dim3(blocks_for(total, 4)), dim3(kT), 0, ctx->stream, lo, nlo, hi, total,
-                       (uint4*)out);
-    BITZ_LAUNCH_CHECK();
-    return BITZ_OK;
+    return eq_combine(ctx, lo, nlo, hi, nhi, (uint4*)out);
 }
 
 int tensor_args_multi(bitz_ctx* ctx, int njobs, const hgf* const* a, const hgf* const* b, const int* n,

@@ -1046,17 +1046,31 @@ __global__ void k_xor_chunks(const uint4* __restrict__ partial, uint32_t chunks,
 }
 
 // partial[chunk][c] = sum over the chunk's rows b of eq[b] bit(c, b), from
-// column-major rows. Block (group of 64 columns, chunk of kRowsEq rows): the
-// chunk of eq is staged in LDS, thread (column, quarter) walks a quarter of
-// the column's words in the chunk and gathers eq per set bit from LDS.
-constexpr int kRowsEq = 2048;
+// column-major rows, through nibble tables (four Russians): block (group of
+// 64 columns, chunk of kRowsEq = 1024 rows) builds the 16 XOR sums of each
+// 4-row nibble's eq weights in LDS (256 nibbles x 16 entries, 64 KiB), and
+// the four threads of a column (quarters of its words, adjacent lanes) XOR
+// one entry per nibble of their words; the quarters merge by lane XOR and
+// each chunk's column sums go to `partial` (reduce_columns adds the chunks).
+constexpr int kRowsEq = 1024;
 __global__ __launch_bounds__(kT) void k_fold_rows_eq(const uint64_t* __restrict__ rows, int t, int s,
                                                      const uint4* __restrict__ eq, uint4* __restrict__ partial) {
-    __shared__ uint4 eqs[kRowsEq];
+    __shared__ uint4 tab[kRowsEq / 4 * 16];
     const uint64_t words = 1ull << (t - 6), cols = 1ull << s;
     const uint32_t chunk_rows = (uint32_t)std::min<uint64_t>(kRowsEq, 1ull << t);
     const uint64_t row0 = (uint64_t)blockIdx.y * chunk_rows;
-    for (uint32_t i = threadIdx.x; i < chunk_rows; i += blockDim.x) eqs[i] = eq[row0 + i];
+    if (threadIdx.x < chunk_rows / 4) {
+        const uint4* e = eq + row0 + 4 * threadIdx.x;
+        gf128 w[4], acc[16];
+#pragma unroll
+        for (int j = 0; j < 4; j++) w[j] = gf_load(&e[j]);
+        acc[0] = gf_zero();
+#pragma unroll
+        for (int v = 1; v < 16; v++) acc[v] = gf_add(acc[v & (v - 1)], w[__builtin_ctz(v)]);
+        uint4* tn = tab + 16 * threadIdx.x;
+#pragma unroll
+        for (int v = 0; v < 16; v++) gf_store(&tn[v], acc[v]);
+    }
     __syncthreads();
     const uint64_t c = (uint64_t)blockIdx.x * 64 + (threadIdx.x >> 2);
     const uint32_t q = threadIdx.x & 3, cw = chunk_rows / 64;
@@ -1064,11 +1078,12 @@ __global__ __launch_bounds__(kT) void k_fold_rows_eq(const uint64_t* __restrict_
     if (c < cols) {
         const uint64_t* col = rows + c * words + row0 / 64;
         for (uint32_t w = q; w < cw; w += 4) {
-            uint64_t x = col[w];
-            while (x) {
-                uint4 e = eqs[w * 64 + __builtin_ctzll(x)];
+            const uint64_t x = col[w];
+            const uint4* tw = tab + 256 * w;
+#pragma unroll
+            for (int n = 0; n < 16; n++) {
+                const uint4 e = tw[16 * n + (uint32_t)((x >> (4 * n)) & 15u)];
                 x0 ^= e.x, x1 ^= e.y, x2 ^= e.z, x3 ^= e.w;
-                x &= x - 1;
             }
         }
     }

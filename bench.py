@@ -43,7 +43,7 @@ VALU_PEAK_TOPS = 256 * 64 * 2.4e9 / 1e12
 # HBM MiB per dispatch, FETCH_SIZE and WRITE_SIZE each scaled by what the
 # counter reads for a 1 GiB device copy; "valu_insts": SQ_INSTS_VALU (wave
 # instructions, 64 lane operations each). Keys are the library's launch names.
-PMC_SOURCE = {"file": "profiles/pmc_commit_open_2p24.txt", "commit": "b9d709a"}
+PMC_SOURCE = {"file": "profiles/pmc_commit_open_2p24.txt", "commit": "77865cf"}
 PMC = {
     24: {
         "gkr_fold_round_sums_rowonly": {"kernel": "k_gkr_round_sums<FoldSrc, false>",
